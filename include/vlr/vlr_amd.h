@@ -1,0 +1,97 @@
+/* Extensions of the VLR C API on MI355X.
+ *
+ * None of these has a counterpart in the reference's vlr.h.  They are of three kinds:
+ *   1. runtime knobs for values the reference fixes at compile time, with defaults that
+ *      reproduce the reference (MaxPathLength = 25, path_tracing.cu:78; RGB rendering,
+ *      common_internal.h:23; 500,000 light paths, context.h:303);
+ *   2. progressive-render state access (per-pixel PCG32 states and the Kahan accumulation
+ *      buffer, context.cpp:1167-1190), which makes resume-after-checkpoint bitwise equal to
+ *      an uninterrupted render;
+ *   3. direct entry points into the hot kernels for parity tests and benchmarks: the BVH
+ *      traversal (replaces optixTrace, path_tracing.cu:89-93 and
+ *      light_transport_common.h:169-172) and the flattened BSDF code (replaces the BSDF
+ *      direct callables, materials.cu:282-328).
+ */
+#pragma once
+
+#include <stdint.h>
+#include <stddef.h>
+
+#if defined(__cplusplus)
+extern "C" {
+#endif
+
+/* ---- knobs ---- */
+VLR_API VLRResult vlrContextSetMaxPathLength(VLRContext context, uint32_t maxPathLength);
+VLR_API VLRResult vlrContextGetMaxPathLength(VLRContextConst context, uint32_t* maxPathLength);
+/* Spectral (4 hero wavelengths, Meng upsampling) vs RGB transport.  Takes effect at the
+ * next render with firstFrame = true. */
+VLR_API VLRResult vlrContextSetSpectralRendering(VLRContext context, bool enable);
+VLR_API VLRResult vlrContextGetSpectralRendering(VLRContextConst context, bool* enabled);
+
+/* ---- statistics ---- */
+typedef struct VLRRenderStats {
+    uint64_t numFrames;          /* frames rendered since the last reset */
+    uint64_t paths;              /* camera paths started */
+    uint64_t closestRays;        /* extension (closest-hit) rays traced */
+    uint64_t shadowRays;         /* shadow (any-hit) rays traced */
+    uint64_t droppedNonFinite;   /* samples dropped for a non-finite contribution */
+    /* device time accumulated with HIP events when kernel timing is enabled, in ms */
+    double msGenerate, msIntersect, msShade, msShadow, msAccumulate, msConvert;
+    uint64_t launchesIntersect, launchesShadow, launchesShade;
+} VLRRenderStats;
+/* Counting the rays reads back one small counter buffer per render call. */
+VLR_API VLRResult vlrContextGetRenderStats(VLRContextConst context, VLRRenderStats* stats);
+VLR_API VLRResult vlrContextResetRenderStats(VLRContext context);
+/* Brackets every kernel launch of vlrContextRender with HIP events on the render stream
+ * (synchronising at the end of each render call).  Off by default. */
+VLR_API VLRResult vlrContextSetKernelTiming(VLRContext context, bool enable);
+
+/* ---- progressive state (host pointers, width*height entries, row-major) ---- */
+VLR_API VLRResult vlrContextReadRNGStates(VLRContextConst context, uint64_t* states);
+VLR_API VLRResult vlrContextWriteRNGStates(VLRContext context, const uint64_t* states);
+/* Kahan accumulation pairs: (sum, compensation) per channel, channels = 3 (RGB) or 16
+ * (spectral strata); floats = width*height*channels*2. */
+VLR_API VLRResult vlrContextGetAccumulationChannels(VLRContextConst context, uint32_t* channels);
+VLR_API VLRResult vlrContextReadAccumulation(VLRContextConst context, float* data);
+VLR_API VLRResult vlrContextWriteAccumulation(VLRContext context, const float* data, uint32_t numAccumFrames);
+
+/* ---- traversal entry point ---- */
+typedef struct VLRRay { float org[3]; float tmin; float dir[3]; float tmax; } VLRRay;
+typedef struct VLRHit { int32_t instIndex; int32_t geomInstIndex; int32_t primIndex; float t, b1, b2; } VLRHit;
+/* Sets up `scene` on the device if needed, then traces numRays device-resident rays.
+ * anyHit = false: closest hit (instIndex = -1 on a miss).  anyHit = true: occlusion test
+ * in [tmin, tmax); hits[i].instIndex >= 0 when blocked.  Indices: instances in
+ * depth-first child order of parent nodes that hold surface nodes, geometry instances
+ * (material groups) in order of first appearance. */
+VLR_API VLRResult vlrContextTraceRays(VLRContext context, hipStream_t stream, VLRScene scene, const VLRRay* rays,
+                                      uint32_t numRays, VLRHit* hits, bool anyHit);
+
+/* ---- BSDF entry point ---- */
+typedef struct VLRBSDFQuery {
+    float dirLocal[3];
+    float geomNormalLocal[3];
+    float wlOffset, wlSelect;
+    float uComponent, uDir[2];
+    float evalDir[3];
+    int32_t singleIsSelected;
+} VLRBSDFQuery;
+typedef struct VLRBSDFResult {
+    float sampledDir[3];
+    float sampledPDF;
+    uint32_t sampledType;
+    float sampledValue[4];
+    float evalValue[4];
+    float evalPDF;
+    float revValue[4];
+    float revPDF;
+} VLRBSDFResult;
+/* Evaluates the material's BSDF (immediate parameters) on device-resident queries:
+ * sample / evaluate / evaluatePDF in the shading frame, radiance transport. */
+VLR_API VLRResult vlrContextEvaluateBSDFs(VLRContext context, hipStream_t stream, VLRSurfaceMaterialConst material,
+                                          bool spectral, const VLRBSDFQuery* queries, uint32_t numQueries,
+                                          VLRBSDFResult* results);
+
+#if defined(__cplusplus)
+}
+#endif

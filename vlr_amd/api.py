@@ -1,0 +1,416 @@
+"""ctypes binding of libvlr_amd.so (the C API of include/vlr/vlr.h and vlr_amd.h).
+
+The library is the product: this module only declares prototypes, checks VLRResult codes
+and offers :func:`build_scene`, which creates a vlr_amd.scenes.SceneDesc through the C API
+(TriangleMeshSurfaceNode / InternalNode / Scene / SurfaceMaterial / Camera), the same calls a
+C++ caller of vlrcpp.h would make.  There is no fallback: when the library is missing,
+:func:`lib` raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+from typing import Dict, List
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIB_PATH = PKG / "libvlr_amd.so"
+
+VLRResult_NoError = 0
+RESULT_NAMES = ["NoError", "InvalidContext", "InvalidInstance", "InvalidArgument", "IncompatibleNodeType",
+                "InternalError"]
+
+VLRRenderer_PathTracing = 0
+VLRRenderer_LightTracing = 1
+VLRRenderer_BPT = 2
+VLRRenderer_DebugRendering = 3
+
+PLUG_SPECTRUM = 7
+
+
+class VLRError(RuntimeError):
+    def __init__(self, fn, code):
+        name = RESULT_NAMES[code] if 0 <= code < len(RESULT_NAMES) else str(code)
+        super().__init__(f"{fn} -> VLRResult_{name}")
+        self.code = code
+
+
+class VLRPoint3D(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float)]
+
+
+class VLRQuaternion(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float), ("w", C.c_float)]
+
+
+class VLRImmediateSpectrum(C.Structure):
+    _fields_ = [("colorSpace", C.c_char_p), ("e0", C.c_float), ("e1", C.c_float), ("e2", C.c_float)]
+
+
+class VLRShaderNodePlug(C.Structure):
+    _fields_ = [("nodeRef", C.c_size_t), ("info", C.c_uint32)]
+
+
+class VLRRenderStats(C.Structure):
+    _fields_ = [("numFrames", C.c_uint64), ("paths", C.c_uint64), ("closestRays", C.c_uint64),
+                ("shadowRays", C.c_uint64), ("droppedNonFinite", C.c_uint64), ("msGenerate", C.c_double),
+                ("msIntersect", C.c_double), ("msShade", C.c_double), ("msShadow", C.c_double),
+                ("msAccumulate", C.c_double), ("msConvert", C.c_double), ("launchesIntersect", C.c_uint64),
+                ("launchesShadow", C.c_uint64), ("launchesShade", C.c_uint64)]
+
+
+RAY_DTYPE = np.dtype([("org", np.float32, 3), ("tmin", np.float32), ("dir", np.float32, 3), ("tmax", np.float32)])
+HIT_DTYPE = np.dtype([("inst", np.int32), ("geomInst", np.int32), ("prim", np.int32), ("t", np.float32),
+                      ("b1", np.float32), ("b2", np.float32)])
+BSDF_QUERY_DTYPE = np.dtype([("dirLocal", np.float32, 3), ("geomNormalLocal", np.float32, 3),
+                             ("wlOffset", np.float32), ("wlSelect", np.float32), ("uComponent", np.float32),
+                             ("uDir", np.float32, 2), ("evalDir", np.float32, 3), ("singleIsSelected", np.int32)])
+BSDF_RESULT_DTYPE = np.dtype([("sampledDir", np.float32, 3), ("sampledPDF", np.float32),
+                              ("sampledType", np.uint32), ("sampledValue", np.float32, 4),
+                              ("evalValue", np.float32, 4), ("evalPDF", np.float32), ("revValue", np.float32, 4),
+                              ("revPDF", np.float32)])
+
+H = C.c_void_p
+U32 = C.c_uint32
+F = C.c_float
+P = C.c_void_p
+CP = C.c_char_p
+B = C.c_bool
+
+# name -> argtypes (all return VLRResult)
+PROTOTYPES: Dict[str, list] = {
+    "vlrCreateContext": [H, B, U32, C.POINTER(H)],
+    "vlrDestroyContext": [H],
+    "vlrContextGetCUcontext": [H, C.POINTER(H)],
+    "vlrContextBindOutputBuffer": [H, U32, U32, U32],
+    "vlrContextGetOutputBufferSize": [H, C.POINTER(U32), C.POINTER(U32)],
+    "vlrContextGetOutputBuffer": [H, C.POINTER(H)],
+    "vlrContextReadOutputBuffer": [H, P],
+    "vlrContextSetScene": [H, H],
+    "vlrContextSetRenderer": [H, C.c_int],
+    "vlrContextSetDebugRenderingAttribute": [H, C.c_int],
+    "vlrContextSetProbePixel": [H, C.c_int32, C.c_int32],
+    "vlrContextRender": [H, H, H, B, U32, B, U32, C.POINTER(U32)],
+    "vlrObjectGetType": [H, C.POINTER(CP)],
+    "vlrParameterInfoGetName": [H, C.POINTER(CP)],
+    "vlrParameterInfoGetSocketForm": [H, C.POINTER(C.c_int)],
+    "vlrParameterInfoGetType": [H, C.POINTER(CP)],
+    "vlrParameterInfoGetTupleSize": [H, C.POINTER(U32)],
+    "vlrGetNumEnumMembers": [CP, C.POINTER(U32)],
+    "vlrGetEnumMember": [CP, U32, C.POINTER(CP)],
+    "vlrQueryableGetNumParameters": [H, C.POINTER(U32)],
+    "vlrQueryableGetParameterInfo": [H, U32, C.POINTER(H)],
+    "vlrQueryableGetEnumValue": [H, CP, C.POINTER(CP)],
+    "vlrQueryableGetPoint3D": [H, CP, C.POINTER(VLRPoint3D)],
+    "vlrQueryableGetVector3D": [H, CP, C.POINTER(VLRPoint3D)],
+    "vlrQueryableGetNormal3D": [H, CP, C.POINTER(VLRPoint3D)],
+    "vlrQueryableGetQuaternion": [H, CP, C.POINTER(VLRQuaternion)],
+    "vlrQueryableGetFloat": [H, CP, C.POINTER(F)],
+    "vlrQueryableGetFloatTuple": [H, CP, P, U32],
+    "vlrQueryableGetFloatArray": [H, CP, C.POINTER(P), C.POINTER(U32)],
+    "vlrQueryableGetImage2D": [H, CP, C.POINTER(H)],
+    "vlrQueryableGetImmediateSpectrum": [H, CP, C.POINTER(VLRImmediateSpectrum)],
+    "vlrQueryableGetSurfaceMaterial": [H, CP, C.POINTER(H)],
+    "vlrQueryableGetShaderNodePlug": [H, CP, C.POINTER(VLRShaderNodePlug)],
+    "vlrQueryableSetEnumValue": [H, CP, CP],
+    "vlrQueryableSetPoint3D": [H, CP, C.POINTER(VLRPoint3D)],
+    "vlrQueryableSetVector3D": [H, CP, C.POINTER(VLRPoint3D)],
+    "vlrQueryableSetNormal3D": [H, CP, C.POINTER(VLRPoint3D)],
+    "vlrQueryableSetQuaternion": [H, CP, C.POINTER(VLRQuaternion)],
+    "vlrQueryableSetFloat": [H, CP, F],
+    "vlrQueryableSetFloatTuple": [H, CP, P, U32],
+    "vlrQueryableSetImage2D": [H, CP, H],
+    "vlrQueryableSetImmediateSpectrum": [H, CP, C.POINTER(VLRImmediateSpectrum)],
+    "vlrQueryableSetSurfaceMaterial": [H, CP, H],
+    "vlrQueryableSetShaderNodePlug": [H, CP, VLRShaderNodePlug],
+    "vlrImage2DGetWidth": [H, C.POINTER(U32)],
+    "vlrImage2DGetHeight": [H, C.POINTER(U32)],
+    "vlrImage2DGetStride": [H, C.POINTER(U32)],
+    "vlrImage2DGetOriginalDataFormat": [H, C.POINTER(CP)],
+    "vlrImage2DOriginalHasAlpha": [H, C.POINTER(B)],
+    "vlrLinearImage2DCreate": [H, P, U32, U32, CP, CP, CP, C.POINTER(H)],
+    "vlrLinearImage2DDestroy": [H, H],
+    "vlrBlockCompressedImage2DCreate": [H, P, P, U32, U32, U32, CP, CP, CP, C.POINTER(H)],
+    "vlrBlockCompressedImage2DDestroy": [H, H],
+    "vlrShaderNodeCreate": [H, CP, C.POINTER(H)],
+    "vlrShaderNodeDestroy": [H, H],
+    "vlrShaderNodeGetPlug": [H, C.c_int, U32, C.POINTER(VLRShaderNodePlug)],
+    "vlrSurfaceMaterialCreate": [H, CP, C.POINTER(H)],
+    "vlrSurfaceMaterialDestroy": [H, H],
+    "vlrStaticTransformCreate": [H, P, C.POINTER(H)],
+    "vlrStaticTransformDestroy": [H, H],
+    "vlrStaticTransformGetArrays": [H, P, P],
+    "vlrNodeSetName": [H, CP],
+    "vlrNodeGetName": [H, C.POINTER(CP)],
+    "vlrTriangleMeshSurfaceNodeCreate": [H, CP, C.POINTER(H)],
+    "vlrTriangleMeshSurfaceNodeDestroy": [H, H],
+    "vlrTriangleMeshSurfaceNodeSetVertices": [H, P, U32],
+    "vlrTriangleMeshSurfaceNodeAddMaterialGroup": [H, P, U32, H, VLRShaderNodePlug, VLRShaderNodePlug,
+                                                   VLRShaderNodePlug],
+    "vlrPointSurfaceNodeCreate": [H, CP, C.POINTER(H)],
+    "vlrPointSurfaceNodeDestroy": [H, H],
+    "vlrPointSurfaceNodeSetVertices": [H, P, U32],
+    "vlrPointSurfaceNodeAddMaterialGroup": [H, P, U32, H],
+    "vlrParentNodeSetTransform": [H, H],
+    "vlrParentNodeGetTransform": [H, C.POINTER(H)],
+    "vlrParentNodeAddChild": [H, H],
+    "vlrParentNodeRemoveChild": [H, H],
+    "vlrParentNodeGetNumChildren": [H, C.POINTER(U32)],
+    "vlrParentNodeGetChildren": [H, P],
+    "vlrParentNodeGetChildAt": [H, U32, C.POINTER(H)],
+    "vlrInternalNodeCreate": [H, CP, H, C.POINTER(H)],
+    "vlrInternalNodeDestroy": [H, H],
+    "vlrSceneCreate": [H, H, C.POINTER(H)],
+    "vlrSceneDestroy": [H, H],
+    "vlrSceneSetEnvironment": [H, H],
+    "vlrSceneSetEnvironmentRotation": [H, F],
+    "vlrCameraCreate": [H, CP, C.POINTER(H)],
+    "vlrCameraDestroy": [H, H],
+    # extensions
+    "vlrContextSetMaxPathLength": [H, U32],
+    "vlrContextGetMaxPathLength": [H, C.POINTER(U32)],
+    "vlrContextSetSpectralRendering": [H, B],
+    "vlrContextGetSpectralRendering": [H, C.POINTER(B)],
+    "vlrContextGetRenderStats": [H, C.POINTER(VLRRenderStats)],
+    "vlrContextResetRenderStats": [H],
+    "vlrContextSetKernelTiming": [H, B],
+    "vlrContextReadRNGStates": [H, P],
+    "vlrContextWriteRNGStates": [H, P],
+    "vlrContextGetAccumulationChannels": [H, C.POINTER(U32)],
+    "vlrContextReadAccumulation": [H, P],
+    "vlrContextWriteAccumulation": [H, P, U32],
+    "vlrContextTraceRays": [H, H, H, P, U32, P, B],
+    "vlrContextEvaluateBSDFs": [H, H, H, B, P, U32, P],
+}
+
+_lib = None
+
+
+def lib():
+    """Load libvlr_amd.so; raises if it has not been built (no fallback path exists)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m vlr_amd.build`")
+        L = C.CDLL(str(LIB_PATH))
+        for name, args in PROTOTYPES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = C.c_int
+        L.vlrGetErrorMessage.argtypes = [C.c_int]
+        L.vlrGetErrorMessage.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def call(name, *args):
+    rc = getattr(lib(), name)(*args)
+    if rc != VLRResult_NoError:
+        raise VLRError(name, rc)
+    return rc
+
+
+def try_call(name, *args) -> int:
+    return getattr(lib(), name)(*args)
+
+
+def _b(s):
+    return s.encode() if isinstance(s, str) else s
+
+
+NULL_PLUG = VLRShaderNodePlug(0, 0)
+
+
+class Context:
+    """Owns a VLRContext (vlrcpp.h:684-749 equivalent) and the objects created through it."""
+
+    def __init__(self, spectral: bool = False, max_path_length: int = 25):
+        h = H()
+        call("vlrCreateContext", None, False, 8, C.byref(h))
+        self.h = h
+        self.spectral = spectral
+        if spectral:
+            call("vlrContextSetSpectralRendering", self.h, True)
+        call("vlrContextSetMaxPathLength", self.h, max_path_length)
+        self.width = self.height = 0
+        self.num_accum = U32(0)
+        self._keep: List = []
+
+    def close(self):
+        if self.h:
+            call("vlrDestroyContext", self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # ---- object creation ----
+    def material(self, type_name: str):
+        m = H()
+        call("vlrSurfaceMaterialCreate", self.h, _b(type_name), C.byref(m))
+        return m
+
+    def shader_node(self, type_name: str):
+        n = H()
+        call("vlrShaderNodeCreate", self.h, _b(type_name), C.byref(n))
+        return n
+
+    def transform(self, mat16):
+        t = H()
+        arr = (C.c_float * 16)(*[float(x) for x in mat16])
+        call("vlrStaticTransformCreate", self.h, arr, C.byref(t))
+        return t
+
+    def bind_output(self, width: int, height: int):
+        call("vlrContextBindOutputBuffer", self.h, width, height, 0)
+        self.width, self.height = width, height
+
+    def set_scene(self, scene):
+        call("vlrContextSetScene", self.h, scene)
+
+    def render(self, camera, first_frame: bool, limit: int, stream=None, shrink: int = 1) -> int:
+        call("vlrContextRender", self.h, stream, camera, False, shrink, first_frame, limit, C.byref(self.num_accum))
+        return self.num_accum.value
+
+    def read_output(self) -> np.ndarray:
+        out = np.zeros((self.height, self.width, 4), np.float32)
+        call("vlrContextReadOutputBuffer", self.h, out.ctypes.data)
+        return out
+
+    def read_rng(self) -> np.ndarray:
+        out = np.zeros(self.width * self.height, np.uint64)
+        call("vlrContextReadRNGStates", self.h, out.ctypes.data)
+        return out
+
+    def write_rng(self, states: np.ndarray):
+        s = np.ascontiguousarray(states, np.uint64)
+        call("vlrContextWriteRNGStates", self.h, s.ctypes.data)
+
+    def read_accumulation(self) -> np.ndarray:
+        ch = U32()
+        call("vlrContextGetAccumulationChannels", self.h, C.byref(ch))
+        out = np.zeros((self.height, self.width, ch.value, 2), np.float32)
+        call("vlrContextReadAccumulation", self.h, out.ctypes.data)
+        return out
+
+    def write_accumulation(self, acc: np.ndarray, frames: int):
+        a = np.ascontiguousarray(acc, np.float32)
+        call("vlrContextWriteAccumulation", self.h, a.ctypes.data, frames)
+
+    def stats(self) -> dict:
+        s = VLRRenderStats()
+        call("vlrContextGetRenderStats", self.h, C.byref(s))
+        return {f: getattr(s, f) for f, _ in VLRRenderStats._fields_}
+
+    def reset_stats(self):
+        call("vlrContextResetRenderStats", self.h)
+
+    def set_kernel_timing(self, on: bool):
+        call("vlrContextSetKernelTiming", self.h, on)
+
+    def trace_rays(self, scene, rays_dev_ptr: int, n: int, hits_dev_ptr: int, any_hit: bool, stream=None):
+        call("vlrContextTraceRays", self.h, stream, scene, C.c_void_p(rays_dev_ptr), n, C.c_void_p(hits_dev_ptr),
+             any_hit)
+
+    def evaluate_bsdfs(self, material, spectral: bool, q_ptr: int, n: int, r_ptr: int, stream=None):
+        call("vlrContextEvaluateBSDFs", self.h, stream, material, spectral, C.c_void_p(q_ptr), n, C.c_void_p(r_ptr))
+
+
+def set_spectrum(obj, name: str, cs: str, e0: float, e1: float, e2: float):
+    s = VLRImmediateSpectrum(_b(cs), e0, e1, e2)
+    call("vlrQueryableSetImmediateSpectrum", obj, _b(name), C.byref(s))
+
+
+def set_float(obj, name: str, v: float):
+    call("vlrQueryableSetFloat", obj, _b(name), float(v))
+
+
+def build_material(ctx: Context, m, env_texture_node=None):
+    h = ctx.material(m.type)
+    for name, (cs, e0, e1, e2) in m.spectra.items():
+        if m.use_env_texture and name == "emittance" and env_texture_node is not None:
+            continue
+        set_spectrum(h, name, cs, e0, e1, e2)
+    for name, v in m.floats.items():
+        set_float(h, name, v)
+    if m.use_env_texture and env_texture_node is not None:
+        plug = VLRShaderNodePlug()
+        call("vlrShaderNodeGetPlug", env_texture_node, PLUG_SPECTRUM, 0, C.byref(plug))
+        call("vlrQueryableSetShaderNodePlug", h, b"emittance", plug)
+    return h
+
+
+class BuiltScene:
+    def __init__(self):
+        self.scene = None
+        self.camera = None
+        self.materials = []
+        self.meshes = []
+        self.instances = []
+        self.keep = []
+
+
+def build_scene(ctx: Context, desc) -> BuiltScene:
+    """Create the objects of a SceneDesc through the C API and make it the context's scene."""
+    bs = BuiltScene()
+    env_node = None
+    if desc.env is not None and desc.env.texture_half is not None:
+        tex = np.ascontiguousarray(desc.env.texture_half, np.uint16)
+        bs.keep.append(tex)
+        img = H()
+        call("vlrLinearImage2DCreate", ctx.h, tex.ctypes.data, tex.shape[1], tex.shape[0], b"RGBA16Fx4",
+             b"Light Source", b"Rec709(D65)", C.byref(img))
+        env_node = ctx.shader_node("EnvironmentTexture")
+        call("vlrQueryableSetImage2D", env_node, b"image", img)
+    for m in desc.materials:
+        bs.materials.append(build_material(ctx, m, env_node))
+    for mesh in desc.meshes:
+        n = H()
+        v = np.ascontiguousarray(mesh.vertices, np.float32)
+        assert v.shape[1] == 11
+        if mesh.points:
+            call("vlrPointSurfaceNodeCreate", ctx.h, _b(mesh.name), C.byref(n))
+            call("vlrPointSurfaceNodeSetVertices", n, v.ctypes.data, len(v))
+            for idx, mi in mesh.groups:
+                ix = np.ascontiguousarray(idx, np.uint32)
+                call("vlrPointSurfaceNodeAddMaterialGroup", n, ix.ctypes.data, len(ix), bs.materials[mi])
+        else:
+            call("vlrTriangleMeshSurfaceNodeCreate", ctx.h, _b(mesh.name), C.byref(n))
+            call("vlrTriangleMeshSurfaceNodeSetVertices", n, v.ctypes.data, len(v))
+            for idx, mi in mesh.groups:
+                ix = np.ascontiguousarray(idx, np.uint32)
+                call("vlrTriangleMeshSurfaceNodeAddMaterialGroup", n, ix.ctypes.data, len(ix), bs.materials[mi],
+                     NULL_PLUG, NULL_PLUG, NULL_PLUG)
+        bs.meshes.append(n)
+    ident = ctx.transform(np.eye(4, dtype=np.float32).reshape(16, order="F"))
+    scene = H()
+    call("vlrSceneCreate", ctx.h, ident, C.byref(scene))
+    for inst in desc.instances:
+        xf = ctx.transform(inst.transform)
+        node = H()
+        call("vlrInternalNodeCreate", ctx.h, _b(inst.name), xf, C.byref(node))
+        for mi in inst.meshes:
+            call("vlrParentNodeAddChild", node, bs.meshes[mi])
+        call("vlrParentNodeAddChild", scene, node)
+        bs.instances.append(node)
+    if desc.env is not None:
+        call("vlrSceneSetEnvironment", scene, bs.materials[desc.env.material])
+        call("vlrSceneSetEnvironmentRotation", scene, float(desc.env.rotation))
+    cam = H()
+    call("vlrCameraCreate", ctx.h, b"Perspective", C.byref(cam))
+    c = desc.camera
+    call("vlrQueryableSetPoint3D", cam, b"position", C.byref(VLRPoint3D(*c.position)))
+    call("vlrQueryableSetQuaternion", cam, b"orientation", C.byref(VLRQuaternion(*c.orientation)))
+    set_float(cam, "aspect", c.aspect)
+    set_float(cam, "sensitivity", c.sensitivity)
+    set_float(cam, "fovy", c.fovy)
+    set_float(cam, "lens radius", c.lens_radius)
+    set_float(cam, "op distance", c.op_distance)
+    ctx.set_scene(scene)
+    bs.scene = scene
+    bs.camera = cam
+    return bs

@@ -1,0 +1,228 @@
+// Context: output/accumulation/RNG buffers and render orchestration.
+//
+// Follows libVLR/context.cpp: bindOutputBuffer (:1130-1236) allocates the output, the Kahan
+// accumulation buffer and one PCG32 state per pixel seeded by mt19937_64(591842031321323413)
+// in row-major order (:1182-1187); render (:1267-1591) sets the scene up, resets the
+// accumulation on firstFrame and renders one sample per pixel while
+// numAccumFrames < limitNumAccumFrames (:1416-1443), then converts to RGBA (:1582-1589).
+#include "runtime.h"
+
+#include <cstring>
+#include <random>
+#include <stdexcept>
+
+namespace vlra {
+
+Context::Context(hipCtx_t hctx) : Object(ObjType::Context, nullptr) {
+    ctx = this;
+    (void)hctx;
+    // no device work here: the scene graph can be built on a host without a GPU; device
+    // buffers are allocated by bindOutputBuffer / the first render or trace call
+    if (!loadSpectralTables()) throw std::runtime_error("spectral tables not found next to the library");
+}
+
+void Context::ensureDevice() {
+    if (device >= 0) return;
+    VLRA_HIP(hipGetDevice(&device));
+    counters.reserve(kNumCounters * sizeof(uint32_t));
+    VLRA_HIP(hipMemset(counters.ptr, 0, kNumCounters * sizeof(uint32_t)));
+}
+
+Context::~Context() {
+    if (device >= 0) (void)hipDeviceSynchronize();
+    for (Object *o : std::set<Object *>(objects)) delete o;
+}
+
+void Context::resizeAccum() {
+    accumChannels = spectralActive ? 16 : 3;
+    size_t n = size_t(width) * height;
+    accum.reserve(n * accumChannels * 2 * sizeof(float));
+    VLRA_HIP(hipMemset(accum.ptr, 0, n * accumChannels * 2 * sizeof(float)));
+}
+
+void Context::bindOutputBuffer(uint32_t w, uint32_t h) {
+    if (w == 0 || h == 0) throw ApiError{3, "output size must be positive"};
+    ensureDevice();
+    VLRA_HIP(hipDeviceSynchronize());
+    width = w;
+    height = h;
+    const size_t n = size_t(w) * h;
+    output.reserve(n * sizeof(float4));
+    VLRA_HIP(hipMemset(output.ptr, 0, n * sizeof(float4)));
+    rng.reserve(n * sizeof(uint64_t));
+    alpha.reserve(n * sizeof(float4));
+    contrib.reserve(n * sizeof(float4));
+    meta.reserve(n * sizeof(float4));
+    for (int k = 0; k < 2; ++k) {
+        rayOrg[k].reserve(n * sizeof(float4));
+        rayDir[k].reserve(n * sizeof(float4));
+    }
+    hitA.reserve(n * sizeof(int4));
+    hitB.reserve(n * sizeof(float2));
+    shOrg.reserve(n * sizeof(float4));
+    shDir.reserve(n * sizeof(float4));
+    shVal.reserve(n * sizeof(float4));
+    resizeAccum();
+    std::vector<uint64_t> seeds(n);
+    std::mt19937_64 seedRng(591842031321323413ULL);
+    for (size_t i = 0; i < n; ++i) seeds[i] = seedRng();
+    VLRA_HIP(hipMemcpy(rng.ptr, seeds.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice));
+    numAccumFrames = 0;
+}
+
+void Context::ensureScene(const Scene &s) {
+    ensureDevice();
+    if (gpu.valid && gpu.scene == &s && gpu.sceneVersion == version && gpu.spectral == spectralActive) return;
+    VLRA_HIP(hipDeviceSynchronize());
+    buildSceneGPU(*this, s, spectralActive, &gpu);
+    gpu.scene = &s;
+    gpu.sceneVersion = version;
+}
+
+namespace {
+struct EventTimer : public LaunchTimer {
+    hipStream_t stream;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> events;
+    hipEvent_t pending = nullptr;
+    explicit EventTimer(hipStream_t s) : stream(s) {}
+    void begin(int) override {
+        VLRA_HIP(hipEventCreate(&pending));
+        VLRA_HIP(hipEventRecord(pending, stream));
+    }
+    void end(int stage) override {
+        hipEvent_t e;
+        VLRA_HIP(hipEventCreate(&e));
+        VLRA_HIP(hipEventRecord(e, stream));
+        events.push_back({stage, {pending, e}});
+        pending = nullptr;
+    }
+    void collect(double *ms, uint64_t *launches) {
+        VLRA_HIP(hipStreamSynchronize(stream));
+        for (auto &ev : events) {
+            float t = 0;
+            VLRA_HIP(hipEventElapsedTime(&t, ev.second.first, ev.second.second));
+            ms[ev.first] += t;
+            launches[ev.first] += 1;
+            (void)hipEventDestroy(ev.second.first);
+            (void)hipEventDestroy(ev.second.second);
+        }
+        events.clear();
+    }
+};
+} // namespace
+
+void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCoeff, bool firstFrame, uint32_t limit,
+                     uint32_t *numAccum) {
+    if (!scene) throw ApiError{3, "no scene is set"};
+    if (!camera) throw ApiError{3, "camera is null"};
+    if (!camera->perspective) throw ApiError{4, "only the perspective camera is on the render path"};
+    if (width == 0 || height == 0) throw ApiError{3, "bind an output buffer first"};
+    if (renderer != 0) throw ApiError{3, "renderer not available on this build: only VLRRenderer_PathTracing"};
+    if (shrinkCoeff == 0) shrinkCoeff = 1;
+    if (firstFrame) {
+        if (spectralActive != spectralRequested) {
+            spectralActive = spectralRequested;
+            resizeAccum();
+        }
+        numAccumFrames = 0;
+        // the camera is read on the first frame only (context.cpp:1416-1421)
+        frameCamera = camera->toDevice();
+    }
+    ensureScene(*scene);
+    gpu.dev.camera = frameCamera;
+    if (numAccumFrames + 1 <= limit) {
+        ++numAccumFrames;
+        FrameParams fp;
+        fp.width = width;
+        fp.height = height;
+        fp.imageW = width / shrinkCoeff;
+        fp.imageH = height / shrinkCoeff;
+        fp.frameIndex = numAccumFrames;
+        fp.maxPathLength = std::min<uint32_t>(maxPathLength, kMaxDepthCounters - 1);
+        FrameBuffers fb;
+        fb.ps.rng = static_cast<uint64_t *>(rng.ptr);
+        fb.ps.alpha = static_cast<float4 *>(alpha.ptr);
+        fb.ps.contrib = static_cast<float4 *>(contrib.ptr);
+        fb.ps.meta = static_cast<float4 *>(meta.ptr);
+        for (int k = 0; k < 2; ++k) {
+            fb.rays[k].org = static_cast<float4 *>(rayOrg[k].ptr);
+            fb.rays[k].dir = static_cast<float4 *>(rayDir[k].ptr);
+        }
+        fb.hits.a = static_cast<int4 *>(hitA.ptr);
+        fb.hits.b = static_cast<float2 *>(hitB.ptr);
+        fb.shadow.org = static_cast<float4 *>(shOrg.ptr);
+        fb.shadow.dir = static_cast<float4 *>(shDir.ptr);
+        fb.shadow.value = static_cast<float4 *>(shVal.ptr);
+        fb.counters = static_cast<uint32_t *>(counters.ptr);
+        fb.accum = static_cast<float *>(accum.ptr);
+        fb.output = static_cast<float4 *>(output.ptr);
+        if (kernelTiming) {
+            EventTimer t(stream);
+            VLRA_HIP(launchFrame(gpu.dev, fp, fb, spectralActive, stream, &t));
+            t.collect(stageMs, launches);
+        } else {
+            VLRA_HIP(launchFrame(gpu.dev, fp, fb, spectralActive, stream, nullptr));
+        }
+        lastFrame = fp;
+        ++framesRendered;
+        pathsStarted += uint64_t(fp.imageW) * fp.imageH;
+    }
+    if (numAccum) *numAccum = numAccumFrames;
+}
+
+void Context::readOutput(float *dst) {
+    VLRA_HIP(hipDeviceSynchronize());
+    VLRA_HIP(hipMemcpy(dst, output.ptr, size_t(width) * height * sizeof(float4), hipMemcpyDeviceToHost));
+}
+
+void Context::traceRays(hipStream_t stream, const Scene &s, const void *rays, uint32_t n, void *hits, bool anyHit) {
+    ensureScene(s);
+    VLRA_HIP(launchTraceRays(gpu.dev, rays, n, hits, anyHit, stream));
+}
+
+void Context::evaluateBSDFs(hipStream_t stream, const SurfaceMaterial &m, bool spectral, const void *q, uint32_t n,
+                            void *r) {
+    if (!m.supportedOnRenderPath()) throw ApiError{4, "material type not on the render path"};
+    ensureDevice();
+    bool usesTex;
+    const Image2D *img;
+    DevMaterial dm = m.toDevice(spectral, &usesTex, &img);
+    // the BSDF kernels only read the Meng tables from the scene record
+    if (!gpu.mengPoints.ptr) {
+        gpu.mengGrid.upload(mengGrid().data(), mengGrid().size() * sizeof(MengCellDev));
+        gpu.mengPoints.upload(mengPoints().data(), mengPoints().size() * sizeof(float));
+    }
+    DevScene sc = gpu.dev;
+    sc.mengGrid = static_cast<const MengCellDev *>(gpu.mengGrid.ptr);
+    sc.mengPoints = static_cast<const float *>(gpu.mengPoints.ptr);
+    VLRA_HIP(launchBsdfQueries(sc, dm, spectral, q, n, r, stream));
+}
+
+void Context::statistics(uint64_t *closest, uint64_t *shadow, uint64_t *dropped) {
+    *closest = *shadow = *dropped = 0;
+    if (device < 0) return;
+    uint32_t host[kNumCounters];
+    VLRA_HIP(hipDeviceSynchronize());
+    VLRA_HIP(hipMemcpy(host, counters.ptr, sizeof(host), hipMemcpyDeviceToHost));
+    uint64_t c, s;
+    std::memcpy(&c, host + kStatClosest, 8);
+    std::memcpy(&s, host + kStatShadow, 8);
+    *closest = c;
+    *shadow = s;
+    *dropped = host[kStatDropped];
+}
+
+void Context::resetStats() {
+    if (device >= 0) {
+        VLRA_HIP(hipDeviceSynchronize());
+        VLRA_HIP(hipMemset(counters.ptr, 0, kNumCounters * sizeof(uint32_t)));
+    }
+    for (int i = 0; i < NUM_STAGES; ++i) {
+        stageMs[i] = 0;
+        launches[i] = 0;
+    }
+    framesRendered = 0;
+    pathsStarted = 0;
+}
+
+} // namespace vlra

@@ -1,0 +1,191 @@
+// Device data layout of the light-transport path (shared by host set-up code and kernels).
+//
+// HBM layout, all arrays 16-byte aligned:
+//   BVH nodes      64 B: the two children's boxes stored in the parent (Aila-Laine layout),
+//                  child index > 0 -> inner node, < 0 -> leaf code ~((first << 3) | (count - 1)),
+//                  0 -> empty slot (index 0 is the root, never a child).
+//   BLAS triangles 48 B: three float4 = object-space vertices; .w lanes carry the global
+//                  geometry-instance index and the triangle index inside it.
+//   Instances     128 B: world->object and object->world 3x4 rows plus BLAS bases and
+//                  light-distribution offsets.
+//   Vertices       48 B: (position, u) (normal, v) (tc0Direction, 0).
+//   Path state     per pixel SoA: rng u64, throughput float4, contribution float4,
+//                  meta float4 (wavelength offset, previous dir pdf, initial importance, flags).
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIPCC__)
+#define VLRA_HD __host__ __device__
+#else
+#define VLRA_HD
+#endif
+
+namespace vlra {
+
+constexpr uint32_t kLeafShift = 3;          // leaf code: (first << 3) | (count - 1)
+constexpr uint32_t kMaxLeafPrims = 8;
+constexpr int32_t kInvalidIndex = -1;
+
+struct alignas(16) BVHNode {
+    float lo0[3], hi0[3];
+    float lo1[3], hi1[3];
+    int32_t child[2];
+    uint32_t pad[2];
+};
+static_assert(sizeof(BVHNode) == 64, "node layout");
+
+struct alignas(16) TriRecord {
+    float v0[3]; uint32_t geomInst;
+    float v1[3]; uint32_t prim;
+    float v2[3]; uint32_t pad;
+};
+static_assert(sizeof(TriRecord) == 48, "tri layout");
+
+struct alignas(16) DevVertex {
+    float p[3]; float u;
+    float n[3]; float v;
+    float t[3]; float pad;
+};
+static_assert(sizeof(DevVertex) == 48, "vertex layout");
+
+struct alignas(16) DevInstance {
+    float w2o[12];
+    float o2w[12];
+    uint32_t blasNodeBase;
+    uint32_t blasTriBase;
+    uint32_t geomInstListBase;
+    uint32_t numGeomInsts;
+    uint32_t lightDistBase;   // PMF at base, CDF at base + numGeomInsts
+    float lightDistIntegral;
+    uint32_t hasGeometry;
+    uint32_t pad;
+};
+static_assert(sizeof(DevInstance) == 128, "instance layout");
+
+enum DevMaterialType : uint32_t {
+    MAT_NULL = 0,
+    MAT_MATTE,
+    MAT_SPECULAR_REFLECTION,
+    MAT_SPECULAR_SCATTERING,
+    MAT_MICROFACET_REFLECTION,
+    MAT_UE4,
+    MAT_DIFFUSE_EMITTER,
+    MAT_POINT_EMITTER,
+    MAT_ENVIRONMENT_EMITTER,
+};
+
+// A spectrum parameter: RGB build -> (r, g, b, 0); spectral build -> Meng coefficients
+// (adjacent-point indices packed in .x bits, s | t << 16 in .y bits, scale in .z).
+struct alignas(16) DevSpectrum {
+    float x, y, z, w;
+};
+
+struct alignas(16) DevMaterial {
+    uint32_t type;
+    uint32_t useEnvTexture;
+    uint32_t isEmitter;
+    uint32_t pad;
+    float scalars[4];
+    DevSpectrum spec[3];
+};
+static_assert(sizeof(DevMaterial) == 80, "material layout");
+
+struct alignas(16) DevGeomInst {
+    uint32_t material;
+    uint32_t vertexBase;
+    uint32_t indexBase;     // into the index pool (3 per triangle, 1 per point)
+    uint32_t numPrims;
+    uint32_t distBase;      // prim distribution: PMF at base, CDF at base + numPrims
+    float distIntegral;
+    float importance;
+    uint32_t isPoints;
+};
+static_assert(sizeof(DevGeomInst) == 32, "geomInst layout");
+
+struct MengCellDev { uint8_t inside, numPoints, idx[6]; };
+
+struct DevCamera {
+    float position[3];
+    float rot[9];            // column-major rotation from the orientation quaternion
+    float sensitivity;
+    float lensRadius;
+    float opWidth, opHeight;
+    float imgPlaneDistance, objPlaneDistance, imgPlaneArea;
+};
+
+struct DevScene {
+    const BVHNode* tlasNodes;
+    const BVHNode* blasNodes;
+    const TriRecord* blasTris;
+    const DevInstance* instances;
+    const uint32_t* instGeomInsts;
+    const DevGeomInst* geomInsts;
+    const DevVertex* vertices;
+    const uint32_t* indices;
+    const float* distPool;
+    const DevMaterial* materials;
+    const float4* envTex;
+    const MengCellDev* mengGrid;
+    const float* mengPoints;     // 186 x 99 floats: xy*(2) uv(2) spectrum(95)
+    uint32_t numInstances;
+    uint32_t hasTlas;
+    // light selection over [environment, instances...]
+    uint32_t lightInstBase;
+    uint32_t numLightInsts;
+    float lightInstIntegral;
+    // environment
+    uint32_t hasEnv;
+    uint32_t envMaterial;
+    float envImportance;
+    uint32_t envW, envH;
+    float envRotation;
+    uint32_t envMapW, envMapH;   // importance map size
+    uint32_t envRowBase;         // row r: PDF at envRowBase + r*(2*mapW+1), CDF after it
+    uint32_t envTopBase;         // PDF[mapH], CDF[mapH+1]
+    // scene bounds (setup_scene.cu:82-92)
+    float center[3];
+    float worldRadius, worldDiscArea;
+    DevCamera camera;
+    float cmfX[16], cmfY[16], cmfZ[16];
+    float integralCMF;
+};
+
+// Wavefront queues and per-pixel path state.
+struct PathState {
+    uint64_t* rng;
+    float4* alpha;
+    float4* contrib;
+    float4* meta;     // x wlOffset, y prevDirPDF, z initImportance, w flags (as bits)
+};
+
+struct RayQueue {
+    float4* org;      // xyz, w = path index (bits)
+    float4* dir;      // xyz, w = tmax
+};
+
+struct HitBuffer {
+    int4* a;          // inst, geomInst, prim, t (bits)
+    float2* b;        // b1, b2
+};
+
+struct ShadowQueue {
+    float4* org;      // xyz, w = path index (bits)
+    float4* dir;      // xyz, w = tmax
+    float4* value;    // contribution added when unoccluded
+};
+
+// counters layout (uint32): [2*d] = rays at depth d, [2*d + 1] = shadow rays at depth d,
+// [kStatBase + ...] = statistics
+constexpr uint32_t kMaxDepthCounters = 64;
+constexpr uint32_t kStatClosest = 2 * kMaxDepthCounters + 0;   // 64-bit counters as 2 x u32
+constexpr uint32_t kStatShadow = 2 * kMaxDepthCounters + 2;
+constexpr uint32_t kStatDropped = 2 * kMaxDepthCounters + 4;
+constexpr uint32_t kNumCounters = 2 * kMaxDepthCounters + 8;
+
+// path flag bits in meta.w
+constexpr uint32_t kFlagSelMask = 0x3;
+constexpr uint32_t kFlagSingle = 1u << 2;
+constexpr uint32_t kFlagPrevTypeShift = 8;   // 7 bits of DirectionType
+constexpr uint32_t kFlagPathLenShift = 16;
+
+} // namespace vlra

@@ -1,0 +1,1028 @@
+// Wavefront path tracer for MI355X (gfx950).
+//
+// One vlrContextRender call = one sample per pixel, run as a sequence of kernels joined by
+// device queues (the reference runs the same integrator as one OptiX megakernel,
+// libVLR/GPU_kernels/path_tracing.cu:30-337):
+//   k_frameBegin  clear the per-depth queue counters
+//   k_generate    camera ray per pixel (path_tracing.cu:30-70, cameras.cu:43-76, 145-157)
+//   per depth d:  k_intersect  closest hit over the two-level BVH
+//                 k_shade      hit/miss programs (path_tracing.cu:133-337): implicit emission
+//                              with MIS, Russian roulette, next-event estimation (builds a
+//                              shadow ray, light_transport_common.h:147-192), BSDF sampling;
+//                              appends the next ray and the shadow ray with wave-aggregated
+//                              atomics
+//                 k_shadow     any-hit traversal, adds the NEE contribution if unoccluded
+//   k_accumulate  Kahan accumulation (path_tracing.cu:119-127)
+//   k_convert     XYZ -> Rec.709 output (post_process.cu:93-163)
+// Per-path random numbers are drawn in the reference's order, so a path consumes the same
+// PCG32 stream as the reference's megakernel.
+//
+// Traversal: BVH2 nodes with both child boxes in the parent (64 B), per-thread short stack
+// in LDS with an overflow area in scratch, watertight ray/triangle test (Woop, Benthin and
+// Wald, JCGT 2(1) 2013), compiled without floating-point contraction so that closest hits
+// reproduce the oracle's brute force bit for bit.
+#include "dev_shading.h"
+#include "launch.h"
+
+#pragma clang fp contract(off)
+
+namespace vlra {
+
+constexpr int kTraceBlock = 128;
+constexpr int kLdsStack = 24;
+constexpr int kScratchStack = 48;
+constexpr int32_t kSentinel = INT32_MIN;
+
+// ---------------------------------------------------------------------------------------------
+// ray set-up and primitive tests
+struct RayPre {
+    V3 org, dir, inv;
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+    __device__ __forceinline__ void init(const V3 &o, const V3 &d) {
+        org = o;
+        dir = d;
+        kz = 0;
+        float adx = fabsf(d.x), ady = fabsf(d.y), adz = fabsf(d.z);
+        float best = adx;
+        if (ady > best) { kz = 1; best = ady; }
+        if (adz > best) { kz = 2; }
+        kx = kz == 2 ? 0 : kz + 1;
+        ky = kx == 2 ? 0 : kx + 1;
+        float dkz = d[kz];
+        if (dkz < 0.0f) { int t = kx; kx = ky; ky = t; }
+        Sx = d[kx] / dkz;
+        Sy = d[ky] / dkz;
+        Sz = 1.0f / dkz;
+        inv = V3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    }
+};
+
+__device__ __forceinline__ bool intersectTri(const RayPre &r, const V3 &p0, const V3 &p1, const V3 &p2, float tmin,
+                                             float tmax, float *tOut, float *b1Out, float *b2Out) {
+    const V3 A = p0 - r.org, B = p1 - r.org, C = p2 - r.org;
+    const float Akz = A[r.kz], Bkz = B[r.kz], Ckz = C[r.kz];
+    const float Ax = A[r.kx] - r.Sx * Akz;
+    const float Ay = A[r.ky] - r.Sy * Akz;
+    const float Bx = B[r.kx] - r.Sx * Bkz;
+    const float By = B[r.ky] - r.Sy * Bkz;
+    const float Cx = C[r.kx] - r.Sx * Ckz;
+    const float Cy = C[r.ky] - r.Sy * Ckz;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = static_cast<float>(static_cast<double>(Cx) * static_cast<double>(By) - static_cast<double>(Cy) * static_cast<double>(Bx));
+        V = static_cast<float>(static_cast<double>(Ax) * static_cast<double>(Cy) - static_cast<double>(Ay) * static_cast<double>(Cx));
+        W = static_cast<float>(static_cast<double>(Bx) * static_cast<double>(Ay) - static_cast<double>(By) * static_cast<double>(Ax));
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    const float det = U + V + W;
+    if (det == 0.0f) return false;
+    const float T = U * (r.Sz * Akz) + V * (r.Sz * Bkz) + W * (r.Sz * Ckz);
+    const float rcpDet = 1.0f / det;
+    const float t = T * rcpDet;
+    if (!(t >= tmin && t < tmax)) return false;
+    *tOut = t;
+    *b1Out = V * rcpDet;
+    *b2Out = W * rcpDet;
+    return true;
+}
+
+// Conservative slab test: the exit distance is widened by a few ulps so rounding never culls
+// a box that holds a hit the triangle test would report.
+__device__ __forceinline__ bool slab(const V3 &lo, const V3 &hi, const RayPre &r, float tmin, float tmax,
+                                     float *tEnter) {
+    float tx0 = (lo.x - r.org.x) * r.inv.x, tx1 = (hi.x - r.org.x) * r.inv.x;
+    float ty0 = (lo.y - r.org.y) * r.inv.y, ty1 = (hi.y - r.org.y) * r.inv.y;
+    float tz0 = (lo.z - r.org.z) * r.inv.z, tz1 = (hi.z - r.org.z) * r.inv.z;
+    float tn = fmaxf(fmaxf(fminf(tx0, tx1), fminf(ty0, ty1)), fmaxf(fminf(tz0, tz1), tmin));
+    float tf = fminf(fminf(fmaxf(tx0, tx1), fmaxf(ty0, ty1)), fmaxf(tz0, tz1));
+    tf = fminf(tf * (1.0f + 8.0f * __FLT_EPSILON__), tmax * (1.0f + 8.0f * __FLT_EPSILON__));
+    *tEnter = tn;
+    return tn <= tf;
+}
+
+struct TraceHit {
+    int32_t inst, geomInst, prim;
+    float t, b1, b2;
+};
+
+template <bool ANY>
+__device__ __forceinline__ bool traceRay(const DevScene &sc, const V3 &org, const V3 &dir, float tmin, float tmax,
+                                         TraceHit *hit, int32_t *lds, int lane) {
+    if (!sc.hasTlas) return false;
+    RayPre world;
+    world.init(org, dir);
+    RayPre ray = world;
+    int32_t ovf[kScratchStack];
+    int sp = 0;
+    bool inBlas = false;
+    const BVHNode *nodes = sc.tlasNodes;
+    const TriRecord *tris = sc.blasTris;
+    int32_t curInst = -1;
+    bool found = false;
+    int32_t cur = 0;
+    while (true) {
+        if (cur >= 0) {
+            const float4 *np = reinterpret_cast<const float4 *>(nodes + cur);
+            const float4 na = np[0], nb = np[1], nc = np[2];
+            const int4 nd = reinterpret_cast<const int4 *>(np)[3];
+            float t0, t1;
+            int32_t c0 = nd.x, c1 = nd.y;
+            // child index 0 is the root and therefore marks an empty slot
+            bool h0 = c0 != 0 && slab(V3(na.x, na.y, na.z), V3(na.w, nb.x, nb.y), ray, tmin, tmax, &t0);
+            bool h1 = c1 != 0 && slab(V3(nb.z, nb.w, nc.x), V3(nc.y, nc.z, nc.w), ray, tmin, tmax, &t1);
+            if (h0 && h1) {
+                int32_t nearC = c0, farC = c1;
+                if (t1 < t0) { nearC = c1; farC = c0; }
+                if (sp < kLdsStack) lds[sp * kTraceBlock + lane] = farC;
+                else ovf[min(sp - kLdsStack, kScratchStack - 1)] = farC;
+                ++sp;
+                cur = nearC;
+                continue;
+            } else if (h0) {
+                cur = c0;
+                continue;
+            } else if (h1) {
+                cur = c1;
+                continue;
+            }
+        } else if (cur != kSentinel) {
+            uint32_t code = ~static_cast<uint32_t>(cur);
+            uint32_t first = code >> kLeafShift;
+            uint32_t count = (code & (kMaxLeafPrims - 1)) + 1;
+            if (!inBlas) {
+                // enter the instance: object-space ray, BLAS root, return marker on the stack
+                const DevInstance &I = sc.instances[first];
+                V3 o = xfmPoint(I.w2o, world.org);
+                V3 d = xfmVector(I.w2o, world.dir);
+                ray.init(o, d);
+                curInst = static_cast<int32_t>(first);
+                nodes = sc.blasNodes + I.blasNodeBase;
+                tris = sc.blasTris + I.blasTriBase;
+                inBlas = true;
+                if (sp < kLdsStack) lds[sp * kTraceBlock + lane] = kSentinel;
+                else ovf[min(sp - kLdsStack, kScratchStack - 1)] = kSentinel;
+                ++sp;
+                cur = 0;
+                continue;
+            }
+            for (uint32_t k = 0; k < count; ++k) {
+                const float4 *tp = reinterpret_cast<const float4 *>(tris + first + k);
+                const float4 q0 = tp[0], q1 = tp[1], q2 = tp[2];
+                float t, b1, b2;
+                if (intersectTri(ray, xyz(q0), xyz(q1), xyz(q2), tmin, tmax, &t, &b1, &b2)) {
+                    tmax = t;
+                    found = true;
+                    hit->inst = curInst;
+                    hit->geomInst = static_cast<int32_t>(__float_as_uint(q0.w));
+                    hit->prim = static_cast<int32_t>(__float_as_uint(q1.w));
+                    hit->t = t;
+                    hit->b1 = b1;
+                    hit->b2 = b2;
+                    if (ANY) return true;
+                }
+            }
+        } else {
+            // leave the instance
+            inBlas = false;
+            ray = world;
+            nodes = sc.tlasNodes;
+        }
+        if (sp == 0) break;
+        --sp;
+        cur = sp < kLdsStack ? lds[sp * kTraceBlock + lane] : ovf[min(sp - kLdsStack, kScratchStack - 1)];
+    }
+    return found;
+}
+
+// wave-aggregated queue append: one atomic per wave, lanes get consecutive slots
+__device__ __forceinline__ uint32_t appendSlot(uint32_t *counter, bool active) {
+    uint64_t mask = __ballot(active);
+    if (!active) return 0;
+    uint32_t lanes = __popcll(mask);
+    uint32_t lane = __lane_id();
+    uint32_t rank = __popcll(mask & ((1ull << lane) - 1ull));
+    uint32_t leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, lanes);
+    base = __shfl(base, static_cast<int>(leader));
+    return base + rank;
+}
+
+// ---------------------------------------------------------------------------------------------
+// distributions (shared.h:182-298)
+__device__ __forceinline__ uint32_t nextPow2(uint32_t x) { return x <= 1 ? 1 : 1u << (32 - __clz(x - 1)); }
+
+__device__ __forceinline__ uint32_t discreteSearch(const float *cdf, uint32_t n, float u) {
+    int idx = 0;
+    for (int d = static_cast<int>(nextPow2(n) >> 1); d >= 1; d >>= 1) {
+        if (idx + d >= static_cast<int>(n)) continue;
+        if (cdf[idx + d] <= u) idx += d;
+    }
+    return static_cast<uint32_t>(idx);
+}
+// Discrete1D: PMF at base, CDF (n + 1 entries) after it
+__device__ __forceinline__ uint32_t sampleDiscrete(const float *pool, uint32_t base, uint32_t n, float u, float *prob,
+                                                   float *remapped) {
+    const float *pmf = pool + base, *cdf = pool + base + n;
+    uint32_t idx = discreteSearch(cdf, n, u);
+    *prob = pmf[idx];
+    if (remapped) *remapped = (u - cdf[idx]) / (cdf[idx + 1] - cdf[idx]);
+    return idx;
+}
+// RegularConstantContinuous1D: PDF at base, CDF after it
+__device__ __forceinline__ float sampleContinuous(const float *pool, uint32_t base, uint32_t n, float u, float *pd) {
+    const float *pdf = pool + base, *cdf = pool + base + n;
+    uint32_t idx = discreteSearch(cdf, n, u);
+    *pd = pdf[idx];
+    float t = (u - cdf[idx]) / (cdf[idx + 1] - cdf[idx]);
+    return (idx + t) / n;
+}
+__device__ __forceinline__ float continuousPDF(const float *pool, uint32_t base, uint32_t n, float smp) {
+    int32_t idx = min(static_cast<int32_t>(n) - 1, static_cast<int32_t>(smp * n));
+    return pool[base + idx];
+}
+__device__ __forceinline__ void envMapSample(const DevScene &sc, float u0, float u1, float *d0, float *d1, float *pd) {
+    float topPDF;
+    *d1 = sampleContinuous(sc.distPool, sc.envTopBase, sc.envMapH, u1, &topPDF);
+    uint32_t row = min(static_cast<uint32_t>(sc.envMapH * *d1), sc.envMapH - 1);
+    *d0 = sampleContinuous(sc.distPool, sc.envRowBase + row * (2 * sc.envMapW + 1), sc.envMapW, u0, pd);
+    *pd *= topPDF;
+}
+__device__ __forceinline__ float envMapPDF(const DevScene &sc, float d0, float d1) {
+    uint32_t row = min(static_cast<uint32_t>(sc.envMapH * d1), sc.envMapH - 1);
+    return continuousPDF(sc.distPool, sc.envTopBase, sc.envMapH, d1) *
+           continuousPDF(sc.distPool, sc.envRowBase + row * (2 * sc.envMapW + 1), sc.envMapW, d0);
+}
+
+// bilinear environment fetch, repeat wrapping in both axes
+__device__ __forceinline__ float4 envFetch(const DevScene &sc, float u, float v) {
+    float x = u * sc.envW - 0.5f, y = v * sc.envH - 0.5f;
+    float fx0 = floorf(x), fy0 = floorf(y);
+    float wx = x - fx0, wy = y - fy0;
+    int32_t x0 = static_cast<int32_t>(fx0), y0 = static_cast<int32_t>(fy0);
+    int32_t W = static_cast<int32_t>(sc.envW), H = static_cast<int32_t>(sc.envH);
+    int32_t xa = ((x0 % W) + W) % W, xb = (((x0 + 1) % W) + W) % W;
+    int32_t ya = ((y0 % H) + H) % H, yb = (((y0 + 1) % H) + H) % H;
+    float4 t00 = sc.envTex[ya * W + xa], t10 = sc.envTex[ya * W + xb];
+    float4 t01 = sc.envTex[yb * W + xa], t11 = sc.envTex[yb * W + xb];
+    float4 r;
+    r.x = (1 - wy) * ((1 - wx) * t00.x + wx * t10.x) + wy * ((1 - wx) * t01.x + wx * t11.x);
+    r.y = (1 - wy) * ((1 - wx) * t00.y + wx * t10.y) + wy * ((1 - wx) * t01.y + wx * t11.y);
+    r.z = (1 - wy) * ((1 - wx) * t00.z + wx * t10.z) + wy * ((1 - wx) * t01.z + wx * t11.z);
+    r.w = 1.0f;
+    return r;
+}
+
+template <int N>
+__device__ __forceinline__ Spec<N> envEmittance(const DevScene &sc, const DevMaterial &m, float texU, float texV,
+                                                const Wls<N> &wls) {
+    Spec<N> e;
+    if (!m.useEnvTexture || sc.envTex == nullptr) {
+        e = evalSpectrum<N>(sc, m.spec[0], wls);
+    } else {
+        float4 t = envFetch(sc, texU, texV);
+        if constexpr (N == 3) {
+            e[0] = t.x; e[1] = t.y; e[2] = t.z;
+        } else {
+            uint32_t adj, st;
+            mengAdjacents(sc, t.x, t.y, &adj, &st);
+            e = mengEvaluate(sc, adj, st, t.z / 0.009355121400914532f, wls);
+        }
+    }
+    return e * m.scalars[0];
+}
+
+// ---------------------------------------------------------------------------------------------
+// surface points
+struct SurfPt {
+    V3 position, gn;
+    Frame frame;
+    float texU, texV;
+    bool isPoint, atInfinity;
+};
+
+// triangle.cu:6-53 + renderer_common.h:609-626: decode in object space, then to world
+__device__ __forceinline__ void decodeHit(const DevScene &sc, int32_t instIdx, int32_t giIdx, int32_t prim, float b1,
+                                          float b2, SurfPt *sp, float *hypAreaPDF) {
+    const DevInstance &I = sc.instances[instIdx];
+    const DevGeomInst &G = sc.geomInsts[giIdx];
+    const uint32_t *ix = sc.indices + G.indexBase + 3 * prim;
+    const DevVertex &v0 = sc.vertices[G.vertexBase + ix[0]];
+    const DevVertex &v1 = sc.vertices[G.vertexBase + ix[1]];
+    const DevVertex &v2 = sc.vertices[G.vertexBase + ix[2]];
+    V3 p0 = ld3(v0.p), p1 = ld3(v1.p), p2 = ld3(v2.p);
+    V3 gn = cross(p1 - p0, p2 - p0);
+    float area = gn.length() / 2;
+    gn = gn / (2 * area);
+    float probLightPrim = G.distIntegral > 0.0f ? area / G.distIntegral : 0.0f;
+    *hypAreaPDF = probLightPrim / area;
+    float b0 = 1 - b1 - b2;
+    V3 position = b0 * p0 + b1 * p1 + b2 * p2;
+    V3 sn = normalize(b0 * ld3(v0.n) + b1 * ld3(v1.n) + b2 * ld3(v2.n));
+    V3 tc0 = b0 * ld3(v0.t) + b1 * ld3(v1.t) + b2 * ld3(v2.t);
+    if (!sn.allFinite() || !tc0.allFinite()) {
+        V3 bt;
+        sn = gn;
+        makeCoordinateSystem(sn, &tc0, &bt);
+    }
+    sp->texU = b0 * v0.u + b1 * v1.u + b2 * v2.u;
+    sp->texV = b0 * v0.v + b1 * v1.v + b2 * v2.v;
+    float dotNT = dot(sn, tc0);
+    tc0 = tc0 - dotNT * sn;
+    sp->position = xfmPoint(I.o2w, position);
+    sp->frame = Frame(normalize(xfmVector(I.o2w, tc0)), normalize(xfmNormal(I.w2o, sn)));
+    sp->gn = normalize(xfmNormal(I.w2o, gn));
+    sp->isPoint = false;
+    sp->atInfinity = false;
+}
+
+// environment surface point from a direction (path_tracing.cu:282-310)
+__device__ __forceinline__ void envPointFromDir(const DevScene &sc, const V3 &direction, SurfPt *sp, float *hypAreaPDF) {
+    float theta = acosf(fminf(fmaxf(direction.y, -1.0f), 1.0f));
+    float phi = fmodf(atan2f(-direction.x, direction.z) + 2 * kPiF, 2 * kPiF);
+    float sinPhi, cosPhi;
+    sincosf(phi, &sinPhi, &cosPhi);
+    V3 tc0 = normalize(V3(-cosPhi, 0.0f, -sinPhi));
+    sp->frame.x = tc0;
+    sp->frame.z = -direction;
+    sp->frame.y = cross(sp->frame.z, sp->frame.x);
+    sp->position = direction;
+    sp->gn = -direction;
+    sp->isPoint = false;
+    sp->atInfinity = true;
+    phi += sc.envRotation;
+    phi = phi - floorf(phi / (2 * kPiF)) * 2 * kPiF;
+    sp->texU = phi / (2 * kPiF);
+    sp->texV = theta / kPiF;
+    float uvPDF = envMapPDF(sc, phi / (2 * kPiF), theta / kPiF);
+    *hypAreaPDF = uvPDF / (2 * kPiF * kPiF * sinf(theta));
+}
+
+struct LightSample {
+    SurfPt sp;
+    float areaPDF;
+    uint32_t posType;
+    uint32_t material;
+};
+
+// selectSurfaceLight + sample{TriangleMesh, Point, InfiniteSphere}
+// (light_transport_common.h:179-192, triangle.cu:113-177, point.cu:38-80, infinite_sphere.cu:60-114)
+__device__ __forceinline__ void sampleLight(const DevScene &sc, float uLight, float uP0, float uP1, LightSample *ls,
+                                            float *lightProb) {
+    float instProb, uInst;
+    uint32_t li = sampleDiscrete(sc.distPool, sc.lightInstBase, sc.numLightInsts, uLight, &instProb, &uInst);
+    if (li == 0) {
+        *lightProb = instProb;
+        float u, v, uvPDF;
+        envMapSample(sc, uP0, uP1, &u, &v, &uvPDF);
+        float phi = 2 * kPiF * u;
+        float theta = kPiF * v;
+        float posPhi = phi - sc.envRotation;
+        posPhi = posPhi - floorf(posPhi / (2 * kPiF)) * 2 * kPiF;
+        V3 dir = fromPolarYUp(posPhi, theta);
+        float sinPhi, cosPhi;
+        sincosf(posPhi, &sinPhi, &cosPhi);
+        V3 tc0 = normalize(V3(-cosPhi, 0.0f, -sinPhi));
+        V3 gn = -dir;
+        ls->sp.position = dir;
+        ls->sp.frame.x = tc0;
+        ls->sp.frame.z = gn;
+        ls->sp.frame.y = cross(gn, tc0);
+        ls->sp.gn = gn;
+        ls->sp.isPoint = false;
+        ls->sp.atInfinity = true;
+        ls->sp.texU = phi / (2 * kPiF);
+        ls->sp.texV = theta / kPiF;
+        ls->areaPDF = uvPDF / (2 * kPiF * kPiF * sinf(theta));
+        ls->posType = dt::Reflection | dt::LowFreq;
+        ls->material = sc.envMaterial;
+        return;
+    }
+    const DevInstance &I = sc.instances[li - 1];
+    float geomProb, uPrim;
+    uint32_t lgi = sampleDiscrete(sc.distPool, I.lightDistBase, I.numGeomInsts, uInst, &geomProb, &uPrim);
+    uint32_t gi = sc.instGeomInsts[I.geomInstListBase + lgi];
+    const DevGeomInst &G = sc.geomInsts[gi];
+    *lightProb = instProb * geomProb;
+    float primProb;
+    uint32_t prim = sampleDiscrete(sc.distPool, G.distBase, G.numPrims, uPrim, &primProb, nullptr);
+    ls->material = G.material;
+    if (G.isPoints) {
+        const DevVertex &v = sc.vertices[G.vertexBase + sc.indices[G.indexBase + prim]];
+        ls->areaPDF = primProb / (4 * kPiF);
+        ls->posType = dt::Reflection | dt::Delta0D;
+        ls->sp.position = xfmPoint(I.o2w, ld3(v.p));
+        V3 sn = normalize(xfmNormal(I.w2o, ld3(v.n)));
+        V3 tc0 = normalize(xfmVector(I.o2w, ld3(v.t)));
+        ls->sp.frame = Frame(tc0, sn);
+        ls->sp.gn = sn;
+        ls->sp.isPoint = true;
+        ls->sp.atInfinity = false;
+        ls->sp.texU = ls->sp.texV = 0.0f;
+        return;
+    }
+    const uint32_t *ix = sc.indices + G.indexBase + 3 * prim;
+    const DevVertex &v0 = sc.vertices[G.vertexBase + ix[0]];
+    const DevVertex &v1 = sc.vertices[G.vertexBase + ix[1]];
+    const DevVertex &v2 = sc.vertices[G.vertexBase + ix[2]];
+    V3 e1 = xfmVector(I.o2w, ld3(v1.p) - ld3(v0.p));
+    V3 e2 = xfmVector(I.o2w, ld3(v2.p) - ld3(v0.p));
+    V3 gn = cross(e1, e2);
+    float area = gn.length() / 2;
+    gn = gn / (2 * area);
+    ls->areaPDF = primProb / area;
+    ls->posType = dt::Reflection | dt::LowFreq;
+    float b0, b1;
+    uniformSampleTriangle(uP0, uP1, &b0, &b1);
+    float b2 = 1.0f - b0 - b1;
+    V3 position = b0 * ld3(v0.p) + b1 * ld3(v1.p) + b2 * ld3(v2.p);
+    V3 sn = b0 * ld3(v0.n) + b1 * ld3(v1.n) + b2 * ld3(v2.n);
+    V3 tc0 = b0 * ld3(v0.t) + b1 * ld3(v1.t) + b2 * ld3(v2.t);
+    ls->sp.texU = b0 * v0.u + b1 * v1.u + b2 * v2.u;
+    ls->sp.texV = b0 * v0.v + b1 * v1.v + b2 * v2.v;
+    position = xfmPoint(I.o2w, position);
+    sn = normalize(xfmNormal(I.w2o, sn));
+    tc0 = xfmVector(I.o2w, tc0);
+    float dotNT = dot(sn, tc0);
+    tc0 = normalize(tc0 - dotNT * sn);
+    ls->sp.position = position;
+    ls->sp.frame = Frame(tc0, sn);
+    ls->sp.gn = gn;
+    ls->sp.isPoint = false;
+    ls->sp.atInfinity = false;
+}
+
+// EDF of an emitter material at a surface point (materials.cu:2040-2096, 2194-2250, 2830-2896):
+// returns emittance M (evaluateEmittance) and the directional factor for dirLocal.
+template <int N>
+__device__ __forceinline__ bool evalEDF(const DevScene &sc, const DevMaterial &m, const SurfPt &sp, const Wls<N> &wls,
+                                        const V3 &dirLocal, Spec<N> *M, float *fe) {
+    switch (m.type) {
+    case MAT_DIFFUSE_EMITTER:
+        *M = evalSpectrum<N>(sc, m.spec[0], wls) * m.scalars[0];
+        *fe = dirLocal.z > 0.0f ? 1.0f / kPiF : 0.0f;
+        return true;
+    case MAT_POINT_EMITTER:
+        *M = evalSpectrum<N>(sc, m.spec[0], wls) * m.scalars[0];
+        *fe = 1.0f / (4 * kPiF);
+        return true;
+    case MAT_ENVIRONMENT_EMITTER:
+        *M = kPiF * envEmittance<N>(sc, m, sp.texU, sp.texV, wls);
+        *fe = dirLocal.z > 0.0f ? 1.0f / kPiF : 0.0f;
+        return true;
+    default:
+        return false;
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// path-state packing
+__device__ __forceinline__ uint32_t packFlags(uint32_t sel, bool single, uint32_t prevType, uint32_t pathLen) {
+    return (sel & kFlagSelMask) | (single ? kFlagSingle : 0u) | ((prevType & 0x7Fu) << kFlagPrevTypeShift) |
+           (pathLen << kFlagPathLenShift);
+}
+
+template <int N>
+__device__ __forceinline__ Spec<N> loadSpec(const float4 &v) {
+    Spec<N> s;
+    s[0] = v.x; s[1] = v.y; s[2] = v.z;
+    if constexpr (N == 4) s[3] = v.w;
+    return s;
+}
+template <int N>
+__device__ __forceinline__ float4 storeSpec(const Spec<N> &s) {
+    return make_float4(s[0], s[1], s[2], N == 4 ? s[N - 1] : 0.0f);
+}
+
+struct KArgs {
+    DevScene sc;
+    FrameParams fp;
+    FrameBuffers fb;
+    uint32_t depth;
+};
+
+__global__ void k_frameBegin(uint32_t *counters, uint32_t numPaths) {
+    uint32_t i = threadIdx.x;
+    for (uint32_t k = i; k < 2 * kMaxDepthCounters; k += blockDim.x) counters[k] = 0;
+    if (i == 0) counters[0] = numPaths;
+}
+
+// ---- generate: path_tracing.cu:30-70 ----
+template <int N>
+__global__ void __launch_bounds__(256) k_generate(KArgs a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t numPaths = a.fp.imageW * a.fp.imageH;
+    if (i >= numPaths) return;
+    const uint32_t px = i % a.fp.imageW, py = i / a.fp.imageW;
+    const uint32_t path = py * a.fp.width + px;
+    Pcg32 rng{a.fb.ps.rng[path]};
+    float pX = px + rng.uniform();
+    float pY = py + rng.uniform();
+    float wlOffset = rng.uniform();
+    float wlSelect = rng.uniform();
+    uint32_t sel = min(static_cast<uint32_t>(static_cast<uint16_t>(N * wlSelect)), static_cast<uint32_t>(N - 1));
+    float selectWLPDF = N == 4 ? N / (830.0f - 360.0f) : 1.0f;
+    const DevCamera &cam = a.sc.camera;
+    float ul0 = rng.uniform(), ul1 = rng.uniform();
+    float lx, ly;
+    concentricSampleDisk(ul0, ul1, &lx, &ly);
+    V3 c0(cam.rot[0], cam.rot[1], cam.rot[2]), c1(cam.rot[3], cam.rot[4], cam.rot[5]), c2(cam.rot[6], cam.rot[7], cam.rot[8]);
+    V3 camN = normalize(c2);
+    Frame cf;
+    cf.z = camN;
+    cf.x = normalize(c0);
+    cf.y = cross(cf.z, cf.x);
+    V3 orgLocal(cam.lensRadius * lx, cam.lensRadius * ly, 0.0f);
+    V3 lensPos = (c0 * orgLocal.x + c1 * orgLocal.y + c2 * orgLocal.z) + ld3(cam.position);
+    float lensAreaPDF = cam.lensRadius > 0.0f ? 1.0f / (kPiF * cam.lensRadius * cam.lensRadius) : 1.0f;
+    float uD0 = pX / a.fp.imageW, uD1 = pY / a.fp.imageH;
+    V3 pFocus(cam.opWidth * (0.5f - uD0), cam.opHeight * (0.5f - uD1), cam.objPlaneDistance);
+    V3 dirLocal = normalize(pFocus - V3(cam.lensRadius * lx, cam.lensRadius * ly, 0.0f));
+    float dirPDF = sq(cam.imgPlaneDistance) / (cube(dirLocal.z) * cam.imgPlaneArea);
+    dirPDF *= a.fp.imageW * a.fp.imageH;
+    V3 rayDir = cf.fromLocal(dirLocal);
+    Spec<N> alpha(cam.sensitivity);
+    alpha = alpha * (absDot(rayDir, camN) / (lensAreaPDF * dirPDF * selectWLPDF));
+    float initImportance = alpha.importance(sel);
+    a.fb.ps.rng[path] = rng.s;
+    a.fb.ps.alpha[path] = storeSpec<N>(alpha);
+    a.fb.ps.contrib[path] = make_float4(0, 0, 0, 0);
+    uint32_t flags = packFlags(sel, false, dt::Reflection | dt::LowFreq, 0);
+    a.fb.ps.meta[path] = make_float4(wlOffset, dirPDF, initImportance, __uint_as_float(flags));
+    a.fb.rays[0].org[i] = make_float4(lensPos.x, lensPos.y, lensPos.z, __uint_as_float(path));
+    a.fb.rays[0].dir[i] = make_float4(rayDir.x, rayDir.y, rayDir.z, 3.402823466e+38f);
+}
+
+// ---- intersect: closest hit ----
+__global__ void __launch_bounds__(kTraceBlock) k_intersect(KArgs a) {
+    __shared__ int32_t lds[kLdsStack * kTraceBlock];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const RayQueue q = a.fb.rays[a.depth & 1];
+    const uint32_t count = a.fb.counters[2 * a.depth];
+    if (i >= count) return;
+    float4 o = q.org[i], d = q.dir[i];
+    TraceHit h;
+    h.inst = -1;
+    bool found = traceRay<false>(a.sc, xyz(o), xyz(d), 0.0f, d.w, &h, lds, threadIdx.x);
+    if (found) {
+        a.fb.hits.a[i] = make_int4(h.inst, h.geomInst, h.prim, __float_as_int(h.t));
+        a.fb.hits.b[i] = make_float2(h.b1, h.b2);
+    } else {
+        a.fb.hits.a[i] = make_int4(-1, -1, -1, 0);
+    }
+}
+
+// ---- shade: path_tracing.cu:133-259 (hit) and :266-337 (miss) ----
+template <int N>
+__global__ void __launch_bounds__(256) k_shade(KArgs a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t depth = a.depth;
+    const uint32_t count = a.fb.counters[2 * depth];
+    const bool valid = i < count;
+    bool emitNext = false, emitShadow = false;
+    float4 nextOrg, nextDir, shOrg, shDir, shVal;
+    uint32_t path = 0;
+    if (valid) {
+        const DevScene &sc = a.sc;
+        const RayQueue q = a.fb.rays[depth & 1];
+        float4 o4 = q.org[i], d4 = q.dir[i];
+        path = __float_as_uint(o4.w);
+        V3 rayOrg = xyz(o4), rayDir = xyz(d4);
+        int4 ha = a.fb.hits.a[i];
+        Spec<N> alpha = loadSpec<N>(a.fb.ps.alpha[path]);
+        Spec<N> contrib = loadSpec<N>(a.fb.ps.contrib[path]);
+        float4 meta = a.fb.ps.meta[path];
+        uint32_t flags = __float_as_uint(meta.w);
+        Wls<N> wls;
+        wls.offset = meta.x;
+        wls.sel = flags & kFlagSelMask;
+        wls.single = (flags & kFlagSingle) != 0;
+        float prevDirPDF = meta.y;
+        const float initImportance = meta.z;
+        uint32_t prevType = (flags >> kFlagPrevTypeShift) & 0x7Fu;
+        const uint32_t pathLength = depth + 1;
+        const bool maxLengthTerminate = pathLength >= a.fp.maxPathLength;
+        Pcg32 rng{a.fb.ps.rng[path]};
+        bool rngUsed = false;
+
+        if (ha.x < 0) {
+            // miss: environment light with MIS
+            if (sc.hasEnv && sc.envImportance > 0.0f) {
+                SurfPt sp;
+                float hypAreaPDF;
+                envPointFromDir(sc, rayDir, &sp, &hypAreaPDF);
+                const DevMaterial &m = sc.materials[sc.envMaterial];
+                V3 dirOutLocal = sp.frame.toLocal(-rayDir);
+                Spec<N> M;
+                float fe;
+                evalEDF<N>(sc, m, sp, wls, dirOutLocal, &M, &fe);
+                if (!M.isZero()) {
+                    float mis = 1.0f;
+                    if (!dt::isDelta(prevType) && pathLength > 1) {
+                        float instProb = 1.0f / sc.lightInstIntegral;
+                        float bsdfPDF = prevDirPDF;
+                        float lightPDF = instProb * hypAreaPDF * 1.0f / fabsf(dirOutLocal.z);
+                        mis = (bsdfPDF * bsdfPDF) / (lightPDF * lightPDF + bsdfPDF * bsdfPDF);
+                    }
+                    contrib += alpha * (M * fe) * mis;
+                }
+            }
+        } else {
+            float2 hb = a.fb.hits.b[i];
+            SurfPt sp;
+            float hypAreaPDF;
+            decodeHit(sc, ha.x, ha.y, ha.z, hb.x, hb.y, &sp, &hypAreaPDF);
+            const DevGeomInst &G = sc.geomInsts[ha.y];
+            const DevMaterial &mat = sc.materials[G.material];
+            V3 dirOutLocal = sp.frame.toLocal(-rayDir);
+            bool alive = true;
+            if (mat.isEmitter) {
+                Spec<N> M;
+                float fe;
+                evalEDF<N>(sc, mat, sp, wls, dirOutLocal, &M, &fe);
+                if (!M.isZero()) {
+                    float mis = 1.0f;
+                    if (!dt::isDelta(prevType) && pathLength > 1) {
+                        const DevInstance &I = sc.instances[ha.x];
+                        float instProb = I.lightDistIntegral / sc.lightInstIntegral;
+                        float geomInstProb = G.importance / I.lightDistIntegral;
+                        float bsdfPDF = prevDirPDF;
+                        float dist2 = (sp.position - rayOrg).sqLength();
+                        float lightPDF = instProb * geomInstProb * hypAreaPDF * dist2 / fabsf(dirOutLocal.z);
+                        mis = (bsdfPDF * bsdfPDF) / (lightPDF * lightPDF + bsdfPDF * bsdfPDF);
+                    }
+                    contrib += alpha * (M * fe) * mis;
+                }
+            }
+            if (maxLengthTerminate) alive = false;
+            if (alive) {
+                rngUsed = true;
+                float continueProb = fminf(alpha.importance(wls.sel) / initImportance, 1.0f);
+                if (rng.uniform() >= continueProb) alive = false;
+                else alpha = alpha / continueProb;
+            }
+            if (alive) {
+                Bsdf<N> bsdf = setupBsdf<N>(sc, mat, wls);
+                V3 gnL = sp.frame.toLocal(sp.gn);
+                if (bsdf.hasNonDelta()) {
+                    float uLight = rng.uniform();
+                    float uP0 = rng.uniform(), uP1 = rng.uniform();
+                    // a scene without emitters has nothing to sample (the draws are still made)
+                    if (sc.lightInstIntegral > 0.0f) {
+                    LightSample ls;
+                    float lightProb;
+                    sampleLight(sc, uLight, uP0, uP1, &ls, &lightProb);
+                    const DevMaterial &lm = sc.materials[ls.material];
+                    // shadow ray direction (SurfacePoint::calcDirectionFrom)
+                    V3 sdir;
+                    float dist2;
+                    if (ls.sp.atInfinity) {
+                        dist2 = 1.0f;
+                        sdir = normalize(ls.sp.position);
+                    } else {
+                        V3 dv = ls.sp.position - sp.position;
+                        dist2 = dv.sqLength();
+                        sdir = dv / sqrtf(dist2);
+                    }
+                    V3 sdl = ls.sp.frame.toLocal(-sdir);
+                    Spec<N> M;
+                    float fe;
+                    evalEDF<N>(sc, lm, ls.sp, wls, sdl, &M, &fe);
+                    if (!M.isZero()) {
+                        float rec = 1.0f / dist2;
+                        V3 sdsn = sp.frame.toLocal(sdir);
+                        float lightPDF = lightProb * ls.areaPDF;
+                        Spec<N> fs = bsdf.evaluate(dirOutLocal, gnL, sdsn);
+                        float cosLight = ls.sp.isPoint ? 1.0f : absDot(-sdir, ls.sp.gn);
+                        float bsdfPDF = bsdf.evaluatePDF(dirOutLocal, sdsn, wls.sel) * cosLight * rec;
+                        float mis = 1.0f;
+                        if (!dt::isDelta(ls.posType) && !isinf(lightPDF))
+                            mis = (lightPDF * lightPDF) / (lightPDF * lightPDF + bsdfPDF * bsdfPDF);
+                        float G = 1.0f * absDot(sdsn, gnL) * cosLight * rec;
+                        float scalarCoeff = G * mis / lightPDF;
+                        Spec<N> value = alpha * (M * fe) * fs * scalarCoeff;
+                        // testVisibility (light_transport_common.h:147-177)
+                        bool front = dot(sp.gn, sdir) > 0;
+                        V3 so = offsetRayOrigin(sp.position, front ? sp.gn : -sp.gn);
+                        front = -dot(ls.sp.gn, sdir) > 0;
+                        V3 lo = offsetRayOrigin(ls.sp.position, front ? ls.sp.gn : -ls.sp.gn);
+                        float tmax = 3.402823466e+38f;
+                        if (!ls.sp.atInfinity) tmax = (lo - so).length() * 0.9999f;
+                        emitShadow = true;
+                        shOrg = make_float4(so.x, so.y, so.z, __uint_as_float(path));
+                        shDir = make_float4(sdir.x, sdir.y, sdir.z, tmax);
+                        shVal = storeSpec<N>(value);
+                    }
+                    }
+                }
+                float uC = rng.uniform(), uD0 = rng.uniform(), uD1 = rng.uniform();
+                V3 dirL;
+                float dirPDF;
+                uint32_t sType;
+                Spec<N> fs = bsdf.sample(dirOutLocal, gnL, wls.sel, uC, uD0, uD1, &dirL, &dirPDF, &sType);
+                if (!(fs.isZero() || dirPDF == 0.0f)) {
+                    if ((sType & dt::Dispersive) && !wls.single) {
+                        dirPDF /= N;
+                        wls.single = true;
+                    }
+                    float cosFactor = dot(dirL, gnL);
+                    alpha = alpha * (fs * (fabsf(cosFactor) / dirPDF));
+                    V3 dirIn = sp.frame.fromLocal(dirL);
+                    V3 no = offsetRayOrigin(sp.position, cosFactor > 0.0f ? sp.gn : -sp.gn);
+                    emitNext = true;
+                    nextOrg = make_float4(no.x, no.y, no.z, __uint_as_float(path));
+                    nextDir = make_float4(dirIn.x, dirIn.y, dirIn.z, 3.402823466e+38f);
+                    prevDirPDF = dirPDF;
+                    prevType = sType;
+                }
+            }
+        }
+        a.fb.ps.contrib[path] = storeSpec<N>(contrib);
+        if (rngUsed) a.fb.ps.rng[path] = rng.s;
+        if (emitNext) {
+            a.fb.ps.alpha[path] = storeSpec<N>(alpha);
+            a.fb.ps.meta[path] = make_float4(wls.offset, prevDirPDF, initImportance,
+                                             __uint_as_float(packFlags(wls.sel, wls.single, prevType, pathLength)));
+        }
+    }
+    // queue appends (every lane of the wave takes part in the ballots)
+    uint32_t slot = appendSlot(&a.fb.counters[2 * (depth + 1)], emitNext);
+    if (emitNext) {
+        const RayQueue nq = a.fb.rays[(depth + 1) & 1];
+        nq.org[slot] = nextOrg;
+        nq.dir[slot] = nextDir;
+    }
+    uint32_t sslot = appendSlot(&a.fb.counters[2 * depth + 1], emitShadow);
+    if (emitShadow) {
+        a.fb.shadow.org[sslot] = shOrg;
+        a.fb.shadow.dir[sslot] = shDir;
+        a.fb.shadow.value[sslot] = shVal;
+    }
+}
+
+// ---- shadow: any-hit, adds the NEE contribution when unoccluded ----
+__global__ void __launch_bounds__(kTraceBlock) k_shadow(KArgs a) {
+    __shared__ int32_t lds[kLdsStack * kTraceBlock];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t count = a.fb.counters[2 * a.depth + 1];
+    if (i >= count) return;
+    float4 o = a.fb.shadow.org[i], d = a.fb.shadow.dir[i];
+    TraceHit h;
+    bool occluded = traceRay<true>(a.sc, xyz(o), xyz(d), 0.0f, d.w, &h, lds, threadIdx.x);
+    if (!occluded) {
+        uint32_t path = __float_as_uint(o.w);
+        float4 v = a.fb.shadow.value[i];
+        float4 c = a.fb.ps.contrib[path];
+        a.fb.ps.contrib[path] = make_float4(c.x + v.x, c.y + v.y, c.z + v.z, c.w + v.w);
+    }
+}
+
+// ---- accumulate: Kahan sum per channel (common_internal.h:350-368, spectrum_types.h:674-684) ----
+template <int N>
+__global__ void __launch_bounds__(256) k_accumulate(KArgs a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t numPaths = a.fp.imageW * a.fp.imageH;
+    if (i >= numPaths) return;
+    const uint32_t px = i % a.fp.imageW, py = i / a.fp.imageW;
+    const uint32_t p = py * a.fp.width + px;
+    constexpr int C = N == 4 ? 16 : 3;
+    Spec<N> c = loadSpec<N>(a.fb.ps.contrib[p]);
+    bool finite = c.allFinite();
+    float *acc = a.fb.accum + static_cast<size_t>(p) * C * 2;
+    const bool reset = a.fp.frameIndex == 1;
+    if (!finite) {
+        atomicAdd(&a.fb.counters[kStatDropped], 1u);
+        if (reset) {
+            for (int k = 0; k < 2 * C; ++k) acc[k] = 0.0f;
+        }
+        return;
+    }
+    float addend[C];
+    if constexpr (N == 4) {
+        float wlOffset = a.fb.ps.meta[p].x;
+        const float recBinWidth = 16 / (830.0f - 360.0f);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) addend[k] = 0.0f;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            float lam = 360.0f + (830.0f - 360.0f) * (w + wlOffset) / 4;
+            uint32_t bin = min(static_cast<uint32_t>((lam - 360.0f) / (830.0f - 360.0f) * 16), 15u);
+#pragma unroll
+            for (int k = 0; k < 16; ++k)
+                if (k == static_cast<int>(bin)) addend[k] += c[w] * recBinWidth;
+        }
+    } else {
+        addend[0] = c[0]; addend[1] = c[1]; addend[2] = c[2];
+    }
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+        float sum = reset ? 0.0f : acc[2 * k];
+        float comp = reset ? 0.0f : acc[2 * k + 1];
+        float cIn = addend[k] - comp;
+        float tmp = sum + cIn;
+        comp = (tmp - sum) - cIn;
+        acc[2 * k] = tmp;
+        acc[2 * k + 1] = comp;
+    }
+}
+
+// ---- convert: post_process.cu:93-163 ----
+template <int N>
+__global__ void __launch_bounds__(256) k_convert(KArgs a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t numPaths = a.fp.imageW * a.fp.imageH;
+    if (i >= numPaths) return;
+    const uint32_t px = i % a.fp.imageW, py = i / a.fp.imageW;
+    const uint32_t p = py * a.fp.width + px;
+    constexpr int C = N == 4 ? 16 : 3;
+    const float *acc = a.fb.accum + static_cast<size_t>(p) * C * 2;
+    float X, Y, Z;
+    if constexpr (N == 4) {
+        X = Y = Z = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            float v = acc[2 * k];
+            X += a.sc.cmfX[k] * v;
+            Y += a.sc.cmfY[k] * v;
+            Z += a.sc.cmfZ[k] * v;
+        }
+        X /= a.sc.integralCMF;
+        Y /= a.sc.integralCMF;
+        Z /= a.sc.integralCMF;
+    } else {
+        float r = acc[0], g = acc[2], b = acc[4];
+        X = 0.4124564f * r + 0.3575761f * g + 0.1804375f * b;
+        Y = 0.2126729f * r + 0.7151522f * g + 0.0721750f * b;
+        Z = 0.0193339f * r + 0.1191920f * g + 0.9503041f * b;
+    }
+    float rec = 1.0f / a.fp.frameIndex;
+    X *= rec; Y *= rec; Z *= rec;
+    float R = 3.2404542f * X + -1.5371385f * Y + -0.4985314f * Z;
+    float G = -0.9692660f * X + 1.8760108f * Y + 0.0415560f * Z;
+    float B = 0.0556434f * X + -0.2040259f * Y + 1.0572252f * Z;
+    float res = static_cast<float>(a.fp.imageW) * static_cast<float>(a.fp.imageH);
+    a.fb.output[p] = make_float4(R * res, G * res, B * res, 1.0f);
+}
+
+__global__ void k_frameEnd(uint32_t *counters, uint32_t maxPathLength) {
+    // fold this frame's queue sizes into the 64-bit ray statistics
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    uint64_t closest = 0, shadow = 0;
+    for (uint32_t d = 0; d < maxPathLength && d < kMaxDepthCounters - 1; ++d) {
+        closest += counters[2 * d];
+        shadow += counters[2 * d + 1];
+    }
+    uint64_t *c64 = reinterpret_cast<uint64_t *>(counters + kStatClosest);
+    c64[0] += closest;
+    c64[1] += shadow;
+}
+
+// ---------------------------------------------------------------------------------------------
+// entry kernels for parity tests
+struct RayIn { float org[3]; float tmin; float dir[3]; float tmax; };
+struct HitOut { int32_t inst, geomInst, prim; float t, b1, b2; };
+
+template <bool ANY>
+__global__ void __launch_bounds__(kTraceBlock) k_traceRays(DevScene sc, const RayIn *rays, uint32_t n, HitOut *hits) {
+    __shared__ int32_t lds[kLdsStack * kTraceBlock];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    RayIn r = rays[i];
+    TraceHit h;
+    h.inst = -1; h.geomInst = -1; h.prim = -1; h.t = 0; h.b1 = 0; h.b2 = 0;
+    bool f = traceRay<ANY>(sc, V3(r.org[0], r.org[1], r.org[2]), V3(r.dir[0], r.dir[1], r.dir[2]), r.tmin, r.tmax,
+                           &h, lds, threadIdx.x);
+    HitOut o;
+    o.inst = f ? h.inst : -1;
+    o.geomInst = f ? h.geomInst : -1;
+    o.prim = f ? h.prim : -1;
+    o.t = f ? h.t : 0.0f;
+    o.b1 = f ? h.b1 : 0.0f;
+    o.b2 = f ? h.b2 : 0.0f;
+    hits[i] = o;
+}
+
+struct BsdfQueryIn {
+    float dirLocal[3];
+    float gnLocal[3];
+    float wlOffset, wlSelect;
+    float uComponent, uDir[2];
+    float evalDir[3];
+    int32_t single;
+};
+struct BsdfResultOut {
+    float sampledDir[3];
+    float sampledPDF;
+    uint32_t sampledType;
+    float sampledValue[4];
+    float evalValue[4];
+    float evalPDF;
+    float revValue[4];
+    float revPDF;
+};
+
+template <int N>
+__global__ void k_bsdfQueries(DevScene sc, DevMaterial m, const BsdfQueryIn *q, uint32_t n, BsdfResultOut *out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    BsdfQueryIn Q = q[i];
+    Wls<N> wls;
+    wls.offset = Q.wlOffset;
+    wls.sel = min(static_cast<uint32_t>(static_cast<uint16_t>(N * Q.wlSelect)), static_cast<uint32_t>(N - 1));
+    wls.single = Q.single != 0;
+    Bsdf<N> b = setupBsdf<N>(sc, m, wls);
+    V3 qd(Q.dirLocal[0], Q.dirLocal[1], Q.dirLocal[2]);
+    V3 gn(Q.gnLocal[0], Q.gnLocal[1], Q.gnLocal[2]);
+    V3 dir(0, 0, 0);
+    float pdf;
+    uint32_t type;
+    Spec<N> f = b.sample(qd, gn, wls.sel, Q.uComponent, Q.uDir[0], Q.uDir[1], &dir, &pdf, &type);
+    BsdfResultOut o;
+    o.sampledDir[0] = dir.x; o.sampledDir[1] = dir.y; o.sampledDir[2] = dir.z;
+    o.sampledPDF = pdf;
+    o.sampledType = type;
+    V3 ed(Q.evalDir[0], Q.evalDir[1], Q.evalDir[2]);
+    Spec<N> e = b.evaluate(qd, gn, ed);
+    o.evalPDF = b.evaluatePDF(qd, ed, wls.sel);
+    for (int c = 0; c < 4; ++c) {
+        o.sampledValue[c] = c < N ? f[c] : 0.0f;
+        o.evalValue[c] = c < N ? e[c] : 0.0f;
+        o.revValue[c] = 0.0f;
+    }
+    o.revPDF = 0.0f;
+    out[i] = o;
+}
+
+// ---------------------------------------------------------------------------------------------
+// launchers
+static inline uint32_t blocksFor(uint32_t n, uint32_t bs) { return (n + bs - 1) / bs; }
+
+template <int N>
+static hipError_t launchFrameT(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb, hipStream_t s,
+                               LaunchTimer *timer) {
+    KArgs a;
+    a.sc = sc;
+    a.fp = fp;
+    a.fb = fb;
+    a.depth = 0;
+    const uint32_t n = fp.imageW * fp.imageH;
+    auto t0 = [&](int st) { if (timer) timer->begin(st); };
+    auto t1 = [&](int st) { if (timer) timer->end(st); };
+    hipLaunchKernelGGL(k_frameBegin, dim3(1), dim3(256), 0, s, fb.counters, n);
+    t0(STAGE_GENERATE);
+    hipLaunchKernelGGL(k_generate<N>, dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
+    t1(STAGE_GENERATE);
+    for (uint32_t d = 0; d < fp.maxPathLength && d + 1 < kMaxDepthCounters; ++d) {
+        a.depth = d;
+        t0(STAGE_INTERSECT);
+        hipLaunchKernelGGL(k_intersect, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
+        t1(STAGE_INTERSECT);
+        t0(STAGE_SHADE);
+        hipLaunchKernelGGL(k_shade<N>, dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
+        t1(STAGE_SHADE);
+        t0(STAGE_SHADOW);
+        hipLaunchKernelGGL(k_shadow, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
+        t1(STAGE_SHADOW);
+    }
+    t0(STAGE_ACCUMULATE);
+    hipLaunchKernelGGL(k_accumulate<N>, dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
+    t1(STAGE_ACCUMULATE);
+    t0(STAGE_CONVERT);
+    hipLaunchKernelGGL(k_convert<N>, dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
+    t1(STAGE_CONVERT);
+    hipLaunchKernelGGL(k_frameEnd, dim3(1), dim3(64), 0, s, fb.counters, fp.maxPathLength);
+    return hipGetLastError();
+}
+
+hipError_t launchFrame(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb, bool spectral,
+                       hipStream_t s, LaunchTimer *timer) {
+    return spectral ? launchFrameT<4>(sc, fp, fb, s, timer) : launchFrameT<3>(sc, fp, fb, s, timer);
+}
+
+hipError_t launchTraceRays(const DevScene &sc, const void *rays, uint32_t n, void *hits, bool anyHit, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    if (anyHit)
+        hipLaunchKernelGGL(k_traceRays<true>, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, sc,
+                           static_cast<const RayIn *>(rays), n, static_cast<HitOut *>(hits));
+    else
+        hipLaunchKernelGGL(k_traceRays<false>, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, sc,
+                           static_cast<const RayIn *>(rays), n, static_cast<HitOut *>(hits));
+    return hipGetLastError();
+}
+
+hipError_t launchBsdfQueries(const DevScene &sc, const DevMaterial &m, bool spectral, const void *q, uint32_t n,
+                             void *r, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    if (spectral)
+        hipLaunchKernelGGL(k_bsdfQueries<4>, dim3(blocksFor(n, 128)), dim3(128), 0, s, sc, m,
+                           static_cast<const BsdfQueryIn *>(q), n, static_cast<BsdfResultOut *>(r));
+    else
+        hipLaunchKernelGGL(k_bsdfQueries<3>, dim3(blocksFor(n, 128)), dim3(128), 0, s, sc, m,
+                           static_cast<const BsdfQueryIn *>(q), n, static_cast<BsdfResultOut *>(r));
+    return hipGetLastError();
+}
+
+} // namespace vlra

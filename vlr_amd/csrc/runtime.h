@@ -1,0 +1,412 @@
+// Host runtime behind the VLR C API on MI355X.
+//
+// Object model mirrors the reference's scene graph (libVLR/scene.h:137-530, materials.h,
+// shader_nodes.h, image.h, queryable.h): surface nodes own vertices and material groups,
+// parent nodes (internal nodes and the scene) own transforms and children, materials and
+// shader nodes are queryable by parameter name.  Context::render flattens the graph into
+// the device layout of dev_types.h (instances, geometry instances, BVHs, light
+// distributions) and launches the wavefront kernels of kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "dev_types.h"
+#include "launch.h"
+
+#include "../../include/vlr/public_types.h"
+
+namespace vlra {
+
+class Context;
+
+// ---- errors ----
+struct ApiError {
+    int code;   // VLRResult value
+    std::string msg;
+};
+
+// ---- colour ----
+enum class SpectrumType : uint32_t { Reflectance = 0, LightSource = 1, IndexOfRefraction = 2, NA = 3 };
+enum class ColorSpace : uint32_t { Rec709_D65_sRGBGamma = 0, Rec709_D65 = 1, XYZ = 2, xyY = 3 };
+
+struct ImmSpectrum {
+    ColorSpace cs = ColorSpace::Rec709_D65;
+    float e[3] = {0, 0, 0};
+};
+
+// spectral_host.cpp
+bool loadSpectralTables();
+const std::vector<MengCellDev> &mengGrid();
+const std::vector<float> &mengPoints();
+void cmfBins(float x[16], float y[16], float z[16], float *integral);
+// device spectrum parameter for an immediate spectrum (RGB build: r,g,b; spectral: Meng)
+DevSpectrum makeDevSpectrum(SpectrumType type, const ImmSpectrum &s, bool spectral);
+bool colorSpaceFromString(const char *s, ColorSpace *cs);
+const char *colorSpaceToString(ColorSpace cs);
+bool spectrumTypeFromString(const char *s, SpectrumType *t);
+void mengXYtoUV(const float xy[2], float uv[2]);
+uint16_t floatToHalf(float f);
+float halfToFloat(uint16_t h);
+
+// ---- objects ----
+enum class ObjType : uint32_t {
+    Context = 1,
+    ParameterInfo,
+    LinearImage2D,
+    BlockCompressedImage2D,
+    ShaderNode,
+    SurfaceMaterial,
+    StaticTransform,
+    TriangleMeshSurfaceNode,
+    PointSurfaceNode,
+    InternalNode,
+    Scene,
+    Camera,
+};
+
+constexpr uint32_t kObjMagic = 0x564C5241u;   // "VLRA"
+
+struct ParamInfo;
+
+class Object {
+public:
+    const uint32_t magic = kObjMagic;
+    const ObjType type;
+    Context *ctx;
+    Object(ObjType t, Context *c) : type(t), ctx(c) {}
+    virtual ~Object();
+    virtual const char *typeName() const = 0;
+    bool isA(const char *abstractName) const;
+    bool isQueryable() const;
+    bool isImage() const { return type == ObjType::LinearImage2D || type == ObjType::BlockCompressedImage2D; }
+    bool isNode() const {
+        return type == ObjType::TriangleMeshSurfaceNode || type == ObjType::PointSurfaceNode ||
+               type == ObjType::InternalNode || type == ObjType::Scene;
+    }
+    bool isSurfaceNode() const { return type == ObjType::TriangleMeshSurfaceNode || type == ObjType::PointSurfaceNode; }
+    bool isParentNode() const { return type == ObjType::InternalNode || type == ObjType::Scene; }
+};
+
+// Queryable parameter description (queryable.h:25-70)
+struct ParamInfo : public Object {
+    std::string name;
+    uint32_t formFlags;
+    std::string typeStr;
+    uint32_t tupleSize;
+    ParamInfo(const char *n, uint32_t form, const char *t, uint32_t tuple = 1)
+        : Object(ObjType::ParameterInfo, nullptr), name(n), formFlags(form), typeStr(t), tupleSize(tuple) {}
+    const char *typeName() const override { return "ParameterInfo"; }
+};
+
+class ShaderNode;
+class Image2D;
+class SurfaceMaterial;
+
+struct PlugRef {
+    const ShaderNode *node = nullptr;
+    uint32_t plugType = 0;
+    uint32_t option = 0;
+    bool valid() const { return node != nullptr; }
+};
+
+class Queryable : public Object {
+public:
+    using Object::Object;
+    virtual const std::vector<ParamInfo *> &params() const = 0;
+    // generic parameter access; returns false when the name/form does not apply
+    virtual bool getEnum(const char *, const char **) const { return false; }
+    virtual bool setEnum(const char *, const char *) { return false; }
+    virtual bool getPoint(const char *, float *) const { return false; }
+    virtual bool setPoint(const char *, const float *) { return false; }
+    virtual bool getVector(const char *, float *) const { return false; }
+    virtual bool setVector(const char *, const float *) { return false; }
+    virtual bool getNormal(const char *, float *) const { return false; }
+    virtual bool setNormal(const char *, const float *) { return false; }
+    virtual bool getQuaternion(const char *, float *) const { return false; }
+    virtual bool setQuaternion(const char *, const float *) { return false; }
+    virtual bool getFloatTuple(const char *, float *, uint32_t) const { return false; }
+    virtual bool setFloatTuple(const char *, const float *, uint32_t) { return false; }
+    virtual bool getFloatArray(const char *, const float **, uint32_t *) const { return false; }
+    virtual bool getImage(const char *, const Image2D **) const { return false; }
+    virtual bool setImage(const char *, const Image2D *) { return false; }
+    virtual bool getSpectrum(const char *, ImmSpectrum *) const { return false; }
+    virtual bool setSpectrum(const char *, const ImmSpectrum &) { return false; }
+    virtual bool getMaterial(const char *, const SurfaceMaterial **) const { return false; }
+    virtual bool setMaterial(const char *, const SurfaceMaterial *) { return false; }
+    virtual bool getPlug(const char *, PlugRef *) const { return false; }
+    // returns 0 ok, 1 unknown name, 2 incompatible node
+    virtual int setPlug(const char *, const PlugRef &) { return 1; }
+};
+
+// ---- images (image.h) ----
+class Image2D : public Queryable {
+public:
+    uint32_t width = 0, height = 0;
+    std::string formatStr;
+    SpectrumType spectrumType = SpectrumType::NA;
+    ColorSpace colorSpace = ColorSpace::Rec709_D65;
+    bool hasAlpha = false;
+    using Queryable::Queryable;
+    const std::vector<ParamInfo *> &params() const override;
+    virtual uint32_t stride() const = 0;
+    // RGBA linear floats for environment use; false when the format cannot be decoded
+    virtual bool toRGBA32F(std::vector<float> *out) const = 0;
+};
+
+class LinearImage2D : public Image2D {
+public:
+    std::vector<uint8_t> data;
+    LinearImage2D(Context *c) : Image2D(ObjType::LinearImage2D, c) {}
+    const char *typeName() const override { return "LinearImage2D"; }
+    uint32_t stride() const override;
+    bool toRGBA32F(std::vector<float> *out) const override;
+};
+
+class BlockCompressedImage2D : public Image2D {
+public:
+    std::vector<std::vector<uint8_t>> mips;
+    BlockCompressedImage2D(Context *c) : Image2D(ObjType::BlockCompressedImage2D, c) {}
+    const char *typeName() const override { return "BlockCompressedImage2D"; }
+    uint32_t stride() const override { return 0; }
+    bool toRGBA32F(std::vector<float> *) const override { return false; }
+};
+
+// ---- shader nodes (shader_nodes.h) ----
+enum class NodeKind : uint32_t {
+    Geometry, Tangent, Float2, Float3, Float4, ScaleAndOffsetFloat, TripletSpectrum, RegularSampledSpectrum,
+    IrregularSampledSpectrum, Float3ToSpectrum, ScaleAndOffsetUVTextureMap2D, Image2DTexture, EnvironmentTexture,
+};
+
+class ShaderNode : public Queryable {
+public:
+    NodeKind kind;
+    std::map<std::string, std::vector<float>> floats;     // immediate float tuples / arrays
+    std::map<std::string, PlugRef> plugs;
+    std::map<std::string, std::string> enums;
+    const Image2D *image = nullptr;
+    ShaderNode(Context *c, NodeKind k);
+    const char *typeName() const override;
+    const std::vector<ParamInfo *> &params() const override;
+    bool providesPlug(uint32_t plugType) const;
+    bool getEnum(const char *, const char **) const override;
+    bool setEnum(const char *, const char *) override;
+    bool getFloatTuple(const char *, float *, uint32_t) const override;
+    bool setFloatTuple(const char *, const float *, uint32_t) override;
+    bool getFloatArray(const char *, const float **, uint32_t *) const override;
+    bool getImage(const char *, const Image2D **) const override;
+    bool setImage(const char *, const Image2D *) override;
+    bool getPlug(const char *, PlugRef *) const override;
+    int setPlug(const char *, const PlugRef &) override;
+    // constant folding of a plug into an immediate value (returns false if not constant)
+    bool constantFloat(uint32_t option, float *v) const;
+    bool constantSpectrum(ImmSpectrum *s, SpectrumType *t) const;
+};
+
+// ---- materials (materials.h) ----
+enum class MatKind : uint32_t {
+    Matte, SpecularReflection, SpecularScattering, MicrofacetReflection, MicrofacetScattering, LambertianScattering,
+    UE4, OldStyle, DiffuseEmitter, DirectionalEmitter, PointEmitter, Multi, EnvironmentEmitter,
+};
+
+class SurfaceMaterial : public Queryable {
+public:
+    MatKind kind;
+    // spectrum slots (names per kind), float slots, node plugs per parameter name
+    std::vector<std::string> specNames;
+    std::vector<ImmSpectrum> specs;
+    std::vector<std::string> floatNames;
+    std::vector<float> floatVals;
+    std::map<std::string, PlugRef> plugs;
+    const SurfaceMaterial *sub[4] = {nullptr, nullptr, nullptr, nullptr};
+    float direction[3] = {0, 0, 1};
+    SurfaceMaterial(Context *c, MatKind k);
+    const char *typeName() const override;
+    const std::vector<ParamInfo *> &params() const override;
+    bool isEmitting() const {
+        return kind == MatKind::DiffuseEmitter || kind == MatKind::DirectionalEmitter ||
+               kind == MatKind::PointEmitter || kind == MatKind::EnvironmentEmitter;
+    }
+    bool supportedOnRenderPath() const;
+    bool getSpectrum(const char *, ImmSpectrum *) const override;
+    bool setSpectrum(const char *, const ImmSpectrum &) override;
+    bool getFloatTuple(const char *, float *, uint32_t) const override;
+    bool setFloatTuple(const char *, const float *, uint32_t) override;
+    bool getVector(const char *, float *) const override;
+    bool setVector(const char *, const float *) override;
+    bool getMaterial(const char *, const SurfaceMaterial **) const override;
+    bool setMaterial(const char *, const SurfaceMaterial *) override;
+    bool getPlug(const char *, PlugRef *) const override;
+    int setPlug(const char *, const PlugRef &) override;
+    // device descriptor (immediate values with constant node plugs folded in)
+    DevMaterial toDevice(bool spectral, bool *usesEnvTexture, const Image2D **envImage) const;
+};
+
+// ---- transforms ----
+class StaticTransform : public Object {
+public:
+    float mat[16];      // column-major
+    float invMat[16];
+    StaticTransform(Context *c, const float m[16]);
+    const char *typeName() const override { return "StaticTransform"; }
+};
+
+// ---- nodes (scene.h) ----
+class ParentNode;
+
+class Node : public Object {
+public:
+    std::string name;
+    std::set<ParentNode *> parents;
+    using Object::Object;
+};
+
+struct MaterialGroup {
+    std::vector<uint32_t> indices;
+    const SurfaceMaterial *material = nullptr;
+    PlugRef nodeNormal, nodeTangent, nodeAlpha;
+};
+
+class SurfaceNode : public Node {
+public:
+    std::vector<VLRVertex> vertices;
+    std::vector<MaterialGroup> groups;
+    uint64_t version = 0;
+    using Node::Node;
+};
+
+class TriangleMeshSurfaceNode : public SurfaceNode {
+public:
+    TriangleMeshSurfaceNode(Context *c) : SurfaceNode(ObjType::TriangleMeshSurfaceNode, c) {}
+    const char *typeName() const override { return "TriangleMeshSurfaceNode"; }
+};
+
+class PointSurfaceNode : public SurfaceNode {
+public:
+    PointSurfaceNode(Context *c) : SurfaceNode(ObjType::PointSurfaceNode, c) {}
+    const char *typeName() const override { return "PointSurfaceNode"; }
+};
+
+class ParentNode : public Node {
+public:
+    const StaticTransform *transform = nullptr;
+    std::vector<Node *> children;
+    using Node::Node;
+};
+
+class InternalNode : public ParentNode {
+public:
+    InternalNode(Context *c) : ParentNode(ObjType::InternalNode, c) {}
+    const char *typeName() const override { return "InternalNode"; }
+};
+
+class Scene : public ParentNode {
+public:
+    const SurfaceMaterial *envMaterial = nullptr;
+    float envRotation = 0.0f;
+    Scene(Context *c) : ParentNode(ObjType::Scene, c) {}
+    const char *typeName() const override { return "Scene"; }
+};
+
+// ---- camera ----
+class Camera : public Queryable {
+public:
+    bool perspective = true;
+    float position[3] = {0, 0, 0};
+    float orientation[4] = {0, 0, 0, 1};
+    float sensitivity = 1.0f, aspect = 1.0f, fovY = 0.8726646f, lensRadius = 0.0f, opDistance = 1.0f;
+    float phiAngle = 6.28318530718f, thetaAngle = 3.14159265359f;
+    Camera(Context *c, bool persp) : Queryable(ObjType::Camera, c), perspective(persp) {}
+    const char *typeName() const override { return perspective ? "PerspectiveCamera" : "EquirectangularCamera"; }
+    const std::vector<ParamInfo *> &params() const override;
+    bool getPoint(const char *, float *) const override;
+    bool setPoint(const char *, const float *) override;
+    bool getQuaternion(const char *, float *) const override;
+    bool setQuaternion(const char *, const float *) override;
+    bool getFloatTuple(const char *, float *, uint32_t) const override;
+    bool setFloatTuple(const char *, const float *, uint32_t) override;
+    DevCamera toDevice() const;
+};
+
+// ---- device scene (scene_setup.cpp) ----
+struct DeviceBuffer {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    ~DeviceBuffer();
+    void upload(const void *src, size_t n);
+    void reserve(size_t n);
+    void release();
+};
+
+struct SceneGPU {
+    DeviceBuffer tlasNodes, blasNodes, blasTris, instances, instGeomInsts, geomInsts, vertices, indices, distPool,
+        materials, envTex, mengGrid, mengPoints;
+    DevScene dev = {};
+    bool spectral = false;
+    bool valid = false;
+    uint64_t sceneVersion = ~0ull;
+    const Scene *scene = nullptr;
+    const Camera *camera = nullptr;
+    uint32_t numInstances = 0, numGeomInsts = 0;
+    uint64_t numTriangles = 0;
+};
+
+void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *out);
+
+// ---- context ----
+class Context : public Object {
+public:
+    int device = -1;
+    uint64_t version = 1;     // bumped on every scene-affecting change
+    std::set<Object *> objects;
+    Scene *scene = nullptr;
+    uint32_t renderer = 0;
+    uint32_t debugAttr = 0;
+    int32_t probeX = 0, probeY = 0;
+    uint32_t maxPathLength = 25;
+    bool spectralRequested = false;
+    bool spectralActive = false;
+    bool kernelTiming = false;
+    uint32_t width = 0, height = 0;
+    uint32_t numAccumFrames = 0;
+    FrameParams lastFrame = {};
+    DevCamera frameCamera = {};
+    // device buffers
+    DeviceBuffer rng, alpha, contrib, meta, rayOrg[2], rayDir[2], hitA, hitB, shOrg, shDir, shVal, counters, accum,
+        output, scratch;
+    uint32_t accumChannels = 3;
+    SceneGPU gpu;
+    double stageMs[NUM_STAGES] = {};
+    uint64_t launches[NUM_STAGES] = {};
+    uint64_t framesRendered = 0, pathsStarted = 0;
+
+    Context(hipCtx_t hctx);
+    ~Context() override;
+    const char *typeName() const override { return "Context"; }
+    void touch() { ++version; }
+    void ensureDevice();
+    void bindOutputBuffer(uint32_t w, uint32_t h);
+    void render(hipStream_t stream, const Camera *camera, uint32_t shrinkCoeff, bool firstFrame, uint32_t limit,
+                uint32_t *numAccum);
+    void readOutput(float *dst);
+    void ensureScene(const Scene &s);
+    void resizeAccum();
+    void traceRays(hipStream_t stream, const Scene &s, const void *rays, uint32_t n, void *hits, bool anyHit);
+    void evaluateBSDFs(hipStream_t stream, const SurfaceMaterial &m, bool spectral, const void *q, uint32_t n, void *r);
+    void statistics(uint64_t *closest, uint64_t *shadow, uint64_t *dropped);
+    void resetStats();
+};
+
+void throwHip(hipError_t e, const char *what);
+#define VLRA_HIP(x) ::vlra::throwHip((x), #x)
+
+bool paramNameEquals(const std::string &a, const std::string &b);
+void affineInverse(const float m[16], float inv[12]);
+
+} // namespace vlra
