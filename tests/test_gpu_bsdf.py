@@ -63,16 +63,33 @@ def test_bsdf_matches_oracle(torch_gpu, mi, spectral):
         h = api.build_material(ctx, m)
         got = gpu_bsdf(torch_gpu, ctx, h, spectral, q)
     nc = 4 if spectral else 3
-    ok_pdf = rel_close(got["sampledPDF"], ref["sampledPDF"])
-    ok_val = np.all(rel_close(got["sampledValue"][:, :nc], ref["sampledValue"][:, :nc], rtol=2e-5), axis=1)
-    ok_eval = np.all(rel_close(got["evalValue"][:, :nc], ref["evalValue"][:, :nc], rtol=2e-5), axis=1)
+    # evaluate / evaluatePDF at identical inputs: 1e-5 relative for every query
+    ok_eval = np.all(rel_close(got["evalValue"][:, :nc], ref["evalValue"][:, :nc]), axis=1)
     ok_epdf = rel_close(got["evalPDF"], ref["evalPDF"])
+    assert ok_eval.all() and ok_epdf.all(), (np.where(~ok_eval)[0][:5], np.where(~ok_epdf)[0][:5])
+    # sampling: same lobe and type, direction within 1e-5. The sampled direction goes through
+    # sin/cos/sqrt, whose last bit differs between the device and libm; at low roughness the
+    # GGX lobe turns that into a much larger change of the value. The bound is therefore
+    # 1e-5 relative plus the change the oracle's own evaluate shows between the two
+    # directions (zero for the delta lobes and wherever the directions agree bit for bit).
     sampled = ref["sampledPDF"] > 0
-    ok_dir = np.all(np.abs(got["sampledDir"] - ref["sampledDir"]) < 1e-5, axis=1) | ~sampled
-    frac = [x.mean() for x in (ok_pdf, ok_val, ok_eval, ok_epdf, ok_dir)]
-    print(m.type, "spectral" if spectral else "rgb", ["%.5f" % f for f in frac])
-    # transcendental functions (sin/cos/sqrt/pow) differ in the last bits between the GPU and
-    # libm; a handful of grazing samples may flip a branch
-    for f in frac:
-        assert f > 0.999
-    assert np.all(got["sampledType"][sampled & ok_pdf] == ref["sampledType"][sampled & ok_pdf])
+    assert np.array_equal(got["sampledPDF"] > 0, sampled)
+    assert np.array_equal(got["sampledType"][sampled], ref["sampledType"][sampled])
+    assert np.all(np.abs(got["sampledDir"][sampled] - ref["sampledDir"][sampled]) < 1e-5)
+    at_ref, at_got = q.copy(), q.copy()
+    at_ref["evalDir"] = ref["sampledDir"]
+    at_got["evalDir"] = got["sampledDir"]
+    e_ref = ob.bsdf_queries(m, spectral, at_ref.view(ob.query_dtype()))
+    e_got = ob.bsdf_queries(m, spectral, at_got.view(ob.query_dtype()))
+    # rejected samples (pdf 0, direction 0) carry no evaluation
+    with np.errstate(invalid="ignore"):
+        sens_val = np.where(sampled[:, None], np.abs(e_got["evalValue"][:, :nc] - e_ref["evalValue"][:, :nc]), 0)
+        sens_pdf = np.where(sampled, np.abs(e_got["evalPDF"] - e_ref["evalPDF"]), 0)
+    tol_val = 1e-6 + 1e-5 * np.abs(ref["sampledValue"][:, :nc]) + 4 * sens_val
+    tol_pdf = 1e-6 + 1e-5 * np.abs(ref["sampledPDF"]) + 4 * sens_pdf
+    ok_val = np.all(np.abs(got["sampledValue"][:, :nc] - ref["sampledValue"][:, :nc]) <= tol_val, axis=1)
+    ok_pdf = np.abs(got["sampledPDF"] - ref["sampledPDF"]) <= tol_pdf
+    exact_dir = np.all(got["sampledDir"] == ref["sampledDir"], axis=1)
+    print(m.type, "spectral" if spectral else "rgb", "dirs bit-exact", int(exact_dir.sum()), "of", len(q),
+          "value fails", np.where(~ok_val)[0][:5], "pdf fails", np.where(~ok_pdf)[0][:5])
+    assert ok_val.all() and ok_pdf.all()

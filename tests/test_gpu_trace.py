@@ -41,7 +41,8 @@ def test_cornell_closest_hit_bitexact(torch_gpu):
     print(f"cornell: {len(rays)} rays, mismatches {bad}, exact ties {ties}")
     assert bad == 0
     assert t_ok and b_ok
-    assert (ref["inst"] >= 0).mean() > 0.95
+    # the box is open towards the camera (+z), so some rays escape
+    assert (ref["inst"] >= 0).mean() > 0.8
 
 
 def test_icosphere_grid_closest_hit_bitexact(torch_gpu):
