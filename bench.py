@@ -1,0 +1,216 @@
+"""Throughput of the MI355X wavefront path tracer on the 1080p spectral icosphere-grid workload.
+
+Workload: spectral path tracing (4 hero wavelengths, 360-830 nm, Meng upsampling) of a 32x32
+grid of 1,280-triangle icosphere instances over a 2-triangle ground plane (1,310,722
+triangles; GGX/UE4 metals with roughness U[0.05,0.8], 10 % smooth dielectrics of IOR 1.5),
+lit by a 1024x512 value-noise environment map exp-mapped to [0.01, 50]; 1920x1080, max
+path length 16.  A step is one vlrContextRender call: one sample per pixel over the whole
+image (generate, then per depth intersect / shade / shadow, then accumulate and convert).
+The timed region is K steps starting from firstFrame, i.e. a K-spp render.
+
+The JSON line carries
+  value          extension + shadow rays traced per second (device counters / wall clock)
+  roofline       the dominant kernel's algorithmic bytes per launch over its mean launch
+                 time (HIP events on the render stream) against 8 TB/s HBM; the per-ray node,
+                 triangle and instance counts come from extra frames rendered after the timed
+                 region with the traversal kernels' counting variant; `traffic` is the HBM
+                 bytes per launch measured by rocprofv3 counters, read from the committed
+                 summary under profiles/ when it exists
+  cpu_baseline   the oracle (C++/OpenMP restatement of the same integrator) on the host's
+                 cores, whole 1920x1080 frames of the same scene and seeds, about 10 s of work
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from vlr_amd import api, scenes  # noqa: E402
+
+METRIC = "rays/sec at 1080p + equal-time RMSE vs FP64 64k-spp ref, cached vs plain PT"
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = ("spectral path tracing, 32x32 grid of 1280-tri icosphere instances + ground "
+            "(1,310,722 tris), 1024x512 value-noise env, 1920x1080, max depth 16")
+
+# bytes each kernel moves per unit of work, fixed by the device layout
+RAY_IN = 32          # origin.xyz + path index, direction.xyz + tmax
+HIT_OUT = 24         # (inst, geomInst, prim, t) + (b1, b2)
+SHADOW_IN = 48       # origin, direction, contribution
+SHADOW_RMW = 32      # contribution read + write when unoccluded
+NODE = 64            # BVH2 node: two child boxes + two child indices
+TRI = 48             # three vertices + geometry-instance / primitive indices
+INST = 48            # world-to-object 3x4 rows read on instance entry
+
+
+def hip_sync():
+    for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+        try:
+            hip = C.CDLL(name)
+            break
+        except OSError:
+            continue
+    rc = hip.hipDeviceSynchronize()
+    if rc != 0:
+        raise RuntimeError(f"hipDeviceSynchronize failed: {rc}")
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def read_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary, if present."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        data = json.load(f)
+    entry = data.get(kernel)
+    return None if entry is None else float(entry["hbm_bytes_per_launch"])
+
+
+def cpu_baseline(desc, width, height, depth, budget_s):
+    from oracle import binding as ob
+    threads = min(16, len(os.sched_getaffinity(0)))
+    scene = ob.Scene(desc)
+    rng = ob.seed_states(width, height)
+    rays = 0
+    frames = 0
+    elapsed = 0.0
+    while elapsed < budget_s or frames == 0:
+        t0 = time.perf_counter()
+        _, st, rng, _ = scene.render(width, height, 1, depth, threads=threads, rng_state=rng)
+        elapsed += time.perf_counter() - t0
+        rays += st["closest_rays"] + st["shadow_rays"]
+        frames += 1
+    return {"value": rays / elapsed, "unit": "rays/s", "cores": threads, "kind": "port",
+            "sample": f"{frames} full {width}x{height} frames (1 spp each, depth {depth}) of the same scene "
+                      f"and per-pixel seeds, {elapsed:.1f} s on {threads} threads of {cpu_model()}",
+            "samples_per_s": frames * width * height / elapsed}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--depth", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--stat-frames", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by pixel "
+              f"tiles with one framebuffer gather, which this build does not run)", file=sys.stderr)
+        sys.exit(2)
+    if not os.path.exists(api.LIB_PATH):
+        raise RuntimeError(f"{api.LIB_PATH} is missing: build it with vlr_amd/build.py")
+
+    desc = scenes.icosphere_grid(args.seed)
+    ctx = api.Context(spectral=desc.spectral, max_path_length=args.depth)
+    built = api.build_scene(ctx, desc)
+    ctx.bind_output(args.width, args.height)
+
+    t_setup = time.perf_counter()
+    for w in range(args.warmup):
+        ctx.render(built.camera, first_frame=(w == 0), limit=args.warmup)
+    hip_sync()
+    setup_s = time.perf_counter() - t_setup
+
+    ctx.reset_stats()
+    ctx.set_kernel_timing(True)
+    hip_sync()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ctx.render(built.camera, first_frame=(k == 0), limit=args.steps)
+    hip_sync()
+    wall = time.perf_counter() - t0
+    st = ctx.stats()
+    ctx.set_kernel_timing(False)
+    img = ctx.read_output()
+
+    rays = st["closestRays"] + st["shadowRays"]
+    value = rays / wall
+    samples_per_s = st["paths"] / wall
+
+    # traversal work per ray, from extra frames with the counting kernels
+    ctx.reset_stats()
+    ctx.set_traversal_stats(True)
+    for k in range(args.stat_frames):
+        ctx.render(built.camera, first_frame=True, limit=1)
+    hip_sync()
+    tv = ctx.stats()
+    ctx.set_traversal_stats(False)
+    per_closest = {k: tv[k + "Closest"] / max(1, tv["closestRays"]) for k in ("nodes", "triangles", "instances")}
+    per_shadow = {k: tv[k + "Shadow"] / max(1, tv["shadowRays"]) for k in ("nodes", "triangles", "instances")}
+
+    def traversal_bytes(io, per):
+        return io + per["nodes"] * NODE + per["triangles"] * TRI + per["instances"] * INST
+
+    b_closest = traversal_bytes(RAY_IN + HIT_OUT, per_closest)
+    b_shadow = traversal_bytes(SHADOW_IN + SHADOW_RMW, per_shadow)
+    kernels = {
+        "k_intersect": (st["msIntersect"], st["launchesIntersect"], st["closestRays"] * b_closest),
+        "k_shadow": (st["msShadow"], st["launchesShadow"], st["shadowRays"] * b_shadow),
+    }
+    dominant = max(kernels, key=lambda k: kernels[k][0])
+    ms, launches, total_bytes = kernels[dominant]
+    per_launch_bytes = total_bytes / max(1, launches)
+    mean_launch_s = ms / 1e3 / max(1, launches)
+    achieved = per_launch_bytes / mean_launch_s / 1e9
+    traffic = read_traffic(dominant)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": WORKLOAD, "width": args.width, "height": args.height, "spp": args.steps,
+                   "max_path_length": args.depth, "scene_seed": args.seed, "spectral": True},
+        "samples_per_s": samples_per_s,
+        "rays": {"closest": st["closestRays"], "shadow": st["shadowRays"], "paths": st["paths"],
+                 "dropped_non_finite": st["droppedNonFinite"]},
+        "stage_ms": {"generate": st["msGenerate"], "intersect": st["msIntersect"], "shade": st["msShade"],
+                     "shadow": st["msShadow"], "accumulate": st["msAccumulate"], "convert": st["msConvert"]},
+        "traversal_per_ray": {"closest": per_closest, "shadow": per_shadow},
+        "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes_per_launch": per_launch_bytes, "mean_launch_ms": mean_launch_s * 1e3},
+        "image_mean_rgb": [float(x) for x in img[..., :3].reshape(-1, 3).mean(axis=0)],
+        "setup_s": setup_s,
+        "rmse": None,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(desc, args.width, args.height, args.depth, args.cpu_seconds)
+        out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -39,6 +39,10 @@ typedef struct VLRRenderStats {
     /* device time accumulated with HIP events when kernel timing is enabled, in ms */
     double msGenerate, msIntersect, msShade, msShadow, msAccumulate, msConvert;
     uint64_t launchesIntersect, launchesShadow, launchesShade;
+    /* traversal work, counted only while traversal statistics are enabled: inner BVH nodes
+     * fetched, triangles tested and instances entered, for closest-hit and shadow rays */
+    uint64_t nodesClosest, trianglesClosest, instancesClosest;
+    uint64_t nodesShadow, trianglesShadow, instancesShadow;
 } VLRRenderStats;
 /* Counting the rays reads back one small counter buffer per render call. */
 VLR_API VLRResult vlrContextGetRenderStats(VLRContextConst context, VLRRenderStats* stats);
@@ -46,6 +50,8 @@ VLR_API VLRResult vlrContextResetRenderStats(VLRContext context);
 /* Brackets every kernel launch of vlrContextRender with HIP events on the render stream
  * (synchronising at the end of each render call).  Off by default. */
 VLR_API VLRResult vlrContextSetKernelTiming(VLRContext context, bool enable);
+/* Renders with the traversal kernels' counting variant (slower; off by default). */
+VLR_API VLRResult vlrContextSetTraversalStatistics(VLRContext context, bool enable);
 
 /* ---- progressive state (host pointers, width*height entries, row-major) ---- */
 VLR_API VLRResult vlrContextReadRNGStates(VLRContextConst context, uint64_t* states);

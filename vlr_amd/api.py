@@ -57,7 +57,9 @@ class VLRRenderStats(C.Structure):
                 ("shadowRays", C.c_uint64), ("droppedNonFinite", C.c_uint64), ("msGenerate", C.c_double),
                 ("msIntersect", C.c_double), ("msShade", C.c_double), ("msShadow", C.c_double),
                 ("msAccumulate", C.c_double), ("msConvert", C.c_double), ("launchesIntersect", C.c_uint64),
-                ("launchesShadow", C.c_uint64), ("launchesShade", C.c_uint64)]
+                ("launchesShadow", C.c_uint64), ("launchesShade", C.c_uint64), ("nodesClosest", C.c_uint64),
+                ("trianglesClosest", C.c_uint64), ("instancesClosest", C.c_uint64), ("nodesShadow", C.c_uint64),
+                ("trianglesShadow", C.c_uint64), ("instancesShadow", C.c_uint64)]
 
 
 RAY_DTYPE = np.dtype([("org", np.float32, 3), ("tmin", np.float32), ("dir", np.float32, 3), ("tmax", np.float32)])
@@ -175,6 +177,7 @@ PROTOTYPES: Dict[str, list] = {
     "vlrContextGetRenderStats": [H, C.POINTER(VLRRenderStats)],
     "vlrContextResetRenderStats": [H],
     "vlrContextSetKernelTiming": [H, B],
+    "vlrContextSetTraversalStatistics": [H, B],
     "vlrContextReadRNGStates": [H, P],
     "vlrContextWriteRNGStates": [H, P],
     "vlrContextGetAccumulationChannels": [H, C.POINTER(U32)],
@@ -311,6 +314,9 @@ class Context:
 
     def set_kernel_timing(self, on: bool):
         call("vlrContextSetKernelTiming", self.h, on)
+
+    def set_traversal_stats(self, on: bool):
+        call("vlrContextSetTraversalStatistics", self.h, on)
 
     def trace_rays(self, scene, rays_dev_ptr: int, n: int, hits_dev_ptr: int, any_hit: bool, stream=None):
         call("vlrContextTraceRays", self.h, stream, scene, C.c_void_p(rays_dev_ptr), n, C.c_void_p(hits_dev_ptr),

@@ -1041,7 +1041,14 @@ VLR_API VLRResult vlrContextGetRenderStats(VLRContextConst context, VLRRenderSta
     if (!stats) return VLRResult_InvalidArgument;
     Context *c = CTX(context);
     std::memset(stats, 0, sizeof(*stats));
-    c->statistics(&stats->closestRays, &stats->shadowRays, &stats->droppedNonFinite);
+    uint64_t trav[6];
+    c->statistics(&stats->closestRays, &stats->shadowRays, &stats->droppedNonFinite, trav);
+    stats->nodesClosest = trav[0];
+    stats->trianglesClosest = trav[1];
+    stats->instancesClosest = trav[2];
+    stats->nodesShadow = trav[3];
+    stats->trianglesShadow = trav[4];
+    stats->instancesShadow = trav[5];
     stats->numFrames = c->framesRendered;
     stats->paths = c->pathsStarted;
     stats->msGenerate = c->stageMs[STAGE_GENERATE];
@@ -1067,6 +1074,13 @@ VLR_API VLRResult vlrContextSetKernelTiming(VLRContext context, bool enable) {
     VLRA_TRY
     CHECK(context, IS(context, Context));
     CTX(context)->kernelTiming = enable;
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+VLR_API VLRResult vlrContextSetTraversalStatistics(VLRContext context, bool enable) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    CTX(context)->traversalStats = enable;
     return VLRResult_NoError;
     VLRA_CATCH
 }

@@ -30,6 +30,7 @@ void Context::ensureDevice() {
 
 Context::~Context() {
     if (device >= 0) (void)hipDeviceSynchronize();
+    for (hipEvent_t e : eventPool) (void)hipEventDestroy(e);
     for (Object *o : std::set<Object *>(objects)) delete o;
 }
 
@@ -80,33 +81,37 @@ void Context::ensureScene(const Scene &s) {
 }
 
 namespace {
+// Records a pair of events around each launch, from a pool the context keeps, and folds the
+// elapsed times into the per-stage totals once the frame has been enqueued.
 struct EventTimer : public LaunchTimer {
     hipStream_t stream;
-    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> events;
-    hipEvent_t pending = nullptr;
-    explicit EventTimer(hipStream_t s) : stream(s) {}
-    void begin(int) override {
-        VLRA_HIP(hipEventCreate(&pending));
-        VLRA_HIP(hipEventRecord(pending, stream));
+    std::vector<hipEvent_t> &pool;
+    std::vector<int> stages;
+    size_t used = 0;
+    EventTimer(hipStream_t s, std::vector<hipEvent_t> &p) : stream(s), pool(p) {}
+    hipEvent_t next() {
+        if (used == pool.size()) {
+            hipEvent_t e;
+            VLRA_HIP(hipEventCreate(&e));
+            pool.push_back(e);
+        }
+        return pool[used++];
     }
+    void begin(int) override { VLRA_HIP(hipEventRecord(next(), stream)); }
     void end(int stage) override {
-        hipEvent_t e;
-        VLRA_HIP(hipEventCreate(&e));
-        VLRA_HIP(hipEventRecord(e, stream));
-        events.push_back({stage, {pending, e}});
-        pending = nullptr;
+        VLRA_HIP(hipEventRecord(next(), stream));
+        stages.push_back(stage);
     }
     void collect(double *ms, uint64_t *launches) {
         VLRA_HIP(hipStreamSynchronize(stream));
-        for (auto &ev : events) {
+        for (size_t k = 0; k < stages.size(); ++k) {
             float t = 0;
-            VLRA_HIP(hipEventElapsedTime(&t, ev.second.first, ev.second.second));
-            ms[ev.first] += t;
-            launches[ev.first] += 1;
-            (void)hipEventDestroy(ev.second.first);
-            (void)hipEventDestroy(ev.second.second);
+            VLRA_HIP(hipEventElapsedTime(&t, pool[2 * k], pool[2 * k + 1]));
+            ms[stages[k]] += t;
+            launches[stages[k]] += 1;
         }
-        events.clear();
+        stages.clear();
+        used = 0;
     }
 };
 } // namespace
@@ -139,6 +144,7 @@ void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCo
         fp.imageH = height / shrinkCoeff;
         fp.frameIndex = numAccumFrames;
         fp.maxPathLength = std::min<uint32_t>(maxPathLength, kMaxDepthCounters - 1);
+        fp.traversalStats = traversalStats ? 1 : 0;
         FrameBuffers fb;
         fb.ps.rng = static_cast<uint64_t *>(rng.ptr);
         fb.ps.alpha = static_cast<float4 *>(alpha.ptr);
@@ -157,7 +163,7 @@ void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCo
         fb.accum = static_cast<float *>(accum.ptr);
         fb.output = static_cast<float4 *>(output.ptr);
         if (kernelTiming) {
-            EventTimer t(stream);
+            EventTimer t(stream, eventPool);
             VLRA_HIP(launchFrame(gpu.dev, fp, fb, spectralActive, stream, &t));
             t.collect(stageMs, launches);
         } else {
@@ -198,18 +204,18 @@ void Context::evaluateBSDFs(hipStream_t stream, const SurfaceMaterial &m, bool s
     VLRA_HIP(launchBsdfQueries(sc, dm, spectral, q, n, r, stream));
 }
 
-void Context::statistics(uint64_t *closest, uint64_t *shadow, uint64_t *dropped) {
+void Context::statistics(uint64_t *closest, uint64_t *shadow, uint64_t *dropped, uint64_t trav[6]) {
     *closest = *shadow = *dropped = 0;
+    for (int k = 0; k < 6; ++k) trav[k] = 0;
     if (device < 0) return;
     uint32_t host[kNumCounters];
     VLRA_HIP(hipDeviceSynchronize());
     VLRA_HIP(hipMemcpy(host, counters.ptr, sizeof(host), hipMemcpyDeviceToHost));
-    uint64_t c, s;
-    std::memcpy(&c, host + kStatClosest, 8);
-    std::memcpy(&s, host + kStatShadow, 8);
-    *closest = c;
-    *shadow = s;
+    std::memcpy(closest, host + kStatClosest, 8);
+    std::memcpy(shadow, host + kStatShadow, 8);
     *dropped = host[kStatDropped];
+    std::memcpy(trav, host + kStatTravClosest, 24);
+    std::memcpy(trav + 3, host + kStatTravShadow, 24);
 }
 
 void Context::resetStats() {

@@ -180,7 +180,10 @@ constexpr uint32_t kMaxDepthCounters = 64;
 constexpr uint32_t kStatClosest = 2 * kMaxDepthCounters + 0;   // 64-bit counters as 2 x u32
 constexpr uint32_t kStatShadow = 2 * kMaxDepthCounters + 2;
 constexpr uint32_t kStatDropped = 2 * kMaxDepthCounters + 4;
-constexpr uint32_t kNumCounters = 2 * kMaxDepthCounters + 8;
+// traversal work (64-bit each: inner nodes fetched, triangles tested, instances entered)
+constexpr uint32_t kStatTravClosest = 2 * kMaxDepthCounters + 6;
+constexpr uint32_t kStatTravShadow = 2 * kMaxDepthCounters + 12;
+constexpr uint32_t kNumCounters = 2 * kMaxDepthCounters + 18;
 
 // path flag bits in meta.w
 constexpr uint32_t kFlagSelMask = 0x3;

@@ -108,9 +108,15 @@ struct TraceHit {
     float t, b1, b2;
 };
 
-template <bool ANY>
+// Work counters of one traversal (inner nodes fetched, triangles tested, instances entered),
+// kept only by the statistics instantiation.
+struct TravCount {
+    uint32_t nodes = 0, tris = 0, insts = 0;
+};
+
+template <bool ANY, bool STATS = false>
 __device__ __forceinline__ bool traceRay(const DevScene &sc, const V3 &org, const V3 &dir, float tmin, float tmax,
-                                         TraceHit *hit, int32_t *lds, int lane) {
+                                         TraceHit *hit, int32_t *lds, int lane, TravCount *tc = nullptr) {
     if (!sc.hasTlas) return false;
     RayPre world;
     world.init(org, dir);
@@ -125,6 +131,7 @@ __device__ __forceinline__ bool traceRay(const DevScene &sc, const V3 &org, cons
     int32_t cur = 0;
     while (true) {
         if (cur >= 0) {
+            if constexpr (STATS) ++tc->nodes;
             const float4 *np = reinterpret_cast<const float4 *>(nodes + cur);
             const float4 na = np[0], nb = np[1], nc = np[2];
             const int4 nd = reinterpret_cast<const int4 *>(np)[3];
@@ -154,6 +161,7 @@ __device__ __forceinline__ bool traceRay(const DevScene &sc, const V3 &org, cons
             uint32_t count = (code & (kMaxLeafPrims - 1)) + 1;
             if (!inBlas) {
                 // enter the instance: object-space ray, BLAS root, return marker on the stack
+                if constexpr (STATS) ++tc->insts;
                 const DevInstance &I = sc.instances[first];
                 V3 o = xfmPoint(I.w2o, world.org);
                 V3 d = xfmVector(I.w2o, world.dir);
@@ -168,6 +176,7 @@ __device__ __forceinline__ bool traceRay(const DevScene &sc, const V3 &org, cons
                 cur = 0;
                 continue;
             }
+            if constexpr (STATS) tc->tris += count;
             for (uint32_t k = 0; k < count; ++k) {
                 const float4 *tp = reinterpret_cast<const float4 *>(tris + first + k);
                 const float4 q0 = tp[0], q1 = tp[1], q2 = tp[2];
@@ -556,23 +565,44 @@ __global__ void __launch_bounds__(256) k_generate(KArgs a) {
     a.fb.rays[0].dir[i] = make_float4(rayDir.x, rayDir.y, rayDir.z, 3.402823466e+38f);
 }
 
+// wave-wide sum (all 64 lanes must be active)
+__device__ __forceinline__ uint32_t waveSum(uint32_t v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+// adds one wave's traversal counters to three 64-bit statistics slots
+__device__ __forceinline__ void addTravStats(uint32_t *counters, uint32_t slot, const TravCount &tc) {
+    uint32_t n = waveSum(tc.nodes), t = waveSum(tc.tris), in = waveSum(tc.insts);
+    if (__lane_id() == 0) {
+        unsigned long long *c64 = reinterpret_cast<unsigned long long *>(counters + slot);
+        atomicAdd(c64 + 0, static_cast<unsigned long long>(n));
+        atomicAdd(c64 + 1, static_cast<unsigned long long>(t));
+        atomicAdd(c64 + 2, static_cast<unsigned long long>(in));
+    }
+}
+
 // ---- intersect: closest hit ----
+template <bool STATS>
 __global__ void __launch_bounds__(kTraceBlock) k_intersect(KArgs a) {
     __shared__ int32_t lds[kLdsStack * kTraceBlock];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const RayQueue q = a.fb.rays[a.depth & 1];
     const uint32_t count = a.fb.counters[2 * a.depth];
-    if (i >= count) return;
-    float4 o = q.org[i], d = q.dir[i];
-    TraceHit h;
-    h.inst = -1;
-    bool found = traceRay<false>(a.sc, xyz(o), xyz(d), 0.0f, d.w, &h, lds, threadIdx.x);
-    if (found) {
-        a.fb.hits.a[i] = make_int4(h.inst, h.geomInst, h.prim, __float_as_int(h.t));
-        a.fb.hits.b[i] = make_float2(h.b1, h.b2);
-    } else {
-        a.fb.hits.a[i] = make_int4(-1, -1, -1, 0);
+    TravCount tc;
+    if (i < count) {
+        float4 o = q.org[i], d = q.dir[i];
+        TraceHit h;
+        h.inst = -1;
+        bool found = traceRay<false, STATS>(a.sc, xyz(o), xyz(d), 0.0f, d.w, &h, lds, threadIdx.x, &tc);
+        if (found) {
+            a.fb.hits.a[i] = make_int4(h.inst, h.geomInst, h.prim, __float_as_int(h.t));
+            a.fb.hits.b[i] = make_float2(h.b1, h.b2);
+        } else {
+            a.fb.hits.a[i] = make_int4(-1, -1, -1, 0);
+        }
     }
+    if constexpr (STATS) addTravStats(a.fb.counters, kStatTravClosest, tc);
 }
 
 // ---- shade: path_tracing.cu:133-259 (hit) and :266-337 (miss) ----
@@ -764,20 +794,24 @@ __global__ void __launch_bounds__(256) k_shade(KArgs a) {
 }
 
 // ---- shadow: any-hit, adds the NEE contribution when unoccluded ----
+template <bool STATS>
 __global__ void __launch_bounds__(kTraceBlock) k_shadow(KArgs a) {
     __shared__ int32_t lds[kLdsStack * kTraceBlock];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t count = a.fb.counters[2 * a.depth + 1];
-    if (i >= count) return;
-    float4 o = a.fb.shadow.org[i], d = a.fb.shadow.dir[i];
-    TraceHit h;
-    bool occluded = traceRay<true>(a.sc, xyz(o), xyz(d), 0.0f, d.w, &h, lds, threadIdx.x);
-    if (!occluded) {
-        uint32_t path = __float_as_uint(o.w);
-        float4 v = a.fb.shadow.value[i];
-        float4 c = a.fb.ps.contrib[path];
-        a.fb.ps.contrib[path] = make_float4(c.x + v.x, c.y + v.y, c.z + v.z, c.w + v.w);
+    TravCount tc;
+    if (i < count) {
+        float4 o = a.fb.shadow.org[i], d = a.fb.shadow.dir[i];
+        TraceHit h;
+        bool occluded = traceRay<true, STATS>(a.sc, xyz(o), xyz(d), 0.0f, d.w, &h, lds, threadIdx.x, &tc);
+        if (!occluded) {
+            uint32_t path = __float_as_uint(o.w);
+            float4 v = a.fb.shadow.value[i];
+            float4 c = a.fb.ps.contrib[path];
+            a.fb.ps.contrib[path] = make_float4(c.x + v.x, c.y + v.y, c.z + v.z, c.w + v.w);
+        }
     }
+    if constexpr (STATS) addTravStats(a.fb.counters, kStatTravShadow, tc);
 }
 
 // ---- accumulate: Kahan sum per channel (common_internal.h:350-368, spectrum_types.h:674-684) ----
@@ -978,13 +1012,19 @@ static hipError_t launchFrameT(const DevScene &sc, const FrameParams &fp, const 
     for (uint32_t d = 0; d < fp.maxPathLength && d + 1 < kMaxDepthCounters; ++d) {
         a.depth = d;
         t0(STAGE_INTERSECT);
-        hipLaunchKernelGGL(k_intersect, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
+        if (fp.traversalStats)
+            hipLaunchKernelGGL(k_intersect<true>, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
+        else
+            hipLaunchKernelGGL(k_intersect<false>, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
         t1(STAGE_INTERSECT);
         t0(STAGE_SHADE);
         hipLaunchKernelGGL(k_shade<N>, dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
         t1(STAGE_SHADE);
         t0(STAGE_SHADOW);
-        hipLaunchKernelGGL(k_shadow, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
+        if (fp.traversalStats)
+            hipLaunchKernelGGL(k_shadow<true>, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
+        else
+            hipLaunchKernelGGL(k_shadow<false>, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
         t1(STAGE_SHADOW);
     }
     t0(STAGE_ACCUMULATE);

@@ -10,6 +10,7 @@ struct FrameParams {
     uint32_t imageW, imageH;       // rendered sub-image (shrinkCoeff)
     uint32_t frameIndex;           // numAccumFrames (1-based)
     uint32_t maxPathLength;
+    uint32_t traversalStats;       // count nodes / triangles / instances per traversal
 };
 
 struct FrameBuffers {

@@ -373,6 +373,8 @@ public:
     bool spectralRequested = false;
     bool spectralActive = false;
     bool kernelTiming = false;
+    bool traversalStats = false;
+    std::vector<hipEvent_t> eventPool;   // reused by kernel timing, 2 per timed launch
     uint32_t width = 0, height = 0;
     uint32_t numAccumFrames = 0;
     FrameParams lastFrame = {};
@@ -399,7 +401,8 @@ public:
     void resizeAccum();
     void traceRays(hipStream_t stream, const Scene &s, const void *rays, uint32_t n, void *hits, bool anyHit);
     void evaluateBSDFs(hipStream_t stream, const SurfaceMaterial &m, bool spectral, const void *q, uint32_t n, void *r);
-    void statistics(uint64_t *closest, uint64_t *shadow, uint64_t *dropped);
+    // trav: nodes, triangles, instances of closest-hit then of shadow traversals
+    void statistics(uint64_t *closest, uint64_t *shadow, uint64_t *dropped, uint64_t trav[6]);
     void resetStats();
 };
 
