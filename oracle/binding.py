@@ -131,6 +131,10 @@ def lib():
         L.oracle_pcg32_stream.restype = C.c_uint32
         L.oracle_pcg32_stream.argtypes = [C.c_uint64, C.c_uint32, C.c_void_p, C.c_void_p]
         L.oracle_seed_states.argtypes = [C.c_uint32, C.c_uint32, C.c_void_p]
+        L.oracle_sample_lights.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]
+        L.oracle_env_lookup.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]
+        L.oracle_pcg32_floats.argtypes = [C.c_uint64, C.c_uint32, C.c_void_p]
+        L.oracle_mt19937_64.argtypes = [C.c_uint64, C.c_uint32, C.c_void_p]
         L.oracle_meng_eval.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_void_p, C.c_void_p]
         L.oracle_cmf_bins.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.oracle_triplet_rgb.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_void_p]
@@ -155,6 +159,11 @@ def material_struct(m) -> OracleMaterial:
         om.scalars[k] = m.floats.get(name, FLOAT_DEFAULTS[(m.type, name)])
     om.useEnvTexture = 1 if m.use_env_texture else 0
     return om
+
+
+LIGHT_SAMPLE_DTYPE = np.dtype([("position", np.float32, 3), ("normal", np.float32, 3), ("areaPDF", np.float32),
+                               ("lightProb", np.float32), ("inst", np.int32), ("prim", np.int32),
+                               ("atInfinity", np.int32), ("valid", np.int32)])
 
 
 class Scene:
@@ -244,6 +253,20 @@ class Scene:
         lib().oracle_trace_any(self.handle, rays.ctypes.data, len(rays), occ.ctypes.data, int(use_bvh), threads)
         return occ
 
+    def sample_lights(self, u: np.ndarray) -> np.ndarray:
+        """u: (n, 3) uniform numbers -> LIGHT_SAMPLE_DTYPE records."""
+        u = np.ascontiguousarray(u, np.float32)
+        out = np.zeros(len(u), LIGHT_SAMPLE_DTYPE)
+        lib().oracle_sample_lights(self.handle, len(u), u.ctypes.data, out.ctypes.data)
+        return out
+
+    def env_lookup(self, dirs: np.ndarray) -> np.ndarray:
+        """dirs: (n, 3) world directions -> (n, 4): tex u, tex v, solid-angle pdf, texel channel 0."""
+        d = np.ascontiguousarray(dirs, np.float32)
+        out = np.zeros((len(d), 4), np.float32)
+        lib().oracle_env_lookup(self.handle, len(d), d.ctypes.data, out.ctypes.data)
+        return out
+
     def render(self, width, height, frames, max_path_length=25, threads=0, rng_state=None, want_accum=False,
                pixel_range=None):
         out = np.zeros((height, width, 4), np.float32)
@@ -293,6 +316,18 @@ def pcg32_stream(state: int, n: int):
     fin = np.zeros(1, np.uint64)
     lib().oracle_pcg32_stream(C.c_uint64(state), n, out.ctypes.data, fin.ctypes.data)
     return out, int(fin[0])
+
+
+def pcg32_floats(state: int, n: int) -> np.ndarray:
+    out = np.zeros(n, np.float32)
+    lib().oracle_pcg32_floats(C.c_uint64(state), n, out.ctypes.data)
+    return out
+
+
+def mt19937_64(seed: int, n: int) -> np.ndarray:
+    out = np.zeros(n, np.uint64)
+    lib().oracle_mt19937_64(C.c_uint64(seed), n, out.ctypes.data)
+    return out
 
 
 def seed_states(width: int, height: int) -> np.ndarray:

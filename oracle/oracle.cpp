@@ -295,14 +295,17 @@ struct AABB {
     }
 };
 
-// Conservative slab test; the far distance is widened so float rounding never culls a hit.
+// Conservative slab test: the far distances and the current closest distance are widened by
+// 8 ulp-sized relative steps, so float rounding (including the re-rounding of t in another
+// instance's object space) never culls a box that holds a closer hit.
 static inline bool hitBox(const AABB &b, const RayPre &r, float tmin, float tmax) {
-    float t0 = tmin, t1 = tmax;
+    constexpr float kWiden = 1.0f + 8.0f * FLT_EPSILON;
+    float t0 = tmin, t1 = tmax * kWiden;
     for (int a = 0; a < 3; ++a) {
         float tn = (b.lo[a] - r.org[a]) * r.invDir[a];
         float tf = (b.hi[a] - r.org[a]) * r.invDir[a];
         if (tn > tf) std::swap(tn, tf);
-        tf *= 1.0f + 4.0f * FLT_EPSILON;
+        tf *= kWiden;
         if (tn == tn) t0 = tn > t0 ? tn : t0;  // NaN (0*inf) leaves the interval as is
         if (tf == tf) t1 = tf < t1 ? tf : t1;
         if (t0 > t1)
@@ -1435,6 +1438,56 @@ uint32_t oracle_pcg32_stream(uint64_t state, uint32_t n, uint32_t *out, uint64_t
 void oracle_seed_states(uint32_t W, uint32_t H, uint64_t *out) {
     std::mt19937_64 seedRng(591842031321323413ULL);
     for (size_t i = 0; i < size_t(W) * H; ++i) out[i] = seedRng();
+}
+
+void oracle_sample_lights(void *sp, uint32_t n, const float *u, OracleLightSample *out) {
+    const Scene &s = *static_cast<Scene *>(sp);
+    for (uint32_t i = 0; i < n; ++i) {
+        LightSample ls;
+        float lightProb = 0;
+        bool valid = false;
+        OracleLightSample &o = out[i];
+        std::memset(&o, 0, sizeof(o));
+        if (s.lightInstDist.integral <= 0.0f) continue;
+        sampleLight(s, u[3 * i], u[3 * i + 1], u[3 * i + 2], &ls, &lightProb, &valid);
+        o.position[0] = ls.sp.position.x; o.position[1] = ls.sp.position.y; o.position[2] = ls.sp.position.z;
+        o.normal[0] = ls.sp.geometricNormal.x; o.normal[1] = ls.sp.geometricNormal.y;
+        o.normal[2] = ls.sp.geometricNormal.z;
+        o.areaPDF = ls.areaPDF;
+        o.lightProb = lightProb;
+        o.instIndex = ls.sp.atInfinity ? -1 : static_cast<int32_t>(ls.sp.instIndex);
+        o.primIndex = static_cast<int32_t>(ls.sp.primIndex);
+        o.atInfinity = ls.sp.atInfinity ? 1 : 0;
+        o.valid = valid ? 1 : 0;
+    }
+}
+
+void oracle_env_lookup(void *sp, uint32_t n, const float *dirs, float *out) {
+    const Scene &s = *static_cast<Scene *>(sp);
+    for (uint32_t i = 0; i < n; ++i) {
+        V3 d(dirs[3 * i], dirs[3 * i + 1], dirs[3 * i + 2]);
+        float phi, theta;
+        d.toPolarYUp(&theta, &phi);
+        phi += s.envRotation;
+        phi = phi - std::floor(phi / (2 * kPi)) * 2 * kPi;
+        float u = phi / (2 * kPi), v = theta / kPi;
+        out[4 * i + 0] = u;
+        out[4 * i + 1] = v;
+        out[4 * i + 2] = s.hasEnv ? s.envImportanceMap.evaluatePDF(u, v) / (2 * kPi * kPi * std::sin(theta)) : 0.0f;
+        float tex[4] = {0, 0, 0, 0};
+        if (!s.envTex.empty()) envFetch(s, u, v, tex);
+        out[4 * i + 3] = tex[0];
+    }
+}
+
+void oracle_pcg32_floats(uint64_t state, uint32_t n, float *out) {
+    PCG32 r{state};
+    for (uint32_t i = 0; i < n; ++i) out[i] = r.uniform();
+}
+
+void oracle_mt19937_64(uint64_t seed, uint32_t n, uint64_t *out) {
+    std::mt19937_64 g(seed);
+    for (uint32_t i = 0; i < n; ++i) out[i] = g();
 }
 
 void oracle_meng_eval(int spectrumType, int colorSpace, float e0, float e1, float e2, const float lambdas[4],

@@ -135,6 +135,28 @@ int oracle_bsdf_queries(const OracleMaterial* mat, int spectral, const OracleBSD
 /* Reference helpers exposed for golden-vector checks. */
 uint32_t oracle_pcg32_stream(uint64_t state, uint32_t n, uint32_t* out, uint64_t* finalState);
 void oracle_seed_states(uint32_t width, uint32_t height, uint64_t* out);
+/* Light sampling as next-event estimation does it (light_transport_common.h:179-192,
+ * triangle.cu:113-177, point.cu:38-80, infinite_sphere.cu:60-114): u holds 3 numbers per
+ * sample (uLight, uPos0, uPos1).  areaPDF is per unit area (per unit solid angle / sin-free
+ * for the environment, where position is the direction). */
+typedef struct OracleLightSample {
+    float position[3];
+    float normal[3];
+    float areaPDF;
+    float lightProb;
+    int32_t instIndex;      /* -1 for the environment */
+    int32_t primIndex;
+    int32_t atInfinity;
+    int32_t valid;
+} OracleLightSample;
+void oracle_sample_lights(void* scene, uint32_t n, const float* u, OracleLightSample* out);
+/* The environment along world directions (3 floats each): texture u, v, the solid-angle pdf
+ * of environment light sampling, and the first channel of the bilinear texture fetch. */
+void oracle_env_lookup(void* scene, uint32_t n, const float* dirs, float* out);
+/* getFloat0cTo1o stream (random_distributions.h:23-26) */
+void oracle_pcg32_floats(uint64_t state, uint32_t n, float* out);
+/* the seeding generator of context.cpp:1182-1187 with an arbitrary seed */
+void oracle_mt19937_64(uint64_t seed, uint32_t n, uint64_t* out);
 void oracle_meng_eval(int spectrumType, int colorSpace, float e0, float e1, float e2,
                       const float lambdas[4], float out[4]);
 void oracle_cmf_bins(float x[16], float y[16], float z[16], float* integral);

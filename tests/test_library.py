@@ -1,0 +1,175 @@
+"""The C-ABI library without a GPU: it loads, exports every function the headers declare, its
+headers compile as C and C++, and the host side of the API (object model, parameters, error
+codes of vlr.cpp:23-34) behaves like the reference's.  No kernel is launched here."""
+import ctypes as C
+import re
+import shutil
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from vlr_amd import api
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADERS = sorted((ROOT / "include" / "vlr").glob("*.h"))
+
+# vlr.cpp:870-906 and 955-991 (names the reference's factories accept)
+MATERIAL_NAMES = ["Matte", "SpecularReflection", "SpecularScattering", "MicrofacetReflection",
+                  "MicrofacetScattering", "LambertianScattering", "UE4", "OldStyle", "DiffuseEmitter",
+                  "DirectionalEmitter", "PointEmitter", "Multi", "EnvironmentEmitter"]
+NODE_NAMES = ["Geometry", "Tangent", "Float2", "Float3", "Float4", "ScaleAndOffsetFloat", "TripletSpectrum",
+              "RegularSampledSpectrum", "IrregularSampledSpectrum", "Float3ToSpectrum",
+              "ScaleAndOffsetUVTextureMap2D", "Image2DTexture", "EnvironmentTexture"]
+
+
+def declared_functions():
+    names = set()
+    for h in HEADERS:
+        text = h.read_text()
+        names.update(re.findall(r"VLR_API\s+[\w\s\*]+?\b(vlr\w+)\s*\(", text))
+    return names
+
+
+def test_every_declared_function_is_exported():
+    names = declared_functions()
+    assert len(names) > 100
+    L = C.CDLL(str(api.LIB_PATH))
+    missing = [n for n in sorted(names) if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_every_declared_function():
+    names = declared_functions() - {"vlrGetErrorMessage"}
+    assert names == set(api.PROTOTYPES), sorted(names ^ set(api.PROTOTYPES))
+
+
+@pytest.mark.parametrize("lang,compiler", [("c", "gcc"), ("c++", "g++")])
+def test_headers_compile(tmp_path, lang, compiler):
+    if shutil.which(compiler) is None:
+        pytest.skip(f"{compiler} not available")
+    src = tmp_path / ("t.c" if lang == "c" else "t.cpp")
+    src.write_text('#include "vlr/vlr.h"\nint main(void) { VLRContext c = 0; (void)c; return 0; }\n')
+    r = subprocess.run([compiler, "-fsyntax-only", "-x", lang, "-D__HIP_PLATFORM_AMD__", f"-I{ROOT / 'include'}",
+                        "-I/opt/rocm/include", str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+# ---------------------------------------------------------------------------------------------
+@pytest.fixture()
+def ctx():
+    c = api.Context()
+    yield c
+    c.close()
+
+
+def code_of(name, *args):
+    return api.try_call(name, *args)
+
+
+def test_error_codes_follow_the_reference(ctx):
+    h = api.H()
+    assert code_of("vlrSurfaceMaterialCreate", ctx.h, b"NoSuchMaterial", C.byref(h)) == 3
+    assert code_of("vlrSurfaceMaterialCreate", None, b"Matte", C.byref(h)) == 3   # null handle
+    assert code_of("vlrSurfaceMaterialCreate", ctx.h, b"Matte", None) == 3
+    m = ctx.material("Matte")
+    # a material handle where a node is expected -> InvalidInstance
+    assert code_of("vlrTriangleMeshSurfaceNodeSetVertices", m, None, 0) == 2
+    msg = api.lib().vlrGetErrorMessage(2)
+    assert b"Instance" in msg or b"instance" in msg
+
+
+@pytest.mark.parametrize("name", MATERIAL_NAMES)
+def test_every_material_type_is_creatable(ctx, name):
+    m = ctx.material(name)
+    t = api.CP()
+    api.call("vlrObjectGetType", m, C.byref(t))
+    assert t.value
+    n = api.U32()
+    api.call("vlrQueryableGetNumParameters", m, C.byref(n))
+    for i in range(n.value):
+        info = api.H()
+        api.call("vlrQueryableGetParameterInfo", m, i, C.byref(info))
+        pname = api.CP()
+        api.call("vlrParameterInfoGetName", info, C.byref(pname))
+        assert pname.value
+
+
+@pytest.mark.parametrize("name", NODE_NAMES)
+def test_every_shader_node_type_is_creatable(ctx, name):
+    ctx.shader_node(name)
+
+
+def test_parameters_round_trip(ctx):
+    m = ctx.material("UE4")
+    api.set_float(m, "roughness", 0.37)
+    f = api.F()
+    api.call("vlrQueryableGetFloat", m, b"roughness", C.byref(f))
+    assert abs(f.value - 0.37) < 1e-7
+    api.set_spectrum(m, "base color", "Rec709(D65)", 0.1, 0.2, 0.3)
+    s = api.VLRImmediateSpectrum()
+    api.call("vlrQueryableGetImmediateSpectrum", m, b"base color", C.byref(s))
+    assert (s.e0, s.e1, s.e2) == pytest.approx((0.1, 0.2, 0.3))
+    assert s.colorSpace == b"Rec709(D65)"
+    # parameter names are matched without regard to case and spaces (queryable.cpp)
+    api.call("vlrQueryableSetFloat", m, b"Roughness", 0.5)
+    api.call("vlrQueryableGetFloat", m, b"roughness", C.byref(f))
+    assert abs(f.value - 0.5) < 1e-7
+    assert code_of("vlrQueryableSetFloat", m, b"no such parameter", 1.0) == 3
+
+
+def test_scene_graph(ctx):
+    mesh = api.H()
+    api.call("vlrTriangleMeshSurfaceNodeCreate", ctx.h, b"quad", C.byref(mesh))
+    verts = np.zeros((4, 11), np.float32)
+    verts[:, 0:3] = [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]]
+    verts[:, 3:6] = [0, 0, 1]
+    verts[:, 6:9] = [1, 0, 0]
+    api.call("vlrTriangleMeshSurfaceNodeSetVertices", mesh, verts.ctypes.data, 4)
+    idx = np.array([0, 1, 2, 0, 2, 3], np.uint32)
+    mat = ctx.material("Matte")
+    api.call("vlrTriangleMeshSurfaceNodeAddMaterialGroup", mesh, idx.ctypes.data, 6, mat, api.NULL_PLUG,
+             api.NULL_PLUG, api.NULL_PLUG)
+    xf = ctx.transform([1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 2, 3, 4, 1])
+    node = api.H()
+    api.call("vlrInternalNodeCreate", ctx.h, b"inst", xf, C.byref(node))
+    api.call("vlrParentNodeAddChild", node, mesh)
+    n = api.U32()
+    api.call("vlrParentNodeGetNumChildren", node, C.byref(n))
+    assert n.value == 1
+    child = api.H()
+    api.call("vlrParentNodeGetChildAt", node, 0, C.byref(child))
+    assert child.value == mesh.value
+    name = api.CP()
+    api.call("vlrNodeGetName", node, C.byref(name))
+    assert name.value == b"inst"
+    m, inv = (C.c_float * 16)(), (C.c_float * 16)()
+    api.call("vlrStaticTransformGetArrays", xf, m, inv)
+    M = np.array(m[:]).reshape(4, 4).T
+    I = np.array(inv[:]).reshape(4, 4).T
+    np.testing.assert_allclose(M @ I, np.eye(4), atol=1e-6)
+    api.call("vlrParentNodeRemoveChild", node, mesh)
+    api.call("vlrParentNodeGetNumChildren", node, C.byref(n))
+    assert n.value == 0
+
+
+def test_camera_parameters(ctx):
+    cam = api.H()
+    api.call("vlrCameraCreate", ctx.h, b"Perspective", C.byref(cam))
+    api.call("vlrQueryableSetFloat", cam, b"fovy", 0.7)
+    f = api.F()
+    api.call("vlrQueryableGetFloat", cam, b"fovy", C.byref(f))
+    assert abs(f.value - 0.7) < 1e-7
+    assert code_of("vlrCameraCreate", ctx.h, b"Orthographic", C.byref(cam)) == 3
+
+
+def test_knobs(ctx):
+    api.call("vlrContextSetMaxPathLength", ctx.h, 7)
+    n = api.U32()
+    api.call("vlrContextGetMaxPathLength", ctx.h, C.byref(n))
+    assert n.value == 7
+    api.call("vlrContextSetSpectralRendering", ctx.h, True)
+    b = C.c_bool()
+    api.call("vlrContextGetSpectralRendering", ctx.h, C.byref(b))
+    assert b.value

@@ -283,6 +283,20 @@ def icosphere(subdivisions: int = 3):
     return v, np.asarray(faces, np.uint32).reshape(-1)
 
 
+def flat_shaded(vertices: np.ndarray, indices: np.ndarray):
+    """Unshares the vertices of a triangle list so that every vertex normal is its face normal."""
+    tri = indices.reshape(-1, 3)
+    v = vertices[tri.reshape(-1)].copy()
+    p = v[:, 0:3].astype(np.float64).reshape(-1, 3, 3)
+    n = np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0])
+    n /= np.linalg.norm(n, axis=1, keepdims=True)
+    v[:, 3:6] = np.repeat(n, 3, axis=0)
+    t = p[:, 1] - p[:, 0]
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    v[:, 6:9] = np.repeat(t, 3, axis=0)
+    return v, np.arange(len(v), dtype=np.uint32)
+
+
 def value_noise_env(width: int, height: int, seed: int, lo: float = 0.01, hi: float = 50.0) -> np.ndarray:
     """RGBA16F equirectangular map of smooth value noise mapped exponentially onto [lo, hi]."""
     rng = np.random.default_rng(seed)
@@ -360,12 +374,16 @@ def icosphere_grid(seed: int = 20261202, grid: int = 32, spacing: float = 2.2, e
     return SceneDesc(mats, meshes, insts, cam, env, True, "icosphere_grid")
 
 
-def white_furnace(spectral: bool = False, env_size=(64, 32)) -> SceneDesc:
+def white_furnace(spectral: bool = False, env_size=(64, 32), flat: bool = False) -> SceneDesc:
     """A unit Lambert sphere of albedo 1 inside a constant environment of radiance 1.
 
-    Every pixel of a converged render equals the environment radiance times the camera's
-    cos^4 vignetting; the sphere is invisible (a closed-form check of energy conservation)."""
+    With flat shading (shading normal = geometric normal) every pixel of a converged render
+    equals the environment radiance times the camera weight (pixel-averaged cos^4 times
+    4 tan^2(fovy / 2) aspect): the sphere is invisible, a closed-form check of energy
+    conservation.  Smooth vertex normals make the shading-normal correction lose a little energy."""
     v, i = icosphere(3)
+    if flat:
+        v, i = flat_shaded(v, i)
     mats = [MaterialDesc(MATTE, {"albedo": (REC709, 1.0, 1.0, 1.0)}),
             MaterialDesc(ENVIRONMENT_EMITTER, {"emittance": (REC709, 1.0, 1.0, 1.0)}, {"scale": 1.0},
                          use_env_texture=True)]
