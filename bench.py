@@ -49,16 +49,39 @@ TRI = 48             # three vertices + geometry-instance / primitive indices
 INST = 48            # world-to-object 3x4 rows read on instance entry
 
 
-def hip_sync():
+def hip():
     for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
         try:
-            hip = C.CDLL(name)
-            break
+            return C.CDLL(name)
         except OSError:
             continue
-    rc = hip.hipDeviceSynchronize()
+    raise RuntimeError("libamdhip64.so not found")
+
+
+def hip_sync():
+    rc = hip().hipDeviceSynchronize()
     if rc != 0:
         raise RuntimeError(f"hipDeviceSynchronize failed: {rc}")
+
+
+def copy_probe(nbytes):
+    """One device-to-device copy of nbytes: the reference dispatch for scaling the HBM
+    counters (FETCH_SIZE / WRITE_SIZE of a plain copy kernel of known size)."""
+    H = hip()
+    src, dst = C.c_void_p(), C.c_void_p()
+    for p in (src, dst):
+        rc = H.hipMalloc(C.byref(p), C.c_size_t(nbytes))
+        if rc != 0:
+            raise RuntimeError(f"hipMalloc failed: {rc}")
+    H.hipMemset(src, 1, C.c_size_t(nbytes))
+    hip_sync()
+    rc = H.hipMemcpy(dst, src, C.c_size_t(nbytes), 3)    # hipMemcpyDeviceToDevice
+    hip_sync()
+    H.hipFree(src)
+    H.hipFree(dst)
+    if rc != 0:
+        raise RuntimeError(f"hipMemcpy failed: {rc}")
+    print(f"copy probe: {nbytes} bytes device to device", file=sys.stderr, flush=True)
 
 
 def cpu_model():
@@ -73,14 +96,13 @@ def cpu_model():
 
 
 def read_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed counter summary, if present."""
-    path = os.path.join(ROOT, "profiles", "traffic.json")
+    """HBM bytes per launch of `kernel` on this workload, from the committed counter summary
+    (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a copy kernel of known size), if present."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_icosphere_grid_1080p_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        data = json.load(f)
-    entry = data.get(kernel)
-    return None if entry is None else float(entry["hbm_bytes_per_launch"])
+        return float(json.load(f)["hbm_bytes_per_launch"])
 
 
 def cpu_baseline(desc, width, height, depth, budget_s):
@@ -115,6 +137,8 @@ def main():
     ap.add_argument("--stat-frames", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="also run one device-to-device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by pixel "
@@ -209,6 +233,8 @@ def main():
         out["cpu_baseline"] = cpu_baseline(desc, args.width, args.height, args.depth, args.cpu_seconds)
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
     ctx.close()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
     print(json.dumps(out), flush=True)
 
 
