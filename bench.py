@@ -47,6 +47,9 @@ SHADOW_RMW = 32      # contribution read + write when unoccluded
 NODE = 64            # BVH2 node: two child boxes + two child indices
 TRI = 48             # three vertices + geometry-instance / primitive indices
 INST = 48            # world-to-object 3x4 rows read on instance entry
+# one shaded hit: ray 32 + hit 24 + path state read and written 2 x 56 + three vertices 144 +
+# material 80 + instance 96 + next ray 32 + shadow record 48
+SHADE = 568
 
 
 def hip():
@@ -137,6 +140,8 @@ def main():
     ap.add_argument("--stat-frames", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--wavefront-depth", type=int, default=None,
+                    help="depths run as per-depth kernels before the persistent tail kernel (library default)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also run one device-to-device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -151,6 +156,8 @@ def main():
     ctx = api.Context(spectral=desc.spectral, max_path_length=args.depth)
     built = api.build_scene(ctx, desc)
     ctx.bind_output(args.width, args.height)
+    if args.wavefront_depth is not None:
+        ctx.set_wavefront_depth(args.wavefront_depth)
 
     t_setup = time.perf_counter()
     for w in range(args.warmup):
@@ -167,6 +174,7 @@ def main():
     hip_sync()
     wall = time.perf_counter() - t0
     st = ctx.stats()
+    qc, qs = ctx.queue_counts(args.depth)
     ctx.set_kernel_timing(False)
     img = ctx.read_output()
 
@@ -190,9 +198,14 @@ def main():
 
     b_closest = traversal_bytes(RAY_IN + HIT_OUT, per_closest)
     b_shadow = traversal_bytes(SHADOW_IN + SHADOW_RMW, per_shadow)
+    # rays by stage: per-depth kernels for the first depths, the persistent tail kernel for the rest
+    wf_c, wf_s = st["closestRaysWavefront"], st["shadowRaysWavefront"]
+    tail_c, tail_s = st["closestRays"] - wf_c, st["shadowRays"] - wf_s
     kernels = {
-        "k_intersect": (st["msIntersect"], st["launchesIntersect"], st["closestRays"] * b_closest),
-        "k_shadow": (st["msShadow"], st["launchesShadow"], st["shadowRays"] * b_shadow),
+        "k_intersect": (st["msIntersect"], st["launchesIntersect"], wf_c * b_closest),
+        "k_shadow": (st["msShadow"], st["launchesShadow"], wf_s * b_shadow),
+        "k_shade": (st["msShade"], st["launchesShade"], wf_c * SHADE),
+        "k_tail": (st["msTail"], st["launchesTail"], tail_c * (b_closest + SHADE) + tail_s * b_shadow),
     }
     dominant = max(kernels, key=lambda k: kernels[k][0])
     ms, launches, total_bytes = kernels[dominant]
@@ -220,8 +233,12 @@ def main():
         "rays": {"closest": st["closestRays"], "shadow": st["shadowRays"], "paths": st["paths"],
                  "dropped_non_finite": st["droppedNonFinite"]},
         "stage_ms": {"generate": st["msGenerate"], "intersect": st["msIntersect"], "shade": st["msShade"],
-                     "shadow": st["msShadow"], "accumulate": st["msAccumulate"], "convert": st["msConvert"]},
+                     "shadow": st["msShadow"], "tail": st["msTail"], "accumulate": st["msAccumulate"],
+                     "convert": st["msConvert"]},
+        "rays_by_stage": {"per_depth_closest": wf_c, "per_depth_shadow": wf_s, "tail_closest": tail_c,
+                          "tail_shadow": tail_s},
         "traversal_per_ray": {"closest": per_closest, "shadow": per_shadow},
+        "last_frame_rays_per_depth": {"closest": qc.tolist(), "shadow": qs.tolist()},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_launch": per_launch_bytes, "mean_launch_ms": mean_launch_s * 1e3},

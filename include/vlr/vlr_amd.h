@@ -43,6 +43,10 @@ typedef struct VLRRenderStats {
      * fetched, triangles tested and instances entered, for closest-hit and shadow rays */
     uint64_t nodesClosest, trianglesClosest, instancesClosest;
     uint64_t nodesShadow, trianglesShadow, instancesShadow;
+    double msTail;               /* the persistent tail kernel (see vlrContextSetWavefrontDepth) */
+    uint64_t launchesTail;
+    /* rays traced by the per-depth kernels; closestRays - closestRaysWavefront went through the tail */
+    uint64_t closestRaysWavefront, shadowRaysWavefront;
 } VLRRenderStats;
 /* Counting the rays reads back one small counter buffer per render call. */
 VLR_API VLRResult vlrContextGetRenderStats(VLRContextConst context, VLRRenderStats* stats);
@@ -52,6 +56,15 @@ VLR_API VLRResult vlrContextResetRenderStats(VLRContext context);
 VLR_API VLRResult vlrContextSetKernelTiming(VLRContext context, bool enable);
 /* Renders with the traversal kernels' counting variant (slower; off by default). */
 VLR_API VLRResult vlrContextSetTraversalStatistics(VLRContext context, bool enable);
+/* Frame schedule: depths below `depth` run as separate intersect / shade / shadow kernels over
+ * device queues; from `depth` on every remaining path runs to its end inside one persistent
+ * kernel.  The image does not depend on it (paths draw and add in the same order). */
+VLR_API VLRResult vlrContextSetWavefrontDepth(VLRContext context, uint32_t depth);
+VLR_API VLRResult vlrContextGetWavefrontDepth(VLRContextConst context, uint32_t* depth);
+/* Queue sizes of the last rendered frame: closest[d] extension rays and shadow[d] shadow rays
+ * traced at depth d, for d < maxDepth (entries past the frame's path length are 0). */
+VLR_API VLRResult vlrContextGetQueueCounts(VLRContextConst context, uint32_t* closest, uint32_t* shadow,
+                                           uint32_t maxDepth);
 
 /* ---- progressive state (host pointers, width*height entries, row-major) ---- */
 VLR_API VLRResult vlrContextReadRNGStates(VLRContextConst context, uint64_t* states);

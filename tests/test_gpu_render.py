@@ -14,8 +14,10 @@ from vlr_amd import api, scenes
 pytestmark = pytest.mark.gpu
 
 
-def render_gpu(desc, W, H, frames, max_len, spectral):
+def render_gpu(desc, W, H, frames, max_len, spectral, wavefront_depth=None, want_accum=False):
     with api.Context(spectral=spectral, max_path_length=max_len) as ctx:
+        if wavefront_depth is not None:
+            ctx.set_wavefront_depth(wavefront_depth)
         bs = api.build_scene(ctx, desc)
         ctx.bind_output(W, H)
         imgs = []
@@ -24,7 +26,20 @@ def render_gpu(desc, W, H, frames, max_len, spectral):
             imgs.append(ctx.read_output())
         st = ctx.stats()
         rng = ctx.read_rng()
+        acc = ctx.read_accumulation() if want_accum else None
+    if want_accum:
+        return imgs, st, rng, acc
     return imgs, st, rng
+
+
+def assert_pixels_match_where_paths_match(gpu_img, ref_img, rng_gpu, rng_ref, W, H, rtol):
+    """Pixels whose RNG states agree drew the same random numbers through the same branches."""
+    eq = (rng_gpu == rng_ref).reshape(H, W)
+    a, b = gpu_img[eq].astype(np.float64), ref_img[..., :3][eq].astype(np.float64)
+    rel = np.abs(a - b) / np.maximum(np.abs(b), 1e-3 * max(1e-12, np.abs(b).mean()))
+    q = np.quantile(rel, 0.999)
+    print("per-pixel relative difference, 99.9th percentile:", q)
+    assert q < rtol
 
 
 def per_frame_images(imgs):
@@ -57,6 +72,7 @@ def test_cornell_mean_within_3_sigma(torch_gpu, spectral):
     same_rng = np.mean(rng_gpu == rng_ref)
     print("pixels with identical RNG state after", frames, "frames:", same_rng)
     assert same_rng > 0.97
+    assert_pixels_match_where_paths_match(gpu_img, ref, rng_gpu, rng_ref, W, H, 1e-3)
     assert st["closestRays"] > 0 and st["numFrames"] == frames
     assert st["droppedNonFinite"] == 0
 
@@ -76,3 +92,26 @@ def test_icosphere_grid_spectral_mean(torch_gpu):
     assert np.all(np.abs(diff) < 3 * np.sqrt(2) * sigma + 1e-6)
     print("identical RNG states:", np.mean(rng_gpu == rng_ref))
     assert np.mean(rng_gpu == rng_ref) > 0.9
+    assert_pixels_match_where_paths_match(imgs[-1][..., :3], ref, rng_gpu, rng_ref, W, H, 1e-3)
+
+
+@pytest.mark.parametrize("scene,spectral", [("cornell", False), ("grid", True)])
+def test_frame_schedule_does_not_change_the_image(torch_gpu, scene, spectral):
+    """Running the deep bounces in the persistent tail kernel gives the same bits as running
+    every depth as its own intersect / shade / shadow kernels."""
+    if scene == "cornell":
+        desc, W, H, depth = scenes.cornell_box(), 48, 48, 8
+    else:
+        desc, W, H, depth = scenes.icosphere_grid(grid=6, env_size=(128, 64)), 96, 54, 16
+    runs = {}
+    for wd in (64, 0, 2, 3):
+        imgs, st, rng, acc = render_gpu(desc, W, H, 3, depth, spectral, wavefront_depth=wd, want_accum=True)
+        runs[wd] = (imgs[-1], st, rng, acc)
+        print("wavefront depth", wd, "rays", st["closestRays"], st["shadowRays"])
+    base = runs[64]
+    for wd in (0, 2, 3):
+        img, st, rng, acc = runs[wd]
+        assert np.array_equal(rng, base[2]), wd
+        assert np.array_equal(acc.view(np.uint32), base[3].view(np.uint32)), wd
+        assert np.array_equal(img.view(np.uint32), base[0].view(np.uint32)), wd
+        assert st["closestRays"] == base[1]["closestRays"] and st["shadowRays"] == base[1]["shadowRays"], wd

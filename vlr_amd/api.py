@@ -59,7 +59,9 @@ class VLRRenderStats(C.Structure):
                 ("msAccumulate", C.c_double), ("msConvert", C.c_double), ("launchesIntersect", C.c_uint64),
                 ("launchesShadow", C.c_uint64), ("launchesShade", C.c_uint64), ("nodesClosest", C.c_uint64),
                 ("trianglesClosest", C.c_uint64), ("instancesClosest", C.c_uint64), ("nodesShadow", C.c_uint64),
-                ("trianglesShadow", C.c_uint64), ("instancesShadow", C.c_uint64)]
+                ("trianglesShadow", C.c_uint64), ("instancesShadow", C.c_uint64), ("msTail", C.c_double),
+                ("launchesTail", C.c_uint64), ("closestRaysWavefront", C.c_uint64),
+                ("shadowRaysWavefront", C.c_uint64)]
 
 
 RAY_DTYPE = np.dtype([("org", np.float32, 3), ("tmin", np.float32), ("dir", np.float32, 3), ("tmax", np.float32)])
@@ -178,6 +180,9 @@ PROTOTYPES: Dict[str, list] = {
     "vlrContextResetRenderStats": [H],
     "vlrContextSetKernelTiming": [H, B],
     "vlrContextSetTraversalStatistics": [H, B],
+    "vlrContextGetQueueCounts": [H, P, P, U32],
+    "vlrContextSetWavefrontDepth": [H, U32],
+    "vlrContextGetWavefrontDepth": [H, C.POINTER(U32)],
     "vlrContextReadRNGStates": [H, P],
     "vlrContextWriteRNGStates": [H, P],
     "vlrContextGetAccumulationChannels": [H, C.POINTER(U32)],
@@ -314,6 +319,15 @@ class Context:
 
     def set_kernel_timing(self, on: bool):
         call("vlrContextSetKernelTiming", self.h, on)
+
+    def queue_counts(self, max_depth: int = 64):
+        c = np.zeros(max_depth, np.uint32)
+        s = np.zeros(max_depth, np.uint32)
+        call("vlrContextGetQueueCounts", self.h, c.ctypes.data, s.ctypes.data, max_depth)
+        return c, s
+
+    def set_wavefront_depth(self, depth: int):
+        call("vlrContextSetWavefrontDepth", self.h, depth)
 
     def set_traversal_stats(self, on: bool):
         call("vlrContextSetTraversalStatistics", self.h, on)

@@ -1041,8 +1041,10 @@ VLR_API VLRResult vlrContextGetRenderStats(VLRContextConst context, VLRRenderSta
     if (!stats) return VLRResult_InvalidArgument;
     Context *c = CTX(context);
     std::memset(stats, 0, sizeof(*stats));
-    uint64_t trav[6];
-    c->statistics(&stats->closestRays, &stats->shadowRays, &stats->droppedNonFinite, trav);
+    uint64_t trav[6], wavefront[2];
+    c->statistics(&stats->closestRays, &stats->shadowRays, &stats->droppedNonFinite, trav, wavefront);
+    stats->closestRaysWavefront = wavefront[0];
+    stats->shadowRaysWavefront = wavefront[1];
     stats->nodesClosest = trav[0];
     stats->trianglesClosest = trav[1];
     stats->instancesClosest = trav[2];
@@ -1060,6 +1062,8 @@ VLR_API VLRResult vlrContextGetRenderStats(VLRContextConst context, VLRRenderSta
     stats->launchesIntersect = c->launches[STAGE_INTERSECT];
     stats->launchesShadow = c->launches[STAGE_SHADOW];
     stats->launchesShade = c->launches[STAGE_SHADE];
+    stats->msTail = c->stageMs[STAGE_TAIL];
+    stats->launchesTail = c->launches[STAGE_TAIL];
     return VLRResult_NoError;
     VLRA_CATCH
 }
@@ -1074,6 +1078,30 @@ VLR_API VLRResult vlrContextSetKernelTiming(VLRContext context, bool enable) {
     VLRA_TRY
     CHECK(context, IS(context, Context));
     CTX(context)->kernelTiming = enable;
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+VLR_API VLRResult vlrContextSetWavefrontDepth(VLRContext context, uint32_t depth) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    CTX(context)->wavefrontDepth = depth;
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+VLR_API VLRResult vlrContextGetWavefrontDepth(VLRContextConst context, uint32_t *depth) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    if (!depth) return VLRResult_InvalidArgument;
+    *depth = CTX(context)->wavefrontDepth;
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+VLR_API VLRResult vlrContextGetQueueCounts(VLRContextConst context, uint32_t *closest, uint32_t *shadow,
+                                           uint32_t maxDepth) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    if (!closest || !shadow) return VLRResult_InvalidArgument;
+    CTX(context)->queueCounts(closest, shadow, maxDepth);
     return VLRResult_NoError;
     VLRA_CATCH
 }

@@ -145,6 +145,7 @@ void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCo
         fp.frameIndex = numAccumFrames;
         fp.maxPathLength = std::min<uint32_t>(maxPathLength, kMaxDepthCounters - 1);
         fp.traversalStats = traversalStats ? 1 : 0;
+        fp.wavefrontDepth = wavefrontDepth;
         FrameBuffers fb;
         fb.ps.rng = static_cast<uint64_t *>(rng.ptr);
         fb.ps.alpha = static_cast<float4 *>(alpha.ptr);
@@ -204,9 +205,11 @@ void Context::evaluateBSDFs(hipStream_t stream, const SurfaceMaterial &m, bool s
     VLRA_HIP(launchBsdfQueries(sc, dm, spectral, q, n, r, stream));
 }
 
-void Context::statistics(uint64_t *closest, uint64_t *shadow, uint64_t *dropped, uint64_t trav[6]) {
+void Context::statistics(uint64_t *closest, uint64_t *shadow, uint64_t *dropped, uint64_t trav[6],
+                         uint64_t wavefront[2]) {
     *closest = *shadow = *dropped = 0;
     for (int k = 0; k < 6; ++k) trav[k] = 0;
+    wavefront[0] = wavefront[1] = 0;
     if (device < 0) return;
     uint32_t host[kNumCounters];
     VLRA_HIP(hipDeviceSynchronize());
@@ -216,6 +219,20 @@ void Context::statistics(uint64_t *closest, uint64_t *shadow, uint64_t *dropped,
     *dropped = host[kStatDropped];
     std::memcpy(trav, host + kStatTravClosest, 24);
     std::memcpy(trav + 3, host + kStatTravShadow, 24);
+    std::memcpy(wavefront, host + kStatWavefront, 16);
+}
+
+void Context::queueCounts(uint32_t *closest, uint32_t *shadow, uint32_t maxDepth) {
+    uint32_t host[kNumCounters] = {};
+    if (device >= 0) {
+        VLRA_HIP(hipDeviceSynchronize());
+        VLRA_HIP(hipMemcpy(host, counters.ptr, sizeof(host), hipMemcpyDeviceToHost));
+    }
+    for (uint32_t d = 0; d < maxDepth; ++d) {
+        const bool inFrame = d < kMaxDepthCounters && d < lastFrame.maxPathLength;
+        closest[d] = inFrame ? host[2 * d] : 0;
+        shadow[d] = inFrame ? host[2 * d + 1] : 0;
+    }
 }
 
 void Context::resetStats() {

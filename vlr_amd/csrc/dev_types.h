@@ -183,7 +183,10 @@ constexpr uint32_t kStatDropped = 2 * kMaxDepthCounters + 4;
 // traversal work (64-bit each: inner nodes fetched, triangles tested, instances entered)
 constexpr uint32_t kStatTravClosest = 2 * kMaxDepthCounters + 6;
 constexpr uint32_t kStatTravShadow = 2 * kMaxDepthCounters + 12;
-constexpr uint32_t kNumCounters = 2 * kMaxDepthCounters + 18;
+constexpr uint32_t kTailWork = 2 * kMaxDepthCounters + 18;   // next queue entry for the tail kernel
+// rays traced by the per-depth kernels (64-bit: extension, shadow); the rest went through the tail
+constexpr uint32_t kStatWavefront = 2 * kMaxDepthCounters + 20;
+constexpr uint32_t kNumCounters = 2 * kMaxDepthCounters + 24;
 
 // path flag bits in meta.w
 constexpr uint32_t kFlagSelMask = 0x3;

@@ -32,6 +32,7 @@ constexpr int kTraceBlock = 128;
 constexpr int kLdsStack = 24;
 constexpr int kScratchStack = 48;
 constexpr int32_t kSentinel = INT32_MIN;
+constexpr uint32_t kTailBlocks = 2048;   // persistent tail grid: 8 blocks of 128 per CU
 
 // ---------------------------------------------------------------------------------------------
 // ray set-up and primitive tests
@@ -516,7 +517,10 @@ struct KArgs {
 __global__ void k_frameBegin(uint32_t *counters, uint32_t numPaths) {
     uint32_t i = threadIdx.x;
     for (uint32_t k = i; k < 2 * kMaxDepthCounters; k += blockDim.x) counters[k] = 0;
-    if (i == 0) counters[0] = numPaths;
+    if (i == 0) {
+        counters[0] = numPaths;
+        counters[kTailWork] = 0;
+    }
 }
 
 // ---- generate: path_tracing.cu:30-70 ----
@@ -606,177 +610,201 @@ __global__ void __launch_bounds__(kTraceBlock) k_intersect(KArgs a) {
 }
 
 // ---- shade: path_tracing.cu:133-259 (hit) and :266-337 (miss) ----
+// Per-path registers of the integrator between two bounces.
+template <int N>
+struct PathRegs {
+    Spec<N> alpha, contrib;
+    Wls<N> wls;
+    float prevDirPDF, initImportance;
+    uint32_t prevType;
+    Pcg32 rng;
+    bool rngUsed;
+};
+
+template <int N>
+__device__ __forceinline__ void loadPath(const FrameBuffers &fb, uint32_t path, PathRegs<N> *P) {
+    P->alpha = loadSpec<N>(fb.ps.alpha[path]);
+    P->contrib = loadSpec<N>(fb.ps.contrib[path]);
+    float4 meta = fb.ps.meta[path];
+    uint32_t flags = __float_as_uint(meta.w);
+    P->wls.offset = meta.x;
+    P->wls.sel = flags & kFlagSelMask;
+    P->wls.single = (flags & kFlagSingle) != 0;
+    P->prevDirPDF = meta.y;
+    P->initImportance = meta.z;
+    P->prevType = (flags >> kFlagPrevTypeShift) & 0x7Fu;
+    P->rng.s = fb.ps.rng[path];
+    P->rngUsed = false;
+}
+
+// One bounce of path_tracing.cu:133-337 for a ray that hit (ha.x >= 0) or missed: implicit
+// emission with MIS, Russian roulette, next-event estimation (returns the shadow ray and the
+// contribution it carries when unoccluded) and BSDF sampling (returns the next ray).
+template <int N>
+__device__ __forceinline__ void shadeStep(const DevScene &sc, uint32_t pathLength, uint32_t maxPathLength,
+                                          uint32_t path, const V3 &rayOrg, const V3 &rayDir, int4 ha, float2 hb,
+                                          PathRegs<N> &P, bool *emitShadow, float4 *shOrg, float4 *shDir,
+                                          float4 *shVal, bool *emitNext, float4 *nextOrg, float4 *nextDir) {
+    *emitShadow = false;
+    *emitNext = false;
+    const bool maxLengthTerminate = pathLength >= maxPathLength;
+    if (ha.x < 0) {
+        // miss: environment light with MIS
+        if (sc.hasEnv && sc.envImportance > 0.0f) {
+            SurfPt sp;
+            float hypAreaPDF;
+            envPointFromDir(sc, rayDir, &sp, &hypAreaPDF);
+            const DevMaterial &m = sc.materials[sc.envMaterial];
+            V3 dirOutLocal = sp.frame.toLocal(-rayDir);
+            Spec<N> M;
+            float fe;
+            evalEDF<N>(sc, m, sp, P.wls, dirOutLocal, &M, &fe);
+            if (!M.isZero()) {
+                float mis = 1.0f;
+                if (!dt::isDelta(P.prevType) && pathLength > 1) {
+                    float instProb = 1.0f / sc.lightInstIntegral;
+                    float bsdfPDF = P.prevDirPDF;
+                    float lightPDF = instProb * hypAreaPDF * 1.0f / fabsf(dirOutLocal.z);
+                    mis = (bsdfPDF * bsdfPDF) / (lightPDF * lightPDF + bsdfPDF * bsdfPDF);
+                }
+                P.contrib += P.alpha * (M * fe) * mis;
+            }
+        }
+        return;
+    }
+    SurfPt sp;
+    float hypAreaPDF;
+    decodeHit(sc, ha.x, ha.y, ha.z, hb.x, hb.y, &sp, &hypAreaPDF);
+    const DevGeomInst &G = sc.geomInsts[ha.y];
+    const DevMaterial &mat = sc.materials[G.material];
+    V3 dirOutLocal = sp.frame.toLocal(-rayDir);
+    if (mat.isEmitter) {
+        Spec<N> M;
+        float fe;
+        evalEDF<N>(sc, mat, sp, P.wls, dirOutLocal, &M, &fe);
+        if (!M.isZero()) {
+            float mis = 1.0f;
+            if (!dt::isDelta(P.prevType) && pathLength > 1) {
+                const DevInstance &I = sc.instances[ha.x];
+                float instProb = I.lightDistIntegral / sc.lightInstIntegral;
+                float geomInstProb = G.importance / I.lightDistIntegral;
+                float bsdfPDF = P.prevDirPDF;
+                float dist2 = (sp.position - rayOrg).sqLength();
+                float lightPDF = instProb * geomInstProb * hypAreaPDF * dist2 / fabsf(dirOutLocal.z);
+                mis = (bsdfPDF * bsdfPDF) / (lightPDF * lightPDF + bsdfPDF * bsdfPDF);
+            }
+            P.contrib += P.alpha * (M * fe) * mis;
+        }
+    }
+    if (maxLengthTerminate) return;
+    P.rngUsed = true;
+    float continueProb = fminf(P.alpha.importance(P.wls.sel) / P.initImportance, 1.0f);
+    if (P.rng.uniform() >= continueProb) return;
+    P.alpha = P.alpha / continueProb;
+    Bsdf<N> bsdf = setupBsdf<N>(sc, mat, P.wls);
+    V3 gnL = sp.frame.toLocal(sp.gn);
+    if (bsdf.hasNonDelta()) {
+        float uLight = P.rng.uniform();
+        float uP0 = P.rng.uniform(), uP1 = P.rng.uniform();
+        // a scene without emitters has nothing to sample (the draws are still made)
+        if (sc.lightInstIntegral > 0.0f) {
+            LightSample ls;
+            float lightProb;
+            sampleLight(sc, uLight, uP0, uP1, &ls, &lightProb);
+            const DevMaterial &lm = sc.materials[ls.material];
+            // shadow ray direction (SurfacePoint::calcDirectionFrom)
+            V3 sdir;
+            float dist2;
+            if (ls.sp.atInfinity) {
+                dist2 = 1.0f;
+                sdir = normalize(ls.sp.position);
+            } else {
+                V3 dv = ls.sp.position - sp.position;
+                dist2 = dv.sqLength();
+                sdir = dv / sqrtf(dist2);
+            }
+            V3 sdl = ls.sp.frame.toLocal(-sdir);
+            Spec<N> M;
+            float fe;
+            evalEDF<N>(sc, lm, ls.sp, P.wls, sdl, &M, &fe);
+            if (!M.isZero()) {
+                float rec = 1.0f / dist2;
+                V3 sdsn = sp.frame.toLocal(sdir);
+                float lightPDF = lightProb * ls.areaPDF;
+                Spec<N> fs = bsdf.evaluate(dirOutLocal, gnL, sdsn);
+                float cosLight = ls.sp.isPoint ? 1.0f : absDot(-sdir, ls.sp.gn);
+                float bsdfPDF = bsdf.evaluatePDF(dirOutLocal, sdsn, P.wls.sel) * cosLight * rec;
+                float mis = 1.0f;
+                if (!dt::isDelta(ls.posType) && !isinf(lightPDF))
+                    mis = (lightPDF * lightPDF) / (lightPDF * lightPDF + bsdfPDF * bsdfPDF);
+                float Gf = 1.0f * absDot(sdsn, gnL) * cosLight * rec;
+                float scalarCoeff = Gf * mis / lightPDF;
+                Spec<N> value = P.alpha * (M * fe) * fs * scalarCoeff;
+                // testVisibility (light_transport_common.h:147-177)
+                bool front = dot(sp.gn, sdir) > 0;
+                V3 so = offsetRayOrigin(sp.position, front ? sp.gn : -sp.gn);
+                front = -dot(ls.sp.gn, sdir) > 0;
+                V3 lo = offsetRayOrigin(ls.sp.position, front ? ls.sp.gn : -ls.sp.gn);
+                float tmax = 3.402823466e+38f;
+                if (!ls.sp.atInfinity) tmax = (lo - so).length() * 0.9999f;
+                *emitShadow = true;
+                *shOrg = make_float4(so.x, so.y, so.z, __uint_as_float(path));
+                *shDir = make_float4(sdir.x, sdir.y, sdir.z, tmax);
+                *shVal = storeSpec<N>(value);
+            }
+        }
+    }
+    float uC = P.rng.uniform(), uD0 = P.rng.uniform(), uD1 = P.rng.uniform();
+    V3 dirL;
+    float dirPDF;
+    uint32_t sType;
+    Spec<N> fs = bsdf.sample(dirOutLocal, gnL, P.wls.sel, uC, uD0, uD1, &dirL, &dirPDF, &sType);
+    if (fs.isZero() || dirPDF == 0.0f) return;
+    if ((sType & dt::Dispersive) && !P.wls.single) {
+        dirPDF /= N;
+        P.wls.single = true;
+    }
+    float cosFactor = dot(dirL, gnL);
+    P.alpha = P.alpha * (fs * (fabsf(cosFactor) / dirPDF));
+    V3 dirIn = sp.frame.fromLocal(dirL);
+    V3 no = offsetRayOrigin(sp.position, cosFactor > 0.0f ? sp.gn : -sp.gn);
+    *emitNext = true;
+    *nextOrg = make_float4(no.x, no.y, no.z, __uint_as_float(path));
+    *nextDir = make_float4(dirIn.x, dirIn.y, dirIn.z, 3.402823466e+38f);
+    P.prevDirPDF = dirPDF;
+    P.prevType = sType;
+}
+
+template <int N>
+__device__ __forceinline__ void storePath(const FrameBuffers &fb, uint32_t path, uint32_t pathLength,
+                                          const PathRegs<N> &P, bool continues) {
+    fb.ps.contrib[path] = storeSpec<N>(P.contrib);
+    if (P.rngUsed) fb.ps.rng[path] = P.rng.s;
+    if (continues) {
+        fb.ps.alpha[path] = storeSpec<N>(P.alpha);
+        fb.ps.meta[path] = make_float4(P.wls.offset, P.prevDirPDF, P.initImportance,
+                                       __uint_as_float(packFlags(P.wls.sel, P.wls.single, P.prevType, pathLength)));
+    }
+}
+
 template <int N>
 __global__ void __launch_bounds__(256) k_shade(KArgs a) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t depth = a.depth;
     const uint32_t count = a.fb.counters[2 * depth];
-    const bool valid = i < count;
     bool emitNext = false, emitShadow = false;
     float4 nextOrg, nextDir, shOrg, shDir, shVal;
-    uint32_t path = 0;
-    if (valid) {
-        const DevScene &sc = a.sc;
+    if (i < count) {
         const RayQueue q = a.fb.rays[depth & 1];
         float4 o4 = q.org[i], d4 = q.dir[i];
-        path = __float_as_uint(o4.w);
-        V3 rayOrg = xyz(o4), rayDir = xyz(d4);
+        const uint32_t path = __float_as_uint(o4.w);
         int4 ha = a.fb.hits.a[i];
-        Spec<N> alpha = loadSpec<N>(a.fb.ps.alpha[path]);
-        Spec<N> contrib = loadSpec<N>(a.fb.ps.contrib[path]);
-        float4 meta = a.fb.ps.meta[path];
-        uint32_t flags = __float_as_uint(meta.w);
-        Wls<N> wls;
-        wls.offset = meta.x;
-        wls.sel = flags & kFlagSelMask;
-        wls.single = (flags & kFlagSingle) != 0;
-        float prevDirPDF = meta.y;
-        const float initImportance = meta.z;
-        uint32_t prevType = (flags >> kFlagPrevTypeShift) & 0x7Fu;
-        const uint32_t pathLength = depth + 1;
-        const bool maxLengthTerminate = pathLength >= a.fp.maxPathLength;
-        Pcg32 rng{a.fb.ps.rng[path]};
-        bool rngUsed = false;
-
-        if (ha.x < 0) {
-            // miss: environment light with MIS
-            if (sc.hasEnv && sc.envImportance > 0.0f) {
-                SurfPt sp;
-                float hypAreaPDF;
-                envPointFromDir(sc, rayDir, &sp, &hypAreaPDF);
-                const DevMaterial &m = sc.materials[sc.envMaterial];
-                V3 dirOutLocal = sp.frame.toLocal(-rayDir);
-                Spec<N> M;
-                float fe;
-                evalEDF<N>(sc, m, sp, wls, dirOutLocal, &M, &fe);
-                if (!M.isZero()) {
-                    float mis = 1.0f;
-                    if (!dt::isDelta(prevType) && pathLength > 1) {
-                        float instProb = 1.0f / sc.lightInstIntegral;
-                        float bsdfPDF = prevDirPDF;
-                        float lightPDF = instProb * hypAreaPDF * 1.0f / fabsf(dirOutLocal.z);
-                        mis = (bsdfPDF * bsdfPDF) / (lightPDF * lightPDF + bsdfPDF * bsdfPDF);
-                    }
-                    contrib += alpha * (M * fe) * mis;
-                }
-            }
-        } else {
-            float2 hb = a.fb.hits.b[i];
-            SurfPt sp;
-            float hypAreaPDF;
-            decodeHit(sc, ha.x, ha.y, ha.z, hb.x, hb.y, &sp, &hypAreaPDF);
-            const DevGeomInst &G = sc.geomInsts[ha.y];
-            const DevMaterial &mat = sc.materials[G.material];
-            V3 dirOutLocal = sp.frame.toLocal(-rayDir);
-            bool alive = true;
-            if (mat.isEmitter) {
-                Spec<N> M;
-                float fe;
-                evalEDF<N>(sc, mat, sp, wls, dirOutLocal, &M, &fe);
-                if (!M.isZero()) {
-                    float mis = 1.0f;
-                    if (!dt::isDelta(prevType) && pathLength > 1) {
-                        const DevInstance &I = sc.instances[ha.x];
-                        float instProb = I.lightDistIntegral / sc.lightInstIntegral;
-                        float geomInstProb = G.importance / I.lightDistIntegral;
-                        float bsdfPDF = prevDirPDF;
-                        float dist2 = (sp.position - rayOrg).sqLength();
-                        float lightPDF = instProb * geomInstProb * hypAreaPDF * dist2 / fabsf(dirOutLocal.z);
-                        mis = (bsdfPDF * bsdfPDF) / (lightPDF * lightPDF + bsdfPDF * bsdfPDF);
-                    }
-                    contrib += alpha * (M * fe) * mis;
-                }
-            }
-            if (maxLengthTerminate) alive = false;
-            if (alive) {
-                rngUsed = true;
-                float continueProb = fminf(alpha.importance(wls.sel) / initImportance, 1.0f);
-                if (rng.uniform() >= continueProb) alive = false;
-                else alpha = alpha / continueProb;
-            }
-            if (alive) {
-                Bsdf<N> bsdf = setupBsdf<N>(sc, mat, wls);
-                V3 gnL = sp.frame.toLocal(sp.gn);
-                if (bsdf.hasNonDelta()) {
-                    float uLight = rng.uniform();
-                    float uP0 = rng.uniform(), uP1 = rng.uniform();
-                    // a scene without emitters has nothing to sample (the draws are still made)
-                    if (sc.lightInstIntegral > 0.0f) {
-                    LightSample ls;
-                    float lightProb;
-                    sampleLight(sc, uLight, uP0, uP1, &ls, &lightProb);
-                    const DevMaterial &lm = sc.materials[ls.material];
-                    // shadow ray direction (SurfacePoint::calcDirectionFrom)
-                    V3 sdir;
-                    float dist2;
-                    if (ls.sp.atInfinity) {
-                        dist2 = 1.0f;
-                        sdir = normalize(ls.sp.position);
-                    } else {
-                        V3 dv = ls.sp.position - sp.position;
-                        dist2 = dv.sqLength();
-                        sdir = dv / sqrtf(dist2);
-                    }
-                    V3 sdl = ls.sp.frame.toLocal(-sdir);
-                    Spec<N> M;
-                    float fe;
-                    evalEDF<N>(sc, lm, ls.sp, wls, sdl, &M, &fe);
-                    if (!M.isZero()) {
-                        float rec = 1.0f / dist2;
-                        V3 sdsn = sp.frame.toLocal(sdir);
-                        float lightPDF = lightProb * ls.areaPDF;
-                        Spec<N> fs = bsdf.evaluate(dirOutLocal, gnL, sdsn);
-                        float cosLight = ls.sp.isPoint ? 1.0f : absDot(-sdir, ls.sp.gn);
-                        float bsdfPDF = bsdf.evaluatePDF(dirOutLocal, sdsn, wls.sel) * cosLight * rec;
-                        float mis = 1.0f;
-                        if (!dt::isDelta(ls.posType) && !isinf(lightPDF))
-                            mis = (lightPDF * lightPDF) / (lightPDF * lightPDF + bsdfPDF * bsdfPDF);
-                        float G = 1.0f * absDot(sdsn, gnL) * cosLight * rec;
-                        float scalarCoeff = G * mis / lightPDF;
-                        Spec<N> value = alpha * (M * fe) * fs * scalarCoeff;
-                        // testVisibility (light_transport_common.h:147-177)
-                        bool front = dot(sp.gn, sdir) > 0;
-                        V3 so = offsetRayOrigin(sp.position, front ? sp.gn : -sp.gn);
-                        front = -dot(ls.sp.gn, sdir) > 0;
-                        V3 lo = offsetRayOrigin(ls.sp.position, front ? ls.sp.gn : -ls.sp.gn);
-                        float tmax = 3.402823466e+38f;
-                        if (!ls.sp.atInfinity) tmax = (lo - so).length() * 0.9999f;
-                        emitShadow = true;
-                        shOrg = make_float4(so.x, so.y, so.z, __uint_as_float(path));
-                        shDir = make_float4(sdir.x, sdir.y, sdir.z, tmax);
-                        shVal = storeSpec<N>(value);
-                    }
-                    }
-                }
-                float uC = rng.uniform(), uD0 = rng.uniform(), uD1 = rng.uniform();
-                V3 dirL;
-                float dirPDF;
-                uint32_t sType;
-                Spec<N> fs = bsdf.sample(dirOutLocal, gnL, wls.sel, uC, uD0, uD1, &dirL, &dirPDF, &sType);
-                if (!(fs.isZero() || dirPDF == 0.0f)) {
-                    if ((sType & dt::Dispersive) && !wls.single) {
-                        dirPDF /= N;
-                        wls.single = true;
-                    }
-                    float cosFactor = dot(dirL, gnL);
-                    alpha = alpha * (fs * (fabsf(cosFactor) / dirPDF));
-                    V3 dirIn = sp.frame.fromLocal(dirL);
-                    V3 no = offsetRayOrigin(sp.position, cosFactor > 0.0f ? sp.gn : -sp.gn);
-                    emitNext = true;
-                    nextOrg = make_float4(no.x, no.y, no.z, __uint_as_float(path));
-                    nextDir = make_float4(dirIn.x, dirIn.y, dirIn.z, 3.402823466e+38f);
-                    prevDirPDF = dirPDF;
-                    prevType = sType;
-                }
-            }
-        }
-        a.fb.ps.contrib[path] = storeSpec<N>(contrib);
-        if (rngUsed) a.fb.ps.rng[path] = rng.s;
-        if (emitNext) {
-            a.fb.ps.alpha[path] = storeSpec<N>(alpha);
-            a.fb.ps.meta[path] = make_float4(wls.offset, prevDirPDF, initImportance,
-                                             __uint_as_float(packFlags(wls.sel, wls.single, prevType, pathLength)));
-        }
+        float2 hb = ha.x >= 0 ? a.fb.hits.b[i] : make_float2(0.0f, 0.0f);
+        PathRegs<N> P;
+        loadPath<N>(a.fb, path, &P);
+        shadeStep<N>(a.sc, depth + 1, a.fp.maxPathLength, path, xyz(o4), xyz(d4), ha, hb, P, &emitShadow, &shOrg,
+                     &shDir, &shVal, &emitNext, &nextOrg, &nextDir);
+        storePath<N>(a.fb, path, depth + 1, P, emitNext);
     }
     // queue appends (every lane of the wave takes part in the ballots)
     uint32_t slot = appendSlot(&a.fb.counters[2 * (depth + 1)], emitNext);
@@ -812,6 +840,66 @@ __global__ void __launch_bounds__(kTraceBlock) k_shadow(KArgs a) {
         }
     }
     if constexpr (STATS) addTravStats(a.fb.counters, kStatTravShadow, tc);
+}
+
+// ---- tail: from depth a.depth on, each path runs intersect -> shade -> shadow in a loop ----
+// Persistent waves take 64 queue entries at a time.  Every path consumes its random numbers
+// and adds its contributions in the same order as the per-depth kernels, so a frame's result
+// does not depend on the depth at which paths switch to this kernel.
+template <int N, bool STATS>
+__global__ void __launch_bounds__(kTraceBlock) k_tail(KArgs a) {
+    __shared__ int32_t lds[kLdsStack * kTraceBlock];
+    const DevScene &sc = a.sc;
+    const uint32_t depth0 = a.depth;
+    const uint32_t count = a.fb.counters[2 * depth0];
+    const RayQueue q = a.fb.rays[depth0 & 1];
+    TravCount tc, tcShadow;
+    while (true) {
+        uint32_t base = 0;
+        if (__lane_id() == 0) base = atomicAdd(&a.fb.counters[kTailWork], 64u);
+        base = __shfl(base, 0);
+        if (base >= count) break;
+        const uint32_t i = base + __lane_id();
+        if (i >= count) continue;
+        float4 o4 = q.org[i], d4 = q.dir[i];
+        const uint32_t path = __float_as_uint(o4.w);
+        PathRegs<N> P;
+        loadPath<N>(a.fb, path, &P);
+        V3 org = xyz(o4), dir = xyz(d4);
+        float tmax = d4.w;
+        uint32_t depth = depth0;
+        while (true) {
+            TraceHit h;
+            h.inst = -1;
+            h.b1 = h.b2 = 0.0f;
+            bool found = traceRay<false, STATS>(sc, org, dir, 0.0f, tmax, &h, lds, threadIdx.x, &tc);
+            int4 ha = found ? make_int4(h.inst, h.geomInst, h.prim, 0) : make_int4(-1, -1, -1, 0);
+            bool emitShadow, emitNext;
+            float4 shOrg, shDir, shVal, nextOrg, nextDir;
+            shadeStep<N>(sc, depth + 1, a.fp.maxPathLength, path, org, dir, ha, make_float2(h.b1, h.b2), P,
+                         &emitShadow, &shOrg, &shDir, &shVal, &emitNext, &nextOrg, &nextDir);
+            if (emitShadow) {
+                atomicAdd(&a.fb.counters[2 * depth + 1], 1u);
+                TraceHit sh;
+                bool occluded = traceRay<true, STATS>(sc, xyz(shOrg), xyz(shDir), 0.0f, shDir.w, &sh, lds,
+                                                      threadIdx.x, &tcShadow);
+                if (!occluded) P.contrib += loadSpec<N>(shVal);
+            }
+            if (!emitNext) {
+                storePath<N>(a.fb, path, depth + 1, P, false);
+                break;
+            }
+            ++depth;
+            atomicAdd(&a.fb.counters[2 * depth], 1u);
+            org = xyz(nextOrg);
+            dir = xyz(nextDir);
+            tmax = nextDir.w;
+        }
+    }
+    if constexpr (STATS) {
+        addTravStats(a.fb.counters, kStatTravClosest, tc);
+        addTravStats(a.fb.counters, kStatTravShadow, tcShadow);
+    }
 }
 
 // ---- accumulate: Kahan sum per channel (common_internal.h:350-368, spectrum_types.h:674-684) ----
@@ -901,17 +989,24 @@ __global__ void __launch_bounds__(256) k_convert(KArgs a) {
     a.fb.output[p] = make_float4(R * res, G * res, B * res, 1.0f);
 }
 
-__global__ void k_frameEnd(uint32_t *counters, uint32_t maxPathLength) {
+__global__ void k_frameEnd(uint32_t *counters, uint32_t maxPathLength, uint32_t wavefrontDepth) {
     // fold this frame's queue sizes into the 64-bit ray statistics
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    uint64_t closest = 0, shadow = 0;
+    uint64_t closest = 0, shadow = 0, wfClosest = 0, wfShadow = 0;
     for (uint32_t d = 0; d < maxPathLength && d < kMaxDepthCounters - 1; ++d) {
         closest += counters[2 * d];
         shadow += counters[2 * d + 1];
+        if (d < wavefrontDepth) {
+            wfClosest += counters[2 * d];
+            wfShadow += counters[2 * d + 1];
+        }
     }
     uint64_t *c64 = reinterpret_cast<uint64_t *>(counters + kStatClosest);
     c64[0] += closest;
     c64[1] += shadow;
+    uint64_t *w64 = reinterpret_cast<uint64_t *>(counters + kStatWavefront);
+    w64[0] += wfClosest;
+    w64[1] += wfShadow;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -1009,7 +1104,9 @@ static hipError_t launchFrameT(const DevScene &sc, const FrameParams &fp, const 
     t0(STAGE_GENERATE);
     hipLaunchKernelGGL(k_generate<N>, dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
     t1(STAGE_GENERATE);
-    for (uint32_t d = 0; d < fp.maxPathLength && d + 1 < kMaxDepthCounters; ++d) {
+    const uint32_t numDepths = std::min<uint32_t>(fp.maxPathLength, kMaxDepthCounters - 1);
+    const uint32_t wavefrontDepths = std::min(fp.wavefrontDepth, numDepths);
+    for (uint32_t d = 0; d < wavefrontDepths; ++d) {
         a.depth = d;
         t0(STAGE_INTERSECT);
         if (fp.traversalStats)
@@ -1027,13 +1124,23 @@ static hipError_t launchFrameT(const DevScene &sc, const FrameParams &fp, const 
             hipLaunchKernelGGL(k_shadow<false>, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
         t1(STAGE_SHADOW);
     }
+    if (wavefrontDepths < numDepths) {
+        a.depth = wavefrontDepths;
+        const uint32_t blocks = std::min(blocksFor(n, kTraceBlock), kTailBlocks);
+        t0(STAGE_TAIL);
+        if (fp.traversalStats)
+            hipLaunchKernelGGL((k_tail<N, true>), dim3(blocks), dim3(kTraceBlock), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_tail<N, false>), dim3(blocks), dim3(kTraceBlock), 0, s, a);
+        t1(STAGE_TAIL);
+    }
     t0(STAGE_ACCUMULATE);
     hipLaunchKernelGGL(k_accumulate<N>, dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
     t1(STAGE_ACCUMULATE);
     t0(STAGE_CONVERT);
     hipLaunchKernelGGL(k_convert<N>, dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
     t1(STAGE_CONVERT);
-    hipLaunchKernelGGL(k_frameEnd, dim3(1), dim3(64), 0, s, fb.counters, fp.maxPathLength);
+    hipLaunchKernelGGL(k_frameEnd, dim3(1), dim3(64), 0, s, fb.counters, fp.maxPathLength, wavefrontDepths);
     return hipGetLastError();
 }
 

@@ -374,6 +374,7 @@ public:
     bool spectralActive = false;
     bool kernelTiming = false;
     bool traversalStats = false;
+    uint32_t wavefrontDepth = kDefaultWavefrontDepth;
     std::vector<hipEvent_t> eventPool;   // reused by kernel timing, 2 per timed launch
     uint32_t width = 0, height = 0;
     uint32_t numAccumFrames = 0;
@@ -402,8 +403,10 @@ public:
     void traceRays(hipStream_t stream, const Scene &s, const void *rays, uint32_t n, void *hits, bool anyHit);
     void evaluateBSDFs(hipStream_t stream, const SurfaceMaterial &m, bool spectral, const void *q, uint32_t n, void *r);
     // trav: nodes, triangles, instances of closest-hit then of shadow traversals
-    void statistics(uint64_t *closest, uint64_t *shadow, uint64_t *dropped, uint64_t trav[6]);
+    // wavefront: extension and shadow rays traced by the per-depth kernels
+    void statistics(uint64_t *closest, uint64_t *shadow, uint64_t *dropped, uint64_t trav[6], uint64_t wavefront[2]);
     void resetStats();
+    void queueCounts(uint32_t *closest, uint32_t *shadow, uint32_t maxDepth);
 };
 
 void throwHip(hipError_t e, const char *what);
