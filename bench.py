@@ -44,7 +44,7 @@ RAY_IN = 32          # origin.xyz + path index, direction.xyz + tmax
 HIT_OUT = 24         # (inst, geomInst, prim, t) + (b1, b2)
 SHADOW_IN = 48       # origin, direction, contribution
 SHADOW_RMW = 32      # contribution read + write when unoccluded
-NODE = 64            # BVH2 node: two child boxes + two child indices
+NODE = 128           # 4-wide BVH node: four child boxes + four child codes
 TRI = 48             # three vertices + geometry-instance / primitive indices
 INST = 48            # world-to-object 3x4 rows read on instance entry
 # one shaded hit: ray 32 + hit 24 + path state read and written 2 x 56 + three vertices 144 +

@@ -1146,9 +1146,7 @@ VLR_API VLRResult vlrContextReadAccumulation(VLRContextConst context, float *dat
     CHECK(context, IS(context, Context));
     Context *c = CTX(context);
     if (!data || !c->width) return VLRResult_InvalidArgument;
-    VLRA_HIP(hipDeviceSynchronize());
-    VLRA_HIP(hipMemcpy(data, c->accum.ptr, size_t(c->width) * c->height * c->accumChannels * 2 * 4,
-                       hipMemcpyDeviceToHost));
+    c->readAccumulation(data);
     return VLRResult_NoError;
     VLRA_CATCH
 }
@@ -1157,9 +1155,7 @@ VLR_API VLRResult vlrContextWriteAccumulation(VLRContext context, const float *d
     CHECK(context, IS(context, Context));
     Context *c = CTX(context);
     if (!data || !c->width) return VLRResult_InvalidArgument;
-    VLRA_HIP(hipDeviceSynchronize());
-    VLRA_HIP(hipMemcpy(c->accum.ptr, data, size_t(c->width) * c->height * c->accumChannels * 2 * 4,
-                       hipMemcpyHostToDevice));
+    c->writeAccumulation(data);
     c->numAccumFrames = numAccumFrames;
     return VLRResult_NoError;
     VLRA_CATCH

@@ -235,6 +235,24 @@ void Context::queueCounts(uint32_t *closest, uint32_t *shadow, uint32_t maxDepth
     }
 }
 
+void Context::readAccumulation(float *perPixel) {
+    const size_t n = size_t(width) * height, planes = size_t(accumChannels) * 2;
+    std::vector<float> dev(n * planes);
+    VLRA_HIP(hipDeviceSynchronize());
+    VLRA_HIP(hipMemcpy(dev.data(), accum.ptr, dev.size() * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t q = 0; q < planes; ++q)
+        for (size_t p = 0; p < n; ++p) perPixel[p * planes + q] = dev[q * n + p];
+}
+
+void Context::writeAccumulation(const float *perPixel) {
+    const size_t n = size_t(width) * height, planes = size_t(accumChannels) * 2;
+    std::vector<float> dev(n * planes);
+    for (size_t q = 0; q < planes; ++q)
+        for (size_t p = 0; p < n; ++p) dev[q * n + p] = perPixel[p * planes + q];
+    VLRA_HIP(hipDeviceSynchronize());
+    VLRA_HIP(hipMemcpy(accum.ptr, dev.data(), dev.size() * sizeof(float), hipMemcpyHostToDevice));
+}
+
 void Context::resetStats() {
     if (device >= 0) {
         VLRA_HIP(hipDeviceSynchronize());

@@ -1,8 +1,8 @@
 // Device data layout of the light-transport path (shared by host set-up code and kernels).
 //
 // HBM layout, all arrays 16-byte aligned:
-//   BVH nodes      64 B: the two children's boxes stored in the parent (Aila-Laine layout),
-//                  child index > 0 -> inner node, < 0 -> leaf code ~((first << 3) | (count - 1)),
+//   BVH nodes     128 B, 4-wide: the four children's boxes stored in the parent (SoA), child
+//                  index > 0 -> inner node, < 0 -> leaf code ~((first << 3) | (count - 1)),
 //                  0 -> empty slot (index 0 is the root, never a child).
 //   BLAS triangles 48 B: three float4 = object-space vertices; .w lanes carry the global
 //                  geometry-instance index and the triangle index inside it.
@@ -33,6 +33,16 @@ struct alignas(16) BVHNode {
     uint32_t pad[2];
 };
 static_assert(sizeof(BVHNode) == 64, "node layout");
+
+// 4-wide node, the layout the traversal reads (128 B, one L2 line): the four child boxes as
+// structure of arrays, then the four child codes (> 0 inner node, < 0 leaf code, 0 empty slot).
+// Built by collapsing the binned-SAH BVH2 (two levels into one where it pays).
+struct alignas(16) BVH4Node {
+    float loX[4], hiX[4], loY[4], hiY[4], loZ[4], hiZ[4];
+    int32_t child[4];
+    uint32_t pad[4];
+};
+static_assert(sizeof(BVH4Node) == 128, "4-wide node layout");
 
 struct alignas(16) TriRecord {
     float v0[3]; uint32_t geomInst;
@@ -114,8 +124,8 @@ struct DevCamera {
 };
 
 struct DevScene {
-    const BVHNode* tlasNodes;
-    const BVHNode* blasNodes;
+    const BVH4Node* tlasNodes;
+    const BVH4Node* blasNodes;
     const TriRecord* blasTris;
     const DevInstance* instances;
     const uint32_t* instGeomInsts;

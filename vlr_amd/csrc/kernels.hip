@@ -30,7 +30,7 @@ namespace vlra {
 
 constexpr int kTraceBlock = 128;
 constexpr int kLdsStack = 24;
-constexpr int kScratchStack = 48;
+constexpr int kScratchStack = 64;
 constexpr int32_t kSentinel = INT32_MIN;
 constexpr uint32_t kTailBlocks = 2048;   // persistent tail grid: 8 blocks of 128 per CU
 
@@ -125,35 +125,49 @@ __device__ __forceinline__ bool traceRay(const DevScene &sc, const V3 &org, cons
     int32_t ovf[kScratchStack];
     int sp = 0;
     bool inBlas = false;
-    const BVHNode *nodes = sc.tlasNodes;
+    const BVH4Node *nodes = sc.tlasNodes;
     const TriRecord *tris = sc.blasTris;
     int32_t curInst = -1;
     bool found = false;
     int32_t cur = 0;
+    auto push = [&](int32_t v) {
+        if (sp < kLdsStack) lds[sp * kTraceBlock + lane] = v;
+        else ovf[min(sp - kLdsStack, kScratchStack - 1)] = v;
+        ++sp;
+    };
     while (true) {
         if (cur >= 0) {
             if constexpr (STATS) ++tc->nodes;
             const float4 *np = reinterpret_cast<const float4 *>(nodes + cur);
-            const float4 na = np[0], nb = np[1], nc = np[2];
-            const int4 nd = reinterpret_cast<const int4 *>(np)[3];
-            float t0, t1;
-            int32_t c0 = nd.x, c1 = nd.y;
-            // child index 0 is the root and therefore marks an empty slot
-            bool h0 = c0 != 0 && slab(V3(na.x, na.y, na.z), V3(na.w, nb.x, nb.y), ray, tmin, tmax, &t0);
-            bool h1 = c1 != 0 && slab(V3(nb.z, nb.w, nc.x), V3(nc.y, nc.z, nc.w), ray, tmin, tmax, &t1);
-            if (h0 && h1) {
-                int32_t nearC = c0, farC = c1;
-                if (t1 < t0) { nearC = c1; farC = c0; }
-                if (sp < kLdsStack) lds[sp * kTraceBlock + lane] = farC;
-                else ovf[min(sp - kLdsStack, kScratchStack - 1)] = farC;
-                ++sp;
-                cur = nearC;
-                continue;
-            } else if (h0) {
+            const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+            const int4 cc = reinterpret_cast<const int4 *>(np)[6];
+            // child code 0 (the root's index) marks an empty slot
+            constexpr float kMiss = __builtin_huge_valf();
+            float t0 = kMiss, t1 = kMiss, t2 = kMiss, t3 = kMiss;
+            int32_t c0 = cc.x, c1 = cc.y, c2 = cc.z, c3 = cc.w;
+            float tt;
+            if (c0 != 0 && slab(V3(lx.x, ly.x, lz.x), V3(hx.x, hy.x, hz.x), ray, tmin, tmax, &tt)) t0 = tt;
+            if (c1 != 0 && slab(V3(lx.y, ly.y, lz.y), V3(hx.y, hy.y, hz.y), ray, tmin, tmax, &tt)) t1 = tt;
+            if (c2 != 0 && slab(V3(lx.z, ly.z, lz.z), V3(hx.z, hy.z, hz.z), ray, tmin, tmax, &tt)) t2 = tt;
+            if (c3 != 0 && slab(V3(lx.w, ly.w, lz.w), V3(hx.w, hy.w, hz.w), ray, tmin, tmax, &tt)) t3 = tt;
+            const int hits = (t0 != kMiss) + (t1 != kMiss) + (t2 != kMiss) + (t3 != kMiss);
+            if (hits > 0) {
+                // sort the four (distance, child) pairs, nearest first; misses sort last
+                auto cswap = [](float &ta, int32_t &ca, float &tb, int32_t &cb) {
+                    if (tb < ta) {
+                        float t = ta; ta = tb; tb = t;
+                        int32_t c = ca; ca = cb; cb = c;
+                    }
+                };
+                cswap(t0, c0, t1, c1);
+                cswap(t2, c2, t3, c3);
+                cswap(t0, c0, t2, c2);
+                cswap(t1, c1, t3, c3);
+                cswap(t1, c1, t2, c2);
+                if (hits > 3) push(c3);
+                if (hits > 2) push(c2);
+                if (hits > 1) push(c1);
                 cur = c0;
-                continue;
-            } else if (h1) {
-                cur = c1;
                 continue;
             }
         } else if (cur != kSentinel) {
@@ -171,9 +185,7 @@ __device__ __forceinline__ bool traceRay(const DevScene &sc, const V3 &org, cons
                 nodes = sc.blasNodes + I.blasNodeBase;
                 tris = sc.blasTris + I.blasTriBase;
                 inBlas = true;
-                if (sp < kLdsStack) lds[sp * kTraceBlock + lane] = kSentinel;
-                else ovf[min(sp - kLdsStack, kScratchStack - 1)] = kSentinel;
-                ++sp;
+                push(kSentinel);
                 cur = 0;
                 continue;
             }
@@ -913,12 +925,14 @@ __global__ void __launch_bounds__(256) k_accumulate(KArgs a) {
     constexpr int C = N == 4 ? 16 : 3;
     Spec<N> c = loadSpec<N>(a.fb.ps.contrib[p]);
     bool finite = c.allFinite();
-    float *acc = a.fb.accum + static_cast<size_t>(p) * C * 2;
+    // planes of width*height floats: sum of channel k at plane 2k, compensation at 2k + 1
+    const size_t plane = static_cast<size_t>(a.fp.width) * a.fp.height;
+    float *acc = a.fb.accum + p;
     const bool reset = a.fp.frameIndex == 1;
     if (!finite) {
         atomicAdd(&a.fb.counters[kStatDropped], 1u);
         if (reset) {
-            for (int k = 0; k < 2 * C; ++k) acc[k] = 0.0f;
+            for (int k = 0; k < 2 * C; ++k) acc[k * plane] = 0.0f;
         }
         return;
     }
@@ -941,13 +955,13 @@ __global__ void __launch_bounds__(256) k_accumulate(KArgs a) {
     }
 #pragma unroll
     for (int k = 0; k < C; ++k) {
-        float sum = reset ? 0.0f : acc[2 * k];
-        float comp = reset ? 0.0f : acc[2 * k + 1];
+        float sum = reset ? 0.0f : acc[(2 * k) * plane];
+        float comp = reset ? 0.0f : acc[(2 * k + 1) * plane];
         float cIn = addend[k] - comp;
         float tmp = sum + cIn;
         comp = (tmp - sum) - cIn;
-        acc[2 * k] = tmp;
-        acc[2 * k + 1] = comp;
+        acc[(2 * k) * plane] = tmp;
+        acc[(2 * k + 1) * plane] = comp;
     }
 }
 
@@ -959,14 +973,14 @@ __global__ void __launch_bounds__(256) k_convert(KArgs a) {
     if (i >= numPaths) return;
     const uint32_t px = i % a.fp.imageW, py = i / a.fp.imageW;
     const uint32_t p = py * a.fp.width + px;
-    constexpr int C = N == 4 ? 16 : 3;
-    const float *acc = a.fb.accum + static_cast<size_t>(p) * C * 2;
+    const size_t plane = static_cast<size_t>(a.fp.width) * a.fp.height;
+    const float *acc = a.fb.accum + p;
     float X, Y, Z;
     if constexpr (N == 4) {
         X = Y = Z = 0.0f;
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
-            float v = acc[2 * k];
+            float v = acc[(2 * k) * plane];
             X += a.sc.cmfX[k] * v;
             Y += a.sc.cmfY[k] * v;
             Z += a.sc.cmfZ[k] * v;
@@ -975,7 +989,7 @@ __global__ void __launch_bounds__(256) k_convert(KArgs a) {
         Y /= a.sc.integralCMF;
         Z /= a.sc.integralCMF;
     } else {
-        float r = acc[0], g = acc[2], b = acc[4];
+        float r = acc[0], g = acc[2 * plane], b = acc[4 * plane];
         X = 0.4124564f * r + 0.3575761f * g + 0.1804375f * b;
         Y = 0.2126729f * r + 0.7151522f * g + 0.0721750f * b;
         Z = 0.0193339f * r + 0.1191920f * g + 0.9503041f * b;

@@ -407,6 +407,10 @@ public:
     void statistics(uint64_t *closest, uint64_t *shadow, uint64_t *dropped, uint64_t trav[6], uint64_t wavefront[2]);
     void resetStats();
     void queueCounts(uint32_t *closest, uint32_t *shadow, uint32_t maxDepth);
+    // The device keeps the Kahan pairs as planes (sum of channel k at plane 2k, compensation at
+    // 2k + 1, width*height floats each); the API exchanges them per pixel (channel, pair).
+    void readAccumulation(float *perPixel);
+    void writeAccumulation(const float *perPixel);
 };
 
 void throwHip(hipError_t e, const char *what);

@@ -9,7 +9,8 @@
 //                  [environment, instances...] (scene.cpp:1249-1281, 1380-1392); prim
 //                  distributions over triangle areas for emitters (scene.cpp:209-219)
 //   scene bounds   union of transformed instance boxes (setup_scene.cu:7-92)
-//   BVHs           binned-SAH BVH2 per geometry group (BLAS) and over instances (TLAS)
+//   BVHs           binned-SAH BVH2 per geometry group (BLAS) and over instances (TLAS),
+//                  collapsed into 4-wide nodes
 //   environment    RGBA16F texels (RGB build) or (u, v, s) halves (spectral build,
 //                  image.cpp:542-565), importance map = 4x box-shrunk luminance times
 //                  sin(theta) (shader_nodes.cpp:1992-2010)
@@ -212,6 +213,95 @@ private:
     }
 };
 
+// Collapses a BVH2 into 4-wide nodes: starting from a node's two children, the inner child
+// with the largest box is replaced by its own two children until four slots are filled or
+// only leaves remain.  Leaf codes and child order carry over.
+class Bvh4Collapser {
+public:
+    explicit Bvh4Collapser(const std::vector<BVHNode> &bin) : bin_(bin) {}
+    std::vector<BVH4Node> nodes;
+    void run() {
+        nodes.clear();
+        if (bin_.empty()) return;
+        emit(0);
+    }
+
+private:
+    struct Slot {
+        Box box;
+        int32_t code;
+    };
+    const std::vector<BVHNode> &bin_;
+    Slot slotOf(uint32_t n, int s) const {
+        const BVHNode &N = bin_[n];
+        Slot r;
+        const float *lo = s == 0 ? N.lo0 : N.lo1, *hi = s == 0 ? N.hi0 : N.hi1;
+        for (int a = 0; a < 3; ++a) {
+            r.box.lo[a] = lo[a];
+            r.box.hi[a] = hi[a];
+        }
+        r.code = N.child[s];
+        return r;
+    }
+    uint32_t emit(uint32_t binNode) {
+        uint32_t idx = static_cast<uint32_t>(nodes.size());
+        nodes.push_back(BVH4Node());
+        Slot slots[4];
+        int n = 0;
+        for (int s = 0; s < 2; ++s) {
+            Slot c = slotOf(binNode, s);
+            if (c.code != 0) slots[n++] = c;
+        }
+        while (n < 4) {
+            int best = -1;
+            float bestArea = -1.0f;
+            for (int k = 0; k < n; ++k)
+                if (slots[k].code > 0 && slots[k].box.area() > bestArea) {
+                    best = k;
+                    bestArea = slots[k].box.area();
+                }
+            if (best < 0) break;
+            uint32_t inner = static_cast<uint32_t>(slots[best].code);
+            Slot a = slotOf(inner, 0), b = slotOf(inner, 1);
+            // keep the order: the replaced slot's children take its place
+            Slot tmp[4];
+            int m = 0;
+            for (int k = 0; k < n; ++k) {
+                if (k == best) {
+                    if (a.code != 0) tmp[m++] = a;
+                    if (b.code != 0) tmp[m++] = b;
+                } else {
+                    tmp[m++] = slots[k];
+                }
+            }
+            for (int k = 0; k < m; ++k) slots[k] = tmp[k];
+            n = m;
+        }
+        int32_t codes[4] = {0, 0, 0, 0};
+        for (int k = 0; k < n; ++k) codes[k] = slots[k].code > 0 ? static_cast<int32_t>(emit(static_cast<uint32_t>(slots[k].code)))
+                                                                  : slots[k].code;
+        BVH4Node &N = nodes[idx];
+        for (int k = 0; k < 4; ++k) {
+            const bool used = k < n;
+            N.loX[k] = used ? slots[k].box.lo[0] : 0.0f;
+            N.loY[k] = used ? slots[k].box.lo[1] : 0.0f;
+            N.loZ[k] = used ? slots[k].box.lo[2] : 0.0f;
+            N.hiX[k] = used ? slots[k].box.hi[0] : 0.0f;
+            N.hiY[k] = used ? slots[k].box.hi[1] : 0.0f;
+            N.hiZ[k] = used ? slots[k].box.hi[2] : 0.0f;
+            N.child[k] = codes[k];
+            N.pad[k] = 0;
+        }
+        return idx;
+    }
+};
+
+std::vector<BVH4Node> collapseToBvh4(const std::vector<BVHNode> &bin) {
+    Bvh4Collapser c(bin);
+    c.run();
+    return std::move(c.nodes);
+}
+
 void composeTransform(const float a[16], const float b[16], float out[16]) {
     // column-major 4x4 product out = a * b
     float r[16];
@@ -380,7 +470,7 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
     // instances + BLAS per distinct surface-node list
     std::vector<DevInstance> insts;
     std::vector<uint32_t> instGeoms;
-    std::vector<BVHNode> blasNodes;
+    std::vector<BVH4Node> blasNodes;
     std::vector<TriRecord> blasTris;
     std::map<std::vector<const SurfaceNode *>, std::pair<uint32_t, uint32_t>> blasCache;
     std::vector<Box> instBoxes;
@@ -452,7 +542,8 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
                 bld.build();
                 uint32_t nodeBase = static_cast<uint32_t>(blasNodes.size());
                 uint32_t triBase = static_cast<uint32_t>(blasTris.size());
-                blasNodes.insert(blasNodes.end(), bld.nodes.begin(), bld.nodes.end());
+                std::vector<BVH4Node> wide = collapseToBvh4(bld.nodes);
+                blasNodes.insert(blasNodes.end(), wide.begin(), wide.end());
                 for (uint32_t o : bld.order) blasTris.push_back(recs[o]);
                 numTris += recs.size();
                 cacheIt = blasCache.emplace(fi.surfaces, std::make_pair(nodeBase, triBase)).first;
@@ -480,12 +571,12 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
     }
 
     // TLAS (one instance per leaf); leaf code carries the instance index
-    std::vector<BVHNode> tlasNodes;
+    std::vector<BVH4Node> tlasNodes;
     if (!instBoxes.empty()) {
         BvhBuilder bld(instBoxes, 1);
         bld.build();
-        tlasNodes = bld.nodes;
-        for (BVHNode &n : tlasNodes) {
+        std::vector<BVHNode> bin = bld.nodes;
+        for (BVHNode &n : bin) {
             for (int s = 0; s < 2; ++s) {
                 if (n.child[s] < 0) {
                     uint32_t first = (~static_cast<uint32_t>(n.child[s])) >> kLeafShift;
@@ -493,6 +584,7 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
                 }
             }
         }
+        tlasNodes = collapseToBvh4(bin);
     }
 
     // environment
@@ -587,8 +679,8 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
     }
 
     // upload
-    g->tlasNodes.upload(tlasNodes.data(), tlasNodes.size() * sizeof(BVHNode));
-    g->blasNodes.upload(blasNodes.data(), blasNodes.size() * sizeof(BVHNode));
+    g->tlasNodes.upload(tlasNodes.data(), tlasNodes.size() * sizeof(BVH4Node));
+    g->blasNodes.upload(blasNodes.data(), blasNodes.size() * sizeof(BVH4Node));
     g->blasTris.upload(blasTris.data(), blasTris.size() * sizeof(TriRecord));
     g->instances.upload(insts.data(), insts.size() * sizeof(DevInstance));
     g->instGeomInsts.upload(instGeoms.data(), instGeoms.size() * sizeof(uint32_t));
@@ -601,8 +693,8 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
     g->mengGrid.upload(mengGrid().data(), mengGrid().size() * sizeof(MengCellDev));
     g->mengPoints.upload(mengPoints().data(), mengPoints().size() * sizeof(float));
 
-    D.tlasNodes = static_cast<const BVHNode *>(g->tlasNodes.ptr);
-    D.blasNodes = static_cast<const BVHNode *>(g->blasNodes.ptr);
+    D.tlasNodes = static_cast<const BVH4Node *>(g->tlasNodes.ptr);
+    D.blasNodes = static_cast<const BVH4Node *>(g->blasNodes.ptr);
     D.blasTris = static_cast<const TriRecord *>(g->blasTris.ptr);
     D.instances = static_cast<const DevInstance *>(g->instances.ptr);
     D.instGeomInsts = static_cast<const uint32_t *>(g->instGeomInsts.ptr);
