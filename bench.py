@@ -132,7 +132,8 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=64)
-    ap.add_argument("--warmup", type=int, default=2)
+    # the first 20 frames after a scene change also tune the frame schedule
+    ap.add_argument("--warmup", type=int, default=24)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--depth", type=int, default=16)
@@ -175,6 +176,7 @@ def main():
     wall = time.perf_counter() - t0
     st = ctx.stats()
     qc, qs = ctx.queue_counts(args.depth)
+    sched_wd, sched_tb, sched_tuned = ctx.schedule()
     ctx.set_kernel_timing(False)
     img = ctx.read_output()
 
@@ -235,6 +237,7 @@ def main():
         "stage_ms": {"generate": st["msGenerate"], "intersect": st["msIntersect"], "shade": st["msShade"],
                      "shadow": st["msShadow"], "tail": st["msTail"], "accumulate": st["msAccumulate"],
                      "convert": st["msConvert"]},
+        "schedule": {"wavefront_depth": sched_wd, "tail_blocks": sched_tb, "autotuned": sched_tuned},
         "rays_by_stage": {"per_depth_closest": wf_c, "per_depth_shadow": wf_s, "tail_closest": tail_c,
                           "tail_shadow": tail_s},
         "traversal_per_ray": {"closest": per_closest, "shadow": per_shadow},

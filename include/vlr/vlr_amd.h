@@ -61,6 +61,13 @@ VLR_API VLRResult vlrContextSetTraversalStatistics(VLRContext context, bool enab
  * kernel.  The image does not depend on it (paths draw and add in the same order). */
 VLR_API VLRResult vlrContextSetWavefrontDepth(VLRContext context, uint32_t depth);
 VLR_API VLRResult vlrContextGetWavefrontDepth(VLRContextConst context, uint32_t* depth);
+/* Schedule autotuning (on by default; setting the wavefront depth turns it off): the first
+ * frames after a change of scene, size or mode time candidate schedules (wavefront depth,
+ * then the tail kernel's grid) and keep the fastest.  Tuning frames synchronise the stream. */
+VLR_API VLRResult vlrContextSetScheduleAutotuning(VLRContext context, bool enable);
+/* The schedule of the last rendered frame and whether tuning has finished. */
+VLR_API VLRResult vlrContextGetSchedule(VLRContextConst context, uint32_t* wavefrontDepth, uint32_t* tailBlocks,
+                                        bool* tuned);
 /* Queue sizes of the last rendered frame: closest[d] extension rays and shadow[d] shadow rays
  * traced at depth d, for d < maxDepth (entries past the frame's path length are 0). */
 VLR_API VLRResult vlrContextGetQueueCounts(VLRContextConst context, uint32_t* closest, uint32_t* shadow,

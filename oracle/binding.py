@@ -72,13 +72,16 @@ HIT_DTYPE = np.dtype([("inst", np.int32), ("geomInst", np.int32), ("prim", np.in
                       ("b1", np.float32), ("b2", np.float32)])
 
 MATERIAL_CODES = {"Matte": 0, "SpecularReflection": 1, "SpecularScattering": 2, "MicrofacetReflection": 3,
-                  "UE4": 4, "DiffuseEmitter": 5, "PointEmitter": 6, "EnvironmentEmitter": 7}
+                  "UE4": 4, "DiffuseEmitter": 5, "PointEmitter": 6, "EnvironmentEmitter": 7,
+                  "MicrofacetScattering": 8}
 # slot order of each type's immediate spectra / floats (names as in the reference's materials.cpp)
 SPECTRUM_SLOTS = {"Matte": ["albedo"], "SpecularReflection": ["coeff", "eta", "k"],
                   "SpecularScattering": ["coeff", "eta ext", "eta int"], "MicrofacetReflection": ["eta", "k"],
                   "UE4": ["base color"], "DiffuseEmitter": ["emittance"], "PointEmitter": ["intensity"],
-                  "EnvironmentEmitter": ["emittance"]}
+                  "EnvironmentEmitter": ["emittance"],
+                  "MicrofacetScattering": ["coeff", "eta ext", "eta int"]}
 FLOAT_SLOTS = {"MicrofacetReflection": ["roughness", "anisotropy", "rotation"],
+               "MicrofacetScattering": ["roughness", "anisotropy", "rotation"],
                "UE4": ["occlusion", "roughness", "metallic"], "DiffuseEmitter": ["scale"],
                "PointEmitter": ["scale"], "EnvironmentEmitter": ["scale"]}
 # defaults of the reference's material constructors (materials.cpp:273, 385-386, 533-534, 684-685,
@@ -92,11 +95,16 @@ SPECTRUM_DEFAULTS = {("Matte", "albedo"): (1, 0.18, 0.18, 0.18),
                      ("SpecularScattering", "eta int"): (1, 1.5, 1.5, 1.5),
                      ("MicrofacetReflection", "eta"): (1, 1.0, 1.0, 1.0),
                      ("MicrofacetReflection", "k"): (1, 0.0, 0.0, 0.0),
+                     ("MicrofacetScattering", "coeff"): (1, 0.8, 0.8, 0.8),
+                     ("MicrofacetScattering", "eta ext"): (1, 1.0, 1.0, 1.0),
+                     ("MicrofacetScattering", "eta int"): (1, 1.5, 1.5, 1.5),
                      ("UE4", "base color"): (1, 0.18, 0.18, 0.18),
                      ("DiffuseEmitter", "emittance"): (1, np.pi, np.pi, np.pi),
                      ("PointEmitter", "intensity"): (1, np.pi, np.pi, np.pi),
                      ("EnvironmentEmitter", "emittance"): (1, np.pi, np.pi, np.pi)}
-FLOAT_DEFAULTS = {("MicrofacetReflection", "roughness"): 0.1, ("MicrofacetReflection", "anisotropy"): 0.0,
+FLOAT_DEFAULTS = {("MicrofacetScattering", "roughness"): 0.1, ("MicrofacetScattering", "anisotropy"): 0.0,
+                  ("MicrofacetScattering", "rotation"): 0.0,
+                  ("MicrofacetReflection", "roughness"): 0.1, ("MicrofacetReflection", "anisotropy"): 0.0,
                   ("MicrofacetReflection", "rotation"): 0.0, ("UE4", "occlusion"): 0.0,
                   ("UE4", "roughness"): 0.1, ("UE4", "metallic"): 0.0, ("DiffuseEmitter", "scale"): 1.0,
                   ("PointEmitter", "scale"): 1.0, ("EnvironmentEmitter", "scale"): 1.0}

@@ -879,6 +879,19 @@ static BSDF<N> setupBSDF(const Material &m, const WLS<N> &wls) {
         b.a2 = 2 * kPi * m.scalars[2];
         break;
     }
+    case ORACLE_MAT_MICROFACET_SCATTERING: {
+        // MicrofacetScatteringSurfaceMaterial_setupBSDF (materials.cu:1516-1536)
+        b.kind = BSDFKind::MicrofacetBSDF;
+        b.c0 = evalTriplet<N>(m.t[0], wls);
+        b.c1 = evalTriplet<N>(m.t[1], wls);
+        b.c2 = evalTriplet<N>(m.t[2], wls);
+        float alpha = pow2(m.scalars[0]);
+        float aspect = std::sqrt(1.0f - 0.9f * m.scalars[1]);
+        b.a0 = std::fmax(0.001f, alpha / aspect);
+        b.a1 = std::fmax(0.001f, alpha * aspect);
+        b.a2 = 2 * kPi * m.scalars[2];
+        break;
+    }
     case ORACLE_MAT_UE4: {
         b.kind = BSDFKind::DiffuseAndSpecular;
         SS<N> base = evalTriplet<N>(m.t[0], wls);

@@ -29,7 +29,8 @@ enum OracleMaterialType {
     ORACLE_MAT_UE4 = 4,                  /* spectra[0]=base color; scalars: occlusion, roughness, metallic */
     ORACLE_MAT_DIFFUSE_EMITTER = 5,      /* spectra[0]=emittance; scalars[0]=scale */
     ORACLE_MAT_POINT_EMITTER = 6,        /* spectra[0]=intensity; scalars[0]=scale */
-    ORACLE_MAT_ENVIRONMENT_EMITTER = 7   /* spectra[0]=emittance or the scene texture; scalars[0]=scale */
+    ORACLE_MAT_ENVIRONMENT_EMITTER = 7,  /* spectra[0]=emittance or the scene texture; scalars[0]=scale */
+    ORACLE_MAT_MICROFACET_SCATTERING = 8 /* spectra: coeff, eta ext, eta int; scalars: roughness, anisotropy, rotation */
 };
 
 /* colour spaces: 0 "Rec709(D65) sRGB Gamma", 1 "Rec709(D65)", 2 "XYZ", 3 "xyY" */

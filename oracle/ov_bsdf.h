@@ -161,6 +161,18 @@ struct FresnelDielectric {
         }
         return ret;
     }
+    float evaluate1(float cosEnter, int i) const {
+        cosEnter = clampv(cosEnter, -1.0f, 1.0f);
+        bool entering = cosEnter > 0.0f;
+        float eEnter = entering ? etaExt[i] : etaInt[i];
+        float eExit = entering ? etaInt[i] : etaExt[i];
+        float sinExit = eEnter / eExit * std::sqrt(std::fmax(0.0f, 1.0f - cosEnter * cosEnter));
+        cosEnter = std::fabs(cosEnter);
+        if (sinExit >= 1.0f)
+            return 1.0f;
+        float cosExit = std::sqrt(std::fmax(0.0f, 1.0f - sinExit * sinExit));
+        return fresnelDielectricEvalF(eEnter, eExit, cosEnter, cosExit);
+    }
 };
 
 // materials.cu:133-228
@@ -220,7 +232,7 @@ struct GGX {
     }
 };
 
-enum class BSDFKind { Null = 0, Matte, SpecularBRDF, SpecularBSDF, MicrofacetBRDF, DiffuseAndSpecular };
+enum class BSDFKind { Null = 0, Matte, SpecularBRDF, SpecularBSDF, MicrofacetBRDF, DiffuseAndSpecular, MicrofacetBSDF };
 
 // One BSDF after material setup at a surface point (the content of the reference's 32-dword
 // parameter block).
@@ -229,7 +241,8 @@ struct BSDF {
     BSDFKind kind = BSDFKind::Null;
     SS<N> c0, c1, c2;   // Matte: albedo | SpecBRDF: coeffR, eta, k | SpecBSDF: coeff, etaExt, etaInt
                         // MicrofacetBRDF: eta, k | DiffSpec: diffuseColor, specularF0Color
-    float a0 = 0, a1 = 0, a2 = 0;   // MicrofacetBRDF: alphaX, alphaY, rotation | DiffSpec: roughness
+                        // MicrofacetBSDF: coeff, etaExt, etaInt
+    float a0 = 0, a1 = 0, a2 = 0;   // Microfacet*: alphaX, alphaY, rotation | DiffSpec: roughness
     bool dispersive = false;
 
     bool matches(uint32_t flags) const {
@@ -241,6 +254,7 @@ struct BSDF {
         case BSDFKind::SpecularBSDF: t = DT::WholeSphere | DT::Delta0D; break;
         case BSDFKind::MicrofacetBRDF: t = DT::Reflection | DT::HighFreq; break;
         case BSDFKind::DiffuseAndSpecular: t = DT::Reflection | DT::LowFreq | DT::HighFreq; break;
+        case BSDFKind::MicrofacetBSDF: t = DT::WholeSphere | DT::HighFreq; break;
         }
         return DT::matches(t, flags);
     }
@@ -253,6 +267,7 @@ struct BSDF {
         case BSDFKind::SpecularBSDF: return c0;
         case BSDFKind::MicrofacetBRDF: return FresnelConductor<N>{c0, c1}.evaluate(1.0f);
         case BSDFKind::DiffuseAndSpecular: return c0 + c1;
+        case BSDFKind::MicrofacetBSDF: return c0;
         default: return SS<N>(0.0f);
         }
     }
@@ -288,6 +303,7 @@ struct BSDF {
         case BSDFKind::SpecularBSDF: return sampleSpecularBSDF(q, uComponent, res, rev);
         case BSDFKind::MicrofacetBRDF: return sampleMicrofacetBRDF(q, uDir, res, rev);
         case BSDFKind::DiffuseAndSpecular: return sampleDiffSpec(q, uComponent, uDir, res, rev);
+        case BSDFKind::MicrofacetBSDF: return sampleRoughDielectric(q, uComponent, uDir, res, rev);
         default:
             res->dirPDF = 0.0f;
             if (rev) rev->dirPDF = 0.0f;
@@ -325,6 +341,7 @@ struct BSDF {
             return fs;
         }
         case BSDFKind::DiffuseAndSpecular: return evaluateDiffSpec(q, dir, revValue);
+        case BSDFKind::MicrofacetBSDF: return evaluateRoughDielectric(q, dir, revValue);
         default:
             if (revValue) *revValue = SS<N>(0.0f);
             return SS<N>(0.0f);
@@ -359,6 +376,7 @@ struct BSDF {
             return common * mPDF;
         }
         case BSDFKind::DiffuseAndSpecular: return evaluatePDFDiffSpec(q, dir, revValue);
+        case BSDFKind::MicrofacetBSDF: return evaluatePDFRoughDielectric(q, dir, revValue);
         default:
             if (revValue) *revValue = 0.0f;
             return 0.0f;
@@ -410,6 +428,149 @@ struct BSDF {
     }
 
 private:
+    // Rough dielectric (materials.cu:1246-1538): GGX half-vector sampling, Fresnel-weighted
+    // choice between reflection and refraction at the selected wavelength, per-wavelength
+    // half vectors when evaluating transmission.
+    float roughTransmission(const SS<N> &eEnter, const SS<N> &eExit, const FresnelDielectric<N> &fresnel,
+                            const GGX &ggx, const V3 &dirV, const V3 &dirL, bool entering, int i) const {
+        V3 m = normalize(-(eEnter[i] * dirV + eExit[i] * dirL) * (entering ? 1.0f : -1.0f));
+        float dotHV = dot(dirV, m), dotHL = dot(dirL, m);
+        float F = fresnel.evaluate1(dotHV, i);
+        float G = ggx.evaluateSmithG1(dirV, m) * ggx.evaluateSmithG1(dirL, m);
+        float D = ggx.evaluate(m);
+        return std::fabs(dotHV * dotHL) * (1 - F) * G * D / pow2(eEnter[i] * dotHV + eExit[i] * dotHL);
+    }
+    SS<N> sampleRoughDielectric(const BSDFQuery<N> &q, float uComponent, const float uDir[2], BSDFResult<N> *res,
+                                RevResult<N> *rev) const {
+        if (rev) rev->dirPDF = 0.0f;
+        bool entering = q.dirLocal.z >= 0.0f;
+        const SS<N> &eEnter = entering ? c1 : c2;
+        const SS<N> &eExit = entering ? c2 : c1;
+        FresnelDielectric<N> fresnel{eEnter, eExit};
+        GGX ggx(a0, a1, a2);
+        V3 dirV = entering ? q.dirLocal : -q.dirLocal;
+        V3 m;
+        float mPDF;
+        float D = ggx.sample(dirV, uDir[0], uDir[1], &m, &mPDF);
+        float dotHV = dot(dirV, m);
+        if (dotHV <= 0 || std::isnan(D)) {
+            res->dirPDF = 0.0f;
+            return SS<N>(0.0f);
+        }
+        SS<N> F = fresnel.evaluate(dotHV);
+        float reflectProb = F.importance(q.wlHint);
+        if (DT::isReflection(q.dirTypeFilter)) reflectProb = 1.0f;
+        if (DT::isTransmission(q.dirTypeFilter)) reflectProb = 0.0f;
+        if (uComponent < reflectProb) {
+            V3 dirL = 2 * dotHV * m - dirV;
+            res->dirLocal = entering ? dirL : -dirL;
+            if (dirL.z * dirV.z <= 0) {
+                res->dirPDF = 0.0f;
+                return SS<N>(0.0f);
+            }
+            float common = reflectProb / (4 * dotHV);
+            res->dirPDF = common * mPDF;
+            res->sampledType = DT::Reflection | DT::HighFreq;
+            float G = ggx.evaluateSmithG1(dirV, m) * ggx.evaluateSmithG1(dirL, m);
+            SS<N> ret = c0 * F * D * G / (4 * dirV.z * dirL.z);
+            if (rev) {
+                rev->value = ret;
+                rev->dirPDF = common * ggx.evaluatePDF(dirL, m);
+            }
+            return ret;
+        }
+        const int w = static_cast<int>(q.wlHint);
+        float recRelIOR = eEnter[w] / eExit[w];
+        float innerRoot = 1 + recRelIOR * recRelIOR * (dotHV * dotHV - 1);
+        if (innerRoot < 0) {
+            res->dirPDF = 0.0f;
+            return SS<N>(0.0f);
+        }
+        V3 dirL = (recRelIOR * dotHV - std::sqrt(innerRoot)) * m - recRelIOR * dirV;
+        res->dirLocal = entering ? dirL : -dirL;
+        if (dirL.z * dirV.z >= 0) {
+            res->dirPDF = 0.0f;
+            return SS<N>(0.0f);
+        }
+        float dotHL = dot(dirL, m);
+        float common = (1 - reflectProb) / pow2(eEnter[w] * dotHV + eExit[w] * dotHL);
+        res->dirPDF = common * mPDF * pow2(eExit[w]) * std::fabs(dotHL);
+        res->sampledType = DT::Transmission | DT::HighFreq;
+        SS<N> ret(0.0f);
+        for (int i = 0; i < N; ++i) ret[i] = roughTransmission(eEnter, eExit, fresnel, ggx, dirV, dirL, entering, i);
+        ret /= std::fabs(dirV.z * dirL.z);
+        ret *= c0;
+        ret *= q.transportMode == TransportMode::Radiance ? eEnter * eEnter : eExit * eExit;
+        if (rev) {
+            rev->value = ret;
+            rev->dirPDF = common * ggx.evaluatePDF(dirL, m) * pow2(eEnter[w]) * std::fabs(dotHV);
+        }
+        return ret;
+    }
+    SS<N> evaluateRoughDielectric(const BSDFQuery<N> &q, const V3 &dir, SS<N> *revValue) const {
+        bool entering = q.dirLocal.z >= 0.0f;
+        const SS<N> &eEnter = entering ? c1 : c2;
+        const SS<N> &eExit = entering ? c2 : c1;
+        FresnelDielectric<N> fresnel{eEnter, eExit};
+        GGX ggx(a0, a1, a2);
+        V3 dirV = entering ? q.dirLocal : -q.dirLocal;
+        V3 dirL = entering ? dir : -dir;
+        float dotNVdotNL = dirL.z * dirV.z;
+        if (dotNVdotNL > 0 && DT::matches(DT::Reflection | DT::AllFreq, q.dirTypeFilter)) {
+            V3 m = halfVector(dirV, dirL);
+            float dotHV = dot(dirV, m);
+            float D = ggx.evaluate(m);
+            SS<N> F = fresnel.evaluate(dotHV);
+            float G = ggx.evaluateSmithG1(dirV, m) * ggx.evaluateSmithG1(dirL, m);
+            SS<N> ret = c0 * F * D * G / (4 * dotNVdotNL);
+            if (revValue) *revValue = ret;
+            return ret;
+        }
+        if (dotNVdotNL < 0 && DT::matches(DT::Transmission | DT::AllFreq, q.dirTypeFilter)) {
+            SS<N> ret(0.0f);
+            for (int i = 0; i < N; ++i) ret[i] = roughTransmission(eEnter, eExit, fresnel, ggx, dirV, dirL, entering, i);
+            ret /= std::fabs(dotNVdotNL);
+            ret *= c0;
+            ret *= q.transportMode == TransportMode::Radiance ? eEnter * eEnter : eExit * eExit;
+            if (revValue) *revValue = ret;
+            return ret;
+        }
+        if (revValue) *revValue = SS<N>(0.0f);
+        return SS<N>(0.0f);
+    }
+    float evaluatePDFRoughDielectric(const BSDFQuery<N> &q, const V3 &dir, float *revValue) const {
+        if (revValue) *revValue = 0.0f;
+        bool entering = q.dirLocal.z >= 0.0f;
+        const SS<N> &eEnter = entering ? c1 : c2;
+        const SS<N> &eExit = entering ? c2 : c1;
+        FresnelDielectric<N> fresnel{eEnter, eExit};
+        GGX ggx(a0, a1, a2);
+        V3 dirV = entering ? q.dirLocal : -q.dirLocal;
+        V3 dirL = entering ? dir : -dir;
+        float dotNVdotNL = dirL.z * dirV.z;
+        if (dotNVdotNL == 0)
+            return 0.0f;
+        const int w = static_cast<int>(q.wlHint);
+        V3 m = dotNVdotNL > 0 ? halfVector(dirV, dirL) : normalize(-(eEnter[w] * dirV + eExit[w] * dirL));
+        float dotHV = dot(dirV, m);
+        if (dotHV <= 0)
+            return 0.0f;
+        float mPDF = ggx.evaluatePDF(dirV, m);
+        SS<N> F = fresnel.evaluate(dotHV);
+        float reflectProb = F.importance(q.wlHint);
+        if (DT::isReflection(q.dirTypeFilter)) reflectProb = 1.0f;
+        if (DT::isTransmission(q.dirTypeFilter)) reflectProb = 0.0f;
+        if (dotNVdotNL > 0) {
+            float common = reflectProb / (4 * dotHV);
+            if (revValue) *revValue = common * ggx.evaluatePDF(dirL, m);
+            return common * mPDF;
+        }
+        float dotHL = dot(dirL, m);
+        float common = (1 - reflectProb) / pow2(eEnter[w] * dotHV + eExit[w] * dotHL);
+        if (revValue) *revValue = common * ggx.evaluatePDF(dirL, m) * pow2(eEnter[w]) * std::fabs(dotHV);
+        return common * mPDF * pow2(eExit[w]) * std::fabs(dotHL);
+    }
+
     // materials.cu:817-910
     SS<N> sampleSpecularBSDF(const BSDFQuery<N> &q, float uComponent, BSDFResult<N> *res, RevResult<N> *rev) const {
         if (rev) rev->dirPDF = 0.0f;

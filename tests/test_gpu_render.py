@@ -115,3 +115,21 @@ def test_frame_schedule_does_not_change_the_image(torch_gpu, scene, spectral):
         assert np.array_equal(acc.view(np.uint32), base[3].view(np.uint32)), wd
         assert np.array_equal(img.view(np.uint32), base[0].view(np.uint32)), wd
         assert st["closestRays"] == base[1]["closestRays"] and st["shadowRays"] == base[1]["shadowRays"], wd
+
+
+def test_autotuned_schedule_matches_fixed_schedule(torch_gpu):
+    """The schedule autotuner times candidate schedules on real frames; the frames it renders
+    while tuning are the same bits as a fixed schedule's."""
+    desc, W, H, frames = scenes.cornell_box(), 40, 40, 24
+    with api.Context(spectral=False, max_path_length=8) as ctx:
+        bs = api.build_scene(ctx, desc)
+        ctx.bind_output(W, H)
+        for f in range(frames):
+            ctx.render(bs.camera, f == 0, frames)
+        tuned_img, tuned_rng = ctx.read_output(), ctx.read_rng()
+        wd, tb, tuned = ctx.schedule()
+    print("tuned schedule: wavefront depth", wd, "tail blocks", tb, "finished", tuned)
+    assert tuned and 1 <= wd <= 6 and tb in (512, 1024, 2048, 4096)
+    imgs, _, rng = render_gpu(desc, W, H, frames, 8, False, wavefront_depth=64)
+    assert np.array_equal(rng, tuned_rng)
+    assert np.array_equal(imgs[-1].view(np.uint32), tuned_img.view(np.uint32))

@@ -183,6 +183,8 @@ PROTOTYPES: Dict[str, list] = {
     "vlrContextGetQueueCounts": [H, P, P, U32],
     "vlrContextSetWavefrontDepth": [H, U32],
     "vlrContextGetWavefrontDepth": [H, C.POINTER(U32)],
+    "vlrContextSetScheduleAutotuning": [H, B],
+    "vlrContextGetSchedule": [H, C.POINTER(U32), C.POINTER(U32), C.POINTER(B)],
     "vlrContextReadRNGStates": [H, P],
     "vlrContextWriteRNGStates": [H, P],
     "vlrContextGetAccumulationChannels": [H, C.POINTER(U32)],
@@ -328,6 +330,14 @@ class Context:
 
     def set_wavefront_depth(self, depth: int):
         call("vlrContextSetWavefrontDepth", self.h, depth)
+
+    def set_autotune(self, on: bool):
+        call("vlrContextSetScheduleAutotuning", self.h, on)
+
+    def schedule(self):
+        wd, tb, tuned = U32(), U32(), B()
+        call("vlrContextGetSchedule", self.h, C.byref(wd), C.byref(tb), C.byref(tuned))
+        return wd.value, tb.value, bool(tuned.value)
 
     def set_traversal_stats(self, on: bool):
         call("vlrContextSetTraversalStatistics", self.h, on)

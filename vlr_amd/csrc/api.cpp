@@ -1085,6 +1085,7 @@ VLR_API VLRResult vlrContextSetWavefrontDepth(VLRContext context, uint32_t depth
     VLRA_TRY
     CHECK(context, IS(context, Context));
     CTX(context)->wavefrontDepth = depth;
+    CTX(context)->autotune = false;
     return VLRResult_NoError;
     VLRA_CATCH
 }
@@ -1093,6 +1094,27 @@ VLR_API VLRResult vlrContextGetWavefrontDepth(VLRContextConst context, uint32_t 
     CHECK(context, IS(context, Context));
     if (!depth) return VLRResult_InvalidArgument;
     *depth = CTX(context)->wavefrontDepth;
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+VLR_API VLRResult vlrContextSetScheduleAutotuning(VLRContext context, bool enable) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    Context *c = CTX(context);
+    c->autotune = enable;
+    if (enable) c->tuner = ScheduleTuner();
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+VLR_API VLRResult vlrContextGetSchedule(VLRContextConst context, uint32_t *wavefrontDepth, uint32_t *tailBlocks,
+                                        bool *tuned) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    if (!wavefrontDepth || !tailBlocks || !tuned) return VLRResult_InvalidArgument;
+    const Context *c = CTX(context);
+    *wavefrontDepth = c->lastFrame.wavefrontDepth;
+    *tailBlocks = c->lastFrame.tailBlocks;
+    *tuned = c->autotune && c->tuner.done();
     return VLRResult_NoError;
     VLRA_CATCH
 }

@@ -31,6 +31,8 @@ void Context::ensureDevice() {
 Context::~Context() {
     if (device >= 0) (void)hipDeviceSynchronize();
     for (hipEvent_t e : eventPool) (void)hipEventDestroy(e);
+    for (hipEvent_t e : tuneEvents)
+        if (e) (void)hipEventDestroy(e);
     for (Object *o : std::set<Object *>(objects)) delete o;
 }
 
@@ -146,6 +148,30 @@ void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCo
         fp.maxPathLength = std::min<uint32_t>(maxPathLength, kMaxDepthCounters - 1);
         fp.traversalStats = traversalStats ? 1 : 0;
         fp.wavefrontDepth = wavefrontDepth;
+        fp.tailBlocks = tailBlocks;
+        // autotune the schedule per (scene version, size, mode, path length)
+        const uint64_t tuneKey = (version * 1000003ull) ^ (uint64_t(width) << 40) ^ (uint64_t(height) << 20) ^
+                                 (uint64_t(fp.imageW) << 8) ^ (spectralActive ? 1u : 0u) ^
+                                 (uint64_t(fp.maxPathLength) << 2);
+        bool tuning = false;
+        if (autotune && !traversalStats) {
+            if (tuner.key != tuneKey || tuner.cands.empty()) tuner.start(tuneKey);
+            if (!tuner.done()) {
+                tuning = true;
+                fp.wavefrontDepth = tuner.current().wavefrontDepth;
+                fp.tailBlocks = tuner.current().tailBlocks;
+            } else {
+                fp.wavefrontDepth = tuner.best().wavefrontDepth;
+                fp.tailBlocks = tuner.best().tailBlocks;
+            }
+        }
+        if (tuning) {
+            if (!tuneEvents[0]) {
+                VLRA_HIP(hipEventCreate(&tuneEvents[0]));
+                VLRA_HIP(hipEventCreate(&tuneEvents[1]));
+            }
+            VLRA_HIP(hipEventRecord(tuneEvents[0], stream));
+        }
         FrameBuffers fb;
         fb.ps.rng = static_cast<uint64_t *>(rng.ptr);
         fb.ps.alpha = static_cast<float4 *>(alpha.ptr);
@@ -169,6 +195,13 @@ void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCo
             t.collect(stageMs, launches);
         } else {
             VLRA_HIP(launchFrame(gpu.dev, fp, fb, spectralActive, stream, nullptr));
+        }
+        if (tuning) {
+            VLRA_HIP(hipEventRecord(tuneEvents[1], stream));
+            VLRA_HIP(hipEventSynchronize(tuneEvents[1]));
+            float ms = 0.0f;
+            VLRA_HIP(hipEventElapsedTime(&ms, tuneEvents[0], tuneEvents[1]));
+            tuner.report(ms);
         }
         lastFrame = fp;
         ++framesRendered;

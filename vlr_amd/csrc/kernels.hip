@@ -32,7 +32,6 @@ constexpr int kTraceBlock = 128;
 constexpr int kLdsStack = 24;
 constexpr int kScratchStack = 64;
 constexpr int32_t kSentinel = INT32_MIN;
-constexpr uint32_t kTailBlocks = 2048;   // persistent tail grid: 8 blocks of 128 per CU
 
 // ---------------------------------------------------------------------------------------------
 // ray set-up and primitive tests
@@ -1140,7 +1139,7 @@ static hipError_t launchFrameT(const DevScene &sc, const FrameParams &fp, const 
     }
     if (wavefrontDepths < numDepths) {
         a.depth = wavefrontDepths;
-        const uint32_t blocks = std::min(blocksFor(n, kTraceBlock), kTailBlocks);
+        const uint32_t blocks = std::max(1u, std::min(blocksFor(n, kTraceBlock), fp.tailBlocks));
         t0(STAGE_TAIL);
         if (fp.traversalStats)
             hipLaunchKernelGGL((k_tail<N, true>), dim3(blocks), dim3(kTraceBlock), 0, s, a);

@@ -1,5 +1,7 @@
 // Host-side launchers of the light-transport kernels (implemented in kernels.hip).
 #pragma once
+#include <algorithm>
+#include <vector>
 #include <hip/hip_runtime.h>
 #include "dev_types.h"
 
@@ -17,6 +19,56 @@ struct FrameParams {
     uint32_t maxPathLength;
     uint32_t traversalStats;       // count nodes / triangles / instances per traversal
     uint32_t wavefrontDepth;       // depths run as separate kernels; the rest in the tail kernel
+    uint32_t tailBlocks;           // persistent grid of the tail kernel (blocks of 128)
+};
+
+constexpr uint32_t kDefaultTailBlocks = 2048;   // 8 blocks of 128 per CU
+
+// Frame-schedule autotuner.  The image does not depend on the schedule, so candidates are
+// timed on real frames: first the wavefront depth (tail grid fixed), then the tail grid at
+// the best depth; each candidate gets kFramesPerCandidate frames and keeps its fastest.
+struct ScheduleTuner {
+    static constexpr uint32_t kFramesPerCandidate = 2;
+    struct Cand {
+        uint32_t wavefrontDepth, tailBlocks;
+        float bestMs;
+    };
+    std::vector<Cand> cands;
+    uint32_t phase = 0;           // 0: depths, 1: tail grids, 2: done
+    uint32_t index = 0, frames = 0;
+    uint64_t key = 0;
+    bool done() const { return phase >= 2; }
+    void start(uint64_t k) {
+        key = k;
+        phase = 0;
+        index = frames = 0;
+        cands.clear();
+        for (uint32_t d : {1u, 2u, 3u, 4u, 5u, 6u}) cands.push_back({d, kDefaultTailBlocks, 1e30f});
+    }
+    const Cand &current() const { return cands[index]; }
+    Cand best() const {
+        Cand b = cands[0];
+        for (const Cand &c : cands)
+            if (c.bestMs < b.bestMs) b = c;
+        return b;
+    }
+    void report(float ms) {
+        Cand &c = cands[index];
+        c.bestMs = std::min(c.bestMs, ms);
+        if (++frames < kFramesPerCandidate) return;
+        frames = 0;
+        if (++index < cands.size()) return;
+        Cand b = best();
+        index = 0;
+        if (phase == 0) {
+            phase = 1;
+            cands.clear();
+            for (uint32_t t : {512u, 1024u, 2048u, 4096u}) cands.push_back({b.wavefrontDepth, t, 1e30f});
+        } else {
+            phase = 2;
+            cands.assign(1, b);
+        }
+    }
 };
 
 struct FrameBuffers {

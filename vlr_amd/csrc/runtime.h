@@ -375,6 +375,10 @@ public:
     bool kernelTiming = false;
     bool traversalStats = false;
     uint32_t wavefrontDepth = kDefaultWavefrontDepth;
+    uint32_t tailBlocks = kDefaultTailBlocks;
+    bool autotune = true;            // cleared when the caller sets the schedule explicitly
+    ScheduleTuner tuner;
+    hipEvent_t tuneEvents[2] = {nullptr, nullptr};
     std::vector<hipEvent_t> eventPool;   // reused by kernel timing, 2 per timed launch
     uint32_t width = 0, height = 0;
     uint32_t numAccumFrames = 0;

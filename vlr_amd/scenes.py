@@ -28,6 +28,7 @@ MATTE = "Matte"
 SPECULAR_REFLECTION = "SpecularReflection"
 SPECULAR_SCATTERING = "SpecularScattering"
 MICROFACET_REFLECTION = "MicrofacetReflection"
+MICROFACET_SCATTERING = "MicrofacetScattering"
 UE4 = "UE4"
 DIFFUSE_EMITTER = "DiffuseEmitter"
 POINT_EMITTER = "PointEmitter"
