@@ -101,6 +101,7 @@ typedef struct VLRBSDFQuery {
     float uComponent, uDir[2];
     float evalDir[3];
     int32_t singleIsSelected;
+    int32_t transportMode;       /* 0 radiance, 1 importance (adjoint) */
 } VLRBSDFQuery;
 typedef struct VLRBSDFResult {
     float sampledDir[3];
@@ -109,11 +110,14 @@ typedef struct VLRBSDFResult {
     float sampledValue[4];
     float evalValue[4];
     float evalPDF;
-    float revValue[4];
+    float revValue[4];           /* reverse (bidirectional) value and pdf of evaluate / evaluatePDF */
     float revPDF;
+    float sampledRevValue[4];    /* reverse value and pdf of the sample */
+    float sampledRevPDF;
 } VLRBSDFResult;
 /* Evaluates the material's BSDF (immediate parameters) on device-resident queries:
- * sample / evaluate / evaluatePDF in the shading frame, radiance transport. */
+ * sample / evaluate / evaluatePDF in the shading frame, radiance or importance transport,
+ * with the reverse quantities bidirectional methods use. */
 VLR_API VLRResult vlrContextEvaluateBSDFs(VLRContext context, hipStream_t stream, VLRSurfaceMaterialConst material,
                                           bool spectral, const VLRBSDFQuery* queries, uint32_t numQueries,
                                           VLRBSDFResult* results);

@@ -310,13 +310,14 @@ def bsdf_queries(material, spectral: bool, queries: np.ndarray) -> np.ndarray:
 def query_dtype():
     return np.dtype([("dirLocal", np.float32, 3), ("geomNormalLocal", np.float32, 3), ("wlOffset", np.float32),
                      ("wlSelect", np.float32), ("uComponent", np.float32), ("uDir", np.float32, 2),
-                     ("evalDir", np.float32, 3), ("singleIsSelected", np.int32)])
+                     ("evalDir", np.float32, 3), ("singleIsSelected", np.int32), ("transportMode", np.int32)])
 
 
 def result_dtype():
     return np.dtype([("sampledDir", np.float32, 3), ("sampledPDF", np.float32), ("sampledType", np.uint32),
                      ("sampledValue", np.float32, 4), ("evalValue", np.float32, 4), ("evalPDF", np.float32),
-                     ("revValue", np.float32, 4), ("revPDF", np.float32)])
+                     ("revValue", np.float32, 4), ("revPDF", np.float32), ("sampledRevValue", np.float32, 4),
+                     ("sampledRevPDF", np.float32)])
 
 
 def pcg32_stream(state: int, n: int):

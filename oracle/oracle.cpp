@@ -469,6 +469,7 @@ static Scene *buildScene(const OracleScene *d, bool spectral) {
         switch (m.type) {
         case ORACLE_MAT_SPECULAR_REFLECTION:
         case ORACLE_MAT_SPECULAR_SCATTERING:
+        case ORACLE_MAT_MICROFACET_SCATTERING:
             types[1] = types[2] = SpectrumType::IndexOfRefraction;
             break;
         case ORACLE_MAT_MICROFACET_REFLECTION:
@@ -1244,6 +1245,777 @@ static SS<N> tracePath(const Scene &s, PCG32 &rng, uint32_t px, uint32_t py, uin
     return contribution;
 }
 
+// ==========================================================================================
+// LVC-BPT (lvc_bpt.cu:144-971): light subpaths are traced first and every vertex is stored in
+// a light-vertex cache; each eye vertex, including the lens vertex, is connected to one cache
+// vertex chosen uniformly, with the partial MIS bookkeeping of the reference (Davidovic et
+// al. 2014).  Lens connections are splatted to the pixel they project to.
+template <int N>
+struct LightVertex {
+    int32_t kind = 0;                 // 0 triangle, 1 point, 2 environment
+    uint32_t inst = 0, geomInst = 0, prim = 0;
+    float u = 0, v = 0;
+    float probDensity = 0, prevProbDensity = 0;
+    float secondPrevPartialDenomMisWeight = 0, secondPrevProbRatioToFirst = 0;
+    float backwardConversionFactor = 0;
+    SS<N> flux;
+    V3 dirInLocal;
+    bool deltaSampled = false, prevDeltaSampled = false, wlSelected = false;
+    uint32_t pathLength = 0;
+};
+
+template <int N>
+static LightVertex<N> makeLightVertex(const SurfPt &sp, const SS<N> &flux, const V3 &dirInLocal, float bcf,
+                                      float probDensity, float prevProbDensity, float spdmw, float sprtf,
+                                      bool delta, bool prevDelta, bool wlSelected, uint32_t pathLength) {
+    LightVertex<N> lv;
+    lv.kind = sp.atInfinity ? 2 : (sp.isPoint ? 1 : 0);
+    lv.inst = sp.instIndex;
+    lv.geomInst = sp.geomInstIndex;
+    lv.prim = sp.primIndex;
+    lv.u = sp.u;
+    lv.v = sp.v;
+    lv.probDensity = probDensity;
+    lv.prevProbDensity = prevProbDensity;
+    lv.secondPrevPartialDenomMisWeight = spdmw;
+    lv.secondPrevProbRatioToFirst = sprtf;
+    lv.backwardConversionFactor = bcf;
+    lv.flux = flux;
+    lv.dirInLocal = dirInLocal;
+    lv.deltaSampled = delta;
+    lv.prevDeltaSampled = prevDelta;
+    lv.wlSelected = wlSelected;
+    lv.pathLength = pathLength;
+    return lv;
+}
+
+// decodeHitPointForTriangle (triangle.cu:55-109), decodeHitPointForPoint (point.cu:6-35),
+// decodeHitPointForInfiniteSphere (infinite_sphere.cu:17-56)
+template <int N>
+static int decodeLightVertex(const Scene &s, const LightVertex<N> &lv, SurfPt *sp) {
+    if (lv.kind == 2) {
+        float posPhi = lv.u, theta = lv.v;
+        float phi = posPhi + s.envRotation;
+        phi = phi - std::floor(phi / (2 * kPi)) * 2 * kPi;
+        V3 direction = V3::fromPolarYUp(posPhi, theta);
+        float sinPhi = std::sin(posPhi), cosPhi = std::cos(posPhi);
+        V3 tc0 = normalize(V3(-cosPhi, 0.0f, -sinPhi));
+        V3 gn = -direction;
+        sp->frame.x = tc0;
+        sp->frame.z = gn;
+        sp->frame.y = cross(sp->frame.z, sp->frame.x);
+        sp->position = direction;
+        sp->geometricNormal = gn;
+        sp->isPoint = false;
+        sp->atInfinity = true;
+        sp->u = posPhi;
+        sp->v = theta;
+        sp->texU = phi / (2 * kPi);
+        sp->texV = theta / kPi;
+        return s.envMaterial;
+    }
+    const Instance &in = s.instances[lv.inst];
+    const GeomInst &g = s.geomInsts[lv.geomInst];
+    sp->instIndex = lv.inst;
+    sp->geomInstIndex = lv.geomInst;
+    sp->primIndex = lv.prim;
+    sp->atInfinity = false;
+    if (lv.kind == 1) {
+        uint32_t pi = g.idx[lv.prim];
+        sp->position = in.xfm.point(g.pos(pi));
+        V3 sn = normalize(in.xfm.normal(g.nrm(pi)));
+        V3 tc0 = normalize(in.xfm.vector(g.tc0(pi)));
+        sp->frame = Frame(tc0, sn);
+        sp->isPoint = true;
+        sp->geometricNormal = sn;
+        sp->u = lv.u;
+        sp->v = lv.v;
+        sp->texU = sp->texV = 0;
+        return g.material;
+    }
+    uint32_t i0 = g.idx[3 * lv.prim], i1 = g.idx[3 * lv.prim + 1], i2 = g.idx[3 * lv.prim + 2];
+    V3 e1 = in.xfm.vector(g.pos(i1) - g.pos(i0));
+    V3 e2 = in.xfm.vector(g.pos(i2) - g.pos(i0));
+    V3 gn = cross(e1, e2);
+    float area = gn.length() / 2;
+    gn /= 2 * area;
+    float b0 = lv.u, b1 = lv.v, b2 = 1.0f - lv.u - lv.v;
+    V3 position = b0 * g.pos(i0) + b1 * g.pos(i1) + b2 * g.pos(i2);
+    V3 sn = b0 * g.nrm(i0) + b1 * g.nrm(i1) + b2 * g.nrm(i2);
+    V3 tc0 = b0 * g.tc0(i0) + b1 * g.tc0(i1) + b2 * g.tc0(i2);
+    sp->texU = b0 * g.verts[i0].texCoord[0] + b1 * g.verts[i1].texCoord[0] + b2 * g.verts[i2].texCoord[0];
+    sp->texV = b0 * g.verts[i0].texCoord[1] + b1 * g.verts[i1].texCoord[1] + b2 * g.verts[i2].texCoord[1];
+    position = in.xfm.point(position);
+    sn = normalize(in.xfm.normal(sn));
+    tc0 = in.xfm.vector(tc0);
+    if (!sn.allFinite() || !tc0.allFinite()) {
+        V3 bt;
+        sn = gn;
+        sn.makeCoordinateSystem(&tc0, &bt);
+    }
+    float dotNT = dot(sn, tc0);
+    tc0 = normalize(tc0 - dotNT * sn);
+    sp->position = position;
+    sp->frame = Frame(tc0, sn);
+    sp->isPoint = false;
+    sp->geometricNormal = gn;
+    sp->u = b0;
+    sp->v = b1;
+    return g.material;
+}
+
+// computeRRProbability (lvc_bpt.cu:137-141)
+template <int N>
+static float rrProbability(const SS<N> &fs, const V3 &dirLocal, float dirDensity, const V3 &gnLocal, uint32_t sel) {
+    SS<N> localThroughput = fs * absDot(dirLocal, gnLocal) / dirDensity;
+    return std::fmin(localThroughput.importance(sel), 1.0f);
+}
+
+// BSDF of a cached light vertex: the material's BSDF, or the emitter's EDF used as a BSDF at
+// the light source itself (the fromEDF constructor of renderer_common.h:192-216), with the
+// importance-transport shading-normal correction of renderer_common.h:224-264.
+template <int N>
+struct VertexBSDF {
+    bool fromEDF = false;
+    BSDF<N> bsdf;
+    EDF<N> edf;
+    bool hasNonDelta() const { return fromEDF ? edf.matches(DT::WholeSphere | DT::NonDelta) : bsdf.hasNonDelta(); }
+    SS<N> evaluate(const BSDFQuery<N> &q, const V3 &dir, SS<N> *rev) const {
+        if (!fromEDF) return bsdf.evaluate(q, dir, rev);
+        float sn = std::fabs(q.dirLocal.z / dot(q.dirLocal, q.geometricNormalLocal));
+        *rev = SS<N>(0.0f);
+        if (!std::isfinite(sn)) return SS<N>(0.0f);
+        return edf.evaluate(dir) * sn;
+    }
+    float evaluatePDF(const BSDFQuery<N> &q, const V3 &dir, float *rev) const {
+        if (!fromEDF) return bsdf.evaluatePDF(q, dir, rev);
+        *rev = 0.0f;
+        if (!edf.matches(q.dirTypeFilter)) return 0.0f;
+        return edf.evaluatePDF(dir);
+    }
+};
+
+// PerspectiveCameraIDF (cameras.cu:124-211)
+struct LensIDF {
+    float sensitivity, xOnLens, yOnLens, opWidth, opHeight, imgPlaneDistance, objPlaneDistance, imgPlaneArea;
+    bool project(const V3 &d, float *u0, float *u1) const {
+        if (d.z <= 0.0f) return false;
+        float objPlaneZ = objPlaneDistance / d.z;
+        V3 pFocus = d * objPlaneZ + V3(xOnLens, yOnLens, 0.0f);
+        *u0 = -pFocus.x / opWidth + 0.5f;
+        *u1 = -pFocus.y / opHeight + 0.5f;
+        return !(*u0 < 0.0f || *u0 >= 1.0f || *u1 < 0.0f || *u1 >= 1.0f);
+    }
+    float directionalImportance(const V3 &d) const {
+        float a, b;
+        return project(d, &a, &b) ? 1.0f : 0.0f;
+    }
+    float pdf(const V3 &d) const {
+        float a, b;
+        if (!project(d, &a, &b)) return 0.0f;
+        return pow2(imgPlaneDistance) / (pow3(d.z) * imgPlaneArea);
+    }
+};
+
+struct BPTSettings {
+    uint32_t numLightPaths;
+    uint32_t maxPathLength;
+};
+
+template <int N>
+static void traceLightPath(const Scene &s, const WLS<N> &wlsCommon, const BPTSettings &bs, PCG32 &rng,
+                           std::vector<LightVertex<N>> *cache, OracleRenderStats &st) {
+    if (s.lightInstDist.integral <= 0.0f) return;
+    float uLight = rng.uniform();
+    float uP0 = rng.uniform(), uP1 = rng.uniform();
+    LightSample ls;
+    float lightProb;
+    bool valid;
+    sampleLight(s, uLight, uP0, uP1, &ls, &lightProb, &valid);
+    const Material &lmat = s.materials[ls.material];
+    EDF<N> edf = setupEDF<N>(s, lmat, ls.sp, wlsCommon);
+    float probDensity0 = bs.numLightPaths * lightProb * ls.areaPDF;
+    SS<N> Le0 = edf.evaluateEmittance();
+    SS<N> alpha = Le0 / probDensity0;
+    cache->push_back(makeLightVertex<N>(ls.sp, alpha, V3(0, 0, 1), 0.0f, probDensity0, 1.0f, 0.0f, 1.0f,
+                                        DT::isDelta(ls.posType), false, false, 0));
+    float uC = rng.uniform(), uD0 = rng.uniform(), uD1 = rng.uniform();
+    (void)uC;
+    V3 dirLocal;
+    float dirPDF;
+    uint32_t sampledType;
+    SS<N> Le1 = edf.sample(uD0, uD1, &dirLocal, &dirPDF, &sampledType);
+    V3 rayOrg = offsetRayOrigin(ls.sp.position, ls.sp.geometricNormal);
+    if (ls.sp.atInfinity) {
+        rayOrg = s.center + 1.1f * s.worldRadius * ls.sp.position + ls.sp.frame.x * dirLocal.x +
+                 ls.sp.frame.y * dirLocal.y;
+        dirLocal.x = 0;
+        dirLocal.y = 0;
+    }
+    V3 rayDir = ls.sp.frame.fromLocal(dirLocal);
+    float cosTerm = ls.sp.calcCosTerm(rayDir);
+    alpha *= Le1 * (cosTerm / dirPDF);
+
+    // payload state
+    float roDirPDF = dirPDF, roCosTerm = cosTerm, roPrevRevAreaPDF = 1;
+    bool roSecondPrevDelta = false, roPrevDelta = DT::isDelta(ls.posType);
+    uint32_t roSampledType = sampledType;
+    float probDensity = probDensity0, prevProbDensity = 1.0f;
+    float secondPrevPartialDenomMisWeight = 0, secondPrevProbRatioToFirst = 1;
+    bool single = false;
+    bool originIsPoint = ls.sp.isPoint, originIsInfinity = ls.sp.atInfinity;
+    uint32_t pathLength = 0;
+    while (true) {
+        bool terminate = true;
+        ++pathLength;
+        HitRec hit;
+        ++st.closestRays;
+        bool found = traceRay(s, rayOrg, rayDir, 0.0f, FLT_MAX, &hit, false, true);
+        V3 nextOrg, nextDir;
+        float woDirPDF = 0, woCosTerm = 0, woRevAreaPDF = 0;
+        uint32_t woSampledType = 0;
+        if (found) {
+            // closest-hit program for light paths (lvc_bpt.cu:246-334)
+            WLS<N> wls = wlsCommon;
+            if (single) wls.single = true;
+            SurfPt sp;
+            float hypAreaPDF;
+            calcSurfacePoint(s, hit, &sp, &hypAreaPDF);
+            const GeomInst &g = s.geomInsts[hit.geomInst];
+            BSDF<N> bsdf = setupBSDF<N>(s.materials[g.material], wls);
+            V3 dirInLocal = sp.frame.toLocal(-rayDir);
+            V3 gnLocal = sp.frame.toLocal(sp.geometricNormal);
+            BSDFQuery<N> q{dirInLocal, gnLocal, DT::All, TransportMode::Importance, wls.sel};
+            bool secondLastValid = (!roPrevDelta && !roSecondPrevDelta) && pathLength > 2;
+            float probRatio = roPrevRevAreaPDF / prevProbDensity;
+            secondPrevPartialDenomMisWeight = pow2(probRatio) * (secondPrevPartialDenomMisWeight + (secondLastValid ? 1 : 0));
+            secondPrevProbRatioToFirst *= probRatio;
+            prevProbDensity = probDensity;
+            float lastDist2 = sqDistance(rayOrg, sp.position);
+            if (originIsInfinity) lastDist2 = 1;
+            probDensity = roDirPDF * absDot(dirInLocal, gnLocal) / lastDist2;
+            cache->push_back(makeLightVertex<N>(sp, alpha, dirInLocal, roCosTerm / lastDist2, probDensity,
+                                                prevProbDensity, secondPrevPartialDenomMisWeight,
+                                                secondPrevProbRatioToFirst, DT::isDelta(roSampledType), roPrevDelta,
+                                                single, pathLength));
+            float sC = rng.uniform(), s0 = rng.uniform(), s1 = rng.uniform();
+            BSDFResult<N> res;
+            RevResult<N> rev;
+            SS<N> fs = bsdf.sample(q, sC, s0, s1, &res, &rev);
+            if (!(fs == SS<N>(0.0f) || res.dirPDF == 0.0f)) {
+                if (DT::isDispersive(res.sampledType) && !single) single = true;
+                float cosT = dot(res.dirLocal, gnLocal);
+                SS<N> localThroughput = fs * (std::fabs(cosT) / res.dirPDF);
+                alpha *= localThroughput;
+                float continueProb = std::fmin(localThroughput.importance(wlsCommon.sel), 1.0f);
+                if (rng.uniform() < continueProb) {
+                    alpha /= continueProb;
+                    terminate = false;
+                    V3 dirOut = sp.frame.fromLocal(res.dirLocal);
+                    nextOrg = offsetRayOrigin(sp.position, cosT > 0.0f ? sp.geometricNormal : -sp.geometricNormal);
+                    nextDir = dirOut;
+                    woDirPDF = res.dirPDF;
+                    woCosTerm = std::fabs(cosT);
+                    woRevAreaPDF = rev.dirPDF * roCosTerm / lastDist2;
+                    woSampledType = res.sampledType;
+                    woDirPDF *= continueProb;
+                    woRevAreaPDF *= rrProbability<N>(rev.value, q.dirLocal, rev.dirPDF, gnLocal, wlsCommon.sel);
+                    if (originIsPoint) woRevAreaPDF = 0;
+                    originIsPoint = false;
+                    originIsInfinity = false;
+                }
+            }
+        }
+        if (pathLength >= bs.maxPathLength) terminate = true;
+        if (terminate) break;
+        rayOrg = nextOrg;
+        rayDir = nextDir;
+        roDirPDF = woDirPDF;
+        roCosTerm = woCosTerm;
+        roPrevRevAreaPDF = woRevAreaPDF;
+        roSecondPrevDelta = roPrevDelta;
+        roPrevDelta = DT::isDelta(roSampledType);
+        roSampledType = woSampledType;
+    }
+}
+
+// One connection between an eye vertex E and a randomly chosen cache vertex
+// (lvc_bpt.cu:383-478 for the lens vertex, 662-803 for surface vertices).
+template <int N>
+struct EyeConnection {
+    bool valid = false;
+    SS<N> value;      // unweighted contribution divided by nothing yet (flux * conTerm)
+    float partialL = 0, partialE = 0;
+    V3 conRayDir;
+};
+
+template <int N>
+static SS<N> eyePathContribution(const Scene &s, const WLS<N> &wlsCommon, float wavelengthProbability,
+                                 const BPTSettings &bs, const std::vector<LightVertex<N>> &cache, PCG32 &rng,
+                                 uint32_t px, uint32_t py, uint32_t W, uint32_t H, std::vector<float> *splat,
+                                 OracleRenderStats &st) {
+    constexpr int C = N == 4 ? 16 : 3;
+    const uint32_t numLightVertices = static_cast<uint32_t>(cache.size());
+    float pX = px + rng.uniform();
+    float pY = py + rng.uniform();
+    const PerspectiveCamera &cam = s.camera;
+    M3 rot = cam.orientation.toMatrix3x3();
+    float lx, ly;
+    float ul0 = rng.uniform(), ul1 = rng.uniform();
+    concentricSampleDisk(ul0, ul1, &lx, &ly);
+    V3 orgLocal(cam.lensRadius * lx, cam.lensRadius * ly, 0.0f);
+    SurfPt spE0;
+    spE0.frame.z = normalize(rot * V3(0, 0, 1));
+    spE0.frame.x = normalize(rot * V3(1, 0, 0));
+    spE0.frame.y = cross(spE0.frame.z, spE0.frame.x);
+    spE0.position = rot * orgLocal + cam.position;
+    spE0.geometricNormal = spE0.frame.z;
+    spE0.isPoint = false;
+    spE0.atInfinity = false;
+    spE0.u = lx;
+    spE0.v = ly;
+    float lensAreaPDF = cam.lensRadius > 0.0f ? 1.0f / (kPi * cam.lensRadius * cam.lensRadius) : 1.0f;
+    uint32_t lensPosType = cam.lensRadius > 0.0f ? uint32_t(DT::LowFreq) : uint32_t(DT::Delta0D);
+    LensIDF idf{cam.sensitivity, cam.lensRadius * lx, cam.lensRadius * ly, cam.opWidth, cam.opHeight,
+                cam.imgPlaneDistance, cam.objPlaneDistance, cam.imgPlaneArea};
+    SS<N> We0(cam.sensitivity);
+    SS<N> alpha = We0 / (lensAreaPDF * wavelengthProbability * W * H);
+    const float vertexProb = 1.0f / numLightVertices;
+
+    // connection of the lens vertex (t = 1), splatted to the pixel it projects to
+    if (numLightVertices > 0) {
+        uint32_t idx = std::min<uint32_t>(static_cast<uint32_t>(numLightVertices * rng.uniform()), numLightVertices - 1);
+        const LightVertex<N> &vtx = cache[idx];
+        SurfPt spL;
+        int matL = decodeLightVertex<N>(s, vtx, &spL);
+        V3 conRayDir;
+        float sqDist;
+        if (testVisibility(s, spE0, spL, &conRayDir, &sqDist, &st.shadowRays)) {
+            float recSq = 1.0f / sqDist;
+            VertexBSDF<N> bL;
+            bL.fromEDF = vtx.pathLength == 0;
+            if (bL.fromEDF) bL.edf = setupEDF<N>(s, s.materials[matL], spL, wlsCommon);
+            else bL.bsdf = setupBSDF<N>(s.materials[matL], wlsCommon);
+            if (bL.hasNonDelta()) {
+                V3 gnLocalL = spL.frame.toLocal(spL.geometricNormal);
+                BSDFQuery<N> qL{vtx.dirInLocal, gnLocalL, DT::All, TransportMode::Importance, wlsCommon.sel};
+                V3 conLocalL = spL.frame.toLocal(-conRayDir);
+                V3 conLocalE = spE0.frame.toLocal(conRayDir);
+                float cosL = spL.calcCosTerm(conRayDir);
+                float cosE = spE0.calcCosTerm(conRayDir);
+                float G = cosL * cosE * recSq;
+                float scalarConTerm = G * 1.0f / vertexProb;
+                if (vtx.wlSelected) scalarConTerm *= N;
+                SS<N> backwardFsL;
+                SS<N> forwardFsL = bL.evaluate(qL, conLocalL, &backwardFsL);
+                float backwardDirDensityL;
+                bL.evaluatePDF(qL, conLocalL, &backwardDirDensityL);
+                backwardDirDensityL *= rrProbability<N>(backwardFsL, qL.dirLocal, backwardDirDensityL, gnLocalL, wlsCommon.sel);
+                float backwardAreaDensityL = backwardDirDensityL * vtx.backwardConversionFactor;
+                if (vtx.prevDeltaSampled) backwardAreaDensityL = 0;
+                SS<N> forwardFsE(idf.directionalImportance(conLocalE));
+                float forwardDirDensityE = idf.pdf(conLocalE);
+                float forwardAreaDensityE = forwardDirDensityE * cosL * recSq;
+                if (spL.isPoint) forwardAreaDensityE = 0;
+                float u0, u1;
+                bool onScreen = idf.project(conLocalE, &u0, &u1);
+                float partialL;
+                {
+                    bool lastValid = (!vtx.deltaSampled && !vtx.prevDeltaSampled) && vtx.pathLength > 1;
+                    float lastToSecondLastProbRatio = backwardAreaDensityL / vtx.prevProbDensity;
+                    partialL = pow2(lastToSecondLastProbRatio) * (vtx.secondPrevPartialDenomMisWeight + (lastValid ? 1 : 0));
+                    float probRatioToFirst = vtx.secondPrevProbRatioToFirst;
+                    if (vtx.pathLength > 0) probRatioToFirst *= lastToSecondLastProbRatio;
+                    bool curValid = !vtx.deltaSampled && vtx.pathLength > 0;
+                    float curToLastProbRatio = forwardAreaDensityE / vtx.probDensity;
+                    partialL = pow2(curToLastProbRatio) * (partialL + (curValid ? 1 : 0));
+                    probRatioToFirst *= curToLastProbRatio;
+                    partialL += pow2(probRatioToFirst / vertexProb);
+                }
+                SS<N> conTerm = forwardFsL * scalarConTerm * forwardFsE;
+                SS<N> unweighted = vtx.flux * conTerm * alpha;
+                float misWeight = 1.0f / (1.0f + partialL);
+                SS<N> contribution = misWeight * unweighted;
+                if (contribution.allFinite() && onScreen) {
+                    uint32_t ipx = static_cast<uint32_t>(u0 * W), ipy = static_cast<uint32_t>(u1 * H);
+                    if (ipx < W && ipy < H) {
+                        float *dst = splat->data() + (size_t(ipy) * W + ipx) * C;
+                        if constexpr (N == 4) {
+                            SS<16> b = binSpectral(wlsCommon, contribution);
+                            for (int c = 0; c < 16; ++c) {
+#pragma omp atomic
+                                dst[c] += b[c];
+                            }
+                        } else {
+                            for (int c = 0; c < 3; ++c) {
+#pragma omp atomic
+                                dst[c] += contribution[c];
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+
+    // first eye ray (cameras.cu:145-157)
+    float uD0 = pX / W, uD1 = pY / H;
+    V3 pFocus(cam.opWidth * (0.5f - uD0), cam.opHeight * (0.5f - uD1), cam.objPlaneDistance);
+    V3 dirLocal = normalize(pFocus - V3(idf.xOnLens, idf.yOnLens, 0.0f));
+    float dirPDF = pow2(cam.imgPlaneDistance) / (pow3(dirLocal.z) * cam.imgPlaneArea);
+    V3 rayOrg = spE0.position;
+    V3 rayDir = spE0.frame.fromLocal(dirLocal);
+    float cosTerm = spE0.calcCosTerm(rayDir);
+    alpha *= SS<N>(1.0f) * (cosTerm / dirPDF);
+
+    float roDirPDF = dirPDF, roCosTerm = cosTerm, roPrevRevAreaPDF = 1;
+    bool roSecondPrevDelta = false, roPrevDelta = DT::isDelta(lensPosType);
+    uint32_t roSampledType = DT::Acquisition | DT::LowFreq;
+    float probDensity = lensAreaPDF, prevProbDensity = 1.0f, secondPrevPartialDenomMisWeight = 0;
+    bool single = false;
+    uint32_t pathLength = 0;
+    SS<N> contribution(0.0f);
+    while (true) {
+        bool terminate = true;
+        ++pathLength;
+        HitRec hit;
+        ++st.closestRays;
+        bool found = traceRay(s, rayOrg, rayDir, 0.0f, FLT_MAX, &hit, false, true);
+        WLS<N> wls = wlsCommon;
+        if (single) wls.single = true;
+        V3 nextOrg, nextDir;
+        float woDirPDF = 0, woCosTerm = 0, woRevAreaPDF = 0;
+        uint32_t woSampledType = 0;
+        if (!found) {
+            // miss program (lvc_bpt.cu:841-971)
+            if (s.hasEnv && s.envImportance > 0) {
+                SurfPt spE;
+                float theta, posPhi;
+                rayDir.toPolarYUp(&theta, &posPhi);
+                float sinPhi = std::sin(posPhi), cosPhi = std::cos(posPhi);
+                V3 tc0 = normalize(V3(-cosPhi, 0.0f, -sinPhi));
+                spE.frame.x = tc0;
+                spE.frame.z = -rayDir;
+                spE.frame.y = cross(spE.frame.z, spE.frame.x);
+                spE.position = rayDir;
+                spE.isPoint = false;
+                spE.atInfinity = true;
+                spE.geometricNormal = -rayDir;
+                spE.u = posPhi;
+                spE.v = theta;
+                float phi = posPhi + s.envRotation;
+                phi = phi - std::floor(phi / (2 * kPi)) * 2 * kPi;
+                spE.texU = phi / (2 * kPi);
+                spE.texV = theta / kPi;
+                float uvPDF = s.envImportanceMap.evaluatePDF(phi / (2 * kPi), theta / kPi);
+                float hypAreaPDF = uvPDF / (2 * kPi * kPi * std::sin(theta));
+                EDF<N> edf = setupEDF<N>(s, s.materials[s.envMaterial], spE, wls);
+                V3 dirOutLocalE = spE.frame.toLocal(-rayDir);
+                bool secondLastValid = (!roPrevDelta && !roSecondPrevDelta) && pathLength > 2;
+                secondPrevPartialDenomMisWeight = pow2(roPrevRevAreaPDF / prevProbDensity) *
+                                                  (secondPrevPartialDenomMisWeight + (secondLastValid ? 1 : 0));
+                prevProbDensity = probDensity;
+                float lastDist2 = 1.0f;
+                probDensity = roDirPDF / lastDist2;
+                SS<N> spEmittance = edf.evaluateEmittance();
+                if (spEmittance.hasNonZero() && edf.matches(DT::Emission | DT::NonDelta)) {
+                    SS<N> Le = spEmittance * edf.evaluate(dirOutLocalE);
+                    SS<N> unweighted = alpha * Le;
+                    if (single) unweighted *= float(N);
+                    float instProb = 1.0f / s.lightInstDist.integral;
+                    float geomInstProb = 1.0f;
+                    float forwardAreaDensityL = bs.numLightPaths * instProb * geomInstProb * hypAreaPDF;
+                    float backwardDirDensityE = edf.evaluatePDF(dirOutLocalE);
+                    float backwardAreaDensityE = backwardDirDensityE * roCosTerm / lastDist2;
+                    float partialE;
+                    {
+                        bool lastValid = (!DT::isDelta(roSampledType) && !roPrevDelta) && pathLength > 1;
+                        float r = backwardAreaDensityE / prevProbDensity;
+                        partialE = pow2(r) * (secondPrevPartialDenomMisWeight + (lastValid ? 1 : 0));
+                        bool curValid = !DT::isDelta(roSampledType);
+                        float r2 = forwardAreaDensityL / probDensity;
+                        partialE = pow2(r2) * (partialE + (curValid ? 1 : 0));
+                        partialE *= pow2(vertexProb);
+                    }
+                    SS<N> c = (1.0f / (1.0f + partialE)) * unweighted;
+                    if (c.allFinite()) contribution += c;
+                }
+            }
+            break;
+        }
+        // closest-hit program for eye paths (lvc_bpt.cu:570-837)
+        SurfPt spE;
+        float hypAreaPDF;
+        calcSurfacePoint(s, hit, &spE, &hypAreaPDF);
+        const GeomInst &g = s.geomInsts[hit.geomInst];
+        const Material &matE = s.materials[g.material];
+        BSDF<N> bsdfE = setupBSDF<N>(matE, wls);
+        EDF<N> edf = setupEDF<N>(s, matE, spE, wls);
+        V3 dirOutLocalE = spE.frame.toLocal(-rayDir);
+        V3 gnLocalE = spE.frame.toLocal(spE.geometricNormal);
+        BSDFQuery<N> qE{dirOutLocalE, gnLocalE, DT::All, TransportMode::Radiance, wls.sel};
+        bool secondLastValidE = (!roPrevDelta && !roSecondPrevDelta) && pathLength > 2;
+        secondPrevPartialDenomMisWeight = pow2(roPrevRevAreaPDF / prevProbDensity) *
+                                          (secondPrevPartialDenomMisWeight + (secondLastValidE ? 1 : 0));
+        prevProbDensity = probDensity;
+        float lastDist2 = sqDistance(rayOrg, spE.position);
+        probDensity = roDirPDF * absDot(dirOutLocalE, gnLocalE) / lastDist2;
+
+        SS<N> spEmittance = edf.evaluateEmittance();
+        if (spEmittance.hasNonZero() && edf.matches(DT::Emission | DT::NonDelta)) {
+            SS<N> Le = spEmittance * edf.evaluate(dirOutLocalE);
+            SS<N> unweighted = alpha * Le;
+            if (single) unweighted *= float(N);
+            const Instance &in = s.instances[hit.inst];
+            float instProb = in.lightDist.integral / s.lightInstDist.integral;
+            float geomInstProb = g.importance / in.lightDist.integral;
+            float forwardAreaDensityL = bs.numLightPaths * instProb * geomInstProb * hypAreaPDF;
+            float backwardDirDensityE = edf.evaluatePDF(dirOutLocalE);
+            float backwardAreaDensityE = backwardDirDensityE * roCosTerm / lastDist2;
+            float partialE;
+            {
+                bool lastValid = (!DT::isDelta(roSampledType) && !roPrevDelta) && pathLength > 1;
+                float r = backwardAreaDensityE / prevProbDensity;
+                partialE = pow2(r) * (secondPrevPartialDenomMisWeight + (lastValid ? 1 : 0));
+                bool curValid = !DT::isDelta(roSampledType);
+                float r2 = forwardAreaDensityL / probDensity;
+                partialE = pow2(r2) * (partialE + (curValid ? 1 : 0));
+                partialE *= pow2(vertexProb);
+            }
+            SS<N> c = (1.0f / (1.0f + partialE)) * unweighted;
+            if (c.allFinite()) contribution += c;
+        }
+
+        if (bsdfE.hasNonDelta() && numLightVertices > 0) {
+            uint32_t idx = std::min<uint32_t>(static_cast<uint32_t>(numLightVertices * rng.uniform()), numLightVertices - 1);
+            const LightVertex<N> &vtx = cache[idx];
+            SurfPt spL;
+            int matL = decodeLightVertex<N>(s, vtx, &spL);
+            V3 conRayDir;
+            float sqDist;
+            if (testVisibility(s, spE, spL, &conRayDir, &sqDist, &st.shadowRays)) {
+                float recSq = 1.0f / sqDist;
+                VertexBSDF<N> bL;
+                bL.fromEDF = vtx.pathLength == 0;
+                if (bL.fromEDF) bL.edf = setupEDF<N>(s, s.materials[matL], spL, wls);
+                else bL.bsdf = setupBSDF<N>(s.materials[matL], wls);
+                if (bL.hasNonDelta()) {
+                    V3 gnLocalL = spL.frame.toLocal(spL.geometricNormal);
+                    BSDFQuery<N> qL{vtx.dirInLocal, gnLocalL, DT::All, TransportMode::Importance, wls.sel};
+                    V3 conLocalL = spL.frame.toLocal(-conRayDir);
+                    V3 conLocalE = spE.frame.toLocal(conRayDir);
+                    float cosL = spL.calcCosTerm(conRayDir);
+                    float cosE = spE.calcCosTerm(conRayDir);
+                    float G = cosL * cosE * recSq;
+                    float scalarConTerm = G * 1.0f / vertexProb;
+                    if (vtx.wlSelected || single) scalarConTerm *= N;
+                    SS<N> backwardFsL;
+                    SS<N> forwardFsL = bL.evaluate(qL, conLocalL, &backwardFsL);
+                    float backwardDirDensityL;
+                    float forwardDirDensityL = bL.evaluatePDF(qL, conLocalL, &backwardDirDensityL);
+                    if (vtx.pathLength > 0)
+                        forwardDirDensityL *= rrProbability<N>(forwardFsL, conLocalL, forwardDirDensityL, gnLocalL, wlsCommon.sel);
+                    backwardDirDensityL *= rrProbability<N>(backwardFsL, qL.dirLocal, backwardDirDensityL, gnLocalL, wlsCommon.sel);
+                    float forwardAreaDensityL = forwardDirDensityL * cosE * recSq;
+                    float backwardAreaDensityL = backwardDirDensityL * vtx.backwardConversionFactor;
+                    if (vtx.prevDeltaSampled) backwardAreaDensityL = 0;
+
+                    SS<N> backwardFsE;
+                    SS<N> forwardFsE = bsdfE.evaluate(qE, conLocalE, &backwardFsE);
+                    float backwardDirDensityE;
+                    float forwardDirDensityE = bsdfE.evaluatePDF(qE, conLocalE, &backwardDirDensityE);
+                    forwardDirDensityE *= rrProbability<N>(forwardFsE, conLocalE, forwardDirDensityE, gnLocalE, wlsCommon.sel);
+                    backwardDirDensityE *= rrProbability<N>(backwardFsE, qE.dirLocal, backwardDirDensityE, gnLocalE, wlsCommon.sel);
+                    float forwardAreaDensityE = forwardDirDensityE * cosL * recSq;
+                    if (spL.isPoint) forwardAreaDensityE = 0;
+                    float backwardAreaDensityE = backwardDirDensityE * roCosTerm / lastDist2;
+
+                    float partialL;
+                    {
+                        bool lastValid = (!vtx.deltaSampled && !vtx.prevDeltaSampled) && vtx.pathLength > 1;
+                        float r = backwardAreaDensityL / vtx.prevProbDensity;
+                        partialL = pow2(r) * (vtx.secondPrevPartialDenomMisWeight + (lastValid ? 1 : 0));
+                        float probRatioToFirst = vtx.secondPrevProbRatioToFirst;
+                        if (vtx.pathLength > 0) probRatioToFirst *= r;
+                        bool curValid = !vtx.deltaSampled && vtx.pathLength > 0;
+                        float r2 = forwardAreaDensityE / vtx.probDensity;
+                        partialL = pow2(r2) * (partialL + (curValid ? 1 : 0));
+                        probRatioToFirst *= r2;
+                        partialL += pow2(probRatioToFirst / vertexProb);
+                    }
+                    float partialE;
+                    {
+                        bool lastValid = !DT::isDelta(roSampledType) && !roPrevDelta;
+                        float r = backwardAreaDensityE / prevProbDensity;
+                        partialE = pow2(r) * (secondPrevPartialDenomMisWeight + (lastValid ? 1 : 0));
+                        bool curValid = !DT::isDelta(roSampledType);
+                        float r2 = forwardAreaDensityL / probDensity;
+                        partialE = pow2(r2) * (partialE + (curValid ? 1 : 0));
+                    }
+                    SS<N> conTerm = forwardFsL * scalarConTerm * forwardFsE;
+                    SS<N> unweighted = vtx.flux * conTerm * alpha;
+                    SS<N> c = (1.0f / (1.0f + partialL + partialE)) * unweighted;
+                    if (c.allFinite()) contribution += c;
+                }
+            }
+        }
+
+        float sC = rng.uniform(), s0 = rng.uniform(), s1 = rng.uniform();
+        BSDFResult<N> res;
+        RevResult<N> rev;
+        SS<N> fs = bsdfE.sample(qE, sC, s0, s1, &res, &rev);
+        if (!(fs == SS<N>(0.0f) || res.dirPDF == 0.0f)) {
+            if (DT::isDispersive(res.sampledType) && !single) single = true;
+            float cosT = dot(res.dirLocal, gnLocalE);
+            SS<N> localThroughput = fs * (std::fabs(cosT) / res.dirPDF);
+            alpha *= localThroughput;
+            float continueProb = std::fmin(localThroughput.importance(wlsCommon.sel), 1.0f);
+            if (rng.uniform() < continueProb) {
+                alpha /= continueProb;
+                terminate = false;
+                V3 dirInE = spE.frame.fromLocal(res.dirLocal);
+                nextOrg = offsetRayOrigin(spE.position, cosT > 0.0f ? spE.geometricNormal : -spE.geometricNormal);
+                nextDir = dirInE;
+                woDirPDF = res.dirPDF * continueProb;
+                woCosTerm = std::fabs(cosT);
+                woRevAreaPDF = rev.dirPDF * roCosTerm / lastDist2;
+                woRevAreaPDF *= rrProbability<N>(rev.value, qE.dirLocal, rev.dirPDF, gnLocalE, wlsCommon.sel);
+                woSampledType = res.sampledType;
+            }
+        }
+        if (pathLength >= bs.maxPathLength) terminate = true;
+        if (terminate) break;
+        rayOrg = nextOrg;
+        rayDir = nextDir;
+        roDirPDF = woDirPDF;
+        roCosTerm = woCosTerm;
+        roPrevRevAreaPDF = woRevAreaPDF;
+        roSecondPrevDelta = roPrevDelta;
+        roPrevDelta = DT::isDelta(roSampledType);
+        roSampledType = woSampledType;
+    }
+    return contribution;
+}
+
+// context.cpp:1464-1494: per frame, common wavelengths from the context's mt19937 (two
+// uniform floats), the light-path pass into a fresh cache, the eye pass, then the splats.
+template <int N, int C>
+static void renderBPTImpl(const Scene &s, uint32_t W, uint32_t H, uint32_t numFrames, uint32_t maxPathLength,
+                          uint32_t numLightPaths, int numThreads, float *outRGBA, uint64_t *rngState,
+                          uint64_t *lightRngState, float *accumOut, OracleRenderStats *stats) {
+    std::vector<uint64_t> rngs(size_t(W) * H), lrngs(numLightPaths);
+    if (rngState) std::memcpy(rngs.data(), rngState, rngs.size() * 8);
+    else {
+        std::mt19937_64 seedRng(591842031321323413ULL);
+        for (auto &r : rngs) r = seedRng();
+    }
+    if (lightRngState) std::memcpy(lrngs.data(), lightRngState, lrngs.size() * 8);
+    else {
+        std::mt19937_64 seedRng(459182033132123413ULL);
+        for (auto &r : lrngs) r = seedRng();
+    }
+    std::mt19937 wlRng;
+    std::uniform_real_distribution<float> u01;
+    BPTSettings bs{numLightPaths, maxPathLength};
+    std::vector<Kahan<SS<C>>> accum(size_t(W) * H);
+    for (auto &a : accum) a.reset();
+    OracleRenderStats total = {};
+    if (numThreads > 0) omp_set_num_threads(numThreads);
+    for (uint32_t frame = 1; frame <= numFrames; ++frame) {
+        float wlOff = u01(wlRng), wlSel = u01(wlRng);
+        float wavelengthProbability;
+        WLS<N> wls = WLS<N>::createWithEqualOffsets(wlOff, wlSel, &wavelengthProbability);
+        // light paths: each light path appends its vertices in order; the cache is the paths'
+        // vertex lists concatenated in light-path order
+        std::vector<std::vector<LightVertex<N>>> perPath(numLightPaths);
+#pragma omp parallel
+        {
+            OracleRenderStats st = {};
+#pragma omp for schedule(dynamic, 256)
+            for (int64_t i = 0; i < int64_t(numLightPaths); ++i) {
+                PCG32 rng{lrngs[i]};
+                traceLightPath<N>(s, wls, bs, rng, &perPath[i], st);
+                lrngs[i] = rng.state;
+            }
+#pragma omp critical
+            {
+                total.closestRays += st.closestRays;
+            }
+        }
+        std::vector<LightVertex<N>> cache;
+        for (auto &v : perPath) cache.insert(cache.end(), v.begin(), v.end());
+        std::vector<float> splat(size_t(W) * H * C, 0.0f);
+        std::vector<SS<C>> eye(size_t(W) * H);
+        std::vector<uint8_t> eyeOk(size_t(W) * H, 0);
+#pragma omp parallel
+        {
+            OracleRenderStats st = {};
+#pragma omp for schedule(dynamic, 64)
+            for (int64_t p = 0; p < int64_t(W) * H; ++p) {
+                uint32_t x = uint32_t(p % W), y = uint32_t(p / W);
+                PCG32 rng{rngs[p]};
+                SS<N> c = eyePathContribution<N>(s, wls, wavelengthProbability, bs, cache, rng, x, y, W, H, &splat, st);
+                rngs[p] = rng.state;
+                ++st.paths;
+                if (!c.allFinite()) {
+                    ++st.droppedNonFinite;
+                    continue;
+                }
+                eyeOk[p] = 1;
+                if constexpr (N == 3) eye[p] = c;
+                else eye[p] = binSpectral(wls, c);
+            }
+#pragma omp critical
+            {
+                total.closestRays += st.closestRays;
+                total.shadowRays += st.shadowRays;
+                total.paths += st.paths;
+                total.droppedNonFinite += st.droppedNonFinite;
+            }
+        }
+        for (size_t p = 0; p < accum.size(); ++p) {
+            if (eyeOk[p]) {
+                if (frame == 1) accum[p].reset();
+                accum[p].add(eye[p]);
+            } else if (frame == 1) {
+                accum[p].reset();
+            }
+            SS<C> sv;
+            for (int c = 0; c < C; ++c) sv[c] = splat[p * C + c];
+            accum[p].add(sv);
+        }
+    }
+    if (rngState) std::memcpy(rngState, rngs.data(), rngs.size() * 8);
+    if (lightRngState) std::memcpy(lightRngState, lrngs.data(), lrngs.size() * 8);
+    const float recN = 1.0f / numFrames;
+    const float resCorrection = float(W) * float(H);
+    for (size_t p = 0; p < accum.size(); ++p) {
+        float XYZ[3];
+        if constexpr (N == 3) {
+            float RGB[3] = {accum[p].result[0], accum[p].result[1], accum[p].result[2]};
+            transformTristimulus(mat_Rec709_D65_to_XYZ, RGB, XYZ);
+        } else {
+            strataToXYZ(accum[p].result, XYZ);
+        }
+        XYZ[0] *= recN; XYZ[1] *= recN; XYZ[2] *= recN;
+        float RGB[3];
+        transformTristimulus(mat_XYZ_to_Rec709_D65, XYZ, RGB);
+        if (outRGBA) {
+            outRGBA[4 * p + 0] = RGB[0] * resCorrection;
+            outRGBA[4 * p + 1] = RGB[1] * resCorrection;
+            outRGBA[4 * p + 2] = RGB[2] * resCorrection;
+            outRGBA[4 * p + 3] = 1.0f;
+        }
+        if (accumOut) {
+            for (int c = 0; c < C; ++c) {
+                accumOut[(p * C + c) * 2 + 0] = accum[p].result[c];
+                accumOut[(p * C + c) * 2 + 1] = accum[p].comp[c];
+            }
+        }
+    }
+    if (stats) *stats = total;
+}
+
 template <int N, int C>
 static void renderImpl(const Scene &s, uint32_t W, uint32_t H, uint32_t numFrames, uint32_t maxPathLength,
                        int numThreads, float *outRGBA, uint64_t *rngState, float *accumOut,
@@ -1383,6 +2155,20 @@ int oracle_trace_any(void *sp, const OracleRay *rays, uint32_t n, uint8_t *occ, 
     return 0;
 }
 
+int oracle_render_bpt(void *sp, uint32_t W, uint32_t H, uint32_t numFrames, uint32_t maxPathLength,
+                      uint32_t numLightPaths, int numThreads, float *outRGBA, uint64_t *rngState,
+                      uint64_t *lightRngState, float *accumOut, OracleRenderStats *stats) {
+    Scene *s = static_cast<Scene *>(sp);
+    if (!s || numLightPaths == 0) return 1;
+    if (s->spectral)
+        renderBPTImpl<4, 16>(*s, W, H, numFrames, maxPathLength, numLightPaths, numThreads, outRGBA, rngState,
+                             lightRngState, accumOut, stats);
+    else
+        renderBPTImpl<3, 3>(*s, W, H, numFrames, maxPathLength, numLightPaths, numThreads, outRGBA, rngState,
+                            lightRngState, accumOut, stats);
+    return 0;
+}
+
 int oracle_render(void *sp, uint32_t W, uint32_t H, uint32_t numFrames, uint32_t maxPathLength, int numThreads,
                   float *outRGBA, uint64_t *rngState, float *accumOut, const int32_t *range,
                   OracleRenderStats *stats) {
@@ -1412,9 +2198,10 @@ static void bsdfQueries(const OracleMaterial *mat, const OracleBSDFQuery *q, uin
         BSDF<N> b = setupBSDF<N>(s->materials[0], wls);
         BSDFQuery<N> bq{V3(q[i].dirLocal[0], q[i].dirLocal[1], q[i].dirLocal[2]),
                         V3(q[i].geomNormalLocal[0], q[i].geomNormalLocal[1], q[i].geomNormalLocal[2]), DT::All,
-                        TransportMode::Radiance, wls.sel};
+                        q[i].transportMode ? TransportMode::Importance : TransportMode::Radiance, wls.sel};
         BSDFResult<N> r;
-        SS<N> f = b.sample(bq, q[i].uComponent, q[i].uDir[0], q[i].uDir[1], &r);
+        RevResult<N> sr;
+        SS<N> f = b.sample(bq, q[i].uComponent, q[i].uDir[0], q[i].uDir[1], &r, &sr);
         OracleBSDFResult &o = out[i];
         std::memset(&o, 0, sizeof(o));
         o.sampledDir[0] = r.dirLocal.x; o.sampledDir[1] = r.dirLocal.y; o.sampledDir[2] = r.dirLocal.z;
@@ -1428,6 +2215,8 @@ static void bsdfQueries(const OracleMaterial *mat, const OracleBSDFQuery *q, uin
         o.evalPDF = b.evaluatePDF(bq, ed, &revPDF);
         o.revPDF = revPDF;
         for (int c = 0; c < N; ++c) { o.evalValue[c] = e[c]; o.revValue[c] = rv[c]; }
+        o.sampledRevPDF = r.dirPDF > 0.0f ? sr.dirPDF : 0.0f;
+        for (int c = 0; c < N; ++c) o.sampledRevValue[c] = r.dirPDF > 0.0f ? sr.value[c] : 0.0f;
     }
 }
 

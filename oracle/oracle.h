@@ -110,6 +110,12 @@ int oracle_render(void* scene, uint32_t width, uint32_t height, uint32_t numFram
                   int numThreads, float* outRGBA, uint64_t* rngState, float* accumOut,
                   const int32_t* pixelRange, OracleRenderStats* stats);
 
+/* LVC-BPT render (lvc_bpt.cu:144-971, context.cpp:1464-1494).  lightRngState: numLightPaths
+ * PCG32 states of the light paths (seeded from mt19937_64(459182033132123413) when null). */
+int oracle_render_bpt(void* scene, uint32_t width, uint32_t height, uint32_t numFrames, uint32_t maxPathLength,
+                      uint32_t numLightPaths, int numThreads, float* outRGBA, uint64_t* rngState,
+                      uint64_t* lightRngState, float* accumOut, OracleRenderStats* stats);
+
 /* BSDF parity helper: builds the BSDF of `mat` (immediate values only) for the given
  * wavelength set and evaluates sample / evaluate / evaluatePDF in the shading frame. */
 typedef struct OracleBSDFQuery {
@@ -119,6 +125,7 @@ typedef struct OracleBSDFQuery {
     float uComponent, uDir[2];   /* sample inputs */
     float evalDir[3];            /* direction for evaluate / evaluatePDF */
     int32_t singleIsSelected;
+    int32_t transportMode;       /* 0 radiance, 1 importance */
 } OracleBSDFQuery;
 typedef struct OracleBSDFResult {
     float sampledDir[3];
@@ -129,6 +136,8 @@ typedef struct OracleBSDFResult {
     float evalPDF;
     float revValue[4];           /* bidirectional reverse quantities of evaluate/evaluatePDF */
     float revPDF;
+    float sampledRevValue[4];    /* reverse quantities of the sample */
+    float sampledRevPDF;
 } OracleBSDFResult;
 int oracle_bsdf_queries(const OracleMaterial* mat, int spectral, const OracleBSDFQuery* q, uint32_t n,
                         OracleBSDFResult* out);

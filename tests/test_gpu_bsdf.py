@@ -24,6 +24,10 @@ MATERIALS = [
     scenes.MaterialDesc(scenes.SPECULAR_REFLECTION, {"coeff": (scenes.REC709, 0.9, 0.9, 0.9),
                                                      "eta": (scenes.REC709, 0.2, 0.9, 1.1),
                                                      "k": (scenes.REC709, 3.9, 2.4, 2.2)}),
+    scenes.MaterialDesc(scenes.MICROFACET_SCATTERING, {"coeff": (scenes.REC709, 0.95, 0.9, 1.0),
+                                                       "eta ext": (scenes.REC709, 1.0, 1.0, 1.0),
+                                                       "eta int": (scenes.REC709, 1.45, 1.5, 1.55)},
+                        {"roughness": 0.4, "anisotropy": 0.3, "rotation": 0.0}),
 ]
 
 
@@ -44,7 +48,8 @@ def make_queries(n, seed):
     q["uDir"] = rng.uniform(size=(n, 2))
     e = rng.normal(size=(n, 3))
     e /= np.linalg.norm(e, axis=1, keepdims=True)
-    e[:, 2] = np.abs(e[:, 2]) * np.sign(d[:, 2])
+    # mostly the query's hemisphere; 30 % on the other side to exercise transmission
+    e[:, 2] = np.abs(e[:, 2]) * np.sign(d[:, 2]) * np.where(rng.uniform(size=n) < 0.7, 1, -1)
     q["evalDir"] = e
     return q
 
@@ -53,11 +58,13 @@ def rel_close(a, b, rtol=1e-5, atol=1e-6):
     return np.abs(a - b) <= atol + rtol * np.maximum(np.abs(a), np.abs(b))
 
 
+@pytest.mark.parametrize("transport", [0, 1])
 @pytest.mark.parametrize("spectral", [False, True])
 @pytest.mark.parametrize("mi", range(len(MATERIALS)))
-def test_bsdf_matches_oracle(torch_gpu, mi, spectral):
+def test_bsdf_matches_oracle(torch_gpu, mi, spectral, transport):
     m = MATERIALS[mi]
     q = make_queries(4096, seed=100 + mi)
+    q["transportMode"] = transport
     ref = ob.bsdf_queries(m, spectral, q.view(ob.query_dtype()))
     with api.Context(spectral=spectral) as ctx:
         h = api.build_material(ctx, m)
@@ -67,6 +74,10 @@ def test_bsdf_matches_oracle(torch_gpu, mi, spectral):
     ok_eval = np.all(rel_close(got["evalValue"][:, :nc], ref["evalValue"][:, :nc]), axis=1)
     ok_epdf = rel_close(got["evalPDF"], ref["evalPDF"])
     assert ok_eval.all() and ok_epdf.all(), (np.where(~ok_eval)[0][:5], np.where(~ok_epdf)[0][:5])
+    # reverse (bidirectional) quantities of evaluate / evaluatePDF
+    ok_rev = np.all(rel_close(got["revValue"][:, :nc], ref["revValue"][:, :nc]), axis=1)
+    ok_rpdf = rel_close(got["revPDF"], ref["revPDF"])
+    assert ok_rev.all() and ok_rpdf.all(), (np.where(~ok_rev)[0][:5], np.where(~ok_rpdf)[0][:5])
     # sampling: same lobe and type, direction within 1e-5. The sampled direction goes through
     # sin/cos/sqrt, whose last bit differs between the device and libm; at low roughness the
     # GGX lobe turns that into a much larger change of the value. The bound is therefore
@@ -89,6 +100,15 @@ def test_bsdf_matches_oracle(torch_gpu, mi, spectral):
     tol_pdf = 1e-6 + 1e-5 * np.abs(ref["sampledPDF"]) + 4 * sens_pdf
     ok_val = np.all(np.abs(got["sampledValue"][:, :nc] - ref["sampledValue"][:, :nc]) <= tol_val, axis=1)
     ok_pdf = np.abs(got["sampledPDF"] - ref["sampledPDF"]) <= tol_pdf
+    # reverse quantities of the sample, with the same direction-sensitivity allowance
+    with np.errstate(invalid="ignore"):
+        sens_rv = np.where(sampled[:, None], np.abs(e_got["revValue"][:, :nc] - e_ref["revValue"][:, :nc]), 0)
+        sens_rp = np.where(sampled, np.abs(e_got["revPDF"] - e_ref["revPDF"]), 0)
+    tol_rv = 1e-6 + 1e-5 * np.abs(ref["sampledRevValue"][:, :nc]) + 4 * sens_rv
+    tol_rp = 1e-6 + 1e-5 * np.abs(ref["sampledRevPDF"]) + 4 * sens_rp
+    ok_rv = np.all(np.abs(got["sampledRevValue"][:, :nc] - ref["sampledRevValue"][:, :nc]) <= tol_rv, axis=1)
+    ok_rp = np.abs(got["sampledRevPDF"] - ref["sampledRevPDF"]) <= tol_rp
+    assert ok_rv.all() and ok_rp.all(), (np.where(~ok_rv)[0][:5], np.where(~ok_rp)[0][:5])
     exact_dir = np.all(got["sampledDir"] == ref["sampledDir"], axis=1)
     print(m.type, "spectral" if spectral else "rgb", "dirs bit-exact", int(exact_dir.sum()), "of", len(q),
           "value fails", np.where(~ok_val)[0][:5], "pdf fails", np.where(~ok_pdf)[0][:5])

@@ -69,11 +69,13 @@ HIT_DTYPE = np.dtype([("inst", np.int32), ("geomInst", np.int32), ("prim", np.in
                       ("b1", np.float32), ("b2", np.float32)])
 BSDF_QUERY_DTYPE = np.dtype([("dirLocal", np.float32, 3), ("geomNormalLocal", np.float32, 3),
                              ("wlOffset", np.float32), ("wlSelect", np.float32), ("uComponent", np.float32),
-                             ("uDir", np.float32, 2), ("evalDir", np.float32, 3), ("singleIsSelected", np.int32)])
+                             ("uDir", np.float32, 2), ("evalDir", np.float32, 3), ("singleIsSelected", np.int32),
+                             ("transportMode", np.int32)])
 BSDF_RESULT_DTYPE = np.dtype([("sampledDir", np.float32, 3), ("sampledPDF", np.float32),
                               ("sampledType", np.uint32), ("sampledValue", np.float32, 4),
                               ("evalValue", np.float32, 4), ("evalPDF", np.float32), ("revValue", np.float32, 4),
-                              ("revPDF", np.float32)])
+                              ("revPDF", np.float32), ("sampledRevValue", np.float32, 4),
+                              ("sampledRevPDF", np.float32)])
 
 H = C.c_void_p
 U32 = C.c_uint32

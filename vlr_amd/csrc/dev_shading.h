@@ -250,6 +250,12 @@ __device__ __forceinline__ V3 cosineSampleHemisphere(float u0, float u1) {
     concentricSampleDisk(u0, u1, &x, &y);
     return V3(x, y, sqrtf(fmaxf(0.0f, 1.0f - x * x - y * y)));
 }
+// random_distributions.h: uniform direction on the sphere
+__device__ __forceinline__ V3 uniformSampleSphere(float u0, float u1) {
+    float phi = 2 * kPiF * u1;
+    float theta = acosf(1 - 2 * u0);
+    return fromPolarZUp(phi, theta);
+}
 __device__ __forceinline__ void uniformSampleTriangle(float u0, float u1, float *b0, float *b1) {
     *b0 = 0.5f * u0;
     *b1 = 0.5f * u1;
@@ -302,6 +308,19 @@ __device__ __forceinline__ Spec<N> fresnelDielectric(const Spec<N> &etaExt, cons
         }
     }
     return r;
+}
+
+// FresnelDielectric::evaluate(cosEnter, wlIdx) for one wavelength (materials.cu:79-95)
+__device__ __forceinline__ float fresnelDielectric1(float etaExt, float etaInt, float cosEnter) {
+    cosEnter = fminf(fmaxf(cosEnter, -1.0f), 1.0f);
+    bool entering = cosEnter > 0.0f;
+    float eEnter = entering ? etaExt : etaInt;
+    float eExit = entering ? etaInt : etaExt;
+    float sinExit = eEnter / eExit * sqrtf(fmaxf(0.0f, 1.0f - cosEnter * cosEnter));
+    cosEnter = fabsf(cosEnter);
+    if (sinExit >= 1.0f) return 1.0f;
+    float cosExit = sqrtf(fmaxf(0.0f, 1.0f - sinExit * sinExit));
+    return fresnelDielectricF(eEnter, eExit, cosEnter, cosExit);
 }
 
 struct Ggx {
@@ -358,7 +377,19 @@ struct Ggx {
     }
 };
 
-enum BsdfKind : uint32_t { BSDF_NULL = 0, BSDF_MATTE, BSDF_SPEC_BRDF, BSDF_SPEC_BSDF, BSDF_MICRO_BRDF, BSDF_DIFFSPEC };
+enum BsdfKind : uint32_t {
+    BSDF_NULL = 0, BSDF_MATTE, BSDF_SPEC_BRDF, BSDF_SPEC_BSDF, BSDF_MICRO_BRDF, BSDF_DIFFSPEC, BSDF_MICRO_BSDF,
+    // an emitter's directional distribution used as a BSDF at the first light-path vertex
+    // (the EDF_as_BSDF_* callables of materials.cu:415-481); a0 = world disc area for the environment
+    BSDF_EDF_DIFFUSE, BSDF_EDF_POINT, BSDF_EDF_ENV
+};
+
+// Reverse (adjoint-direction) quantities of a BSDF query, used by bidirectional estimators.
+template <int N>
+struct BsdfRev {
+    Spec<N> value;
+    float pdf;
+};
 
 template <int N>
 struct Bsdf {
@@ -374,13 +405,15 @@ struct Bsdf {
         case BSDF_SPEC_BSDF: return dt::WholeSphere | dt::Delta0D;
         case BSDF_MICRO_BRDF: return dt::Reflection | dt::HighFreq;
         case BSDF_DIFFSPEC: return dt::Reflection | dt::LowFreq | dt::HighFreq;
+        case BSDF_MICRO_BSDF: return dt::WholeSphere | dt::HighFreq;
+        case BSDF_EDF_DIFFUSE: case BSDF_EDF_POINT: case BSDF_EDF_ENV: return dt::Reflection | dt::LowFreq;
         default: return 0u;
         }
     }
     __device__ __forceinline__ bool matches(uint32_t flags) const { return dt::matches(typeMask(), flags); }
     __device__ __forceinline__ bool hasNonDelta() const { return matches(dt::WholeSphere | dt::NonDelta); }
 
-    // UE4 selection weights (materials.cu:1724-1734)
+    // UE4 selection weights (materials.cu:1724-1734) for a query direction of cosine qz
     __device__ __forceinline__ void diffSpecWeights(float qz, float vz, uint32_t wl, float *wd, float *ws) const {
         const float rough = a0;
         float eF = 0.5f * rough + 2 * rough * sq(qz);
@@ -389,11 +422,33 @@ struct Bsdf {
         *wd = c0.importance(wl) * sq(edf) * lerpf(1.0f, 1.0f / 1.51f, rough);
         *ws = lerpf(c1.importance(wl), 1.0f, om5);
     }
+    // reverse pdf of the UE4 lobes for the light direction l
+    __device__ __forceinline__ float diffSpecRevPDF(const Ggx &g, const V3 &v, const V3 &l, const V3 &m, float dotLH,
+                                                     uint32_t wl) const {
+        float revDiffuse = v.z / kPiF;
+        float revSpecular = g.pdf(l, m) / (4 * dotLH);
+        float rF_D90 = 0.5f * a0 + 2 * a0 * sq(l.z);
+        float rDiffFresnel = lerpf(1.0f, rF_D90, pow5f(1 - l.z));
+        float rDiffW = c0.importance(wl) * sq(rDiffFresnel) * lerpf(1.0f, 1.0f / 1.51f, a0);
+        float rSpecW = lerpf(c1.importance(wl), 1.0f, pow5f(1 - l.z));
+        return (revDiffuse * rDiffW + revSpecular * rSpecW) / (rDiffW + rSpecW);
+    }
+    // one wavelength of the rough-dielectric transmission term (materials.cu:1352-1371)
+    __device__ __forceinline__ float roughTransmission(const Spec<N> &eEnter, const Spec<N> &eExit,
+                                                       const Spec<N> &fEnter, const Spec<N> &fExit, const Ggx &g,
+                                                       const V3 &v, const V3 &l, bool entering, int i) const {
+        V3 m = normalize(-(eEnter[i] * v + eExit[i] * l) * (entering ? 1.0f : -1.0f));
+        float dotHV = dot(v, m), dotHL = dot(l, m);
+        float F = fresnelDielectric1(fEnter[i], fExit[i], dotHV);
+        float G = g.G1(v, m) * g.G1(l, m);
+        return fabsf(dotHV * dotHL) * (1 - F) * G * g.D(m) / sq(eEnter[i] * dotHV + eExit[i] * dotHL);
+    }
 
-    __device__ Spec<N> sampleInternal(const V3 &q, uint32_t wl, float uComponent, float uD0, float uD1, V3 *dir,
-                                      float *dirPDF, uint32_t *type) const {
+    __device__ Spec<N> sampleInternal(const V3 &q, uint32_t wl, bool importance, float uComponent, float uD0,
+                                      float uD1, V3 *dir, float *dirPDF, uint32_t *type, BsdfRev<N> *rev) const {
         *dirPDF = 0.0f;
         *type = 0;
+        if (rev) rev->pdf = 0.0f;
         switch (kind) {
         case BSDF_MATTE: {
             V3 d = cosineSampleHemisphere(uD0, uD1);
@@ -401,13 +456,23 @@ struct Bsdf {
             *type = dt::Reflection | dt::LowFreq;
             d.z *= q.z >= 0 ? 1.0f : -1.0f;
             *dir = d;
-            return c0 / kPiF;
+            Spec<N> fs = c0 / kPiF;
+            if (rev) {
+                rev->value = fs;
+                rev->pdf = fabsf(q.z) / kPiF;
+            }
+            return fs;
         }
         case BSDF_SPEC_BRDF: {
             *dir = V3(-q.x, -q.y, q.z);
             *dirPDF = 1.0f;
             *type = dt::Reflection | dt::Delta0D;
-            return c0 * fresnelConductor<N>(c1, c2, q.z) * (1.0f / fabsf(q.z));
+            Spec<N> ret = c0 * fresnelConductor<N>(c1, c2, q.z) * (1.0f / fabsf(q.z));
+            if (rev) {
+                rev->value = ret;
+                rev->pdf = 1.0f;
+            }
+            return ret;
         }
         case BSDF_SPEC_BSDF: {
             bool entering = q.z >= 0.0f;
@@ -422,7 +487,12 @@ struct Bsdf {
                 *dir = entering ? l : -l;
                 *dirPDF = reflectProb;
                 *type = dt::Reflection | dt::Delta0D;
-                return c0 * F * (1.0f / fabsf(v.z));
+                Spec<N> ret = c0 * F * (1.0f / fabsf(v.z));
+                if (rev) {
+                    rev->value = ret;
+                    rev->pdf = reflectProb;
+                }
+                return ret;
             }
             float sinEnter2 = 1.0f - v.z * v.z;
             float rr = eEnter.get(wl) / eExit.get(wl);
@@ -434,8 +504,16 @@ struct Bsdf {
             *dirPDF = 1.0f - reflectProb;
             *type = dt::Transmission | dt::Delta0D | (dispersive ? dt::Dispersive : 0u);
             Spec<N> ret(0.0f);
-            float squeeze = sq(eEnter.get(wl) / eExit.get(wl));
-            ret.set(wl, c0.get(wl) * (1.0f - F.get(wl)) * (squeeze / fabsf(cosExit)));
+            ret.set(wl, c0.get(wl) * (1.0f - F.get(wl)));
+            if (rev) {
+                float revSqueeze = importance ? sq(eExit.get(wl) / eEnter.get(wl)) : 1.0f;
+                Spec<N> revRet = ret;
+                revRet.set(wl, revRet.get(wl) * (revSqueeze / fabsf(q.z)));
+                rev->value = revRet;
+                rev->pdf = revSqueeze * *dirPDF;
+            }
+            float squeeze = importance ? 1.0f : sq(eEnter.get(wl) / eExit.get(wl));
+            ret.set(wl, ret.get(wl) * (squeeze / fabsf(cosExit)));
             *dirPDF *= squeeze;
             return ret;
         }
@@ -451,10 +529,16 @@ struct Bsdf {
             V3 l = 2 * dotHV * m - v;
             *dir = entering ? l : -l;
             if (l.z * v.z <= 0) return Spec<N>(0.0f);
-            *dirPDF = mPDF / (4 * dotHV);
+            float common = 1.0f / (4 * dotHV);
+            *dirPDF = common * mPDF;
             *type = dt::Reflection | dt::HighFreq;
             float G = g.G1(v, m) * g.G1(l, m);
-            return fresnelConductor<N>(c0, c1, dotHV) * D * G / (4 * v.z * l.z);
+            Spec<N> fs = fresnelConductor<N>(c0, c1, dotHV) * D * G / (4 * v.z * l.z);
+            if (rev) {
+                rev->value = fs;
+                rev->pdf = common * g.pdf(l, m);
+            }
+            return fs;
         }
         case BSDF_DIFFSPEC: {
             Ggx g(sq(a0), sq(a0), 0.0f);
@@ -496,29 +580,112 @@ struct Bsdf {
             Spec<N> diffuse = c0 * (fo * fi * lerpf(1.0f, 1.0f / 1.51f, a0) / kPiF);
             *dir = entering ? l : -l;
             *dirPDF = (pd * wd + ps * ws) / sumW;
-            return diffuse + specular;
+            Spec<N> ret = diffuse + specular;
+            if (rev) {
+                rev->value = ret;
+                rev->pdf = diffSpecRevPDF(g, v, l, m, dotLH, wl);
+            }
+            return ret;
+        }
+        case BSDF_MICRO_BSDF: {
+            // materials.cu:1270-1368
+            bool entering = q.z >= 0.0f;
+            const Spec<N> &eEnter = entering ? c1 : c2;
+            const Spec<N> &eExit = entering ? c2 : c1;
+            Ggx g(a0, a1, a2);
+            V3 v = entering ? q : -q;
+            V3 m;
+            float mPDF;
+            float D = g.sample(v, uD0, uD1, &m, &mPDF);
+            float dotHV = dot(v, m);
+            if (dotHV <= 0 || isnan(D)) return Spec<N>(0.0f);
+            Spec<N> F = fresnelDielectric<N>(eEnter, eExit, dotHV);
+            float reflectProb = F.importance(wl);
+            if (uComponent < reflectProb) {
+                V3 l = 2 * dotHV * m - v;
+                *dir = entering ? l : -l;
+                if (l.z * v.z <= 0) return Spec<N>(0.0f);
+                float common = reflectProb / (4 * dotHV);
+                *dirPDF = common * mPDF;
+                *type = dt::Reflection | dt::HighFreq;
+                float G = g.G1(v, m) * g.G1(l, m);
+                Spec<N> ret = c0 * F * D * G / (4 * v.z * l.z);
+                if (rev) {
+                    rev->value = ret;
+                    rev->pdf = common * g.pdf(l, m);
+                }
+                return ret;
+            }
+            const float eEw = eEnter.get(wl), eXw = eExit.get(wl);
+            float rr = eEw / eXw;
+            float innerRoot = 1 + rr * rr * (dotHV * dotHV - 1);
+            if (innerRoot < 0) return Spec<N>(0.0f);
+            V3 l = (rr * dotHV - sqrtf(innerRoot)) * m - rr * v;
+            *dir = entering ? l : -l;
+            if (l.z * v.z >= 0) return Spec<N>(0.0f);
+            float dotHL = dot(l, m);
+            float common = (1 - reflectProb) / sq(eEw * dotHV + eXw * dotHL);
+            *dirPDF = common * mPDF * sq(eXw) * fabsf(dotHL);
+            *type = dt::Transmission | dt::HighFreq;
+            Spec<N> ret;
+#pragma unroll
+            for (int i = 0; i < N; ++i) ret[i] = roughTransmission(eEnter, eExit, eEnter, eExit, g, v, l, entering, i);
+            ret = ret / fabsf(v.z * l.z);
+            ret *= c0;
+            ret *= importance ? eExit * eExit : eEnter * eEnter;
+            if (rev) {
+                rev->value = ret;
+                rev->pdf = common * g.pdf(l, m) * sq(eEw) * fabsf(dotHV);
+            }
+            return ret;
+        }
+        case BSDF_EDF_DIFFUSE: {
+            *dir = cosineSampleHemisphere(uD0, uD1);
+            *dirPDF = dir->z / kPiF;
+            *type = dt::Reflection | dt::LowFreq;
+            if (rev) rev->value = Spec<N>(0.0f);
+            return Spec<N>(1.0f / kPiF);
+        }
+        case BSDF_EDF_POINT: {
+            *dir = uniformSampleSphere(uD0, uD1);
+            *dirPDF = 1.0f / (4 * kPiF);
+            *type = dt::Reflection | dt::LowFreq;
+            if (rev) rev->value = Spec<N>(0.0f);
+            return Spec<N>(1.0f / (4 * kPiF));
+        }
+        case BSDF_EDF_ENV: {
+            float dx, dy;
+            concentricSampleDisk(uD0, uD1, &dx, &dy);
+            *dir = V3(a1 * dx, a1 * dy, 1.0f);
+            *dirPDF = 1.0f / a0;
+            *type = dt::Reflection | dt::LowFreq;
+            if (rev) rev->value = Spec<N>(0.0f);
+            return Spec<N>(1.0f / kPiF);
         }
         default:
             return Spec<N>(0.0f);
         }
     }
 
-    __device__ Spec<N> evaluateInternal(const V3 &q, const V3 &d) const {
+    __device__ Spec<N> evaluateInternal(const V3 &q, const V3 &d, uint32_t wl, bool importance,
+                                        Spec<N> *revValue) const {
+        Spec<N> ret(0.0f);
         switch (kind) {
         case BSDF_MATTE:
-            if (q.z * d.z <= 0.0f) return Spec<N>(0.0f);
-            return c0 / kPiF;
+            if (q.z * d.z > 0.0f) ret = c0 / kPiF;
+            break;
         case BSDF_MICRO_BRDF: {
             bool entering = q.z >= 0.0f;
             V3 v = entering ? q : -q;
             V3 l = entering ? d : -d;
             float nn = l.z * v.z;
-            if (nn <= 0) return Spec<N>(0.0f);
+            if (nn <= 0) return Spec<N>(0.0f);   // the reference leaves the reverse value unset here
             Ggx g(a0, a1, a2);
             V3 m = halfVector(v, l);
             float dotHV = dot(v, m);
             float G = g.G1(v, m) * g.G1(l, m);
-            return fresnelConductor<N>(c0, c1, dotHV) * g.D(m) * G / (4 * nn);
+            ret = fresnelConductor<N>(c0, c1, dotHV) * g.D(m) * G / (4 * nn);
+            break;
         }
         case BSDF_DIFFSPEC: {
             if (d.z * q.z <= 0) return Spec<N>(0.0f);
@@ -533,17 +700,52 @@ struct Bsdf {
             float FD90 = 0.5f * a0 + 2 * a0 * sq(dotLH);
             float fo = lerpf(1.0f, FD90, pow5f(1 - v.z));
             float fi = lerpf(1.0f, FD90, pow5f(1 - l.z));
-            return c0 * (fo * fi * lerpf(1.0f, 1.0f / 1.51f, a0) / kPiF) + specular;
+            ret = c0 * (fo * fi * lerpf(1.0f, 1.0f / 1.51f, a0) / kPiF) + specular;
+            break;
         }
+        case BSDF_MICRO_BSDF: {
+            bool entering = q.z >= 0.0f;
+            const Spec<N> &eEnter = entering ? c1 : c2;
+            const Spec<N> &eExit = entering ? c2 : c1;
+            Ggx g(a0, a1, a2);
+            V3 v = entering ? q : -q;
+            V3 l = entering ? d : -d;
+            float nn = l.z * v.z;
+            if (nn > 0) {
+                V3 m = halfVector(v, l);
+                float dotHV = dot(v, m);
+                Spec<N> F = fresnelDielectric<N>(eEnter, eExit, dotHV);
+                float G = g.G1(v, m) * g.G1(l, m);
+                ret = c0 * F * g.D(m) * G / (4 * nn);
+            } else if (nn < 0) {
+#pragma unroll
+                for (int i = 0; i < N; ++i) ret[i] = roughTransmission(eEnter, eExit, eEnter, eExit, g, v, l, entering, i);
+                ret = ret / fabsf(nn);
+                ret *= c0;
+                ret *= importance ? eExit * eExit : eEnter * eEnter;
+            }
+            break;
+        }
+        case BSDF_EDF_DIFFUSE:
+        case BSDF_EDF_ENV:
+            if (revValue) *revValue = Spec<N>(0.0f);
+            return Spec<N>(d.z > 0.0f ? 1.0f / kPiF : 0.0f);
+        case BSDF_EDF_POINT:
+            if (revValue) *revValue = Spec<N>(0.0f);
+            return Spec<N>(1.0f / (4 * kPiF));
         default:
-            return Spec<N>(0.0f);
+            break;
         }
+        if (revValue) *revValue = ret;
+        return ret;
     }
 
-    __device__ float evaluatePDFInternal(const V3 &q, const V3 &d, uint32_t wl) const {
+    __device__ float evaluatePDFInternal(const V3 &q, const V3 &d, uint32_t wl, float *revPDF) const {
+        if (revPDF) *revPDF = 0.0f;
         switch (kind) {
         case BSDF_MATTE:
             if (q.z * d.z <= 0.0f) return 0.0f;
+            if (revPDF) *revPDF = fabsf(q.z) / kPiF;
             return fabsf(d.z) / kPiF;
         case BSDF_MICRO_BRDF: {
             bool entering = q.z >= 0.0f;
@@ -554,7 +756,9 @@ struct Bsdf {
             float dotHV = dot(v, m);
             if (dotHV <= 0) return 0.0f;
             Ggx g(a0, a1, a2);
-            return g.pdf(v, m) / (4 * dotHV);
+            float common = 1.0f / (4 * dotHV);
+            if (revPDF) *revPDF = common * g.pdf(l, m);
+            return common * g.pdf(v, m);
         }
         case BSDF_DIFFSPEC: {
             Ggx g(sq(a0), sq(a0), 0.0f);
@@ -567,33 +771,76 @@ struct Bsdf {
             diffSpecWeights(q.z, v.z, wl, &wd, &ws);
             float pd = fmaxf(l.z / kPiF, 0.0f);
             float ps = g.pdf(v, m) / (4 * dotLH);
+            if (revPDF) *revPDF = diffSpecRevPDF(g, v, l, m, dotLH, wl);
             return (pd * wd + ps * ws) / (wd + ws);
         }
+        case BSDF_MICRO_BSDF: {
+            // materials.cu:1462-1510; the transmission half vector carries no sign flip for
+            // queries from inside, as in the reference
+            bool entering = q.z >= 0.0f;
+            const Spec<N> &eEnter = entering ? c1 : c2;
+            const Spec<N> &eExit = entering ? c2 : c1;
+            V3 v = entering ? q : -q;
+            V3 l = entering ? d : -d;
+            float nn = l.z * v.z;
+            if (nn == 0) return 0.0f;
+            const float eEw = eEnter.get(wl), eXw = eExit.get(wl);
+            V3 m = nn > 0 ? halfVector(v, l) : normalize(-(eEw * v + eXw * l));
+            float dotHV = dot(v, m);
+            if (dotHV <= 0) return 0.0f;
+            Ggx g(a0, a1, a2);
+            float mPDF = g.pdf(v, m);
+            Spec<N> F = fresnelDielectric<N>(eEnter, eExit, dotHV);
+            float reflectProb = F.importance(wl);
+            if (nn > 0) {
+                float common = reflectProb / (4 * dotHV);
+                if (revPDF) *revPDF = common * g.pdf(l, m);
+                return common * mPDF;
+            }
+            float dotHL = dot(l, m);
+            float common = (1 - reflectProb) / sq(eEw * dotHV + eXw * dotHL);
+            if (revPDF) *revPDF = common * g.pdf(l, m) * sq(eEw) * fabsf(dotHV);
+            return common * mPDF * sq(eXw) * fabsf(dotHL);
+        }
+        case BSDF_EDF_DIFFUSE:
+            return d.z <= 0.0f ? 0.0f : d.z / kPiF;
+        case BSDF_EDF_POINT:
+            return 1.0f / (4 * kPiF);
+        case BSDF_EDF_ENV:
+            return d.z <= 0.0f ? 0.0f : 1.0f / a0;
         default:
             return 0.0f;
         }
     }
 
-    // wrapper with the |ws.ns / ws.ng| correction, radiance transport
+    // wrappers with the shading-normal correction (renderer_common.h:107-115, 130-136, 224-264):
+    // |out.ns / out.ng| for radiance transport, |q.ns / q.ng| for importance transport
     __device__ Spec<N> sample(const V3 &q, const V3 &gnL, uint32_t wl, float uC, float u0, float u1, V3 *dir,
-                              float *pdf, uint32_t *type) const {
+                              float *pdf, uint32_t *type, bool importance = false, BsdfRev<N> *rev = nullptr) const {
         *pdf = 0.0f;
         *type = 0;
+        if (rev) rev->pdf = 0.0f;
         if (!matches(dt::All)) return Spec<N>(0.0f);
-        Spec<N> f = sampleInternal(q, wl, uC, u0, u1, dir, pdf, type);
+        Spec<N> f = sampleInternal(q, wl, importance, uC, u0, u1, dir, pdf, type, rev);
         if (*pdf == 0.0f) return Spec<N>(0.0f);
-        float sn = fabsf(dir->z / dot(*dir, gnL));
+        float sn = importance ? fabsf(q.z / dot(q, gnL)) : fabsf(dir->z / dot(*dir, gnL));
         if (!isfinite(sn)) return Spec<N>(0.0f);
+        if (rev) rev->value = rev->value * sn;
         return f * sn;
     }
-    __device__ Spec<N> evaluate(const V3 &q, const V3 &gnL, const V3 &d) const {
-        float sn = fabsf(d.z / dot(d, gnL));
+    __device__ Spec<N> evaluate(const V3 &q, const V3 &gnL, const V3 &d, uint32_t wl = 0, bool importance = false,
+                                Spec<N> *revValue = nullptr) const {
+        float sn = importance ? fabsf(q.z / dot(q, gnL)) : fabsf(d.z / dot(d, gnL));
+        if (revValue) *revValue = Spec<N>(0.0f);
         if (!isfinite(sn)) return Spec<N>(0.0f);
-        return evaluateInternal(q, d) * sn;
+        Spec<N> r = evaluateInternal(q, d, wl, importance, revValue) * sn;
+        if (revValue) *revValue = *revValue * sn;
+        return r;
     }
-    __device__ float evaluatePDF(const V3 &q, const V3 &d, uint32_t wl) const {
+    __device__ float evaluatePDF(const V3 &q, const V3 &d, uint32_t wl, float *revPDF = nullptr) const {
+        if (revPDF) *revPDF = 0.0f;
         if (!matches(dt::All)) return 0.0f;
-        return evaluatePDFInternal(q, d, wl);
+        return evaluatePDFInternal(q, d, wl, revPDF);
     }
 };
 
@@ -627,6 +874,16 @@ __device__ __forceinline__ Bsdf<N> setupBsdf(const DevScene &sc, const DevMateri
         b.kind = BSDF_MICRO_BRDF;
         b.c0 = evalSpectrum<N>(sc, m.spec[0], wls);
         b.c1 = evalSpectrum<N>(sc, m.spec[1], wls);
+        b.a0 = m.scalars[0];
+        b.a1 = m.scalars[1];
+        b.a2 = m.scalars[2];
+        break;
+    case MAT_MICROFACET_SCATTERING:
+        // alphaX, alphaY and rotation were derived on the host (materials.cu:1516-1536)
+        b.kind = BSDF_MICRO_BSDF;
+        b.c0 = evalSpectrum<N>(sc, m.spec[0], wls);
+        b.c1 = evalSpectrum<N>(sc, m.spec[1], wls);
+        b.c2 = evalSpectrum<N>(sc, m.spec[2], wls);
         b.a0 = m.scalars[0];
         b.a1 = m.scalars[1];
         b.a2 = m.scalars[2];

@@ -82,6 +82,7 @@ enum DevMaterialType : uint32_t {
     MAT_DIFFUSE_EMITTER,
     MAT_POINT_EMITTER,
     MAT_ENVIRONMENT_EMITTER,
+    MAT_MICROFACET_SCATTERING,
 };
 
 // A spectrum parameter: RGB build -> (r, g, b, 0); spectral build -> Meng coefficients

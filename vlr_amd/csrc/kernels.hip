@@ -1054,6 +1054,7 @@ struct BsdfQueryIn {
     float uComponent, uDir[2];
     float evalDir[3];
     int32_t single;
+    int32_t transportMode;
 };
 struct BsdfResultOut {
     float sampledDir[3];
@@ -1064,6 +1065,8 @@ struct BsdfResultOut {
     float evalPDF;
     float revValue[4];
     float revPDF;
+    float sampledRevValue[4];
+    float sampledRevPDF;
 };
 
 template <int N>
@@ -1075,26 +1078,33 @@ __global__ void k_bsdfQueries(DevScene sc, DevMaterial m, const BsdfQueryIn *q, 
     wls.offset = Q.wlOffset;
     wls.sel = min(static_cast<uint32_t>(static_cast<uint16_t>(N * Q.wlSelect)), static_cast<uint32_t>(N - 1));
     wls.single = Q.single != 0;
+    const bool importance = Q.transportMode != 0;
     Bsdf<N> b = setupBsdf<N>(sc, m, wls);
     V3 qd(Q.dirLocal[0], Q.dirLocal[1], Q.dirLocal[2]);
     V3 gn(Q.gnLocal[0], Q.gnLocal[1], Q.gnLocal[2]);
     V3 dir(0, 0, 0);
     float pdf;
     uint32_t type;
-    Spec<N> f = b.sample(qd, gn, wls.sel, Q.uComponent, Q.uDir[0], Q.uDir[1], &dir, &pdf, &type);
+    BsdfRev<N> srev;
+    srev.value = Spec<N>(0.0f);
+    Spec<N> f = b.sample(qd, gn, wls.sel, Q.uComponent, Q.uDir[0], Q.uDir[1], &dir, &pdf, &type, importance, &srev);
     BsdfResultOut o;
     o.sampledDir[0] = dir.x; o.sampledDir[1] = dir.y; o.sampledDir[2] = dir.z;
     o.sampledPDF = pdf;
     o.sampledType = type;
     V3 ed(Q.evalDir[0], Q.evalDir[1], Q.evalDir[2]);
-    Spec<N> e = b.evaluate(qd, gn, ed);
-    o.evalPDF = b.evaluatePDF(qd, ed, wls.sel);
+    Spec<N> rv;
+    Spec<N> e = b.evaluate(qd, gn, ed, wls.sel, importance, &rv);
+    float rp = 0.0f;
+    o.evalPDF = b.evaluatePDF(qd, ed, wls.sel, &rp);
+    o.revPDF = rp;
+    o.sampledRevPDF = pdf > 0.0f ? srev.pdf : 0.0f;
     for (int c = 0; c < 4; ++c) {
         o.sampledValue[c] = c < N ? f[c] : 0.0f;
         o.evalValue[c] = c < N ? e[c] : 0.0f;
-        o.revValue[c] = 0.0f;
+        o.revValue[c] = c < N ? rv[c] : 0.0f;
+        o.sampledRevValue[c] = (c < N && pdf > 0.0f) ? srev.value[c] : 0.0f;
     }
-    o.revPDF = 0.0f;
     out[i] = o;
 }
 

@@ -379,7 +379,8 @@ const std::vector<ParamInfo *> &SurfaceMaterial::params() const { return matSpec
 bool SurfaceMaterial::supportedOnRenderPath() const {
     switch (kind) {
     case MatKind::Matte: case MatKind::SpecularReflection: case MatKind::SpecularScattering:
-    case MatKind::MicrofacetReflection: case MatKind::UE4: case MatKind::DiffuseEmitter:
+    case MatKind::MicrofacetReflection: case MatKind::MicrofacetScattering: case MatKind::UE4:
+    case MatKind::DiffuseEmitter:
     case MatKind::PointEmitter: case MatKind::EnvironmentEmitter:
         return true;
     default:
@@ -517,6 +518,22 @@ DevMaterial SurfaceMaterial::toDevice(bool spectral, bool *usesEnvTexture, const
         d.type = MAT_MICROFACET_REFLECTION;
         d.spec[0] = spectrumOf(0, SpectrumType::IndexOfRefraction);
         d.spec[1] = spectrumOf(1, SpectrumType::IndexOfRefraction);
+        float r = tupleOf("roughness/anisotropy/rotation", 0, floatOf("roughness"));
+        float an = tupleOf("roughness/anisotropy/rotation", 1, floatOf("anisotropy"));
+        float rot = tupleOf("roughness/anisotropy/rotation", 2, floatOf("rotation"));
+        float alpha = r * r;
+        float aspect = std::sqrt(1.0f - 0.9f * an);
+        d.scalars[0] = std::fmax(0.001f, alpha / aspect);
+        d.scalars[1] = std::fmax(0.001f, alpha * aspect);
+        d.scalars[2] = 2 * kPiHost * rot;
+        break;
+    }
+    case MatKind::MicrofacetScattering: {
+        // materials.cu:1516-1536
+        d.type = MAT_MICROFACET_SCATTERING;
+        d.spec[0] = spectrumOf(0, SpectrumType::Reflectance);
+        d.spec[1] = spectrumOf(1, SpectrumType::IndexOfRefraction);
+        d.spec[2] = spectrumOf(2, SpectrumType::IndexOfRefraction);
         float r = tupleOf("roughness/anisotropy/rotation", 0, floatOf("roughness"));
         float an = tupleOf("roughness/anisotropy/rotation", 1, floatOf("anisotropy"));
         float rot = tupleOf("roughness/anisotropy/rotation", 2, floatOf("rotation"));
