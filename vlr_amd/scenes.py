@@ -242,6 +242,39 @@ def cornell_box(seed: int = 20261202, light_size: float = 0.25) -> SceneDesc:
     return SceneDesc(mats, [room, light, short_box, tall_box], insts, cam, None, False, "cornell_box")
 
 
+def cornell_many_lights(seed: int = 20261202, num_points: int = 64, num_quads: int = 8) -> SceneDesc:
+    """The Cornell box with its ceiling light plus `num_points` point lights of seeded intensity
+    U[1, 20] (one PointSurfaceNode, one PointEmitter material per point) and `num_quads`
+    0.2 x 0.2 ceiling quad lights of seeded colour."""
+    desc = cornell_box(seed)
+    rng = np.random.default_rng(seed + 7)
+    mats, meshes, insts = list(desc.materials), list(desc.meshes), list(desc.instances)
+    pts = np.zeros((num_points, 11), np.float32)
+    pts[:, 0] = rng.uniform(-1.2, 1.2, num_points)
+    pts[:, 1] = rng.uniform(0.3, 2.7, num_points)
+    pts[:, 2] = rng.uniform(-1.2, 1.2, num_points)
+    pts[:, 3:6] = (0.0, 1.0, 0.0)
+    pts[:, 6:9] = (1.0, 0.0, 0.0)
+    groups = []
+    for k in range(num_points):
+        v = float(rng.uniform(1.0, 20.0))
+        mats.append(MaterialDesc(POINT_EMITTER, {"intensity": (REC709, v, v, v)}, {"scale": 1.0}))
+        groups.append((np.array([k], np.uint32), len(mats) - 1))
+    meshes.append(MeshDesc("PointLights", pts, groups, points=True))
+    quad_v, quad_groups = [], []
+    for k in range(num_quads):
+        cx, cz = rng.uniform(-1.2, 1.2, 2)
+        col = rng.uniform(2.0, 8.0, 3)
+        mats.append(MaterialDesc(DIFFUSE_EMITTER, {"emittance": (REC709, *map(float, math.pi * col))},
+                                 {"scale": 1.0}))
+        quad_v.append(_quad((cx - 0.1, 2.985, cz - 0.1), (cx + 0.1, 2.985, cz - 0.1), (cx + 0.1, 2.985, cz + 0.1),
+                            (cx - 0.1, 2.985, cz + 0.1), (0, -1, 0), (1, 0, 0)))
+        quad_groups.append((quad_indices(1) + 4 * k, len(mats) - 1))
+    meshes.append(MeshDesc("QuadLights", np.concatenate(quad_v), quad_groups))
+    insts.append(InstanceDesc("Lights", identity4(), [len(meshes) - 2, len(meshes) - 1]))
+    return SceneDesc(mats, meshes, insts, desc.camera, None, False, "cornell_many_lights")
+
+
 # --------------------------------------------------------------------------------------------
 # Icosphere grid (spectral, GGX + dielectric, environment light)
 
