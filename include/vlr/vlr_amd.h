@@ -47,6 +47,8 @@ typedef struct VLRRenderStats {
     uint64_t launchesTail;
     /* rays traced by the per-depth kernels; closestRays - closestRaysWavefront went through the tail */
     uint64_t closestRaysWavefront, shadowRaysWavefront;
+    /* VLRRenderer_BPT: light-path pass (tracing and cache compaction) and eye-path pass */
+    double msLightPaths, msEyePaths;
 } VLRRenderStats;
 /* Counting the rays reads back one small counter buffer per render call. */
 VLR_API VLRResult vlrContextGetRenderStats(VLRContextConst context, VLRRenderStats* stats);
@@ -73,9 +75,18 @@ VLR_API VLRResult vlrContextGetSchedule(VLRContextConst context, uint32_t* wavef
 VLR_API VLRResult vlrContextGetQueueCounts(VLRContextConst context, uint32_t* closest, uint32_t* shadow,
                                            uint32_t maxDepth);
 
+/* Light paths per frame of VLRRenderer_BPT (OptiX::numLightPaths = 500,000, context.h:303),
+ * at most 1,048,576.  Changing it re-seeds the light-path RNG states. */
+VLR_API VLRResult vlrContextSetNumLightPaths(VLRContext context, uint32_t numLightPaths);
+VLR_API VLRResult vlrContextGetNumLightPaths(VLRContextConst context, uint32_t* numLightPaths);
+
 /* ---- progressive state (host pointers, width*height entries, row-major) ---- */
 VLR_API VLRResult vlrContextReadRNGStates(VLRContextConst context, uint64_t* states);
 VLR_API VLRResult vlrContextWriteRNGStates(VLRContext context, const uint64_t* states);
+/* PCG32 states of the BPT light paths (numLightPaths entries; linearRngBuffer,
+ * context.cpp:618-626 of the reference) */
+VLR_API VLRResult vlrContextReadLightRNGStates(VLRContextConst context, uint64_t* states);
+VLR_API VLRResult vlrContextWriteLightRNGStates(VLRContext context, const uint64_t* states);
 /* Kahan accumulation pairs: (sum, compensation) per channel, channels = 3 (RGB) or 16
  * (spectral strata); floats = width*height*channels*2. */
 VLR_API VLRResult vlrContextGetAccumulationChannels(VLRContextConst context, uint32_t* channels);

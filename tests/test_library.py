@@ -173,3 +173,10 @@ def test_knobs(ctx):
     b = C.c_bool()
     api.call("vlrContextGetSpectralRendering", ctx.h, C.byref(b))
     assert b.value
+    # light paths of VLRRenderer_BPT: 500,000 by default (context.h:303), at most 2^20
+    assert ctx.num_light_paths() == 500000
+    ctx.set_num_light_paths(921600)
+    assert ctx.num_light_paths() == 921600
+    assert code_of("vlrContextSetNumLightPaths", ctx.h, 0) == 3
+    assert code_of("vlrContextSetNumLightPaths", ctx.h, (1 << 20) + 1) == 3
+    assert ctx.num_light_paths() == 921600

@@ -48,6 +48,10 @@ class VLRImmediateSpectrum(C.Structure):
     _fields_ = [("colorSpace", C.c_char_p), ("e0", C.c_float), ("e1", C.c_float), ("e2", C.c_float)]
 
 
+# VLRRenderer (public_types.h:37-42)
+RENDERER_PATH_TRACING, RENDERER_LIGHT_TRACING, RENDERER_BPT, RENDERER_DEBUG = 0, 1, 2, 3
+
+
 class VLRShaderNodePlug(C.Structure):
     _fields_ = [("nodeRef", C.c_size_t), ("info", C.c_uint32)]
 
@@ -61,7 +65,7 @@ class VLRRenderStats(C.Structure):
                 ("trianglesClosest", C.c_uint64), ("instancesClosest", C.c_uint64), ("nodesShadow", C.c_uint64),
                 ("trianglesShadow", C.c_uint64), ("instancesShadow", C.c_uint64), ("msTail", C.c_double),
                 ("launchesTail", C.c_uint64), ("closestRaysWavefront", C.c_uint64),
-                ("shadowRaysWavefront", C.c_uint64)]
+                ("shadowRaysWavefront", C.c_uint64), ("msLightPaths", C.c_double), ("msEyePaths", C.c_double)]
 
 
 RAY_DTYPE = np.dtype([("org", np.float32, 3), ("tmin", np.float32), ("dir", np.float32, 3), ("tmax", np.float32)])
@@ -189,6 +193,10 @@ PROTOTYPES: Dict[str, list] = {
     "vlrContextGetSchedule": [H, C.POINTER(U32), C.POINTER(U32), C.POINTER(B)],
     "vlrContextReadRNGStates": [H, P],
     "vlrContextWriteRNGStates": [H, P],
+    "vlrContextReadLightRNGStates": [H, P],
+    "vlrContextWriteLightRNGStates": [H, P],
+    "vlrContextSetNumLightPaths": [H, U32],
+    "vlrContextGetNumLightPaths": [H, C.POINTER(U32)],
     "vlrContextGetAccumulationChannels": [H, C.POINTER(U32)],
     "vlrContextReadAccumulation": [H, P],
     "vlrContextWriteAccumulation": [H, P, U32],
@@ -301,6 +309,28 @@ class Context:
     def write_rng(self, states: np.ndarray):
         s = np.ascontiguousarray(states, np.uint64)
         call("vlrContextWriteRNGStates", self.h, s.ctypes.data)
+
+    # ---- LVC-BPT ----
+    def set_renderer(self, renderer: int):
+        call("vlrContextSetRenderer", self.h, renderer)
+
+    def set_num_light_paths(self, n: int):
+        call("vlrContextSetNumLightPaths", self.h, n)
+
+    def num_light_paths(self) -> int:
+        n = U32()
+        call("vlrContextGetNumLightPaths", self.h, C.byref(n))
+        return n.value
+
+    def read_light_rng(self) -> np.ndarray:
+        out = np.zeros(self.num_light_paths(), np.uint64)
+        call("vlrContextReadLightRNGStates", self.h, out.ctypes.data)
+        return out
+
+    def write_light_rng(self, states: np.ndarray):
+        s = np.ascontiguousarray(states, np.uint64)
+        assert s.size == self.num_light_paths()
+        call("vlrContextWriteLightRNGStates", self.h, s.ctypes.data)
 
     def read_accumulation(self) -> np.ndarray:
         ch = U32()

@@ -1064,6 +1064,8 @@ VLR_API VLRResult vlrContextGetRenderStats(VLRContextConst context, VLRRenderSta
     stats->launchesShade = c->launches[STAGE_SHADE];
     stats->msTail = c->stageMs[STAGE_TAIL];
     stats->launchesTail = c->launches[STAGE_TAIL];
+    stats->msLightPaths = c->stageMs[STAGE_LIGHT_PATHS];
+    stats->msEyePaths = c->stageMs[STAGE_EYE_PATHS];
     return VLRResult_NoError;
     VLRA_CATCH
 }
@@ -1152,6 +1154,37 @@ VLR_API VLRResult vlrContextWriteRNGStates(VLRContext context, const uint64_t *s
     if (!states || !c->width) return VLRResult_InvalidArgument;
     VLRA_HIP(hipDeviceSynchronize());
     VLRA_HIP(hipMemcpy(c->rng.ptr, states, size_t(c->width) * c->height * 8, hipMemcpyHostToDevice));
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+VLR_API VLRResult vlrContextReadLightRNGStates(VLRContextConst context, uint64_t *states) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    if (!states) return VLRResult_InvalidArgument;
+    CTX(context)->readLightRngStates(states);
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+VLR_API VLRResult vlrContextWriteLightRNGStates(VLRContext context, const uint64_t *states) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    if (!states) return VLRResult_InvalidArgument;
+    CTX(context)->writeLightRngStates(states);
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+VLR_API VLRResult vlrContextSetNumLightPaths(VLRContext context, uint32_t numLightPaths) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    CTX(context)->setNumLightPaths(numLightPaths);
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+VLR_API VLRResult vlrContextGetNumLightPaths(VLRContextConst context, uint32_t *numLightPaths) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    if (!numLightPaths) return VLRResult_InvalidArgument;
+    *numLightPaths = CTX(context)->numLightPaths;
     return VLRResult_NoError;
     VLRA_CATCH
 }

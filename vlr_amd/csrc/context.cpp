@@ -124,7 +124,9 @@ void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCo
     if (!camera) throw ApiError{3, "camera is null"};
     if (!camera->perspective) throw ApiError{4, "only the perspective camera is on the render path"};
     if (width == 0 || height == 0) throw ApiError{3, "bind an output buffer first"};
-    if (renderer != 0) throw ApiError{3, "renderer not available on this build: only VLRRenderer_PathTracing"};
+    if (renderer != VLRRenderer_PathTracing && renderer != VLRRenderer_BPT)
+        throw ApiError{3, "renderer not available on this build: VLRRenderer_PathTracing and VLRRenderer_BPT"};
+    const bool bpt = renderer == VLRRenderer_BPT;
     if (shrinkCoeff == 0) shrinkCoeff = 1;
     if (firstFrame) {
         if (spectralActive != spectralRequested) {
@@ -149,6 +151,11 @@ void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCo
         fp.traversalStats = traversalStats ? 1 : 0;
         fp.wavefrontDepth = wavefrontDepth;
         fp.tailBlocks = tailBlocks;
+        if (bpt) {
+            renderBptFrame(stream, fp, firstFrame);
+            if (numAccum) *numAccum = numAccumFrames;
+            return;
+        }
         // autotune the schedule per (scene version, size, mode, path length)
         const uint64_t tuneKey = (version * 1000003ull) ^ (uint64_t(width) << 40) ^ (uint64_t(height) << 20) ^
                                  (uint64_t(fp.imageW) << 8) ^ (spectralActive ? 1u : 0u) ^
@@ -172,23 +179,7 @@ void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCo
             }
             VLRA_HIP(hipEventRecord(tuneEvents[0], stream));
         }
-        FrameBuffers fb;
-        fb.ps.rng = static_cast<uint64_t *>(rng.ptr);
-        fb.ps.alpha = static_cast<float4 *>(alpha.ptr);
-        fb.ps.contrib = static_cast<float4 *>(contrib.ptr);
-        fb.ps.meta = static_cast<float4 *>(meta.ptr);
-        for (int k = 0; k < 2; ++k) {
-            fb.rays[k].org = static_cast<float4 *>(rayOrg[k].ptr);
-            fb.rays[k].dir = static_cast<float4 *>(rayDir[k].ptr);
-        }
-        fb.hits.a = static_cast<int4 *>(hitA.ptr);
-        fb.hits.b = static_cast<float2 *>(hitB.ptr);
-        fb.shadow.org = static_cast<float4 *>(shOrg.ptr);
-        fb.shadow.dir = static_cast<float4 *>(shDir.ptr);
-        fb.shadow.value = static_cast<float4 *>(shVal.ptr);
-        fb.counters = static_cast<uint32_t *>(counters.ptr);
-        fb.accum = static_cast<float *>(accum.ptr);
-        fb.output = static_cast<float4 *>(output.ptr);
+        const FrameBuffers fb = frameBuffers();
         if (kernelTiming) {
             EventTimer t(stream, eventPool);
             VLRA_HIP(launchFrame(gpu.dev, fp, fb, spectralActive, stream, &t));
@@ -208,6 +199,109 @@ void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCo
         pathsStarted += uint64_t(fp.imageW) * fp.imageH;
     }
     if (numAccum) *numAccum = numAccumFrames;
+}
+
+FrameBuffers Context::frameBuffers() {
+    FrameBuffers fb;
+    fb.ps.rng = static_cast<uint64_t *>(rng.ptr);
+    fb.ps.alpha = static_cast<float4 *>(alpha.ptr);
+    fb.ps.contrib = static_cast<float4 *>(contrib.ptr);
+    fb.ps.meta = static_cast<float4 *>(meta.ptr);
+    for (int k = 0; k < 2; ++k) {
+        fb.rays[k].org = static_cast<float4 *>(rayOrg[k].ptr);
+        fb.rays[k].dir = static_cast<float4 *>(rayDir[k].ptr);
+    }
+    fb.hits.a = static_cast<int4 *>(hitA.ptr);
+    fb.hits.b = static_cast<float2 *>(hitB.ptr);
+    fb.shadow.org = static_cast<float4 *>(shOrg.ptr);
+    fb.shadow.dir = static_cast<float4 *>(shDir.ptr);
+    fb.shadow.value = static_cast<float4 *>(shVal.ptr);
+    fb.counters = static_cast<uint32_t *>(counters.ptr);
+    fb.accum = static_cast<float *>(accum.ptr);
+    fb.output = static_cast<float4 *>(output.ptr);
+    return fb;
+}
+
+// ---- LVC-BPT (context.cpp:1444-1481 of the reference) ----
+void Context::setNumLightPaths(uint32_t n) {
+    if (n == 0 || n > kMaxLightPaths) throw ApiError{3, "number of light paths must be in [1, 1048576]"};
+    numLightPaths = n;
+}
+
+void Context::ensureBpt(uint32_t slotsPerPath) {
+    ensureDevice();
+    if (lightRngPaths != numLightPaths) {
+        // the light-path RNG states are seeded once, as the reference does at context creation
+        std::vector<uint64_t> seeds(numLightPaths);
+        std::mt19937_64 seedRng(459182033132123413ULL);
+        for (auto &v : seeds) v = seedRng();
+        lightRng.upload(seeds.data(), seeds.size() * sizeof(uint64_t));
+        lightRngPaths = numLightPaths;
+    }
+    const size_t vertices = size_t(numLightPaths) * slotsPerPath;
+    lvSlots.reserve(vertices * kLightVertexFloat4s * sizeof(float4));
+    lvCache.reserve(vertices * kLightVertexFloat4s * sizeof(float4));
+    lvCounts.reserve(size_t(numLightPaths) * sizeof(uint32_t));
+    lvOffsets.reserve(size_t(numLightPaths) * sizeof(uint32_t));
+    lvBlockSums.reserve(1024 * sizeof(uint32_t));
+    lvNum.reserve(sizeof(uint32_t));
+    const size_t splatBytes = size_t(width) * height * accumChannels * sizeof(float);
+    void *before = splat.ptr;
+    splat.reserve(splatBytes);
+    if (splat.ptr != before || splatLayout != splatBytes) {
+        VLRA_HIP(hipMemset(splat.ptr, 0, splatBytes));
+        splatLayout = splatBytes;
+    }
+}
+
+void Context::renderBptFrame(hipStream_t stream, const FrameParams &fp, bool firstFrame) {
+    const uint32_t slotsPerPath = fp.maxPathLength + 1;
+    ensureBpt(slotsPerPath);
+    if (firstFrame)
+        VLRA_HIP(hipMemsetAsync(splat.ptr, 0, size_t(width) * height * accumChannels * sizeof(float), stream));
+    // common wavelengths of the frame: two draws, offset first (WavelengthSamples::
+    // createWithEqualOffsets, spectrum_types.h)
+    std::uniform_real_distribution<float> u01;
+    const float wlOffset = u01(bptWlRng);
+    const float wlSelect = u01(bptWlRng);
+    const uint32_t N = spectralActive ? 4 : 3;
+    BptParams bp;
+    bp.wlOffset = wlOffset;
+    bp.wlSel = std::min<uint32_t>(static_cast<uint16_t>(N * wlSelect), N - 1);
+    bp.wlProb = spectralActive ? 4 / (830.0f - 360.0f) : 1.0f;
+    bp.numLightPaths = numLightPaths;
+    bp.slotsPerPath = slotsPerPath;
+    bp.slots = static_cast<float4 *>(lvSlots.ptr);
+    bp.slotCounts = static_cast<uint32_t *>(lvCounts.ptr);
+    bp.slotOffsets = static_cast<uint32_t *>(lvOffsets.ptr);
+    bp.blockSums = static_cast<uint32_t *>(lvBlockSums.ptr);
+    bp.cache = static_cast<float4 *>(lvCache.ptr);
+    bp.numVertices = static_cast<uint32_t *>(lvNum.ptr);
+    bp.lightRng = static_cast<uint64_t *>(lightRng.ptr);
+    bp.splat = static_cast<float *>(splat.ptr);
+    const FrameBuffers fb = frameBuffers();
+    if (kernelTiming) {
+        EventTimer t(stream, eventPool);
+        VLRA_HIP(launchBptFrame(gpu.dev, fp, fb, bp, spectralActive, stream, &t));
+        t.collect(stageMs, launches);
+    } else {
+        VLRA_HIP(launchBptFrame(gpu.dev, fp, fb, bp, spectralActive, stream, nullptr));
+    }
+    lastFrame = fp;
+    ++framesRendered;
+    pathsStarted += uint64_t(fp.imageW) * fp.imageH;
+}
+
+void Context::readLightRngStates(uint64_t *dst) {
+    ensureBpt(std::min<uint32_t>(maxPathLength, kMaxDepthCounters - 1) + 1);
+    VLRA_HIP(hipDeviceSynchronize());
+    VLRA_HIP(hipMemcpy(dst, lightRng.ptr, size_t(numLightPaths) * 8, hipMemcpyDeviceToHost));
+}
+
+void Context::writeLightRngStates(const uint64_t *src) {
+    ensureBpt(std::min<uint32_t>(maxPathLength, kMaxDepthCounters - 1) + 1);
+    VLRA_HIP(hipDeviceSynchronize());
+    VLRA_HIP(hipMemcpy(lightRng.ptr, src, size_t(numLightPaths) * 8, hipMemcpyHostToDevice));
 }
 
 void Context::readOutput(float *dst) {

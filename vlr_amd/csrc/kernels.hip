@@ -323,6 +323,10 @@ struct SurfPt {
     Frame frame;
     float texU, texV;
     bool isPoint, atInfinity;
+    // hit identity and surface parameters (barycentrics b0, b1; phi, theta at infinity)
+    uint32_t instIndex, geomInstIndex, primIndex;
+    float u, v;
+    __device__ __forceinline__ float cosTerm(const V3 &w) const { return isPoint ? 1.0f : absDot(w, gn); }
 };
 
 // triangle.cu:6-53 + renderer_common.h:609-626: decode in object space, then to world
@@ -358,6 +362,11 @@ __device__ __forceinline__ void decodeHit(const DevScene &sc, int32_t instIdx, i
     sp->gn = normalize(xfmNormal(I.w2o, gn));
     sp->isPoint = false;
     sp->atInfinity = false;
+    sp->instIndex = static_cast<uint32_t>(instIdx);
+    sp->geomInstIndex = static_cast<uint32_t>(giIdx);
+    sp->primIndex = static_cast<uint32_t>(prim);
+    sp->u = b0;
+    sp->v = b1;
 }
 
 // environment surface point from a direction (path_tracing.cu:282-310)
@@ -374,6 +383,9 @@ __device__ __forceinline__ void envPointFromDir(const DevScene &sc, const V3 &di
     sp->gn = -direction;
     sp->isPoint = false;
     sp->atInfinity = true;
+    sp->instIndex = sp->geomInstIndex = sp->primIndex = 0;
+    sp->u = phi;
+    sp->v = theta;
     phi += sc.envRotation;
     phi = phi - floorf(phi / (2 * kPiF)) * 2 * kPiF;
     sp->texU = phi / (2 * kPiF);
@@ -417,6 +429,9 @@ __device__ __forceinline__ void sampleLight(const DevScene &sc, float uLight, fl
         ls->sp.atInfinity = true;
         ls->sp.texU = phi / (2 * kPiF);
         ls->sp.texV = theta / kPiF;
+        ls->sp.instIndex = ls->sp.geomInstIndex = ls->sp.primIndex = 0;
+        ls->sp.u = posPhi;
+        ls->sp.v = theta;
         ls->areaPDF = uvPDF / (2 * kPiF * kPiF * sinf(theta));
         ls->posType = dt::Reflection | dt::LowFreq;
         ls->material = sc.envMaterial;
@@ -431,6 +446,10 @@ __device__ __forceinline__ void sampleLight(const DevScene &sc, float uLight, fl
     float primProb;
     uint32_t prim = sampleDiscrete(sc.distPool, G.distBase, G.numPrims, uPrim, &primProb, nullptr);
     ls->material = G.material;
+    ls->sp.instIndex = li - 1;
+    ls->sp.geomInstIndex = gi;
+    ls->sp.primIndex = prim;
+    ls->sp.u = ls->sp.v = 0.0f;
     if (G.isPoints) {
         const DevVertex &v = sc.vertices[G.vertexBase + sc.indices[G.indexBase + prim]];
         ls->areaPDF = primProb / (4 * kPiF);
@@ -474,6 +493,8 @@ __device__ __forceinline__ void sampleLight(const DevScene &sc, float uLight, fl
     ls->sp.gn = gn;
     ls->sp.isPoint = false;
     ls->sp.atInfinity = false;
+    ls->sp.u = b0;
+    ls->sp.v = b1;
 }
 
 // EDF of an emitter material at a surface point (materials.cu:2040-2096, 2194-2250, 2830-2896):
@@ -1022,6 +1043,841 @@ __global__ void k_frameEnd(uint32_t *counters, uint32_t maxPathLength, uint32_t 
     w64[1] += wfShadow;
 }
 
+// =============================================================================================
+// LVC-BPT (lvc_bpt.cu:144-971).  Per frame: light paths (one thread each) write their vertices
+// into fixed per-path slots; a prefix sum over the per-path counts compacts them into a cache
+// ordered by light path and depth; eye paths (one thread per pixel) connect each vertex,
+// including the lens vertex, to one uniformly chosen cache vertex, lens connections being
+// splatted to the pixel they project to; the eye contributions and the splats are then added
+// to the Kahan accumulation.
+constexpr int kLVF4 = kLightVertexFloat4s;
+
+struct BptArgs {
+    DevScene sc;
+    FrameParams fp;
+    FrameBuffers fb;
+    BptParams bp;
+};
+
+// a: inst, geomInst, prim, flags | b: u, v, probDensity, prevProbDensity
+// c: secondPrevPartialDenomMisWeight, secondPrevProbRatioToFirst, backwardConversionFactor, dirIn.x
+// d: dirIn.y, dirIn.z, 0, 0 | e: flux
+constexpr uint32_t kLVDelta = 1u << 2, kLVPrevDelta = 1u << 3, kLVWlSel = 1u << 4;
+struct LVert {
+    uint32_t kind, inst, geomInst, prim, pathLength;
+    bool delta, prevDelta, wlSelected;
+    float u, v, probDensity, prevProbDensity, spdmw, sprtf, bcf;
+    V3 dirIn;
+    float4 flux;
+};
+
+template <int N>
+__device__ __forceinline__ void storeLV(float4 *dst, const SurfPt &sp, const Spec<N> &flux, const V3 &dirIn,
+                                        float bcf, float pd, float ppd, float spdmw, float sprtf, bool delta,
+                                        bool prevDelta, bool wlSelected, uint32_t pathLength) {
+    uint32_t kind = sp.atInfinity ? 2u : (sp.isPoint ? 1u : 0u);
+    uint32_t flags = kind | (delta ? kLVDelta : 0u) | (prevDelta ? kLVPrevDelta : 0u) | (wlSelected ? kLVWlSel : 0u) |
+                     (pathLength << 8);
+    dst[0] = make_float4(__uint_as_float(sp.instIndex), __uint_as_float(sp.geomInstIndex),
+                         __uint_as_float(sp.primIndex), __uint_as_float(flags));
+    dst[1] = make_float4(sp.u, sp.v, pd, ppd);
+    dst[2] = make_float4(spdmw, sprtf, bcf, dirIn.x);
+    dst[3] = make_float4(dirIn.y, dirIn.z, 0.0f, 0.0f);
+    dst[4] = storeSpec<N>(flux);
+}
+
+__device__ __forceinline__ LVert loadLV(const float4 *src) {
+    float4 a = src[0], b = src[1], c = src[2], d = src[3];
+    LVert r;
+    uint32_t flags = __float_as_uint(a.w);
+    r.kind = flags & 3u;
+    r.delta = (flags & kLVDelta) != 0;
+    r.prevDelta = (flags & kLVPrevDelta) != 0;
+    r.wlSelected = (flags & kLVWlSel) != 0;
+    r.pathLength = flags >> 8;
+    r.inst = __float_as_uint(a.x);
+    r.geomInst = __float_as_uint(a.y);
+    r.prim = __float_as_uint(a.z);
+    r.u = b.x; r.v = b.y; r.probDensity = b.z; r.prevProbDensity = b.w;
+    r.spdmw = c.x; r.sprtf = c.y; r.bcf = c.z;
+    r.dirIn = V3(c.w, d.x, d.y);
+    r.flux = src[4];
+    return r;
+}
+
+// decodeHitPointForTriangle (triangle.cu:55-109), ForPoint (point.cu:6-35), ForInfiniteSphere
+// (infinite_sphere.cu:17-56); returns the material index
+__device__ __forceinline__ uint32_t decodeLightVertex(const DevScene &sc, const LVert &lv, SurfPt *sp) {
+    sp->instIndex = lv.inst;
+    sp->geomInstIndex = lv.geomInst;
+    sp->primIndex = lv.prim;
+    if (lv.kind == 2) {
+        float posPhi = lv.u, theta = lv.v;
+        float phi = posPhi + sc.envRotation;
+        phi = phi - floorf(phi / (2 * kPiF)) * 2 * kPiF;
+        V3 direction = fromPolarYUp(posPhi, theta);
+        float sinPhi, cosPhi;
+        sincosf(posPhi, &sinPhi, &cosPhi);
+        V3 tc0 = normalize(V3(-cosPhi, 0.0f, -sinPhi));
+        V3 gn = -direction;
+        sp->frame.x = tc0;
+        sp->frame.z = gn;
+        sp->frame.y = cross(sp->frame.z, sp->frame.x);
+        sp->position = direction;
+        sp->gn = gn;
+        sp->isPoint = false;
+        sp->atInfinity = true;
+        sp->u = posPhi;
+        sp->v = theta;
+        sp->texU = phi / (2 * kPiF);
+        sp->texV = theta / kPiF;
+        return sc.envMaterial;
+    }
+    const DevInstance &I = sc.instances[lv.inst];
+    const DevGeomInst &G = sc.geomInsts[lv.geomInst];
+    sp->atInfinity = false;
+    if (lv.kind == 1) {
+        const DevVertex &vx = sc.vertices[G.vertexBase + sc.indices[G.indexBase + lv.prim]];
+        sp->position = xfmPoint(I.o2w, ld3(vx.p));
+        V3 sn = normalize(xfmNormal(I.w2o, ld3(vx.n)));
+        V3 tc0 = normalize(xfmVector(I.o2w, ld3(vx.t)));
+        sp->frame = Frame(tc0, sn);
+        sp->isPoint = true;
+        sp->gn = sn;
+        sp->u = lv.u;
+        sp->v = lv.v;
+        sp->texU = sp->texV = 0.0f;
+        return G.material;
+    }
+    const uint32_t *ix = sc.indices + G.indexBase + 3 * lv.prim;
+    const DevVertex &v0 = sc.vertices[G.vertexBase + ix[0]];
+    const DevVertex &v1 = sc.vertices[G.vertexBase + ix[1]];
+    const DevVertex &v2 = sc.vertices[G.vertexBase + ix[2]];
+    V3 e1 = xfmVector(I.o2w, ld3(v1.p) - ld3(v0.p));
+    V3 e2 = xfmVector(I.o2w, ld3(v2.p) - ld3(v0.p));
+    V3 gn = cross(e1, e2);
+    float area = gn.length() / 2;
+    gn = gn / (2 * area);
+    float b0 = lv.u, b1 = lv.v, b2 = 1.0f - lv.u - lv.v;
+    V3 position = b0 * ld3(v0.p) + b1 * ld3(v1.p) + b2 * ld3(v2.p);
+    V3 sn = b0 * ld3(v0.n) + b1 * ld3(v1.n) + b2 * ld3(v2.n);
+    V3 tc0 = b0 * ld3(v0.t) + b1 * ld3(v1.t) + b2 * ld3(v2.t);
+    sp->texU = b0 * v0.u + b1 * v1.u + b2 * v2.u;
+    sp->texV = b0 * v0.v + b1 * v1.v + b2 * v2.v;
+    position = xfmPoint(I.o2w, position);
+    sn = normalize(xfmNormal(I.w2o, sn));
+    tc0 = xfmVector(I.o2w, tc0);
+    if (!sn.allFinite() || !tc0.allFinite()) {
+        V3 bt;
+        sn = gn;
+        makeCoordinateSystem(sn, &tc0, &bt);
+    }
+    float dotNT = dot(sn, tc0);
+    tc0 = normalize(tc0 - dotNT * sn);
+    sp->position = position;
+    sp->frame = Frame(tc0, sn);
+    sp->isPoint = false;
+    sp->gn = gn;
+    sp->u = b0;
+    sp->v = b1;
+    return G.material;
+}
+
+// computeRRProbability (lvc_bpt.cu:137-141)
+template <int N>
+__device__ __forceinline__ float rrProbability(const Spec<N> &fs, const V3 &dirLocal, float dirDensity, const V3 &gnLocal,
+                                               uint32_t sel) {
+    Spec<N> localThroughput = fs * absDot(dirLocal, gnLocal) / dirDensity;
+    return fminf(localThroughput.importance(sel), 1.0f);
+}
+
+// EDF directional sampling and pdf (materials.cu:2052-2096, 2206-2229, 2849-2880)
+__device__ __forceinline__ float edfPDF(uint32_t type, const V3 &d, float worldDiscArea) {
+    switch (type) {
+    case MAT_DIFFUSE_EMITTER: return d.z <= 0.0f ? 0.0f : d.z / kPiF;
+    case MAT_POINT_EMITTER: return 1.0f / (4 * kPiF);
+    case MAT_ENVIRONMENT_EMITTER: return d.z <= 0.0f ? 0.0f : 1.0f / worldDiscArea;
+    default: return 0.0f;
+    }
+}
+
+// The emitter's directional distribution used as the BSDF of a light-source vertex
+template <int N>
+__device__ __forceinline__ Bsdf<N> edfAsBsdf(const DevScene &sc, uint32_t type) {
+    Bsdf<N> b;
+    b.kind = type == MAT_DIFFUSE_EMITTER ? BSDF_EDF_DIFFUSE
+           : type == MAT_POINT_EMITTER ? BSDF_EDF_POINT
+           : type == MAT_ENVIRONMENT_EMITTER ? BSDF_EDF_ENV : BSDF_NULL;
+    b.a0 = sc.worldDiscArea;
+    b.a1 = sc.worldRadius;
+    b.a2 = 0.0f;
+    b.dispersive = false;
+    return b;
+}
+
+// PerspectiveCameraIDF (cameras.cu:124-211)
+struct LensIdf {
+    float xOnLens, yOnLens, opWidth, opHeight, imgPlaneDistance, objPlaneDistance, imgPlaneArea;
+    __device__ __forceinline__ bool project(const V3 &d, float *u0, float *u1) const {
+        if (d.z <= 0.0f) return false;
+        float objPlaneZ = objPlaneDistance / d.z;
+        V3 pFocus = d * objPlaneZ + V3(xOnLens, yOnLens, 0.0f);
+        *u0 = -pFocus.x / opWidth + 0.5f;
+        *u1 = -pFocus.y / opHeight + 0.5f;
+        return !(*u0 < 0.0f || *u0 >= 1.0f || *u1 < 0.0f || *u1 >= 1.0f);
+    }
+    __device__ __forceinline__ float pdf(const V3 &d) const {
+        float a, b;
+        if (!project(d, &a, &b)) return 0.0f;
+        return sq(imgPlaneDistance) / (cube(d.z) * imgPlaneArea);
+    }
+};
+
+// testVisibility (light_transport_common.h:147-177) returning the connection direction
+__device__ __forceinline__ bool connectVisible(const DevScene &sc, const SurfPt &shading, const SurfPt &light, V3 *dir,
+                                               float *dist2, int32_t *lds, int lane) {
+    if (light.atInfinity) {
+        *dist2 = 1.0f;
+        *dir = normalize(light.position);
+    } else {
+        V3 dv = light.position - shading.position;
+        *dist2 = dv.sqLength();
+        *dir = dv / sqrtf(*dist2);
+    }
+    bool front = dot(shading.gn, *dir) > 0;
+    V3 so = offsetRayOrigin(shading.position, front ? shading.gn : -shading.gn);
+    front = -dot(light.gn, *dir) > 0;
+    V3 lo = offsetRayOrigin(light.position, front ? light.gn : -light.gn);
+    float tmax = 3.402823466e+38f;
+    if (!light.atInfinity) tmax = (lo - so).length() * 0.9999f;
+    TraceHit h;
+    return !traceRay<true>(sc, so, *dir, 0.0f, tmax, &h, lds, lane);
+}
+
+template <int N>
+__device__ __forceinline__ Wls<N> commonWls(const BptParams &bp, bool single) {
+    Wls<N> w;
+    w.offset = bp.wlOffset;
+    w.sel = bp.wlSel;
+    w.single = single;
+    return w;
+}
+
+// ---- light subpaths (lvc_bpt.cu:144-334) ----
+template <int N>
+__global__ void __launch_bounds__(kTraceBlock) k_bptLight(BptArgs a) {
+    __shared__ int32_t lds[kLdsStack * kTraceBlock];
+    const DevScene &sc = a.sc;
+    const BptParams &bp = a.bp;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t closestRays = 0;
+    if (i < bp.numLightPaths) {
+        float4 *slots = bp.slots + static_cast<size_t>(i) * bp.slotsPerPath * kLVF4;
+        uint32_t written = 0;
+        auto store = [&](const SurfPt &sp, const Spec<N> &flux, const V3 &dirIn, float bcf, float pd, float ppd,
+                         float spdmw, float sprtf, bool delta, bool prevDelta, bool wlSel, uint32_t pl) {
+            if (written < bp.slotsPerPath)
+                storeLV<N>(slots + written * kLVF4, sp, flux, dirIn, bcf, pd, ppd, spdmw, sprtf, delta, prevDelta,
+                           wlSel, pl);
+            ++written;
+        };
+        Pcg32 rng{bp.lightRng[i]};
+        const Wls<N> wlsC = commonWls<N>(bp, false);
+        if (sc.lightInstIntegral > 0.0f) {
+            float uLight = rng.uniform();
+            float uP0 = rng.uniform(), uP1 = rng.uniform();
+            LightSample ls;
+            float lightProb;
+            sampleLight(sc, uLight, uP0, uP1, &ls, &lightProb);
+            const DevMaterial &lm = sc.materials[ls.material];
+            Spec<N> M;
+            float fe;
+            evalEDF<N>(sc, lm, ls.sp, wlsC, V3(0, 0, 1), &M, &fe);
+            float probDensity0 = bp.numLightPaths * lightProb * ls.areaPDF;
+            Spec<N> alpha = M / probDensity0;
+            store(ls.sp, alpha, V3(0, 0, 1), 0.0f, probDensity0, 1.0f, 0.0f, 1.0f, dt::isDelta(ls.posType), false,
+                  false, 0);
+            float uC = rng.uniform(), uD0 = rng.uniform(), uD1 = rng.uniform();
+            (void)uC;
+            // EDF::sample (materials.cu:2052-2063, 2206-2213, 2849-2860)
+            V3 dirLocal;
+            float dirPDF;
+            float fe1;
+            if (lm.type == MAT_POINT_EMITTER) {
+                dirLocal = uniformSampleSphere(uD0, uD1);
+                dirPDF = 1.0f / (4 * kPiF);
+                fe1 = 1.0f / (4 * kPiF);
+            } else if (lm.type == MAT_ENVIRONMENT_EMITTER) {
+                float dx, dy;
+                concentricSampleDisk(uD0, uD1, &dx, &dy);
+                dirLocal = V3(sc.worldRadius * dx, sc.worldRadius * dy, 1);
+                dirPDF = 1.0f / sc.worldDiscArea;
+                fe1 = 1 / kPiF;
+            } else {
+                dirLocal = cosineSampleHemisphere(uD0, uD1);
+                dirPDF = dirLocal.z / kPiF;
+                fe1 = 1.0f / kPiF;
+            }
+            const uint32_t sampledType0 = dt::Reflection | dt::LowFreq;
+            V3 rayOrg = offsetRayOrigin(ls.sp.position, ls.sp.gn);
+            if (ls.sp.atInfinity) {
+                rayOrg = ld3(sc.center) + 1.1f * sc.worldRadius * ls.sp.position + ls.sp.frame.x * dirLocal.x +
+                         ls.sp.frame.y * dirLocal.y;
+                dirLocal.x = 0;
+                dirLocal.y = 0;
+            }
+            V3 rayDir = ls.sp.frame.fromLocal(dirLocal);
+            float cosTerm = ls.sp.cosTerm(rayDir);
+            alpha *= Spec<N>(fe1) * (cosTerm / dirPDF);
+            float roDirPDF = dirPDF, roCosTerm = cosTerm, roPrevRevAreaPDF = 1.0f;
+            bool roSecondPrevDelta = false, roPrevDelta = dt::isDelta(ls.posType);
+            uint32_t roSampledType = sampledType0;
+            float probDensity = probDensity0, prevProbDensity = 1.0f;
+            float spdmw = 0.0f, sprtf = 1.0f;
+            bool single = false;
+            bool originIsPoint = ls.sp.isPoint, originIsInfinity = ls.sp.atInfinity;
+            uint32_t pathLength = 0;
+            while (true) {
+                bool terminate = true;
+                ++pathLength;
+                ++closestRays;
+                TraceHit h;
+                h.inst = -1;
+                h.b1 = h.b2 = 0.0f;
+                bool found = traceRay<false>(sc, rayOrg, rayDir, 0.0f, 3.402823466e+38f, &h, lds, threadIdx.x);
+                V3 nextOrg, nextDir;
+                float woDirPDF = 0, woCosTerm = 0, woRevAreaPDF = 0;
+                uint32_t woSampledType = 0;
+                if (found) {
+                    const Wls<N> wls = commonWls<N>(bp, single);
+                    SurfPt sp;
+                    float hypAreaPDF;
+                    decodeHit(sc, h.inst, h.geomInst, h.prim, h.b1, h.b2, &sp, &hypAreaPDF);
+                    const DevGeomInst &G = sc.geomInsts[h.geomInst];
+                    Bsdf<N> bsdf = setupBsdf<N>(sc, sc.materials[G.material], wls);
+                    V3 dirInLocal = sp.frame.toLocal(-rayDir);
+                    V3 gnLocal = sp.frame.toLocal(sp.gn);
+                    bool secondLastValid = (!roPrevDelta && !roSecondPrevDelta) && pathLength > 2;
+                    float probRatio = roPrevRevAreaPDF / prevProbDensity;
+                    spdmw = sq(probRatio) * (spdmw + (secondLastValid ? 1 : 0));
+                    sprtf *= probRatio;
+                    prevProbDensity = probDensity;
+                    float lastDist2 = (sp.position - rayOrg).sqLength();
+                    if (originIsInfinity) lastDist2 = 1;
+                    probDensity = roDirPDF * absDot(dirInLocal, gnLocal) / lastDist2;
+                    store(sp, alpha, dirInLocal, roCosTerm / lastDist2, probDensity, prevProbDensity, spdmw, sprtf,
+                          dt::isDelta(roSampledType), roPrevDelta, single, pathLength);
+                    float sC = rng.uniform(), s0 = rng.uniform(), s1 = rng.uniform();
+                    V3 dl;
+                    float pdf;
+                    uint32_t stype;
+                    BsdfRev<N> rev;
+                    rev.value = Spec<N>(0.0f);
+                    Spec<N> fs = bsdf.sample(dirInLocal, gnLocal, wls.sel, sC, s0, s1, &dl, &pdf, &stype, true, &rev);
+                    if (!(fs.isZero() || pdf == 0.0f)) {
+                        if ((stype & dt::Dispersive) && !single) single = true;
+                        float cosT = dot(dl, gnLocal);
+                        Spec<N> localThroughput = fs * (fabsf(cosT) / pdf);
+                        alpha *= localThroughput;
+                        float continueProb = fminf(localThroughput.importance(bp.wlSel), 1.0f);
+                        if (rng.uniform() < continueProb) {
+                            alpha = alpha / continueProb;
+                            terminate = false;
+                            nextDir = sp.frame.fromLocal(dl);
+                            nextOrg = offsetRayOrigin(sp.position, cosT > 0.0f ? sp.gn : -sp.gn);
+                            woDirPDF = pdf * continueProb;
+                            woCosTerm = fabsf(cosT);
+                            woRevAreaPDF = rev.pdf * roCosTerm / lastDist2;
+                            woRevAreaPDF *= rrProbability<N>(rev.value, dirInLocal, rev.pdf, gnLocal, bp.wlSel);
+                            if (originIsPoint) woRevAreaPDF = 0;
+                            originIsPoint = false;
+                            originIsInfinity = false;
+                            woSampledType = stype;
+                        }
+                    }
+                }
+                if (pathLength >= a.fp.maxPathLength) terminate = true;
+                if (terminate) break;
+                rayOrg = nextOrg;
+                rayDir = nextDir;
+                roDirPDF = woDirPDF;
+                roCosTerm = woCosTerm;
+                roPrevRevAreaPDF = woRevAreaPDF;
+                roSecondPrevDelta = roPrevDelta;
+                roPrevDelta = dt::isDelta(roSampledType);
+                roSampledType = woSampledType;
+            }
+        }
+        bp.slotCounts[i] = min(written, bp.slotsPerPath);
+        bp.lightRng[i] = rng.s;
+    }
+    uint32_t c = waveSum(closestRays);
+    if (__lane_id() == 0 && c) atomicAdd(reinterpret_cast<unsigned long long *>(a.fb.counters + kStatClosest), c);
+}
+
+// ---- cache compaction: exclusive prefix sum of the per-path counts, then a gather ----
+constexpr uint32_t kScanBlock = 1024;
+__global__ void __launch_bounds__(kScanBlock) k_bptScanBlocks(BptParams bp) {
+    __shared__ uint32_t waveTotals[kScanBlock / 64];
+    const uint32_t i = blockIdx.x * kScanBlock + threadIdx.x;
+    uint32_t v = i < bp.numLightPaths ? bp.slotCounts[i] : 0u;
+    // inclusive scan inside the wave
+    uint32_t x = v;
+    const uint32_t lane = __lane_id();
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off);
+        if (lane >= static_cast<uint32_t>(off)) x += y;
+    }
+    const uint32_t wave = threadIdx.x / 64;
+    if (lane == 63) waveTotals[wave] = x;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        uint32_t t = threadIdx.x < kScanBlock / 64 ? waveTotals[threadIdx.x] : 0u;
+        uint32_t s = t;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            uint32_t y = __shfl_up(s, off);
+            if (lane >= static_cast<uint32_t>(off)) s += y;
+        }
+        if (threadIdx.x < kScanBlock / 64) waveTotals[threadIdx.x] = s - t;   // exclusive wave offsets
+        if (threadIdx.x == kScanBlock / 64 - 1) bp.blockSums[blockIdx.x] = s;
+    }
+    __syncthreads();
+    if (i < bp.numLightPaths) bp.slotOffsets[i] = waveTotals[wave] + x - v;
+}
+
+// one block: exclusive scan of the block totals (at most 1024 blocks), total vertex count
+__global__ void __launch_bounds__(kScanBlock) k_bptScanTop(BptParams bp, uint32_t numBlocks) {
+    __shared__ uint32_t waveTotals[kScanBlock / 64];
+    const uint32_t lane = __lane_id();
+    uint32_t v = threadIdx.x < numBlocks ? bp.blockSums[threadIdx.x] : 0u;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off);
+        if (lane >= static_cast<uint32_t>(off)) x += y;
+    }
+    const uint32_t wave = threadIdx.x / 64;
+    if (lane == 63) waveTotals[wave] = x;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        uint32_t t = threadIdx.x < kScanBlock / 64 ? waveTotals[threadIdx.x] : 0u;
+        uint32_t s = t;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            uint32_t y = __shfl_up(s, off);
+            if (lane >= static_cast<uint32_t>(off)) s += y;
+        }
+        if (threadIdx.x < kScanBlock / 64) waveTotals[threadIdx.x] = s - t;
+        if (threadIdx.x == kScanBlock / 64 - 1) *bp.numVertices = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < numBlocks) bp.blockSums[threadIdx.x] = waveTotals[wave] + x - v;
+}
+
+__global__ void __launch_bounds__(256) k_bptCompact(BptParams bp) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= bp.numLightPaths) return;
+    const uint32_t n = bp.slotCounts[i];
+    const uint32_t base = bp.blockSums[i / kScanBlock] + bp.slotOffsets[i];
+    const float4 *src = bp.slots + static_cast<size_t>(i) * bp.slotsPerPath * kLVF4;
+    float4 *dst = bp.cache + static_cast<size_t>(base) * kLVF4;
+    for (uint32_t k = 0; k < n * kLVF4; ++k) dst[k] = src[k];
+}
+
+// ---- eye subpaths (lvc_bpt.cu:338-971) ----
+template <int N>
+__global__ void __launch_bounds__(kTraceBlock) k_bptEye(BptArgs a) {
+    __shared__ int32_t lds[kLdsStack * kTraceBlock];
+    const DevScene &sc = a.sc;
+    const BptParams &bp = a.bp;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t numPaths = a.fp.imageW * a.fp.imageH;
+    uint32_t closestRays = 0, shadowRays = 0;
+    if (i < numPaths) {
+        constexpr int C = N == 4 ? 16 : 3;
+        const uint32_t px = i % a.fp.imageW, py = i / a.fp.imageW;
+        const uint32_t path = py * a.fp.width + px;
+        const uint32_t W = a.fp.imageW, H = a.fp.imageH;
+        const uint32_t numLightVertices = *bp.numVertices;
+        const float vertexProb = 1.0f / numLightVertices;
+        const size_t plane = static_cast<size_t>(a.fp.width) * a.fp.height;
+        Pcg32 rng{a.fb.ps.rng[path]};
+        const Wls<N> wlsC = commonWls<N>(bp, false);
+        float pX = px + rng.uniform();
+        float pY = py + rng.uniform();
+        const DevCamera &cam = sc.camera;
+        float lx, ly;
+        float ul0 = rng.uniform(), ul1 = rng.uniform();
+        concentricSampleDisk(ul0, ul1, &lx, &ly);
+        V3 c0(cam.rot[0], cam.rot[1], cam.rot[2]), c1(cam.rot[3], cam.rot[4], cam.rot[5]),
+            c2(cam.rot[6], cam.rot[7], cam.rot[8]);
+        SurfPt spE0;
+        spE0.frame.z = normalize(c2);
+        spE0.frame.x = normalize(c0);
+        spE0.frame.y = cross(spE0.frame.z, spE0.frame.x);
+        V3 orgLocal(cam.lensRadius * lx, cam.lensRadius * ly, 0.0f);
+        spE0.position = (c0 * orgLocal.x + c1 * orgLocal.y + c2 * orgLocal.z) + ld3(cam.position);
+        spE0.gn = spE0.frame.z;
+        spE0.isPoint = false;
+        spE0.atInfinity = false;
+        float lensAreaPDF = cam.lensRadius > 0.0f ? 1.0f / (kPiF * cam.lensRadius * cam.lensRadius) : 1.0f;
+        const uint32_t lensPosType = cam.lensRadius > 0.0f ? dt::LowFreq : dt::Delta0D;
+        LensIdf idf{cam.lensRadius * lx, cam.lensRadius * ly, cam.opWidth, cam.opHeight, cam.imgPlaneDistance,
+                    cam.objPlaneDistance, cam.imgPlaneArea};
+        Spec<N> alpha = Spec<N>(cam.sensitivity) / (lensAreaPDF * bp.wlProb * W * H);
+
+        // lens vertex connection (t = 1), splatted
+        if (numLightVertices > 0) {
+            uint32_t idx = min(static_cast<uint32_t>(numLightVertices * rng.uniform()), numLightVertices - 1);
+            LVert vtx = loadLV(bp.cache + static_cast<size_t>(idx) * kLVF4);
+            SurfPt spL;
+            uint32_t matL = decodeLightVertex(sc, vtx, &spL);
+            V3 conRayDir;
+            float sqDist;
+            ++shadowRays;
+            if (connectVisible(sc, spE0, spL, &conRayDir, &sqDist, lds, threadIdx.x)) {
+                float recSq = 1.0f / sqDist;
+                const DevMaterial &mL = sc.materials[matL];
+                Bsdf<N> bL = vtx.pathLength == 0 ? edfAsBsdf<N>(sc, mL.type) : setupBsdf<N>(sc, mL, wlsC);
+                if (bL.hasNonDelta()) {
+                    V3 gnLocalL = spL.frame.toLocal(spL.gn);
+                    V3 conLocalL = spL.frame.toLocal(-conRayDir);
+                    V3 conLocalE = spE0.frame.toLocal(conRayDir);
+                    float cosL = spL.cosTerm(conRayDir);
+                    float cosE = spE0.cosTerm(conRayDir);
+                    float G = cosL * cosE * recSq;
+                    float scalarConTerm = G * 1.0f / vertexProb;
+                    if (vtx.wlSelected) scalarConTerm *= N;
+                    Spec<N> backwardFsL;
+                    Spec<N> forwardFsL = bL.evaluate(vtx.dirIn, gnLocalL, conLocalL, wlsC.sel, true, &backwardFsL);
+                    float backwardDirDensityL;
+                    bL.evaluatePDF(vtx.dirIn, conLocalL, wlsC.sel, &backwardDirDensityL);
+                    backwardDirDensityL *= rrProbability<N>(backwardFsL, vtx.dirIn, backwardDirDensityL, gnLocalL, bp.wlSel);
+                    float backwardAreaDensityL = backwardDirDensityL * vtx.bcf;
+                    if (vtx.prevDelta) backwardAreaDensityL = 0;
+                    float u0 = 0.0f, u1 = 0.0f;
+                    bool onScreen = idf.project(conLocalE, &u0, &u1);
+                    Spec<N> forwardFsE(onScreen ? 1.0f : 0.0f);
+                    float forwardDirDensityE = idf.pdf(conLocalE);
+                    float forwardAreaDensityE = forwardDirDensityE * cosL * recSq;
+                    if (spL.isPoint) forwardAreaDensityE = 0;
+                    float partialL;
+                    {
+                        bool lastValid = (!vtx.delta && !vtx.prevDelta) && vtx.pathLength > 1;
+                        float r = backwardAreaDensityL / vtx.prevProbDensity;
+                        partialL = sq(r) * (vtx.spdmw + (lastValid ? 1 : 0));
+                        float probRatioToFirst = vtx.sprtf;
+                        if (vtx.pathLength > 0) probRatioToFirst *= r;
+                        bool curValid = !vtx.delta && vtx.pathLength > 0;
+                        float r2 = forwardAreaDensityE / vtx.probDensity;
+                        partialL = sq(r2) * (partialL + (curValid ? 1 : 0));
+                        probRatioToFirst *= r2;
+                        partialL += sq(probRatioToFirst / vertexProb);
+                    }
+                    Spec<N> conTerm = forwardFsL * scalarConTerm * forwardFsE;
+                    Spec<N> unweighted = loadSpec<N>(vtx.flux) * conTerm * alpha;
+                    Spec<N> contribution = (1.0f / (1.0f + partialL)) * unweighted;
+                    if (contribution.allFinite() && onScreen) {
+                        uint32_t ipx = static_cast<uint32_t>(u0 * W), ipy = static_cast<uint32_t>(u1 * H);
+                        if (ipx < W && ipy < H) {
+                            float *dst = bp.splat + static_cast<size_t>(ipy) * a.fp.width + ipx;
+                            if constexpr (N == 4) {
+                                const float recBinWidth = 16 / (830.0f - 360.0f);
+#pragma unroll
+                                for (int w = 0; w < 4; ++w) {
+                                    float lam = 360.0f + (830.0f - 360.0f) * (w + wlsC.offset) / 4;
+                                    uint32_t bin = min(static_cast<uint32_t>((lam - 360.0f) / (830.0f - 360.0f) * 16), 15u);
+                                    atomicAdd(dst + bin * plane, contribution[w] * recBinWidth);
+                                }
+                            } else {
+#pragma unroll
+                                for (int c = 0; c < 3; ++c) atomicAdd(dst + c * plane, contribution[c]);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+
+        // first eye ray (cameras.cu:145-157)
+        float uD0 = pX / W, uD1 = pY / H;
+        V3 pFocus(cam.opWidth * (0.5f - uD0), cam.opHeight * (0.5f - uD1), cam.objPlaneDistance);
+        V3 dirLocal = normalize(pFocus - V3(idf.xOnLens, idf.yOnLens, 0.0f));
+        float dirPDF = sq(cam.imgPlaneDistance) / (cube(dirLocal.z) * cam.imgPlaneArea);
+        V3 rayOrg = spE0.position;
+        V3 rayDir = spE0.frame.fromLocal(dirLocal);
+        float cosTerm0 = spE0.cosTerm(rayDir);
+        alpha *= Spec<N>(1.0f) * (cosTerm0 / dirPDF);
+        float roDirPDF = dirPDF, roCosTerm = cosTerm0, roPrevRevAreaPDF = 1.0f;
+        bool roSecondPrevDelta = false, roPrevDelta = dt::isDelta(lensPosType);
+        uint32_t roSampledType = dt::Reflection | dt::LowFreq;
+        float probDensity = lensAreaPDF, prevProbDensity = 1.0f, spdmw = 0.0f;
+        bool single = false;
+        uint32_t pathLength = 0;
+        Spec<N> contribution(0.0f);
+        while (true) {
+            bool terminate = true;
+            ++pathLength;
+            ++closestRays;
+            TraceHit h;
+            h.inst = -1;
+            h.b1 = h.b2 = 0.0f;
+            bool found = traceRay<false>(sc, rayOrg, rayDir, 0.0f, 3.402823466e+38f, &h, lds, threadIdx.x);
+            const Wls<N> wls = commonWls<N>(bp, single);
+            V3 nextOrg, nextDir;
+            float woDirPDF = 0, woCosTerm = 0, woRevAreaPDF = 0;
+            uint32_t woSampledType = 0;
+            if (!found) {
+                // miss program (lvc_bpt.cu:841-971)
+                if (sc.hasEnv && sc.envImportance > 0.0f) {
+                    SurfPt spE;
+                    float hypAreaPDF;
+                    envPointFromDir(sc, rayDir, &spE, &hypAreaPDF);
+                    V3 dirOutLocalE = spE.frame.toLocal(-rayDir);
+                    bool secondLastValid = (!roPrevDelta && !roSecondPrevDelta) && pathLength > 2;
+                    spdmw = sq(roPrevRevAreaPDF / prevProbDensity) * (spdmw + (secondLastValid ? 1 : 0));
+                    prevProbDensity = probDensity;
+                    const float lastDist2 = 1.0f;
+                    probDensity = roDirPDF / lastDist2;
+                    Spec<N> M;
+                    float fe;
+                    evalEDF<N>(sc, sc.materials[sc.envMaterial], spE, wls, dirOutLocalE, &M, &fe);
+                    if (!M.isZero()) {
+                        Spec<N> unweighted = alpha * (M * fe);
+                        if (single) unweighted = unweighted * static_cast<float>(N);
+                        float instProb = 1.0f / sc.lightInstIntegral;
+                        float forwardAreaDensityL = bp.numLightPaths * instProb * 1.0f * hypAreaPDF;
+                        float backwardDirDensityE = edfPDF(MAT_ENVIRONMENT_EMITTER, dirOutLocalE, sc.worldDiscArea);
+                        float backwardAreaDensityE = backwardDirDensityE * roCosTerm / lastDist2;
+                        float partialE;
+                        {
+                            bool lastValid = (!dt::isDelta(roSampledType) && !roPrevDelta) && pathLength > 1;
+                            float r = backwardAreaDensityE / prevProbDensity;
+                            partialE = sq(r) * (spdmw + (lastValid ? 1 : 0));
+                            bool curValid = !dt::isDelta(roSampledType);
+                            float r2 = forwardAreaDensityL / probDensity;
+                            partialE = sq(r2) * (partialE + (curValid ? 1 : 0));
+                            partialE *= sq(vertexProb);
+                        }
+                        Spec<N> c = (1.0f / (1.0f + partialE)) * unweighted;
+                        if (c.allFinite()) contribution += c;
+                    }
+                }
+                break;
+            }
+            // closest-hit program for eye paths (lvc_bpt.cu:570-837)
+            SurfPt spE;
+            float hypAreaPDF;
+            decodeHit(sc, h.inst, h.geomInst, h.prim, h.b1, h.b2, &spE, &hypAreaPDF);
+            const DevGeomInst &G = sc.geomInsts[h.geomInst];
+            const DevMaterial &matE = sc.materials[G.material];
+            Bsdf<N> bsdfE = setupBsdf<N>(sc, matE, wls);
+            V3 dirOutLocalE = spE.frame.toLocal(-rayDir);
+            V3 gnLocalE = spE.frame.toLocal(spE.gn);
+            bool secondLastValidE = (!roPrevDelta && !roSecondPrevDelta) && pathLength > 2;
+            spdmw = sq(roPrevRevAreaPDF / prevProbDensity) * (spdmw + (secondLastValidE ? 1 : 0));
+            prevProbDensity = probDensity;
+            float lastDist2 = (rayOrg - spE.position).sqLength();
+            probDensity = roDirPDF * absDot(dirOutLocalE, gnLocalE) / lastDist2;
+
+            if (matE.isEmitter) {
+                Spec<N> M;
+                float fe;
+                evalEDF<N>(sc, matE, spE, wls, dirOutLocalE, &M, &fe);
+                if (!M.isZero()) {
+                    Spec<N> unweighted = alpha * (M * fe);
+                    if (single) unweighted = unweighted * static_cast<float>(N);
+                    const DevInstance &I = sc.instances[h.inst];
+                    float instProb = I.lightDistIntegral / sc.lightInstIntegral;
+                    float geomInstProb = G.importance / I.lightDistIntegral;
+                    float forwardAreaDensityL = bp.numLightPaths * instProb * geomInstProb * hypAreaPDF;
+                    float backwardDirDensityE = edfPDF(matE.type, dirOutLocalE, sc.worldDiscArea);
+                    float backwardAreaDensityE = backwardDirDensityE * roCosTerm / lastDist2;
+                    float partialE;
+                    {
+                        bool lastValid = (!dt::isDelta(roSampledType) && !roPrevDelta) && pathLength > 1;
+                        float r = backwardAreaDensityE / prevProbDensity;
+                        partialE = sq(r) * (spdmw + (lastValid ? 1 : 0));
+                        bool curValid = !dt::isDelta(roSampledType);
+                        float r2 = forwardAreaDensityL / probDensity;
+                        partialE = sq(r2) * (partialE + (curValid ? 1 : 0));
+                        partialE *= sq(vertexProb);
+                    }
+                    Spec<N> c = (1.0f / (1.0f + partialE)) * unweighted;
+                    if (c.allFinite()) contribution += c;
+                }
+            }
+
+            if (bsdfE.hasNonDelta() && numLightVertices > 0) {
+                uint32_t idx = min(static_cast<uint32_t>(numLightVertices * rng.uniform()), numLightVertices - 1);
+                LVert vtx = loadLV(bp.cache + static_cast<size_t>(idx) * kLVF4);
+                SurfPt spL;
+                uint32_t matL = decodeLightVertex(sc, vtx, &spL);
+                V3 conRayDir;
+                float sqDist;
+                ++shadowRays;
+                if (connectVisible(sc, spE, spL, &conRayDir, &sqDist, lds, threadIdx.x)) {
+                    float recSq = 1.0f / sqDist;
+                    const DevMaterial &mL = sc.materials[matL];
+                    Bsdf<N> bL = vtx.pathLength == 0 ? edfAsBsdf<N>(sc, mL.type) : setupBsdf<N>(sc, mL, wls);
+                    if (bL.hasNonDelta()) {
+                        V3 gnLocalL = spL.frame.toLocal(spL.gn);
+                        V3 conLocalL = spL.frame.toLocal(-conRayDir);
+                        V3 conLocalE = spE.frame.toLocal(conRayDir);
+                        float cosL = spL.cosTerm(conRayDir);
+                        float cosE = spE.cosTerm(conRayDir);
+                        float Gc = cosL * cosE * recSq;
+                        float scalarConTerm = Gc * 1.0f / vertexProb;
+                        if (vtx.wlSelected || single) scalarConTerm *= N;
+                        Spec<N> backwardFsL;
+                        Spec<N> forwardFsL = bL.evaluate(vtx.dirIn, gnLocalL, conLocalL, wls.sel, true, &backwardFsL);
+                        float backwardDirDensityL;
+                        float forwardDirDensityL = bL.evaluatePDF(vtx.dirIn, conLocalL, wls.sel, &backwardDirDensityL);
+                        if (vtx.pathLength > 0)
+                            forwardDirDensityL *= rrProbability<N>(forwardFsL, conLocalL, forwardDirDensityL, gnLocalL, bp.wlSel);
+                        backwardDirDensityL *= rrProbability<N>(backwardFsL, vtx.dirIn, backwardDirDensityL, gnLocalL, bp.wlSel);
+                        float forwardAreaDensityL = forwardDirDensityL * cosE * recSq;
+                        float backwardAreaDensityL = backwardDirDensityL * vtx.bcf;
+                        if (vtx.prevDelta) backwardAreaDensityL = 0;
+                        Spec<N> backwardFsE;
+                        Spec<N> forwardFsE = bsdfE.evaluate(dirOutLocalE, gnLocalE, conLocalE, wls.sel, false, &backwardFsE);
+                        float backwardDirDensityE;
+                        float forwardDirDensityE = bsdfE.evaluatePDF(dirOutLocalE, conLocalE, wls.sel, &backwardDirDensityE);
+                        forwardDirDensityE *= rrProbability<N>(forwardFsE, conLocalE, forwardDirDensityE, gnLocalE, bp.wlSel);
+                        backwardDirDensityE *= rrProbability<N>(backwardFsE, dirOutLocalE, backwardDirDensityE, gnLocalE, bp.wlSel);
+                        float forwardAreaDensityE = forwardDirDensityE * cosL * recSq;
+                        if (spL.isPoint) forwardAreaDensityE = 0;
+                        float backwardAreaDensityE = backwardDirDensityE * roCosTerm / lastDist2;
+                        float partialL;
+                        {
+                            bool lastValid = (!vtx.delta && !vtx.prevDelta) && vtx.pathLength > 1;
+                            float r = backwardAreaDensityL / vtx.prevProbDensity;
+                            partialL = sq(r) * (vtx.spdmw + (lastValid ? 1 : 0));
+                            float probRatioToFirst = vtx.sprtf;
+                            if (vtx.pathLength > 0) probRatioToFirst *= r;
+                            bool curValid = !vtx.delta && vtx.pathLength > 0;
+                            float r2 = forwardAreaDensityE / vtx.probDensity;
+                            partialL = sq(r2) * (partialL + (curValid ? 1 : 0));
+                            probRatioToFirst *= r2;
+                            partialL += sq(probRatioToFirst / vertexProb);
+                        }
+                        float partialE;
+                        {
+                            bool lastValid = !dt::isDelta(roSampledType) && !roPrevDelta;
+                            float r = backwardAreaDensityE / prevProbDensity;
+                            partialE = sq(r) * (spdmw + (lastValid ? 1 : 0));
+                            bool curValid = !dt::isDelta(roSampledType);
+                            float r2 = forwardAreaDensityL / probDensity;
+                            partialE = sq(r2) * (partialE + (curValid ? 1 : 0));
+                        }
+                        Spec<N> conTerm = forwardFsL * scalarConTerm * forwardFsE;
+                        Spec<N> unweighted = loadSpec<N>(vtx.flux) * conTerm * alpha;
+                        Spec<N> c = (1.0f / (1.0f + partialL + partialE)) * unweighted;
+                        if (c.allFinite()) contribution += c;
+                    }
+                }
+            }
+
+            float sC = rng.uniform(), s0 = rng.uniform(), s1 = rng.uniform();
+            V3 dl;
+            float pdf;
+            uint32_t stype;
+            BsdfRev<N> rev;
+            rev.value = Spec<N>(0.0f);
+            Spec<N> fs = bsdfE.sample(dirOutLocalE, gnLocalE, wls.sel, sC, s0, s1, &dl, &pdf, &stype, false, &rev);
+            if (!(fs.isZero() || pdf == 0.0f)) {
+                if ((stype & dt::Dispersive) && !single) single = true;
+                float cosT = dot(dl, gnLocalE);
+                Spec<N> localThroughput = fs * (fabsf(cosT) / pdf);
+                alpha *= localThroughput;
+                float continueProb = fminf(localThroughput.importance(bp.wlSel), 1.0f);
+                if (rng.uniform() < continueProb) {
+                    alpha = alpha / continueProb;
+                    terminate = false;
+                    nextDir = spE.frame.fromLocal(dl);
+                    nextOrg = offsetRayOrigin(spE.position, cosT > 0.0f ? spE.gn : -spE.gn);
+                    woDirPDF = pdf * continueProb;
+                    woCosTerm = fabsf(cosT);
+                    woRevAreaPDF = rev.pdf * roCosTerm / lastDist2;
+                    woRevAreaPDF *= rrProbability<N>(rev.value, dirOutLocalE, rev.pdf, gnLocalE, bp.wlSel);
+                    woSampledType = stype;
+                }
+            }
+            if (pathLength >= a.fp.maxPathLength) terminate = true;
+            if (terminate) break;
+            rayOrg = nextOrg;
+            rayDir = nextDir;
+            roDirPDF = woDirPDF;
+            roCosTerm = woCosTerm;
+            roPrevRevAreaPDF = woRevAreaPDF;
+            roSecondPrevDelta = roPrevDelta;
+            roPrevDelta = dt::isDelta(roSampledType);
+            roSampledType = woSampledType;
+        }
+        a.fb.ps.rng[path] = rng.s;
+        a.fb.ps.contrib[path] = storeSpec<N>(contribution);
+    }
+    uint32_t c = waveSum(closestRays), sh = waveSum(shadowRays);
+    if (__lane_id() == 0) {
+        if (c) atomicAdd(reinterpret_cast<unsigned long long *>(a.fb.counters + kStatClosest), c);
+        if (sh) atomicAdd(reinterpret_cast<unsigned long long *>(a.fb.counters + kStatShadow), sh);
+    }
+}
+
+// Eye contribution, then the frame's splats, into the Kahan accumulation (lvc_bpt.cu:560-565,
+// post_process.cu:18-31); the splat planes are cleared for the next frame.
+template <int N>
+__global__ void __launch_bounds__(256) k_bptAccumulate(BptArgs a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t numPaths = a.fp.imageW * a.fp.imageH;
+    if (i >= numPaths) return;
+    const uint32_t px = i % a.fp.imageW, py = i / a.fp.imageW;
+    const uint32_t p = py * a.fp.width + px;
+    constexpr int C = N == 4 ? 16 : 3;
+    const size_t plane = static_cast<size_t>(a.fp.width) * a.fp.height;
+    float *acc = a.fb.accum + p;
+    const bool reset = a.fp.frameIndex == 1;
+    Spec<N> c = loadSpec<N>(a.fb.ps.contrib[p]);
+    bool finite = c.allFinite();
+    if (!finite) atomicAdd(&a.fb.counters[kStatDropped], 1u);
+    float addend[C];
+    if constexpr (N == 4) {
+        const float recBinWidth = 16 / (830.0f - 360.0f);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) addend[k] = 0.0f;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            float lam = 360.0f + (830.0f - 360.0f) * (w + a.bp.wlOffset) / 4;
+            uint32_t bin = min(static_cast<uint32_t>((lam - 360.0f) / (830.0f - 360.0f) * 16), 15u);
+#pragma unroll
+            for (int k = 0; k < 16; ++k)
+                if (k == static_cast<int>(bin)) addend[k] += c[w] * recBinWidth;
+        }
+    } else {
+        addend[0] = c[0]; addend[1] = c[1]; addend[2] = c[2];
+    }
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+        float sum = reset ? 0.0f : acc[(2 * k) * plane];
+        float comp = reset ? 0.0f : acc[(2 * k + 1) * plane];
+        if (finite) {
+            float cIn = addend[k] - comp;
+            float tmp = sum + cIn;
+            comp = (tmp - sum) - cIn;
+            sum = tmp;
+        }
+        float sv = a.bp.splat[k * plane + p];
+        float cIn = sv - comp;
+        float tmp = sum + cIn;
+        comp = (tmp - sum) - cIn;
+        acc[(2 * k) * plane] = tmp;
+        acc[(2 * k + 1) * plane] = comp;
+        a.bp.splat[k * plane + p] = 0.0f;
+    }
+}
+
 // ---------------------------------------------------------------------------------------------
 // entry kernels for parity tests
 struct RayIn { float org[3]; float tmin; float dir[3]; float tmax; };
@@ -1170,6 +2026,47 @@ static hipError_t launchFrameT(const DevScene &sc, const FrameParams &fp, const 
 hipError_t launchFrame(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb, bool spectral,
                        hipStream_t s, LaunchTimer *timer) {
     return spectral ? launchFrameT<4>(sc, fp, fb, s, timer) : launchFrameT<3>(sc, fp, fb, s, timer);
+}
+
+template <int N>
+static hipError_t launchBptFrameT(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb,
+                                  const BptParams &bp, hipStream_t s, LaunchTimer *timer) {
+    BptArgs a;
+    a.sc = sc;
+    a.fp = fp;
+    a.fb = fb;
+    a.bp = bp;
+    KArgs k;
+    k.sc = sc;
+    k.fp = fp;
+    k.fb = fb;
+    k.depth = 0;
+    const uint32_t n = fp.imageW * fp.imageH;
+    const uint32_t scanBlocks = blocksFor(bp.numLightPaths, kScanBlock);
+    if (bp.numLightPaths == 0 || scanBlocks > kScanBlock) return hipErrorInvalidValue;
+    auto t0 = [&](int st) { if (timer) timer->begin(st); };
+    auto t1 = [&](int st) { if (timer) timer->end(st); };
+    t0(STAGE_LIGHT_PATHS);
+    hipLaunchKernelGGL(k_bptLight<N>, dim3(blocksFor(bp.numLightPaths, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
+    hipLaunchKernelGGL(k_bptScanBlocks, dim3(scanBlocks), dim3(kScanBlock), 0, s, bp);
+    hipLaunchKernelGGL(k_bptScanTop, dim3(1), dim3(kScanBlock), 0, s, bp, scanBlocks);
+    hipLaunchKernelGGL(k_bptCompact, dim3(blocksFor(bp.numLightPaths, 256)), dim3(256), 0, s, bp);
+    t1(STAGE_LIGHT_PATHS);
+    t0(STAGE_EYE_PATHS);
+    hipLaunchKernelGGL(k_bptEye<N>, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
+    t1(STAGE_EYE_PATHS);
+    t0(STAGE_ACCUMULATE);
+    hipLaunchKernelGGL(k_bptAccumulate<N>, dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
+    t1(STAGE_ACCUMULATE);
+    t0(STAGE_CONVERT);
+    hipLaunchKernelGGL(k_convert<N>, dim3(blocksFor(n, 256)), dim3(256), 0, s, k);
+    t1(STAGE_CONVERT);
+    return hipGetLastError();
+}
+
+hipError_t launchBptFrame(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb, const BptParams &bp,
+                          bool spectral, hipStream_t s, LaunchTimer *timer) {
+    return spectral ? launchBptFrameT<4>(sc, fp, fb, bp, s, timer) : launchBptFrameT<3>(sc, fp, fb, bp, s, timer);
 }
 
 hipError_t launchTraceRays(const DevScene &sc, const void *rays, uint32_t n, void *hits, bool anyHit, hipStream_t s) {

@@ -87,10 +87,38 @@ struct LaunchTimer {
     virtual void end(int stage) = 0;
     virtual ~LaunchTimer() {}
 };
-enum Stage { STAGE_GENERATE = 0, STAGE_INTERSECT, STAGE_SHADE, STAGE_SHADOW, STAGE_ACCUMULATE, STAGE_CONVERT, STAGE_TAIL, NUM_STAGES };
+enum Stage {
+    STAGE_GENERATE = 0, STAGE_INTERSECT, STAGE_SHADE, STAGE_SHADOW, STAGE_ACCUMULATE, STAGE_CONVERT, STAGE_TAIL,
+    STAGE_LIGHT_PATHS, STAGE_EYE_PATHS, NUM_STAGES
+};
+
+// LVC-BPT frame state (lvc_bpt.cu:144-971).  Light paths write their vertices to fixed slots
+// (slotsPerPath per path); a prefix sum of the per-path counts compacts them into `cache` in
+// light-path order, so the cache does not depend on thread scheduling.
+constexpr uint32_t kLightVertexFloat4s = 5;     // 80 bytes per cached vertex
+constexpr uint32_t kMaxLightPaths = 1024u * 1024u;   // two-level scan of 1024-wide blocks
+struct BptParams {
+    float wlOffset;              // common wavelengths of the frame (context.cpp:1464-1467)
+    uint32_t wlSel;
+    float wlProb;                // their probability density
+    uint32_t numLightPaths;
+    uint32_t slotsPerPath;       // maxPathLength + 1
+    float4* slots;               // numLightPaths * slotsPerPath vertices
+    uint32_t* slotCounts;        // vertices written by each light path
+    uint32_t* slotOffsets;       // exclusive prefix sum inside each 1024-path block
+    uint32_t* blockSums;         // per-block totals, then their exclusive prefix sum
+    float4* cache;               // compacted vertices
+    uint32_t* numVertices;
+    uint64_t* lightRng;          // PCG32 state per light path
+    float* splat;                // lens-connection splats: channel planes of width * height
+};
 
 hipError_t launchFrame(const DevScene& scene, const FrameParams& fp, const FrameBuffers& fb, bool spectral,
                        hipStream_t stream, LaunchTimer* timer);
+
+// One LVC-BPT frame: light paths, cache compaction, eye paths, accumulation, conversion.
+hipError_t launchBptFrame(const DevScene& scene, const FrameParams& fp, const FrameBuffers& fb, const BptParams& bp,
+                          bool spectral, hipStream_t stream, LaunchTimer* timer);
 
 hipError_t launchTraceRays(const DevScene& scene, const void* rays, uint32_t numRays, void* hits, bool anyHit,
                            hipStream_t stream);

@@ -13,6 +13,7 @@
 #include <cstdint>
 #include <map>
 #include <memory>
+#include <random>
 #include <set>
 #include <string>
 #include <vector>
@@ -388,6 +389,14 @@ public:
     DeviceBuffer rng, alpha, contrib, meta, rayOrg[2], rayDir[2], hitA, hitB, shOrg, shDir, shVal, counters, accum,
         output, scratch;
     uint32_t accumChannels = 3;
+    // LVC-BPT state: light-path PCG32 states seeded by mt19937_64(459182033132123413)
+    // (context.cpp:618-626) and the common-wavelength generator (context.h:226, a default
+    // std::mt19937 that two uniform floats are drawn from per frame, context.cpp:1464-1467)
+    uint32_t numLightPaths = 500000;
+    uint32_t lightRngPaths = 0;      // light paths the light RNG buffer was seeded for
+    std::mt19937 bptWlRng;
+    DeviceBuffer lightRng, lvSlots, lvCounts, lvOffsets, lvBlockSums, lvCache, lvNum, splat;
+    size_t splatLayout = 0;          // bytes of the splat planes last cleared
     SceneGPU gpu;
     double stageMs[NUM_STAGES] = {};
     uint64_t launches[NUM_STAGES] = {};
@@ -415,6 +424,13 @@ public:
     // 2k + 1, width*height floats each); the API exchanges them per pixel (channel, pair).
     void readAccumulation(float *perPixel);
     void writeAccumulation(const float *perPixel);
+    // LVC-BPT buffers for the current light-path count, path length and output size
+    void ensureBpt(uint32_t slotsPerPath);
+    void setNumLightPaths(uint32_t n);
+    void renderBptFrame(hipStream_t stream, const FrameParams &fp, bool firstFrame);
+    void readLightRngStates(uint64_t *dst);
+    void writeLightRngStates(const uint64_t *src);
+    FrameBuffers frameBuffers();
 };
 
 void throwHip(hipError_t e, const char *what);

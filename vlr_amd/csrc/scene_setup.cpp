@@ -675,7 +675,7 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
         for (int a = 0; a < 3; ++a) D.center[a] = 0.5f * (sceneBox.lo[a] + sceneBox.hi[a]);
         float d0 = sceneBox.hi[0] - D.center[0], d1 = sceneBox.hi[1] - D.center[1], d2 = sceneBox.hi[2] - D.center[2];
         D.worldRadius = std::sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-        D.worldDiscArea = 3.14159265358979323846f * D.worldRadius * D.worldRadius;
+        D.worldDiscArea = 3.14159265358979323846f * (D.worldRadius * D.worldRadius);
     }
 
     // upload
