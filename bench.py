@@ -128,6 +128,134 @@ def cpu_baseline(desc, width, height, depth, budget_s):
             "samples_per_s": frames * width * height / elapsed}
 
 
+def rmse_curve(ctx, camera, frames, checkpoints, ref, ref_var, first_frame=True):
+    """Renders `frames` frames; at each checkpoint spp returns (spp, seconds, RMSE vs ref with the
+    reference's own noise variance removed).  Read-backs happen outside the timed spans."""
+    out = []
+    elapsed = 0.0
+    hip_sync()
+    t0 = time.perf_counter()
+    for k in range(frames):
+        ctx.render(camera, first_frame=first_frame and k == 0, limit=1 << 30)
+        if (k + 1) in checkpoints:
+            hip_sync()
+            elapsed += time.perf_counter() - t0
+            img = ctx.read_output()[..., :3].astype(np.float64)
+            mse = float(((img - ref) ** 2).mean()) - ref_var
+            out.append({"spp": k + 1, "seconds": elapsed, "rmse": float(np.sqrt(max(mse, 0.0)))})
+            hip_sync()
+            t0 = time.perf_counter()
+    return out
+
+
+def interp_log(points, x_key, y_key, x):
+    """log-log interpolation of y at x over points sorted by x (None outside the range)."""
+    xs = [p[x_key] for p in points]
+    ys = [p[y_key] for p in points]
+    if x < xs[0] or x > xs[-1]:
+        return None
+    for i in range(1, len(xs)):
+        if x <= xs[i]:
+            a = (np.log(x) - np.log(xs[i - 1])) / (np.log(xs[i]) - np.log(xs[i - 1]))
+            return float(np.exp(np.log(ys[i - 1]) + a * (np.log(ys[i]) - np.log(ys[i - 1]))))
+    return float(ys[-1])
+
+
+def cache_experiment(width, height, ref_spp, spp, warmup_frames, seed=20261202):
+    """Radiance cache vs plain path tracing on the Cornell box (RGB, max depth 8): RMSE against
+    an FP64-accumulated plain reference over time.  The reference is two independent halves
+    (their own seeds); their difference estimates the reference's noise variance, which is
+    removed from every measured MSE.  The cached run's wall time includes its training-only
+    warm-up frames."""
+    desc = scenes.cornell_box(seed)
+    n = width * height
+
+    def context(cache, rng_seed=None):
+        ctx = api.Context(spectral=False, max_path_length=8)
+        built = api.build_scene(ctx, desc)
+        ctx.bind_output(width, height)
+        if rng_seed is not None:
+            ctx.write_rng(np.random.default_rng(rng_seed).integers(1, 2 ** 63, size=n, dtype=np.uint64))
+        if cache:
+            ctx.set_radiance_cache(True)
+        return ctx, built
+
+    halves = []
+    for h, rs in enumerate((101, 202)):
+        ctx, built = context(False, rs)
+        acc = np.zeros((height, width, 3))
+        chunk = 1024
+        done = 0
+        while done < ref_spp // 2:
+            m = min(chunk, ref_spp // 2 - done)
+            for k in range(m):
+                ctx.render(built.camera, first_frame=(k == 0), limit=m)
+            acc += ctx.read_output()[..., :3].astype(np.float64) * m
+            done += m
+        halves.append(acc / done)
+        ctx.close()
+    ref = 0.5 * (halves[0] + halves[1])
+    ref_var = float(((halves[0] - halves[1]) ** 2).mean()) / 4.0
+    checkpoints = sorted({c for c in (16, 32, 64, 128, 256, 512, 1024, 2048, 4096) if c <= spp} | {spp})
+
+    ctx, built = context(False)
+    for k in range(8):   # schedule tuning and first-use allocations outside the timed region
+        ctx.render(built.camera, first_frame=(k == 0), limit=1 << 30)
+    plain = rmse_curve(ctx, built.camera, spp, checkpoints, ref, ref_var)
+    ctx.close()
+
+    ctx, built = context(True)
+    hip_sync()
+    t0 = time.perf_counter()
+    for k in range(warmup_frames):
+        ctx.render(built.camera, first_frame=(k == 0), limit=1 << 30)
+    hip_sync()
+    warm_s = time.perf_counter() - t0
+    ctx.reset_stats()
+    ctx.set_kernel_timing(True)
+    cached = rmse_curve(ctx, built.camera, spp, checkpoints, ref, ref_var)
+    st = ctx.stats()
+    loss, steps = ctx.cache_loss()
+    ctx.close()
+    for p in cached:
+        p["seconds"] += warm_s
+
+    # equal time: the plain path tracer's RMSE at each cached checkpoint's wall time
+    equal_time = []
+    for p in cached:
+        r = interp_log(plain, "seconds", "rmse", p["seconds"])
+        if r is not None:
+            equal_time.append({"seconds": p["seconds"], "cached": p["rmse"], "plain": r})
+    # time for the cached form to reach the plain form's RMSE at its largest checkpoint
+    target = plain[-1]["rmse"]
+    reach = None
+    for i, p in enumerate(cached):
+        if p["rmse"] <= target:
+            if i == 0:
+                reach = p["seconds"]
+            else:
+                q = cached[i - 1]
+                a = (np.log(q["rmse"]) - np.log(target)) / (np.log(q["rmse"]) - np.log(p["rmse"]))
+                reach = float(np.exp(np.log(q["seconds"]) + a * (np.log(p["seconds"]) - np.log(q["seconds"]))))
+            break
+    frames = max(1, st["numFrames"])
+    return {
+        "workload": f"Cornell box (C1 scene), RGB, max depth 8, {width}x{height}",
+        "reference": {"spp": ref_spp, "accumulation": "fp64 on the host over fp32 Kahan chunks of <= 1024 frames",
+                      "noise_rmse_removed": float(np.sqrt(ref_var))},
+        "plain": plain,
+        "cached": cached,
+        "cache_warmup": {"frames": warmup_frames, "seconds": warm_s},
+        "equal_time": equal_time,
+        "plain_rmse_at_spp": {"spp": plain[-1]["spp"], "rmse": target, "seconds": plain[-1]["seconds"]},
+        "cached_seconds_to_reach_it": reach,
+        "cached_time_fraction": None if reach is None else reach / plain[-1]["seconds"],
+        "cache_ms_per_frame": {"query": st["msCacheQuery"] / frames, "train": st["msCacheTrain"] / frames},
+        "cache_loss": loss,
+        "cache_steps": steps,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +273,11 @@ def main():
                     help="depths run as per-depth kernels before the persistent tail kernel (library default)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also run one device-to-device copy of this many bytes (counter calibration)")
+    ap.add_argument("--cache-experiment", action="store_true",
+                    help="only run the radiance-cache RMSE experiment and print its JSON line")
+    ap.add_argument("--cache-ref-spp", type=int, default=2048)
+    ap.add_argument("--cache-spp", type=int, default=256)
+    ap.add_argument("--cache-warmup", type=int, default=64)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by pixel "
@@ -152,6 +285,10 @@ def main():
         sys.exit(2)
     if not os.path.exists(api.LIB_PATH):
         raise RuntimeError(f"{api.LIB_PATH} is missing: build it with vlr_amd/build.py")
+    if args.cache_experiment:
+        res = cache_experiment(args.width, args.height, args.cache_ref_spp, args.cache_spp, args.cache_warmup)
+        print(json.dumps({"radiance_cache": res}), flush=True)
+        return
 
     desc = scenes.icosphere_grid(args.seed)
     ctx = api.Context(spectral=desc.spectral, max_path_length=args.depth)

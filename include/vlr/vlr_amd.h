@@ -49,6 +49,8 @@ typedef struct VLRRenderStats {
     uint64_t closestRaysWavefront, shadowRaysWavefront;
     /* VLRRenderer_BPT: light-path pass (tracing and cache compaction) and eye-path pass */
     double msLightPaths, msEyePaths;
+    /* radiance cache: queries (inference) and training (sample compaction + one Adam step) */
+    double msCacheQuery, msCacheTrain;
 } VLRRenderStats;
 /* Counting the rays reads back one small counter buffer per render call. */
 VLR_API VLRResult vlrContextGetRenderStats(VLRContextConst context, VLRRenderStats* stats);
@@ -92,6 +94,50 @@ VLR_API VLRResult vlrContextWriteLightRNGStates(VLRContext context, const uint64
 VLR_API VLRResult vlrContextGetAccumulationChannels(VLRContextConst context, uint32_t* channels);
 VLR_API VLRResult vlrContextReadAccumulation(VLRContextConst context, float* data);
 VLR_API VLRResult vlrContextWriteAccumulation(VLRContext context, const float* data, uint32_t numAccumFrames);
+
+/* ---- radiance cache ----
+ * A 5 x 64 ReLU MLP (BF16 weights and activations, FP32 accumulation) trained online on the
+ * path tracer's own vertices and queried to end paths once their spread passes a threshold
+ * (Mueller et al. 2021, "Real-time Neural Radiance Caching for Path Tracing").  It has no
+ * counterpart in the reference; the path tracer without it is the reference's.  Each frame
+ * a set of training paths (one pixel in `stride`, a different offset every frame) is traced
+ * to the end; vertices 1..3 of each become samples whose target is the radiance the path
+ * gathered after that vertex divided by the throughput on arrival; the other paths end in
+ * the cache at the first vertex after a bounce whose spread exceeds threshold * a_0.  The
+ * first `warmupSteps` frames after a reset only train. */
+typedef struct VLRRadianceCacheSettings {
+    float learningRate;          /* Adam step size, 1e-2 */
+    float beta1, beta2, epsilon; /* 0.9, 0.99, 1e-15 */
+    uint32_t trainingPaths;      /* training paths per frame, 1 .. 21845 (default 21845) */
+    float terminationThreshold;  /* spread threshold c, 0.01 */
+    uint32_t warmupSteps;        /* training-only frames after a reset, 16 */
+    uint32_t seed;               /* weight initialisation (Glorot uniform from PCG32) */
+} VLRRadianceCacheSettings;
+VLR_API VLRResult vlrContextSetRadianceCache(VLRContext context, bool enable);
+VLR_API VLRResult vlrContextGetRadianceCache(VLRContextConst context, bool* enabled);
+VLR_API VLRResult vlrContextSetRadianceCacheSettings(VLRContext context, const VLRRadianceCacheSettings* settings);
+VLR_API VLRResult vlrContextGetRadianceCacheSettings(VLRContextConst context, VLRRadianceCacheSettings* settings);
+/* Re-initialises the weights from the seed and clears the optimiser state. */
+VLR_API VLRResult vlrContextResetRadianceCache(VLRContext context);
+/* FP32 master parameters (host arrays of vlrRadianceCacheNumParameters floats): per layer
+ * l = 0..5 the row-major weights [rows_l][cols_l] (rows 64, the output layer padded to 16 rows
+ * of which 4 are used; cols 96 for l = 0, else 64), then the biases (5 x 64, then 16).
+ * Writing rounds them to BF16 for the network and clears the optimiser state. */
+VLR_API VLRResult vlrRadianceCacheNumParameters(uint32_t* numParameters);
+VLR_API VLRResult vlrContextReadRadianceCacheWeights(VLRContextConst context, float* parameters);
+VLR_API VLRResult vlrContextWriteRadianceCacheWeights(VLRContext context, const float* parameters);
+/* Gradient of the loss of the last training step (same layout), its mean loss (relative L2,
+ * (y - t)^2 / (y^2 + 0.01) averaged over samples and outputs), and the number of steps taken. */
+VLR_API VLRResult vlrContextReadRadianceCacheGradient(VLRContextConst context, float* gradient);
+VLR_API VLRResult vlrContextGetRadianceCacheLoss(VLRContextConst context, float* loss, uint32_t* steps);
+/* Direct entry points (device-resident arrays): records are 16 floats each (position in the
+ * scene's bounding cube [0,1]^3, outgoing direction theta/pi and phi/(2 pi), shading normal
+ * (n+1)/2, roughness, diffuse RGB, specular RGB, hero-wavelength offset), outputs and targets
+ * 4 floats each.  Training takes one Adam step on the first numOutputs outputs. */
+VLR_API VLRResult vlrContextRadianceCacheInfer(VLRContext context, hipStream_t stream, const float* records,
+                                               uint32_t numRecords, float* outputs);
+VLR_API VLRResult vlrContextRadianceCacheTrain(VLRContext context, hipStream_t stream, const float* records,
+                                               const float* targets, uint32_t numRecords, uint32_t numOutputs);
 
 /* ---- traversal entry point ---- */
 typedef struct VLRRay { float org[3]; float tmin; float dir[3]; float tmax; } VLRRay;

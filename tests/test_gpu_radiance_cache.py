@@ -1,0 +1,179 @@
+"""Radiance cache on the GPU against the oracle (oracle/radiance_cache.py) and against the plain
+path tracer.
+
+  inference    GPU outputs equal the oracle's BF16 network on the same weights and records
+  gradient     one training step: loss and every layer's gradient equal the oracle's (the two
+               differ only by FP32 vs FP64 accumulation and the BF16 roundings that follow)
+  Adam         the updated weights equal the oracle's Adam step applied to the GPU gradient
+  determinism  the same seed and batches give bit-identical weights (fixed-order reductions)
+  fitting      the loss on a smooth target function falls by 20x and held-out predictions
+               follow the function
+  rendering    with the cache on, the Cornell box at equal sample count is closer to a
+               converged plain render than the plain path tracer is
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import radiance_cache as rc
+from vlr_amd import api, scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def records(n, seed):
+    return np.random.default_rng(seed).uniform(0.0, 1.0, size=(n, 16)).astype(np.float32)
+
+
+def infer(torch, ctx, raw):
+    r = torch.from_numpy(raw).cuda()
+    out = torch.zeros((len(raw), 4), dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
+    ctx.cache_infer(r.data_ptr(), len(raw), out.data_ptr())
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+def train(torch, ctx, raw, targets, num_out):
+    r = torch.from_numpy(raw).cuda()
+    t = torch.from_numpy(np.ascontiguousarray(targets, np.float32)).cuda()
+    torch.cuda.synchronize()
+    ctx.cache_train(r.data_ptr(), t.data_ptr(), len(raw), num_out)
+    torch.cuda.synchronize()
+
+
+def with_biases(params, seed):
+    p = params.copy()
+    p[rc.NUM_WEIGHTS:rc.NUM_WEIGHTS + 320] = np.random.default_rng(seed).normal(0, 0.05, 320)
+    p[rc.NUM_WEIGHTS + 320:rc.NUM_WEIGHTS + 324] = 0.3
+    return p
+
+
+def test_inference_matches_oracle(torch_gpu):
+    with api.Context() as ctx:
+        ctx.reset_radiance_cache()
+        params = with_biases(ctx.read_cache_weights(), 1)
+        ctx.write_cache_weights(params)
+        assert np.array_equal(ctx.read_cache_weights(), params)
+        raw = records(5000, 2)
+        y = infer(torch_gpu, ctx, raw)
+    ref, _ = rc.forward(params, raw)
+    scale = np.sqrt((ref ** 2).mean())
+    err = np.abs(y - ref) / (np.abs(ref) + 0.05 * scale)
+    print("inference: rms output", scale, "relative error p50 / p99 / max", np.quantile(err, [0.5, 0.99]), err.max())
+    assert np.quantile(err, 0.99) < 2e-2 and np.median(err) < 2e-3
+
+
+@pytest.mark.parametrize("num_out", [3, 4])
+def test_training_step_matches_oracle(torch_gpu, num_out):
+    n = 3000   # not a multiple of the 128-sample training block
+    raw = records(n, 3)
+    targets = np.random.default_rng(4).uniform(0.0, 2.0, size=(n, 4)).astype(np.float32)
+    with api.Context() as ctx:
+        ctx.reset_radiance_cache()
+        params = with_biases(ctx.read_cache_weights(), 5)
+        ctx.write_cache_weights(params)
+        train(torch_gpu, ctx, raw, targets, num_out)
+        g = ctx.read_cache_gradient().astype(np.float64)
+        loss, steps = ctx.cache_loss()
+        after = ctx.read_cache_weights()
+        st = ctx.cache_settings()
+    assert steps == 1
+    ref_loss, ref_g = rc.loss_and_grad(params, raw, targets, num_out)
+    print("loss gpu", loss, "oracle", ref_loss)
+    assert abs(loss - ref_loss) < 2e-3 * ref_loss
+    for l in range(rc.LAYERS):
+        r, c, o = rc.layer_rows(l), rc.layer_cols(l), rc.weight_offset(l)
+        for name, a, b in (("W", g[o:o + r * c], ref_g[o:o + r * c]),
+                           ("b", g[rc.NUM_WEIGHTS + 64 * l:rc.NUM_WEIGHTS + 64 * l + r],
+                            ref_g[rc.NUM_WEIGHTS + 64 * l:rc.NUM_WEIGHTS + 64 * l + r])):
+            rel = np.linalg.norm(a - b) / np.linalg.norm(b)
+            cos = a @ b / (np.linalg.norm(a) * np.linalg.norm(b))
+            print(f"layer {l} {name}: relative error {rel:.2e}, cosine {cos:.6f}")
+            assert rel < 3e-2 and cos > 0.9995, (l, name)
+    o5 = rc.weight_offset(5)
+    assert np.all(g[o5 + 64 * num_out:o5 + 16 * 64] == 0)   # outputs outside the loss get nothing
+    # Adam from zero moments on the GPU's own gradient
+    want, _, _ = rc.adam(params, np.zeros_like(params), np.zeros_like(params), g.astype(np.float32), 1,
+                         st["learningRate"], st["beta1"], st["beta2"], st["epsilon"])
+    np.testing.assert_allclose(after, want, rtol=1e-5, atol=1e-7)
+
+
+def test_training_is_deterministic(torch_gpu):
+    raw = records(20000, 6)
+    targets = np.random.default_rng(7).uniform(0.0, 1.0, size=(20000, 4)).astype(np.float32)
+    ws = []
+    for _ in range(2):
+        with api.Context() as ctx:
+            ctx.reset_radiance_cache()
+            for k in range(3):
+                train(torch_gpu, ctx, raw[k * 5000:(k + 2) * 5000], targets[k * 5000:(k + 2) * 5000], 4)
+            ws.append(ctx.read_cache_weights())
+    assert np.array_equal(ws[0].view(np.uint32), ws[1].view(np.uint32))
+
+
+def smooth_target(raw):
+    x, y, z = raw[:, 0], raw[:, 1], raw[:, 2]
+    base = 0.6 + 0.4 * np.sin(2 * math.pi * x) * np.cos(2 * math.pi * z)
+    t = np.stack([base * (0.5 + 0.5 * y), base * (1.0 - 0.5 * y), 0.3 + 0.2 * x, 0.5 + 0.5 * raw[:, 9]], axis=1)
+    return t.astype(np.float32)
+
+
+def test_training_fits_a_smooth_function(torch_gpu):
+    train_raw = records(65536, 8)
+    train_t = smooth_target(train_raw)
+    test_raw = records(4096, 9)
+    with api.Context() as ctx:
+        ctx.reset_radiance_cache()
+        losses = []
+        for step in range(300):
+            lo = (step * 8192) % 65536
+            train(torch_gpu, ctx, train_raw[lo:lo + 8192], train_t[lo:lo + 8192], 4)
+            losses.append(ctx.cache_loss()[0])
+        pred = infer(torch_gpu, ctx, test_raw)
+    first, last = np.mean(losses[:5]), np.mean(losses[-20:])
+    err = np.abs(pred - smooth_target(test_raw)).mean() / np.abs(smooth_target(test_raw)).mean()
+    print("loss first", first, "last", last, "held-out mean relative error", err)
+    assert last < first / 20
+    assert err < 0.1
+
+
+# ---------------------------------------------------------------------------------------------
+def render(desc, W, H, frames, cache=False, seed=None, warmup_frames=0, **settings):
+    """Renders `frames` samples per pixel; with the cache, `warmup_frames` frames train it first
+    and the accumulation restarts afterwards (the network keeps its weights)."""
+    with api.Context(max_path_length=8) as ctx:
+        bs = api.build_scene(ctx, desc)
+        ctx.bind_output(W, H)
+        if seed is not None:
+            ctx.write_rng(np.random.default_rng(seed).integers(1, 2 ** 63, size=W * H, dtype=np.uint64))
+        if cache:
+            ctx.set_radiance_cache(True)
+            ctx.set_cache_settings(**settings)
+            for f in range(warmup_frames):
+                ctx.render(bs.camera, f == 0, 1 << 30)
+            ctx.reset_stats()
+        for f in range(frames):
+            ctx.render(bs.camera, f == 0, frames)
+        img = ctx.read_output()[..., :3].astype(np.float64)
+        st = ctx.stats()
+        loss = ctx.cache_loss() if cache else None
+    return img, st, loss
+
+
+def test_cache_lowers_error_at_equal_samples(torch_gpu):
+    desc = scenes.cornell_box()
+    W = H = 96
+    ref, _, _ = render(desc, W, H, 4096, seed=11)          # independent of the renders below
+    spp = 256
+    plain, pst, _ = render(desc, W, H, spp)
+    cached, cst, loss = render(desc, W, H, spp, cache=True, warmup_frames=128, trainingPaths=W * H // 8)
+    rmse = lambda im: math.sqrt(((im - ref) ** 2).mean())
+    rp, rcache = rmse(plain), rmse(cached)
+    print(f"RMSE at {spp} spp against 4096 spp: plain {rp:.5f}, cached {rcache:.5f}; closest rays plain "
+          f"{pst['closestRays']}, cached {cst['closestRays']}; loss {loss}; means {plain.mean():.5f} "
+          f"{cached.mean():.5f} {ref.mean():.5f}")
+    assert cst["closestRays"] < pst["closestRays"]          # paths end early in the cache
+    assert abs(cached.mean() - ref.mean()) < 0.02 * ref.mean()
+    assert rcache < 0.85 * rp

@@ -65,7 +65,14 @@ class VLRRenderStats(C.Structure):
                 ("trianglesClosest", C.c_uint64), ("instancesClosest", C.c_uint64), ("nodesShadow", C.c_uint64),
                 ("trianglesShadow", C.c_uint64), ("instancesShadow", C.c_uint64), ("msTail", C.c_double),
                 ("launchesTail", C.c_uint64), ("closestRaysWavefront", C.c_uint64),
-                ("shadowRaysWavefront", C.c_uint64), ("msLightPaths", C.c_double), ("msEyePaths", C.c_double)]
+                ("shadowRaysWavefront", C.c_uint64), ("msLightPaths", C.c_double), ("msEyePaths", C.c_double),
+                ("msCacheQuery", C.c_double), ("msCacheTrain", C.c_double)]
+
+
+class VLRRadianceCacheSettings(C.Structure):
+    _fields_ = [("learningRate", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("epsilon", C.c_float),
+                ("trainingPaths", C.c_uint32), ("terminationThreshold", C.c_float), ("warmupSteps", C.c_uint32),
+                ("seed", C.c_uint32)]
 
 
 RAY_DTYPE = np.dtype([("org", np.float32, 3), ("tmin", np.float32), ("dir", np.float32, 3), ("tmax", np.float32)])
@@ -196,6 +203,18 @@ PROTOTYPES: Dict[str, list] = {
     "vlrContextReadLightRNGStates": [H, P],
     "vlrContextWriteLightRNGStates": [H, P],
     "vlrContextSetNumLightPaths": [H, U32],
+    "vlrContextSetRadianceCache": [H, B],
+    "vlrContextGetRadianceCache": [H, C.POINTER(B)],
+    "vlrContextSetRadianceCacheSettings": [H, C.POINTER(VLRRadianceCacheSettings)],
+    "vlrContextGetRadianceCacheSettings": [H, C.POINTER(VLRRadianceCacheSettings)],
+    "vlrContextResetRadianceCache": [H],
+    "vlrRadianceCacheNumParameters": [C.POINTER(U32)],
+    "vlrContextReadRadianceCacheWeights": [H, P],
+    "vlrContextWriteRadianceCacheWeights": [H, P],
+    "vlrContextReadRadianceCacheGradient": [H, P],
+    "vlrContextGetRadianceCacheLoss": [H, C.POINTER(C.c_float), C.POINTER(U32)],
+    "vlrContextRadianceCacheInfer": [H, H, P, U32, P],
+    "vlrContextRadianceCacheTrain": [H, H, P, P, U32, U32],
     "vlrContextGetNumLightPaths": [H, C.POINTER(U32)],
     "vlrContextGetAccumulationChannels": [H, C.POINTER(U32)],
     "vlrContextReadAccumulation": [H, P],
@@ -332,6 +351,51 @@ class Context:
         assert s.size == self.num_light_paths()
         call("vlrContextWriteLightRNGStates", self.h, s.ctypes.data)
 
+    # ---- radiance cache ----
+    def set_radiance_cache(self, on: bool):
+        call("vlrContextSetRadianceCache", self.h, on)
+
+    def cache_settings(self) -> dict:
+        st = VLRRadianceCacheSettings()
+        call("vlrContextGetRadianceCacheSettings", self.h, C.byref(st))
+        return {f: getattr(st, f) for f, _ in VLRRadianceCacheSettings._fields_}
+
+    def set_cache_settings(self, **kw):
+        cur = self.cache_settings()
+        cur.update(kw)
+        st = VLRRadianceCacheSettings(**cur)
+        call("vlrContextSetRadianceCacheSettings", self.h, C.byref(st))
+
+    def reset_radiance_cache(self):
+        call("vlrContextResetRadianceCache", self.h)
+
+    def read_cache_weights(self) -> np.ndarray:
+        out = np.zeros(cache_num_parameters(), np.float32)
+        call("vlrContextReadRadianceCacheWeights", self.h, out.ctypes.data)
+        return out
+
+    def write_cache_weights(self, params: np.ndarray):
+        a = np.ascontiguousarray(params, np.float32)
+        assert a.size == cache_num_parameters()
+        call("vlrContextWriteRadianceCacheWeights", self.h, a.ctypes.data)
+
+    def read_cache_gradient(self) -> np.ndarray:
+        out = np.zeros(cache_num_parameters(), np.float32)
+        call("vlrContextReadRadianceCacheGradient", self.h, out.ctypes.data)
+        return out
+
+    def cache_loss(self):
+        loss, steps = C.c_float(), U32()
+        call("vlrContextGetRadianceCacheLoss", self.h, C.byref(loss), C.byref(steps))
+        return loss.value, steps.value
+
+    def cache_infer(self, records_ptr: int, n: int, out_ptr: int, stream=None):
+        call("vlrContextRadianceCacheInfer", self.h, stream, C.c_void_p(records_ptr), n, C.c_void_p(out_ptr))
+
+    def cache_train(self, records_ptr: int, targets_ptr: int, n: int, num_out: int, stream=None):
+        call("vlrContextRadianceCacheTrain", self.h, stream, C.c_void_p(records_ptr), C.c_void_p(targets_ptr), n,
+             num_out)
+
     def read_accumulation(self) -> np.ndarray:
         ch = U32()
         call("vlrContextGetAccumulationChannels", self.h, C.byref(ch))
@@ -380,6 +444,12 @@ class Context:
 
     def evaluate_bsdfs(self, material, spectral: bool, q_ptr: int, n: int, r_ptr: int, stream=None):
         call("vlrContextEvaluateBSDFs", self.h, stream, material, spectral, C.c_void_p(q_ptr), n, C.c_void_p(r_ptr))
+
+
+def cache_num_parameters() -> int:
+    n = U32()
+    call("vlrRadianceCacheNumParameters", C.byref(n))
+    return n.value
 
 
 def set_spectrum(obj, name: str, cs: str, e0: float, e1: float, e2: float):

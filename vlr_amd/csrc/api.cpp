@@ -1066,6 +1066,8 @@ VLR_API VLRResult vlrContextGetRenderStats(VLRContextConst context, VLRRenderSta
     stats->launchesTail = c->launches[STAGE_TAIL];
     stats->msLightPaths = c->stageMs[STAGE_LIGHT_PATHS];
     stats->msEyePaths = c->stageMs[STAGE_EYE_PATHS];
+    stats->msCacheQuery = c->stageMs[STAGE_CACHE_QUERY];
+    stats->msCacheTrain = c->stageMs[STAGE_CACHE_TRAIN];
     return VLRResult_NoError;
     VLRA_CATCH
 }
@@ -1188,6 +1190,134 @@ VLR_API VLRResult vlrContextGetNumLightPaths(VLRContextConst context, uint32_t *
     return VLRResult_NoError;
     VLRA_CATCH
 }
+// ---- radiance cache ----
+VLR_API VLRResult vlrContextSetRadianceCache(VLRContext context, bool enable) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    CTX(context)->cacheEnabled = enable;
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+VLR_API VLRResult vlrContextGetRadianceCache(VLRContextConst context, bool *enabled) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    if (!enabled) return VLRResult_InvalidArgument;
+    *enabled = CTX(context)->cacheEnabled;
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+VLR_API VLRResult vlrContextSetRadianceCacheSettings(VLRContext context, const VLRRadianceCacheSettings *st) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    if (!st) return VLRResult_InvalidArgument;
+    if (st->trainingPaths < 1 || st->trainingPaths > rc::kMaxBatch / kCacheTrainVerts) return VLRResult_InvalidArgument;
+    if (!(st->learningRate > 0.0f) || !(st->beta1 >= 0.0f && st->beta1 < 1.0f) || !(st->beta2 >= 0.0f && st->beta2 < 1.0f) ||
+        !(st->epsilon > 0.0f) || !(st->terminationThreshold >= 0.0f))
+        return VLRResult_InvalidArgument;
+    Context::CacheSettings &c = CTX(context)->cacheSettings;
+    c.learningRate = st->learningRate;
+    c.beta1 = st->beta1;
+    c.beta2 = st->beta2;
+    c.epsilon = st->epsilon;
+    c.trainingPaths = st->trainingPaths;
+    c.threshold = st->terminationThreshold;
+    c.warmupSteps = st->warmupSteps;
+    c.seed = st->seed;
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+VLR_API VLRResult vlrContextGetRadianceCacheSettings(VLRContextConst context, VLRRadianceCacheSettings *st) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    if (!st) return VLRResult_InvalidArgument;
+    const Context::CacheSettings &c = CTX(context)->cacheSettings;
+    st->learningRate = c.learningRate;
+    st->beta1 = c.beta1;
+    st->beta2 = c.beta2;
+    st->epsilon = c.epsilon;
+    st->trainingPaths = c.trainingPaths;
+    st->terminationThreshold = c.threshold;
+    st->warmupSteps = c.warmupSteps;
+    st->seed = c.seed;
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+VLR_API VLRResult vlrContextResetRadianceCache(VLRContext context) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    CTX(context)->resetCache();
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+VLR_API VLRResult vlrRadianceCacheNumParameters(uint32_t *numParameters) {
+    if (!numParameters) return VLRResult_InvalidArgument;
+    *numParameters = rc::kNumParams;
+    return VLRResult_NoError;
+}
+VLR_API VLRResult vlrContextReadRadianceCacheWeights(VLRContextConst context, float *parameters) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    if (!parameters) return VLRResult_InvalidArgument;
+    CTX(context)->readCacheWeights(parameters);
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+VLR_API VLRResult vlrContextWriteRadianceCacheWeights(VLRContext context, const float *parameters) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    if (!parameters) return VLRResult_InvalidArgument;
+    Context *c = CTX(context);
+    c->ensureCacheNet();
+    VLRA_HIP(hipDeviceSynchronize());
+    c->writeCacheWeights(parameters);
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+VLR_API VLRResult vlrContextReadRadianceCacheGradient(VLRContextConst context, float *gradient) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    if (!gradient) return VLRResult_InvalidArgument;
+    Context *c = CTX(context);
+    c->ensureCacheNet();
+    VLRA_HIP(hipDeviceSynchronize());
+    VLRA_HIP(hipMemcpy(gradient, c->rcGrad.ptr, rc::kNumParams * sizeof(float), hipMemcpyDeviceToHost));
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+VLR_API VLRResult vlrContextGetRadianceCacheLoss(VLRContextConst context, float *loss, uint32_t *steps) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    Context *c = CTX(context);
+    if (loss) {
+        *loss = 0.0f;
+        if (c->cacheInitialized) {
+            VLRA_HIP(hipDeviceSynchronize());
+            VLRA_HIP(hipMemcpy(loss, c->rcLoss.ptr, sizeof(float), hipMemcpyDeviceToHost));
+        }
+    }
+    if (steps) *steps = c->cacheStep;
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+VLR_API VLRResult vlrContextRadianceCacheInfer(VLRContext context, hipStream_t stream, const float *records,
+                                               uint32_t numRecords, float *outputs) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    if ((!records || !outputs) && numRecords) return VLRResult_InvalidArgument;
+    CTX(context)->cacheInfer(stream, records, numRecords, outputs);
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+VLR_API VLRResult vlrContextRadianceCacheTrain(VLRContext context, hipStream_t stream, const float *records,
+                                               const float *targets, uint32_t numRecords, uint32_t numOutputs) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    if (!records || !targets) return VLRResult_InvalidArgument;
+    CTX(context)->cacheTrain(stream, records, targets, numRecords, numOutputs);
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+
 VLR_API VLRResult vlrContextGetAccumulationChannels(VLRContextConst context, uint32_t *channels) {
     VLRA_TRY
     CHECK(context, IS(context, Context));

@@ -151,6 +151,11 @@ void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCo
         fp.traversalStats = traversalStats ? 1 : 0;
         fp.wavefrontDepth = wavefrontDepth;
         fp.tailBlocks = tailBlocks;
+        fp.cacheMode = 0;
+        fp.frameStamp = 0;
+        fp.cacheThreshold = 0.0f;
+        fp.trainStride = 1;
+        fp.trainOffset = 0;
         if (bpt) {
             renderBptFrame(stream, fp, firstFrame);
             if (numAccum) *numAccum = numAccumFrames;
@@ -159,7 +164,7 @@ void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCo
         // autotune the schedule per (scene version, size, mode, path length)
         const uint64_t tuneKey = (version * 1000003ull) ^ (uint64_t(width) << 40) ^ (uint64_t(height) << 20) ^
                                  (uint64_t(fp.imageW) << 8) ^ (spectralActive ? 1u : 0u) ^
-                                 (uint64_t(fp.maxPathLength) << 2);
+                                 (uint64_t(fp.maxPathLength) << 2) ^ (cacheEnabled ? 0x2ull : 0ull);
         bool tuning = false;
         if (autotune && !traversalStats) {
             if (tuner.key != tuneKey || tuner.cands.empty()) tuner.start(tuneKey);
@@ -179,13 +184,51 @@ void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCo
             }
             VLRA_HIP(hipEventRecord(tuneEvents[0], stream));
         }
-        const FrameBuffers fb = frameBuffers();
+        FrameBuffers fb = frameBuffers();
+        CacheLaunch cl;
+        const CacheLaunch *clp = nullptr;
+        if (cacheEnabled) {
+            ensureCacheNet();
+            ensureCacheFrameBuffers();
+            const uint32_t n = fp.imageW * fp.imageH;
+            const uint32_t stride = std::max(1u, (n + cacheSettings.trainingPaths - 1) / cacheSettings.trainingPaths);
+            fp.frameStamp = ++frameStamp;
+            fp.cacheMode = cacheStep >= cacheSettings.warmupSteps ? 2 : 1;
+            fp.cacheThreshold = cacheSettings.threshold;
+            fp.trainStride = stride;
+            fp.trainOffset = fp.frameStamp % stride;
+            fb.cache.queryRaw = static_cast<float4 *>(cqRaw.ptr);
+            fb.cache.queryThroughput = static_cast<float4 *>(cqThroughput.ptr);
+            fb.cache.queryStamp = static_cast<uint32_t *>(cqStamp.ptr);
+            fb.cache.spread = static_cast<float2 *>(cSpread.ptr);
+            fb.cache.trainRaw = static_cast<float4 *>(ctRaw.ptr);
+            fb.cache.trainThroughput = static_cast<float4 *>(ctThroughput.ptr);
+            fb.cache.trainContrib = static_cast<float4 *>(ctContrib.ptr);
+            fb.cache.trainStamp = static_cast<uint32_t *>(ctStamp.ptr);
+            fb.cache.trainAfterStamp = static_cast<uint32_t *>(ctAfterStamp.ptr);
+            cl.net = rcDevice();
+            cl.train.learningRate = cacheSettings.learningRate;
+            cl.train.beta1 = cacheSettings.beta1;
+            cl.train.beta2 = cacheSettings.beta2;
+            cl.train.epsilon = cacheSettings.epsilon;
+            cl.train.step = ++cacheStep;
+            cl.train.numOut = spectralActive ? 4 : 3;
+            cl.numSlots = ((n + stride - 1) / stride) * kCacheTrainVerts;
+            cl.batchRaw = static_cast<float4 *>(cbRaw.ptr);
+            cl.batchTarget = static_cast<float4 *>(cbTarget.ptr);
+            cl.batchCount = static_cast<uint32_t *>(cbCount.ptr);
+            cl.slotFlags = static_cast<uint32_t *>(csFlags.ptr);
+            cl.slotOffsets = static_cast<uint32_t *>(csOffsets.ptr);
+            cl.slotBlockSums = static_cast<uint32_t *>(csBlockSums.ptr);
+            cl.slotTargets = static_cast<float4 *>(csTargets.ptr);
+            clp = &cl;
+        }
         if (kernelTiming) {
             EventTimer t(stream, eventPool);
-            VLRA_HIP(launchFrame(gpu.dev, fp, fb, spectralActive, stream, &t));
+            VLRA_HIP(launchFrame(gpu.dev, fp, fb, spectralActive, stream, &t, clp));
             t.collect(stageMs, launches);
         } else {
-            VLRA_HIP(launchFrame(gpu.dev, fp, fb, spectralActive, stream, nullptr));
+            VLRA_HIP(launchFrame(gpu.dev, fp, fb, spectralActive, stream, nullptr, clp));
         }
         if (tuning) {
             VLRA_HIP(hipEventRecord(tuneEvents[1], stream));
@@ -219,6 +262,7 @@ FrameBuffers Context::frameBuffers() {
     fb.counters = static_cast<uint32_t *>(counters.ptr);
     fb.accum = static_cast<float *>(accum.ptr);
     fb.output = static_cast<float4 *>(output.ptr);
+    fb.cache = CacheBuffers{};
     return fb;
 }
 
@@ -302,6 +346,151 @@ void Context::writeLightRngStates(const uint64_t *src) {
     ensureBpt(std::min<uint32_t>(maxPathLength, kMaxDepthCounters - 1) + 1);
     VLRA_HIP(hipDeviceSynchronize());
     VLRA_HIP(hipMemcpy(lightRng.ptr, src, size_t(numLightPaths) * 8, hipMemcpyHostToDevice));
+}
+
+// ---- radiance cache ----
+namespace {
+uint16_t bf16Host(float x) {   // round to nearest even
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return static_cast<uint16_t>(u >> 16);
+}
+// PCG32 (O'Neill 2014) for the weight initialisation
+struct HostPcg {
+    uint64_t state;
+    uint32_t next() {
+        uint64_t old = state;
+        state = old * 6364136223846793005ULL + 1ULL;
+        uint32_t xs = static_cast<uint32_t>(((old >> 18u) ^ old) >> 27u);
+        uint32_t rot = static_cast<uint32_t>(old >> 59u);
+        return (xs >> rot) | (xs << ((32 - rot) & 31));
+    }
+    float uniform() { return (next() >> 8) * (1.0f / 16777216.0f); }
+};
+}  // namespace
+
+RcDevice Context::rcDevice() {
+    RcDevice d;
+    d.w = static_cast<uint16_t *>(rcW.ptr);
+    d.wt = static_cast<uint16_t *>(rcWT.ptr);
+    d.bias = static_cast<float *>(rcBias.ptr);
+    d.master = static_cast<float *>(rcMaster.ptr);
+    d.adamM = static_cast<float *>(rcM.ptr);
+    d.adamV = static_cast<float *>(rcV.ptr);
+    d.grad = static_cast<float *>(rcGrad.ptr);
+    d.partial = static_cast<float *>(rcPartial.ptr);
+    d.act = static_cast<uint16_t *>(rcAct.ptr);
+    d.delta = static_cast<uint16_t *>(rcDelta.ptr);
+    d.lossPartial = static_cast<float *>(rcLossPartial.ptr);
+    d.loss = static_cast<float *>(rcLoss.ptr);
+    return d;
+}
+
+void Context::writeCacheWeights(const float *master) {
+    ensureDevice();
+    std::vector<uint16_t> w(rc::kNumWeights), wt(rc::kNumWeights);
+    for (uint32_t l = 0; l < rc::kLayers; ++l) {
+        const uint32_t rows = rc::layerRows(l), cols = rc::layerCols(l), off = rc::weightOffset(l);
+        for (uint32_t r = 0; r < rows; ++r)
+            for (uint32_t c = 0; c < cols; ++c) {
+                uint16_t h = bf16Host(master[off + r * cols + c]);
+                w[off + r * cols + c] = h;
+                wt[off + c * rows + r] = h;
+            }
+    }
+    rcMaster.upload(master, rc::kNumParams * sizeof(float));
+    rcW.upload(w.data(), w.size() * 2);
+    rcWT.upload(wt.data(), wt.size() * 2);
+    rcBias.upload(master + rc::kNumWeights, rc::kNumBiases * sizeof(float));
+    std::vector<float> zeros(rc::kNumParams, 0.0f);
+    rcM.upload(zeros.data(), zeros.size() * sizeof(float));
+    rcV.upload(zeros.data(), zeros.size() * sizeof(float));
+    rcGrad.upload(zeros.data(), zeros.size() * sizeof(float));
+    cacheStep = 0;
+}
+
+void Context::readCacheWeights(float *master) {
+    ensureCacheNet();
+    VLRA_HIP(hipDeviceSynchronize());
+    VLRA_HIP(hipMemcpy(master, rcMaster.ptr, rc::kNumParams * sizeof(float), hipMemcpyDeviceToHost));
+}
+
+// Glorot-uniform weights from PCG32(seed), zero biases; padding rows of the output layer stay 0.
+void Context::resetCache() {
+    ensureDevice();
+    std::vector<float> master(rc::kNumParams, 0.0f);
+    HostPcg rng{0x853c49e6748fea9bULL ^ (static_cast<uint64_t>(cacheSettings.seed) * 0x9E3779B97F4A7C15ULL)};
+    for (uint32_t l = 0; l < rc::kLayers; ++l) {
+        const uint32_t rows = rc::layerRows(l), cols = rc::layerCols(l), off = rc::weightOffset(l);
+        const uint32_t fanOut = l == rc::kLayers - 1 ? rc::kOut : rows;
+        const float a = std::sqrt(6.0f / static_cast<float>(cols + fanOut));
+        for (uint32_t r = 0; r < rows; ++r)
+            for (uint32_t c = 0; c < cols; ++c) {
+                float u = rng.uniform();
+                master[off + r * cols + c] = r < fanOut ? a * (2.0f * u - 1.0f) : 0.0f;
+            }
+    }
+    rcPartial.reserve(size_t(rc::kMaxSlices) * rc::kNumParams * sizeof(float));
+    rcAct.reserve(size_t(rc::kActRows) * rc::kMaxBatch * 2);
+    rcDelta.reserve(size_t(rc::kDeltaRows) * rc::kMaxBatch * 2);
+    rcLossPartial.reserve(rc::kMaxBatch / 128 * sizeof(float));
+    rcLoss.reserve(sizeof(float));
+    VLRA_HIP(hipMemset(rcLoss.ptr, 0, sizeof(float)));
+    cbRaw.reserve(size_t(rc::kMaxBatch) * 4 * sizeof(float4));
+    cbTarget.reserve(size_t(rc::kMaxBatch) * sizeof(float4));
+    cbCount.reserve(sizeof(uint32_t));
+    csFlags.reserve(size_t(rc::kMaxBatch) * sizeof(uint32_t));
+    csOffsets.reserve(size_t(rc::kMaxBatch) * sizeof(uint32_t));
+    csBlockSums.reserve(64 * sizeof(uint32_t));
+    csTargets.reserve(size_t(rc::kMaxBatch) * sizeof(float4));
+    writeCacheWeights(master.data());
+    cacheInitialized = true;
+}
+
+void Context::ensureCacheNet() {
+    if (!cacheInitialized) resetCache();
+}
+
+void Context::ensureCacheFrameBuffers() {
+    const size_t n = size_t(width) * height;
+    const size_t bytesRaw = n * 4 * sizeof(float4);
+    if (cqRaw.bytes < bytesRaw) {
+        cqRaw.reserve(bytesRaw);
+        cqThroughput.reserve(n * sizeof(float4));
+        cqStamp.reserve(n * sizeof(uint32_t));
+        VLRA_HIP(hipMemset(cqStamp.ptr, 0, n * sizeof(uint32_t)));
+        cSpread.reserve(n * sizeof(float2));
+    }
+    if (!ctRaw.ptr) {
+        ctRaw.reserve(size_t(rc::kMaxBatch) * 4 * sizeof(float4));
+        ctThroughput.reserve(size_t(rc::kMaxBatch) * sizeof(float4));
+        ctContrib.reserve(size_t(rc::kMaxBatch) * sizeof(float4));
+        ctStamp.reserve(size_t(rc::kMaxBatch) * sizeof(uint32_t));
+        VLRA_HIP(hipMemset(ctStamp.ptr, 0, size_t(rc::kMaxBatch) * sizeof(uint32_t)));
+        ctAfterStamp.reserve(size_t(rc::kMaxBatch) * sizeof(uint32_t));
+        VLRA_HIP(hipMemset(ctAfterStamp.ptr, 0, size_t(rc::kMaxBatch) * sizeof(uint32_t)));
+    }
+}
+
+void Context::cacheInfer(hipStream_t stream, const float *records, uint32_t n, float *out) {
+    ensureCacheNet();
+    VLRA_HIP(rcLaunchInfer(rcDevice(), records, nullptr, n, out, stream));
+}
+
+void Context::cacheTrain(hipStream_t stream, const float *records, const float *targets, uint32_t n,
+                         uint32_t numOut) {
+    if (n == 0 || n > rc::kMaxBatch) throw ApiError{3, "training batch must hold 1 to 65536 samples"};
+    if (numOut < 1 || numOut > 4) throw ApiError{3, "outputs in the loss must be 1 to 4"};
+    ensureCacheNet();
+    RcTrainParams p;
+    p.learningRate = cacheSettings.learningRate;
+    p.beta1 = cacheSettings.beta1;
+    p.beta2 = cacheSettings.beta2;
+    p.epsilon = cacheSettings.epsilon;
+    p.step = ++cacheStep;
+    p.numOut = numOut;
+    VLRA_HIP(rcLaunchTrain(rcDevice(), p, records, targets, nullptr, n, stream));
 }
 
 void Context::readOutput(float *dst) {

@@ -138,6 +138,7 @@ struct DevScene {
     const float4* envTex;
     const MengCellDev* mengGrid;
     const float* mengPoints;     // 186 x 99 floats: xy*(2) uv(2) spectrum(95)
+    const float4* cacheMaterials; // per material: (diffuse rgb, roughness), (specular rgb, 0) for the radiance cache
     uint32_t numInstances;
     uint32_t hasTlas;
     // light selection over [environment, instances...]

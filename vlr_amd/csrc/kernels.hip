@@ -23,6 +23,7 @@
 // reproduce the oracle's brute force bit for bit.
 #include "dev_shading.h"
 #include "launch.h"
+#include "radiance_cache.h"
 
 #pragma clang fp contract(off)
 
@@ -651,6 +652,7 @@ struct PathRegs {
     uint32_t prevType;
     Pcg32 rng;
     bool rngUsed;
+    float a0, a;    // radiance cache: path spread at the first vertex and accumulated since
 };
 
 template <int N>
@@ -667,6 +669,51 @@ __device__ __forceinline__ void loadPath(const FrameBuffers &fb, uint32_t path, 
     P->prevType = (flags >> kFlagPrevTypeShift) & 0x7Fu;
     P->rng.s = fb.ps.rng[path];
     P->rngUsed = false;
+    P->a0 = P->a = 0.0f;
+    if (fb.cache.spread) {
+        float2 sp = fb.cache.spread[path];
+        P->a0 = sp.x;
+        P->a = sp.y;
+    }
+}
+
+// Radiance-cache view of a frame for the shading code
+struct CacheCtx {
+    uint32_t mode, stamp, stride, offset, width, imageW;
+    float threshold;
+    CacheBuffers b;
+};
+__device__ __forceinline__ CacheCtx cacheCtx(const FrameParams &fp, const FrameBuffers &fb) {
+    CacheCtx c;
+    c.mode = fp.cacheMode;
+    c.stamp = fp.frameStamp;
+    c.stride = fp.trainStride;
+    c.offset = fp.trainOffset;
+    c.width = fp.width;
+    c.imageW = fp.imageW;
+    c.threshold = fp.cacheThreshold;
+    c.b = fb.cache;
+    return c;
+}
+
+// Raw radiance-cache input record of a surface point (4 float4; the layout the cache's
+// encoder reads): position in the scene's bounding cube, outgoing direction (theta / pi,
+// phi / 2 pi), shading normal (n + 1) / 2, roughness, diffuse and specular albedo, hero offset.
+__device__ __forceinline__ void writeCacheRecord(const DevScene &sc, const SurfPt &sp, uint32_t material,
+                                                 const V3 &rayDir, float wlOffset, float4 *dst) {
+    const float inv = 0.5f / sc.worldRadius;
+    const float px = fminf(fmaxf((sp.position.x - sc.center[0]) * inv + 0.5f, 0.0f), 1.0f);
+    const float py = fminf(fmaxf((sp.position.y - sc.center[1]) * inv + 0.5f, 0.0f), 1.0f);
+    const float pz = fminf(fmaxf((sp.position.z - sc.center[2]) * inv + 0.5f, 0.0f), 1.0f);
+    const V3 wo = -rayDir;
+    const float theta = acosf(fminf(fmaxf(wo.y, -1.0f), 1.0f)) / kPiF;
+    const float phi = (atan2f(wo.z, wo.x) + kPiF) / (2 * kPiF);
+    const V3 n = sp.frame.z;
+    const float4 m0 = sc.cacheMaterials[2 * material], m1 = sc.cacheMaterials[2 * material + 1];
+    dst[0] = make_float4(px, py, pz, theta);
+    dst[1] = make_float4(phi, 0.5f * (n.x + 1.0f), 0.5f * (n.y + 1.0f), 0.5f * (n.z + 1.0f));
+    dst[2] = make_float4(m0.w, m0.x, m0.y, m0.z);
+    dst[3] = make_float4(m1.x, m1.y, m1.z, wlOffset);
 }
 
 // One bounce of path_tracing.cu:133-337 for a ray that hit (ha.x >= 0) or missed: implicit
@@ -675,11 +722,22 @@ __device__ __forceinline__ void loadPath(const FrameBuffers &fb, uint32_t path, 
 template <int N>
 __device__ __forceinline__ void shadeStep(const DevScene &sc, uint32_t pathLength, uint32_t maxPathLength,
                                           uint32_t path, const V3 &rayOrg, const V3 &rayDir, int4 ha, float2 hb,
-                                          PathRegs<N> &P, bool *emitShadow, float4 *shOrg, float4 *shDir,
-                                          float4 *shVal, bool *emitNext, float4 *nextOrg, float4 *nextDir) {
+                                          PathRegs<N> &P, const CacheCtx &cc, bool *emitShadow, float4 *shOrg,
+                                          float4 *shDir, float4 *shVal, bool *emitNext, float4 *nextOrg,
+                                          float4 *nextDir) {
     *emitShadow = false;
     *emitNext = false;
     const bool maxLengthTerminate = pathLength >= maxPathLength;
+    if (cc.mode != 0 && pathLength >= 2 && pathLength - 1 <= kCacheTrainVerts) {
+        // training path: the contribution before this vertex's emission closes the previous
+        // vertex's sample (everything it gathered up to and including its next-event estimate)
+        const uint32_t pix = (path / cc.width) * cc.imageW + path % cc.width;
+        if (pix % cc.stride == cc.offset) {
+            const uint32_t slot = (pix / cc.stride) * kCacheTrainVerts + pathLength - 2;
+            cc.b.trainContrib[slot] = storeSpec<N>(P.contrib);
+            cc.b.trainAfterStamp[slot] = cc.stamp;
+        }
+    }
     if (ha.x < 0) {
         // miss: environment light with MIS
         if (sc.hasEnv && sc.envImportance > 0.0f) {
@@ -728,11 +786,39 @@ __device__ __forceinline__ void shadeStep(const DevScene &sc, uint32_t pathLengt
             P.contrib += P.alpha * (M * fe) * mis;
         }
     }
+    bool cacheEnd = false;
+    if (cc.mode != 0) {
+        // path spread (Mueller et al. 2021, "Real-time Neural Radiance Caching for Path
+        // Tracing"): a_0 from the primary vertex, then one term per non-delta bounce
+        const float dist2 = (sp.position - rayOrg).sqLength();
+        const float cosT = fabsf(dirOutLocal.z);
+        if (pathLength == 1) {
+            P.a0 = sqrtf(dist2 / (4 * kPiF * cosT));
+            P.a = 0.0f;
+        } else if (!dt::isDelta(P.prevType)) {
+            P.a += sqrtf(dist2 / (P.prevDirPDF * cosT));
+        }
+        const uint32_t pix = (path / cc.width) * cc.imageW + path % cc.width;
+        if (pix % cc.stride == cc.offset) {
+            // training path: vertices 1..kCacheTrainVerts become samples once the path ends
+            if (pathLength <= kCacheTrainVerts) {
+                const uint32_t slot = (pix / cc.stride) * kCacheTrainVerts + pathLength - 1;
+                writeCacheRecord(sc, sp, G.material, rayDir, N == 4 ? P.wls.offset : 0.0f, cc.b.trainRaw + 4 * slot);
+                cc.b.trainThroughput[slot] = storeSpec<N>(P.alpha);
+                cc.b.trainStamp[slot] = cc.stamp;
+            }
+        } else {
+            cacheEnd = cc.mode == 2 && pathLength >= 2 && !maxLengthTerminate && !P.wls.single &&
+                       P.a > cc.threshold * P.a0;
+        }
+    }
     if (maxLengthTerminate) return;
     P.rngUsed = true;
-    float continueProb = fminf(P.alpha.importance(P.wls.sel) / P.initImportance, 1.0f);
-    if (P.rng.uniform() >= continueProb) return;
-    P.alpha = P.alpha / continueProb;
+    if (!cacheEnd) {
+        float continueProb = fminf(P.alpha.importance(P.wls.sel) / P.initImportance, 1.0f);
+        if (P.rng.uniform() >= continueProb) return;
+        P.alpha = P.alpha / continueProb;
+    }
     Bsdf<N> bsdf = setupBsdf<N>(sc, mat, P.wls);
     V3 gnL = sp.frame.toLocal(sp.gn);
     if (bsdf.hasNonDelta()) {
@@ -786,6 +872,14 @@ __device__ __forceinline__ void shadeStep(const DevScene &sc, uint32_t pathLengt
             }
         }
     }
+    if (cacheEnd) {
+        // the cache supplies the indirect radiance leaving this vertex (what a continued path
+        // would add after the next-event estimate above); the path ends here
+        writeCacheRecord(sc, sp, G.material, rayDir, N == 4 ? P.wls.offset : 0.0f, cc.b.queryRaw + 4 * path);
+        cc.b.queryThroughput[path] = storeSpec<N>(P.alpha);
+        cc.b.queryStamp[path] = cc.stamp;
+        return;
+    }
     float uC = P.rng.uniform(), uD0 = P.rng.uniform(), uD1 = P.rng.uniform();
     V3 dirL;
     float dirPDF;
@@ -816,6 +910,7 @@ __device__ __forceinline__ void storePath(const FrameBuffers &fb, uint32_t path,
         fb.ps.alpha[path] = storeSpec<N>(P.alpha);
         fb.ps.meta[path] = make_float4(P.wls.offset, P.prevDirPDF, P.initImportance,
                                        __uint_as_float(packFlags(P.wls.sel, P.wls.single, P.prevType, pathLength)));
+        if (fb.cache.spread) fb.cache.spread[path] = make_float2(P.a0, P.a);
     }
 }
 
@@ -834,8 +929,8 @@ __global__ void __launch_bounds__(256) k_shade(KArgs a) {
         float2 hb = ha.x >= 0 ? a.fb.hits.b[i] : make_float2(0.0f, 0.0f);
         PathRegs<N> P;
         loadPath<N>(a.fb, path, &P);
-        shadeStep<N>(a.sc, depth + 1, a.fp.maxPathLength, path, xyz(o4), xyz(d4), ha, hb, P, &emitShadow, &shOrg,
-                     &shDir, &shVal, &emitNext, &nextOrg, &nextDir);
+        shadeStep<N>(a.sc, depth + 1, a.fp.maxPathLength, path, xyz(o4), xyz(d4), ha, hb, P, cacheCtx(a.fp, a.fb),
+                     &emitShadow, &shOrg, &shDir, &shVal, &emitNext, &nextOrg, &nextDir);
         storePath<N>(a.fb, path, depth + 1, P, emitNext);
     }
     // queue appends (every lane of the wave takes part in the ballots)
@@ -875,63 +970,185 @@ __global__ void __launch_bounds__(kTraceBlock) k_shadow(KArgs a) {
 }
 
 // ---- tail: from depth a.depth on, each path runs intersect -> shade -> shadow in a loop ----
-// Persistent waves take 64 queue entries at a time.  Every path consumes its random numbers
-// and adds its contributions in the same order as the per-depth kernels, so a frame's result
-// does not depend on the depth at which paths switch to this kernel.
+// Persistent waves; a lane whose path has ended takes the next queue entry at the top of the
+// loop (one atomic per wave for all lanes that need work), so lanes stay busy while the queue
+// lasts instead of waiting for the longest path of a fixed chunk.  Every path consumes its
+// random numbers and adds its contributions in the same order as the per-depth kernels, so a
+// frame's result does not depend on the depth at which paths switch to this kernel.  Rays per
+// depth are counted in LDS and added to the frame counters once per block.
 template <int N, bool STATS>
 __global__ void __launch_bounds__(kTraceBlock) k_tail(KArgs a) {
     __shared__ int32_t lds[kLdsStack * kTraceBlock];
+    __shared__ uint32_t depthCounts[2 * kMaxDepthCounters];
     const DevScene &sc = a.sc;
     const uint32_t depth0 = a.depth;
     const uint32_t count = a.fb.counters[2 * depth0];
     const RayQueue q = a.fb.rays[depth0 & 1];
+    const CacheCtx cc = cacheCtx(a.fp, a.fb);
+    for (uint32_t k = threadIdx.x; k < 2 * kMaxDepthCounters; k += blockDim.x) depthCounts[k] = 0;
+    __syncthreads();
     TravCount tc, tcShadow;
+    const uint32_t lane = __lane_id();
+    bool active = false, drained = false;
+    uint32_t path = 0, depth = depth0;
+    PathRegs<N> P;
+    V3 org(0.0f), dir(0.0f, 0.0f, 1.0f);
+    float tmax = 0.0f;
     while (true) {
-        uint32_t base = 0;
-        if (__lane_id() == 0) base = atomicAdd(&a.fb.counters[kTailWork], 64u);
-        base = __shfl(base, 0);
-        if (base >= count) break;
-        const uint32_t i = base + __lane_id();
-        if (i >= count) continue;
-        float4 o4 = q.org[i], d4 = q.dir[i];
-        const uint32_t path = __float_as_uint(o4.w);
-        PathRegs<N> P;
-        loadPath<N>(a.fb, path, &P);
-        V3 org = xyz(o4), dir = xyz(d4);
-        float tmax = d4.w;
-        uint32_t depth = depth0;
-        while (true) {
-            TraceHit h;
-            h.inst = -1;
-            h.b1 = h.b2 = 0.0f;
-            bool found = traceRay<false, STATS>(sc, org, dir, 0.0f, tmax, &h, lds, threadIdx.x, &tc);
-            int4 ha = found ? make_int4(h.inst, h.geomInst, h.prim, 0) : make_int4(-1, -1, -1, 0);
-            bool emitShadow, emitNext;
-            float4 shOrg, shDir, shVal, nextOrg, nextDir;
-            shadeStep<N>(sc, depth + 1, a.fp.maxPathLength, path, org, dir, ha, make_float2(h.b1, h.b2), P,
-                         &emitShadow, &shOrg, &shDir, &shVal, &emitNext, &nextOrg, &nextDir);
-            if (emitShadow) {
-                atomicAdd(&a.fb.counters[2 * depth + 1], 1u);
-                TraceHit sh;
-                bool occluded = traceRay<true, STATS>(sc, xyz(shOrg), xyz(shDir), 0.0f, shDir.w, &sh, lds,
-                                                      threadIdx.x, &tcShadow);
-                if (!occluded) P.contrib += loadSpec<N>(shVal);
+        if (!drained) {
+            const uint64_t need = __ballot(!active);
+            if (need) {
+                const uint32_t n = __popcll(need);
+                const uint32_t leader = __ffsll(static_cast<unsigned long long>(need)) - 1;
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(&a.fb.counters[kTailWork], n);
+                base = __shfl(base, static_cast<int>(leader));
+                if (base + n >= count) drained = true;
+                if (!active) {
+                    const uint32_t i = base + __popcll(need & ((1ull << lane) - 1ull));
+                    if (i < count) {
+                        const float4 o4 = q.org[i], d4 = q.dir[i];
+                        path = __float_as_uint(o4.w);
+                        loadPath<N>(a.fb, path, &P);
+                        org = xyz(o4);
+                        dir = xyz(d4);
+                        tmax = d4.w;
+                        depth = depth0;
+                        active = true;
+                    }
+                }
             }
-            if (!emitNext) {
-                storePath<N>(a.fb, path, depth + 1, P, false);
-                break;
-            }
-            ++depth;
-            atomicAdd(&a.fb.counters[2 * depth], 1u);
-            org = xyz(nextOrg);
-            dir = xyz(nextDir);
-            tmax = nextDir.w;
         }
+        if (!__ballot(active)) break;
+        if (!active) continue;
+        TraceHit h;
+        h.inst = -1;
+        h.b1 = h.b2 = 0.0f;
+        bool found = traceRay<false, STATS>(sc, org, dir, 0.0f, tmax, &h, lds, threadIdx.x, &tc);
+        int4 ha = found ? make_int4(h.inst, h.geomInst, h.prim, 0) : make_int4(-1, -1, -1, 0);
+        bool emitShadow, emitNext;
+        float4 shOrg, shDir, shVal, nextOrg, nextDir;
+        shadeStep<N>(sc, depth + 1, a.fp.maxPathLength, path, org, dir, ha, make_float2(h.b1, h.b2), P, cc,
+                     &emitShadow, &shOrg, &shDir, &shVal, &emitNext, &nextOrg, &nextDir);
+        if (emitShadow) {
+            atomicAdd(&depthCounts[2 * depth + 1], 1u);
+            TraceHit sh;
+            bool occluded = traceRay<true, STATS>(sc, xyz(shOrg), xyz(shDir), 0.0f, shDir.w, &sh, lds,
+                                                  threadIdx.x, &tcShadow);
+            if (!occluded) P.contrib += loadSpec<N>(shVal);
+        }
+        if (!emitNext) {
+            storePath<N>(a.fb, path, depth + 1, P, false);
+            active = false;
+            continue;
+        }
+        ++depth;
+        atomicAdd(&depthCounts[2 * depth], 1u);
+        org = xyz(nextOrg);
+        dir = xyz(nextDir);
+        tmax = nextDir.w;
     }
     if constexpr (STATS) {
         addTravStats(a.fb.counters, kStatTravClosest, tc);
         addTravStats(a.fb.counters, kStatTravShadow, tcShadow);
     }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < 2 * kMaxDepthCounters; k += blockDim.x)
+        if (depthCounts[k]) atomicAdd(&a.fb.counters[k], depthCounts[k]);
+}
+
+// ---- radiance cache: training samples from the finished training paths ----
+// target of slot k (vertex v of training path t) = (C_final - C_v) / T_v per channel: the
+// radiance the path gathered after the next-event estimate at v (the indirect light leaving
+// v, Russian roulette included), divided by the throughput on arrival at v.
+template <int N>
+__device__ __forceinline__ bool cacheTarget(const KArgs &a, uint32_t k, float4 *target) {
+    const FrameParams &fp = a.fp;
+    const CacheBuffers &cb = a.fb.cache;
+    if (cb.trainStamp[k] != fp.frameStamp) return false;
+    const uint32_t pix = (k / kCacheTrainVerts) * fp.trainStride + fp.trainOffset;
+    if (pix >= fp.imageW * fp.imageH) return false;
+    const uint32_t path = (pix / fp.imageW) * fp.width + pix % fp.imageW;
+    const float4 cf = a.fb.ps.contrib[path], T = cb.trainThroughput[k];
+    // a path that ended at this vertex gathered nothing after its next-event estimate
+    const float4 c = cb.trainAfterStamp[k] == fp.frameStamp ? cb.trainContrib[k] : cf;
+    float t[4] = {(cf.x - c.x) / T.x, (cf.y - c.y) / T.y, (cf.z - c.z) / T.z, N == 4 ? (cf.w - c.w) / T.w : 0.0f};
+    const float tt[4] = {T.x, T.y, T.z, T.w};
+    for (int ch = 0; ch < N; ++ch)
+        if (!(tt[ch] > 0.0f) || !isfinite(t[ch])) return false;
+    *target = make_float4(t[0], t[1], t[2], t[3]);
+    return true;
+}
+
+// Compaction of the live slots in slot order (deterministic): flags and targets per slot, an
+// exclusive scan in blocks of 1024 slots plus one scan of the block totals, then a gather.
+template <int N>
+__global__ void __launch_bounds__(256) k_rcSlotFlags(KArgs a, uint32_t numSlots, uint32_t *flags,
+                                                      float4 *targets) {
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= numSlots) return;
+    float4 tg = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    flags[k] = cacheTarget<N>(a, k, &tg) ? 1u : 0u;
+    targets[k] = tg;
+}
+
+constexpr uint32_t kSlotScanBlock = 1024;
+__global__ void __launch_bounds__(kSlotScanBlock) k_rcScanSlots(const uint32_t *flags, uint32_t numSlots,
+                                                                uint32_t *offsets, uint32_t *blockSums) {
+    __shared__ uint32_t waveTotals[kSlotScanBlock / 64];
+    const uint32_t k = blockIdx.x * kSlotScanBlock + threadIdx.x;
+    const uint32_t lane = __lane_id(), wave = threadIdx.x / 64;
+    const uint32_t v = k < numSlots ? flags[k] : 0u;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off);
+        if (lane >= static_cast<uint32_t>(off)) x += y;
+    }
+    if (lane == 63) waveTotals[wave] = x;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        uint32_t t = threadIdx.x < kSlotScanBlock / 64 ? waveTotals[threadIdx.x] : 0u, st = t;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            uint32_t y = __shfl_up(st, off);
+            if (lane >= static_cast<uint32_t>(off)) st += y;
+        }
+        if (threadIdx.x < kSlotScanBlock / 64) waveTotals[threadIdx.x] = st - t;
+        if (threadIdx.x == kSlotScanBlock / 64 - 1) blockSums[blockIdx.x] = st;
+    }
+    __syncthreads();
+    if (k < numSlots) offsets[k] = waveTotals[wave] + x - v;
+}
+
+// one wave: exclusive scan of at most 64 block totals, and the total
+__global__ void __launch_bounds__(64) k_rcScanSlotBlocks(uint32_t *blockSums, uint32_t numBlocks, uint32_t *total) {
+    const uint32_t lane = threadIdx.x;
+    const uint32_t v = lane < numBlocks ? blockSums[lane] : 0u;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off);
+        if (lane >= static_cast<uint32_t>(off)) x += y;
+    }
+    if (lane < numBlocks) blockSums[lane] = x - v;
+    if (lane == 63) *total = x;
+}
+
+__global__ void __launch_bounds__(256) k_rcGatherSlots(const float4 *trainRaw, const uint32_t *flags,
+                                                        const uint32_t *offsets, const uint32_t *blockOffsets,
+                                                        const float4 *targets, uint32_t numSlots, float4 *batchRaw,
+                                                        float4 *batchTarget) {
+    const uint32_t k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= numSlots || !flags[k]) return;
+    const uint32_t out = blockOffsets[k / kSlotScanBlock] + offsets[k];
+    const float4 *src = trainRaw + 4 * k;
+    float4 *dst = batchRaw + 4 * out;
+    dst[0] = src[0];
+    dst[1] = src[1];
+    dst[2] = src[2];
+    dst[3] = src[3];
+    batchTarget[out] = targets[k];
 }
 
 // ---- accumulate: Kahan sum per channel (common_internal.h:350-368, spectrum_types.h:674-684) ----
@@ -1970,7 +2187,7 @@ static inline uint32_t blocksFor(uint32_t n, uint32_t bs) { return (n + bs - 1) 
 
 template <int N>
 static hipError_t launchFrameT(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb, hipStream_t s,
-                               LaunchTimer *timer) {
+                               LaunchTimer *timer, const CacheLaunch *cache) {
     KArgs a;
     a.sc = sc;
     a.fp = fp;
@@ -2013,6 +2230,14 @@ static hipError_t launchFrameT(const DevScene &sc, const FrameParams &fp, const 
             hipLaunchKernelGGL((k_tail<N, false>), dim3(blocks), dim3(kTraceBlock), 0, s, a);
         t1(STAGE_TAIL);
     }
+    if (cache && fp.cacheMode == 2) {
+        t0(STAGE_CACHE_QUERY);
+        hipError_t e = rcLaunchQuery(cache->net, reinterpret_cast<const float *>(fb.cache.queryRaw),
+                                     fb.cache.queryThroughput, fb.cache.queryStamp, fp.frameStamp,
+                                     fp.width * fp.height, fb.ps.contrib, N, s);
+        if (e != hipSuccess) return e;
+        t1(STAGE_CACHE_QUERY);
+    }
     t0(STAGE_ACCUMULATE);
     hipLaunchKernelGGL(k_accumulate<N>, dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
     t1(STAGE_ACCUMULATE);
@@ -2020,12 +2245,31 @@ static hipError_t launchFrameT(const DevScene &sc, const FrameParams &fp, const 
     hipLaunchKernelGGL(k_convert<N>, dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
     t1(STAGE_CONVERT);
     hipLaunchKernelGGL(k_frameEnd, dim3(1), dim3(64), 0, s, fb.counters, fp.maxPathLength, wavefrontDepths);
+    if (cache && fp.cacheMode != 0) {
+        t0(STAGE_CACHE_TRAIN);
+        const uint32_t ns = cache->numSlots, scanBlocks = blocksFor(ns, kSlotScanBlock);
+        if (scanBlocks > 64) return hipErrorInvalidValue;
+        hipLaunchKernelGGL(k_rcSlotFlags<N>, dim3(blocksFor(ns, 256)), dim3(256), 0, s, a, ns, cache->slotFlags,
+                           cache->slotTargets);
+        hipLaunchKernelGGL(k_rcScanSlots, dim3(scanBlocks), dim3(kSlotScanBlock), 0, s, cache->slotFlags, ns,
+                           cache->slotOffsets, cache->slotBlockSums);
+        hipLaunchKernelGGL(k_rcScanSlotBlocks, dim3(1), dim3(64), 0, s, cache->slotBlockSums, scanBlocks,
+                           cache->batchCount);
+        hipLaunchKernelGGL(k_rcGatherSlots, dim3(blocksFor(ns, 256)), dim3(256), 0, s, fb.cache.trainRaw,
+                           cache->slotFlags, cache->slotOffsets, cache->slotBlockSums, cache->slotTargets, ns,
+                           cache->batchRaw, cache->batchTarget);
+        hipError_t e = rcLaunchTrain(cache->net, cache->train, reinterpret_cast<const float *>(cache->batchRaw),
+                                     reinterpret_cast<const float *>(cache->batchTarget), cache->batchCount,
+                                     cache->numSlots, s);
+        if (e != hipSuccess) return e;
+        t1(STAGE_CACHE_TRAIN);
+    }
     return hipGetLastError();
 }
 
 hipError_t launchFrame(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb, bool spectral,
-                       hipStream_t s, LaunchTimer *timer) {
-    return spectral ? launchFrameT<4>(sc, fp, fb, s, timer) : launchFrameT<3>(sc, fp, fb, s, timer);
+                       hipStream_t s, LaunchTimer *timer, const CacheLaunch *cache) {
+    return spectral ? launchFrameT<4>(sc, fp, fb, s, timer, cache) : launchFrameT<3>(sc, fp, fb, s, timer, cache);
 }
 
 template <int N>

@@ -4,6 +4,7 @@
 #include <vector>
 #include <hip/hip_runtime.h>
 #include "dev_types.h"
+#include "radiance_cache.h"
 
 namespace vlra {
 
@@ -20,6 +21,28 @@ struct FrameParams {
     uint32_t traversalStats;       // count nodes / triangles / instances per traversal
     uint32_t wavefrontDepth;       // depths run as separate kernels; the rest in the tail kernel
     uint32_t tailBlocks;           // persistent grid of the tail kernel (blocks of 128)
+    // radiance cache
+    uint32_t cacheMode;            // 0 off; 1 record training vertices only; 2 also end paths in the cache
+    uint32_t frameStamp;           // grows every frame: marks the cache records written this frame
+    float cacheThreshold;          // c of the spread test a(x_1..x_n) > c a_0
+    uint32_t trainStride, trainOffset;   // pixel i traces a training path when i % stride == offset
+};
+
+constexpr uint32_t kCacheTrainVerts = 3;   // vertices 1..3 of a training path become samples
+
+// Radiance-cache records written by the path tracer (indexed by path, or by training slot =
+// training path * kCacheTrainVerts + vertex - 1); a record is live when its stamp equals the
+// frame's stamp.
+struct CacheBuffers {
+    float4* queryRaw;              // 4 float4 per path: the raw input record
+    float4* queryThroughput;       // path throughput at the query vertex
+    uint32_t* queryStamp;
+    float2* spread;                // (a_0, a) per path
+    float4* trainRaw;              // 4 float4 per slot
+    float4* trainThroughput;       // throughput on arrival at the vertex
+    float4* trainContrib;          // path contribution after the vertex's next-event estimate
+    uint32_t* trainStamp;          // the vertex was reached this frame
+    uint32_t* trainAfterStamp;     // the path went on to a next vertex (trainContrib is set)
 };
 
 constexpr uint32_t kDefaultTailBlocks = 2048;   // 8 blocks of 128 per CU
@@ -79,6 +102,7 @@ struct FrameBuffers {
     uint32_t* counters;            // kNumCounters
     float* accum;                  // W*H*channels*2 (sum, comp)
     float4* output;                // W*H
+    CacheBuffers cache;
 };
 
 // timing hook: called before/after each launch (may be null)
@@ -89,7 +113,7 @@ struct LaunchTimer {
 };
 enum Stage {
     STAGE_GENERATE = 0, STAGE_INTERSECT, STAGE_SHADE, STAGE_SHADOW, STAGE_ACCUMULATE, STAGE_CONVERT, STAGE_TAIL,
-    STAGE_LIGHT_PATHS, STAGE_EYE_PATHS, NUM_STAGES
+    STAGE_LIGHT_PATHS, STAGE_EYE_PATHS, STAGE_CACHE_QUERY, STAGE_CACHE_TRAIN, NUM_STAGES
 };
 
 // LVC-BPT frame state (lvc_bpt.cu:144-971).  Light paths write their vertices to fixed slots
@@ -113,8 +137,23 @@ struct BptParams {
     float* splat;                // lens-connection splats: channel planes of width * height
 };
 
+// Radiance-cache work of a frame: the network, this step's optimiser settings and the batch
+// buffers the training samples are compacted into.
+struct CacheLaunch {
+    RcDevice net;
+    RcTrainParams train;
+    uint32_t numSlots;             // training slots this frame (training paths * kCacheTrainVerts)
+    float4* batchRaw;              // kMaxBatch records of 4 float4
+    float4* batchTarget;
+    uint32_t* batchCount;
+    uint32_t* slotFlags;           // per slot: a live sample
+    uint32_t* slotOffsets;         // exclusive scan inside blocks of 1024 slots
+    uint32_t* slotBlockSums;       // per block total, then its exclusive scan (<= 64 blocks)
+    float4* slotTargets;
+};
+
 hipError_t launchFrame(const DevScene& scene, const FrameParams& fp, const FrameBuffers& fb, bool spectral,
-                       hipStream_t stream, LaunchTimer* timer);
+                       hipStream_t stream, LaunchTimer* timer, const CacheLaunch* cache = nullptr);
 
 // One LVC-BPT frame: light paths, cache compaction, eye paths, accumulation, conversion.
 hipError_t launchBptFrame(const DevScene& scene, const FrameParams& fp, const FrameBuffers& fb, const BptParams& bp,

@@ -584,6 +584,69 @@ DevMaterial SurfaceMaterial::toDevice(bool spectral, bool *usesEnvTexture, const
     return d;
 }
 
+// Radiance-cache material inputs.  These only steer the network (no parity with a reference
+// quantity): the RGB triple of the colour parameter as given, split into a diffuse and a
+// specular part by material kind, and the roughness.
+void SurfaceMaterial::cacheFeatures(float out[8]) const {
+    auto rgbOf = [&](size_t slot, float rgb[3]) {
+        rgb[0] = rgb[1] = rgb[2] = 0.0f;
+        if (slot >= specs.size()) return;
+        ImmSpectrum s = specs[slot];
+        auto it = plugs.find(specNames[slot]);
+        if (it != plugs.end() && it->second.valid()) {
+            SpectrumType t;
+            it->second.node->constantSpectrum(&s, &t);
+        }
+        for (int k = 0; k < 3; ++k) rgb[k] = std::fmin(std::fmax(s.e[k], 0.0f), 1.0f);
+    };
+    auto floatOf = [&](const char *name, float fallback) {
+        for (size_t i = 0; i < floatNames.size(); ++i)
+            if (paramNameEquals(floatNames[i], name)) return floatVals[i];
+        return fallback;
+    };
+    float d[3] = {0, 0, 0}, sp[3] = {0, 0, 0}, rough = 1.0f;
+    switch (kind) {
+    case MatKind::Matte:
+        rgbOf(0, d);
+        break;
+    case MatKind::SpecularReflection:
+    case MatKind::SpecularScattering:
+        rgbOf(0, sp);
+        rough = 0.0f;
+        break;
+    case MatKind::MicrofacetReflection:
+        sp[0] = sp[1] = sp[2] = 1.0f;
+        rough = floatOf("roughness", 0.5f);
+        break;
+    case MatKind::MicrofacetScattering:
+        rgbOf(0, sp);
+        rough = floatOf("roughness", 0.5f);
+        break;
+    case MatKind::UE4: {
+        float base[3];
+        rgbOf(0, base);
+        const float metallic = floatOf("metallic", 0.0f);
+        rough = floatOf("roughness", 0.5f);
+        for (int k = 0; k < 3; ++k) {
+            d[k] = base[k] * (1.0f - metallic);
+            sp[k] = 0.04f + (base[k] - 0.04f) * metallic;
+        }
+        break;
+    }
+    default:
+        rough = 0.0f;
+        break;
+    }
+    out[0] = d[0];
+    out[1] = d[1];
+    out[2] = d[2];
+    out[3] = rough;
+    out[4] = sp[0];
+    out[5] = sp[1];
+    out[6] = sp[2];
+    out[7] = 0.0f;
+}
+
 // ---------------------------------------------------------------------------------------------
 // transforms: the inverse of an affine matrix is computed in double from the float matrix by
 // cofactors and rounded once (affineInverse), the definition shared with the oracle so that

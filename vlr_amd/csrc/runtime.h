@@ -247,6 +247,8 @@ public:
     int setPlug(const char *, const PlugRef &) override;
     // device descriptor (immediate values with constant node plugs folded in)
     DevMaterial toDevice(bool spectral, bool *usesEnvTexture, const Image2D **envImage) const;
+    // radiance-cache inputs: diffuse RGB, roughness, specular RGB, 0
+    void cacheFeatures(float out[8]) const;
 };
 
 // ---- transforms ----
@@ -347,7 +349,7 @@ struct DeviceBuffer {
 
 struct SceneGPU {
     DeviceBuffer tlasNodes, blasNodes, blasTris, instances, instGeomInsts, geomInsts, vertices, indices, distPool,
-        materials, envTex, mengGrid, mengPoints;
+        materials, envTex, mengGrid, mengPoints, cacheMaterials;
     DevScene dev = {};
     bool spectral = false;
     bool valid = false;
@@ -397,6 +399,22 @@ public:
     std::mt19937 bptWlRng;
     DeviceBuffer lightRng, lvSlots, lvCounts, lvOffsets, lvBlockSums, lvCache, lvNum, splat;
     size_t splatLayout = 0;          // bytes of the splat planes last cleared
+    // radiance cache (no counterpart in the reference)
+    struct CacheSettings {
+        float learningRate = 1e-2f;
+        float beta1 = 0.9f, beta2 = 0.99f, epsilon = 1e-15f;
+        uint32_t trainingPaths = 21845;   // x kCacheTrainVerts vertices <= rc::kMaxBatch
+        float threshold = 0.01f;
+        uint32_t warmupSteps = 16;
+        uint32_t seed = 1;
+    } cacheSettings;
+    bool cacheEnabled = false;
+    bool cacheInitialized = false;
+    uint32_t cacheStep = 0;          // Adam steps taken since the last reset
+    uint32_t frameStamp = 0;
+    DeviceBuffer rcW, rcWT, rcBias, rcMaster, rcM, rcV, rcGrad, rcPartial, rcAct, rcDelta, rcLossPartial, rcLoss;
+    DeviceBuffer cqRaw, cqThroughput, cqStamp, cSpread, ctRaw, ctThroughput, ctContrib, ctStamp, ctAfterStamp, cbRaw,
+        cbTarget, cbCount, csFlags, csOffsets, csBlockSums, csTargets;
     SceneGPU gpu;
     double stageMs[NUM_STAGES] = {};
     uint64_t launches[NUM_STAGES] = {};
@@ -430,6 +448,15 @@ public:
     void renderBptFrame(hipStream_t stream, const FrameParams &fp, bool firstFrame);
     void readLightRngStates(uint64_t *dst);
     void writeLightRngStates(const uint64_t *src);
+    // radiance cache
+    void ensureCacheNet();
+    void resetCache();
+    void writeCacheWeights(const float *master);   // also clears the optimiser state
+    void readCacheWeights(float *master);
+    RcDevice rcDevice();
+    void ensureCacheFrameBuffers();
+    void cacheInfer(hipStream_t stream, const float *records, uint32_t n, float *out);
+    void cacheTrain(hipStream_t stream, const float *records, const float *targets, uint32_t n, uint32_t numOut);
     FrameBuffers frameBuffers();
 };
 

@@ -385,6 +385,7 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
 
     // materials
     std::vector<DevMaterial> mats;
+    std::vector<float> cacheMats;   // 8 floats per material
     std::map<const SurfaceMaterial *, uint32_t> matIndex;
     const Image2D *envImage = nullptr;
     auto matOf = [&](const SurfaceMaterial *m) -> uint32_t {
@@ -399,6 +400,9 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
         if (usesTex) envImage = img;
         uint32_t idx = static_cast<uint32_t>(mats.size());
         mats.push_back(d);
+        float cf[8];
+        m->cacheFeatures(cf);
+        cacheMats.insert(cacheMats.end(), cf, cf + 8);
         matIndex[m] = idx;
         return idx;
     };
@@ -689,6 +693,7 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
     g->indices.upload(indices.data(), indices.size() * sizeof(uint32_t));
     g->distPool.upload(pool.data(), pool.size() * sizeof(float));
     g->materials.upload(mats.data(), mats.size() * sizeof(DevMaterial));
+    g->cacheMaterials.upload(cacheMats.data(), cacheMats.size() * sizeof(float));
     g->envTex.upload(envTex.data(), envTex.size() * sizeof(float4));
     g->mengGrid.upload(mengGrid().data(), mengGrid().size() * sizeof(MengCellDev));
     g->mengPoints.upload(mengPoints().data(), mengPoints().size() * sizeof(float));
@@ -703,6 +708,7 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
     D.indices = static_cast<const uint32_t *>(g->indices.ptr);
     D.distPool = static_cast<const float *>(g->distPool.ptr);
     D.materials = static_cast<const DevMaterial *>(g->materials.ptr);
+    D.cacheMaterials = static_cast<const float4 *>(g->cacheMaterials.ptr);
     D.envTex = envTex.empty() ? nullptr : static_cast<const float4 *>(g->envTex.ptr);
     D.mengGrid = static_cast<const MengCellDev *>(g->mengGrid.ptr);
     D.mengPoints = static_cast<const float *>(g->mengPoints.ptr);
