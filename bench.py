@@ -256,6 +256,104 @@ def cache_experiment(width, height, ref_spp, spp, warmup_frames, seed=20261202):
     }
 
 
+def point_light_pixels(desc, width, height):
+    """Pixels a point light projects to: the lens connection of LVC-BPT sees the light itself
+    there, which path tracing cannot hit, so those pixels are left out of the comparison."""
+    cam = desc.camera
+    pts = desc.meshes[-2].vertices[:, :3].astype(np.float64)
+    th = np.tan(cam.fovy / 2)
+    rel = pts - np.array(cam.position)
+    lx, ly, lz = -rel[:, 0], rel[:, 1], -rel[:, 2]     # camera looks down -z (qRotateY(pi))
+    u = 0.5 - (lx / lz) / (2 * th * cam.aspect)
+    v = 0.5 - (ly / lz) / (2 * th)
+    mask = np.ones((height, width), bool)
+    for x, y in zip((u * width).astype(int), (v * height).astype(int)):
+        mask[max(0, y - 1):y + 2, max(0, x - 1):x + 2] = False
+    return mask
+
+
+def bpt_experiment(width, height, spp, ref_spp, light_paths, seed=20261202):
+    """LVC-BPT vs path tracing on the Cornell box with 64 point lights and 8 quad lights (RGB,
+    max depth 8): throughput of both and RMSE over time against an FP64-accumulated path-traced
+    reference (two independent halves; their difference estimates the reference noise, which is
+    removed).  Pixels that see a point light directly are masked."""
+    desc = scenes.cornell_many_lights(seed)
+    n = width * height
+    mask = point_light_pixels(desc, width, height)
+
+    def context(renderer, rng_seed=None):
+        ctx = api.Context(spectral=False, max_path_length=8)
+        built = api.build_scene(ctx, desc)
+        ctx.bind_output(width, height)
+        ctx.set_renderer(renderer)
+        if renderer == api.RENDERER_BPT:
+            ctx.set_num_light_paths(light_paths)
+        if rng_seed is not None:
+            ctx.write_rng(np.random.default_rng(rng_seed).integers(1, 2 ** 63, size=n, dtype=np.uint64))
+        return ctx, built
+
+    halves = []
+    for rs in (303, 404):
+        ctx, built = context(api.RENDERER_PATH_TRACING, rs)
+        acc = np.zeros((height, width, 3))
+        done = 0
+        while done < ref_spp // 2:
+            m = min(1024, ref_spp // 2 - done)
+            for k in range(m):
+                ctx.render(built.camera, first_frame=(k == 0), limit=m)
+            acc += ctx.read_output()[..., :3].astype(np.float64) * m
+            done += m
+        halves.append(acc / done)
+        ctx.close()
+    ref = 0.5 * (halves[0] + halves[1])
+    ref_var = float(((halves[0] - halves[1])[mask] ** 2).mean()) / 4.0
+    checkpoints = sorted({c for c in (4, 8, 16, 32, 64, 128, 256) if c <= spp} | {spp})
+
+    def curve(renderer):
+        ctx, built = context(renderer)
+        for k in range(4):
+            ctx.render(built.camera, first_frame=(k == 0), limit=1 << 30)
+        ctx.reset_stats()
+        ctx.set_kernel_timing(True)
+        out, elapsed = [], 0.0
+        hip_sync()
+        t0 = time.perf_counter()
+        for k in range(spp):
+            ctx.render(built.camera, first_frame=(k == 0), limit=1 << 30)
+            if (k + 1) in checkpoints:
+                hip_sync()
+                elapsed += time.perf_counter() - t0
+                img = ctx.read_output()[..., :3].astype(np.float64)
+                mse = float(((img - ref)[mask] ** 2).mean()) - ref_var
+                out.append({"spp": k + 1, "seconds": elapsed, "rmse": float(np.sqrt(max(mse, 0.0)))})
+                hip_sync()
+                t0 = time.perf_counter()
+        st = ctx.stats()
+        ctx.close()
+        rays = st["closestRays"] + st["shadowRays"]
+        frames = max(1, st["numFrames"])
+        return out, {"rays_per_s": rays / elapsed, "samples_per_s": st["paths"] / elapsed,
+                     "ms_per_frame": elapsed * 1e3 / spp, "closest_rays": st["closestRays"],
+                     "shadow_rays": st["shadowRays"],
+                     "ms_light_paths_per_frame": st["msLightPaths"] / frames,
+                     "ms_eye_paths_per_frame": st["msEyePaths"] / frames}
+
+    bpt, bst = curve(api.RENDERER_BPT)
+    pt, pst = curve(api.RENDERER_PATH_TRACING)
+    equal_time = []
+    for p in bpt:
+        r = interp_log(pt, "seconds", "rmse", p["seconds"])
+        if r is not None:
+            equal_time.append({"seconds": p["seconds"], "bpt": p["rmse"], "pt": r})
+    return {
+        "workload": f"Cornell box + 64 point lights + 8 quad lights, RGB, max depth 8, {width}x{height}, "
+                    f"{light_paths} light paths per frame",
+        "reference": {"renderer": "path tracing", "spp": ref_spp, "noise_rmse_removed": float(np.sqrt(ref_var)),
+                      "masked_pixels": int((~mask).sum())},
+        "bpt": bpt, "bpt_stats": bst, "pt": pt, "pt_stats": pst, "equal_time": equal_time,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -278,6 +376,10 @@ def main():
     ap.add_argument("--cache-ref-spp", type=int, default=2048)
     ap.add_argument("--cache-spp", type=int, default=256)
     ap.add_argument("--cache-warmup", type=int, default=64)
+    ap.add_argument("--bpt-experiment", action="store_true",
+                    help="only run the LVC-BPT vs path tracing experiment (1280x720) and print its JSON line")
+    ap.add_argument("--bpt-spp", type=int, default=32)
+    ap.add_argument("--bpt-ref-spp", type=int, default=2048)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by pixel "
@@ -285,6 +387,10 @@ def main():
         sys.exit(2)
     if not os.path.exists(api.LIB_PATH):
         raise RuntimeError(f"{api.LIB_PATH} is missing: build it with vlr_amd/build.py")
+    if args.bpt_experiment:
+        res = bpt_experiment(1280, 720, args.bpt_spp, args.bpt_ref_spp, 1280 * 720)
+        print(json.dumps({"lvc_bpt": res}), flush=True)
+        return
     if args.cache_experiment:
         res = cache_experiment(args.width, args.height, args.cache_ref_spp, args.cache_spp, args.cache_warmup)
         print(json.dumps({"radiance_cache": res}), flush=True)
