@@ -354,6 +354,78 @@ def bpt_experiment(width, height, spp, ref_spp, light_paths, seed=20261202):
     }
 
 
+def batch_experiment(num_jobs, seed=20261202):
+    """C4: render jobs drawn from one seed (tessellated Cornell boxes of 10k-1M unique
+    triangles, log-uniform; square images of 128, 256, 512 or 1024 pixels; 1-64 samples;
+    max path length 4-16), once as one batch (graph-captured steps, autotuned schedule) and
+    once serially, one job after another, each in its own context prepared beforehand.
+    Reports aggregate rays/s and the p50 / p99 job completion latency of both."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for k in range(num_jobs):
+        tris = int(round(np.exp(rng.uniform(np.log(1e4), np.log(1e6)))))
+        res = int(rng.choice([128, 256, 512, 1024]))
+        specs.append((tris, res, int(rng.integers(1, 65)), int(rng.integers(4, 17)), seed + 1000 + k))
+    descs = [scenes.tessellated_cornell(sd, tris) for (tris, _, _, _, sd) in specs]
+
+    # batch
+    bctx = api.Context(spectral=False)
+    built = [api.build_scene(bctx, d) for d in descs]
+    jobs = [(built[k].scene, built[k].camera, res, res, spp, depth)
+            for k, (_, res, spp, depth, _) in enumerate(specs)]
+    t0 = time.perf_counter()
+    bctx.render_batch([(b, c, 16, 16, 1, d) for (b, c, _, _, _, d) in jobs])   # scene set-up, first-use allocations
+    setup_s = time.perf_counter() - t0
+    bctx.reset_stats()
+    hip_sync()
+    t0 = time.perf_counter()
+    done_ms = bctx.render_batch(jobs)
+    hip_sync()
+    batch_s = time.perf_counter() - t0
+    bst = bctx.stats()
+    sched = bctx.batch_schedule()
+    bctx.close()
+
+    # serial: each job alone, in order, in its own prepared context
+    ctxs = []
+    for k, d in enumerate(descs):
+        c = api.Context(spectral=False, max_path_length=specs[k][3])
+        bs = api.build_scene(c, d)
+        c.bind_output(specs[k][1], specs[k][1])
+        c.set_wavefront_depth(3)
+        c.render(bs.camera, True, 1)          # scene set-up outside the timed region
+        ctxs.append((c, bs))
+    hip_sync()
+    serial_done, rays = [], 0
+    t0 = time.perf_counter()
+    for k, (c, bs) in enumerate(ctxs):
+        c.reset_stats()
+        for f in range(specs[k][2]):
+            c.render(bs.camera, f == 0, specs[k][2])
+        hip_sync()
+        serial_done.append((time.perf_counter() - t0) * 1e3)
+    serial_s = time.perf_counter() - t0
+    for c, _ in ctxs:
+        st = c.stats()
+        rays += st["closestRays"] + st["shadowRays"]
+        c.close()
+    brays = bst["closestRays"] + bst["shadowRays"]
+    q = lambda v, p: float(np.percentile(np.asarray(v, np.float64), p))
+    return {
+        "workload": f"{num_jobs} jobs: tessellated Cornell boxes of 10k-1M unique triangles (log-uniform), "
+                    "128^2-1024^2, 1-64 spp, max depth 4-16, RGB",
+        "jobs": [{"triangles": descs[k].num_triangles, "res": r, "spp": p, "depth": d}
+                 for k, (_, r, p, d, _) in enumerate(specs)],
+        "batch": {"seconds": batch_s, "rays": brays, "rays_per_s": brays / batch_s,
+                  "latency_ms_p50": q(done_ms, 50), "latency_ms_p99": q(done_ms, 99),
+                  "schedule": {"wavefront_depth": sched[0], "tail_blocks": sched[1]},
+                  "setup_seconds": setup_s},
+        "serial": {"seconds": serial_s, "rays": rays, "rays_per_s": rays / serial_s,
+                   "latency_ms_p50": q(serial_done, 50), "latency_ms_p99": q(serial_done, 99)},
+        "rays_equal": brays == rays,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -376,6 +448,9 @@ def main():
     ap.add_argument("--cache-ref-spp", type=int, default=2048)
     ap.add_argument("--cache-spp", type=int, default=256)
     ap.add_argument("--cache-warmup", type=int, default=64)
+    ap.add_argument("--batch-experiment", action="store_true",
+                    help="only run the batched-vs-serial render-job experiment and print its JSON line")
+    ap.add_argument("--batch-jobs", type=int, default=64)
     ap.add_argument("--bpt-experiment", action="store_true",
                     help="only run the LVC-BPT vs path tracing experiment (1280x720) and print its JSON line")
     ap.add_argument("--bpt-spp", type=int, default=32)
@@ -387,6 +462,10 @@ def main():
         sys.exit(2)
     if not os.path.exists(api.LIB_PATH):
         raise RuntimeError(f"{api.LIB_PATH} is missing: build it with vlr_amd/build.py")
+    if args.batch_experiment:
+        res = batch_experiment(args.batch_jobs)
+        print(json.dumps({"batch": res}), flush=True)
+        return
     if args.bpt_experiment:
         res = bpt_experiment(1280, 720, args.bpt_spp, args.bpt_ref_spp, 1280 * 720)
         print(json.dumps({"lvc_bpt": res}), flush=True)

@@ -82,6 +82,32 @@ VLR_API VLRResult vlrContextGetQueueCounts(VLRContextConst context, uint32_t* cl
 VLR_API VLRResult vlrContextSetNumLightPaths(VLRContext context, uint32_t numLightPaths);
 VLR_API VLRResult vlrContextGetNumLightPaths(VLRContextConst context, uint32_t* numLightPaths);
 
+/* ---- batches of render jobs ----
+ * Renders many jobs (scene, camera, image size, samples, max path length) together: each step
+ * renders one sample per pixel of every job that has samples left, over one set of path
+ * buffers, with the step's launches captured once into a HIP graph and replayed.  Each job's
+ * pixels are seeded like a single render, so its image is bit-identical to rendering it alone
+ * with vlrContextRender (samples from firstFrame, VLRRenderer_PathTracing, no radiance cache).
+ * The spectral/RGB mode is the context's.  With autotuning on (the default), the first steps
+ * of a batch of 8 or more samples time candidate wavefront depths and keep the fastest.  The
+ * call returns once the batch has finished; completionMs (host array of numJobs, may be
+ * NULL) receives each job's completion time from the start of the batch. */
+typedef struct VLRBatchJob {
+    VLRSceneConst scene;
+    VLRCameraConst camera;
+    uint32_t width, height;
+    uint32_t numSamples;         /* >= 1 */
+    uint32_t maxPathLength;      /* 1 .. 63 */
+} VLRBatchJob;
+VLR_API VLRResult vlrContextRenderBatch(VLRContext context, hipStream_t stream, const VLRBatchJob* jobs,
+                                        uint32_t numJobs, float* completionMs);
+/* RGBA32F image of job `job` of the last batch (width * height float4, row-major). */
+VLR_API VLRResult vlrContextReadBatchOutput(VLRContextConst context, uint32_t job, float* rgba);
+/* Replay captured graphs (default) or launch every kernel of every step directly. */
+VLR_API VLRResult vlrContextSetBatchGraphs(VLRContext context, bool enable);
+/* The wavefront depth and tail grid the last batch ran with. */
+VLR_API VLRResult vlrContextGetBatchSchedule(VLRContextConst context, uint32_t* wavefrontDepth, uint32_t* tailBlocks);
+
 /* ---- progressive state (host pointers, width*height entries, row-major) ---- */
 VLR_API VLRResult vlrContextReadRNGStates(VLRContextConst context, uint64_t* states);
 VLR_API VLRResult vlrContextWriteRNGStates(VLRContext context, const uint64_t* states);

@@ -69,6 +69,11 @@ class VLRRenderStats(C.Structure):
                 ("msCacheQuery", C.c_double), ("msCacheTrain", C.c_double)]
 
 
+class VLRBatchJob(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("camera", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("numSamples", C.c_uint32), ("maxPathLength", C.c_uint32)]
+
+
 class VLRRadianceCacheSettings(C.Structure):
     _fields_ = [("learningRate", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("epsilon", C.c_float),
                 ("trainingPaths", C.c_uint32), ("terminationThreshold", C.c_float), ("warmupSteps", C.c_uint32),
@@ -203,6 +208,10 @@ PROTOTYPES: Dict[str, list] = {
     "vlrContextReadLightRNGStates": [H, P],
     "vlrContextWriteLightRNGStates": [H, P],
     "vlrContextSetNumLightPaths": [H, U32],
+    "vlrContextRenderBatch": [H, H, P, U32, P],
+    "vlrContextReadBatchOutput": [H, U32, P],
+    "vlrContextSetBatchGraphs": [H, B],
+    "vlrContextGetBatchSchedule": [H, C.POINTER(U32), C.POINTER(U32)],
     "vlrContextSetRadianceCache": [H, B],
     "vlrContextGetRadianceCache": [H, C.POINTER(B)],
     "vlrContextSetRadianceCacheSettings": [H, C.POINTER(VLRRadianceCacheSettings)],
@@ -350,6 +359,33 @@ class Context:
         s = np.ascontiguousarray(states, np.uint64)
         assert s.size == self.num_light_paths()
         call("vlrContextWriteLightRNGStates", self.h, s.ctypes.data)
+
+    # ---- batches ----
+    def render_batch(self, jobs, stream=None):
+        """jobs: list of (scene handle, camera handle, width, height, samples, max path length).
+        Returns the per-job completion times in ms."""
+        arr = (VLRBatchJob * len(jobs))()
+        for k, (scene, cam, w, h, spp, depth) in enumerate(jobs):
+            arr[k] = VLRBatchJob(C.cast(scene, C.c_void_p).value if not isinstance(scene, int) else scene,
+                                 C.cast(cam, C.c_void_p).value if not isinstance(cam, int) else cam, w, h, spp, depth)
+        ms = np.zeros(len(jobs), np.float32)
+        call("vlrContextRenderBatch", self.h, stream, C.cast(arr, C.c_void_p), len(jobs), ms.ctypes.data)
+        self._batch_sizes = [(w, h) for (_, _, w, h, _, _) in jobs]
+        return ms
+
+    def read_batch_output(self, job: int) -> np.ndarray:
+        w, h = self._batch_sizes[job]
+        out = np.zeros((h, w, 4), np.float32)
+        call("vlrContextReadBatchOutput", self.h, job, out.ctypes.data)
+        return out
+
+    def set_batch_graphs(self, on: bool):
+        call("vlrContextSetBatchGraphs", self.h, on)
+
+    def batch_schedule(self):
+        wd, tb = U32(), U32()
+        call("vlrContextGetBatchSchedule", self.h, C.byref(wd), C.byref(tb))
+        return wd.value, tb.value
 
     # ---- radiance cache ----
     def set_radiance_cache(self, on: bool):

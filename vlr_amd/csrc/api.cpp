@@ -1190,6 +1190,51 @@ VLR_API VLRResult vlrContextGetNumLightPaths(VLRContextConst context, uint32_t *
     return VLRResult_NoError;
     VLRA_CATCH
 }
+// ---- batches ----
+VLR_API VLRResult vlrContextRenderBatch(VLRContext context, hipStream_t stream, const VLRBatchJob *jobs,
+                                        uint32_t numJobs, float *completionMs) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    if (numJobs && !jobs) return VLRResult_InvalidArgument;
+    std::vector<BatchJobDesc> descs(numJobs);
+    for (uint32_t k = 0; k < numJobs; ++k) {
+        CHECK(jobs[k].scene, IS(jobs[k].scene, Scene));
+        CHECK(jobs[k].camera, IS(jobs[k].camera, Camera));
+        descs[k].scene = static_cast<const Scene *>(obj(jobs[k].scene));
+        descs[k].camera = static_cast<const Camera *>(obj(jobs[k].camera));
+        descs[k].width = jobs[k].width;
+        descs[k].height = jobs[k].height;
+        descs[k].spp = jobs[k].numSamples;
+        descs[k].maxPathLength = jobs[k].maxPathLength;
+    }
+    CTX(context)->renderBatch(stream, descs, completionMs);
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+VLR_API VLRResult vlrContextReadBatchOutput(VLRContextConst context, uint32_t job, float *rgba) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    if (!rgba) return VLRResult_InvalidArgument;
+    CTX(context)->readBatchOutput(job, rgba);
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+VLR_API VLRResult vlrContextSetBatchGraphs(VLRContext context, bool enable) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    CTX(context)->batchGraphs = enable;
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+VLR_API VLRResult vlrContextGetBatchSchedule(VLRContextConst context, uint32_t *wavefrontDepth, uint32_t *tailBlocks) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    if (wavefrontDepth) *wavefrontDepth = CTX(context)->batch.wavefrontDepth;
+    if (tailBlocks) *tailBlocks = CTX(context)->batch.tailBlocks;
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+
 // ---- radiance cache ----
 VLR_API VLRResult vlrContextSetRadianceCache(VLRContext context, bool enable) {
     VLRA_TRY

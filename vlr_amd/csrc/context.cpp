@@ -493,6 +493,219 @@ void Context::cacheTrain(hipStream_t stream, const float *records, const float *
     VLRA_HIP(rcLaunchTrain(rcDevice(), p, records, targets, nullptr, n, stream));
 }
 
+// ---- batches ----
+BatchState::~BatchState() {
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+namespace {
+struct GraphExec {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~GraphExec() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+}  // namespace
+
+// Renders every job to its sample count.  Each step renders one sample per pixel of every job
+// that has samples left, over one set of path buffers; a step's launch sequence is captured
+// once into a graph and replayed.  Pixels are seeded per job like a single render
+// (mt19937_64(591842031321323413), row-major), so each job's image is bit-identical to
+// rendering it alone.
+void Context::renderBatch(hipStream_t userStream, const std::vector<BatchJobDesc> &jobs, float *completionMs) {
+    ensureDevice();
+    if (jobs.empty()) return;
+    for (const BatchJobDesc &j : jobs) {
+        if (!j.scene || !j.camera) throw ApiError{3, "batch job without scene or camera"};
+        if (!j.camera->perspective) throw ApiError{4, "only the perspective camera is on the render path"};
+        if (j.width == 0 || j.height == 0 || j.spp == 0) throw ApiError{3, "batch job with an empty image or no samples"};
+        if (j.maxPathLength == 0 || j.maxPathLength >= kMaxDepthCounters)
+            throw ApiError{3, "batch job max path length must be 1 to 63"};
+    }
+    VLRA_HIP(hipDeviceSynchronize());
+    BatchState &b = batch;
+    const bool spectral = spectralRequested;
+    if (b.sceneVersion != version || b.spectral != spectral) {
+        b.scenes.clear();
+        b.sceneVersion = version;
+        b.spectral = spectral;
+    }
+    for (const BatchJobDesc &j : jobs) {
+        auto &g = b.scenes[j.scene];
+        if (!g) {
+            g = std::make_unique<SceneGPU>();
+            buildSceneGPU(*this, *j.scene, spectral, g.get());
+            g->scene = j.scene;
+            g->sceneVersion = version;
+        }
+    }
+    // path buffers over all jobs' pixels
+    const uint32_t C = spectral ? 16 : 3;
+    std::vector<uint32_t> offsets(jobs.size());
+    size_t total = 0;
+    uint32_t maxLen = 0, maxSpp = 0;
+    for (size_t k = 0; k < jobs.size(); ++k) {
+        offsets[k] = static_cast<uint32_t>(total);
+        total += size_t(jobs[k].width) * jobs[k].height;
+        maxLen = std::max(maxLen, jobs[k].maxPathLength);
+        maxSpp = std::max(maxSpp, jobs[k].spp);
+    }
+    if (total >= (1ull << 31)) throw ApiError{3, "batch too large"};
+    const size_t n = total;
+    b.rng.reserve(n * sizeof(uint64_t));
+    b.alpha.reserve(n * sizeof(float4));
+    b.contrib.reserve(n * sizeof(float4));
+    b.meta.reserve(n * sizeof(float4));
+    for (int k = 0; k < 2; ++k) {
+        b.rayOrg[k].reserve(n * sizeof(float4));
+        b.rayDir[k].reserve(n * sizeof(float4));
+    }
+    b.hitA.reserve(n * sizeof(int4));
+    b.hitB.reserve(n * sizeof(float2));
+    b.shOrg.reserve(n * sizeof(float4));
+    b.shDir.reserve(n * sizeof(float4));
+    b.shVal.reserve(n * sizeof(float4));
+    b.accum.resize(jobs.size());
+    b.output.resize(jobs.size());
+    b.widths.resize(jobs.size());
+    b.heights.resize(jobs.size());
+    std::vector<uint64_t> seeds(n);
+    std::vector<DevJob> table(jobs.size());
+    for (size_t k = 0; k < jobs.size(); ++k) {
+        const BatchJobDesc &j = jobs[k];
+        const size_t px = size_t(j.width) * j.height;
+        if (!b.accum[k]) b.accum[k] = std::make_unique<DeviceBuffer>();
+        if (!b.output[k]) b.output[k] = std::make_unique<DeviceBuffer>();
+        b.accum[k]->reserve(px * C * 2 * sizeof(float));
+        b.output[k]->reserve(px * sizeof(float4));
+        b.widths[k] = j.width;
+        b.heights[k] = j.height;
+        std::mt19937_64 seedRng(591842031321323413ULL);
+        for (size_t i = 0; i < px; ++i) seeds[offsets[k] + i] = seedRng();
+        DevJob &d = table[k];
+        std::memset(&d, 0, sizeof(d));
+        d.scene = b.scenes[j.scene]->dev;
+        d.scene.camera = j.camera->toDevice();
+        d.accum = static_cast<float *>(b.accum[k]->ptr);
+        d.output = static_cast<float4 *>(b.output[k]->ptr);
+        d.pixelOffset = offsets[k];
+        d.width = j.width;
+        d.height = j.height;
+        d.spp = j.spp;
+        d.maxPathLength = j.maxPathLength;
+    }
+    VLRA_HIP(hipMemcpy(b.rng.ptr, seeds.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice));
+    b.jobs.upload(table.data(), table.size() * sizeof(DevJob));
+    b.jobOffsets.upload(offsets.data(), offsets.size() * sizeof(uint32_t));
+    if (!b.stream) VLRA_HIP(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
+
+    BatchLaunch L;
+    L.anyScene = table[0].scene;
+    std::memset(&L.fp, 0, sizeof(L.fp));
+    L.fp.width = L.fp.imageW = static_cast<uint32_t>(n);
+    L.fp.height = L.fp.imageH = 1;
+    L.fp.maxPathLength = maxLen;
+    L.fp.trainStride = 1;
+    L.fb = FrameBuffers{};
+    L.fb.ps.rng = static_cast<uint64_t *>(b.rng.ptr);
+    L.fb.ps.alpha = static_cast<float4 *>(b.alpha.ptr);
+    L.fb.ps.contrib = static_cast<float4 *>(b.contrib.ptr);
+    L.fb.ps.meta = static_cast<float4 *>(b.meta.ptr);
+    for (int k = 0; k < 2; ++k) {
+        L.fb.rays[k].org = static_cast<float4 *>(b.rayOrg[k].ptr);
+        L.fb.rays[k].dir = static_cast<float4 *>(b.rayDir[k].ptr);
+    }
+    L.fb.hits.a = static_cast<int4 *>(b.hitA.ptr);
+    L.fb.hits.b = static_cast<float2 *>(b.hitB.ptr);
+    L.fb.shadow.org = static_cast<float4 *>(b.shOrg.ptr);
+    L.fb.shadow.dir = static_cast<float4 *>(b.shDir.ptr);
+    L.fb.shadow.value = static_cast<float4 *>(b.shVal.ptr);
+    L.fb.counters = static_cast<uint32_t *>(counters.ptr);
+    L.jobs = static_cast<DevJob *>(b.jobs.ptr);
+    L.jobOffsets = static_cast<const uint32_t *>(b.jobOffsets.ptr);
+    L.numJobs = static_cast<uint32_t>(jobs.size());
+
+    // schedule candidates (wavefront depth; the tail grid fixed), each timed on one step when
+    // autotuning is on and there are enough steps; otherwise the context's schedule
+    std::vector<uint32_t> cands;
+    if (autotune && maxSpp >= 8) cands = {1, 2, 3, 4, 6};
+    else cands = {wavefrontDepth};
+    std::map<uint32_t, std::unique_ptr<GraphExec>> graphs;
+    auto stepLaunch = [&](uint32_t wd, hipStream_t s) {
+        BatchLaunch Lw = L;
+        Lw.fp.wavefrontDepth = wd;
+        Lw.fp.tailBlocks = kDefaultTailBlocks;
+        if (!batchGraphs) {
+            VLRA_HIP(launchBatchStep(Lw, spectral, s));
+            return;
+        }
+        auto &ge = graphs[wd];
+        if (!ge) {
+            ge = std::make_unique<GraphExec>();
+            VLRA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            hipError_t le = launchBatchStep(Lw, spectral, s);
+            hipError_t ce = hipStreamEndCapture(s, &ge->graph);
+            VLRA_HIP(le);
+            VLRA_HIP(ce);
+            VLRA_HIP(hipGraphInstantiate(&ge->exec, ge->graph, nullptr, nullptr, 0));
+        }
+        VLRA_HIP(hipGraphLaunch(ge->exec, s));
+    };
+    std::vector<hipEvent_t> ev(maxSpp + 1);
+    for (auto &e : ev) VLRA_HIP(hipEventCreate(&e));
+    hipEvent_t ready;
+    VLRA_HIP(hipEventCreate(&ready));
+    VLRA_HIP(hipEventRecord(ready, userStream));
+    VLRA_HIP(hipStreamWaitEvent(b.stream, ready, 0));
+    VLRA_HIP(hipEventRecord(ev[0], b.stream));
+    uint32_t chosen = cands[0];
+    float best = 1e30f;
+    for (uint32_t step = 0; step < maxSpp; ++step) {
+        const bool tuning = cands.size() > 1 && step < cands.size();
+        const uint32_t wd = tuning ? cands[step] : chosen;
+        stepLaunch(wd, b.stream);
+        VLRA_HIP(hipEventRecord(ev[step + 1], b.stream));
+        if (tuning) {
+            VLRA_HIP(hipEventSynchronize(ev[step + 1]));
+            float ms = 0.0f;
+            VLRA_HIP(hipEventElapsedTime(&ms, ev[step], ev[step + 1]));
+            if (ms < best) {
+                best = ms;
+                chosen = wd;
+            }
+        }
+    }
+    VLRA_HIP(launchBatchConvert(L, spectral, b.stream));
+    hipEvent_t done;
+    VLRA_HIP(hipEventCreate(&done));
+    VLRA_HIP(hipEventRecord(done, b.stream));
+    VLRA_HIP(hipStreamWaitEvent(userStream, done, 0));
+    b.wavefrontDepth = chosen;
+    b.tailBlocks = kDefaultTailBlocks;
+    VLRA_HIP(hipEventSynchronize(done));
+    if (completionMs) {
+        for (size_t k = 0; k < jobs.size(); ++k) {
+            float ms = 0.0f;
+            VLRA_HIP(hipEventElapsedTime(&ms, ev[0], ev[jobs[k].spp]));
+            completionMs[k] = ms;
+        }
+    }
+    for (auto &e : ev) (void)hipEventDestroy(e);
+    (void)hipEventDestroy(ready);
+    (void)hipEventDestroy(done);
+    framesRendered += maxSpp;
+    for (const BatchJobDesc &j : jobs) pathsStarted += uint64_t(j.width) * j.height * j.spp;
+}
+
+void Context::readBatchOutput(uint32_t job, float *rgba) {
+    if (job >= batch.output.size() || !batch.output[job]) throw ApiError{3, "no such batch job"};
+    VLRA_HIP(hipDeviceSynchronize());
+    VLRA_HIP(hipMemcpy(rgba, batch.output[job]->ptr, size_t(batch.widths[job]) * batch.heights[job] * sizeof(float4),
+                       hipMemcpyDeviceToHost));
+}
+
 void Context::readOutput(float *dst) {
     VLRA_HIP(hipDeviceSynchronize());
     VLRA_HIP(hipMemcpy(dst, output.ptr, size_t(width) * height * sizeof(float4), hipMemcpyDeviceToHost));

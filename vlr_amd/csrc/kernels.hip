@@ -545,7 +545,34 @@ struct KArgs {
     FrameParams fp;
     FrameBuffers fb;
     uint32_t depth;
+    // batches: path i belongs to the job j with jobOffsets[j] <= i < jobOffsets[j + 1]
+    DevJob *jobs;
+    const uint32_t *jobOffsets;
+    uint32_t numJobs;
 };
+
+__device__ __forceinline__ uint32_t jobOf(const KArgs &a, uint32_t path) {
+    uint32_t lo = 0, hi = a.numJobs;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a.jobOffsets[mid] <= path) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// one thread per job: a job takes part in this step while it has samples left
+__global__ void k_batchStepBegin(DevJob *jobs, uint32_t numJobs) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= numJobs) return;
+    DevJob &J = jobs[j];
+    if (J.frameIndex < J.spp) {
+        J.frameIndex += 1;
+        J.active = 1;
+    } else {
+        J.active = 0;
+    }
+}
 
 __global__ void k_frameBegin(uint32_t *counters, uint32_t numPaths) {
     uint32_t i = threadIdx.x;
@@ -557,21 +584,18 @@ __global__ void k_frameBegin(uint32_t *counters, uint32_t numPaths) {
 }
 
 // ---- generate: path_tracing.cu:30-70 ----
+// Starts the path of pixel (px, py) of a W x H image: path state and primary ray.
 template <int N>
-__global__ void __launch_bounds__(256) k_generate(KArgs a) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t numPaths = a.fp.imageW * a.fp.imageH;
-    if (i >= numPaths) return;
-    const uint32_t px = i % a.fp.imageW, py = i / a.fp.imageW;
-    const uint32_t path = py * a.fp.width + px;
-    Pcg32 rng{a.fb.ps.rng[path]};
+__device__ __forceinline__ void generatePath(const FrameBuffers &fb, const DevCamera &cam, uint32_t W, uint32_t H,
+                                             uint32_t px, uint32_t py, uint32_t path, float4 *rayOrg,
+                                             float4 *rayDirOut) {
+    Pcg32 rng{fb.ps.rng[path]};
     float pX = px + rng.uniform();
     float pY = py + rng.uniform();
     float wlOffset = rng.uniform();
     float wlSelect = rng.uniform();
     uint32_t sel = min(static_cast<uint32_t>(static_cast<uint16_t>(N * wlSelect)), static_cast<uint32_t>(N - 1));
     float selectWLPDF = N == 4 ? N / (830.0f - 360.0f) : 1.0f;
-    const DevCamera &cam = a.sc.camera;
     float ul0 = rng.uniform(), ul1 = rng.uniform();
     float lx, ly;
     concentricSampleDisk(ul0, ul1, &lx, &ly);
@@ -584,22 +608,53 @@ __global__ void __launch_bounds__(256) k_generate(KArgs a) {
     V3 orgLocal(cam.lensRadius * lx, cam.lensRadius * ly, 0.0f);
     V3 lensPos = (c0 * orgLocal.x + c1 * orgLocal.y + c2 * orgLocal.z) + ld3(cam.position);
     float lensAreaPDF = cam.lensRadius > 0.0f ? 1.0f / (kPiF * cam.lensRadius * cam.lensRadius) : 1.0f;
-    float uD0 = pX / a.fp.imageW, uD1 = pY / a.fp.imageH;
+    float uD0 = pX / W, uD1 = pY / H;
     V3 pFocus(cam.opWidth * (0.5f - uD0), cam.opHeight * (0.5f - uD1), cam.objPlaneDistance);
     V3 dirLocal = normalize(pFocus - V3(cam.lensRadius * lx, cam.lensRadius * ly, 0.0f));
     float dirPDF = sq(cam.imgPlaneDistance) / (cube(dirLocal.z) * cam.imgPlaneArea);
-    dirPDF *= a.fp.imageW * a.fp.imageH;
+    dirPDF *= W * H;
     V3 rayDir = cf.fromLocal(dirLocal);
     Spec<N> alpha(cam.sensitivity);
     alpha = alpha * (absDot(rayDir, camN) / (lensAreaPDF * dirPDF * selectWLPDF));
     float initImportance = alpha.importance(sel);
-    a.fb.ps.rng[path] = rng.s;
-    a.fb.ps.alpha[path] = storeSpec<N>(alpha);
-    a.fb.ps.contrib[path] = make_float4(0, 0, 0, 0);
+    fb.ps.rng[path] = rng.s;
+    fb.ps.alpha[path] = storeSpec<N>(alpha);
+    fb.ps.contrib[path] = make_float4(0, 0, 0, 0);
     uint32_t flags = packFlags(sel, false, dt::Reflection | dt::LowFreq, 0);
-    a.fb.ps.meta[path] = make_float4(wlOffset, dirPDF, initImportance, __uint_as_float(flags));
-    a.fb.rays[0].org[i] = make_float4(lensPos.x, lensPos.y, lensPos.z, __uint_as_float(path));
-    a.fb.rays[0].dir[i] = make_float4(rayDir.x, rayDir.y, rayDir.z, 3.402823466e+38f);
+    fb.ps.meta[path] = make_float4(wlOffset, dirPDF, initImportance, __uint_as_float(flags));
+    *rayOrg = make_float4(lensPos.x, lensPos.y, lensPos.z, __uint_as_float(path));
+    *rayDirOut = make_float4(rayDir.x, rayDir.y, rayDir.z, 3.402823466e+38f);
+}
+
+template <int N, bool MULTI = false>
+__global__ void __launch_bounds__(256) k_generate(KArgs a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t numPaths = a.fp.imageW * a.fp.imageH;
+    if constexpr (MULTI) {
+        // paths of the jobs taking part in this step; the queue is appended wave by wave
+        bool emit = false;
+        float4 o, d;
+        if (i < numPaths) {
+            const DevJob &J = a.jobs[jobOf(a, i)];
+            if (J.active) {
+                const uint32_t local = i - J.pixelOffset;
+                generatePath<N>(a.fb, J.scene.camera, J.width, J.height, local % J.width, local / J.width, i, &o, &d);
+                emit = true;
+            }
+        }
+        const uint32_t slot = appendSlot(&a.fb.counters[0], emit);
+        if (emit) {
+            a.fb.rays[0].org[slot] = o;
+            a.fb.rays[0].dir[slot] = d;
+        }
+    } else {
+        if (i >= numPaths) return;
+        const uint32_t px = i % a.fp.imageW, py = i / a.fp.imageW;
+        float4 o, d;
+        generatePath<N>(a.fb, a.sc.camera, a.fp.imageW, a.fp.imageH, px, py, py * a.fp.width + px, &o, &d);
+        a.fb.rays[0].org[i] = o;
+        a.fb.rays[0].dir[i] = d;
+    }
 }
 
 // wave-wide sum (all 64 lanes must be active)
@@ -620,7 +675,7 @@ __device__ __forceinline__ void addTravStats(uint32_t *counters, uint32_t slot, 
 }
 
 // ---- intersect: closest hit ----
-template <bool STATS>
+template <bool STATS, bool MULTI = false>
 __global__ void __launch_bounds__(kTraceBlock) k_intersect(KArgs a) {
     __shared__ int32_t lds[kLdsStack * kTraceBlock];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -631,7 +686,12 @@ __global__ void __launch_bounds__(kTraceBlock) k_intersect(KArgs a) {
         float4 o = q.org[i], d = q.dir[i];
         TraceHit h;
         h.inst = -1;
-        bool found = traceRay<false, STATS>(a.sc, xyz(o), xyz(d), 0.0f, d.w, &h, lds, threadIdx.x, &tc);
+        bool found;
+        if constexpr (MULTI)
+            found = traceRay<false, STATS>(a.jobs[jobOf(a, __float_as_uint(o.w))].scene, xyz(o), xyz(d), 0.0f, d.w,
+                                           &h, lds, threadIdx.x, &tc);
+        else
+            found = traceRay<false, STATS>(a.sc, xyz(o), xyz(d), 0.0f, d.w, &h, lds, threadIdx.x, &tc);
         if (found) {
             a.fb.hits.a[i] = make_int4(h.inst, h.geomInst, h.prim, __float_as_int(h.t));
             a.fb.hits.b[i] = make_float2(h.b1, h.b2);
@@ -914,7 +974,7 @@ __device__ __forceinline__ void storePath(const FrameBuffers &fb, uint32_t path,
     }
 }
 
-template <int N>
+template <int N, bool MULTI = false>
 __global__ void __launch_bounds__(256) k_shade(KArgs a) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t depth = a.depth;
@@ -929,8 +989,14 @@ __global__ void __launch_bounds__(256) k_shade(KArgs a) {
         float2 hb = ha.x >= 0 ? a.fb.hits.b[i] : make_float2(0.0f, 0.0f);
         PathRegs<N> P;
         loadPath<N>(a.fb, path, &P);
-        shadeStep<N>(a.sc, depth + 1, a.fp.maxPathLength, path, xyz(o4), xyz(d4), ha, hb, P, cacheCtx(a.fp, a.fb),
-                     &emitShadow, &shOrg, &shDir, &shVal, &emitNext, &nextOrg, &nextDir);
+        if constexpr (MULTI) {
+            const DevJob &J = a.jobs[jobOf(a, path)];
+            shadeStep<N>(J.scene, depth + 1, J.maxPathLength, path, xyz(o4), xyz(d4), ha, hb, P, cacheCtx(a.fp, a.fb),
+                         &emitShadow, &shOrg, &shDir, &shVal, &emitNext, &nextOrg, &nextDir);
+        } else {
+            shadeStep<N>(a.sc, depth + 1, a.fp.maxPathLength, path, xyz(o4), xyz(d4), ha, hb, P, cacheCtx(a.fp, a.fb),
+                         &emitShadow, &shOrg, &shDir, &shVal, &emitNext, &nextOrg, &nextDir);
+        }
         storePath<N>(a.fb, path, depth + 1, P, emitNext);
     }
     // queue appends (every lane of the wave takes part in the ballots)
@@ -949,7 +1015,7 @@ __global__ void __launch_bounds__(256) k_shade(KArgs a) {
 }
 
 // ---- shadow: any-hit, adds the NEE contribution when unoccluded ----
-template <bool STATS>
+template <bool STATS, bool MULTI = false>
 __global__ void __launch_bounds__(kTraceBlock) k_shadow(KArgs a) {
     __shared__ int32_t lds[kLdsStack * kTraceBlock];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -958,7 +1024,12 @@ __global__ void __launch_bounds__(kTraceBlock) k_shadow(KArgs a) {
     if (i < count) {
         float4 o = a.fb.shadow.org[i], d = a.fb.shadow.dir[i];
         TraceHit h;
-        bool occluded = traceRay<true, STATS>(a.sc, xyz(o), xyz(d), 0.0f, d.w, &h, lds, threadIdx.x, &tc);
+        bool occluded;
+        if constexpr (MULTI)
+            occluded = traceRay<true, STATS>(a.jobs[jobOf(a, __float_as_uint(o.w))].scene, xyz(o), xyz(d), 0.0f,
+                                             d.w, &h, lds, threadIdx.x, &tc);
+        else
+            occluded = traceRay<true, STATS>(a.sc, xyz(o), xyz(d), 0.0f, d.w, &h, lds, threadIdx.x, &tc);
         if (!occluded) {
             uint32_t path = __float_as_uint(o.w);
             float4 v = a.fb.shadow.value[i];
@@ -976,7 +1047,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_shadow(KArgs a) {
 // random numbers and adds its contributions in the same order as the per-depth kernels, so a
 // frame's result does not depend on the depth at which paths switch to this kernel.  Rays per
 // depth are counted in LDS and added to the frame counters once per block.
-template <int N, bool STATS>
+template <int N, bool STATS, bool MULTI = false>
 __global__ void __launch_bounds__(kTraceBlock) k_tail(KArgs a) {
     __shared__ int32_t lds[kLdsStack * kTraceBlock];
     __shared__ uint32_t depthCounts[2 * kMaxDepthCounters];
@@ -990,7 +1061,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_tail(KArgs a) {
     TravCount tc, tcShadow;
     const uint32_t lane = __lane_id();
     bool active = false, drained = false;
-    uint32_t path = 0, depth = depth0;
+    uint32_t path = 0, depth = depth0, job = 0;
     PathRegs<N> P;
     V3 org(0.0f), dir(0.0f, 0.0f, 1.0f);
     float tmax = 0.0f;
@@ -1009,6 +1080,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_tail(KArgs a) {
                     if (i < count) {
                         const float4 o4 = q.org[i], d4 = q.dir[i];
                         path = __float_as_uint(o4.w);
+                        if constexpr (MULTI) job = jobOf(a, path);
                         loadPath<N>(a.fb, path, &P);
                         org = xyz(o4);
                         dir = xyz(d4);
@@ -1021,19 +1093,21 @@ __global__ void __launch_bounds__(kTraceBlock) k_tail(KArgs a) {
         }
         if (!__ballot(active)) break;
         if (!active) continue;
+        const DevScene &psc = MULTI ? a.jobs[job].scene : sc;
+        const uint32_t maxLen = MULTI ? a.jobs[job].maxPathLength : a.fp.maxPathLength;
         TraceHit h;
         h.inst = -1;
         h.b1 = h.b2 = 0.0f;
-        bool found = traceRay<false, STATS>(sc, org, dir, 0.0f, tmax, &h, lds, threadIdx.x, &tc);
+        bool found = traceRay<false, STATS>(psc, org, dir, 0.0f, tmax, &h, lds, threadIdx.x, &tc);
         int4 ha = found ? make_int4(h.inst, h.geomInst, h.prim, 0) : make_int4(-1, -1, -1, 0);
         bool emitShadow, emitNext;
         float4 shOrg, shDir, shVal, nextOrg, nextDir;
-        shadeStep<N>(sc, depth + 1, a.fp.maxPathLength, path, org, dir, ha, make_float2(h.b1, h.b2), P, cc,
-                     &emitShadow, &shOrg, &shDir, &shVal, &emitNext, &nextOrg, &nextDir);
+        shadeStep<N>(psc, depth + 1, maxLen, path, org, dir, ha, make_float2(h.b1, h.b2), P, cc, &emitShadow,
+                     &shOrg, &shDir, &shVal, &emitNext, &nextOrg, &nextDir);
         if (emitShadow) {
             atomicAdd(&depthCounts[2 * depth + 1], 1u);
             TraceHit sh;
-            bool occluded = traceRay<true, STATS>(sc, xyz(shOrg), xyz(shDir), 0.0f, shDir.w, &sh, lds,
+            bool occluded = traceRay<true, STATS>(psc, xyz(shOrg), xyz(shDir), 0.0f, shDir.w, &sh, lds,
                                                   threadIdx.x, &tcShadow);
             if (!occluded) P.contrib += loadSpec<N>(shVal);
         }
@@ -1152,20 +1226,33 @@ __global__ void __launch_bounds__(256) k_rcGatherSlots(const float4 *trainRaw, c
 }
 
 // ---- accumulate: Kahan sum per channel (common_internal.h:350-368, spectrum_types.h:674-684) ----
-template <int N>
+template <int N, bool MULTI = false>
 __global__ void __launch_bounds__(256) k_accumulate(KArgs a) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t numPaths = a.fp.imageW * a.fp.imageH;
     if (i >= numPaths) return;
-    const uint32_t px = i % a.fp.imageW, py = i / a.fp.imageW;
-    const uint32_t p = py * a.fp.width + px;
+    uint32_t p;
+    size_t plane;
+    float *acc;
+    bool reset;
+    if constexpr (MULTI) {
+        const DevJob &J = a.jobs[jobOf(a, i)];
+        if (!J.active) return;
+        p = i;
+        plane = static_cast<size_t>(J.width) * J.height;
+        acc = J.accum + (i - J.pixelOffset);
+        reset = J.frameIndex == 1;
+    } else {
+        const uint32_t px = i % a.fp.imageW, py = i / a.fp.imageW;
+        p = py * a.fp.width + px;
+        // planes of width*height floats: sum of channel k at plane 2k, compensation at 2k + 1
+        plane = static_cast<size_t>(a.fp.width) * a.fp.height;
+        acc = a.fb.accum + p;
+        reset = a.fp.frameIndex == 1;
+    }
     constexpr int C = N == 4 ? 16 : 3;
     Spec<N> c = loadSpec<N>(a.fb.ps.contrib[p]);
     bool finite = c.allFinite();
-    // planes of width*height floats: sum of channel k at plane 2k, compensation at 2k + 1
-    const size_t plane = static_cast<size_t>(a.fp.width) * a.fp.height;
-    float *acc = a.fb.accum + p;
-    const bool reset = a.fp.frameIndex == 1;
     if (!finite) {
         atomicAdd(&a.fb.counters[kStatDropped], 1u);
         if (reset) {
@@ -1203,41 +1290,64 @@ __global__ void __launch_bounds__(256) k_accumulate(KArgs a) {
 }
 
 // ---- convert: post_process.cu:93-163 ----
-template <int N>
+template <int N, bool MULTI = false>
 __global__ void __launch_bounds__(256) k_convert(KArgs a) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t numPaths = a.fp.imageW * a.fp.imageH;
     if (i >= numPaths) return;
-    const uint32_t px = i % a.fp.imageW, py = i / a.fp.imageW;
-    const uint32_t p = py * a.fp.width + px;
-    const size_t plane = static_cast<size_t>(a.fp.width) * a.fp.height;
-    const float *acc = a.fb.accum + p;
+    size_t plane;
+    const float *acc;
+    float4 *out;
+    uint32_t frames, W, H;
+    const DevScene *scp;
+    if constexpr (MULTI) {
+        const DevJob &J = a.jobs[jobOf(a, i)];
+        plane = static_cast<size_t>(J.width) * J.height;
+        acc = J.accum + (i - J.pixelOffset);
+        out = J.output + (i - J.pixelOffset);
+        frames = J.frameIndex;
+        W = J.width;
+        H = J.height;
+        scp = &J.scene;
+    } else {
+        const uint32_t px = i % a.fp.imageW, py = i / a.fp.imageW;
+        const uint32_t p = py * a.fp.width + px;
+        plane = static_cast<size_t>(a.fp.width) * a.fp.height;
+        acc = a.fb.accum + p;
+        out = a.fb.output + p;
+        frames = a.fp.frameIndex;
+        W = a.fp.imageW;
+        H = a.fp.imageH;
+        scp = nullptr;
+    }
+    const DevScene &sc = MULTI ? *scp : a.sc;
+    if (frames == 0) return;
     float X, Y, Z;
     if constexpr (N == 4) {
         X = Y = Z = 0.0f;
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
             float v = acc[(2 * k) * plane];
-            X += a.sc.cmfX[k] * v;
-            Y += a.sc.cmfY[k] * v;
-            Z += a.sc.cmfZ[k] * v;
+            X += sc.cmfX[k] * v;
+            Y += sc.cmfY[k] * v;
+            Z += sc.cmfZ[k] * v;
         }
-        X /= a.sc.integralCMF;
-        Y /= a.sc.integralCMF;
-        Z /= a.sc.integralCMF;
+        X /= sc.integralCMF;
+        Y /= sc.integralCMF;
+        Z /= sc.integralCMF;
     } else {
         float r = acc[0], g = acc[2 * plane], b = acc[4 * plane];
         X = 0.4124564f * r + 0.3575761f * g + 0.1804375f * b;
         Y = 0.2126729f * r + 0.7151522f * g + 0.0721750f * b;
         Z = 0.0193339f * r + 0.1191920f * g + 0.9503041f * b;
     }
-    float rec = 1.0f / a.fp.frameIndex;
+    float rec = 1.0f / frames;
     X *= rec; Y *= rec; Z *= rec;
     float R = 3.2404542f * X + -1.5371385f * Y + -0.4985314f * Z;
     float G = -0.9692660f * X + 1.8760108f * Y + 0.0415560f * Z;
     float B = 0.0556434f * X + -0.2040259f * Y + 1.0572252f * Z;
-    float res = static_cast<float>(a.fp.imageW) * static_cast<float>(a.fp.imageH);
-    a.fb.output[p] = make_float4(R * res, G * res, B * res, 1.0f);
+    float res = static_cast<float>(W) * static_cast<float>(H);
+    *out = make_float4(R * res, G * res, B * res, 1.0f);
 }
 
 __global__ void k_frameEnd(uint32_t *counters, uint32_t maxPathLength, uint32_t wavefrontDepth) {
@@ -2193,6 +2303,9 @@ static hipError_t launchFrameT(const DevScene &sc, const FrameParams &fp, const 
     a.fp = fp;
     a.fb = fb;
     a.depth = 0;
+    a.jobs = nullptr;
+    a.jobOffsets = nullptr;
+    a.numJobs = 0;
     const uint32_t n = fp.imageW * fp.imageH;
     auto t0 = [&](int st) { if (timer) timer->begin(st); };
     auto t1 = [&](int st) { if (timer) timer->end(st); };
@@ -2272,6 +2385,64 @@ hipError_t launchFrame(const DevScene &sc, const FrameParams &fp, const FrameBuf
     return spectral ? launchFrameT<4>(sc, fp, fb, s, timer, cache) : launchFrameT<3>(sc, fp, fb, s, timer, cache);
 }
 
+// One step of a batch: every job with samples left renders one more sample per pixel.  The
+// launch sequence reads nothing from the host that changes between steps (the jobs' frame
+// counters live on the device), so it can be captured once into a graph and replayed.
+template <int N>
+static hipError_t launchBatchStepT(const BatchLaunch &b, hipStream_t s) {
+    KArgs a;
+    a.sc = b.anyScene;
+    a.fp = b.fp;
+    a.fb = b.fb;
+    a.depth = 0;
+    a.jobs = b.jobs;
+    a.jobOffsets = b.jobOffsets;
+    a.numJobs = b.numJobs;
+    const uint32_t n = b.fp.imageW * b.fp.imageH;
+    hipLaunchKernelGGL(k_batchStepBegin, dim3(blocksFor(b.numJobs, 64)), dim3(64), 0, s, b.jobs, b.numJobs);
+    hipLaunchKernelGGL(k_frameBegin, dim3(1), dim3(256), 0, s, b.fb.counters, 0u);
+    hipLaunchKernelGGL((k_generate<N, true>), dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
+    const uint32_t numDepths = std::min<uint32_t>(b.fp.maxPathLength, kMaxDepthCounters - 1);
+    const uint32_t wavefrontDepths = std::min(b.fp.wavefrontDepth, numDepths);
+    for (uint32_t d = 0; d < wavefrontDepths; ++d) {
+        a.depth = d;
+        hipLaunchKernelGGL((k_intersect<false, true>), dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
+        hipLaunchKernelGGL((k_shade<N, true>), dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_shadow<false, true>), dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
+    }
+    if (wavefrontDepths < numDepths) {
+        a.depth = wavefrontDepths;
+        const uint32_t blocks = std::max(1u, std::min(blocksFor(n, kTraceBlock), b.fp.tailBlocks));
+        hipLaunchKernelGGL((k_tail<N, false, true>), dim3(blocks), dim3(kTraceBlock), 0, s, a);
+    }
+    hipLaunchKernelGGL((k_accumulate<N, true>), dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_frameEnd, dim3(1), dim3(64), 0, s, b.fb.counters, b.fp.maxPathLength, wavefrontDepths);
+    return hipGetLastError();
+}
+
+hipError_t launchBatchStep(const BatchLaunch &b, bool spectral, hipStream_t s) {
+    if (b.numJobs == 0) return hipSuccess;
+    return spectral ? launchBatchStepT<4>(b, s) : launchBatchStepT<3>(b, s);
+}
+
+hipError_t launchBatchConvert(const BatchLaunch &b, bool spectral, hipStream_t s) {
+    if (b.numJobs == 0) return hipSuccess;
+    KArgs a;
+    a.sc = b.anyScene;
+    a.fp = b.fp;
+    a.fb = b.fb;
+    a.depth = 0;
+    a.jobs = b.jobs;
+    a.jobOffsets = b.jobOffsets;
+    a.numJobs = b.numJobs;
+    const uint32_t n = b.fp.imageW * b.fp.imageH;
+    if (spectral)
+        hipLaunchKernelGGL((k_convert<4, true>), dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_convert<3, true>), dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
 template <int N>
 static hipError_t launchBptFrameT(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb,
                                   const BptParams &bp, hipStream_t s, LaunchTimer *timer) {
@@ -2285,6 +2456,9 @@ static hipError_t launchBptFrameT(const DevScene &sc, const FrameParams &fp, con
     k.fp = fp;
     k.fb = fb;
     k.depth = 0;
+    k.jobs = nullptr;
+    k.jobOffsets = nullptr;
+    k.numJobs = 0;
     const uint32_t n = fp.imageW * fp.imageH;
     const uint32_t scanBlocks = blocksFor(bp.numLightPaths, kScanBlock);
     if (bp.numLightPaths == 0 || scanBlocks > kScanBlock) return hipErrorInvalidValue;

@@ -155,6 +155,29 @@ struct CacheLaunch {
 hipError_t launchFrame(const DevScene& scene, const FrameParams& fp, const FrameBuffers& fb, bool spectral,
                        hipStream_t stream, LaunchTimer* timer, const CacheLaunch* cache = nullptr);
 
+// A batch of render jobs over one set of path buffers (paths of job j are the pixels
+// jobOffsets[j] .. jobOffsets[j + 1] - 1).  fp.imageW is the total pixel count (imageH = 1),
+// fp.maxPathLength the largest of the jobs', fp.wavefrontDepth / tailBlocks the schedule.
+struct DevJob {
+    DevScene scene;                // with the job's camera
+    float* accum;                  // Kahan planes of width * height floats
+    float4* output;
+    uint32_t pixelOffset, width, height, spp, maxPathLength;
+    uint32_t frameIndex;           // samples started (advanced on the device at each step)
+    uint32_t active;               // the job renders in the current step
+    uint32_t pad;
+};
+struct BatchLaunch {
+    DevScene anyScene;             // shared tables (colour matching) for the kernels' fixed arguments
+    FrameParams fp;
+    FrameBuffers fb;
+    DevJob* jobs;
+    const uint32_t* jobOffsets;
+    uint32_t numJobs;
+};
+hipError_t launchBatchStep(const BatchLaunch& b, bool spectral, hipStream_t stream);
+hipError_t launchBatchConvert(const BatchLaunch& b, bool spectral, hipStream_t stream);
+
 // One LVC-BPT frame: light paths, cache compaction, eye paths, accumulation, conversion.
 hipError_t launchBptFrame(const DevScene& scene, const FrameParams& fp, const FrameBuffers& fb, const BptParams& bp,
                           bool spectral, hipStream_t stream, LaunchTimer* timer);

@@ -362,6 +362,24 @@ struct SceneGPU {
 
 void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *out);
 
+// ---- batches of render jobs ----
+struct BatchJobDesc {
+    const Scene *scene;
+    const Camera *camera;
+    uint32_t width, height, spp, maxPathLength;
+};
+struct BatchState {
+    std::map<const Scene *, std::unique_ptr<SceneGPU>> scenes;
+    uint64_t sceneVersion = ~0ull;
+    bool spectral = false;
+    DeviceBuffer rng, alpha, contrib, meta, rayOrg[2], rayDir[2], hitA, hitB, shOrg, shDir, shVal, jobs, jobOffsets;
+    std::vector<std::unique_ptr<DeviceBuffer>> accum, output;
+    std::vector<uint32_t> widths, heights;
+    hipStream_t stream = nullptr;
+    uint32_t wavefrontDepth = 0, tailBlocks = 0;   // schedule of the last batch
+    ~BatchState();
+};
+
 // ---- context ----
 class Context : public Object {
 public:
@@ -448,6 +466,11 @@ public:
     void renderBptFrame(hipStream_t stream, const FrameParams &fp, bool firstFrame);
     void readLightRngStates(uint64_t *dst);
     void writeLightRngStates(const uint64_t *src);
+    // batches
+    BatchState batch;
+    bool batchGraphs = true;
+    void renderBatch(hipStream_t stream, const std::vector<BatchJobDesc> &jobs, float *completionMs);
+    void readBatchOutput(uint32_t job, float *rgba);
     // radiance cache
     void ensureCacheNet();
     void resetCache();

@@ -275,6 +275,41 @@ def cornell_many_lights(seed: int = 20261202, num_points: int = 64, num_quads: i
     return SceneDesc(mats, meshes, insts, desc.camera, None, False, "cornell_many_lights")
 
 
+def tessellated_cornell(seed: int, target_tris: int) -> SceneDesc:
+    """The Cornell box holding unit icospheres of jittered radius, each its own mesh (unique
+    triangles, no instancing), subdivided and counted so the scene has about `target_tris`
+    triangles; matte spheres of seeded albedo."""
+    desc = cornell_box(seed)
+    rng = np.random.default_rng(seed + 11)
+    level = 3
+    while level < 7 and 20 * 4 ** (level + 1) * 4 <= target_tris:
+        level += 1
+    per = 20 * 4 ** level
+    count = max(1, int(round((target_tris - desc.num_triangles) / per)))
+    sv, si = _icosphere_cached(level)
+    mats, meshes, insts = list(desc.materials), list(desc.meshes), list(desc.instances)
+    first = len(meshes)
+    for k in range(count):
+        alb = rng.uniform(0.1, 0.8, size=3)
+        mats.append(MaterialDesc(MATTE, {"albedo": (REC709, *map(float, alb))}))
+        r = float(rng.uniform(0.08, 0.3))
+        c = rng.uniform((-1.2, 0.3, -1.2), (1.2, 2.5, 1.2))
+        v = sv.copy()
+        v[:, 0:3] = v[:, 0:3] * r + c.astype(np.float32)
+        meshes.append(MeshDesc(f"Blob{k}", v, [(si, len(mats) - 1)]))
+    insts.append(InstanceDesc("Blobs", identity4(), list(range(first, len(meshes)))))
+    return SceneDesc(mats, meshes, insts, desc.camera, None, False, f"tessellated_cornell_{target_tris}")
+
+
+_ICO_CACHE = {}
+
+
+def _icosphere_cached(level):
+    if level not in _ICO_CACHE:
+        _ICO_CACHE[level] = icosphere(level)
+    return _ICO_CACHE[level]
+
+
 # --------------------------------------------------------------------------------------------
 # Icosphere grid (spectral, GGX + dielectric, environment light)
 
