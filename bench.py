@@ -393,7 +393,9 @@ def batch_experiment(num_jobs, seed=20261202):
         bs = api.build_scene(c, d)
         c.bind_output(specs[k][1], specs[k][1])
         c.set_wavefront_depth(3)
+        seeds = c.read_rng()
         c.render(bs.camera, True, 1)          # scene set-up outside the timed region
+        c.write_rng(seeds)                    # the timed render starts from the same seeds as the batch
         ctxs.append((c, bs))
     hip_sync()
     serial_done, rays = [], 0
