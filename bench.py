@@ -11,7 +11,8 @@ The timed region is K steps starting from firstFrame, i.e. a K-spp render.
 The JSON line carries
   value          extension + shadow rays traced per second (device counters / wall clock)
   roofline       the dominant kernel's algorithmic bytes per launch over its mean launch
-                 time (HIP events on the render stream) against 8 TB/s HBM; the per-ray node,
+                 time (HIP events on the stream it runs on, in extra frames after the timed
+                 region) against 8 TB/s HBM; the per-ray node,
                  triangle and instance counts come from extra frames rendered after the timed
                  region with the traversal kernels' counting variant; `traffic` is the HBM
                  bytes per launch measured by rocprofv3 counters, read from the committed
@@ -44,7 +45,7 @@ RAY_IN = 32          # origin.xyz + path index, direction.xyz + tmax
 HIT_OUT = 24         # (inst, geomInst, prim, t) + (b1, b2)
 SHADOW_IN = 48       # origin, direction, contribution
 SHADOW_RMW = 32      # contribution read + write when unoccluded
-NODE = 128           # 4-wide BVH node: four child boxes + four child codes
+NODE = 64            # 4-wide BVH node: origin, grid steps, four quantised child boxes, four child codes
 TRI = 48             # three vertices + geometry-instance / primitive indices
 INST = 48            # world-to-object 3x4 rows read on instance entry
 # one shaded hit: ray 32 + hit 24 + path state read and written 2 x 56 + three vertices 144 +
@@ -439,6 +440,8 @@ def main():
     ap.add_argument("--depth", type=int, default=16)
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--stat-frames", type=int, default=2)
+    ap.add_argument("--timing-frames", type=int, default=8,
+                    help="frames rendered with per-launch HIP events, after the timed region")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--wavefront-depth", type=int, default=None,
@@ -491,7 +494,6 @@ def main():
     setup_s = time.perf_counter() - t_setup
 
     ctx.reset_stats()
-    ctx.set_kernel_timing(True)
     hip_sync()
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -501,8 +503,17 @@ def main():
     st = ctx.stats()
     qc, qs = ctx.queue_counts(args.depth)
     sched_wd, sched_tb, sched_tuned = ctx.schedule()
-    ctx.set_kernel_timing(False)
     img = ctx.read_output()
+
+    # per-kernel device time: more frames with HIP events around every launch (kernel timing
+    # synchronises each frame, so it stays out of the timed region above)
+    ctx.reset_stats()
+    ctx.set_kernel_timing(True)
+    for k in range(args.timing_frames):
+        ctx.render(built.camera, first_frame=(k == 0), limit=args.timing_frames)
+    hip_sync()
+    kt = ctx.stats()
+    ctx.set_kernel_timing(False)
 
     rays = st["closestRays"] + st["shadowRays"]
     value = rays / wall
@@ -525,13 +536,13 @@ def main():
     b_closest = traversal_bytes(RAY_IN + HIT_OUT, per_closest)
     b_shadow = traversal_bytes(SHADOW_IN + SHADOW_RMW, per_shadow)
     # rays by stage: per-depth kernels for the first depths, the persistent tail kernel for the rest
-    wf_c, wf_s = st["closestRaysWavefront"], st["shadowRaysWavefront"]
-    tail_c, tail_s = st["closestRays"] - wf_c, st["shadowRays"] - wf_s
+    wf_c, wf_s = kt["closestRaysWavefront"], kt["shadowRaysWavefront"]
+    tail_c, tail_s = kt["closestRays"] - wf_c, kt["shadowRays"] - wf_s
     kernels = {
-        "k_intersect": (st["msIntersect"], st["launchesIntersect"], wf_c * b_closest),
-        "k_shadow": (st["msShadow"], st["launchesShadow"], wf_s * b_shadow),
-        "k_shade": (st["msShade"], st["launchesShade"], wf_c * SHADE),
-        "k_tail": (st["msTail"], st["launchesTail"], tail_c * (b_closest + SHADE) + tail_s * b_shadow),
+        "k_intersect": (kt["msIntersect"], kt["launchesIntersect"], wf_c * b_closest),
+        "k_shadow": (kt["msShadow"], kt["launchesShadow"], wf_s * b_shadow),
+        "k_shade": (kt["msShade"], kt["launchesShade"], wf_c * SHADE),
+        "k_tail": (kt["msTail"], kt["launchesTail"], tail_c * (b_closest + SHADE) + tail_s * b_shadow),
     }
     dominant = max(kernels, key=lambda k: kernels[k][0])
     ms, launches, total_bytes = kernels[dominant]
@@ -558,9 +569,9 @@ def main():
         "samples_per_s": samples_per_s,
         "rays": {"closest": st["closestRays"], "shadow": st["shadowRays"], "paths": st["paths"],
                  "dropped_non_finite": st["droppedNonFinite"]},
-        "stage_ms": {"generate": st["msGenerate"], "intersect": st["msIntersect"], "shade": st["msShade"],
-                     "shadow": st["msShadow"], "tail": st["msTail"], "accumulate": st["msAccumulate"],
-                     "convert": st["msConvert"]},
+        "stage_ms_per_frame": {k: kt["ms" + k.capitalize()] / args.timing_frames
+                               for k in ("generate", "intersect", "shade", "shadow", "tail", "accumulate", "convert")},
+        "timing_frames": args.timing_frames,
         "schedule": {"wavefront_depth": sched_wd, "tail_blocks": sched_tb, "autotuned": sched_tuned},
         "rays_by_stage": {"per_depth_closest": wf_c, "per_depth_shadow": wf_s, "tail_closest": tail_c,
                           "tail_shadow": tail_s},

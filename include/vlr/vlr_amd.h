@@ -69,6 +69,10 @@ VLR_API VLRResult vlrContextGetWavefrontDepth(VLRContextConst context, uint32_t*
  * frames after a change of scene, size or mode time candidate schedules (wavefront depth,
  * then the tail kernel's grid) and keep the fastest.  Tuning frames synchronise the stream. */
 VLR_API VLRResult vlrContextSetScheduleAutotuning(VLRContext context, bool enable);
+/* Frames of at least 2^18 pixels rendered as two pixel halves on two streams, each half
+ * running its own frames, so that the deep paths of one half overlap the per-depth kernels of
+ * the other (off by default; never with the radiance cache).  The image does not depend on it. */
+VLR_API VLRResult vlrContextSetSplitFrames(VLRContext context, bool enable);
 /* The schedule of the last rendered frame and whether tuning has finished. */
 VLR_API VLRResult vlrContextGetSchedule(VLRContextConst context, uint32_t* wavefrontDepth, uint32_t* tailBlocks,
                                         bool* tuned);

@@ -14,10 +14,11 @@ from vlr_amd import api, scenes
 pytestmark = pytest.mark.gpu
 
 
-def render_gpu(desc, W, H, frames, max_len, spectral, wavefront_depth=None, want_accum=False):
+def render_gpu(desc, W, H, frames, max_len, spectral, wavefront_depth=None, want_accum=False, split=True):
     with api.Context(spectral=spectral, max_path_length=max_len) as ctx:
         if wavefront_depth is not None:
             ctx.set_wavefront_depth(wavefront_depth)
+        ctx.set_split_frames(split)
         bs = api.build_scene(ctx, desc)
         ctx.bind_output(W, H)
         imgs = []
@@ -115,6 +116,22 @@ def test_frame_schedule_does_not_change_the_image(torch_gpu, scene, spectral):
         assert np.array_equal(acc.view(np.uint32), base[3].view(np.uint32)), wd
         assert np.array_equal(img.view(np.uint32), base[0].view(np.uint32)), wd
         assert st["closestRays"] == base[1]["closestRays"] and st["shadowRays"] == base[1]["shadowRays"], wd
+
+
+def test_split_frames_do_not_change_the_image(torch_gpu):
+    """A frame rendered as two halves on two streams gives the bits of a one-part frame."""
+    desc = scenes.icosphere_grid(grid=6, env_size=(128, 64))
+    W, H = 768, 432          # 331,776 pixels: above the split threshold
+    runs = []
+    for split in (False, True):
+        imgs, st, rng, acc = render_gpu(desc, W, H, 2, 16, True, wavefront_depth=3, want_accum=True, split=split)
+        runs.append((imgs[-1], st, rng, acc))
+        print("split", split, "rays", st["closestRays"], st["shadowRays"])
+    a, b = runs
+    assert np.array_equal(a[2], b[2])
+    assert np.array_equal(a[3].view(np.uint32), b[3].view(np.uint32))
+    assert np.array_equal(a[0].view(np.uint32), b[0].view(np.uint32))
+    assert a[1]["closestRays"] == b[1]["closestRays"] and a[1]["shadowRays"] == b[1]["shadowRays"]
 
 
 def test_autotuned_schedule_matches_fixed_schedule(torch_gpu):

@@ -208,6 +208,7 @@ PROTOTYPES: Dict[str, list] = {
     "vlrContextReadLightRNGStates": [H, P],
     "vlrContextWriteLightRNGStates": [H, P],
     "vlrContextSetNumLightPaths": [H, U32],
+    "vlrContextSetSplitFrames": [H, B],
     "vlrContextRenderBatch": [H, H, P, U32, P],
     "vlrContextReadBatchOutput": [H, U32, P],
     "vlrContextSetBatchGraphs": [H, B],
@@ -462,6 +463,9 @@ class Context:
 
     def set_wavefront_depth(self, depth: int):
         call("vlrContextSetWavefrontDepth", self.h, depth)
+
+    def set_split_frames(self, on: bool):
+        call("vlrContextSetSplitFrames", self.h, on)
 
     def set_autotune(self, on: bool):
         call("vlrContextSetScheduleAutotuning", self.h, on)

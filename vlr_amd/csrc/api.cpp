@@ -1190,6 +1190,14 @@ VLR_API VLRResult vlrContextGetNumLightPaths(VLRContextConst context, uint32_t *
     return VLRResult_NoError;
     VLRA_CATCH
 }
+VLR_API VLRResult vlrContextSetSplitFrames(VLRContext context, bool enable) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    CTX(context)->splitFrames = enable;
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+
 // ---- batches ----
 VLR_API VLRResult vlrContextRenderBatch(VLRContext context, hipStream_t stream, const VLRBatchJob *jobs,
                                         uint32_t numJobs, float *completionMs) {

@@ -33,10 +33,18 @@ Context::~Context() {
     for (hipEvent_t e : eventPool) (void)hipEventDestroy(e);
     for (hipEvent_t e : tuneEvents)
         if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : eventPoolB) (void)hipEventDestroy(e);
+    for (hipEvent_t e : partEvents)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : wavefrontEvents)
+        if (e) (void)hipEventDestroy(e);
+    for (hipStream_t ps : partStreams)
+        if (ps) (void)hipStreamDestroy(ps);
     for (Object *o : std::set<Object *>(objects)) delete o;
 }
 
 void Context::resizeAccum() {
+    VLRA_HIP(hipDeviceSynchronize());   // frames of split halves may still run on their streams
     accumChannels = spectralActive ? 16 : 3;
     size_t n = size_t(width) * height;
     accum.reserve(n * accumChannels * 2 * sizeof(float));
@@ -156,6 +164,8 @@ void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCo
         fp.cacheThreshold = 0.0f;
         fp.trainStride = 1;
         fp.trainOffset = 0;
+        fp.pixelBegin = 0;
+        fp.pixelCount = fp.imageW * fp.imageH;
         if (bpt) {
             renderBptFrame(stream, fp, firstFrame);
             if (numAccum) *numAccum = numAccumFrames;
@@ -164,7 +174,8 @@ void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCo
         // autotune the schedule per (scene version, size, mode, path length)
         const uint64_t tuneKey = (version * 1000003ull) ^ (uint64_t(width) << 40) ^ (uint64_t(height) << 20) ^
                                  (uint64_t(fp.imageW) << 8) ^ (spectralActive ? 1u : 0u) ^
-                                 (uint64_t(fp.maxPathLength) << 2) ^ (cacheEnabled ? 0x2ull : 0ull);
+                                 (uint64_t(fp.maxPathLength) << 2) ^ (cacheEnabled ? 0x2ull : 0ull) ^
+                                 (splitFrames ? 0x1000000000000000ull : 0ull);
         bool tuning = false;
         if (autotune && !traversalStats) {
             if (tuner.key != tuneKey || tuner.cands.empty()) tuner.start(tuneKey);
@@ -223,7 +234,12 @@ void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCo
             cl.slotTargets = static_cast<float4 *>(csTargets.ptr);
             clp = &cl;
         }
-        if (kernelTiming) {
+        const bool split = splitFrames && !cacheEnabled && fp.pixelCount >= kSplitMinPixels;
+        lastFrameSplit = split;
+        if (!split) joinHalves(stream);
+        if (split) {
+            renderSplitFrame(stream, fp, fb, tuning);
+        } else if (kernelTiming) {
             EventTimer t(stream, eventPool);
             VLRA_HIP(launchFrame(gpu.dev, fp, fb, spectralActive, stream, &t, clp));
             t.collect(stageMs, launches);
@@ -242,6 +258,71 @@ void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCo
         pathsStarted += uint64_t(fp.imageW) * fp.imageH;
     }
     if (numAccum) *numAccum = numAccumFrames;
+}
+
+// The frame as two pixel halves, each generated, traced, shaded, accumulated and converted on
+// its own stream with its own queues (the halves of the queue buffers) and counters.  A half
+// depends only on its own previous frame, so the streams are not joined after a frame: the
+// next frame of one half starts while the other half's deepest paths of this frame still
+// run.  Every call that reads or reuses the frame buffers joins first (joinHalves, or a
+// device synchronisation).  Every path computes exactly what it computes in a one-part frame.
+void Context::renderSplitFrame(hipStream_t stream, const FrameParams &fp, const FrameBuffers &fb, bool join) {
+    if (!partStreams[0]) {
+        for (auto &ps : partStreams) VLRA_HIP(hipStreamCreateWithFlags(&ps, hipStreamNonBlocking));
+        for (auto &e : partEvents) VLRA_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        for (auto &e : wavefrontEvents) VLRA_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    if (!counters2.ptr) {
+        counters2.reserve(kNumCounters * sizeof(uint32_t));
+        VLRA_HIP(hipMemset(counters2.ptr, 0, kNumCounters * sizeof(uint32_t)));
+    }
+    VLRA_HIP(hipEventRecord(partEvents[2], stream));
+    const uint32_t half = fp.pixelCount / 2;
+    EventTimer *timers[2] = {nullptr, nullptr};
+    std::unique_ptr<EventTimer> t0, t1;
+    if (kernelTiming) {
+        t0 = std::make_unique<EventTimer>(partStreams[0], eventPool);
+        t1 = std::make_unique<EventTimer>(partStreams[1], eventPoolB);
+        timers[0] = t0.get();
+        timers[1] = t1.get();
+    }
+    for (int k = 0; k < 2; ++k) {
+        VLRA_HIP(hipStreamWaitEvent(partStreams[k], partEvents[2], 0));
+        // the halves take turns with their per-depth kernels: each half's tail kernel (a few
+        // long path chains) overlaps the other half's per-depth kernels
+        if (wavefrontRecorded[1 - k]) VLRA_HIP(hipStreamWaitEvent(partStreams[k], wavefrontEvents[1 - k], 0));
+        FrameParams p = fp;
+        p.pixelBegin = k == 0 ? 0 : half;
+        p.pixelCount = k == 0 ? half : fp.pixelCount - half;
+        FrameBuffers b = fb;
+        const uint32_t o = p.pixelBegin;
+        for (int q = 0; q < 2; ++q) {
+            b.rays[q].org += o;
+            b.rays[q].dir += o;
+        }
+        b.hits.a += o;
+        b.hits.b += o;
+        b.shadow.org += o;
+        b.shadow.dir += o;
+        b.shadow.value += o;
+        if (k == 1) b.counters = static_cast<uint32_t *>(counters2.ptr);
+        VLRA_HIP(launchFrame(gpu.dev, p, b, spectralActive, partStreams[k], timers[k], nullptr, true,
+                             wavefrontEvents[k]));
+        wavefrontRecorded[k] = true;
+        VLRA_HIP(hipEventRecord(partEvents[k], partStreams[k]));
+    }
+    halvesPending = true;
+    if (kernelTiming) {
+        t0->collect(stageMs, launches);
+        t1->collect(stageMs, launches);
+    }
+    if (join) joinHalves(stream);
+}
+
+void Context::joinHalves(hipStream_t stream) {
+    if (!halvesPending) return;
+    for (int k = 0; k < 2; ++k) VLRA_HIP(hipStreamWaitEvent(stream, partEvents[k], 0));
+    halvesPending = false;
 }
 
 FrameBuffers Context::frameBuffers() {
@@ -299,6 +380,7 @@ void Context::ensureBpt(uint32_t slotsPerPath) {
 }
 
 void Context::renderBptFrame(hipStream_t stream, const FrameParams &fp, bool firstFrame) {
+    joinHalves(stream);
     const uint32_t slotsPerPath = fp.maxPathLength + 1;
     ensureBpt(slotsPerPath);
     if (firstFrame)
@@ -712,6 +794,7 @@ void Context::readOutput(float *dst) {
 }
 
 void Context::traceRays(hipStream_t stream, const Scene &s, const void *rays, uint32_t n, void *hits, bool anyHit) {
+    joinHalves(stream);
     ensureScene(s);
     VLRA_HIP(launchTraceRays(gpu.dev, rays, n, hits, anyHit, stream));
 }
@@ -740,27 +823,41 @@ void Context::statistics(uint64_t *closest, uint64_t *shadow, uint64_t *dropped,
     for (int k = 0; k < 6; ++k) trav[k] = 0;
     wavefront[0] = wavefront[1] = 0;
     if (device < 0) return;
-    uint32_t host[kNumCounters];
-    VLRA_HIP(hipDeviceSynchronize());
-    VLRA_HIP(hipMemcpy(host, counters.ptr, sizeof(host), hipMemcpyDeviceToHost));
-    std::memcpy(closest, host + kStatClosest, 8);
-    std::memcpy(shadow, host + kStatShadow, 8);
-    *dropped = host[kStatDropped];
-    std::memcpy(trav, host + kStatTravClosest, 24);
-    std::memcpy(trav + 3, host + kStatTravShadow, 24);
-    std::memcpy(wavefront, host + kStatWavefront, 16);
+    for (int part = 0; part < 2; ++part) {
+        const DeviceBuffer &cb = part == 0 ? counters : counters2;
+        if (!cb.ptr) continue;
+        uint32_t host[kNumCounters];
+        VLRA_HIP(hipDeviceSynchronize());
+        VLRA_HIP(hipMemcpy(host, cb.ptr, sizeof(host), hipMemcpyDeviceToHost));
+        uint64_t v[2];
+        std::memcpy(v, host + kStatClosest, 8);
+        *closest += v[0];
+        std::memcpy(v, host + kStatShadow, 8);
+        *shadow += v[0];
+        *dropped += host[kStatDropped];
+        uint64_t t[6];
+        std::memcpy(t, host + kStatTravClosest, 24);
+        std::memcpy(t + 3, host + kStatTravShadow, 24);
+        for (int k = 0; k < 6; ++k) trav[k] += t[k];
+        std::memcpy(v, host + kStatWavefront, 16);
+        wavefront[0] += v[0];
+        wavefront[1] += v[1];
+    }
 }
 
 void Context::queueCounts(uint32_t *closest, uint32_t *shadow, uint32_t maxDepth) {
-    uint32_t host[kNumCounters] = {};
+    uint32_t host[kNumCounters] = {}, host2[kNumCounters] = {};
     if (device >= 0) {
         VLRA_HIP(hipDeviceSynchronize());
         VLRA_HIP(hipMemcpy(host, counters.ptr, sizeof(host), hipMemcpyDeviceToHost));
+        // the second half's queues of the last frame, when it was split
+        if (counters2.ptr && lastFrameSplit)
+            VLRA_HIP(hipMemcpy(host2, counters2.ptr, sizeof(host2), hipMemcpyDeviceToHost));
     }
     for (uint32_t d = 0; d < maxDepth; ++d) {
         const bool inFrame = d < kMaxDepthCounters && d < lastFrame.maxPathLength;
-        closest[d] = inFrame ? host[2 * d] : 0;
-        shadow[d] = inFrame ? host[2 * d + 1] : 0;
+        closest[d] = inFrame ? host[2 * d] + host2[2 * d] : 0;
+        shadow[d] = inFrame ? host[2 * d + 1] + host2[2 * d + 1] : 0;
     }
 }
 
@@ -786,6 +883,7 @@ void Context::resetStats() {
     if (device >= 0) {
         VLRA_HIP(hipDeviceSynchronize());
         VLRA_HIP(hipMemset(counters.ptr, 0, kNumCounters * sizeof(uint32_t)));
+        if (counters2.ptr) VLRA_HIP(hipMemset(counters2.ptr, 0, kNumCounters * sizeof(uint32_t)));
     }
     for (int i = 0; i < NUM_STAGES; ++i) {
         stageMs[i] = 0;

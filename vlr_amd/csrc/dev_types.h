@@ -34,7 +34,7 @@ struct alignas(16) BVHNode {
 };
 static_assert(sizeof(BVHNode) == 64, "node layout");
 
-// 4-wide node, the layout the traversal reads (128 B, one L2 line): the four child boxes as
+// 4-wide node with full-precision child boxes (the build format): the four child boxes as
 // structure of arrays, then the four child codes (> 0 inner node, < 0 leaf code, 0 empty slot).
 // Built by collapsing the binned-SAH BVH2 (two levels into one where it pays).
 struct alignas(16) BVH4Node {
@@ -43,6 +43,19 @@ struct alignas(16) BVH4Node {
     uint32_t pad[4];
 };
 static_assert(sizeof(BVH4Node) == 128, "4-wide node layout");
+
+// The layout the traversal reads (64 B): child boxes quantised to 8 bits per plane on a grid of
+// the node box, origin + q * 2^(e - 127) per axis, rounded outwards so every decoded box
+// contains the exact one (and the decoding, one exact product and one rounded sum, is done the
+// same way on the host that checks it).  Byte k of qlo[a] / qhi[a] belongs to child k.
+struct alignas(16) BVH4QNode {
+    float origin[3];
+    uint32_t scaleExp;        // bytes 0, 1, 2: biased exponents of the x, y, z grid steps
+    uint32_t qlo[3], qhi[3];
+    int32_t child[4];
+    uint32_t pad[2];
+};
+static_assert(sizeof(BVH4QNode) == 64, "quantised 4-wide node layout");
 
 struct alignas(16) TriRecord {
     float v0[3]; uint32_t geomInst;
@@ -125,8 +138,8 @@ struct DevCamera {
 };
 
 struct DevScene {
-    const BVH4Node* tlasNodes;
-    const BVH4Node* blasNodes;
+    const BVH4QNode* tlasNodes;
+    const BVH4QNode* blasNodes;
     const TriRecord* blasTris;
     const DevInstance* instances;
     const uint32_t* instGeomInsts;

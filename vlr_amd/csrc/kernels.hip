@@ -30,8 +30,14 @@
 namespace vlra {
 
 constexpr int kTraceBlock = 128;
-constexpr int kLdsStack = 24;
+constexpr int kLdsStack = 16;
 constexpr int kScratchStack = 64;
+// per block: kLdsStack entries per lane of node codes, then as many entry distances
+constexpr int kTraceLds = 2 * kLdsStack * kTraceBlock;
+#ifndef VLRA_TAIL_WAVES_PER_EU
+#define VLRA_TAIL_WAVES_PER_EU 4
+#endif
+constexpr int kTailWavesPerEU = VLRA_TAIL_WAVES_PER_EU;   // occupancy asked of the tail kernel
 constexpr int32_t kSentinel = INT32_MIN;
 
 // ---------------------------------------------------------------------------------------------
@@ -115,41 +121,88 @@ struct TravCount {
     uint32_t nodes = 0, tris = 0, insts = 0;
 };
 
-template <bool ANY, bool STATS = false>
-__device__ __forceinline__ bool traceRay(const DevScene &sc, const V3 &org, const V3 &dir, float tmin, float tmax,
-                                         TraceHit *hit, int32_t *lds, int lane, TravCount *tc = nullptr) {
-    if (!sc.hasTlas) return false;
-    RayPre world;
-    world.init(org, dir);
-    RayPre ray = world;
-    int32_t ovf[kScratchStack];
-    int sp = 0;
-    bool inBlas = false;
-    const BVH4Node *nodes = sc.tlasNodes;
-    const TriRecord *tris = sc.blasTris;
-    int32_t curInst = -1;
-    bool found = false;
-    int32_t cur = 0;
-    auto push = [&](int32_t v) {
-        if (sp < kLdsStack) lds[sp * kTraceBlock + lane] = v;
-        else ovf[min(sp - kLdsStack, kScratchStack - 1)] = v;
-        ++sp;
+// Resumable traversal: the state of one ray's walk through the two-level BVH, advanced a
+// bounded number of steps (node fetches, instance entries or leaves) at a time.  The node stack
+// lives in LDS (kLdsStack entries per lane) with an overflow area the caller keeps in scratch.
+constexpr int32_t kTravDone = INT32_MIN + 1;
+struct TravState {
+    RayPre world, ray;
+    float tmin, tmax;
+    int32_t cur, curInst;
+    int sp;
+    uint32_t blasNode, blasTri;   // bases of the instance being traversed
+    bool inBlas, found, anyHit;
+    TraceHit hit;
+};
+
+__device__ __forceinline__ void travBegin(TravState &st, const DevScene &sc, const V3 &org, const V3 &dir, float tmin,
+                                          float tmax, bool anyHit) {
+    st.world.init(org, dir);
+    st.ray = st.world;
+    st.tmin = tmin;
+    st.tmax = tmax;
+    st.cur = sc.hasTlas ? 0 : kTravDone;
+    st.curInst = -1;
+    st.sp = 0;
+    st.blasNode = st.blasTri = 0;
+    st.inBlas = false;
+    st.found = false;
+    st.anyHit = anyHit;
+    st.hit.inst = -1;
+    st.hit.geomInst = st.hit.prim = -1;
+    st.hit.t = st.hit.b1 = st.hit.b2 = 0.0f;
+}
+
+// Advances the traversal by at most maxSteps steps; true once it has finished (then st.found
+// and st.hit hold the closest hit, or any hit for an any-hit ray).
+template <bool STATS>
+__device__ __forceinline__ bool travStep(TravState &st, const DevScene &sc, int32_t *lds, int lane, int32_t *ovf,
+                                         float *ovfT, int maxSteps, TravCount *tc) {
+    float *ldsT = reinterpret_cast<float *>(lds + kLdsStack * kTraceBlock);
+    // each stack entry keeps the distance at which the ray enters its box; once a closer hit
+    // has shrunk tmax past it (with the slab test's widening), the entry is dropped unvisited
+    auto push = [&](int32_t v, float tEnter) {
+        if (st.sp < kLdsStack) {
+            lds[st.sp * kTraceBlock + lane] = v;
+            ldsT[st.sp * kTraceBlock + lane] = tEnter;
+        } else {
+            ovf[min(st.sp - kLdsStack, kScratchStack - 1)] = v;
+            ovfT[min(st.sp - kLdsStack, kScratchStack - 1)] = tEnter;
+        }
+        ++st.sp;
     };
-    while (true) {
+    for (int step = 0; step < maxSteps && st.cur != kTravDone; ++step) {
+        const int32_t cur = st.cur;
         if (cur >= 0) {
             if constexpr (STATS) ++tc->nodes;
-            const float4 *np = reinterpret_cast<const float4 *>(nodes + cur);
-            const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
-            const int4 cc = reinterpret_cast<const int4 *>(np)[6];
+            const BVH4QNode *nodes = st.inBlas ? sc.blasNodes + st.blasNode : sc.tlasNodes;
+            const uint4 *np = reinterpret_cast<const uint4 *>(nodes + cur);
+            const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3];
+            // decode: origin + q * step per axis (step = 2^(e - 127), exact product)
+            const float ox = __uint_as_float(w0.x), oy = __uint_as_float(w0.y), oz = __uint_as_float(w0.z);
+            const float sx = __uint_as_float((w0.w & 0xFFu) << 23), sy = __uint_as_float(((w0.w >> 8) & 0xFFu) << 23),
+                        sz = __uint_as_float(((w0.w >> 16) & 0xFFu) << 23);
+            const uint32_t qlx = w1.x, qly = w1.y, qlz = w1.z, qhx = w1.w, qhy = w2.x, qhz = w2.y;
+            auto lo = [&](int k) {
+                return V3(ox + static_cast<float>((qlx >> (8 * k)) & 0xFFu) * sx,
+                          oy + static_cast<float>((qly >> (8 * k)) & 0xFFu) * sy,
+                          oz + static_cast<float>((qlz >> (8 * k)) & 0xFFu) * sz);
+            };
+            auto hi = [&](int k) {
+                return V3(ox + static_cast<float>((qhx >> (8 * k)) & 0xFFu) * sx,
+                          oy + static_cast<float>((qhy >> (8 * k)) & 0xFFu) * sy,
+                          oz + static_cast<float>((qhz >> (8 * k)) & 0xFFu) * sz);
+            };
             // child code 0 (the root's index) marks an empty slot
             constexpr float kMiss = __builtin_huge_valf();
             float t0 = kMiss, t1 = kMiss, t2 = kMiss, t3 = kMiss;
-            int32_t c0 = cc.x, c1 = cc.y, c2 = cc.z, c3 = cc.w;
+            int32_t c0 = static_cast<int32_t>(w2.z), c1 = static_cast<int32_t>(w2.w), c2 = static_cast<int32_t>(w3.x),
+                    c3 = static_cast<int32_t>(w3.y);
             float tt;
-            if (c0 != 0 && slab(V3(lx.x, ly.x, lz.x), V3(hx.x, hy.x, hz.x), ray, tmin, tmax, &tt)) t0 = tt;
-            if (c1 != 0 && slab(V3(lx.y, ly.y, lz.y), V3(hx.y, hy.y, hz.y), ray, tmin, tmax, &tt)) t1 = tt;
-            if (c2 != 0 && slab(V3(lx.z, ly.z, lz.z), V3(hx.z, hy.z, hz.z), ray, tmin, tmax, &tt)) t2 = tt;
-            if (c3 != 0 && slab(V3(lx.w, ly.w, lz.w), V3(hx.w, hy.w, hz.w), ray, tmin, tmax, &tt)) t3 = tt;
+            if (c0 != 0 && slab(lo(0), hi(0), st.ray, st.tmin, st.tmax, &tt)) t0 = tt;
+            if (c1 != 0 && slab(lo(1), hi(1), st.ray, st.tmin, st.tmax, &tt)) t1 = tt;
+            if (c2 != 0 && slab(lo(2), hi(2), st.ray, st.tmin, st.tmax, &tt)) t2 = tt;
+            if (c3 != 0 && slab(lo(3), hi(3), st.ray, st.tmin, st.tmax, &tt)) t3 = tt;
             const int hits = (t0 != kMiss) + (t1 != kMiss) + (t2 != kMiss) + (t3 != kMiss);
             if (hits > 0) {
                 // sort the four (distance, child) pairs, nearest first; misses sort last
@@ -164,59 +217,92 @@ __device__ __forceinline__ bool traceRay(const DevScene &sc, const V3 &org, cons
                 cswap(t0, c0, t2, c2);
                 cswap(t1, c1, t3, c3);
                 cswap(t1, c1, t2, c2);
-                if (hits > 3) push(c3);
-                if (hits > 2) push(c2);
-                if (hits > 1) push(c1);
-                cur = c0;
+                if (hits > 3) push(c3, t3);
+                if (hits > 2) push(c2, t2);
+                if (hits > 1) push(c1, t1);
+                st.cur = c0;
                 continue;
             }
         } else if (cur != kSentinel) {
-            uint32_t code = ~static_cast<uint32_t>(cur);
-            uint32_t first = code >> kLeafShift;
-            uint32_t count = (code & (kMaxLeafPrims - 1)) + 1;
-            if (!inBlas) {
+            const uint32_t code = ~static_cast<uint32_t>(cur);
+            const uint32_t first = code >> kLeafShift;
+            const uint32_t count = (code & (kMaxLeafPrims - 1)) + 1;
+            if (!st.inBlas) {
                 // enter the instance: object-space ray, BLAS root, return marker on the stack
                 if constexpr (STATS) ++tc->insts;
                 const DevInstance &I = sc.instances[first];
-                V3 o = xfmPoint(I.w2o, world.org);
-                V3 d = xfmVector(I.w2o, world.dir);
-                ray.init(o, d);
-                curInst = static_cast<int32_t>(first);
-                nodes = sc.blasNodes + I.blasNodeBase;
-                tris = sc.blasTris + I.blasTriBase;
-                inBlas = true;
-                push(kSentinel);
-                cur = 0;
+                V3 o = xfmPoint(I.w2o, st.world.org);
+                V3 d = xfmVector(I.w2o, st.world.dir);
+                st.ray.init(o, d);
+                st.curInst = static_cast<int32_t>(first);
+                st.blasNode = I.blasNodeBase;
+                st.blasTri = I.blasTriBase;
+                st.inBlas = true;
+                push(kSentinel, -__builtin_huge_valf());
+                st.cur = 0;
                 continue;
             }
             if constexpr (STATS) tc->tris += count;
+            const TriRecord *tris = sc.blasTris + st.blasTri;
             for (uint32_t k = 0; k < count; ++k) {
                 const float4 *tp = reinterpret_cast<const float4 *>(tris + first + k);
                 const float4 q0 = tp[0], q1 = tp[1], q2 = tp[2];
                 float t, b1, b2;
-                if (intersectTri(ray, xyz(q0), xyz(q1), xyz(q2), tmin, tmax, &t, &b1, &b2)) {
-                    tmax = t;
-                    found = true;
-                    hit->inst = curInst;
-                    hit->geomInst = static_cast<int32_t>(__float_as_uint(q0.w));
-                    hit->prim = static_cast<int32_t>(__float_as_uint(q1.w));
-                    hit->t = t;
-                    hit->b1 = b1;
-                    hit->b2 = b2;
-                    if (ANY) return true;
+                if (intersectTri(st.ray, xyz(q0), xyz(q1), xyz(q2), st.tmin, st.tmax, &t, &b1, &b2)) {
+                    st.tmax = t;
+                    st.found = true;
+                    st.hit.inst = st.curInst;
+                    st.hit.geomInst = static_cast<int32_t>(__float_as_uint(q0.w));
+                    st.hit.prim = static_cast<int32_t>(__float_as_uint(q1.w));
+                    st.hit.t = t;
+                    st.hit.b1 = b1;
+                    st.hit.b2 = b2;
+                    if (st.anyHit) break;
                 }
+            }
+            if (st.anyHit && st.found) {
+                st.cur = kTravDone;
+                break;
             }
         } else {
             // leave the instance
-            inBlas = false;
-            ray = world;
-            nodes = sc.tlasNodes;
+            st.inBlas = false;
+            st.ray = st.world;
         }
-        if (sp == 0) break;
-        --sp;
-        cur = sp < kLdsStack ? lds[sp * kTraceBlock + lane] : ovf[min(sp - kLdsStack, kScratchStack - 1)];
+        // pop, dropping entries that lie beyond the closest hit so far
+        const float tCull = st.tmax * (1.0f + 8.0f * __FLT_EPSILON__);
+        int32_t next = kTravDone;
+        while (st.sp > 0) {
+            --st.sp;
+            float te;
+            int32_t v;
+            if (st.sp < kLdsStack) {
+                v = lds[st.sp * kTraceBlock + lane];
+                te = ldsT[st.sp * kTraceBlock + lane];
+            } else {
+                v = ovf[min(st.sp - kLdsStack, kScratchStack - 1)];
+                te = ovfT[min(st.sp - kLdsStack, kScratchStack - 1)];
+            }
+            if (st.anyHit || te <= tCull) {
+                next = v;
+                break;
+            }
+        }
+        st.cur = next;
     }
-    return found;
+    return st.cur == kTravDone;
+}
+
+template <bool ANY, bool STATS = false>
+__device__ __forceinline__ bool traceRay(const DevScene &sc, const V3 &org, const V3 &dir, float tmin, float tmax,
+                                         TraceHit *hit, int32_t *lds, int lane, TravCount *tc = nullptr) {
+    TravState st;
+    travBegin(st, sc, org, dir, tmin, tmax, ANY);
+    int32_t ovf[kScratchStack];
+    float ovfT[kScratchStack];
+    travStep<STATS>(st, sc, lds, lane, ovf, ovfT, INT32_MAX, tc);
+    if (st.found) *hit = st.hit;
+    return st.found;
 }
 
 // wave-aggregated queue append: one atomic per wave, lanes get consecutive slots
@@ -648,8 +734,9 @@ __global__ void __launch_bounds__(256) k_generate(KArgs a) {
             a.fb.rays[0].dir[slot] = d;
         }
     } else {
-        if (i >= numPaths) return;
-        const uint32_t px = i % a.fp.imageW, py = i / a.fp.imageW;
+        if (i >= a.fp.pixelCount) return;
+        const uint32_t pix = a.fp.pixelBegin + i;
+        const uint32_t px = pix % a.fp.imageW, py = pix / a.fp.imageW;
         float4 o, d;
         generatePath<N>(a.fb, a.sc.camera, a.fp.imageW, a.fp.imageH, px, py, py * a.fp.width + px, &o, &d);
         a.fb.rays[0].org[i] = o;
@@ -677,7 +764,7 @@ __device__ __forceinline__ void addTravStats(uint32_t *counters, uint32_t slot, 
 // ---- intersect: closest hit ----
 template <bool STATS, bool MULTI = false>
 __global__ void __launch_bounds__(kTraceBlock) k_intersect(KArgs a) {
-    __shared__ int32_t lds[kLdsStack * kTraceBlock];
+    __shared__ int32_t lds[kTraceLds];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const RayQueue q = a.fb.rays[a.depth & 1];
     const uint32_t count = a.fb.counters[2 * a.depth];
@@ -1017,7 +1104,7 @@ __global__ void __launch_bounds__(256) k_shade(KArgs a) {
 // ---- shadow: any-hit, adds the NEE contribution when unoccluded ----
 template <bool STATS, bool MULTI = false>
 __global__ void __launch_bounds__(kTraceBlock) k_shadow(KArgs a) {
-    __shared__ int32_t lds[kLdsStack * kTraceBlock];
+    __shared__ int32_t lds[kTraceLds];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t count = a.fb.counters[2 * a.depth + 1];
     TravCount tc;
@@ -1048,8 +1135,9 @@ __global__ void __launch_bounds__(kTraceBlock) k_shadow(KArgs a) {
 // frame's result does not depend on the depth at which paths switch to this kernel.  Rays per
 // depth are counted in LDS and added to the frame counters once per block.
 template <int N, bool STATS, bool MULTI = false>
-__global__ void __launch_bounds__(kTraceBlock) k_tail(KArgs a) {
-    __shared__ int32_t lds[kLdsStack * kTraceBlock];
+__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kTailWavesPerEU)))
+k_tail(KArgs a) {
+    __shared__ int32_t lds[kTraceLds];
     __shared__ uint32_t depthCounts[2 * kMaxDepthCounters];
     const DevScene &sc = a.sc;
     const uint32_t depth0 = a.depth;
@@ -1243,7 +1331,9 @@ __global__ void __launch_bounds__(256) k_accumulate(KArgs a) {
         acc = J.accum + (i - J.pixelOffset);
         reset = J.frameIndex == 1;
     } else {
-        const uint32_t px = i % a.fp.imageW, py = i / a.fp.imageW;
+        if (i >= a.fp.pixelCount) return;
+        const uint32_t pix = a.fp.pixelBegin + i;
+        const uint32_t px = pix % a.fp.imageW, py = pix / a.fp.imageW;
         p = py * a.fp.width + px;
         // planes of width*height floats: sum of channel k at plane 2k, compensation at 2k + 1
         plane = static_cast<size_t>(a.fp.width) * a.fp.height;
@@ -1310,7 +1400,9 @@ __global__ void __launch_bounds__(256) k_convert(KArgs a) {
         H = J.height;
         scp = &J.scene;
     } else {
-        const uint32_t px = i % a.fp.imageW, py = i / a.fp.imageW;
+        if (i >= a.fp.pixelCount) return;
+        const uint32_t pix = a.fp.pixelBegin + i;
+        const uint32_t px = pix % a.fp.imageW, py = pix / a.fp.imageW;
         const uint32_t p = py * a.fp.width + px;
         plane = static_cast<size_t>(a.fp.width) * a.fp.height;
         acc = a.fb.accum + p;
@@ -1593,7 +1685,7 @@ __device__ __forceinline__ Wls<N> commonWls(const BptParams &bp, bool single) {
 // ---- light subpaths (lvc_bpt.cu:144-334) ----
 template <int N>
 __global__ void __launch_bounds__(kTraceBlock) k_bptLight(BptArgs a) {
-    __shared__ int32_t lds[kLdsStack * kTraceBlock];
+    __shared__ int32_t lds[kTraceLds];
     const DevScene &sc = a.sc;
     const BptParams &bp = a.bp;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1816,7 +1908,7 @@ __global__ void __launch_bounds__(256) k_bptCompact(BptParams bp) {
 // ---- eye subpaths (lvc_bpt.cu:338-971) ----
 template <int N>
 __global__ void __launch_bounds__(kTraceBlock) k_bptEye(BptArgs a) {
-    __shared__ int32_t lds[kLdsStack * kTraceBlock];
+    __shared__ int32_t lds[kTraceLds];
     const DevScene &sc = a.sc;
     const BptParams &bp = a.bp;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -2212,7 +2304,7 @@ struct HitOut { int32_t inst, geomInst, prim; float t, b1, b2; };
 
 template <bool ANY>
 __global__ void __launch_bounds__(kTraceBlock) k_traceRays(DevScene sc, const RayIn *rays, uint32_t n, HitOut *hits) {
-    __shared__ int32_t lds[kLdsStack * kTraceBlock];
+    __shared__ int32_t lds[kTraceLds];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     RayIn r = rays[i];
@@ -2297,7 +2389,8 @@ static inline uint32_t blocksFor(uint32_t n, uint32_t bs) { return (n + bs - 1) 
 
 template <int N>
 static hipError_t launchFrameT(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb, hipStream_t s,
-                               LaunchTimer *timer, const CacheLaunch *cache) {
+                               LaunchTimer *timer, const CacheLaunch *cache, bool convert,
+                               hipEvent_t afterWavefront) {
     KArgs a;
     a.sc = sc;
     a.fp = fp;
@@ -2306,7 +2399,7 @@ static hipError_t launchFrameT(const DevScene &sc, const FrameParams &fp, const 
     a.jobs = nullptr;
     a.jobOffsets = nullptr;
     a.numJobs = 0;
-    const uint32_t n = fp.imageW * fp.imageH;
+    const uint32_t n = fp.pixelCount;
     auto t0 = [&](int st) { if (timer) timer->begin(st); };
     auto t1 = [&](int st) { if (timer) timer->end(st); };
     hipLaunchKernelGGL(k_frameBegin, dim3(1), dim3(256), 0, s, fb.counters, n);
@@ -2333,6 +2426,10 @@ static hipError_t launchFrameT(const DevScene &sc, const FrameParams &fp, const 
             hipLaunchKernelGGL(k_shadow<false>, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
         t1(STAGE_SHADOW);
     }
+    if (afterWavefront) {
+        hipError_t e = hipEventRecord(afterWavefront, s);
+        if (e != hipSuccess) return e;
+    }
     if (wavefrontDepths < numDepths) {
         a.depth = wavefrontDepths;
         const uint32_t blocks = std::max(1u, std::min(blocksFor(n, kTraceBlock), fp.tailBlocks));
@@ -2354,9 +2451,11 @@ static hipError_t launchFrameT(const DevScene &sc, const FrameParams &fp, const 
     t0(STAGE_ACCUMULATE);
     hipLaunchKernelGGL(k_accumulate<N>, dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
     t1(STAGE_ACCUMULATE);
-    t0(STAGE_CONVERT);
-    hipLaunchKernelGGL(k_convert<N>, dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
-    t1(STAGE_CONVERT);
+    if (convert) {
+        t0(STAGE_CONVERT);
+        hipLaunchKernelGGL(k_convert<N>, dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
+        t1(STAGE_CONVERT);
+    }
     hipLaunchKernelGGL(k_frameEnd, dim3(1), dim3(64), 0, s, fb.counters, fp.maxPathLength, wavefrontDepths);
     if (cache && fp.cacheMode != 0) {
         t0(STAGE_CACHE_TRAIN);
@@ -2381,8 +2480,30 @@ static hipError_t launchFrameT(const DevScene &sc, const FrameParams &fp, const 
 }
 
 hipError_t launchFrame(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb, bool spectral,
-                       hipStream_t s, LaunchTimer *timer, const CacheLaunch *cache) {
-    return spectral ? launchFrameT<4>(sc, fp, fb, s, timer, cache) : launchFrameT<3>(sc, fp, fb, s, timer, cache);
+                       hipStream_t s, LaunchTimer *timer, const CacheLaunch *cache, bool convert,
+                       hipEvent_t afterWavefront) {
+    return spectral ? launchFrameT<4>(sc, fp, fb, s, timer, cache, convert, afterWavefront)
+                    : launchFrameT<3>(sc, fp, fb, s, timer, cache, convert, afterWavefront);
+}
+
+hipError_t launchConvert(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb, bool spectral,
+                         hipStream_t s, LaunchTimer *timer) {
+    KArgs a;
+    a.sc = sc;
+    a.fp = fp;
+    a.fb = fb;
+    a.depth = 0;
+    a.jobs = nullptr;
+    a.jobOffsets = nullptr;
+    a.numJobs = 0;
+    const uint32_t n = fp.pixelCount;
+    if (timer) timer->begin(STAGE_CONVERT);
+    if (spectral)
+        hipLaunchKernelGGL(k_convert<4>, dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_convert<3>, dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
+    if (timer) timer->end(STAGE_CONVERT);
+    return hipGetLastError();
 }
 
 // One step of a batch: every job with samples left renders one more sample per pixel.  The

@@ -10,8 +10,9 @@ namespace vlra {
 
 // Depths rendered as separate per-depth kernels before the paths move to the tail kernel.
 // On the 1080p icosphere grid (2.07 M, 1.14 M, 0.24 M, 0.11 M, 48 k ... rays at depths 0, 1, 2,
-// 3, 4 ...) a frame takes 14.7 ms with every depth as its own kernels, 9.8 ms with 3 or 4.
-constexpr uint32_t kDefaultWavefrontDepth = 3;
+// 3, 4 ...) a frame takes 5.64 ms with only depth 0 as its own kernels, 6.1 ms with two, 6.7 ms
+// with three: the tail kernel, which refills its lanes per path, carries the rest best.
+constexpr uint32_t kDefaultWavefrontDepth = 1;
 
 struct FrameParams {
     uint32_t width, height;        // buffer size (stride = width)
@@ -26,9 +27,14 @@ struct FrameParams {
     uint32_t frameStamp;           // grows every frame: marks the cache records written this frame
     float cacheThreshold;          // c of the spread test a(x_1..x_n) > c a_0
     uint32_t trainStride, trainOffset;   // pixel i traces a training path when i % stride == offset
+    // the frame's paths are pixels pixelBegin .. pixelBegin + pixelCount - 1 of the image
+    // (row-major over imageW); a frame may be rendered as several such parts
+    uint32_t pixelBegin, pixelCount;
 };
 
-constexpr uint32_t kCacheTrainVerts = 3;   // vertices 1..3 of a training path become samples
+constexpr uint32_t kCacheTrainVerts = 3;
+// frames of at least this many pixels are rendered as two halves on two streams
+constexpr uint32_t kSplitMinPixels = 1u << 18;   // vertices 1..3 of a training path become samples
 
 // Radiance-cache records written by the path tracer (indexed by path, or by training slot =
 // training path * kCacheTrainVerts + vertex - 1); a record is live when its stamp equals the
@@ -51,7 +57,7 @@ constexpr uint32_t kDefaultTailBlocks = 2048;   // 8 blocks of 128 per CU
 // timed on real frames: first the wavefront depth (tail grid fixed), then the tail grid at
 // the best depth; each candidate gets kFramesPerCandidate frames and keeps its fastest.
 struct ScheduleTuner {
-    static constexpr uint32_t kFramesPerCandidate = 2;
+    static constexpr uint32_t kFramesPerCandidate = 3;
     struct Cand {
         uint32_t wavefrontDepth, tailBlocks;
         float bestMs;
@@ -66,7 +72,7 @@ struct ScheduleTuner {
         phase = 0;
         index = frames = 0;
         cands.clear();
-        for (uint32_t d : {1u, 2u, 3u, 4u, 5u, 6u}) cands.push_back({d, kDefaultTailBlocks, 1e30f});
+        for (uint32_t d : {1u, 2u, 3u, 4u}) cands.push_back({d, kDefaultTailBlocks, 1e30f});
     }
     const Cand &current() const { return cands[index]; }
     Cand best() const {
@@ -152,8 +158,15 @@ struct CacheLaunch {
     float4* slotTargets;
 };
 
+// Generate, trace, shade and accumulate the pixels of fp's part (and convert the whole image
+// to RGBA when `convert`).
+// `afterWavefront` (optional) is recorded once the per-depth kernels are enqueued, before the
+// tail kernel.
 hipError_t launchFrame(const DevScene& scene, const FrameParams& fp, const FrameBuffers& fb, bool spectral,
-                       hipStream_t stream, LaunchTimer* timer, const CacheLaunch* cache = nullptr);
+                       hipStream_t stream, LaunchTimer* timer, const CacheLaunch* cache = nullptr,
+                       bool convert = true, hipEvent_t afterWavefront = nullptr);
+hipError_t launchConvert(const DevScene& scene, const FrameParams& fp, const FrameBuffers& fb, bool spectral,
+                         hipStream_t stream, LaunchTimer* timer);
 
 // A batch of render jobs over one set of path buffers (paths of job j are the pixels
 // jobOffsets[j] .. jobOffsets[j + 1] - 1).  fp.imageW is the total pixel count (imageH = 1),

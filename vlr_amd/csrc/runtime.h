@@ -401,6 +401,7 @@ public:
     ScheduleTuner tuner;
     hipEvent_t tuneEvents[2] = {nullptr, nullptr};
     std::vector<hipEvent_t> eventPool;   // reused by kernel timing, 2 per timed launch
+    std::vector<hipEvent_t> eventPoolB;  // the same for the second half of a split frame
     uint32_t width = 0, height = 0;
     uint32_t numAccumFrames = 0;
     FrameParams lastFrame = {};
@@ -466,6 +467,16 @@ public:
     void renderBptFrame(hipStream_t stream, const FrameParams &fp, bool firstFrame);
     void readLightRngStates(uint64_t *dst);
     void writeLightRngStates(const uint64_t *src);
+    // frames rendered as two pixel halves on two streams (one half's deep paths overlap the
+    // other half's per-depth kernels)
+    bool splitFrames = false;
+    bool lastFrameSplit = false;
+    bool halvesPending = false;       // the halves' last frame has not been joined into a caller's stream
+    hipStream_t partStreams[2] = {nullptr, nullptr};
+    hipEvent_t partEvents[3] = {nullptr, nullptr, nullptr};
+    hipEvent_t wavefrontEvents[2] = {nullptr, nullptr};   // each half's per-depth kernels of its last frame
+    bool wavefrontRecorded[2] = {false, false};
+    DeviceBuffer counters2;           // queue counters and statistics of the second half
     // batches
     BatchState batch;
     bool batchGraphs = true;
@@ -481,6 +492,8 @@ public:
     void cacheInfer(hipStream_t stream, const float *records, uint32_t n, float *out);
     void cacheTrain(hipStream_t stream, const float *records, const float *targets, uint32_t n, uint32_t numOut);
     FrameBuffers frameBuffers();
+    void renderSplitFrame(hipStream_t stream, const FrameParams &fp, const FrameBuffers &fb, bool join);
+    void joinHalves(hipStream_t stream);
 };
 
 void throwHip(hipError_t e, const char *what);

@@ -16,6 +16,9 @@
 //                  sin(theta) (shader_nodes.cpp:1992-2010)
 #include "runtime.h"
 
+#include <cmath>
+#include <limits>
+
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
@@ -300,6 +303,60 @@ std::vector<BVH4Node> collapseToBvh4(const std::vector<BVHNode> &bin) {
     Bvh4Collapser c(bin);
     c.run();
     return std::move(c.nodes);
+}
+
+// One quantised node: per axis the grid step is the power of two 2^k with 255 * 2^k >= the
+// node extent; each child plane is rounded outwards and checked with the decoding arithmetic
+// the traversal uses (origin + float(q) * step, no fused multiply-add).
+BVH4QNode quantizeNode(const BVH4Node &f) {
+    BVH4QNode q;
+    std::memset(&q, 0, sizeof(q));
+    const float *lo[3] = {f.loX, f.loY, f.loZ}, *hi[3] = {f.hiX, f.hiY, f.hiZ};
+    for (int k = 0; k < 4; ++k) q.child[k] = f.child[k];
+    for (int a = 0; a < 3; ++a) {
+        float blo = std::numeric_limits<float>::infinity(), bhi = -blo;
+        for (int k = 0; k < 4; ++k)
+            if (f.child[k] != 0) {
+                blo = std::min(blo, lo[a][k]);
+                bhi = std::max(bhi, hi[a][k]);
+            }
+        if (!(blo <= bhi)) blo = bhi = 0.0f;   // no child
+        q.origin[a] = blo;
+        int e;
+        std::frexp(std::max((bhi - blo) / 255.0f, 1e-30f), &e);   // (ext / 255) < 2^e
+        e = std::max(e, -126);
+        for (;; ++e) {
+            const float step = std::ldexp(1.0f, e);
+            uint32_t wl = 0, wh = 0;
+            bool ok = true;
+            for (int k = 0; k < 4; ++k) {
+                if (f.child[k] == 0) continue;
+                float ql = std::floor((lo[a][k] - blo) / step), qh = std::ceil((hi[a][k] - blo) / step);
+                int il = static_cast<int>(std::min(std::max(ql, 0.0f), 255.0f));
+                int ih = static_cast<int>(std::min(std::max(qh, 0.0f), 255.0f));
+                while (il > 0 && blo + static_cast<float>(il) * step > lo[a][k]) --il;
+                while (ih < 255 && blo + static_cast<float>(ih) * step < hi[a][k]) ++ih;
+                if (blo + static_cast<float>(il) * step > lo[a][k] || blo + static_cast<float>(ih) * step < hi[a][k]) {
+                    ok = false;
+                    break;
+                }
+                wl |= static_cast<uint32_t>(il) << (8 * k);
+                wh |= static_cast<uint32_t>(ih) << (8 * k);
+            }
+            if (!ok) continue;
+            q.qlo[a] = wl;
+            q.qhi[a] = wh;
+            q.scaleExp |= static_cast<uint32_t>(e + 127) << (8 * a);
+            break;
+        }
+    }
+    return q;
+}
+
+std::vector<BVH4QNode> quantizeBvh4(const std::vector<BVH4Node> &nodes) {
+    std::vector<BVH4QNode> out(nodes.size());
+    for (size_t i = 0; i < nodes.size(); ++i) out[i] = quantizeNode(nodes[i]);
+    return out;
 }
 
 void composeTransform(const float a[16], const float b[16], float out[16]) {
@@ -683,8 +740,9 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
     }
 
     // upload
-    g->tlasNodes.upload(tlasNodes.data(), tlasNodes.size() * sizeof(BVH4Node));
-    g->blasNodes.upload(blasNodes.data(), blasNodes.size() * sizeof(BVH4Node));
+    const std::vector<BVH4QNode> tlasQ = quantizeBvh4(tlasNodes), blasQ = quantizeBvh4(blasNodes);
+    g->tlasNodes.upload(tlasQ.data(), tlasQ.size() * sizeof(BVH4QNode));
+    g->blasNodes.upload(blasQ.data(), blasQ.size() * sizeof(BVH4QNode));
     g->blasTris.upload(blasTris.data(), blasTris.size() * sizeof(TriRecord));
     g->instances.upload(insts.data(), insts.size() * sizeof(DevInstance));
     g->instGeomInsts.upload(instGeoms.data(), instGeoms.size() * sizeof(uint32_t));
@@ -698,8 +756,8 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
     g->mengGrid.upload(mengGrid().data(), mengGrid().size() * sizeof(MengCellDev));
     g->mengPoints.upload(mengPoints().data(), mengPoints().size() * sizeof(float));
 
-    D.tlasNodes = static_cast<const BVH4Node *>(g->tlasNodes.ptr);
-    D.blasNodes = static_cast<const BVH4Node *>(g->blasNodes.ptr);
+    D.tlasNodes = static_cast<const BVH4QNode *>(g->tlasNodes.ptr);
+    D.blasNodes = static_cast<const BVH4QNode *>(g->blasNodes.ptr);
     D.blasTris = static_cast<const TriRecord *>(g->blasTris.ptr);
     D.instances = static_cast<const DevInstance *>(g->instances.ptr);
     D.instGeomInsts = static_cast<const uint32_t *>(g->instGeomInsts.ptr);
