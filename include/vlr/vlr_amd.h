@@ -9,8 +9,9 @@
  *      an uninterrupted render;
  *   3. direct entry points into the hot kernels for parity tests and benchmarks: the BVH
  *      traversal (replaces optixTrace, path_tracing.cu:89-93 and
- *      light_transport_common.h:169-172) and the flattened BSDF code (replaces the BSDF
- *      direct callables, materials.cu:282-328).
+ *      light_transport_common.h:169-172), the BVH builders (replace the OptiX GAS / IAS
+ *      builds, scene.cpp:1149-1202, 1319-1329) and the flattened BSDF code (replaces the
+ *      BSDF direct callables, materials.cu:282-328).
  */
 #pragma once
 
@@ -179,6 +180,42 @@ typedef struct VLRHit { int32_t instIndex; int32_t geomInstIndex; int32_t primIn
  * (material groups) in order of first appearance. */
 VLR_API VLRResult vlrContextTraceRays(VLRContext context, hipStream_t stream, VLRScene scene, const VLRRay* rays,
                                       uint32_t numRays, VLRHit* hits, bool anyHit);
+
+/* ---- acceleration structures ----
+ * Replace the OptiX GAS / IAS builds (Scene::setup, scene.cpp:1149-1202 and 1319-1329,
+ * PreferFastTrace at scene.cpp:1049): one BLAS per distinct geometry group, a TLAS over the
+ * instances, 64-byte quantised 4-wide nodes.  The device builder (default) and the host
+ * builder produce the same trees up to node numbering. */
+#define VLR_ACCEL_BUILDER_DEVICE 0u
+#define VLR_ACCEL_BUILDER_HOST 1u
+VLR_API VLRResult vlrContextSetAccelBuilder(VLRContext context, uint32_t builder);
+VLR_API VLRResult vlrContextGetAccelBuilder(VLRContextConst context, uint32_t* builder);
+typedef struct VLRAccelStats {
+    uint32_t numBlas, numTlasInstances;
+    uint64_t numTriangles;
+    uint32_t numBlasNodes, numTlasNodes;
+    float msBlas, msTlas;        /* wall time of the last scene setup's builds */
+} VLRAccelStats;
+VLR_API VLRResult vlrContextGetAccelStats(VLRContextConst context, VLRAccelStats* stats);
+/* Copies one part of the acceleration structure of `scene`, as set up for rendering, to host
+ * memory: which = 0 TLAS nodes, 1 BLAS nodes (64 B each), 2 BLAS triangle records (48 B: v0,
+ * geometry instance, v1, primitive, v2, pad), 3 per-instance pairs (first BLAS node, first
+ * triangle record; ~0 for instances without triangles).  dst = NULL queries the size. */
+VLR_API VLRResult vlrContextReadAccel(VLRContext context, VLRScene scene, uint32_t which, void* dst, size_t capacity,
+                                      size_t* bytes);
+/* Builds a 4-wide BVH over numPrims boxes (6 floats each: lo xyz, hi xyz) with at most
+ * maxLeafPrims (1..8) primitives per leaf.  Writes *numNodes nodes (at most
+ * vlrBvh4NodeCapacity(numPrims)) and the primitive order (leaf codes index it).  With
+ * leafRemap (maxLeafPrims 1) a leaf holds ~(leafRemap[primitive] << 3) instead.
+ * vlrBuildBvh4Host works on host memory; vlrContextBuildBvh4 on device memory, on `stream`,
+ * returning when the build has finished. */
+VLR_API VLRResult vlrBvh4NodeCapacity(uint32_t numPrims, uint32_t* capacity);
+VLR_API VLRResult vlrBuildBvh4Host(const float* boxes, uint32_t numPrims, uint32_t maxLeafPrims,
+                                   const uint32_t* leafRemap, void* nodes, uint32_t nodeCapacity, uint32_t* order,
+                                   uint32_t* numNodes);
+VLR_API VLRResult vlrContextBuildBvh4(VLRContext context, hipStream_t stream, const float* boxes, uint32_t numPrims,
+                                      uint32_t maxLeafPrims, const uint32_t* leafRemap, void* nodes,
+                                      uint32_t nodeCapacity, uint32_t* order, uint32_t* numNodes);
 
 /* ---- BSDF entry point ---- */
 typedef struct VLRBSDFQuery {

@@ -56,6 +56,36 @@ class VLRShaderNodePlug(C.Structure):
     _fields_ = [("nodeRef", C.c_size_t), ("info", C.c_uint32)]
 
 
+class VLRAccelStats(C.Structure):
+    _fields_ = [("numBlas", C.c_uint32), ("numTlasInstances", C.c_uint32), ("numTriangles", C.c_uint64),
+                ("numBlasNodes", C.c_uint32), ("numTlasNodes", C.c_uint32), ("msBlas", C.c_float),
+                ("msTlas", C.c_float)]
+
+
+# 64-byte quantised 4-wide BVH node: child k's box on axis a is
+# origin[a] + byte_k(qlo[a] / qhi[a]) * 2^(byte_a(scaleExp) - 127)
+BVH4_NODE = np.dtype([("origin", "<f4", 3), ("scaleExp", "<u4"), ("qlo", "<u4", 3), ("qhi", "<u4", 3),
+                      ("child", "<i4", 4), ("pad", "<u4", 2)])
+TRI_RECORD = np.dtype([("v0", "<f4", 3), ("geomInst", "<u4"), ("v1", "<f4", 3), ("prim", "<u4"),
+                       ("v2", "<f4", 3), ("pad", "<u4")])
+ACCEL_TLAS, ACCEL_BLAS, ACCEL_TRIS, ACCEL_INSTANCES = 0, 1, 2, 3
+ACCEL_BUILDER_DEVICE, ACCEL_BUILDER_HOST = 0, 1
+
+
+def decode_bvh4(nodes: np.ndarray):
+    """Child boxes of quantised nodes: (lo, hi), each (n, 4, 3) float32, computed as the traversal
+    does (origin + float(q) * step, an exact product and one rounding)."""
+    e = ((nodes["scaleExp"][:, None] >> (8 * np.arange(3, dtype=np.uint32))) & 0xFF).astype(np.int32) - 127
+    step = np.ldexp(np.float32(1.0), e).astype(np.float32)                       # (n, 3)
+    shift = (8 * np.arange(4, dtype=np.uint32))[None, :, None]
+    qlo = ((nodes["qlo"][:, None, :] >> shift) & 0xFF).astype(np.float32)     # (n, 4, 3)
+    qhi = ((nodes["qhi"][:, None, :] >> shift) & 0xFF).astype(np.float32)
+    org = nodes["origin"][:, None, :].astype(np.float32)
+    lo = (org + (qlo * step[:, None, :]).astype(np.float32)).astype(np.float32)
+    hi = (org + (qhi * step[:, None, :]).astype(np.float32)).astype(np.float32)
+    return lo, hi
+
+
 class VLRRenderStats(C.Structure):
     _fields_ = [("numFrames", C.c_uint64), ("paths", C.c_uint64), ("closestRays", C.c_uint64),
                 ("shadowRays", C.c_uint64), ("droppedNonFinite", C.c_uint64), ("msGenerate", C.c_double),
@@ -231,6 +261,13 @@ PROTOTYPES: Dict[str, list] = {
     "vlrContextWriteAccumulation": [H, P, U32],
     "vlrContextTraceRays": [H, H, H, P, U32, P, B],
     "vlrContextEvaluateBSDFs": [H, H, H, B, P, U32, P],
+    "vlrContextSetAccelBuilder": [H, U32],
+    "vlrContextGetAccelBuilder": [H, C.POINTER(U32)],
+    "vlrContextGetAccelStats": [H, C.POINTER(VLRAccelStats)],
+    "vlrContextReadAccel": [H, H, U32, P, C.c_size_t, C.POINTER(C.c_size_t)],
+    "vlrBvh4NodeCapacity": [U32, C.POINTER(U32)],
+    "vlrBuildBvh4Host": [P, U32, U32, P, P, U32, P, C.POINTER(U32)],
+    "vlrContextBuildBvh4": [H, H, P, U32, U32, P, P, U32, P, C.POINTER(U32)],
 }
 
 _lib = None
@@ -484,6 +521,61 @@ class Context:
 
     def evaluate_bsdfs(self, material, spectral: bool, q_ptr: int, n: int, r_ptr: int, stream=None):
         call("vlrContextEvaluateBSDFs", self.h, stream, material, spectral, C.c_void_p(q_ptr), n, C.c_void_p(r_ptr))
+
+    def set_accel_builder(self, builder: int):
+        call("vlrContextSetAccelBuilder", self.h, builder)
+
+    def accel_builder(self) -> int:
+        b = U32()
+        call("vlrContextGetAccelBuilder", self.h, C.byref(b))
+        return b.value
+
+    def accel_stats(self) -> dict:
+        s = VLRAccelStats()
+        call("vlrContextGetAccelStats", self.h, C.byref(s))
+        return {f: getattr(s, f) for f, _ in VLRAccelStats._fields_}
+
+    def read_accel(self, scene, which: int) -> np.ndarray:
+        """One part of the scene's acceleration structure (ACCEL_TLAS / _BLAS / _TRIS /
+        _INSTANCES) as a numpy array."""
+        n = C.c_size_t()
+        call("vlrContextReadAccel", self.h, scene, which, None, 0, C.byref(n))
+        buf = np.zeros(n.value, np.uint8)
+        call("vlrContextReadAccel", self.h, scene, which, buf.ctypes.data, n.value, C.byref(n))
+        if which in (ACCEL_TLAS, ACCEL_BLAS):
+            return buf.view(BVH4_NODE)
+        if which == ACCEL_TRIS:
+            return buf.view(TRI_RECORD)
+        return buf.view(np.uint32).reshape(-1, 2)
+
+    def build_bvh4(self, boxes_ptr: int, n: int, max_leaf: int, remap_ptr, nodes_ptr: int, capacity: int,
+                   order_ptr: int, stream=None) -> int:
+        """Device build over device-resident boxes (6 floats each); returns the node count."""
+        cnt = U32()
+        call("vlrContextBuildBvh4", self.h, stream, C.c_void_p(boxes_ptr), n, max_leaf,
+             C.c_void_p(remap_ptr) if remap_ptr else None, C.c_void_p(nodes_ptr), capacity, C.c_void_p(order_ptr),
+             C.byref(cnt))
+        return cnt.value
+
+
+def bvh4_node_capacity(n: int) -> int:
+    c = U32()
+    call("vlrBvh4NodeCapacity", n, C.byref(c))
+    return c.value
+
+
+def build_bvh4_host(boxes: np.ndarray, max_leaf: int, remap=None):
+    """Host build over (n, 6) boxes; returns (nodes, order)."""
+    b = np.ascontiguousarray(boxes, np.float32).reshape(-1, 6)
+    n = len(b)
+    cap = bvh4_node_capacity(n)
+    nodes = np.zeros(cap, BVH4_NODE)
+    order = np.zeros(max(n, 1), np.uint32)
+    r = None if remap is None else np.ascontiguousarray(remap, np.uint32)
+    cnt = U32()
+    call("vlrBuildBvh4Host", b.ctypes.data, n, max_leaf, None if r is None else r.ctypes.data, nodes.ctypes.data, cap,
+         order.ctypes.data, C.byref(cnt))
+    return nodes[:cnt.value].copy(), order[:n].copy()
 
 
 def cache_num_parameters() -> int:

@@ -1410,6 +1410,94 @@ VLR_API VLRResult vlrContextTraceRays(VLRContext context, hipStream_t stream, VL
     VLRA_CATCH
 }
 
+VLR_API VLRResult vlrContextSetAccelBuilder(VLRContext context, uint32_t builder) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    CTX(context)->setAccelBuilder(builder);
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+
+VLR_API VLRResult vlrContextGetAccelBuilder(VLRContextConst context, uint32_t *builder) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    if (!builder) return VLRResult_InvalidArgument;
+    *builder = CTX(context)->accelBuilder;
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+
+VLR_API VLRResult vlrContextGetAccelStats(VLRContextConst context, VLRAccelStats *stats) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    if (!stats) return VLRResult_InvalidArgument;
+    const AccelStats &a = CTX(context)->accelStats;
+    stats->numBlas = a.numBlas;
+    stats->numTlasInstances = a.numTlasInstances;
+    stats->numTriangles = a.numTriangles;
+    stats->numBlasNodes = a.numBlasNodes;
+    stats->numTlasNodes = a.numTlasNodes;
+    stats->msBlas = a.msBlas;
+    stats->msTlas = a.msTlas;
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+
+VLR_API VLRResult vlrContextReadAccel(VLRContext context, VLRScene scene, uint32_t which, void *dst, size_t capacity,
+                                      size_t *bytes) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    CHECK(scene, IS(scene, Scene));
+    if (!bytes) return VLRResult_InvalidArgument;
+    CTX(context)->readAccel(*static_cast<const Scene *>(obj(scene)), which, dst, capacity, bytes);
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+
+VLR_API VLRResult vlrBvh4NodeCapacity(uint32_t numPrims, uint32_t *capacity) {
+    if (!capacity) return VLRResult_InvalidArgument;
+    *capacity = GpuBvhBuilder::nodeCapacity(numPrims);
+    return VLRResult_NoError;
+}
+
+VLR_API VLRResult vlrBuildBvh4Host(const float *boxes, uint32_t numPrims, uint32_t maxLeafPrims,
+                                   const uint32_t *leafRemap, void *nodes, uint32_t nodeCapacity, uint32_t *order,
+                                   uint32_t *numNodes) {
+    VLRA_TRY
+    if ((!boxes || !nodes || !order) && numPrims) return VLRResult_InvalidArgument;
+    if (!numNodes || nodeCapacity < GpuBvhBuilder::nodeCapacity(numPrims)) return VLRResult_InvalidArgument;
+    if (maxLeafPrims < 1 || maxLeafPrims > kMaxLeafPrims || numPrims >= kBvhMaxPrims) return VLRResult_InvalidArgument;
+    if (leafRemap && maxLeafPrims != 1) return VLRResult_InvalidArgument;
+    HostBvh4 hb;
+    hostBuildBvh4(boxes, numPrims, maxLeafPrims, leafRemap, &hb);
+    if (!hb.nodes.empty()) std::memcpy(nodes, hb.nodes.data(), hb.nodes.size() * sizeof(BVH4QNode));
+    if (!hb.order.empty()) std::memcpy(order, hb.order.data(), hb.order.size() * sizeof(uint32_t));
+    *numNodes = static_cast<uint32_t>(hb.nodes.size());
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+
+VLR_API VLRResult vlrContextBuildBvh4(VLRContext context, hipStream_t stream, const float *boxes, uint32_t numPrims,
+                                      uint32_t maxLeafPrims, const uint32_t *leafRemap, void *nodes,
+                                      uint32_t nodeCapacity, uint32_t *order, uint32_t *numNodes) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    if ((!boxes || !nodes || !order) && numPrims) return VLRResult_InvalidArgument;
+    if (!numNodes || nodeCapacity < GpuBvhBuilder::nodeCapacity(numPrims)) return VLRResult_InvalidArgument;
+    if (maxLeafPrims < 1 || maxLeafPrims > kMaxLeafPrims || numPrims >= kBvhMaxPrims) return VLRResult_InvalidArgument;
+    if (leafRemap && maxLeafPrims != 1) return VLRResult_InvalidArgument;
+    Context *ctx = CTX(context);
+    ctx->ensureDevice();
+    GpuBvhBuilder &B = ctx->bvhBuilder();
+    *numNodes = B.build(boxes, numPrims, maxLeafPrims, leafRemap, static_cast<BVH4QNode *>(nodes), stream);
+    if (numPrims) {
+        VLRA_HIP(hipMemcpyAsync(order, B.order(), size_t(numPrims) * 4, hipMemcpyDeviceToDevice, stream));
+        VLRA_HIP(hipStreamSynchronize(stream));
+    }
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+
 VLR_API VLRResult vlrContextEvaluateBSDFs(VLRContext context, hipStream_t stream, VLRSurfaceMaterialConst material,
                                           bool spectral, const VLRBSDFQuery *queries, uint32_t numQueries,
                                           VLRBSDFResult *results) {

@@ -799,6 +799,47 @@ void Context::traceRays(hipStream_t stream, const Scene &s, const void *rays, ui
     VLRA_HIP(launchTraceRays(gpu.dev, rays, n, hits, anyHit, stream));
 }
 
+void Context::setAccelBuilder(uint32_t builder) {
+    if (builder != kAccelBuilderDevice && builder != kAccelBuilderHost) throw ApiError{3, "unknown builder"};
+    if (builder != accelBuilder) {
+        accelBuilder = builder;
+        touch();
+    }
+}
+
+void Context::readAccel(const Scene &s, uint32_t which, void *dst, size_t capacity, size_t *bytes) {
+    if (which > 3) throw ApiError{3, "unknown acceleration-structure part"};
+    ensureScene(s);
+    VLRA_HIP(hipDeviceSynchronize());
+    size_t n = 0;
+    const void *src = nullptr;
+    std::vector<uint32_t> pairs;
+    if (which == 0) {
+        n = size_t(gpu.numTlasNodes) * sizeof(BVH4QNode);
+        src = gpu.tlasNodes.ptr;
+    } else if (which == 1) {
+        n = size_t(gpu.numBlasNodes) * sizeof(BVH4QNode);
+        src = gpu.blasNodes.ptr;
+    } else if (which == 2) {
+        n = size_t(gpu.numTriangles) * sizeof(TriRecord);
+        src = gpu.blasTris.ptr;
+    } else {
+        std::vector<DevInstance> insts(gpu.numInstances);
+        if (!insts.empty())
+            VLRA_HIP(hipMemcpy(insts.data(), gpu.instances.ptr, insts.size() * sizeof(DevInstance), hipMemcpyDeviceToHost));
+        for (const DevInstance &I : insts) {
+            pairs.push_back(I.hasGeometry ? I.blasNodeBase : ~0u);
+            pairs.push_back(I.hasGeometry ? I.blasTriBase : ~0u);
+        }
+        n = pairs.size() * sizeof(uint32_t);
+    }
+    *bytes = n;
+    if (!dst) return;
+    if (capacity < n) throw ApiError{3, "destination too small"};
+    if (which == 3) std::memcpy(dst, pairs.data(), n);
+    else if (n) VLRA_HIP(hipMemcpy(dst, src, n, hipMemcpyDeviceToHost));
+}
+
 void Context::evaluateBSDFs(hipStream_t stream, const SurfaceMaterial &m, bool spectral, const void *q, uint32_t n,
                             void *r) {
     if (!m.supportedOnRenderPath()) throw ApiError{4, "material type not on the render path"};

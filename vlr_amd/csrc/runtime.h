@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "dev_types.h"
+#include "bvh.h"
 #include "launch.h"
 
 #include "../../include/vlr/public_types.h"
@@ -358,6 +359,16 @@ struct SceneGPU {
     const Camera *camera = nullptr;
     uint32_t numInstances = 0, numGeomInsts = 0;
     uint64_t numTriangles = 0;
+    uint32_t numBlasNodes = 0, numTlasNodes = 0;
+};
+
+// Acceleration-structure builds of the last scene setup.
+constexpr uint32_t kAccelBuilderDevice = 0, kAccelBuilderHost = 1;
+struct AccelStats {
+    uint32_t numBlas = 0, numTlasInstances = 0;
+    uint64_t numTriangles = 0;
+    uint32_t numBlasNodes = 0, numTlasNodes = 0;
+    float msBlas = 0.0f, msTlas = 0.0f;
 };
 
 void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *out);
@@ -385,6 +396,13 @@ class Context : public Object {
 public:
     int device = -1;
     uint64_t version = 1;     // bumped on every scene-affecting change
+    uint32_t accelBuilder = kAccelBuilderDevice;
+    AccelStats accelStats;
+    std::unique_ptr<GpuBvhBuilder> gpuBvh;
+    GpuBvhBuilder &bvhBuilder() {
+        if (!gpuBvh) gpuBvh.reset(new GpuBvhBuilder());
+        return *gpuBvh;
+    }
     std::set<Object *> objects;
     Scene *scene = nullptr;
     uint32_t renderer = 0;
@@ -451,6 +469,10 @@ public:
     void ensureScene(const Scene &s);
     void resizeAccum();
     void traceRays(hipStream_t stream, const Scene &s, const void *rays, uint32_t n, void *hits, bool anyHit);
+    // acceleration structure of `s` as set up for rendering: 0 TLAS nodes, 1 BLAS nodes,
+    // 2 BLAS triangle records, 3 per-instance (first BLAS node, first triangle record)
+    void readAccel(const Scene &s, uint32_t which, void *dst, size_t capacity, size_t *bytes);
+    void setAccelBuilder(uint32_t builder);
     void evaluateBSDFs(hipStream_t stream, const SurfaceMaterial &m, bool spectral, const void *q, uint32_t n, void *r);
     // trav: nodes, triangles, instances of closest-hit then of shadow traversals
     // wavefront: extension and shadow rays traced by the per-depth kernels

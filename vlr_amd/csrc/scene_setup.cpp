@@ -15,6 +15,9 @@
 //                  image.cpp:542-565), importance map = 4x box-shrunk luminance times
 //                  sin(theta) (shader_nodes.cpp:1992-2010)
 #include "runtime.h"
+#include "bvh.h"
+
+#include <chrono>
 
 #include <cmath>
 #include <limits>
@@ -73,291 +76,6 @@ struct Box {
     }
     float centroid(int a) const { return 0.5f * (lo[a] + hi[a]); }
 };
-
-// Binned-SAH BVH2 builder writing the 64-byte node layout.
-class BvhBuilder {
-public:
-    BvhBuilder(const std::vector<Box> &boxes, uint32_t maxLeaf) : boxes_(boxes), maxLeaf_(maxLeaf) {
-        order.resize(boxes.size());
-        for (size_t i = 0; i < boxes.size(); ++i) order[i] = static_cast<uint32_t>(i);
-        cent_.resize(boxes.size() * 3);
-        for (size_t i = 0; i < boxes.size(); ++i)
-            for (int a = 0; a < 3; ++a) cent_[3 * i + a] = boxes[i].centroid(a);
-    }
-    std::vector<BVHNode> nodes;
-    std::vector<uint32_t> order;
-    uint32_t maxDepth = 0;
-
-    void build() {
-        nodes.clear();
-        nodes.reserve(boxes_.size() + 1);
-        nodes.push_back(BVHNode());
-        uint32_t n = static_cast<uint32_t>(boxes_.size());
-        if (n <= maxLeaf_) {
-            // root with a single leaf child; child index 0 (the root) marks an empty slot
-            Box b;
-            for (uint32_t i = 0; i < n; ++i) b.grow(boxes_[i]);
-            setChild(0, 0, b, n ? leafCode(0, n) : 0);
-            setChild(0, 1, Box(), 0);
-            return;
-        }
-        splitInto(0, 0, n, 1);
-    }
-
-private:
-    const std::vector<Box> &boxes_;
-    uint32_t maxLeaf_;
-    std::vector<float> cent_;
-
-    static int32_t leafCode(uint32_t first, uint32_t count) {
-        return static_cast<int32_t>(~((first << kLeafShift) | (count - 1)));
-    }
-    void setChild(uint32_t node, int slot, const Box &b, int32_t code) {
-        BVHNode &N = nodes[node];
-        float *lo = slot == 0 ? N.lo0 : N.lo1;
-        float *hi = slot == 0 ? N.hi0 : N.hi1;
-        for (int a = 0; a < 3; ++a) {
-            lo[a] = b.empty() ? 0.0f : b.lo[a];
-            hi[a] = b.empty() ? 0.0f : b.hi[a];
-        }
-        N.child[slot] = code;
-        N.pad[0] = N.pad[1] = 0;
-    }
-    Box rangeBox(uint32_t b, uint32_t e) const {
-        Box r;
-        for (uint32_t i = b; i < e; ++i) r.grow(boxes_[order[i]]);
-        return r;
-    }
-    uint32_t partition(uint32_t b, uint32_t e, uint32_t depth) {
-        const uint32_t n = e - b;
-        float clo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, chi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
-        for (uint32_t i = b; i < e; ++i)
-            for (int a = 0; a < 3; ++a) {
-                clo[a] = std::min(clo[a], cent_[3 * order[i] + a]);
-                chi[a] = std::max(chi[a], cent_[3 * order[i] + a]);
-            }
-        constexpr int NB = 16;
-        float best = FLT_MAX;
-        int bestAxis = -1, bestSplit = -1;
-        if (depth < 40) {
-            for (int a = 0; a < 3; ++a) {
-                float ext = chi[a] - clo[a];
-                if (!(ext > 0.0f)) continue;
-                Box bb[NB];
-                uint32_t bc[NB] = {};
-                for (uint32_t i = b; i < e; ++i) {
-                    int k = std::min(NB - 1, static_cast<int>(NB * (cent_[3 * order[i] + a] - clo[a]) / ext));
-                    bb[k].grow(boxes_[order[i]]);
-                    ++bc[k];
-                }
-                Box left[NB];
-                uint32_t nl[NB];
-                Box acc;
-                uint32_t cnt = 0;
-                for (int k = 0; k < NB; ++k) {
-                    if (bc[k]) acc.grow(bb[k]);
-                    cnt += bc[k];
-                    left[k] = acc;
-                    nl[k] = cnt;
-                }
-                Box racc;
-                uint32_t rc = 0;
-                for (int k = NB - 1; k >= 1; --k) {
-                    if (bc[k]) racc.grow(bb[k]);
-                    rc += bc[k];
-                    uint32_t lcount = nl[k - 1];
-                    if (!lcount || !rc) continue;
-                    float c = left[k - 1].area() * lcount + racc.area() * rc;
-                    if (c < best) {
-                        best = c;
-                        bestAxis = a;
-                        bestSplit = k;
-                    }
-                }
-            }
-        }
-        uint32_t mid = b + n / 2;
-        if (bestAxis >= 0) {
-            float ext = chi[bestAxis] - clo[bestAxis];
-            auto it = std::partition(order.begin() + b, order.begin() + e, [&](uint32_t p) {
-                int k = std::min(NB - 1, static_cast<int>(NB * (cent_[3 * p + bestAxis] - clo[bestAxis]) / ext));
-                return k < bestSplit;
-            });
-            mid = static_cast<uint32_t>(it - order.begin());
-        }
-        if (mid == b || mid == e) {
-            // median split on the widest centroid axis
-            int a = 0;
-            for (int k = 1; k < 3; ++k)
-                if (chi[k] - clo[k] > chi[a] - clo[a]) a = k;
-            mid = b + n / 2;
-            std::nth_element(order.begin() + b, order.begin() + mid, order.begin() + e,
-                             [&](uint32_t x, uint32_t y) { return cent_[3 * x + a] < cent_[3 * y + a]; });
-        }
-        return mid;
-    }
-    // node `idx` gets children for [b, m) and [m, e)
-    void splitInto(uint32_t idx, uint32_t b, uint32_t e, uint32_t depth) {
-        maxDepth = std::max(maxDepth, depth);
-        uint32_t m = partition(b, e, depth);
-        uint32_t ranges[2][2] = {{b, m}, {m, e}};
-        for (int s = 0; s < 2; ++s) {
-            uint32_t rb = ranges[s][0], re = ranges[s][1];
-            Box box = rangeBox(rb, re);
-            if (re - rb <= maxLeaf_) {
-                setChild(idx, s, box, leafCode(rb, re - rb));
-            } else {
-                uint32_t child = static_cast<uint32_t>(nodes.size());
-                nodes.push_back(BVHNode());
-                setChild(idx, s, box, static_cast<int32_t>(child));
-                splitInto(child, rb, re, depth + 1);
-            }
-        }
-    }
-};
-
-// Collapses a BVH2 into 4-wide nodes: starting from a node's two children, the inner child
-// with the largest box is replaced by its own two children until four slots are filled or
-// only leaves remain.  Leaf codes and child order carry over.
-class Bvh4Collapser {
-public:
-    explicit Bvh4Collapser(const std::vector<BVHNode> &bin) : bin_(bin) {}
-    std::vector<BVH4Node> nodes;
-    void run() {
-        nodes.clear();
-        if (bin_.empty()) return;
-        emit(0);
-    }
-
-private:
-    struct Slot {
-        Box box;
-        int32_t code;
-    };
-    const std::vector<BVHNode> &bin_;
-    Slot slotOf(uint32_t n, int s) const {
-        const BVHNode &N = bin_[n];
-        Slot r;
-        const float *lo = s == 0 ? N.lo0 : N.lo1, *hi = s == 0 ? N.hi0 : N.hi1;
-        for (int a = 0; a < 3; ++a) {
-            r.box.lo[a] = lo[a];
-            r.box.hi[a] = hi[a];
-        }
-        r.code = N.child[s];
-        return r;
-    }
-    uint32_t emit(uint32_t binNode) {
-        uint32_t idx = static_cast<uint32_t>(nodes.size());
-        nodes.push_back(BVH4Node());
-        Slot slots[4];
-        int n = 0;
-        for (int s = 0; s < 2; ++s) {
-            Slot c = slotOf(binNode, s);
-            if (c.code != 0) slots[n++] = c;
-        }
-        while (n < 4) {
-            int best = -1;
-            float bestArea = -1.0f;
-            for (int k = 0; k < n; ++k)
-                if (slots[k].code > 0 && slots[k].box.area() > bestArea) {
-                    best = k;
-                    bestArea = slots[k].box.area();
-                }
-            if (best < 0) break;
-            uint32_t inner = static_cast<uint32_t>(slots[best].code);
-            Slot a = slotOf(inner, 0), b = slotOf(inner, 1);
-            // keep the order: the replaced slot's children take its place
-            Slot tmp[4];
-            int m = 0;
-            for (int k = 0; k < n; ++k) {
-                if (k == best) {
-                    if (a.code != 0) tmp[m++] = a;
-                    if (b.code != 0) tmp[m++] = b;
-                } else {
-                    tmp[m++] = slots[k];
-                }
-            }
-            for (int k = 0; k < m; ++k) slots[k] = tmp[k];
-            n = m;
-        }
-        int32_t codes[4] = {0, 0, 0, 0};
-        for (int k = 0; k < n; ++k) codes[k] = slots[k].code > 0 ? static_cast<int32_t>(emit(static_cast<uint32_t>(slots[k].code)))
-                                                                  : slots[k].code;
-        BVH4Node &N = nodes[idx];
-        for (int k = 0; k < 4; ++k) {
-            const bool used = k < n;
-            N.loX[k] = used ? slots[k].box.lo[0] : 0.0f;
-            N.loY[k] = used ? slots[k].box.lo[1] : 0.0f;
-            N.loZ[k] = used ? slots[k].box.lo[2] : 0.0f;
-            N.hiX[k] = used ? slots[k].box.hi[0] : 0.0f;
-            N.hiY[k] = used ? slots[k].box.hi[1] : 0.0f;
-            N.hiZ[k] = used ? slots[k].box.hi[2] : 0.0f;
-            N.child[k] = codes[k];
-            N.pad[k] = 0;
-        }
-        return idx;
-    }
-};
-
-std::vector<BVH4Node> collapseToBvh4(const std::vector<BVHNode> &bin) {
-    Bvh4Collapser c(bin);
-    c.run();
-    return std::move(c.nodes);
-}
-
-// One quantised node: per axis the grid step is the power of two 2^k with 255 * 2^k >= the
-// node extent; each child plane is rounded outwards and checked with the decoding arithmetic
-// the traversal uses (origin + float(q) * step, no fused multiply-add).
-BVH4QNode quantizeNode(const BVH4Node &f) {
-    BVH4QNode q;
-    std::memset(&q, 0, sizeof(q));
-    const float *lo[3] = {f.loX, f.loY, f.loZ}, *hi[3] = {f.hiX, f.hiY, f.hiZ};
-    for (int k = 0; k < 4; ++k) q.child[k] = f.child[k];
-    for (int a = 0; a < 3; ++a) {
-        float blo = std::numeric_limits<float>::infinity(), bhi = -blo;
-        for (int k = 0; k < 4; ++k)
-            if (f.child[k] != 0) {
-                blo = std::min(blo, lo[a][k]);
-                bhi = std::max(bhi, hi[a][k]);
-            }
-        if (!(blo <= bhi)) blo = bhi = 0.0f;   // no child
-        q.origin[a] = blo;
-        int e;
-        std::frexp(std::max((bhi - blo) / 255.0f, 1e-30f), &e);   // (ext / 255) < 2^e
-        e = std::max(e, -126);
-        for (;; ++e) {
-            const float step = std::ldexp(1.0f, e);
-            uint32_t wl = 0, wh = 0;
-            bool ok = true;
-            for (int k = 0; k < 4; ++k) {
-                if (f.child[k] == 0) continue;
-                float ql = std::floor((lo[a][k] - blo) / step), qh = std::ceil((hi[a][k] - blo) / step);
-                int il = static_cast<int>(std::min(std::max(ql, 0.0f), 255.0f));
-                int ih = static_cast<int>(std::min(std::max(qh, 0.0f), 255.0f));
-                while (il > 0 && blo + static_cast<float>(il) * step > lo[a][k]) --il;
-                while (ih < 255 && blo + static_cast<float>(ih) * step < hi[a][k]) ++ih;
-                if (blo + static_cast<float>(il) * step > lo[a][k] || blo + static_cast<float>(ih) * step < hi[a][k]) {
-                    ok = false;
-                    break;
-                }
-                wl |= static_cast<uint32_t>(il) << (8 * k);
-                wh |= static_cast<uint32_t>(ih) << (8 * k);
-            }
-            if (!ok) continue;
-            q.qlo[a] = wl;
-            q.qhi[a] = wh;
-            q.scaleExp |= static_cast<uint32_t>(e + 127) << (8 * a);
-            break;
-        }
-    }
-    return q;
-}
-
-std::vector<BVH4QNode> quantizeBvh4(const std::vector<BVH4Node> &nodes) {
-    std::vector<BVH4QNode> out(nodes.size());
-    for (size_t i = 0; i < nodes.size(); ++i) out[i] = quantizeNode(nodes[i]);
-    return out;
-}
 
 void composeTransform(const float a[16], const float b[16], float out[16]) {
     // column-major 4x4 product out = a * b
@@ -531,9 +249,14 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
     // instances + BLAS per distinct surface-node list
     std::vector<DevInstance> insts;
     std::vector<uint32_t> instGeoms;
-    std::vector<BVH4Node> blasNodes;
-    std::vector<TriRecord> blasTris;
-    std::map<std::vector<const SurfaceNode *>, std::pair<uint32_t, uint32_t>> blasCache;
+    // distinct geometry groups: their triangle records, in the order of first use
+    struct BlasInput {
+        uint32_t triBase, numTris, nodeBase;
+    };
+    std::vector<BlasInput> blasList;
+    std::vector<TriRecord> blasRecs;
+    std::vector<uint32_t> instBlas;   // per instance: index into blasList or ~0
+    std::map<std::vector<const SurfaceNode *>, uint32_t> blasCache;
     std::vector<Box> instBoxes;
     std::vector<uint32_t> tlasInst;
     Box sceneBox;
@@ -573,11 +296,15 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
         }
         lightImp.push_back(sumImp > 0.0f ? 1.0f : 0.0f);
         uint32_t instIdx = static_cast<uint32_t>(insts.size());
-        if (!triGeoms.empty()) {
+        uint32_t blasIdx = ~0u;
+        uint64_t instTris = 0;   // material groups without triangles make no BLAS
+        for (auto &tg : triGeoms) instTris += geoms[tg.first].numPrims;
+        if (instTris > 0) {
             auto cacheIt = blasCache.find(fi.surfaces);
             if (cacheIt == blasCache.end()) {
-                std::vector<Box> boxes;
-                std::vector<TriRecord> recs;
+                BlasInput in;
+                in.triBase = static_cast<uint32_t>(blasRecs.size());
+                in.nodeBase = 0;
                 for (auto &tg : triGeoms) {
                     const DevGeomInst &G = geoms[tg.first];
                     for (uint32_t t = 0; t < G.numPrims; ++t) {
@@ -591,28 +318,15 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
                         std::memcpy(r.v2, c.p, 12);
                         r.geomInst = tg.first;
                         r.prim = t;
-                        Box bb;
-                        bb.grow(a.p);
-                        bb.grow(b.p);
-                        bb.grow(c.p);
-                        boxes.push_back(bb);
-                        recs.push_back(r);
+                        blasRecs.push_back(r);
                     }
                 }
-                BvhBuilder bld(boxes, 4);
-                bld.build();
-                uint32_t nodeBase = static_cast<uint32_t>(blasNodes.size());
-                uint32_t triBase = static_cast<uint32_t>(blasTris.size());
-                std::vector<BVH4Node> wide = collapseToBvh4(bld.nodes);
-                blasNodes.insert(blasNodes.end(), wide.begin(), wide.end());
-                for (uint32_t o : bld.order) blasTris.push_back(recs[o]);
-                numTris += recs.size();
-                cacheIt = blasCache.emplace(fi.surfaces, std::make_pair(nodeBase, triBase)).first;
-            } else {
-                // shared geometry group: same BLAS
+                in.numTris = static_cast<uint32_t>(blasRecs.size()) - in.triBase;
+                numTris += in.numTris;
+                cacheIt = blasCache.emplace(fi.surfaces, static_cast<uint32_t>(blasList.size())).first;
+                blasList.push_back(in);
             }
-            I.blasNodeBase = cacheIt->second.first;
-            I.blasTriBase = cacheIt->second.second;
+            blasIdx = cacheIt->second;
             I.hasGeometry = 1;
             // world box of the instance (8 transformed corners)
             Box wb;
@@ -629,23 +343,7 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
             sceneBox.grow(wb);
         }
         insts.push_back(I);
-    }
-
-    // TLAS (one instance per leaf); leaf code carries the instance index
-    std::vector<BVH4Node> tlasNodes;
-    if (!instBoxes.empty()) {
-        BvhBuilder bld(instBoxes, 1);
-        bld.build();
-        std::vector<BVHNode> bin = bld.nodes;
-        for (BVHNode &n : bin) {
-            for (int s = 0; s < 2; ++s) {
-                if (n.child[s] < 0) {
-                    uint32_t first = (~static_cast<uint32_t>(n.child[s])) >> kLeafShift;
-                    n.child[s] = static_cast<int32_t>(~(tlasInst[bld.order[first]] << kLeafShift));
-                }
-            }
-        }
-        tlasNodes = collapseToBvh4(bin);
+        instBlas.push_back(blasIdx);
     }
 
     // environment
@@ -739,11 +437,98 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
         D.worldDiscArea = 3.14159265358979323846f * (D.worldRadius * D.worldRadius);
     }
 
+    // acceleration structures: one BLAS per distinct geometry group (leaves index its triangle
+    // records, child nodes count from its first node), the TLAS over the instance boxes with the
+    // instance index in each leaf (scene.cpp:1319-1329 rebuilds the GASes and the IAS)
+    AccelStats &as = ctx.accelStats;
+    as = AccelStats();
+    as.numBlas = static_cast<uint32_t>(blasList.size());
+    as.numTlasInstances = static_cast<uint32_t>(instBoxes.size());
+    as.numTriangles = numTris;
+    std::vector<float> tlasBoxes(instBoxes.size() * 6);
+    for (size_t i = 0; i < instBoxes.size(); ++i)
+        for (int a = 0; a < 3; ++a) {
+            tlasBoxes[6 * i + a] = instBoxes[i].lo[a];
+            tlasBoxes[6 * i + 3 + a] = instBoxes[i].hi[a];
+        }
+    const auto tStart = std::chrono::steady_clock::now();
+    auto msSince = [](std::chrono::steady_clock::time_point t) {
+        return std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t).count();
+    };
+    uint32_t blasNodeCount = 0, tlasNodeCount = 0;
+    if (ctx.accelBuilder == kAccelBuilderDevice) {
+        GpuBvhBuilder &B = ctx.bvhBuilder();
+        size_t cap = 0;
+        for (const BlasInput &b : blasList) cap += GpuBvhBuilder::nodeCapacity(b.numTris);
+        DeviceBuffer recsDev;
+        recsDev.upload(blasRecs.data(), blasRecs.size() * sizeof(TriRecord));
+        g->blasNodes.reserve(std::max<size_t>(cap, 1) * sizeof(BVH4QNode));
+        g->blasTris.reserve(std::max<size_t>(blasRecs.size(), 1) * sizeof(TriRecord));
+        const auto *recs = static_cast<const TriRecord *>(recsDev.ptr);
+        auto *nodes = static_cast<BVH4QNode *>(g->blasNodes.ptr);
+        auto *tris = static_cast<TriRecord *>(g->blasTris.ptr);
+        std::vector<uint32_t> segFirst, segCount;
+        for (BlasInput &b : blasList) {
+            b.nodeBase = static_cast<uint32_t>(segFirst.size());   // segment k's root is node k
+            segFirst.push_back(b.triBase);
+            segCount.push_back(b.numTris);
+        }
+        blasNodeCount = B.buildTriangles(recs, segFirst, segCount, 4, nodes, nullptr);
+        gatherTriangles(recs, B.order(), static_cast<uint32_t>(blasRecs.size()), tris, nullptr);
+        VLRA_HIP(hipStreamSynchronize(nullptr));
+        as.msBlas = msSince(tStart);
+        const auto tTlas = std::chrono::steady_clock::now();
+        if (!instBoxes.empty()) {
+            const uint32_t n = static_cast<uint32_t>(instBoxes.size());
+            DeviceBuffer boxDev, remapDev;
+            boxDev.upload(tlasBoxes.data(), tlasBoxes.size() * sizeof(float));
+            remapDev.upload(tlasInst.data(), tlasInst.size() * sizeof(uint32_t));
+            g->tlasNodes.reserve(GpuBvhBuilder::nodeCapacity(n) * sizeof(BVH4QNode));
+            tlasNodeCount = B.build(static_cast<const float *>(boxDev.ptr), n, 1,
+                                    static_cast<const uint32_t *>(remapDev.ptr),
+                                    static_cast<BVH4QNode *>(g->tlasNodes.ptr), nullptr);
+        }
+        as.msTlas = msSince(tTlas);
+    } else {
+        std::vector<BVH4QNode> blasQ;
+        std::vector<TriRecord> ordered(blasRecs.size());
+        std::vector<float> boxes;
+        HostBvh4 hb;
+        for (BlasInput &b : blasList) {
+            boxes.resize(size_t(b.numTris) * 6);
+            for (uint32_t t = 0; t < b.numTris; ++t) {
+                const TriRecord &r = blasRecs[b.triBase + t];
+                for (int a = 0; a < 3; ++a) {
+                    boxes[6 * size_t(t) + a] = std::min(std::min(r.v0[a], r.v1[a]), r.v2[a]);
+                    boxes[6 * size_t(t) + 3 + a] = std::max(std::max(r.v0[a], r.v1[a]), r.v2[a]);
+                }
+            }
+            hostBuildBvh4(boxes.data(), b.numTris, 4, nullptr, &hb);
+            b.nodeBase = static_cast<uint32_t>(blasQ.size());
+            blasQ.insert(blasQ.end(), hb.nodes.begin(), hb.nodes.end());
+            for (uint32_t t = 0; t < b.numTris; ++t) ordered[b.triBase + t] = blasRecs[b.triBase + hb.order[t]];
+        }
+        blasNodeCount = static_cast<uint32_t>(blasQ.size());
+        g->blasNodes.upload(blasQ.data(), blasQ.size() * sizeof(BVH4QNode));
+        g->blasTris.upload(ordered.data(), ordered.size() * sizeof(TriRecord));
+        as.msBlas = msSince(tStart);
+        const auto tTlas = std::chrono::steady_clock::now();
+        hostBuildBvh4(tlasBoxes.data(), static_cast<uint32_t>(instBoxes.size()), 1, tlasInst.data(), &hb);
+        tlasNodeCount = static_cast<uint32_t>(hb.nodes.size());
+        g->tlasNodes.upload(hb.nodes.data(), hb.nodes.size() * sizeof(BVH4QNode));
+        as.msTlas = msSince(tTlas);
+    }
+    as.numBlasNodes = blasNodeCount;
+    as.numTlasNodes = tlasNodeCount;
+    g->numBlasNodes = blasNodeCount;
+    g->numTlasNodes = tlasNodeCount;
+    for (size_t i = 0; i < insts.size(); ++i) {
+        if (instBlas[i] == ~0u) continue;
+        insts[i].blasNodeBase = blasList[instBlas[i]].nodeBase;
+        insts[i].blasTriBase = blasList[instBlas[i]].triBase;
+    }
+
     // upload
-    const std::vector<BVH4QNode> tlasQ = quantizeBvh4(tlasNodes), blasQ = quantizeBvh4(blasNodes);
-    g->tlasNodes.upload(tlasQ.data(), tlasQ.size() * sizeof(BVH4QNode));
-    g->blasNodes.upload(blasQ.data(), blasQ.size() * sizeof(BVH4QNode));
-    g->blasTris.upload(blasTris.data(), blasTris.size() * sizeof(TriRecord));
     g->instances.upload(insts.data(), insts.size() * sizeof(DevInstance));
     g->instGeomInsts.upload(instGeoms.data(), instGeoms.size() * sizeof(uint32_t));
     g->geomInsts.upload(geoms.data(), geoms.size() * sizeof(DevGeomInst));
@@ -771,7 +556,7 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
     D.mengGrid = static_cast<const MengCellDev *>(g->mengGrid.ptr);
     D.mengPoints = static_cast<const float *>(g->mengPoints.ptr);
     D.numInstances = static_cast<uint32_t>(insts.size());
-    D.hasTlas = tlasNodes.empty() ? 0 : 1;
+    D.hasTlas = instBoxes.empty() ? 0 : 1;
     cmfBins(D.cmfX, D.cmfY, D.cmfZ, &D.integralCMF);
     g->spectral = spectral;
     g->valid = true;
