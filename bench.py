@@ -429,6 +429,55 @@ def batch_experiment(num_jobs, seed=20261202):
     }
 
 
+def accel_experiment(seed=20261202):
+    """Acceleration-structure builds inside scene setup, device builder against host builder:
+    the 32 x 32 icosphere grid (1025 geometry groups, 1,310,722 triangles) and tessellated
+    Cornell boxes of about 10k, 100k and 1M unique triangles (the range of the batch workload).
+    Per case: a first setup, then a warm rebuild of every BLAS (vertices of one mesh set again),
+    then a transform-only update (one instance moved: TLAS rebuilt, BLASes kept).  Times are
+    host wall clock of the builds (round trips included) and of the whole scene setup."""
+    cases = [("icosphere grid 32x32", lambda: scenes.icosphere_grid(seed))]
+    for tris in (10_000, 100_000, 1_000_000):
+        cases.append((f"tessellated Cornell ~{tris // 1000}k", lambda t=tris: scenes.tessellated_cornell(seed, t)))
+    rows = []
+    for name, make in cases:
+        desc = make()
+        row = {"scene": name}
+        for builder, key in ((api.ACCEL_BUILDER_DEVICE, "device"), (api.ACCEL_BUILDER_HOST, "host")):
+            ctx = api.Context(spectral=desc.spectral)
+            ctx.set_accel_builder(builder)
+            bs = api.build_scene(ctx, desc)
+            t0 = time.perf_counter()
+            ctx.read_accel(bs.scene, api.ACCEL_INSTANCES)
+            first = time.perf_counter() - t0
+            st_first = ctx.accel_stats()
+            mesh = next(i for i, m in enumerate(desc.meshes) if not m.points)
+            v = np.ascontiguousarray(desc.meshes[mesh].vertices, np.float32)
+            api.call("vlrTriangleMeshSurfaceNodeSetVertices", bs.meshes[mesh], v.ctypes.data, len(v))
+            t0 = time.perf_counter()
+            ctx.read_accel(bs.scene, api.ACCEL_INSTANCES)
+            rebuild = time.perf_counter() - t0
+            st = ctx.accel_stats()
+            xf = np.array(desc.instances[-1].transform, np.float32).reshape(4, 4, order="F").copy()
+            xf[:3, 3] += 0.25
+            api.call("vlrParentNodeSetTransform", bs.instances[-1], ctx.transform(xf.reshape(16, order="F")))
+            t0 = time.perf_counter()
+            ctx.read_accel(bs.scene, api.ACCEL_INSTANCES)
+            update = time.perf_counter() - t0
+            su = ctx.accel_stats()
+            ctx.close()
+            row["triangles"] = st["numTriangles"]
+            row["blas_count"] = st["numBlas"]
+            row[key] = {"first_setup_s": first, "first_ms_blas": st_first["msBlas"],
+                        "rebuild_ms_blas": st["msBlas"], "rebuild_ms_tlas": st["msTlas"], "rebuild_setup_s": rebuild,
+                        "update_blas_reused": bool(su["blasReused"]), "update_ms_tlas": su["msTlas"],
+                        "update_setup_s": update}
+        row["blas_speedup"] = row["host"]["rebuild_ms_blas"] / max(1e-6, row["device"]["rebuild_ms_blas"])
+        rows.append(row)
+        print(json.dumps({"accel_case": row}), flush=True)
+    return rows
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -460,6 +509,8 @@ def main():
                     help="only run the LVC-BPT vs path tracing experiment (1280x720) and print its JSON line")
     ap.add_argument("--bpt-spp", type=int, default=32)
     ap.add_argument("--bpt-ref-spp", type=int, default=2048)
+    ap.add_argument("--accel-experiment", action="store_true",
+                    help="device against host acceleration-structure builds")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path shards by pixel "
@@ -467,6 +518,10 @@ def main():
         sys.exit(2)
     if not os.path.exists(api.LIB_PATH):
         raise RuntimeError(f"{api.LIB_PATH} is missing: build it with vlr_amd/build.py")
+    if args.accel_experiment:
+        res = accel_experiment(args.seed)
+        print(json.dumps({"accel": res}), flush=True)
+        return
     if args.batch_experiment:
         res = batch_experiment(args.batch_jobs)
         print(json.dumps({"batch": res}), flush=True)
@@ -492,6 +547,7 @@ def main():
         ctx.render(built.camera, first_frame=(w == 0), limit=args.warmup)
     hip_sync()
     setup_s = time.perf_counter() - t_setup
+    accel = ctx.accel_stats()
 
     ctx.reset_stats()
     hip_sync()
@@ -582,6 +638,8 @@ def main():
                      "algorithmic_bytes_per_launch": per_launch_bytes, "mean_launch_ms": mean_launch_s * 1e3},
         "image_mean_rgb": [float(x) for x in img[..., :3].reshape(-1, 3).mean(axis=0)],
         "setup_s": setup_s,
+        "accel_build": {"builder": "device", "ms_blas": accel["msBlas"], "ms_tlas": accel["msTlas"],
+                        "blas_count": accel["numBlas"], "triangles": accel["numTriangles"]},
         "rmse": None,
     }
     if not args.no_cpu:

@@ -195,6 +195,7 @@ typedef struct VLRAccelStats {
     uint64_t numTriangles;
     uint32_t numBlasNodes, numTlasNodes;
     float msBlas, msTlas;        /* wall time of the last scene setup's builds */
+    uint32_t blasReused;         /* 1: no geometry changed since the previous setup, BLASes kept */
 } VLRAccelStats;
 VLR_API VLRResult vlrContextGetAccelStats(VLRContextConst context, VLRAccelStats* stats);
 /* Copies one part of the acceleration structure of `scene`, as set up for rendering, to host

@@ -132,3 +132,43 @@ def test_render_identical_with_either_builder(torch_gpu):
     assert np.array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
     assert np.array_equal(out[0][1], out[1][1])
     assert out[0][2] == out[1][2]
+
+
+def _render(ctx, bs, W, H, frames):
+    for f in range(frames):
+        ctx.render(bs.camera, f == 0, frames)
+    return ctx.read_output()
+
+
+@pytest.mark.parametrize("builder", [api.ACCEL_BUILDER_DEVICE, api.ACCEL_BUILDER_HOST])
+def test_transform_change_keeps_blases(torch_gpu, builder):
+    """Moving an instance rebuilds only the TLAS (the BLASes are kept) and renders exactly what a
+    fresh setup of the moved scene renders; changing vertices rebuilds the BLASes."""
+    import copy
+    desc = scenes.icosphere_grid(grid=6, env_size=(64, 32))
+    moved = copy.deepcopy(desc)
+    xf = np.array(moved.instances[7].transform, np.float32).reshape(4, 4, order="F").copy()
+    xf[:3, 3] += np.array([0.4, 0.9, -0.3], np.float32)
+    moved.instances[7].transform = xf.reshape(16, order="F")
+    W, H = 96, 54
+    with api.Context(max_path_length=6) as ctx:
+        ctx.set_accel_builder(builder)
+        bs = api.build_scene(ctx, desc)
+        ctx.bind_output(W, H)
+        ctx.read_accel(bs.scene, api.ACCEL_TRIS)                  # first setup
+        assert ctx.accel_stats()["blasReused"] == 0
+        api.call("vlrParentNodeSetTransform", bs.instances[7], ctx.transform(moved.instances[7].transform))
+        img = _render(ctx, bs, W, H, 3)
+        st = ctx.accel_stats()
+        assert st["blasReused"] == 1, st
+        # new vertices of one mesh: BLASes are built again
+        v = np.ascontiguousarray(desc.meshes[0].vertices, np.float32)
+        api.call("vlrTriangleMeshSurfaceNodeSetVertices", bs.meshes[0], v.ctypes.data, len(v))
+        ctx.read_accel(bs.scene, api.ACCEL_TRIS)
+        assert ctx.accel_stats()["blasReused"] == 0
+    with api.Context(max_path_length=6) as ctx:
+        ctx.set_accel_builder(builder)
+        bs = api.build_scene(ctx, moved)
+        ctx.bind_output(W, H)
+        ref = _render(ctx, bs, W, H, 3)
+    assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))

@@ -743,7 +743,7 @@ VLR_API VLRResult vlrTriangleMeshSurfaceNodeDestroy(VLRContext context, VLRTrian
 static VLRResult setVertices(SurfaceNode *n, const VLRVertex *vertices, uint32_t numVertices) {
     if (!vertices && numVertices) return VLRResult_InvalidArgument;
     n->vertices.assign(vertices, vertices + numVertices);
-    ++n->version;
+    n->version = nextGeometryVersion();
     n->ctx->touch();
     return VLRResult_NoError;
 }
@@ -790,7 +790,7 @@ VLR_API VLRResult vlrTriangleMeshSurfaceNodeAddMaterialGroup(VLRTriangleMeshSurf
         return VLRResult_IncompatibleNodeType;
     }
     n->groups.push_back(std::move(g));
-    ++n->version;
+    n->version = nextGeometryVersion();
     n->ctx->touch();
     return VLRResult_NoError;
     VLRA_CATCH
@@ -834,7 +834,7 @@ VLR_API VLRResult vlrPointSurfaceNodeAddMaterialGroup(VLRPointSurfaceNode surfac
     g.indices.assign(indices, indices + numIndices);
     g.material = static_cast<const SurfaceMaterial *>(obj(material));
     n->groups.push_back(std::move(g));
-    ++n->version;
+    n->version = nextGeometryVersion();
     n->ctx->touch();
     return VLRResult_NoError;
     VLRA_CATCH
@@ -1439,6 +1439,7 @@ VLR_API VLRResult vlrContextGetAccelStats(VLRContextConst context, VLRAccelStats
     stats->numTlasNodes = a.numTlasNodes;
     stats->msBlas = a.msBlas;
     stats->msTlas = a.msTlas;
+    stats->blasReused = a.blasReused;
     return VLRResult_NoError;
     VLRA_CATCH
 }

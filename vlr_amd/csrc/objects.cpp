@@ -6,12 +6,19 @@
 //   enum strings   libVLR/queryable.cpp:34-80
 #include "runtime.h"
 
+#include <atomic>
+
 #include <algorithm>
 #include <cctype>
 #include <cmath>
 #include <cstring>
 
 namespace vlra {
+
+uint64_t nextGeometryVersion() {
+    static std::atomic<uint64_t> counter{0};
+    return ++counter;
+}
 
 bool paramNameEquals(const std::string &a, const std::string &b) {
     if (a.size() != b.size()) return false;

@@ -277,11 +277,15 @@ struct MaterialGroup {
     PlugRef nodeNormal, nodeTangent, nodeAlpha;
 };
 
+// Process-wide counter: a surface node takes a new value at creation and on every geometry
+// change, so (node, version) never repeats even when a node's address is reused.
+uint64_t nextGeometryVersion();
+
 class SurfaceNode : public Node {
 public:
     std::vector<VLRVertex> vertices;
     std::vector<MaterialGroup> groups;
-    uint64_t version = 0;
+    uint64_t version = nextGeometryVersion();
     using Node::Node;
 };
 
@@ -360,6 +364,11 @@ struct SceneGPU {
     uint32_t numInstances = 0, numGeomInsts = 0;
     uint64_t numTriangles = 0;
     uint32_t numBlasNodes = 0, numTlasNodes = 0;
+    // BLAS inputs of the last build (per geometry group: node, version, group, geometry
+    // instance, triangle count) and their layout; an equal key reuses the BLASes
+    std::vector<uint64_t> blasKey;
+    std::vector<uint32_t> blasLayout;   // triBase, numTris, nodeBase per BLAS
+    uint32_t blasBuilder = ~0u;
 };
 
 // Acceleration-structure builds of the last scene setup.
@@ -369,6 +378,7 @@ struct AccelStats {
     uint64_t numTriangles = 0;
     uint32_t numBlasNodes = 0, numTlasNodes = 0;
     float msBlas = 0.0f, msTlas = 0.0f;
+    uint32_t blasReused = 0;
 };
 
 void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *out);

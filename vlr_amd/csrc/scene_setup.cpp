@@ -247,6 +247,10 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
     };
 
     // instances + BLAS per distinct surface-node list
+    struct TriGeom {
+        uint32_t geomInst, group;
+        const SurfaceNode *node;
+    };
     std::vector<DevInstance> insts;
     std::vector<uint32_t> instGeoms;
     // distinct geometry groups: their triangle records, in the order of first use
@@ -254,7 +258,7 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
         uint32_t triBase, numTris, nodeBase;
     };
     std::vector<BlasInput> blasList;
-    std::vector<TriRecord> blasRecs;
+    std::vector<std::vector<TriGeom>> blasGroups;   // per BLAS: its geometry groups
     std::vector<uint32_t> instBlas;   // per instance: index into blasList or ~0
     std::map<std::vector<const SurfaceNode *>, uint32_t> blasCache;
     std::vector<Box> instBoxes;
@@ -271,7 +275,7 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
         affineInverse(fi.xform, I.w2o);
         I.geomInstListBase = static_cast<uint32_t>(instGeoms.size());
         std::vector<float> imp;
-        std::vector<std::pair<uint32_t, const SurfaceNode *>> triGeoms;   // (global geomInst, node)
+        std::vector<TriGeom> triGeoms;
         Box child;
         for (const SurfaceNode *s : fi.surfaces) {
             for (uint32_t gi = 0; gi < s->groups.size(); ++gi) {
@@ -279,7 +283,7 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
                 instGeoms.push_back(g);
                 imp.push_back(geoms[g].importance);
                 if (!geoms[g].isPoints) {
-                    triGeoms.emplace_back(g, s);
+                    triGeoms.push_back({g, gi, s});
                     for (uint32_t k : s->groups[gi].indices) {
                         const VLRVertex &v = s->vertices[k];
                         child.grow(&v.position.x);
@@ -298,33 +302,18 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
         uint32_t instIdx = static_cast<uint32_t>(insts.size());
         uint32_t blasIdx = ~0u;
         uint64_t instTris = 0;   // material groups without triangles make no BLAS
-        for (auto &tg : triGeoms) instTris += geoms[tg.first].numPrims;
+        for (auto &tg : triGeoms) instTris += geoms[tg.geomInst].numPrims;
         if (instTris > 0) {
             auto cacheIt = blasCache.find(fi.surfaces);
             if (cacheIt == blasCache.end()) {
                 BlasInput in;
-                in.triBase = static_cast<uint32_t>(blasRecs.size());
+                in.triBase = static_cast<uint32_t>(numTris);
+                in.numTris = static_cast<uint32_t>(instTris);
                 in.nodeBase = 0;
-                for (auto &tg : triGeoms) {
-                    const DevGeomInst &G = geoms[tg.first];
-                    for (uint32_t t = 0; t < G.numPrims; ++t) {
-                        TriRecord r;
-                        std::memset(&r, 0, sizeof(r));
-                        const DevVertex &a = verts[G.vertexBase + indices[G.indexBase + 3 * t]];
-                        const DevVertex &b = verts[G.vertexBase + indices[G.indexBase + 3 * t + 1]];
-                        const DevVertex &c = verts[G.vertexBase + indices[G.indexBase + 3 * t + 2]];
-                        std::memcpy(r.v0, a.p, 12);
-                        std::memcpy(r.v1, b.p, 12);
-                        std::memcpy(r.v2, c.p, 12);
-                        r.geomInst = tg.first;
-                        r.prim = t;
-                        blasRecs.push_back(r);
-                    }
-                }
-                in.numTris = static_cast<uint32_t>(blasRecs.size()) - in.triBase;
-                numTris += in.numTris;
+                numTris += instTris;
                 cacheIt = blasCache.emplace(fi.surfaces, static_cast<uint32_t>(blasList.size())).first;
                 blasList.push_back(in);
+                blasGroups.push_back(triGeoms);
             }
             blasIdx = cacheIt->second;
             I.hasGeometry = 1;
@@ -456,40 +445,72 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
         return std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t).count();
     };
     uint32_t blasNodeCount = 0, tlasNodeCount = 0;
-    if (ctx.accelBuilder == kAccelBuilderDevice) {
+    // BLASes are kept while no geometry group changed (Scene::setup rebuilds only the dirty
+    // GASes, scene.cpp:1319-1323)
+    std::vector<uint64_t> blasKey;
+    for (const auto &groups : blasGroups) {
+        blasKey.push_back(groups.size());
+        for (const TriGeom &tg : groups) {
+            blasKey.push_back(reinterpret_cast<uintptr_t>(tg.node));
+            blasKey.push_back(tg.node->version);
+            blasKey.push_back((uint64_t(tg.group) << 32) | tg.geomInst);
+            blasKey.push_back(geoms[tg.geomInst].numPrims);
+        }
+    }
+    const bool reuseBlas = g->valid && g->blasBuilder == ctx.accelBuilder && g->blasKey == blasKey &&
+                           g->blasLayout.size() == 3 * blasList.size();
+    std::vector<TriRecord> blasRecs;
+    if (!reuseBlas) {
+        g->blasKey.clear();   // set again once the new BLASes are complete
+        blasRecs.reserve(numTris);
+        for (const auto &groups : blasGroups)
+            for (const TriGeom &tg : groups) {
+                const DevGeomInst &G = geoms[tg.geomInst];
+                for (uint32_t t = 0; t < G.numPrims; ++t) {
+                    TriRecord r;
+                    std::memset(&r, 0, sizeof(r));
+                    const DevVertex &a = verts[G.vertexBase + indices[G.indexBase + 3 * t]];
+                    const DevVertex &b = verts[G.vertexBase + indices[G.indexBase + 3 * t + 1]];
+                    const DevVertex &c = verts[G.vertexBase + indices[G.indexBase + 3 * t + 2]];
+                    std::memcpy(r.v0, a.p, 12);
+                    std::memcpy(r.v1, b.p, 12);
+                    std::memcpy(r.v2, c.p, 12);
+                    r.geomInst = tg.geomInst;
+                    r.prim = t;
+                    blasRecs.push_back(r);
+                }
+            }
+    }
+    if (reuseBlas) {
+        for (size_t k = 0; k < blasList.size(); ++k) blasList[k].nodeBase = g->blasLayout[3 * k + 2];
+        blasNodeCount = g->numBlasNodes;
+        as.blasReused = 1;
+        as.msBlas = msSince(tStart);
+    }
+    const bool onDevice = ctx.accelBuilder == kAccelBuilderDevice;
+    if (!reuseBlas && onDevice) {
+        // all BLASes in one build: segment k (BLAS k) has its root at node k
         GpuBvhBuilder &B = ctx.bvhBuilder();
         size_t cap = 0;
-        for (const BlasInput &b : blasList) cap += GpuBvhBuilder::nodeCapacity(b.numTris);
+        std::vector<uint32_t> segFirst, segCount;
+        for (BlasInput &b : blasList) {
+            cap += GpuBvhBuilder::nodeCapacity(b.numTris);
+            b.nodeBase = static_cast<uint32_t>(segFirst.size());
+            segFirst.push_back(b.triBase);
+            segCount.push_back(b.numTris);
+        }
         DeviceBuffer recsDev;
         recsDev.upload(blasRecs.data(), blasRecs.size() * sizeof(TriRecord));
         g->blasNodes.reserve(std::max<size_t>(cap, 1) * sizeof(BVH4QNode));
         g->blasTris.reserve(std::max<size_t>(blasRecs.size(), 1) * sizeof(TriRecord));
         const auto *recs = static_cast<const TriRecord *>(recsDev.ptr);
-        auto *nodes = static_cast<BVH4QNode *>(g->blasNodes.ptr);
-        auto *tris = static_cast<TriRecord *>(g->blasTris.ptr);
-        std::vector<uint32_t> segFirst, segCount;
-        for (BlasInput &b : blasList) {
-            b.nodeBase = static_cast<uint32_t>(segFirst.size());   // segment k's root is node k
-            segFirst.push_back(b.triBase);
-            segCount.push_back(b.numTris);
-        }
-        blasNodeCount = B.buildTriangles(recs, segFirst, segCount, 4, nodes, nullptr);
-        gatherTriangles(recs, B.order(), static_cast<uint32_t>(blasRecs.size()), tris, nullptr);
+        blasNodeCount = B.buildTriangles(recs, segFirst, segCount, 4, static_cast<BVH4QNode *>(g->blasNodes.ptr), nullptr);
+        gatherTriangles(recs, B.order(), static_cast<uint32_t>(blasRecs.size()),
+                        static_cast<TriRecord *>(g->blasTris.ptr), nullptr);
         VLRA_HIP(hipStreamSynchronize(nullptr));
         as.msBlas = msSince(tStart);
-        const auto tTlas = std::chrono::steady_clock::now();
-        if (!instBoxes.empty()) {
-            const uint32_t n = static_cast<uint32_t>(instBoxes.size());
-            DeviceBuffer boxDev, remapDev;
-            boxDev.upload(tlasBoxes.data(), tlasBoxes.size() * sizeof(float));
-            remapDev.upload(tlasInst.data(), tlasInst.size() * sizeof(uint32_t));
-            g->tlasNodes.reserve(GpuBvhBuilder::nodeCapacity(n) * sizeof(BVH4QNode));
-            tlasNodeCount = B.build(static_cast<const float *>(boxDev.ptr), n, 1,
-                                    static_cast<const uint32_t *>(remapDev.ptr),
-                                    static_cast<BVH4QNode *>(g->tlasNodes.ptr), nullptr);
-        }
-        as.msTlas = msSince(tTlas);
-    } else {
+    } else if (!reuseBlas) {
+        // one host build per BLAS, stored one after the other
         std::vector<BVH4QNode> blasQ;
         std::vector<TriRecord> ordered(blasRecs.size());
         std::vector<float> boxes;
@@ -512,11 +533,34 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
         g->blasNodes.upload(blasQ.data(), blasQ.size() * sizeof(BVH4QNode));
         g->blasTris.upload(ordered.data(), ordered.size() * sizeof(TriRecord));
         as.msBlas = msSince(tStart);
-        const auto tTlas = std::chrono::steady_clock::now();
-        hostBuildBvh4(tlasBoxes.data(), static_cast<uint32_t>(instBoxes.size()), 1, tlasInst.data(), &hb);
+    }
+    // the TLAS is rebuilt on every setup (scene.cpp:1325-1327)
+    const auto tTlas = std::chrono::steady_clock::now();
+    const uint32_t numTlasPrims = static_cast<uint32_t>(instBoxes.size());
+    if (onDevice) {
+        if (numTlasPrims) {
+            DeviceBuffer boxDev, remapDev;
+            boxDev.upload(tlasBoxes.data(), tlasBoxes.size() * sizeof(float));
+            remapDev.upload(tlasInst.data(), tlasInst.size() * sizeof(uint32_t));
+            g->tlasNodes.reserve(GpuBvhBuilder::nodeCapacity(numTlasPrims) * sizeof(BVH4QNode));
+            tlasNodeCount = ctx.bvhBuilder().build(static_cast<const float *>(boxDev.ptr), numTlasPrims, 1,
+                                                   static_cast<const uint32_t *>(remapDev.ptr),
+                                                   static_cast<BVH4QNode *>(g->tlasNodes.ptr), nullptr);
+        }
+    } else {
+        HostBvh4 hb;
+        hostBuildBvh4(tlasBoxes.data(), numTlasPrims, 1, tlasInst.data(), &hb);
         tlasNodeCount = static_cast<uint32_t>(hb.nodes.size());
         g->tlasNodes.upload(hb.nodes.data(), hb.nodes.size() * sizeof(BVH4QNode));
-        as.msTlas = msSince(tTlas);
+    }
+    as.msTlas = msSince(tTlas);
+    g->blasKey = std::move(blasKey);
+    g->blasBuilder = ctx.accelBuilder;
+    g->blasLayout.clear();
+    for (const BlasInput &b : blasList) {
+        g->blasLayout.push_back(b.triBase);
+        g->blasLayout.push_back(b.numTris);
+        g->blasLayout.push_back(b.nodeBase);
     }
     as.numBlasNodes = blasNodeCount;
     as.numTlasNodes = tlasNodeCount;
