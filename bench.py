@@ -559,6 +559,7 @@ def main():
     st = ctx.stats()
     qc, qs = ctx.queue_counts(args.depth)
     sched_wd, sched_tb, sched_tuned = ctx.schedule()
+    sched_sort = ctx.shade_sorting()
     img = ctx.read_output()
 
     # per-kernel device time: more frames with HIP events around every launch (kernel timing
@@ -628,7 +629,8 @@ def main():
         "stage_ms_per_frame": {k: kt["ms" + k.capitalize()] / args.timing_frames
                                for k in ("generate", "intersect", "shade", "shadow", "tail", "accumulate", "convert")},
         "timing_frames": args.timing_frames,
-        "schedule": {"wavefront_depth": sched_wd, "tail_blocks": sched_tb, "autotuned": sched_tuned},
+        "schedule": {"wavefront_depth": sched_wd, "tail_blocks": sched_tb, "material_sort": sched_sort,
+                     "autotuned": sched_tuned},
         "rays_by_stage": {"per_depth_closest": wf_c, "per_depth_shadow": wf_s, "tail_closest": tail_c,
                           "tail_shadow": tail_s},
         "traversal_per_ray": {"closest": per_closest, "shadow": per_shadow},

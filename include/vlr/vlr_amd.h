@@ -74,6 +74,12 @@ VLR_API VLRResult vlrContextSetScheduleAutotuning(VLRContext context, bool enabl
  * running its own frames, so that the deep paths of one half overlap the per-depth kernels of
  * the other (off by default; never with the radiance cache).  The image does not depend on it. */
 VLR_API VLRResult vlrContextSetSplitFrames(VLRContext context, bool enable);
+/* Material sort of the per-depth shade queues: each depth's hits are shaded in order of
+ * material type (misses first), so a wave runs one material's code at a time.  The image does
+ * not change.  Setting it fixes the schedule (turns autotuning off); the autotuner otherwise
+ * times sorting on and off.  Get returns what the last frame used. */
+VLR_API VLRResult vlrContextSetShadeSorting(VLRContext context, bool enable);
+VLR_API VLRResult vlrContextGetShadeSorting(VLRContextConst context, bool* enabled);
 /* The schedule of the last rendered frame and whether tuning has finished. */
 VLR_API VLRResult vlrContextGetSchedule(VLRContextConst context, uint32_t* wavefrontDepth, uint32_t* tailBlocks,
                                         bool* tuned);

@@ -14,10 +14,12 @@ from vlr_amd import api, scenes
 pytestmark = pytest.mark.gpu
 
 
-def render_gpu(desc, W, H, frames, max_len, spectral, wavefront_depth=None, want_accum=False, split=True):
+def render_gpu(desc, W, H, frames, max_len, spectral, wavefront_depth=None, want_accum=False, split=True, sort=None):
     with api.Context(spectral=spectral, max_path_length=max_len) as ctx:
         if wavefront_depth is not None:
             ctx.set_wavefront_depth(wavefront_depth)
+        if sort is not None:
+            ctx.set_shade_sorting(sort)
         ctx.set_split_frames(split)
         bs = api.build_scene(ctx, desc)
         ctx.bind_output(W, H)
@@ -134,10 +136,32 @@ def test_split_frames_do_not_change_the_image(torch_gpu):
     assert a[1]["closestRays"] == b[1]["closestRays"] and a[1]["shadowRays"] == b[1]["shadowRays"]
 
 
+@pytest.mark.parametrize("scene,spectral", [("cornell", False), ("grid", True), ("many_lights", False)])
+def test_material_sort_does_not_change_the_image(torch_gpu, scene, spectral):
+    """Shading each depth's hits in order of material type gives the bits of shading them in
+    queue order: every depth as its own kernels (sorted at every depth), and the default split
+    between per-depth kernels and the tail kernel."""
+    desc, W, H, depth = {"cornell": (scenes.cornell_box(), 48, 48, 8),
+                         "grid": (scenes.icosphere_grid(grid=6, env_size=(128, 64)), 96, 54, 16),
+                         "many_lights": (scenes.cornell_many_lights(), 64, 48, 8)}[scene]
+    for wd in (64, 1):
+        runs = []
+        for sort in (False, True):
+            imgs, st, rng, acc = render_gpu(desc, W, H, 3, depth, spectral, wavefront_depth=wd, want_accum=True,
+                                            sort=sort)
+            runs.append((imgs[-1], st, rng, acc))
+        (a, sa, ra, aa), (b, sb, rb, ab) = runs
+        print(scene, "wavefront depth", wd, "rays", sa["closestRays"], sa["shadowRays"])
+        assert np.array_equal(ra, rb), wd
+        assert np.array_equal(aa.view(np.uint32), ab.view(np.uint32)), wd
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), wd
+        assert sa["closestRays"] == sb["closestRays"] and sa["shadowRays"] == sb["shadowRays"], wd
+
+
 def test_autotuned_schedule_matches_fixed_schedule(torch_gpu):
     """The schedule autotuner times candidate schedules on real frames; the frames it renders
     while tuning are the same bits as a fixed schedule's."""
-    desc, W, H, frames = scenes.cornell_box(), 40, 40, 24
+    desc, W, H, frames = scenes.cornell_box(), 40, 40, 36
     with api.Context(spectral=False, max_path_length=8) as ctx:
         bs = api.build_scene(ctx, desc)
         ctx.bind_output(W, H)

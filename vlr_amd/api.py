@@ -239,6 +239,8 @@ PROTOTYPES: Dict[str, list] = {
     "vlrContextWriteLightRNGStates": [H, P],
     "vlrContextSetNumLightPaths": [H, U32],
     "vlrContextSetSplitFrames": [H, B],
+    "vlrContextSetShadeSorting": [H, B],
+    "vlrContextGetShadeSorting": [H, C.POINTER(B)],
     "vlrContextRenderBatch": [H, H, P, U32, P],
     "vlrContextReadBatchOutput": [H, U32, P],
     "vlrContextSetBatchGraphs": [H, B],
@@ -503,6 +505,14 @@ class Context:
 
     def set_split_frames(self, on: bool):
         call("vlrContextSetSplitFrames", self.h, on)
+
+    def set_shade_sorting(self, on: bool):
+        call("vlrContextSetShadeSorting", self.h, on)
+
+    def shade_sorting(self) -> bool:
+        b = B()
+        call("vlrContextGetShadeSorting", self.h, C.byref(b))
+        return bool(b.value)
 
     def set_autotune(self, on: bool):
         call("vlrContextSetScheduleAutotuning", self.h, on)

@@ -1093,6 +1093,22 @@ VLR_API VLRResult vlrContextSetWavefrontDepth(VLRContext context, uint32_t depth
     return VLRResult_NoError;
     VLRA_CATCH
 }
+VLR_API VLRResult vlrContextSetShadeSorting(VLRContext context, bool enable) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    CTX(context)->shadeSort = enable;
+    CTX(context)->autotune = false;
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+VLR_API VLRResult vlrContextGetShadeSorting(VLRContextConst context, bool *enabled) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    if (!enabled) return VLRResult_InvalidArgument;
+    *enabled = CTX(context)->lastFrame.sortShade != 0;
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
 VLR_API VLRResult vlrContextGetWavefrontDepth(VLRContextConst context, uint32_t *depth) {
     VLRA_TRY
     CHECK(context, IS(context, Context));

@@ -159,6 +159,7 @@ void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCo
         fp.traversalStats = traversalStats ? 1 : 0;
         fp.wavefrontDepth = wavefrontDepth;
         fp.tailBlocks = tailBlocks;
+        fp.sortShade = shadeSort ? 1 : 0;
         fp.cacheMode = 0;
         fp.frameStamp = 0;
         fp.cacheThreshold = 0.0f;
@@ -183,9 +184,11 @@ void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCo
                 tuning = true;
                 fp.wavefrontDepth = tuner.current().wavefrontDepth;
                 fp.tailBlocks = tuner.current().tailBlocks;
+                fp.sortShade = tuner.current().sortShade;
             } else {
                 fp.wavefrontDepth = tuner.best().wavefrontDepth;
                 fp.tailBlocks = tuner.best().tailBlocks;
+                fp.sortShade = tuner.best().sortShade;
             }
         }
         if (tuning) {
@@ -194,6 +197,13 @@ void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCo
                 VLRA_HIP(hipEventCreate(&tuneEvents[1]));
             }
             VLRA_HIP(hipEventRecord(tuneEvents[0], stream));
+        }
+        if (splitFrames && !cacheEnabled && fp.pixelCount >= kSplitMinPixels) fp.sortShade = 0;   // halves share one sort buffer
+        if (fp.sortShade) {
+            const size_t n = size_t(width) * height;
+            sortKey.reserve(n * sizeof(uint32_t));
+            sortPerm.reserve(n * sizeof(uint32_t));
+            sortCounts.reserve(2 * kSortClasses * kMaxDepthCounters * sizeof(uint32_t));
         }
         FrameBuffers fb = frameBuffers();
         CacheLaunch cl;
@@ -341,6 +351,9 @@ FrameBuffers Context::frameBuffers() {
     fb.shadow.dir = static_cast<float4 *>(shDir.ptr);
     fb.shadow.value = static_cast<float4 *>(shVal.ptr);
     fb.counters = static_cast<uint32_t *>(counters.ptr);
+    fb.sortKey = static_cast<uint32_t *>(sortKey.ptr);
+    fb.sortPerm = static_cast<uint32_t *>(sortPerm.ptr);
+    fb.sortCounts = static_cast<uint32_t *>(sortCounts.ptr);
     fb.accum = static_cast<float *>(accum.ptr);
     fb.output = static_cast<float4 *>(output.ptr);
     fb.cache = CacheBuffers{};
@@ -705,6 +718,7 @@ void Context::renderBatch(hipStream_t userStream, const std::vector<BatchJobDesc
     L.fb.shadow.dir = static_cast<float4 *>(b.shDir.ptr);
     L.fb.shadow.value = static_cast<float4 *>(b.shVal.ptr);
     L.fb.counters = static_cast<uint32_t *>(counters.ptr);
+    L.fb.sortKey = L.fb.sortPerm = L.fb.sortCounts = nullptr;   // no material sort in batches
     L.jobs = static_cast<DevJob *>(b.jobs.ptr);
     L.jobOffsets = static_cast<const uint32_t *>(b.jobOffsets.ptr);
     L.numJobs = static_cast<uint32_t>(jobs.size());

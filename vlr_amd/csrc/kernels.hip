@@ -660,9 +660,11 @@ __global__ void k_batchStepBegin(DevJob *jobs, uint32_t numJobs) {
     }
 }
 
-__global__ void k_frameBegin(uint32_t *counters, uint32_t numPaths) {
+__global__ void k_frameBegin(uint32_t *counters, uint32_t numPaths, uint32_t *sortCounts) {
     uint32_t i = threadIdx.x;
     for (uint32_t k = i; k < 2 * kMaxDepthCounters; k += blockDim.x) counters[k] = 0;
+    if (sortCounts)
+        for (uint32_t k = i; k < 2 * kSortClasses * kMaxDepthCounters; k += blockDim.x) sortCounts[k] = 0;
     if (i == 0) {
         counters[0] = numPaths;
         counters[kTailWork] = 0;
@@ -769,6 +771,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_intersect(KArgs a) {
     const RayQueue q = a.fb.rays[a.depth & 1];
     const uint32_t count = a.fb.counters[2 * a.depth];
     TravCount tc;
+    uint32_t cls = kSortClasses;   // shading class for the material sort; kSortClasses = none
     if (i < count) {
         float4 o = q.org[i], d = q.dir[i];
         TraceHit h;
@@ -785,8 +788,52 @@ __global__ void __launch_bounds__(kTraceBlock) k_intersect(KArgs a) {
         } else {
             a.fb.hits.a[i] = make_int4(-1, -1, -1, 0);
         }
+        if (!MULTI && a.fp.sortShade) {
+            cls = found ? min(1u + a.sc.materials[a.sc.geomInsts[h.geomInst].material].type, kSortClasses - 1) : 0u;
+            a.fb.sortKey[i] = cls;
+        }
+    }
+    if (!MULTI && a.fp.sortShade) {
+        // class histogram of this depth's queue, one atomic per class present in the wave
+        uint32_t *counts = a.fb.sortCounts + 2 * kSortClasses * a.depth;
+        unsigned long long left = __ballot(cls < kSortClasses);
+        const int lane = threadIdx.x & 63;
+        while (left) {
+            const int leader = __ffsll(static_cast<long long>(left)) - 1;
+            const uint32_t c = __shfl(cls, leader);
+            const unsigned long long m = __ballot(cls == c);
+            if (lane == leader) atomicAdd(&counts[c], static_cast<uint32_t>(__popcll(m)));
+            left &= ~m;
+        }
     }
     if constexpr (STATS) addTravStats(a.fb.counters, kStatTravClosest, tc);
+}
+
+// Material sort: the queue entries of one depth in order of shading class (within a class,
+// in order of arrival), so that the waves of k_shade run one material's code at a time.
+__global__ void __launch_bounds__(256) k_sortScatter(KArgs a) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t count = a.fb.counters[2 * a.depth];
+    uint32_t *counts = a.fb.sortCounts + 2 * kSortClasses * a.depth;
+    const uint32_t cls = i < count ? a.fb.sortKey[i] : kSortClasses;
+    uint32_t base = 0;
+    if (cls < kSortClasses)
+        for (uint32_t c = 0; c < cls; ++c) base += counts[c];
+    const int lane = threadIdx.x & 63;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    unsigned long long left = __ballot(cls < kSortClasses);
+    uint32_t slot = 0;
+    while (left) {
+        const int leader = __ffsll(static_cast<long long>(left)) - 1;
+        const uint32_t c = __shfl(cls, leader);
+        const unsigned long long m = __ballot(cls == c);
+        uint32_t first = 0;
+        if (lane == leader) first = atomicAdd(&counts[kSortClasses + c], static_cast<uint32_t>(__popcll(m)));
+        first = __shfl(first, leader);
+        if (cls == c) slot = first + static_cast<uint32_t>(__popcll(m & below));
+        left &= ~m;
+    }
+    if (cls < kSortClasses && base + slot < count) a.fb.sortPerm[base + slot] = i;
 }
 
 // ---- shade: path_tracing.cu:133-259 (hit) and :266-337 (miss) ----
@@ -1070,10 +1117,11 @@ __global__ void __launch_bounds__(256) k_shade(KArgs a) {
     float4 nextOrg, nextDir, shOrg, shDir, shVal;
     if (i < count) {
         const RayQueue q = a.fb.rays[depth & 1];
-        float4 o4 = q.org[i], d4 = q.dir[i];
+        const uint32_t j = (!MULTI && a.fp.sortShade) ? a.fb.sortPerm[i] : i;   // queue entry
+        float4 o4 = q.org[j], d4 = q.dir[j];
         const uint32_t path = __float_as_uint(o4.w);
-        int4 ha = a.fb.hits.a[i];
-        float2 hb = ha.x >= 0 ? a.fb.hits.b[i] : make_float2(0.0f, 0.0f);
+        int4 ha = a.fb.hits.a[j];
+        float2 hb = ha.x >= 0 ? a.fb.hits.b[j] : make_float2(0.0f, 0.0f);
         PathRegs<N> P;
         loadPath<N>(a.fb, path, &P);
         if constexpr (MULTI) {
@@ -2402,7 +2450,7 @@ static hipError_t launchFrameT(const DevScene &sc, const FrameParams &fp, const 
     const uint32_t n = fp.pixelCount;
     auto t0 = [&](int st) { if (timer) timer->begin(st); };
     auto t1 = [&](int st) { if (timer) timer->end(st); };
-    hipLaunchKernelGGL(k_frameBegin, dim3(1), dim3(256), 0, s, fb.counters, n);
+    hipLaunchKernelGGL(k_frameBegin, dim3(1), dim3(256), 0, s, fb.counters, n, fp.sortShade ? fb.sortCounts : nullptr);
     t0(STAGE_GENERATE);
     hipLaunchKernelGGL(k_generate<N>, dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
     t1(STAGE_GENERATE);
@@ -2417,6 +2465,7 @@ static hipError_t launchFrameT(const DevScene &sc, const FrameParams &fp, const 
             hipLaunchKernelGGL(k_intersect<false>, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
         t1(STAGE_INTERSECT);
         t0(STAGE_SHADE);
+        if (fp.sortShade) hipLaunchKernelGGL(k_sortScatter, dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
         hipLaunchKernelGGL(k_shade<N>, dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
         t1(STAGE_SHADE);
         t0(STAGE_SHADOW);
@@ -2521,7 +2570,7 @@ static hipError_t launchBatchStepT(const BatchLaunch &b, hipStream_t s) {
     a.numJobs = b.numJobs;
     const uint32_t n = b.fp.imageW * b.fp.imageH;
     hipLaunchKernelGGL(k_batchStepBegin, dim3(blocksFor(b.numJobs, 64)), dim3(64), 0, s, b.jobs, b.numJobs);
-    hipLaunchKernelGGL(k_frameBegin, dim3(1), dim3(256), 0, s, b.fb.counters, 0u);
+    hipLaunchKernelGGL(k_frameBegin, dim3(1), dim3(256), 0, s, b.fb.counters, 0u, static_cast<uint32_t *>(nullptr));
     hipLaunchKernelGGL((k_generate<N, true>), dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
     const uint32_t numDepths = std::min<uint32_t>(b.fp.maxPathLength, kMaxDepthCounters - 1);
     const uint32_t wavefrontDepths = std::min(b.fp.wavefrontDepth, numDepths);

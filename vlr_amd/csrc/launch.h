@@ -30,7 +30,11 @@ struct FrameParams {
     // the frame's paths are pixels pixelBegin .. pixelBegin + pixelCount - 1 of the image
     // (row-major over imageW); a frame may be rendered as several such parts
     uint32_t pixelBegin, pixelCount;
+    uint32_t sortShade;            // shade each per-depth queue in order of shading class
 };
+
+// Shading classes of the material sort: 0 = miss, 1 + material type for a hit.
+constexpr uint32_t kSortClasses = 16;
 
 constexpr uint32_t kCacheTrainVerts = 3;
 // frames of at least this many pixels are rendered as two halves on two streams
@@ -55,24 +59,25 @@ constexpr uint32_t kDefaultTailBlocks = 2048;   // 8 blocks of 128 per CU
 
 // Frame-schedule autotuner.  The image does not depend on the schedule, so candidates are
 // timed on real frames: first the wavefront depth (tail grid fixed), then the tail grid at
-// the best depth; each candidate gets kFramesPerCandidate frames and keeps its fastest.
+// the best depth, then the material sort off and on; each candidate gets
+// kFramesPerCandidate frames and keeps its fastest.
 struct ScheduleTuner {
     static constexpr uint32_t kFramesPerCandidate = 3;
     struct Cand {
-        uint32_t wavefrontDepth, tailBlocks;
+        uint32_t wavefrontDepth, tailBlocks, sortShade;
         float bestMs;
     };
     std::vector<Cand> cands;
-    uint32_t phase = 0;           // 0: depths, 1: tail grids, 2: done
+    uint32_t phase = 0;           // 0: depths, 1: tail grids, 2: material sort off / on, 3: done
     uint32_t index = 0, frames = 0;
     uint64_t key = 0;
-    bool done() const { return phase >= 2; }
+    bool done() const { return phase >= 3; }
     void start(uint64_t k) {
         key = k;
         phase = 0;
         index = frames = 0;
         cands.clear();
-        for (uint32_t d : {1u, 2u, 3u, 4u}) cands.push_back({d, kDefaultTailBlocks, 1e30f});
+        for (uint32_t d : {1u, 2u, 3u, 4u}) cands.push_back({d, kDefaultTailBlocks, 0u, 1e30f});
     }
     const Cand &current() const { return cands[index]; }
     Cand best() const {
@@ -92,9 +97,13 @@ struct ScheduleTuner {
         if (phase == 0) {
             phase = 1;
             cands.clear();
-            for (uint32_t t : {512u, 1024u, 2048u, 4096u}) cands.push_back({b.wavefrontDepth, t, 1e30f});
-        } else {
+            for (uint32_t t : {512u, 1024u, 2048u, 4096u}) cands.push_back({b.wavefrontDepth, t, 0u, 1e30f});
+        } else if (phase == 1) {
             phase = 2;
+            cands.clear();
+            for (uint32_t so : {0u, 1u}) cands.push_back({b.wavefrontDepth, b.tailBlocks, so, 1e30f});
+        } else {
+            phase = 3;
             cands.assign(1, b);
         }
     }
@@ -106,6 +115,11 @@ struct FrameBuffers {
     HitBuffer hits;
     ShadowQueue shadow;
     uint32_t* counters;            // kNumCounters
+    // material sort of the per-depth queues: class per queue entry, the entries in class
+    // order, and per depth kSortClasses class counts then kSortClasses cursors
+    uint32_t* sortKey;
+    uint32_t* sortPerm;
+    uint32_t* sortCounts;          // kMaxDepthCounters * 2 * kSortClasses
     float* accum;                  // W*H*channels*2 (sum, comp)
     float4* output;                // W*H
     CacheBuffers cache;

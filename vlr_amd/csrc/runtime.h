@@ -425,6 +425,8 @@ public:
     bool traversalStats = false;
     uint32_t wavefrontDepth = kDefaultWavefrontDepth;
     uint32_t tailBlocks = kDefaultTailBlocks;
+    bool shadeSort = false;          // material sort of the per-depth shade queues
+    DeviceBuffer sortKey, sortPerm, sortCounts;
     bool autotune = true;            // cleared when the caller sets the schedule explicitly
     ScheduleTuner tuner;
     hipEvent_t tuneEvents[2] = {nullptr, nullptr};
