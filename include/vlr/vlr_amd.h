@@ -224,6 +224,18 @@ VLR_API VLRResult vlrContextBuildBvh4(VLRContext context, hipStream_t stream, co
                                       uint32_t maxLeafPrims, const uint32_t* leafRemap, void* nodes,
                                       uint32_t nodeCapacity, uint32_t* order, uint32_t* numNodes);
 
+/* ---- shader-node entry point ----
+ * Evaluates one shader-node plug (the node programs of shader_nodes.cu:20-483 with calcNode's
+ * conversions, kernel_common.h:658-737) at numPoints device-resident surface points, each 20
+ * floats: position, geometric normal, shading frame x, y, z, texture coordinates, wavelength
+ * offset (spectral: the 4 wavelengths 360 + 470 (i + offset) / 4 nm), 2 unused.  The instance
+ * transform is the identity.  values (device): 4 floats per point, the spectrum at the 4
+ * wavelengths (RGB and 0 in the RGB build) for a Spectrum plug, else the value in the plug's
+ * own type. */
+VLR_API VLRResult vlrContextEvaluateShaderNodePlug(VLRContext context, hipStream_t stream, VLRShaderNodePlug plug,
+                                                   bool spectral, const float* points, uint32_t numPoints,
+                                                   float* values);
+
 /* ---- BSDF entry point ---- */
 typedef struct VLRBSDFQuery {
     float dirLocal[3];

@@ -147,6 +147,7 @@ def lib():
         L.oracle_pcg32_floats.argtypes = [C.c_uint64, C.c_uint32, C.c_void_p]
         L.oracle_mt19937_64.argtypes = [C.c_uint64, C.c_uint32, C.c_void_p]
         L.oracle_meng_eval.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_void_p, C.c_void_p]
+        L.oracle_meng_uvs_eval.argtypes = [C.c_float, C.c_float, C.c_float, C.c_void_p, C.c_void_p]
         L.oracle_cmf_bins.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
         L.oracle_triplet_rgb.argtypes = [C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_void_p]
         if L.oracle_load_tables(str(TABLES).encode()) != 0:
@@ -371,6 +372,13 @@ def meng_eval(spectrum_type: int, color_space: int, rgb, lambdas) -> np.ndarray:
     lam = np.asarray(lambdas, np.float32)
     out = np.zeros(4, np.float32)
     lib().oracle_meng_eval(spectrum_type, color_space, *map(float, rgb), lam.ctypes.data, out.ctypes.data)
+    return out
+
+
+def meng_uvs_eval(u: float, v: float, scale: float, lambdas) -> np.ndarray:
+    lam = np.ascontiguousarray(lambdas, np.float32)
+    out = np.zeros(4, np.float32)
+    lib().oracle_meng_uvs_eval(C.c_float(u), C.c_float(v), C.c_float(scale), lam.ctypes.data, out.ctypes.data)
     return out
 
 

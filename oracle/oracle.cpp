@@ -2303,6 +2303,16 @@ void oracle_meng_eval(int spectrumType, int colorSpace, float e0, float e1, floa
     for (int i = 0; i < 4; ++i) out[i] = r[i];
 }
 
+void oracle_meng_uvs_eval(float u, float v, float scale, const float lambdas[4], float out[4]) {
+    Meng m = Meng::fromUVS(u, v, scale);
+    WLS<4> w;
+    for (int i = 0; i < 4; ++i) w.lambdas[i] = lambdas[i];
+    w.sel = 0;
+    w.single = false;
+    SS<4> r = m.evaluate(w);
+    for (int i = 0; i < 4; ++i) out[i] = r[i];
+}
+
 void oracle_cmf_bins(float x[16], float y[16], float z[16], float *integral) {
     for (int i = 0; i < 16; ++i) { x[i] = g_tables.binX[i]; y[i] = g_tables.binY[i]; z[i] = g_tables.binZ[i]; }
     *integral = g_tables.integralCMF16;

@@ -169,6 +169,8 @@ void oracle_pcg32_floats(uint64_t state, uint32_t n, float* out);
 void oracle_mt19937_64(uint64_t seed, uint32_t n, uint64_t* out);
 void oracle_meng_eval(int spectrumType, int colorSpace, float e0, float e1, float e2,
                       const float lambdas[4], float out[4]);
+/* UpsampledSpectrum(u, v, scale) (grid coordinates) evaluated at 4 wavelengths */
+void oracle_meng_uvs_eval(float u, float v, float scale, const float lambdas[4], float out[4]);
 void oracle_cmf_bins(float x[16], float y[16], float z[16], float* integral);
 void oracle_triplet_rgb(int spectrumType, int colorSpace, float e0, float e1, float e2, float out[3]);
 

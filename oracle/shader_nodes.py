@@ -1,0 +1,397 @@
+"""CPU restatement of the shader-node programs (test infrastructure only).
+
+This is the checker for the device's node programs (vlr_amd/csrc/shader_nodes.h), evaluated
+directly from the node graph, recursively, as the reference's calcNode does
+(shared/kernel_common.h:658-737), with:
+  conversions       NodeTypeInfo<T>::convertFrom (shared/shared.h:390-470); a plug whose value
+                    cannot convert to the wanted type gives the default
+  node programs     GPU_kernels/shader_nodes.cu:20-483
+  sampled spectra   RegularSampledSpectrum / IrregularSampledSpectrum::evaluate
+                    (shared/spectrum_types.cpp:394-412, 496-518)
+  images            LinearImage2D's conversions at creation (image.cpp:410-790): in the
+                    spectral build, spectra from RGB sources stored as quantised Meng grid
+                    coordinates (uvsA8x4 from 8-bit sources, uvsA16Fx4 from half / float ones);
+                    8-bit sRGB texels degammed when read (the hardware sRGB path of
+                    image.h:74, shader_nodes.cpp:1665-1683), half / float sRGB ones at creation
+  sampling          CUDA's normalised-coordinate texture fetch: linear filtering at
+                    u * width - 0.5 with full-precision weights (the hardware's 8-bit weights
+                    are not reproduced), wrap modes on texel indices, tex2Dgather order
+                    (i0, j1), (i1, j1), (i1, j0), (i0, j0); a one-channel texel reads
+                    (v, 0, 0, 1) and a two-channel one (r, g, 0, 1)
+Spectra in the spectral build go through the oracle's Meng upsampling (oracle/ov_spectrum.h).
+Arithmetic is float32 where the device's is.
+"""
+import math
+
+import numpy as np
+
+from oracle import binding as ob
+
+f32 = np.float32
+PLUG = {"float1": 0, "float2": 1, "float3": 2, "float4": 3, "Point3D": 4, "Vector3D": 5, "Normal3D": 6,
+        "Spectrum": 7, "Alpha": 8, "TextureCoordinates": 9}
+# the C API's spectrum-type strings (queryable.cpp:34-37) and values (spectrum_base.h:20-27)
+SPEC_TYPES = {"Reflectance": 0, "Transmittance": 0, "Light Source": 1, "NA": 3}
+COLOR_SPACES = {"Rec709(D65) sRGB Gamma": 0, "Rec709(D65)": 1, "XYZ": 2, "xyY": 3}
+# value classes: the C++ type of a plug's program
+F1, F2, F3, F4, P3, V3, N3, SPEC = range(8)
+CLASS_OF = {0: F1, 1: F2, 2: F3, 3: F4, 4: P3, 5: V3, 6: N3, 7: SPEC, 8: F1, 9: P3}
+EER = f32(0.009355121400914532)
+# spectrum_base.h:109-128 (column-major)
+MAT_D65_TO_XYZ = np.array([0.4124564, 0.2126729, 0.0193339, 0.3575761, 0.7151522, 0.1191920, 0.1804375, 0.0721750,
+                           0.9503041], f32)
+MAT_E_TO_XYZ = np.array([0.4969, 0.2562, 0.0233, 0.3391, 0.6782, 0.1130, 0.1640, 0.0656, 0.8637], f32)
+
+
+def wavelengths(offset):
+    """WavelengthSamples::createWithEqualOffsets (spectrum_types.h:55-65), 4 samples."""
+    return np.array([f32(360.0) + f32(470.0) * (f32(i) + f32(offset)) / f32(4) for i in range(4)], f32)
+
+
+def degamma(v):
+    v = f32(v)
+    return f32(v / f32(12.92)) if v <= f32(0.04045) else f32(math.pow((v + f32(0.055)) / f32(1.055), 2.4))
+
+
+def tristim(m, rgb):
+    r, g, b = (f32(x) for x in rgb)
+    return [f32(m[0] * r + m[3] * g + m[6] * b), f32(m[1] * r + m[4] * g + m[7] * b),
+            f32(m[2] * r + m[5] * g + m[8] * b)]
+
+
+def xy_to_uv(x, y):
+    return (f32(f32(16.730260708356887) * x + f32(7.7801960340706) * y - f32(2.170152247475828)),
+            f32(f32(-7.530081094743006) * x + f32(16.192422314095225) * y + f32(1.1125529268825947)))
+
+
+# ---- images ----
+class Texture:
+    def __init__(self, img, spectral):
+        self.width, self.height = img.width, img.height
+        fmt = img.format
+        st = SPEC_TYPES[img.spectrum_type]
+        cs = COLOR_SPACES[img.color_space]
+        srgb = cs == 0
+        rgb8 = fmt in ("RGB8x3", "RGB_8x4", "RGBA8x4")
+        half, full = fmt == "RGBA16Fx4", fmt == "RGBA32Fx4"
+        self.uvs = spectral and st != 3 and (rgb8 or half or full)
+        self.kind = ("uvs8" if rgb8 else "uvs16") if self.uvs else "rgba"
+        self.gray = fmt in ("Gray32F", "Gray8", "GrayA8x2")
+        self.spectrum_type = st
+        hw = srgb and not self.uvs and fmt in ("RGB8x3", "RGB_8x4", "RGBA8x4", "Gray8")
+        create = srgb and not self.uvs and ((not spectral and (half or full)) or fmt in ("RG32Fx2", "Gray32F"))
+        self.color_space = 1 if (hw or create) else cs
+        raw = np.ascontiguousarray(img.data, np.uint8).reshape(img.height, img.width, -1)
+        t = np.zeros((img.height, img.width, 4), f32)
+        for y in range(img.height):
+            for x in range(img.width):
+                px = raw[y, x]
+                c = [f32(0), f32(0), f32(0), f32(1)]
+                if rgb8:
+                    c[:3] = [f32(px[k]) / f32(255) for k in range(3)]
+                    c[3] = f32(px[3]) / f32(255) if fmt == "RGBA8x4" else f32(1)
+                elif half:
+                    c = list(px[:8].view(np.float16).astype(f32))
+                elif full:
+                    c = list(px[:16].view(f32))
+                elif fmt == "RG32Fx2":
+                    c[:2] = list(px[:8].view(f32))
+                elif fmt == "Gray32F":
+                    c[0] = px[:4].view(f32)[0]
+                elif fmt == "Gray8":
+                    c[0] = f32(px[0]) / f32(255)
+                elif fmt == "GrayA8x2":
+                    c[0], c[1] = f32(px[0]) / f32(255), f32(px[1]) / f32(255)
+                if self.uvs:
+                    rgb = [degamma(v) if srgb else f32(v) for v in c[:3]]
+                    X, Y, Z = tristim(MAT_E_TO_XYZ if st in (0, 2) else MAT_D65_TO_XYZ, rgb)
+                    b = f32(X + Y + Z)
+                    xy = (f32(X / b), f32(Y / b)) if b > 0 else (f32(1.0 / 3.0), f32(1.0 / 3.0))
+                    u, v = xy_to_uv(*xy)
+                    if rgb8:
+                        q = lambda a: f32(int(f32(255) * min(max(a, f32(0)), f32(1)))) / f32(255)
+                        t[y, x] = [q(u / f32(12)), q(v / f32(14)), q(b / f32(3)), c[3]]
+                    else:
+                        h = lambda a: f32(np.float16(a))
+                        t[y, x] = [h(u), h(v), h(b), c[3] if half else h(c[3])]
+                    continue
+                if hw:
+                    c[:3] = [degamma(v) for v in c[:3]]
+                    if fmt == "Gray8":
+                        c[1] = c[2] = f32(0)
+                elif create:
+                    n = 2 if fmt == "RG32Fx2" else (1 if fmt == "Gray32F" else 3)
+                    for k in range(n):
+                        c[k] = degamma(c[k])
+                        if half:
+                            c[k] = f32(np.float16(c[k]))
+                elif srgb and fmt == "GrayA8x2":
+                    c[0] = f32(int(min(255.0, math.floor(256.0 * float(degamma(c[0])))))) / f32(255)
+                t[y, x] = c
+        self.texels = t
+
+    def texel(self, x, y, wu, wv):
+        def wrap(i, n, mode):
+            if mode == 0:
+                return i % n, False
+            if mode == 2:
+                m = i % (2 * n)
+                return (m if m < n else 2 * n - 1 - m), False
+            if mode == 3 and (i < 0 or i >= n):
+                return 0, True
+            return min(max(i, 0), n - 1), False
+        ix, ox = wrap(x, self.width, wu)
+        iy, oy = wrap(y, self.height, wv)
+        return np.zeros(4, f32) if (ox or oy) else self.texels[iy, ix]
+
+    def sample(self, u, v, linear, wu, wv):
+        u, v = f32(u), f32(v)
+        if not linear:
+            return self.texel(int(math.floor(u * f32(self.width))), int(math.floor(v * f32(self.height))), wu, wv)
+        fx, fy = f32(u * f32(self.width) - f32(0.5)), f32(v * f32(self.height) - f32(0.5))
+        x0, y0 = math.floor(fx), math.floor(fy)
+        a, b = f32(fx - f32(x0)), f32(fy - f32(y0))
+        w = [f32((1 - a) * (1 - b)), f32(a * (1 - b)), f32((1 - a) * b), f32(a * b)]
+        t = [self.texel(x0, y0, wu, wv), self.texel(x0 + 1, y0, wu, wv), self.texel(x0, y0 + 1, wu, wv),
+             self.texel(x0 + 1, y0 + 1, wu, wv)]
+        return np.array([f32(w[0] * t[0][c] + w[1] * t[1][c] + w[2] * t[2][c] + w[3] * t[3][c]) for c in range(4)], f32)
+
+    def gather(self, u, v, comp, wu, wv):
+        x0 = math.floor(f32(f32(u) * f32(self.width) - f32(0.5)))
+        y0 = math.floor(f32(f32(v) * f32(self.height) - f32(0.5)))
+        return np.array([self.texel(x0, y0 + 1, wu, wv)[comp], self.texel(x0 + 1, y0 + 1, wu, wv)[comp],
+                         self.texel(x0 + 1, y0, wu, wv)[comp], self.texel(x0, y0, wu, wv)[comp]], f32)
+
+
+# ---- evaluation ----
+class Point:
+    """A surface point: position, geometric normal, shading frame, texture coordinates, and the
+    wavelength offset of the spectral build."""
+
+    def __init__(self, row):
+        r = np.asarray(row, f32)
+        self.p, self.gn = r[0:3], r[3:6]
+        self.tx, self.ty, self.tz = r[6:9], r[9:12], r[12:15]
+        self.uv = r[15:17]
+        self.wl_offset = r[17]
+
+
+def convert(want, src, v):
+    """NodeTypeInfo<want>::convertFrom<src>; None where it is not defined."""
+    v = np.asarray(v, f32)
+    if want == F1:
+        return v.copy() if src == F1 else None
+    if want == F2:
+        return v.copy() if src == F2 else (np.array([v[0], v[0], 0, 0], f32) if src == F1 else None)
+    if want == F3:
+        if src in (F3, P3, V3, N3):
+            return np.array([v[0], v[1], v[2], 0], f32)
+        return np.array([v[0], v[0], v[0], 0], f32) if src == F1 else None
+    if want == F4:
+        return v.copy() if src == F4 else (np.full(4, v[0], f32) if src == F1 else None)
+    if want in (P3, V3):
+        return np.array([v[0], v[1], v[2], 0], f32) if src in (P3, V3, N3, F3) else None
+    if want == N3:
+        if src == N3:
+            return np.array([v[0], v[1], v[2], 0], f32)
+        if src in (F3, P3, V3):
+            ln = f32(math.sqrt(float(f32(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]))))
+            return np.array([v[0] / ln, v[1] / ln, v[2] / ln, 0], f32)
+    return None
+
+
+class Evaluator:
+    def __init__(self, spectral):
+        self.spectral = spectral
+        self.textures = {}
+
+    def texture(self, img):
+        if id(img) not in self.textures:
+            self.textures[id(img)] = Texture(img, self.spectral)
+        return self.textures[id(img)]
+
+    def calc(self, plug, want, default, pt):
+        """calcNode for a numeric type."""
+        if plug is None:
+            return np.asarray(default, f32)
+        node, ptype, opt = plug
+        src = CLASS_OF[ptype]
+        if src == SPEC:
+            return np.asarray(default, f32)
+        out = convert(want, src, self.value(node, ptype, opt, pt))
+        return np.asarray(default, f32) if out is None else out
+
+    def value(self, node, ptype, opt, pt):
+        z = np.zeros(4, f32)
+        k = node.type
+        fl = lambda name, i, d: f32(node.floats.get(name, [d] * (i + 1))[i])
+        inp = lambda name: node.plugs.get(name)
+        if k == "Geometry":
+            if ptype == PLUG["Point3D"]:
+                return np.array([*pt.p, 0], f32)
+            if ptype == PLUG["Normal3D"]:
+                return np.array([*(pt.gn if opt == 0 else pt.tz if opt == 1 else np.zeros(3, f32)), 0], f32)
+            if ptype == PLUG["Vector3D"]:
+                return np.array([*(pt.tx if opt == 0 else pt.ty if opt == 1 else np.zeros(3, f32)), 0], f32)
+            if ptype == PLUG["TextureCoordinates"]:
+                return np.array([pt.uv[0], pt.uv[1], 0, 0], f32)
+            return z
+        if k == "Tangent":
+            # shader_nodes.cu:56-87 with the identity instance transform
+            tt = node.enums.get("tangent type", "TC0 Direction")
+            x, y, zz = pt.p
+            if tt == "TC0 Direction":
+                return np.array([*pt.tx, 0], f32)
+            t = {"Radial X": (0, -zz, y), "Radial Y": (zz, 0, -x), "Radial Z": (-y, x, 0)}[tt]
+            return np.array([*t, 0], f32)
+        if k in ("Float2", "Float3", "Float4"):
+            n = {"Float2": 2, "Float3": 3, "Float4": 4}[k]
+            width = {0: 1, 1: 2, 2: 3, 3: 4}.get(ptype, 0)
+            if opt + width > n:
+                return z
+            c = [self.calc(inp(str(i)), F1, [fl(str(i), 0, 0.0), 0, 0, 0], pt)[0] for i in range(opt, opt + width)]
+            return np.array(c + [0] * (4 - width), f32)
+        if k == "ScaleAndOffsetFloat":
+            value = self.calc(inp("value"), F1, z, pt)[0]
+            scale = self.calc(inp("scale"), F1, [fl("scale", 0, 1.0), 0, 0, 0], pt)[0]
+            offset = self.calc(inp("offset"), F1, [fl("offset", 0, 0.0), 0, 0, 0], pt)[0]
+            return np.array([f32(scale * value) + offset, 0, 0, 0], f32)
+        if k == "ScaleAndOffsetUVTextureMap2D":
+            return np.array([f32(fl("scale", 0, 1) * pt.uv[0]) + fl("offset", 0, 0),
+                             f32(fl("scale", 1, 1) * pt.uv[1]) + fl("offset", 1, 0), 0, 0], f32)
+        if k == "Image2DTexture":
+            tex, wu, wv, linear, bump, coeff = self.sampler(node)
+            tc = self.calc(inp("texcoord"), P3, [pt.uv[0], pt.uv[1], 0, 0], pt)
+            if ptype == PLUG["Normal3D"]:
+                # shader_nodes.cu:367-411
+                n = np.array([0, 0, 1], f32)
+                if bump != 2:
+                    t = tex.sample(tc[0], tc[1], linear, wu, wv)
+                    if opt < 2:
+                        n = (t[0:3] if opt == 0 else t[1:4]).astype(f32)
+                        n = (f32(2) * n - f32(1)).astype(f32)
+                        if bump == 0:
+                            n[1] = -n[1]
+                else:
+                    t = tex.gather(tc[0], tc[1], opt, wu, wv)
+                    c = f32(5.0 / 1024)
+                    dhdu = f32(f32(c * f32(tex.width)) * f32(t[1] - t[0]))
+                    dhdv = f32(f32(c * f32(tex.height)) * f32(t[0] - t[3]))
+                    n = np.array([-dhdu, dhdv, 1], f32)
+                n[0] *= coeff
+                n[1] *= coeff
+                ln = f32(math.sqrt(float(f32(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]))))
+                return np.array([n[0] / ln, n[1] / ln, n[2] / ln, 0], f32)
+            t = tex.sample(tc[0], tc[1], linear, wu, wv)
+            if ptype in (PLUG["float1"], PLUG["Alpha"]) and opt < 4:
+                return np.array([t[opt], 0, 0, 0], f32)
+            if ptype == PLUG["float2"] and opt < 3:
+                return np.array([t[opt], t[opt + 1], 0, 0], f32)
+            if ptype == PLUG["float3"] and opt < 2:
+                return np.array([t[opt], t[opt + 1], t[opt + 2], 0], f32)
+            if ptype == PLUG["float4"]:
+                return t.copy()
+            return z
+        return z
+
+    def sampler(self, node):
+        wraps = ["Repeat", "Clamp to Edge", "Mirror", "Clamp to Border"]
+        bumps = ["Normal Map (DirectX)", "Normal Map (OpenGL)", "Height Map"]
+        wu = wraps.index(node.enums.get("wrap u", "Repeat"))
+        wv = wraps.index(node.enums.get("wrap v", "Repeat"))
+        linear = node.enums.get("mag filter", "Linear") == "Linear"
+        bump = bumps.index(node.enums.get("bump type", "Normal Map (DirectX)"))
+        # 5-bit coefficient (shader_nodes.cpp:1685-1688, shared.h:743-746)
+        q = min(int(round(31 * float(node.floats.get("bump coeff", [1.0])[0]) * 0.5)), 31)
+        return self.texture(node.image), wu, wv, linear, bump, f32(q + 1) / f32(16)
+
+    def spectrum(self, plug, default, pt):
+        """calcNode for a spectrum: the values at the point's 4 wavelengths (spectral build) or
+        RGB (RGB build).  default: a callable giving the immediate's spectrum."""
+        if plug is None or plug[1] != PLUG["Spectrum"]:
+            return default()
+        node, _, _ = plug
+        lam = wavelengths(pt.wl_offset)
+        st = lambda: SPEC_TYPES[node.enums.get("spectrum type", "Reflectance")]
+        cs = lambda: COLOR_SPACES[node.enums.get("color space", "Rec709(D65)")]
+        if node.type == "TripletSpectrum":
+            tr = node.floats.get("triplet", [0.18, 0.18, 0.18])
+            return self.triplet(st(), cs(), tr, lam)
+        if node.type in ("RegularSampledSpectrum", "IrregularSampledSpectrum"):
+            if not self.spectral:
+                # the RGB build's reduction of a sampled spectrum has no counterpart to check
+                # against (parity unpinned)
+                raise NotImplementedError("sampled spectra in the RGB build")
+            return self.sampled(node, lam)
+        if node.type == "Float3ToSpectrum":
+            # shader_nodes.cu:254-274
+            v = self.calc(node.plugs.get("value"), F3, [*node.floats.get("value", [0, 0, 0]), 0], pt)
+            c = [min(max(f32(f32(0.5) * v[i] + f32(0.5)), f32(0)), f32(1)) for i in range(3)]
+            if not self.spectral:
+                return np.array(c, f32)
+            return ob.meng_eval(st(), cs(), c, lam)
+        if node.type == "Image2DTexture":
+            # shader_nodes.cu:413-452
+            tex, wu, wv, linear, _, _ = self.sampler(node)
+            tc = self.calc(node.plugs.get("texcoord"), P3, [pt.uv[0], pt.uv[1], 0, 0], pt)
+            t = tex.sample(tc[0], tc[1], linear, wu, wv).copy()
+            if tex.gray:
+                t[1] = t[2] = t[0]
+            if not self.spectral:
+                return t[:3].copy()
+            if tex.uvs:
+                u, v, s = t[0], t[1], t[2]
+                if tex.kind == "uvs8":
+                    u, v, s = f32(u * f32(12)), f32(v * f32(14)), f32(s * f32(3))
+                return ob.meng_uvs_eval(u, v, f32(s / EER), lam)
+            return ob.meng_eval(tex.spectrum_type, tex.color_space, t[:3], lam)
+        return default()
+
+    def triplet(self, st, cs, tr, lam):
+        if self.spectral:
+            return ob.meng_eval(st, cs, tr, lam)
+        return ob.triplet_rgb(st, cs, tr)
+
+    def sampled(self, node, lam):
+        values = [f32(v) for v in node.floats["values"]]
+        n = len(values)
+        out = np.zeros(4, f32)
+        if node.type == "RegularSampledSpectrum":
+            lo, hi = f32(node.floats["min wavelength"][0]), f32(node.floats["max wavelength"][0])
+            for i, wl in enumerate(lam):
+                binf = f32(f32(f32(wl - lo) / f32(hi - lo)) * f32(n - 1))
+                if binf <= 0:
+                    out[i] = values[0]
+                elif binf >= n - 1:
+                    out[i] = values[n - 1]
+                else:
+                    b = int(binf)
+                    t = f32(binf - f32(b))
+                    out[i] = f32((f32(1) - t) * values[b]) + f32(t * values[b + 1])
+            return out
+        lambdas = [f32(v) for v in node.floats["wavelengths"]]
+        base = 0
+        for i, wl in enumerate(lam):
+            p2 = 1
+            while p2 < n - base:
+                p2 <<= 1
+            d = p2 >> 1
+            while d > 0:
+                if base + d < n and lambdas[base + d] <= wl:
+                    base += d
+                d >>= 1
+            if base >= n - 1:
+                out[i] = values[n - 1]
+                continue
+            t = f32(f32(wl - lambdas[base]) / f32(lambdas[base + 1] - lambdas[base]))
+            out[i] = values[base] if t <= 0 else f32(f32((f32(1) - t) * values[base]) + f32(t * values[base + 1]))
+        return out
+
+    def evaluate(self, plug, pt):
+        """What the device's entry point returns for one plug at one point (4 floats)."""
+        node, ptype, opt = plug
+        if ptype == PLUG["Spectrum"]:
+            s = self.spectrum(plug, lambda: np.zeros(4 if self.spectral else 3, f32), pt)
+            return np.array(list(s) + [0] * (4 - len(s)), f32)
+        cls = CLASS_OF[ptype]
+        return self.calc(plug, cls, np.zeros(4, f32), pt)

@@ -119,6 +119,35 @@ def test_parameters_round_trip(ctx):
     assert code_of("vlrQueryableSetFloat", m, b"no such parameter", 1.0) == 3
 
 
+def test_plug_types_follow_the_conversion_rules(ctx):
+    """A plug connects where its value converts to the socket's type (shared.h:390-470);
+    set() refuses the rest with InvalidArgument (vlr.cpp:705-717)."""
+    def plug(node, ptype, option=0):
+        p = api.VLRShaderNodePlug()
+        api.call("vlrShaderNodeGetPlug", node, ptype, option, C.byref(p))
+        return p
+    geo = ctx.shader_node("Geometry")
+    f3 = ctx.shader_node("Float3")
+    tex = ctx.shader_node("Image2DTexture")
+    sof = ctx.shader_node("ScaleAndOffsetFloat")
+    f2s = ctx.shader_node("Float3ToSpectrum")
+    mat = ctx.material("Matte")
+    ue4 = ctx.material("UE4")
+    setp = lambda q, name, p: code_of("vlrQueryableSetShaderNodePlug", q, name, p)
+    assert setp(sof, b"value", plug(f3, api.PLUG_FLOAT1, 2)) == 0
+    assert setp(sof, b"value", plug(tex, api.PLUG_ALPHA, 3)) == 0
+    assert setp(sof, b"value", plug(f3, api.PLUG_FLOAT3)) == 3          # float from float3
+    assert setp(tex, b"texcoord", plug(geo, api.PLUG_NORMAL3D, 1)) == 0  # Point3D from Normal3D
+    assert setp(tex, b"texcoord", plug(f3, api.PLUG_FLOAT3)) == 0
+    assert setp(tex, b"texcoord", plug(f3, api.PLUG_FLOAT2)) == 3
+    assert setp(f2s, b"value", plug(f3, api.PLUG_FLOAT1)) == 0          # float3 from float
+    assert setp(f2s, b"value", plug(tex, api.PLUG_FLOAT4)) == 3
+    assert setp(mat, b"albedo", plug(tex, api.PLUG_SPECTRUM)) == 0
+    assert setp(mat, b"albedo", plug(tex, api.PLUG_FLOAT3)) == 3
+    assert setp(ue4, b"occlusion/roughness/metallic", plug(tex, api.PLUG_FLOAT3, 1)) == 0
+    assert setp(ue4, b"occlusion/roughness/metallic", plug(tex, api.PLUG_FLOAT2)) == 3
+
+
 def test_scene_graph(ctx):
     mesh = api.H()
     api.call("vlrTriangleMeshSurfaceNodeCreate", ctx.h, b"quad", C.byref(mesh))

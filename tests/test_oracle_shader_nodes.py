@@ -1,0 +1,230 @@
+"""Pins of the shader-node oracle (oracle/shader_nodes.py) by closed forms and invariants:
+bilinear filtering reproduces affine texel fields, the wrap modes' periodicity and symmetry,
+the gather order against the filter's weights, the plug conversions, sampled spectra against
+numpy's piecewise-linear interpolation, sRGB decoding at known values, the Meng texel
+encodings against the direct upsampling of the same colour, normal and height maps of known
+fields, and the radial tangents."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import binding as ob
+from oracle import shader_nodes as osn
+from vlr_amd.scenes import ImageDesc, NodeDesc, REC709, REC709_SRGB
+
+P = osn.PLUG
+
+
+def rgba32f(arr):
+    a = np.ascontiguousarray(arr, np.float32)
+    h, w = a.shape[:2]
+    return ImageDesc(a.view(np.uint8).reshape(h, w, 16), w, h, "RGBA32Fx4", "NA", REC709)
+
+
+def point(u=0.5, v=0.5, p=(0.0, 0.0, 0.0), offset=0.25):
+    row = np.zeros(20, np.float32)
+    row[0:3] = p
+    row[3:6] = (0, 0, 1)
+    row[6:9] = (1, 0, 0)
+    row[9:12] = (0, 1, 0)
+    row[12:15] = (0, 0, 1)
+    row[15:17] = (u, v)
+    row[17] = offset
+    return osn.Point(row)
+
+
+def tex_node(img, **enums):
+    e = {"wrap u": "Clamp to Edge", "wrap v": "Clamp to Edge"}
+    e.update(enums)
+    return NodeDesc("Image2DTexture", enums=e, image=img)
+
+
+def test_wavelengths():
+    np.testing.assert_allclose(osn.wavelengths(0.0), [360.0, 477.5, 595.0, 712.5])
+    np.testing.assert_allclose(osn.wavelengths(0.5), [418.75, 536.25, 653.75, 771.25])
+
+
+def test_bilinear_reproduces_affine_fields():
+    W, H = 7, 5
+    ys, xs = np.mgrid[0:H, 0:W].astype(np.float32)
+    field = np.stack([1 + 2 * xs, 3 - ys, 0.5 * xs + 0.25 * ys, np.ones_like(xs)], -1)
+    tex = osn.Texture(rgba32f(field), False)
+    rng = np.random.default_rng(1)
+    for _ in range(50):
+        # interior: texel-centre coordinates between 0 and W-1 / H-1
+        x, y = rng.uniform(0, W - 1), rng.uniform(0, H - 1)
+        u, v = (x + 0.5) / W, (y + 0.5) / H
+        got = tex.sample(u, v, True, 1, 1)
+        fx, fy = np.float32(u) * W - 0.5, np.float32(v) * H - 0.5
+        want = [1 + 2 * fx, 3 - fy, 0.5 * fx + 0.25 * fy, 1]
+        np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-5)
+
+
+def test_nearest_and_wrap_modes():
+    W, H = 4, 3
+    rng = np.random.default_rng(2)
+    field = rng.uniform(0, 1, (H, W, 4)).astype(np.float32)
+    tex = osn.Texture(rgba32f(field), False)
+    for j in range(H):
+        for i in range(W):
+            u, v = (i + 0.5) / W, (j + 0.5) / H
+            np.testing.assert_array_equal(tex.sample(u, v, False, 0, 0), field[j, i])
+            # repeat: period 1 in both coordinates
+            np.testing.assert_array_equal(tex.sample(u + 2, v - 1, False, 0, 0), field[j, i])
+            # mirror: texel i at u and at 2 - u (and at -u)
+            np.testing.assert_array_equal(tex.sample(2 - u, v, False, 2, 0), field[j, i])
+            np.testing.assert_array_equal(tex.sample(-u, v, False, 2, 0), field[j, i])
+            # clamp to edge: beyond the right edge reads the last column
+            np.testing.assert_array_equal(tex.sample(1.7, v, False, 1, 1), field[j, W - 1])
+    # clamp to border: outside reads zero, linear filtering at the edge blends with zero
+    np.testing.assert_array_equal(tex.sample(-0.2, 0.5, False, 3, 3), np.zeros(4))
+    edge = tex.sample(0.0, 0.5 / H, True, 3, 3)
+    np.testing.assert_allclose(edge, 0.5 * field[0, 0], rtol=1e-6)
+
+
+def test_gather_order_matches_the_filter_weights():
+    W, H = 5, 4
+    rng = np.random.default_rng(3)
+    field = rng.uniform(0, 1, (H, W, 4)).astype(np.float32)
+    tex = osn.Texture(rgba32f(field), False)
+    for _ in range(40):
+        u, v = rng.uniform(0, 1), rng.uniform(0, 1)
+        fx, fy = np.float32(u) * W - 0.5, np.float32(v) * H - 0.5
+        a, b = fx - math.floor(fx), fy - math.floor(fy)
+        for comp in range(4):
+            g = tex.gather(u, v, comp, 0, 0)
+            # CUDA's tex2Dgather: x (i0, j1), y (i1, j1), z (i1, j0), w (i0, j0)
+            blend = (1 - a) * (1 - b) * g[3] + a * (1 - b) * g[2] + (1 - a) * b * g[0] + a * b * g[1]
+            assert abs(blend - tex.sample(u, v, True, 0, 0)[comp]) < 1e-5
+
+
+def test_conversions():
+    ev = osn.Evaluator(False)
+    f3 = NodeDesc("Float3", floats={"0": [3.0], "1": [0.0], "2": [4.0]})
+    # Normal3D from float3: normalised
+    n = osn.convert(osn.N3, osn.F3, ev.value(f3, P["float3"], 0, point()))
+    np.testing.assert_allclose(n[:3], [0.6, 0.0, 0.8], rtol=1e-6)
+    # float2 from float1: broadcast; float4 from float1: broadcast
+    np.testing.assert_array_equal(osn.convert(osn.F2, osn.F1, [2, 0, 0, 0])[:2], [2, 2])
+    np.testing.assert_array_equal(osn.convert(osn.F4, osn.F1, [2, 0, 0, 0]), [2, 2, 2, 2])
+    # float1 from float3 is not defined: the consumer's default is used
+    assert osn.convert(osn.F1, osn.F3, [1, 2, 3, 0]) is None
+    scale = NodeDesc("ScaleAndOffsetFloat", floats={"scale": [2.0], "offset": [0.5]},
+                     plugs={"value": (f3, P["float3"], 0)})
+    np.testing.assert_array_equal(ev.value(scale, P["float1"], 0, point())[0], 0.5)
+    scale.plugs["value"] = (f3, P["float1"], 2)
+    np.testing.assert_array_equal(ev.value(scale, P["float1"], 0, point())[0], 8.5)
+    # a component window past the node's width gives zero
+    np.testing.assert_array_equal(ev.value(f3, P["float2"], 2, point()), np.zeros(4))
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_sampled_spectra_match_linear_interpolation(seed):
+    rng = np.random.default_rng(seed)
+    ev = osn.Evaluator(True)
+    n = int(rng.integers(2, 40))
+    values = rng.uniform(0, 2, n).astype(np.float32)
+    lo, hi = float(rng.uniform(350, 450)), float(rng.uniform(650, 850))
+    reg = NodeDesc("RegularSampledSpectrum", floats={"min wavelength": [lo], "max wavelength": [hi],
+                                                     "values": list(values)})
+    lam = np.sort(rng.uniform(300, 900, n)).astype(np.float32)
+    irr = NodeDesc("IrregularSampledSpectrum", floats={"wavelengths": list(lam), "values": list(values)})
+    for off in (0.0, 0.3, 0.99):
+        pt = point(offset=off)
+        wl = osn.wavelengths(off)
+        grid = np.linspace(lo, hi, n)
+        np.testing.assert_allclose(ev.spectrum((reg, P["Spectrum"], 0), None, pt), np.interp(wl, grid, values),
+                                   rtol=2e-5, atol=2e-5)
+        np.testing.assert_allclose(ev.spectrum((irr, P["Spectrum"], 0), None, pt), np.interp(wl, lam, values),
+                                   rtol=2e-5, atol=2e-5)
+
+
+def test_srgb_texels_decoded():
+    px = np.array([[[0, 128, 255, 255]]], np.uint8)
+    tex = osn.Texture(ImageDesc(px, 1, 1, "RGBA8x4", "NA", REC709_SRGB), False)
+    # sRGB 128 / 255 in linear light
+    np.testing.assert_allclose(tex.texels[0, 0], [0.0, 0.21586050, 1.0, 1.0], rtol=1e-6)
+    assert tex.color_space == osn.COLOR_SPACES[REC709]
+    # spectrum type NA keeps RGBA in the spectral build too
+    assert osn.Texture(ImageDesc(px, 1, 1, "RGBA8x4", "NA", REC709_SRGB), True).kind == "rgba"
+
+
+@pytest.mark.parametrize("fmt,tol", [("RGBA8x4", 0.03), ("RGBA16Fx4", 2e-3), ("RGBA32Fx4", 2e-3)])
+def test_meng_texel_encodings_match_direct_upsampling(fmt, tol):
+    rng = np.random.default_rng(4)
+    rgb = rng.uniform(0.05, 0.95, (6, 3)).astype(np.float32)
+    for c in rgb:
+        if fmt == "RGBA8x4":
+            q = np.round(c * 255).astype(np.uint8)
+            data = np.array([[[q[0], q[1], q[2], 255]]], np.uint8)
+            lin = q.astype(np.float32) / 255
+        elif fmt == "RGBA16Fx4":
+            h = np.array([c[0], c[1], c[2], 1], np.float16)
+            data = h.view(np.uint8).reshape(1, 1, 8)
+            lin = h[:3].astype(np.float32)
+        else:
+            data = np.array([c[0], c[1], c[2], 1], np.float32).view(np.uint8).reshape(1, 1, 16)
+            lin = c
+        img = ImageDesc(data, 1, 1, fmt, "Reflectance", REC709)
+        node = tex_node(img, **{"mag filter": "Nearest"})
+        ev = osn.Evaluator(True)
+        assert ev.texture(img).kind == ("uvs8" if fmt == "RGBA8x4" else "uvs16")
+        pt = point(offset=0.4)
+        got = ev.spectrum((node, P["Spectrum"], 0), None, pt)
+        want = ob.meng_eval(0, 1, lin, osn.wavelengths(0.4))
+        np.testing.assert_allclose(got, want, rtol=tol, atol=tol)
+
+
+def test_normal_maps():
+    flat = rgba32f(np.array([[[0.5, 0.5, 1.0, 1.0]]], np.float32))
+    ev = osn.Evaluator(False)
+    for bump in ("Normal Map (DirectX)", "Normal Map (OpenGL)"):
+        n = ev.value(tex_node(flat, **{"bump type": bump}), P["Normal3D"], 0, point())
+        np.testing.assert_array_equal(n[:3], [0, 0, 1])
+    tilted = rgba32f(np.array([[[0.75, 0.75, 1.0, 1.0]]], np.float32))
+    # the default coefficient 1 is stored in 5 bits as round(31 * 1 / 2) = 16 and read as 17 / 16
+    c = 17 / 16
+    s = 1 / math.sqrt(0.5 * c * c + 1)
+    dx = ev.value(tex_node(tilted, **{"bump type": "Normal Map (DirectX)"}), P["Normal3D"], 0, point())
+    gl = ev.value(tex_node(tilted, **{"bump type": "Normal Map (OpenGL)"}), P["Normal3D"], 0, point())
+    np.testing.assert_allclose(dx[:3], [0.5 * c * s, -0.5 * c * s, s], rtol=1e-6)
+    np.testing.assert_allclose(gl[:3], [0.5 * c * s, 0.5 * c * s, s], rtol=1e-6)
+    # coefficient 2 (stored as 31, read as 32 / 16) doubles the tilt
+    node2 = NodeDesc("Image2DTexture", floats={"bump coeff": [2.0]}, enums={"bump type": "Normal Map (DirectX)"},
+                     image=tilted)
+    dx2 = ev.value(node2, P["Normal3D"], 0, point())
+    np.testing.assert_allclose(dx2[:3] / dx2[2], [1.0, -1.0, 1.0], rtol=1e-6)
+
+
+def test_height_map_of_a_ramp():
+    W, H, k = 16, 8, 0.01
+    ys, xs = np.mgrid[0:H, 0:W].astype(np.float32)
+    field = np.stack([k * xs, np.zeros_like(xs), np.zeros_like(xs), np.ones_like(xs)], -1)
+    img = rgba32f(field)
+    node = tex_node(img, **{"bump type": "Height Map"})
+    n = osn.Evaluator(False).value(node, P["Normal3D"], 0, point(u=0.4, v=0.6))
+    # slope k per texel: dh/du = (5 / 1024) * W * k, tilted by the stored coefficient 17 / 16;
+    # the normal leans against it
+    dhdu = 5.0 / 1024 * W * k * (17 / 16)
+    want = np.array([-dhdu, 0.0, 1.0]) / math.hypot(dhdu, 1.0)
+    np.testing.assert_allclose(n[:3], want, rtol=1e-5, atol=1e-7)
+
+
+def test_radial_tangents():
+    ev = osn.Evaluator(False)
+    cases = {"Radial X": ((0, 1, 0), (0, 0, 1)), "Radial Y": ((0, 0, 1), (1, 0, 0)),
+             "Radial Z": ((1, 0, 0), (0, 1, 0))}
+    for tt, (p, t) in cases.items():
+        node = NodeDesc("Tangent", enums={"tangent type": tt})
+        got = ev.value(node, P["Vector3D"], 0, point(p=p))
+        np.testing.assert_array_equal(got[:3], t)
+        # perpendicular to the position and to the axis
+        assert abs(float(np.dot(got[:3], p))) == 0.0
+
+
+def test_float3_to_spectrum_rgb_maps_minus_one_to_one_onto_zero_to_one():
+    ev = osn.Evaluator(False)
+    f3 = NodeDesc("Float3", floats={"0": [-1.0], "1": [0.0], "2": [3.0]})
+    node = NodeDesc("Float3ToSpectrum", plugs={"value": (f3, P["float3"], 0)})
+    np.testing.assert_array_equal(ev.spectrum((node, P["Spectrum"], 0), None, point()), [0.0, 0.5, 1.0])

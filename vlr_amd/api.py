@@ -26,7 +26,9 @@ VLRRenderer_LightTracing = 1
 VLRRenderer_BPT = 2
 VLRRenderer_DebugRendering = 3
 
+PLUG_FLOAT1, PLUG_FLOAT2, PLUG_FLOAT3, PLUG_FLOAT4, PLUG_POINT3D, PLUG_VECTOR3D, PLUG_NORMAL3D = 0, 1, 2, 3, 4, 5, 6
 PLUG_SPECTRUM = 7
+PLUG_ALPHA, PLUG_TEXCOORD = 8, 9
 
 
 class VLRError(RuntimeError):
@@ -263,6 +265,7 @@ PROTOTYPES: Dict[str, list] = {
     "vlrContextWriteAccumulation": [H, P, U32],
     "vlrContextTraceRays": [H, H, H, P, U32, P, B],
     "vlrContextEvaluateBSDFs": [H, H, H, B, P, U32, P],
+    "vlrContextEvaluateShaderNodePlug": [H, H, VLRShaderNodePlug, B, P, U32, P],
     "vlrContextSetAccelBuilder": [H, U32],
     "vlrContextGetAccelBuilder": [H, C.POINTER(U32)],
     "vlrContextGetAccelStats": [H, C.POINTER(VLRAccelStats)],
@@ -532,6 +535,11 @@ class Context:
     def evaluate_bsdfs(self, material, spectral: bool, q_ptr: int, n: int, r_ptr: int, stream=None):
         call("vlrContextEvaluateBSDFs", self.h, stream, material, spectral, C.c_void_p(q_ptr), n, C.c_void_p(r_ptr))
 
+    def evaluate_node_plug(self, plug, spectral: bool, points_ptr: int, n: int, values_ptr: int, stream=None):
+        """One shader-node plug at n device-resident surface points (20 floats each)."""
+        call("vlrContextEvaluateShaderNodePlug", self.h, stream, plug, spectral, C.c_void_p(points_ptr), n,
+             C.c_void_p(values_ptr))
+
     def set_accel_builder(self, builder: int):
         call("vlrContextSetAccelBuilder", self.h, builder)
 
@@ -603,8 +611,45 @@ def set_float(obj, name: str, v: float):
     call("vlrQueryableSetFloat", obj, _b(name), float(v))
 
 
-def build_material(ctx: Context, m, env_texture_node=None):
+def build_image(ctx: Context, img, keep: list):
+    data = np.ascontiguousarray(img.data, np.uint8)
+    keep.append(data)
+    h = H()
+    call("vlrLinearImage2DCreate", ctx.h, data.ctypes.data, img.width, img.height, _b(img.format),
+         _b(img.spectrum_type), _b(img.color_space), C.byref(h))
+    return h
+
+
+def build_node(ctx: Context, node, cache: dict, keep: list):
+    """Creates a scenes.NodeDesc graph through the C API (each node once); returns its handle."""
+    if id(node) in cache:
+        return cache[id(node)]
+    h = ctx.shader_node(node.type)
+    cache[id(node)] = h
+    for name, vals in node.floats.items():
+        arr = (C.c_float * len(vals))(*[float(v) for v in vals])
+        call("vlrQueryableSetFloatTuple", h, _b(name), arr, len(vals))
+    for name, value in node.enums.items():
+        call("vlrQueryableSetEnumValue", h, _b(name), _b(value))
+    if node.image is not None:
+        call("vlrQueryableSetImage2D", h, b"image", build_image(ctx, node.image, keep))
+    for name, (child, ptype, option) in node.plugs.items():
+        call("vlrQueryableSetShaderNodePlug", h, _b(name), node_plug(ctx, child, ptype, option, cache, keep))
+    return h
+
+
+def node_plug(ctx: Context, node, ptype: int, option: int, cache: dict, keep: list) -> VLRShaderNodePlug:
+    plug = VLRShaderNodePlug()
+    call("vlrShaderNodeGetPlug", build_node(ctx, node, cache, keep), ptype, option, C.byref(plug))
+    return plug
+
+
+def build_material(ctx: Context, m, env_texture_node=None, node_cache=None, keep=None):
     h = ctx.material(m.type)
+    node_cache = {} if node_cache is None else node_cache
+    keep = [] if keep is None else keep
+    for name, (node, ptype, option) in getattr(m, "plugs", {}).items():
+        call("vlrQueryableSetShaderNodePlug", h, _b(name), node_plug(ctx, node, ptype, option, node_cache, keep))
     for name, (cs, e0, e1, e2) in m.spectra.items():
         if m.use_env_texture and name == "emittance" and env_texture_node is not None:
             continue
@@ -640,8 +685,9 @@ def build_scene(ctx: Context, desc) -> BuiltScene:
              b"Light Source", b"Rec709(D65)", C.byref(img))
         env_node = ctx.shader_node("EnvironmentTexture")
         call("vlrQueryableSetImage2D", env_node, b"image", img)
+    node_cache = {}
     for m in desc.materials:
-        bs.materials.append(build_material(ctx, m, env_node))
+        bs.materials.append(build_material(ctx, m, env_node, node_cache, bs.keep))
     for mesh in desc.meshes:
         n = H()
         v = np.ascontiguousarray(mesh.vertices, np.float32)
@@ -655,10 +701,14 @@ def build_scene(ctx: Context, desc) -> BuiltScene:
         else:
             call("vlrTriangleMeshSurfaceNodeCreate", ctx.h, _b(mesh.name), C.byref(n))
             call("vlrTriangleMeshSurfaceNodeSetVertices", n, v.ctypes.data, len(v))
-            for idx, mi in mesh.groups:
+            for gi, (idx, mi) in enumerate(mesh.groups):
                 ix = np.ascontiguousarray(idx, np.uint32)
+                plugs = [NULL_PLUG, NULL_PLUG]
+                for k, p in enumerate(getattr(mesh, "group_nodes", {}).get(gi, (None, None))):
+                    if p is not None:
+                        plugs[k] = node_plug(ctx, p[0], p[1], p[2], node_cache, bs.keep)
                 call("vlrTriangleMeshSurfaceNodeAddMaterialGroup", n, ix.ctypes.data, len(ix), bs.materials[mi],
-                     NULL_PLUG, NULL_PLUG, NULL_PLUG)
+                     plugs[0], plugs[1], NULL_PLUG)
         bs.meshes.append(n)
     ident = ctx.transform(np.eye(4, dtype=np.float32).reshape(16, order="F"))
     scene = H()

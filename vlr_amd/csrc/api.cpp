@@ -782,11 +782,14 @@ VLR_API VLRResult vlrTriangleMeshSurfaceNodeAddMaterialGroup(VLRTriangleMeshSurf
     MaterialGroup g;
     g.indices.assign(indices, indices + numIndices);
     g.material = static_cast<const SurfaceMaterial *>(obj(material));
+    // scene.cpp:225-246: a normal or tangent plug of an inconvertible type is ignored
     g.nodeNormal = toPlug(nodeNormal);
+    if (g.nodeNormal.valid() && !plugConvertible(SocketType::Normal3D, g.nodeNormal.plugType)) g.nodeNormal = PlugRef();
     g.nodeTangent = toPlug(nodeTangent);
+    if (g.nodeTangent.valid() && !plugConvertible(SocketType::Vector3D, g.nodeTangent.plugType)) g.nodeTangent = PlugRef();
     g.nodeAlpha = toPlug(nodeAlpha);
-    if (g.nodeNormal.valid() || g.nodeTangent.valid() || g.nodeAlpha.valid()) {
-        g_lastError = "normal/tangent/alpha shader nodes are not on the render path";
+    if (g.nodeAlpha.valid()) {
+        g_lastError = "alpha shader nodes are not on the render path";
         return VLRResult_IncompatibleNodeType;
     }
     n->groups.push_back(std::move(g));
@@ -1511,6 +1514,19 @@ VLR_API VLRResult vlrContextBuildBvh4(VLRContext context, hipStream_t stream, co
         VLRA_HIP(hipMemcpyAsync(order, B.order(), size_t(numPrims) * 4, hipMemcpyDeviceToDevice, stream));
         VLRA_HIP(hipStreamSynchronize(stream));
     }
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+
+VLR_API VLRResult vlrContextEvaluateShaderNodePlug(VLRContext context, hipStream_t stream, VLRShaderNodePlug plug,
+                                                   bool spectral, const float *points, uint32_t numPoints,
+                                                   float *values) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    if ((!points || !values) && numPoints) return VLRResult_InvalidArgument;
+    const PlugRef r = toPlug(plug);
+    if (!r.valid()) return VLRResult_InvalidArgument;
+    CTX(context)->evaluateNodePlug(stream, r, spectral, points, numPoints, values);
     return VLRResult_NoError;
     VLRA_CATCH
 }

@@ -722,6 +722,8 @@ void Context::renderBatch(hipStream_t userStream, const std::vector<BatchJobDesc
     L.jobs = static_cast<DevJob *>(b.jobs.ptr);
     L.jobOffsets = static_cast<const uint32_t *>(b.jobOffsets.ptr);
     L.numJobs = static_cast<uint32_t>(jobs.size());
+    L.nodePrograms = false;
+    for (const auto &kv : b.scenes) L.nodePrograms = L.nodePrograms || kv.second->dev.hasNodePrograms;
 
     // schedule candidates (wavefront depth; the tail grid fixed), each timed on one step when
     // autotuning is on and there are enough steps; otherwise the context's schedule
@@ -854,13 +856,54 @@ void Context::readAccel(const Scene &s, uint32_t which, void *dst, size_t capaci
     else if (n) VLRA_HIP(hipMemcpy(dst, src, n, hipMemcpyDeviceToHost));
 }
 
+void Context::evaluateNodePlug(hipStream_t stream, const PlugRef &plug, bool spectral, const float *points, uint32_t n,
+                               float *values) {
+    ensureDevice();
+    if (!loadSpectralTables()) throw std::runtime_error("spectral tables not found next to the library");
+    ShaderTable T;
+    T.spectral = spectral;
+    const uint32_t out = plug.plugType == VLRShaderNodePlugType_Spectrum ? 1u : 8u;
+    const auto prog = T.program({{plug, out}});
+    if (T.nodes.empty()) T.nodes.emplace_back();
+    if (T.steps.empty()) T.steps.emplace_back();
+    if (T.textures.empty()) T.textures.emplace_back();
+    if (T.texels.empty()) T.texels.push_back(make_float4(0, 0, 0, 0));
+    if (T.pool.empty()) T.pool.push_back(0.0f);
+    DevInstance ident;
+    std::memset(&ident, 0, sizeof(ident));
+    for (int r = 0; r < 3; ++r) ident.o2w[r * 4 + r] = ident.w2o[r * 4 + r] = 1.0f;
+    DeviceBuffer nodesB, stepsB, texB, texelsB, poolB, instB, gridB, pointsB;
+    nodesB.upload(T.nodes.data(), T.nodes.size() * sizeof(DevShaderNode));
+    stepsB.upload(T.steps.data(), T.steps.size() * sizeof(DevNodeStep));
+    texB.upload(T.textures.data(), T.textures.size() * sizeof(DevTexture));
+    texelsB.upload(T.texels.data(), T.texels.size() * sizeof(float4));
+    poolB.upload(T.pool.data(), T.pool.size() * sizeof(float));
+    instB.upload(&ident, sizeof(ident));
+    gridB.upload(mengGrid().data(), mengGrid().size() * sizeof(MengCellDev));
+    pointsB.upload(mengPoints().data(), mengPoints().size() * sizeof(float));
+    DevScene sc;
+    std::memset(&sc, 0, sizeof(sc));
+    sc.nodes = static_cast<const DevShaderNode *>(nodesB.ptr);
+    sc.nodeSteps = static_cast<const DevNodeStep *>(stepsB.ptr);
+    sc.textures = static_cast<const DevTexture *>(texB.ptr);
+    sc.texels = static_cast<const float4 *>(texelsB.ptr);
+    sc.nodePool = static_cast<const float *>(poolB.ptr);
+    sc.instances = static_cast<const DevInstance *>(instB.ptr);
+    sc.mengGrid = static_cast<const MengCellDev *>(gridB.ptr);
+    sc.mengPoints = static_cast<const float *>(pointsB.ptr);
+    sc.numInstances = 1;
+    sc.hasNodePrograms = 1;
+    VLRA_HIP(launchEvalNodePlug(sc, prog.first, prog.second, plug.plugType, spectral, points, n, values, stream));
+    VLRA_HIP(hipStreamSynchronize(stream));
+}
+
 void Context::evaluateBSDFs(hipStream_t stream, const SurfaceMaterial &m, bool spectral, const void *q, uint32_t n,
                             void *r) {
     if (!m.supportedOnRenderPath()) throw ApiError{4, "material type not on the render path"};
     ensureDevice();
     bool usesTex;
     const Image2D *img;
-    DevMaterial dm = m.toDevice(spectral, &usesTex, &img);
+    DevMaterial dm = m.toDevice(spectral, &usesTex, &img, nullptr);
     // the BSDF kernels only read the Meng tables from the scene record
     if (!gpu.mengPoints.ptr) {
         gpu.mengGrid.upload(mengGrid().data(), mengGrid().size() * sizeof(MengCellDev));

@@ -844,66 +844,17 @@ struct Bsdf {
     }
 };
 
-// Material set-up at a surface point (the *_setupBSDF callables of materials.cu)
-template <int N>
-__device__ __forceinline__ Bsdf<N> setupBsdf(const DevScene &sc, const DevMaterial &m, const Wls<N> &wls) {
-    Bsdf<N> b;
-    b.kind = BSDF_NULL;
-    b.a0 = b.a1 = b.a2 = 0.0f;
-    b.dispersive = false;
-    switch (m.type) {
-    case MAT_MATTE:
-        b.kind = BSDF_MATTE;
-        b.c0 = evalSpectrum<N>(sc, m.spec[0], wls);
-        break;
-    case MAT_SPECULAR_REFLECTION:
-        b.kind = BSDF_SPEC_BRDF;
-        b.c0 = evalSpectrum<N>(sc, m.spec[0], wls);
-        b.c1 = evalSpectrum<N>(sc, m.spec[1], wls);
-        b.c2 = evalSpectrum<N>(sc, m.spec[2], wls);
-        break;
-    case MAT_SPECULAR_SCATTERING:
-        b.kind = BSDF_SPEC_BSDF;
-        b.c0 = evalSpectrum<N>(sc, m.spec[0], wls);
-        b.c1 = evalSpectrum<N>(sc, m.spec[1], wls);
-        b.c2 = evalSpectrum<N>(sc, m.spec[2], wls);
-        b.dispersive = !wls.single;
-        break;
-    case MAT_MICROFACET_REFLECTION:
-        // alphaX, alphaY and rotation were derived on the host (materials.cu:1230-1236)
-        b.kind = BSDF_MICRO_BRDF;
-        b.c0 = evalSpectrum<N>(sc, m.spec[0], wls);
-        b.c1 = evalSpectrum<N>(sc, m.spec[1], wls);
-        b.a0 = m.scalars[0];
-        b.a1 = m.scalars[1];
-        b.a2 = m.scalars[2];
-        break;
-    case MAT_MICROFACET_SCATTERING:
-        // alphaX, alphaY and rotation were derived on the host (materials.cu:1516-1536)
-        b.kind = BSDF_MICRO_BSDF;
-        b.c0 = evalSpectrum<N>(sc, m.spec[0], wls);
-        b.c1 = evalSpectrum<N>(sc, m.spec[1], wls);
-        b.c2 = evalSpectrum<N>(sc, m.spec[2], wls);
-        b.a0 = m.scalars[0];
-        b.a1 = m.scalars[1];
-        b.a2 = m.scalars[2];
-        break;
-    case MAT_UE4: {
-        // materials.cu:1959-1978
-        b.kind = BSDF_DIFFSPEC;
-        Spec<N> base = evalSpectrum<N>(sc, m.spec[0], wls);
-        float roughness = fmaxf(0.01f, m.scalars[1]);
-        float metallic = m.scalars[2];
-        const float specular = 0.5f;
-        b.c0 = base * (1 - metallic);
-        b.c1 = lerpSpec<N>(Spec<N>(0.08f * specular), base, metallic);
-        b.a0 = roughness;
-        break;
-    }
-    default:
-        break;
-    }
-    return b;
-}
+// ---------------------------------------------------------------------------------------------
+// surface points
+struct SurfPt {
+    V3 position, gn;
+    Frame frame;
+    float texU, texV;
+    bool isPoint, atInfinity;
+    // hit identity and surface parameters (barycentrics b0, b1; phi, theta at infinity)
+    uint32_t instIndex, geomInstIndex, primIndex;
+    float u, v;
+    __device__ __forceinline__ float cosTerm(const V3 &w) const { return isPoint ? 1.0f : absDot(w, gn); }
+};
 
 } // namespace vlra

@@ -104,6 +104,10 @@ struct alignas(16) DevSpectrum {
     float x, y, z, w;
 };
 
+// Material record.  spec[k] / scalars are the immediate parameters.  Parameters connected to
+// shader nodes are computed per hit by the node program of progCount steps at progBase
+// (outputs: spec[0..2] and the float3 parameter: roughness / anisotropy / rotation or
+// occlusion / roughness / metallic, whose immediates are raw[0..2]).
 struct alignas(16) DevMaterial {
     uint32_t type;
     uint32_t useEnvTexture;
@@ -111,8 +115,77 @@ struct alignas(16) DevMaterial {
     uint32_t pad;
     float scalars[4];
     DevSpectrum spec[3];
+    uint32_t progBase, progCount, pad2[2];
+    float raw[4];
 };
-static_assert(sizeof(DevMaterial) == 80, "material layout");
+static_assert(sizeof(DevMaterial) == 112, "material layout");
+
+// ---- shader nodes (shader_nodes.cu:20-483) ----
+// A plug: 0 = not connected, else (node index + 1) << 8 | plug type << 4 | option, plug types
+// as VLRShaderNodePlugType (float1..4, Point3D, Vector3D, Normal3D, Spectrum, Alpha,
+// TextureCoordinates).
+enum DevPlugType : uint32_t {
+    PLUG_FLOAT1 = 0, PLUG_FLOAT2, PLUG_FLOAT3, PLUG_FLOAT4, PLUG_POINT3D, PLUG_VECTOR3D, PLUG_NORMAL3D, PLUG_SPECTRUM,
+    PLUG_ALPHA, PLUG_TEXCOORD,
+};
+__host__ __device__ inline uint32_t makePlug(uint32_t node, uint32_t type, uint32_t option) {
+    return ((node + 1) << 8) | (type << 4) | option;
+}
+__host__ __device__ inline uint32_t plugNode(uint32_t p) { return (p >> 8) - 1; }
+__host__ __device__ inline uint32_t plugType(uint32_t p) { return (p >> 4) & 0xF; }
+__host__ __device__ inline uint32_t plugOption(uint32_t p) { return p & 0xF; }
+
+// Node kinds in the order of the C API's type names.
+enum DevNodeKind : uint32_t {
+    NODE_GEOMETRY = 0, NODE_TANGENT, NODE_FLOAT2, NODE_FLOAT3, NODE_FLOAT4, NODE_SCALE_OFFSET_FLOAT,
+    NODE_TRIPLET_SPECTRUM, NODE_REGULAR_SPECTRUM, NODE_IRREGULAR_SPECTRUM, NODE_FLOAT3_TO_SPECTRUM, NODE_UV_MAP,
+    NODE_IMAGE_TEXTURE, NODE_ENV_TEXTURE,
+};
+
+// One node (64 B): its kind and immediates (the inputs it reads are in the node-program
+// steps).  Per kind:
+//   TANGENT             a[0] tangent type (0 TC0 direction, 1..3 radial about x, y, z)
+//   FLOAT2/3/4          f[k]: component k's immediate (input k)
+//   SCALE_OFFSET_FLOAT  inputs value, scale, offset; f[1], f[2] immediates of scale, offset
+//   TRIPLET_SPECTRUM    f[0..3] the spectrum parameter (DevSpectrum) of the render mode
+//   REGULAR_SPECTRUM    spectral build: a[0] pool offset of the values, a[1] count, f[0..1]
+//                       wavelength range; RGB build: f[0..3] the converted triplet, a[2] = 1
+//   IRREGULAR_SPECTRUM  spectral build: a[0] pool offset of wavelengths then values, a[1]
+//                       count; RGB build as REGULAR
+//   FLOAT3_TO_SPECTRUM  input 0 float3, f[0..2] its immediate, a[0] spectrum type, a[1] colour space
+//   UV_MAP              f[0..1] scale, f[2..3] offset
+//   IMAGE_TEXTURE       input 0 texture coordinates, a[0] texture, a[1] filter | wrap u << 2 |
+//                       wrap v << 4 | bump type << 8, f[0] bump coefficient
+struct alignas(16) DevShaderNode {
+    uint32_t kind;
+    uint32_t pad[4];
+    uint32_t a[3];
+    float f[8];
+};
+static_assert(sizeof(DevShaderNode) == 64, "shader node layout");
+
+// One step of a node program: evaluates `node`'s program for plug type (plug & 15) and option
+// (plug >> 4) into slot i of the program, reading its inputs from earlier slots (in[k] = slot
+// + 1, 0 = not connected: the node's immediate).  out: bit k (k < 3) sends the step's spectrum
+// to output spectrum k, bit 3 / 4 its value to output vector 0 / 1.  Programs list the nodes of
+// a graph in post-order, at most kNodeProgramSlots steps.
+constexpr uint32_t kNodeProgramSlots = 8;
+struct alignas(16) DevNodeStep {
+    uint32_t node, plug, out, pad;
+    uint32_t in[4];
+};
+static_assert(sizeof(DevNodeStep) == 32, "node step layout");
+
+// A texture: width x height float4 texels from `offset` in the texel pool, prepared on the
+// host for the render mode (linear RGBA, or (u, v, s, a) grid coordinates of upsampled
+// spectra: format 1 from 8-bit sources as u / 12, v / 14, s / 3, format 2 from half sources).
+constexpr uint32_t kTexRGBA = 0, kTexUVS8 = 1, kTexUVS16 = 2;
+struct alignas(16) DevTexture {
+    uint32_t offset, width, height, format;
+    uint32_t gray;            // one-channel source: the spectrum uses x for all three channels
+    uint32_t spectrumType, colorSpace, pad;
+};
+static_assert(sizeof(DevTexture) == 32, "texture layout");
 
 struct alignas(16) DevGeomInst {
     uint32_t material;
@@ -123,8 +196,10 @@ struct alignas(16) DevGeomInst {
     float distIntegral;
     float importance;
     uint32_t isPoints;
+    // node program of the material group's normal (output vec 0) and tangent (vec 1) nodes
+    uint32_t progBase, progCount, pad2[2];
 };
-static_assert(sizeof(DevGeomInst) == 32, "geomInst layout");
+static_assert(sizeof(DevGeomInst) == 48, "geomInst layout");
 
 struct MengCellDev { uint8_t inside, numPoints, idx[6]; };
 
@@ -152,8 +227,14 @@ struct DevScene {
     const MengCellDev* mengGrid;
     const float* mengPoints;     // 186 x 99 floats: xy*(2) uv(2) spectrum(95)
     const float4* cacheMaterials; // per material: (diffuse rgb, roughness), (specular rgb, 0) for the radiance cache
+    const DevShaderNode* nodes;   // shader nodes reachable from the scene's materials and groups
+    const DevNodeStep* nodeSteps; // node programs of materials and material groups
+    const DevTexture* textures;
+    const float4* texels;
+    const float* nodePool;        // sampled-spectrum tables of nodes
     uint32_t numInstances;
     uint32_t hasTlas;
+    uint32_t hasNodePrograms;    // some material or material group has a node program
     // light selection over [environment, instances...]
     uint32_t lightInstBase;
     uint32_t numLightInsts;

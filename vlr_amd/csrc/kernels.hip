@@ -22,6 +22,7 @@
 // Wald, JCGT 2(1) 2013), compiled without floating-point contraction so that closest hits
 // reproduce the oracle's brute force bit for bit.
 #include "dev_shading.h"
+#include "shader_nodes.h"
 #include "launch.h"
 #include "radiance_cache.h"
 
@@ -403,20 +404,8 @@ __device__ __forceinline__ Spec<N> envEmittance(const DevScene &sc, const DevMat
     return e * m.scalars[0];
 }
 
-// ---------------------------------------------------------------------------------------------
-// surface points
-struct SurfPt {
-    V3 position, gn;
-    Frame frame;
-    float texU, texV;
-    bool isPoint, atInfinity;
-    // hit identity and surface parameters (barycentrics b0, b1; phi, theta at infinity)
-    uint32_t instIndex, geomInstIndex, primIndex;
-    float u, v;
-    __device__ __forceinline__ float cosTerm(const V3 &w) const { return isPoint ? 1.0f : absDot(w, gn); }
-};
-
 // triangle.cu:6-53 + renderer_common.h:609-626: decode in object space, then to world
+template <bool NODES = true>
 __device__ __forceinline__ void decodeHit(const DevScene &sc, int32_t instIdx, int32_t giIdx, int32_t prim, float b1,
                                           float b2, SurfPt *sp, float *hypAreaPDF) {
     const DevInstance &I = sc.instances[instIdx];
@@ -454,6 +443,7 @@ __device__ __forceinline__ void decodeHit(const DevScene &sc, int32_t instIdx, i
     sp->primIndex = static_cast<uint32_t>(prim);
     sp->u = b0;
     sp->v = b1;
+    if constexpr (NODES) applyGroupNodes(sc, G, sp);   // normal and tangent nodes (renderer_common.h:620-625)
 }
 
 // environment surface point from a direction (path_tracing.cu:282-310)
@@ -586,18 +576,26 @@ __device__ __forceinline__ void sampleLight(const DevScene &sc, float uLight, fl
 
 // EDF of an emitter material at a surface point (materials.cu:2040-2096, 2194-2250, 2830-2896):
 // returns emittance M (evaluateEmittance) and the directional factor for dirLocal.
-template <int N>
+template <int N, bool NODES = true>
 __device__ __forceinline__ bool evalEDF(const DevScene &sc, const DevMaterial &m, const SurfPt &sp, const Wls<N> &wls,
                                         const V3 &dirLocal, Spec<N> *M, float *fe) {
     switch (m.type) {
-    case MAT_DIFFUSE_EMITTER:
-        *M = evalSpectrum<N>(sc, m.spec[0], wls) * m.scalars[0];
+    case MAT_DIFFUSE_EMITTER: {
+        NodeOutputs<N> o;
+        if constexpr (NODES) materialNodes<N>(sc, m, &sp, wls, &o);
+        else o.has = 0;
+        *M = materialSpectrum<N>(sc, m, 0, o, wls) * m.scalars[0];
         *fe = dirLocal.z > 0.0f ? 1.0f / kPiF : 0.0f;
         return true;
-    case MAT_POINT_EMITTER:
-        *M = evalSpectrum<N>(sc, m.spec[0], wls) * m.scalars[0];
+    }
+    case MAT_POINT_EMITTER: {
+        NodeOutputs<N> o;
+        if constexpr (NODES) materialNodes<N>(sc, m, &sp, wls, &o);
+        else o.has = 0;
+        *M = materialSpectrum<N>(sc, m, 0, o, wls) * m.scalars[0];
         *fe = 1.0f / (4 * kPiF);
         return true;
+    }
     case MAT_ENVIRONMENT_EMITTER:
         *M = kPiF * envEmittance<N>(sc, m, sp.texU, sp.texV, wls);
         *fe = dirLocal.z > 0.0f ? 1.0f / kPiF : 0.0f;
@@ -913,7 +911,7 @@ __device__ __forceinline__ void writeCacheRecord(const DevScene &sc, const SurfP
 // One bounce of path_tracing.cu:133-337 for a ray that hit (ha.x >= 0) or missed: implicit
 // emission with MIS, Russian roulette, next-event estimation (returns the shadow ray and the
 // contribution it carries when unoccluded) and BSDF sampling (returns the next ray).
-template <int N>
+template <int N, bool NODES = true>
 __device__ __forceinline__ void shadeStep(const DevScene &sc, uint32_t pathLength, uint32_t maxPathLength,
                                           uint32_t path, const V3 &rayOrg, const V3 &rayDir, int4 ha, float2 hb,
                                           PathRegs<N> &P, const CacheCtx &cc, bool *emitShadow, float4 *shOrg,
@@ -942,7 +940,7 @@ __device__ __forceinline__ void shadeStep(const DevScene &sc, uint32_t pathLengt
             V3 dirOutLocal = sp.frame.toLocal(-rayDir);
             Spec<N> M;
             float fe;
-            evalEDF<N>(sc, m, sp, P.wls, dirOutLocal, &M, &fe);
+            evalEDF<N, NODES>(sc, m, sp, P.wls, dirOutLocal, &M, &fe);
             if (!M.isZero()) {
                 float mis = 1.0f;
                 if (!dt::isDelta(P.prevType) && pathLength > 1) {
@@ -958,14 +956,14 @@ __device__ __forceinline__ void shadeStep(const DevScene &sc, uint32_t pathLengt
     }
     SurfPt sp;
     float hypAreaPDF;
-    decodeHit(sc, ha.x, ha.y, ha.z, hb.x, hb.y, &sp, &hypAreaPDF);
+    decodeHit<NODES>(sc, ha.x, ha.y, ha.z, hb.x, hb.y, &sp, &hypAreaPDF);
     const DevGeomInst &G = sc.geomInsts[ha.y];
     const DevMaterial &mat = sc.materials[G.material];
     V3 dirOutLocal = sp.frame.toLocal(-rayDir);
     if (mat.isEmitter) {
         Spec<N> M;
         float fe;
-        evalEDF<N>(sc, mat, sp, P.wls, dirOutLocal, &M, &fe);
+        evalEDF<N, NODES>(sc, mat, sp, P.wls, dirOutLocal, &M, &fe);
         if (!M.isZero()) {
             float mis = 1.0f;
             if (!dt::isDelta(P.prevType) && pathLength > 1) {
@@ -1013,7 +1011,7 @@ __device__ __forceinline__ void shadeStep(const DevScene &sc, uint32_t pathLengt
         if (P.rng.uniform() >= continueProb) return;
         P.alpha = P.alpha / continueProb;
     }
-    Bsdf<N> bsdf = setupBsdf<N>(sc, mat, P.wls);
+    Bsdf<N> bsdf = setupBsdf<N, NODES>(sc, mat, P.wls, &sp);
     V3 gnL = sp.frame.toLocal(sp.gn);
     if (bsdf.hasNonDelta()) {
         float uLight = P.rng.uniform();
@@ -1038,7 +1036,7 @@ __device__ __forceinline__ void shadeStep(const DevScene &sc, uint32_t pathLengt
             V3 sdl = ls.sp.frame.toLocal(-sdir);
             Spec<N> M;
             float fe;
-            evalEDF<N>(sc, lm, ls.sp, P.wls, sdl, &M, &fe);
+            evalEDF<N, NODES>(sc, lm, ls.sp, P.wls, sdl, &M, &fe);
             if (!M.isZero()) {
                 float rec = 1.0f / dist2;
                 V3 sdsn = sp.frame.toLocal(sdir);
@@ -1108,7 +1106,7 @@ __device__ __forceinline__ void storePath(const FrameBuffers &fb, uint32_t path,
     }
 }
 
-template <int N, bool MULTI = false>
+template <int N, bool MULTI = false, bool NODES = true>
 __global__ void __launch_bounds__(256) k_shade(KArgs a) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t depth = a.depth;
@@ -1126,10 +1124,10 @@ __global__ void __launch_bounds__(256) k_shade(KArgs a) {
         loadPath<N>(a.fb, path, &P);
         if constexpr (MULTI) {
             const DevJob &J = a.jobs[jobOf(a, path)];
-            shadeStep<N>(J.scene, depth + 1, J.maxPathLength, path, xyz(o4), xyz(d4), ha, hb, P, cacheCtx(a.fp, a.fb),
+            shadeStep<N, NODES>(J.scene, depth + 1, J.maxPathLength, path, xyz(o4), xyz(d4), ha, hb, P, cacheCtx(a.fp, a.fb),
                          &emitShadow, &shOrg, &shDir, &shVal, &emitNext, &nextOrg, &nextDir);
         } else {
-            shadeStep<N>(a.sc, depth + 1, a.fp.maxPathLength, path, xyz(o4), xyz(d4), ha, hb, P, cacheCtx(a.fp, a.fb),
+            shadeStep<N, NODES>(a.sc, depth + 1, a.fp.maxPathLength, path, xyz(o4), xyz(d4), ha, hb, P, cacheCtx(a.fp, a.fb),
                          &emitShadow, &shOrg, &shDir, &shVal, &emitNext, &nextOrg, &nextDir);
         }
         storePath<N>(a.fb, path, depth + 1, P, emitNext);
@@ -1182,7 +1180,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_shadow(KArgs a) {
 // random numbers and adds its contributions in the same order as the per-depth kernels, so a
 // frame's result does not depend on the depth at which paths switch to this kernel.  Rays per
 // depth are counted in LDS and added to the frame counters once per block.
-template <int N, bool STATS, bool MULTI = false>
+template <int N, bool STATS, bool MULTI = false, bool NODES = true>
 __global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kTailWavesPerEU)))
 k_tail(KArgs a) {
     __shared__ int32_t lds[kTraceLds];
@@ -1238,7 +1236,7 @@ k_tail(KArgs a) {
         int4 ha = found ? make_int4(h.inst, h.geomInst, h.prim, 0) : make_int4(-1, -1, -1, 0);
         bool emitShadow, emitNext;
         float4 shOrg, shDir, shVal, nextOrg, nextDir;
-        shadeStep<N>(psc, depth + 1, maxLen, path, org, dir, ha, make_float2(h.b1, h.b2), P, cc, &emitShadow,
+        shadeStep<N, NODES>(psc, depth + 1, maxLen, path, org, dir, ha, make_float2(h.b1, h.b2), P, cc, &emitShadow,
                      &shOrg, &shDir, &shVal, &emitNext, &nextOrg, &nextDir);
         if (emitShadow) {
             atomicAdd(&depthCounts[2 * depth + 1], 1u);
@@ -1574,6 +1572,7 @@ __device__ __forceinline__ LVert loadLV(const float4 *src) {
 
 // decodeHitPointForTriangle (triangle.cu:55-109), ForPoint (point.cu:6-35), ForInfiniteSphere
 // (infinite_sphere.cu:17-56); returns the material index
+template <bool NODES = true>
 __device__ __forceinline__ uint32_t decodeLightVertex(const DevScene &sc, const LVert &lv, SurfPt *sp) {
     sp->instIndex = lv.inst;
     sp->geomInstIndex = lv.geomInst;
@@ -1647,6 +1646,7 @@ __device__ __forceinline__ uint32_t decodeLightVertex(const DevScene &sc, const 
     sp->gn = gn;
     sp->u = b0;
     sp->v = b1;
+    if constexpr (NODES) applyGroupNodes(sc, G, sp);
     return G.material;
 }
 
@@ -1731,7 +1731,7 @@ __device__ __forceinline__ Wls<N> commonWls(const BptParams &bp, bool single) {
 }
 
 // ---- light subpaths (lvc_bpt.cu:144-334) ----
-template <int N>
+template <int N, bool NODES = true>
 __global__ void __launch_bounds__(kTraceBlock) k_bptLight(BptArgs a) {
     __shared__ int32_t lds[kTraceLds];
     const DevScene &sc = a.sc;
@@ -1759,7 +1759,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_bptLight(BptArgs a) {
             const DevMaterial &lm = sc.materials[ls.material];
             Spec<N> M;
             float fe;
-            evalEDF<N>(sc, lm, ls.sp, wlsC, V3(0, 0, 1), &M, &fe);
+            evalEDF<N, NODES>(sc, lm, ls.sp, wlsC, V3(0, 0, 1), &M, &fe);
             float probDensity0 = bp.numLightPaths * lightProb * ls.areaPDF;
             Spec<N> alpha = M / probDensity0;
             store(ls.sp, alpha, V3(0, 0, 1), 0.0f, probDensity0, 1.0f, 0.0f, 1.0f, dt::isDelta(ls.posType), false,
@@ -1819,9 +1819,9 @@ __global__ void __launch_bounds__(kTraceBlock) k_bptLight(BptArgs a) {
                     const Wls<N> wls = commonWls<N>(bp, single);
                     SurfPt sp;
                     float hypAreaPDF;
-                    decodeHit(sc, h.inst, h.geomInst, h.prim, h.b1, h.b2, &sp, &hypAreaPDF);
+                    decodeHit<NODES>(sc, h.inst, h.geomInst, h.prim, h.b1, h.b2, &sp, &hypAreaPDF);
                     const DevGeomInst &G = sc.geomInsts[h.geomInst];
-                    Bsdf<N> bsdf = setupBsdf<N>(sc, sc.materials[G.material], wls);
+                    Bsdf<N> bsdf = setupBsdf<N, NODES>(sc, sc.materials[G.material], wls, &sp);
                     V3 dirInLocal = sp.frame.toLocal(-rayDir);
                     V3 gnLocal = sp.frame.toLocal(sp.gn);
                     bool secondLastValid = (!roPrevDelta && !roSecondPrevDelta) && pathLength > 2;
@@ -1954,7 +1954,7 @@ __global__ void __launch_bounds__(256) k_bptCompact(BptParams bp) {
 }
 
 // ---- eye subpaths (lvc_bpt.cu:338-971) ----
-template <int N>
+template <int N, bool NODES = true>
 __global__ void __launch_bounds__(kTraceBlock) k_bptEye(BptArgs a) {
     __shared__ int32_t lds[kTraceLds];
     const DevScene &sc = a.sc;
@@ -2000,14 +2000,14 @@ __global__ void __launch_bounds__(kTraceBlock) k_bptEye(BptArgs a) {
             uint32_t idx = min(static_cast<uint32_t>(numLightVertices * rng.uniform()), numLightVertices - 1);
             LVert vtx = loadLV(bp.cache + static_cast<size_t>(idx) * kLVF4);
             SurfPt spL;
-            uint32_t matL = decodeLightVertex(sc, vtx, &spL);
+            uint32_t matL = decodeLightVertex<NODES>(sc, vtx, &spL);
             V3 conRayDir;
             float sqDist;
             ++shadowRays;
             if (connectVisible(sc, spE0, spL, &conRayDir, &sqDist, lds, threadIdx.x)) {
                 float recSq = 1.0f / sqDist;
                 const DevMaterial &mL = sc.materials[matL];
-                Bsdf<N> bL = vtx.pathLength == 0 ? edfAsBsdf<N>(sc, mL.type) : setupBsdf<N>(sc, mL, wlsC);
+                Bsdf<N> bL = vtx.pathLength == 0 ? edfAsBsdf<N>(sc, mL.type) : setupBsdf<N, NODES>(sc, mL, wlsC, &spL);
                 if (bL.hasNonDelta()) {
                     V3 gnLocalL = spL.frame.toLocal(spL.gn);
                     V3 conLocalL = spL.frame.toLocal(-conRayDir);
@@ -2110,7 +2110,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_bptEye(BptArgs a) {
                     probDensity = roDirPDF / lastDist2;
                     Spec<N> M;
                     float fe;
-                    evalEDF<N>(sc, sc.materials[sc.envMaterial], spE, wls, dirOutLocalE, &M, &fe);
+                    evalEDF<N, NODES>(sc, sc.materials[sc.envMaterial], spE, wls, dirOutLocalE, &M, &fe);
                     if (!M.isZero()) {
                         Spec<N> unweighted = alpha * (M * fe);
                         if (single) unweighted = unweighted * static_cast<float>(N);
@@ -2137,10 +2137,10 @@ __global__ void __launch_bounds__(kTraceBlock) k_bptEye(BptArgs a) {
             // closest-hit program for eye paths (lvc_bpt.cu:570-837)
             SurfPt spE;
             float hypAreaPDF;
-            decodeHit(sc, h.inst, h.geomInst, h.prim, h.b1, h.b2, &spE, &hypAreaPDF);
+            decodeHit<NODES>(sc, h.inst, h.geomInst, h.prim, h.b1, h.b2, &spE, &hypAreaPDF);
             const DevGeomInst &G = sc.geomInsts[h.geomInst];
             const DevMaterial &matE = sc.materials[G.material];
-            Bsdf<N> bsdfE = setupBsdf<N>(sc, matE, wls);
+            Bsdf<N> bsdfE = setupBsdf<N, NODES>(sc, matE, wls, &spE);
             V3 dirOutLocalE = spE.frame.toLocal(-rayDir);
             V3 gnLocalE = spE.frame.toLocal(spE.gn);
             bool secondLastValidE = (!roPrevDelta && !roSecondPrevDelta) && pathLength > 2;
@@ -2152,7 +2152,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_bptEye(BptArgs a) {
             if (matE.isEmitter) {
                 Spec<N> M;
                 float fe;
-                evalEDF<N>(sc, matE, spE, wls, dirOutLocalE, &M, &fe);
+                evalEDF<N, NODES>(sc, matE, spE, wls, dirOutLocalE, &M, &fe);
                 if (!M.isZero()) {
                     Spec<N> unweighted = alpha * (M * fe);
                     if (single) unweighted = unweighted * static_cast<float>(N);
@@ -2181,14 +2181,14 @@ __global__ void __launch_bounds__(kTraceBlock) k_bptEye(BptArgs a) {
                 uint32_t idx = min(static_cast<uint32_t>(numLightVertices * rng.uniform()), numLightVertices - 1);
                 LVert vtx = loadLV(bp.cache + static_cast<size_t>(idx) * kLVF4);
                 SurfPt spL;
-                uint32_t matL = decodeLightVertex(sc, vtx, &spL);
+                uint32_t matL = decodeLightVertex<NODES>(sc, vtx, &spL);
                 V3 conRayDir;
                 float sqDist;
                 ++shadowRays;
                 if (connectVisible(sc, spE, spL, &conRayDir, &sqDist, lds, threadIdx.x)) {
                     float recSq = 1.0f / sqDist;
                     const DevMaterial &mL = sc.materials[matL];
-                    Bsdf<N> bL = vtx.pathLength == 0 ? edfAsBsdf<N>(sc, mL.type) : setupBsdf<N>(sc, mL, wls);
+                    Bsdf<N> bL = vtx.pathLength == 0 ? edfAsBsdf<N>(sc, mL.type) : setupBsdf<N, NODES>(sc, mL, wls, &spL);
                     if (bL.hasNonDelta()) {
                         V3 gnLocalL = spL.frame.toLocal(spL.gn);
                         V3 conLocalL = spL.frame.toLocal(-conRayDir);
@@ -2402,7 +2402,7 @@ __global__ void k_bsdfQueries(DevScene sc, DevMaterial m, const BsdfQueryIn *q, 
     wls.sel = min(static_cast<uint32_t>(static_cast<uint16_t>(N * Q.wlSelect)), static_cast<uint32_t>(N - 1));
     wls.single = Q.single != 0;
     const bool importance = Q.transportMode != 0;
-    Bsdf<N> b = setupBsdf<N>(sc, m, wls);
+    Bsdf<N> b = setupBsdf<N>(sc, m, wls, static_cast<const SurfPt *>(nullptr));
     V3 qd(Q.dirLocal[0], Q.dirLocal[1], Q.dirLocal[2]);
     V3 gn(Q.gnLocal[0], Q.gnLocal[1], Q.gnLocal[2]);
     V3 dir(0, 0, 0);
@@ -2466,7 +2466,10 @@ static hipError_t launchFrameT(const DevScene &sc, const FrameParams &fp, const 
         t1(STAGE_INTERSECT);
         t0(STAGE_SHADE);
         if (fp.sortShade) hipLaunchKernelGGL(k_sortScatter, dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
-        hipLaunchKernelGGL(k_shade<N>, dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
+        if (sc.hasNodePrograms)
+            hipLaunchKernelGGL((k_shade<N, false, true>), dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_shade<N, false, false>), dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
         t1(STAGE_SHADE);
         t0(STAGE_SHADOW);
         if (fp.traversalStats)
@@ -2483,10 +2486,14 @@ static hipError_t launchFrameT(const DevScene &sc, const FrameParams &fp, const 
         a.depth = wavefrontDepths;
         const uint32_t blocks = std::max(1u, std::min(blocksFor(n, kTraceBlock), fp.tailBlocks));
         t0(STAGE_TAIL);
-        if (fp.traversalStats)
-            hipLaunchKernelGGL((k_tail<N, true>), dim3(blocks), dim3(kTraceBlock), 0, s, a);
+        if (fp.traversalStats && sc.hasNodePrograms)
+            hipLaunchKernelGGL((k_tail<N, true, false, true>), dim3(blocks), dim3(kTraceBlock), 0, s, a);
+        else if (fp.traversalStats)
+            hipLaunchKernelGGL((k_tail<N, true, false, false>), dim3(blocks), dim3(kTraceBlock), 0, s, a);
+        else if (sc.hasNodePrograms)
+            hipLaunchKernelGGL((k_tail<N, false, false, true>), dim3(blocks), dim3(kTraceBlock), 0, s, a);
         else
-            hipLaunchKernelGGL((k_tail<N, false>), dim3(blocks), dim3(kTraceBlock), 0, s, a);
+            hipLaunchKernelGGL((k_tail<N, false, false, false>), dim3(blocks), dim3(kTraceBlock), 0, s, a);
         t1(STAGE_TAIL);
     }
     if (cache && fp.cacheMode == 2) {
@@ -2577,13 +2584,19 @@ static hipError_t launchBatchStepT(const BatchLaunch &b, hipStream_t s) {
     for (uint32_t d = 0; d < wavefrontDepths; ++d) {
         a.depth = d;
         hipLaunchKernelGGL((k_intersect<false, true>), dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
-        hipLaunchKernelGGL((k_shade<N, true>), dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
+        if (b.nodePrograms)
+            hipLaunchKernelGGL((k_shade<N, true, true>), dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_shade<N, true, false>), dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
         hipLaunchKernelGGL((k_shadow<false, true>), dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
     }
     if (wavefrontDepths < numDepths) {
         a.depth = wavefrontDepths;
         const uint32_t blocks = std::max(1u, std::min(blocksFor(n, kTraceBlock), b.fp.tailBlocks));
-        hipLaunchKernelGGL((k_tail<N, false, true>), dim3(blocks), dim3(kTraceBlock), 0, s, a);
+        if (b.nodePrograms)
+            hipLaunchKernelGGL((k_tail<N, false, true, true>), dim3(blocks), dim3(kTraceBlock), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_tail<N, false, true, false>), dim3(blocks), dim3(kTraceBlock), 0, s, a);
     }
     hipLaunchKernelGGL((k_accumulate<N, true>), dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_frameEnd, dim3(1), dim3(64), 0, s, b.fb.counters, b.fp.maxPathLength, wavefrontDepths);
@@ -2635,13 +2648,19 @@ static hipError_t launchBptFrameT(const DevScene &sc, const FrameParams &fp, con
     auto t0 = [&](int st) { if (timer) timer->begin(st); };
     auto t1 = [&](int st) { if (timer) timer->end(st); };
     t0(STAGE_LIGHT_PATHS);
-    hipLaunchKernelGGL(k_bptLight<N>, dim3(blocksFor(bp.numLightPaths, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
+    if (sc.hasNodePrograms)
+        hipLaunchKernelGGL((k_bptLight<N, true>), dim3(blocksFor(bp.numLightPaths, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_bptLight<N, false>), dim3(blocksFor(bp.numLightPaths, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
     hipLaunchKernelGGL(k_bptScanBlocks, dim3(scanBlocks), dim3(kScanBlock), 0, s, bp);
     hipLaunchKernelGGL(k_bptScanTop, dim3(1), dim3(kScanBlock), 0, s, bp, scanBlocks);
     hipLaunchKernelGGL(k_bptCompact, dim3(blocksFor(bp.numLightPaths, 256)), dim3(256), 0, s, bp);
     t1(STAGE_LIGHT_PATHS);
     t0(STAGE_EYE_PATHS);
-    hipLaunchKernelGGL(k_bptEye<N>, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
+    if (sc.hasNodePrograms)
+        hipLaunchKernelGGL((k_bptEye<N, true>), dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_bptEye<N, false>), dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
     t1(STAGE_EYE_PATHS);
     t0(STAGE_ACCUMULATE);
     hipLaunchKernelGGL(k_bptAccumulate<N>, dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
@@ -2665,6 +2684,59 @@ hipError_t launchTraceRays(const DevScene &sc, const void *rays, uint32_t n, voi
     else
         hipLaunchKernelGGL(k_traceRays<false>, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, sc,
                            static_cast<const RayIn *>(rays), n, static_cast<HitOut *>(hits));
+    return hipGetLastError();
+}
+
+// ---- direct evaluation of a node program at given surface points ----
+// point: position 3, geometric normal 3, shading frame x 3, y 3, z 3, texture coordinates 2,
+// wavelength offset 1, pad 2 (20 floats); instance 0 of the scene record is the identity
+template <int N>
+__global__ void __launch_bounds__(128) k_evalNodePlug(DevScene sc, uint32_t base, uint32_t count, uint32_t cls,
+                                                      const float *points, uint32_t n, float4 *out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float *p = points + 20 * size_t(i);
+    SurfPt sp;
+    sp.position = V3(p[0], p[1], p[2]);
+    sp.gn = V3(p[3], p[4], p[5]);
+    sp.frame.x = V3(p[6], p[7], p[8]);
+    sp.frame.y = V3(p[9], p[10], p[11]);
+    sp.frame.z = V3(p[12], p[13], p[14]);
+    sp.texU = p[15];
+    sp.texV = p[16];
+    sp.isPoint = sp.atInfinity = false;
+    sp.instIndex = sp.geomInstIndex = sp.primIndex = 0;
+    sp.u = sp.v = 0.0f;
+    Wls<N> wls;
+    wls.offset = p[17];
+    wls.sel = 0;
+    wls.single = false;
+    NodeOutputs<N> o;
+    runNodeProgram<N, true>(sc, base, count, sp, wls, cls, cls, &o);
+    float4 r = make_float4(0, 0, 0, 0);
+    if (cls == VC_SPEC) {
+        if (o.has & 1u) {
+            r.x = o.spec[0][0];
+            r.y = o.spec[0][1];
+            r.z = o.spec[0][2];
+            if constexpr (N == 4) r.w = o.spec[0][3];
+        }
+    } else if (o.has & 8u) {
+        r = o.vec[0];
+    }
+    out[i] = r;
+}
+
+hipError_t launchEvalNodePlug(const DevScene &sc, uint32_t base, uint32_t count, uint32_t plugType, bool spectral,
+                              const float *points, uint32_t n, float *out, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const uint32_t cls = valueClass(plugType);
+    if (spectral)
+        hipLaunchKernelGGL(k_evalNodePlug<4>, dim3(blocksFor(n, 128)), dim3(128), 0, s, sc, base, count, cls, points, n,
+                           reinterpret_cast<float4 *>(out));
+    else
+        hipLaunchKernelGGL(k_evalNodePlug<3>, dim3(blocksFor(n, 128)), dim3(128), 0, s, sc, base, count, cls, points, n,
+                           reinterpret_cast<float4 *>(out));
     return hipGetLastError();
 }
 

@@ -201,6 +201,7 @@ struct BatchLaunch {
     DevJob* jobs;
     const uint32_t* jobOffsets;
     uint32_t numJobs;
+    bool nodePrograms;             // some job's scene has node programs: the kernels that run them
 };
 hipError_t launchBatchStep(const BatchLaunch& b, bool spectral, hipStream_t stream);
 hipError_t launchBatchConvert(const BatchLaunch& b, bool spectral, hipStream_t stream);
@@ -212,6 +213,10 @@ hipError_t launchBptFrame(const DevScene& scene, const FrameParams& fp, const Fr
 hipError_t launchTraceRays(const DevScene& scene, const void* rays, uint32_t numRays, void* hits, bool anyHit,
                            hipStream_t stream);
 
+// Node program (base, count) at device-resident surface points (20 floats each), 4 floats out
+// per point: the spectrum (Spectrum plugs) or the value in the plug's own type.
+hipError_t launchEvalNodePlug(const DevScene& scene, uint32_t base, uint32_t count, uint32_t plugType, bool spectral,
+                              const float* points, uint32_t n, float* out, hipStream_t stream);
 hipError_t launchBsdfQueries(const DevScene& scene, const DevMaterial& material, bool spectral, const void* queries,
                              uint32_t n, void* results, hipStream_t stream);
 

@@ -320,9 +320,42 @@ bool ShaderNode::getPlug(const char *n, PlugRef *p) const {
     *p = it == plugs.end() ? PlugRef() : it->second;
     return true;
 }
+bool plugConvertible(SocketType socket, uint32_t plugType) {
+    // the value type of each plug type: float1 and Alpha are float, TextureCoordinates Point3D
+    const bool scalar = plugType == VLRShaderNodePlugType_float1 || plugType == VLRShaderNodePlugType_Alpha;
+    const bool triple = plugType == VLRShaderNodePlugType_float3 || plugType == VLRShaderNodePlugType_Point3D ||
+                        plugType == VLRShaderNodePlugType_TextureCoordinates ||
+                        plugType == VLRShaderNodePlugType_Vector3D || plugType == VLRShaderNodePlugType_Normal3D;
+    switch (socket) {
+    case SocketType::Float: return scalar;
+    case SocketType::Float3: return scalar || triple;
+    case SocketType::Point3D:
+    case SocketType::Vector3D:
+    case SocketType::Normal3D: return triple;
+    case SocketType::Spectrum: return plugType == VLRShaderNodePlugType_Spectrum;
+    }
+    return false;
+}
+
+namespace {
+// the socket type of a parameter that takes a plug
+bool socketOf(const ParamInfo *pi, SocketType *s) {
+    if (pi->typeStr == "Spectrum") *s = SocketType::Spectrum;
+    else if (pi->typeStr == "Float") *s = pi->tupleSize == 3 ? SocketType::Float3 : SocketType::Float;
+    else if (pi->typeStr == "TextureCoordinates") *s = SocketType::Point3D;
+    else if (pi->typeStr == "Vector3D") *s = SocketType::Vector3D;
+    else return false;
+    return true;
+}
+} // namespace
+
 int ShaderNode::setPlug(const char *n, const PlugRef &p) {
     const ParamInfo *pi = findParam(params(), n);
     if (!pi || !(pi->formFlags & NODE)) return 1;
+    SocketType socket;
+    if (!socketOf(pi, &socket)) return 1;
+    // an inconvertible plug is refused as the reference's set() refuses it (shader_nodes.cpp)
+    if (p.valid() && !plugConvertible(socket, p.plugType)) return 1;
     plugs[pi->name] = p;
     ctx->touch();
     return 0;
@@ -450,43 +483,33 @@ bool SurfaceMaterial::getPlug(const char *n, PlugRef *p) const {
 int SurfaceMaterial::setPlug(const char *n, const PlugRef &p) {
     const ParamInfo *pi = findParam(params(), n);
     if (!pi || !(pi->formFlags & NODE)) return 1;
-    if (p.valid()) {
-        const bool wantSpectrum = pi->typeStr == "Spectrum";
-        if (wantSpectrum && p.plugType != VLRShaderNodePlugType_Spectrum) return 2;
-        if (!wantSpectrum && pi->typeStr == "Float" && p.plugType > VLRShaderNodePlugType_float4) return 2;
-        // the render path evaluates constant node chains and the environment texture
-        bool ok = false;
-        if (wantSpectrum) {
-            ImmSpectrum s;
-            SpectrumType t;
-            ok = p.node->constantSpectrum(&s, &t) ||
-                 (p.node->kind == NodeKind::EnvironmentTexture && kind == MatKind::EnvironmentEmitter);
-        } else {
-            float v;
-            ok = p.node->constantFloat(p.option, &v) ||
-                 (pi->tupleSize == 3 && (p.node->kind == NodeKind::Float3 || p.node->kind == NodeKind::Float4));
-        }
-        if (!ok) return 2;
-    }
+    SocketType socket;
+    if (!socketOf(pi, &socket)) return 1;
+    // materials.cpp's set(): a plug whose type does not convert to the socket's is refused
+    if (p.valid() && !plugConvertible(socket, p.plugType)) return 1;
     plugs[pi->name] = p;
     ctx->touch();
     return 0;
 }
 
-DevMaterial SurfaceMaterial::toDevice(bool spectral, bool *usesEnvTexture, const Image2D **envImage) const {
+DevMaterial SurfaceMaterial::toDevice(bool spectral, bool *usesEnvTexture, const Image2D **envImage,
+                                      std::vector<std::pair<PlugRef, uint32_t>> *roots) const {
     DevMaterial d;
     std::memset(&d, 0, sizeof(d));
     *usesEnvTexture = false;
     *envImage = nullptr;
+    // A spectrum slot connected to a node: a TripletSpectrum node folds into the immediate (its
+    // own spectrum type decides the colour conversion); any other node is evaluated per hit
+    // by the material's node program, with the immediate as its default.  Only a Spectrum plug
+    // converts to a spectrum (shared.h:430); another plug type leaves the immediate.
     auto spectrumOf = [&](size_t slot, SpectrumType type) {
         ImmSpectrum s = specs[slot];
         auto it = plugs.find(specNames[slot]);
         if (it != plugs.end() && it->second.valid()) {
             SpectrumType t;
-            if (it->second.node->constantSpectrum(&s, &t)) {
-                // TripletSpectrum node: its own spectrum type decides the colour conversion
-                return makeDevSpectrum(t, s, spectral);
-            }
+            if (it->second.node->constantSpectrum(&s, &t)) return makeDevSpectrum(t, s, spectral);
+            if (roots && it->second.plugType == VLRShaderNodePlugType_Spectrum)
+                roots->push_back({it->second, 1u << slot});
         }
         return makeDevSpectrum(type, s, spectral);
     };
@@ -495,14 +518,24 @@ DevMaterial SurfaceMaterial::toDevice(bool spectral, bool *usesEnvTexture, const
             if (paramNameEquals(floatNames[i], name)) return floatVals[i];
         return 0.0f;
     };
-    auto tupleOf = [&](const char *plugName, int component, float fallback) {
+    // The float3 parameter: folded when its node is constant, else node-program output 8 with
+    // the immediates in raw[0..2] as its default.
+    auto tupleNode = [&](const char *plugName, const char *n0, const char *n1, const char *n2) {
+        d.raw[0] = floatOf(n0);
+        d.raw[1] = floatOf(n1);
+        d.raw[2] = floatOf(n2);
         auto it = plugs.find(plugName);
-        if (it != plugs.end() && it->second.valid()) {
-            float v;
-            if (it->second.node->constantFloat(static_cast<uint32_t>(component), &v)) return v;
+        if (it == plugs.end() || !it->second.valid()) return;
+        float v[3];
+        bool constant = true;
+        for (int k = 0; k < 3; ++k) constant = constant && it->second.node->constantFloat(static_cast<uint32_t>(k), &v[k]);
+        if (constant) {
+            for (int k = 0; k < 3; ++k) d.raw[k] = v[k];
+        } else if (roots) {
+            roots->push_back({it->second, 8u});
         }
-        return fallback;
     };
+    auto tupleOf = [&](const char *, int component, float) { return d.raw[component]; };
     switch (kind) {
     case MatKind::Matte:
         d.type = MAT_MATTE;
@@ -525,6 +558,7 @@ DevMaterial SurfaceMaterial::toDevice(bool spectral, bool *usesEnvTexture, const
         d.type = MAT_MICROFACET_REFLECTION;
         d.spec[0] = spectrumOf(0, SpectrumType::IndexOfRefraction);
         d.spec[1] = spectrumOf(1, SpectrumType::IndexOfRefraction);
+        tupleNode("roughness/anisotropy/rotation", "roughness", "anisotropy", "rotation");
         float r = tupleOf("roughness/anisotropy/rotation", 0, floatOf("roughness"));
         float an = tupleOf("roughness/anisotropy/rotation", 1, floatOf("anisotropy"));
         float rot = tupleOf("roughness/anisotropy/rotation", 2, floatOf("rotation"));
@@ -541,6 +575,7 @@ DevMaterial SurfaceMaterial::toDevice(bool spectral, bool *usesEnvTexture, const
         d.spec[0] = spectrumOf(0, SpectrumType::Reflectance);
         d.spec[1] = spectrumOf(1, SpectrumType::IndexOfRefraction);
         d.spec[2] = spectrumOf(2, SpectrumType::IndexOfRefraction);
+        tupleNode("roughness/anisotropy/rotation", "roughness", "anisotropy", "rotation");
         float r = tupleOf("roughness/anisotropy/rotation", 0, floatOf("roughness"));
         float an = tupleOf("roughness/anisotropy/rotation", 1, floatOf("anisotropy"));
         float rot = tupleOf("roughness/anisotropy/rotation", 2, floatOf("rotation"));
@@ -554,6 +589,7 @@ DevMaterial SurfaceMaterial::toDevice(bool spectral, bool *usesEnvTexture, const
     case MatKind::UE4:
         d.type = MAT_UE4;
         d.spec[0] = spectrumOf(0, SpectrumType::Reflectance);
+        tupleNode("occlusion/roughness/metallic", "occlusion", "roughness", "metallic");
         d.scalars[0] = tupleOf("occlusion/roughness/metallic", 0, floatOf("occlusion"));
         d.scalars[1] = tupleOf("occlusion/roughness/metallic", 1, floatOf("roughness"));
         d.scalars[2] = tupleOf("occlusion/roughness/metallic", 2, floatOf("metallic"));

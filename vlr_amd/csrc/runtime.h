@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <array>
+#include <functional>
 #include <cstdint>
 #include <map>
 #include <memory>
@@ -20,6 +21,7 @@
 
 #include "dev_types.h"
 #include "bvh.h"
+#include "color.h"
 #include "launch.h"
 
 #include "../../include/vlr/public_types.h"
@@ -50,10 +52,11 @@ const std::vector<float> &mengPoints();
 void cmfBins(float x[16], float y[16], float z[16], float *integral);
 // device spectrum parameter for an immediate spectrum (RGB build: r,g,b; spectral: Meng)
 DevSpectrum makeDevSpectrum(SpectrumType type, const ImmSpectrum &s, bool spectral);
+MengTables hostMengTables();
+void sampledSpectrumToRGB(SpectrumType type, const float *lambdas, const float *values, uint32_t n, float rgb[3]);
 bool colorSpaceFromString(const char *s, ColorSpace *cs);
 const char *colorSpaceToString(ColorSpace cs);
 bool spectrumTypeFromString(const char *s, SpectrumType *t);
-void mengXYtoUV(const float xy[2], float uv[2]);
 uint16_t floatToHalf(float f);
 float halfToFloat(uint16_t h);
 
@@ -117,6 +120,11 @@ struct PlugRef {
     uint32_t option = 0;
     bool valid() const { return node != nullptr; }
 };
+
+// The value type a socket reads; a plug connects to it where the implicit conversion is
+// defined (NodeTypeInfo<T>::ConversionIsDefinedFrom, shared.h:390-470).
+enum class SocketType : uint32_t { Float, Float3, Point3D, Vector3D, Normal3D, Spectrum };
+bool plugConvertible(SocketType socket, uint32_t plugType);
 
 class Queryable : public Object {
 public:
@@ -247,7 +255,10 @@ public:
     bool getPlug(const char *, PlugRef *) const override;
     int setPlug(const char *, const PlugRef &) override;
     // device descriptor (immediate values with constant node plugs folded in)
-    DevMaterial toDevice(bool spectral, bool *usesEnvTexture, const Image2D **envImage) const;
+    // roots (may be null) receives the parameters connected to nodes that do not fold into
+    // immediates, with their node-program output bit (1 << k: spectrum k, 8: float3 parameter)
+    DevMaterial toDevice(bool spectral, bool *usesEnvTexture, const Image2D **envImage,
+                         std::vector<std::pair<PlugRef, uint32_t>> *roots) const;
     // radiance-cache inputs: diffuse RGB, roughness, specular RGB, 0
     void cacheFeatures(float out[8]) const;
 };
@@ -342,6 +353,22 @@ public:
     DevCamera toDevice() const;
 };
 
+// ---- shader nodes and textures of a scene as device records (shader_table.cpp) ----
+struct ShaderTable {
+    bool spectral = false;
+    std::vector<DevShaderNode> nodes;
+    std::vector<DevTexture> textures;
+    std::vector<float4> texels;
+    std::vector<float> pool;
+    std::map<const ShaderNode *, uint32_t> nodeIndex;
+    std::map<const Image2D *, uint32_t> texIndex;
+    std::vector<DevNodeStep> steps;
+    uint32_t addNode(const ShaderNode *n);
+    uint32_t addTexture(const Image2D *img);
+    // One node program over the given roots (plug, output bits); returns (first step, count).
+    std::pair<uint32_t, uint32_t> program(const std::vector<std::pair<PlugRef, uint32_t>> &roots);
+};
+
 // ---- device scene (scene_setup.cpp) ----
 struct DeviceBuffer {
     void *ptr = nullptr;
@@ -354,7 +381,7 @@ struct DeviceBuffer {
 
 struct SceneGPU {
     DeviceBuffer tlasNodes, blasNodes, blasTris, instances, instGeomInsts, geomInsts, vertices, indices, distPool,
-        materials, envTex, mengGrid, mengPoints, cacheMaterials;
+        materials, envTex, mengGrid, mengPoints, cacheMaterials, shaderNodes, nodeSteps, textures, texels, nodePool;
     DevScene dev = {};
     bool spectral = false;
     bool valid = false;
@@ -485,6 +512,9 @@ public:
     // 2 BLAS triangle records, 3 per-instance (first BLAS node, first triangle record)
     void readAccel(const Scene &s, uint32_t which, void *dst, size_t capacity, size_t *bytes);
     void setAccelBuilder(uint32_t builder);
+    // one node plug evaluated at device-resident surface points (vlrContextEvaluateShaderNodePlug)
+    void evaluateNodePlug(hipStream_t stream, const PlugRef &plug, bool spectral, const float *points, uint32_t n,
+                          float *values);
     void evaluateBSDFs(hipStream_t stream, const SurfaceMaterial &m, bool spectral, const void *q, uint32_t n, void *r);
     // trav: nodes, triangles, instances of closest-hit then of shadow traversals
     // wavefront: extension and shadow rays traced by the per-depth kernels

@@ -18,6 +18,7 @@
 #include "bvh.h"
 
 #include <chrono>
+#include <tuple>
 
 #include <cmath>
 #include <limits>
@@ -158,6 +159,10 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
     float ident[16] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
     flatten(scene, ident, &flat);
 
+    // shader nodes reachable from materials and material groups
+    ShaderTable shaders;
+    shaders.spectral = spectral;
+
     // materials
     std::vector<DevMaterial> mats;
     std::vector<float> cacheMats;   // 8 floats per material
@@ -171,7 +176,9 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
                            std::string("material type not on the render path: ") + m->typeName()};
         bool usesTex;
         const Image2D *img;
-        DevMaterial d = m->toDevice(spectral, &usesTex, &img);
+        std::vector<std::pair<PlugRef, uint32_t>> roots;
+        DevMaterial d = m->toDevice(spectral, &usesTex, &img, &roots);
+        if (!roots.empty()) std::tie(d.progBase, d.progCount) = shaders.program(roots);
         if (usesTex) envImage = img;
         uint32_t idx = static_cast<uint32_t>(mats.size());
         mats.push_back(d);
@@ -215,6 +222,13 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
         G.vertexBase = vbase;
         G.indexBase = static_cast<uint32_t>(indices.size());
         G.isPoints = s->type == ObjType::PointSurfaceNode ? 1 : 0;
+        if (!G.isPoints && (mg.nodeNormal.valid() || mg.nodeTangent.valid())) {
+            // normal (output vector 0) and tangent (vector 1) nodes; alpha nodes are not evaluated
+            std::vector<std::pair<PlugRef, uint32_t>> roots;
+            if (mg.nodeNormal.valid()) roots.push_back({mg.nodeNormal, 8u});
+            if (mg.nodeTangent.valid()) roots.push_back({mg.nodeTangent, 16u});
+            std::tie(G.progBase, G.progCount) = shaders.program(roots);
+        }
         G.numPrims = G.isPoints ? static_cast<uint32_t>(mg.indices.size()) : static_cast<uint32_t>(mg.indices.size() / 3);
         for (uint32_t k = 0; k < (G.isPoints ? G.numPrims : 3 * G.numPrims); ++k) {
             if (mg.indices[k] >= s->vertices.size())
@@ -580,6 +594,17 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
     g->indices.upload(indices.data(), indices.size() * sizeof(uint32_t));
     g->distPool.upload(pool.data(), pool.size() * sizeof(float));
     g->materials.upload(mats.data(), mats.size() * sizeof(DevMaterial));
+    if (shaders.nodes.empty()) shaders.nodes.emplace_back();
+    if (shaders.textures.empty()) shaders.textures.emplace_back();
+    if (shaders.texels.empty()) shaders.texels.push_back(make_float4(0, 0, 0, 0));
+    if (shaders.pool.empty()) shaders.pool.push_back(0.0f);
+    const bool hasPrograms = !shaders.steps.empty();
+    if (shaders.steps.empty()) shaders.steps.emplace_back();
+    g->shaderNodes.upload(shaders.nodes.data(), shaders.nodes.size() * sizeof(DevShaderNode));
+    g->nodeSteps.upload(shaders.steps.data(), shaders.steps.size() * sizeof(DevNodeStep));
+    g->textures.upload(shaders.textures.data(), shaders.textures.size() * sizeof(DevTexture));
+    g->texels.upload(shaders.texels.data(), shaders.texels.size() * sizeof(float4));
+    g->nodePool.upload(shaders.pool.data(), shaders.pool.size() * sizeof(float));
     g->cacheMaterials.upload(cacheMats.data(), cacheMats.size() * sizeof(float));
     g->envTex.upload(envTex.data(), envTex.size() * sizeof(float4));
     g->mengGrid.upload(mengGrid().data(), mengGrid().size() * sizeof(MengCellDev));
@@ -595,6 +620,12 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
     D.indices = static_cast<const uint32_t *>(g->indices.ptr);
     D.distPool = static_cast<const float *>(g->distPool.ptr);
     D.materials = static_cast<const DevMaterial *>(g->materials.ptr);
+    D.nodes = static_cast<const DevShaderNode *>(g->shaderNodes.ptr);
+    D.nodeSteps = static_cast<const DevNodeStep *>(g->nodeSteps.ptr);
+    D.hasNodePrograms = hasPrograms ? 1 : 0;
+    D.textures = static_cast<const DevTexture *>(g->textures.ptr);
+    D.texels = static_cast<const float4 *>(g->texels.ptr);
+    D.nodePool = static_cast<const float *>(g->nodePool.ptr);
     D.cacheMaterials = static_cast<const float4 *>(g->cacheMaterials.ptr);
     D.envTex = envTex.empty() ? nullptr : static_cast<const float4 *>(g->envTex.ptr);
     D.mengGrid = static_cast<const MengCellDev *>(g->mengGrid.ptr);

@@ -6,6 +6,7 @@
 // Tables: CIE 1931 2-degree colour-matching functions and the Meng et al. 2015 grid, read
 // from vlr_amd/data/spectral_tables.bin (published data, layout in oracle/extract_tables.py).
 #include "runtime.h"
+#include "color.h"
 
 #include <dlfcn.h>
 #include <cmath>
@@ -102,65 +103,6 @@ void loadOnce() {
     T.ok = true;
 }
 
-const float mat_Rec709_D65_to_XYZ[9] = {0.4124564f, 0.2126729f, 0.0193339f, 0.3575761f, 0.7151522f,
-                                        0.1191920f, 0.1804375f, 0.0721750f, 0.9503041f};
-const float mat_XYZ_to_Rec709_D65[9] = {3.2404542f, -0.9692660f, 0.0556434f, -1.5371385f, 1.8760108f,
-                                        -0.2040259f, -0.4985314f, 0.0415560f, 1.0572252f};
-const float mat_Rec709_E_to_XYZ[9] = {0.4969f, 0.2562f, 0.0233f, 0.3391f, 0.6782f, 0.1130f, 0.1640f, 0.0656f, 0.8637f};
-const float mat_XYZ_to_Rec709_E[9] = {2.6897f, -1.0221f, 0.0612f, -1.2759f, 1.9783f, -0.2245f, -0.4138f, 0.0438f, 1.1633f};
-
-inline void tristim(const float m[9], const float s[3], float d[3]) {
-    d[0] = m[0] * s[0] + m[3] * s[1] + m[6] * s[2];
-    d[1] = m[1] * s[0] + m[4] * s[1] + m[7] * s[2];
-    d[2] = m[2] * s[0] + m[5] * s[1] + m[8] * s[2];
-}
-
-inline float degamma(float v) { return v <= 0.04045f ? v / 12.92f : std::pow((v + 0.055f) / 1.055f, 2.4f); }
-
-// Meng grid lookup (spectrum_types.cpp:11-91)
-void mengAdj(float u, float v, uint32_t *adj, uint16_t *s, uint16_t *t) {
-    const Tables &T = g_tables;
-    u = std::fmin(std::fmax(u, 0.0f), 12.0f);
-    v = std::fmin(std::fmax(v, 0.0f), 14.0f);
-    int32_t ui = std::min(static_cast<int32_t>(u), 11), vi = std::min(static_cast<int32_t>(v), 13);
-    const MengCellDev &cell = T.grid[ui + 12 * vi];
-    const uint8_t *ix = cell.idx;
-    const float *P = T.points.data();
-    *adj = 0;
-    *s = *t = 0;
-    if (cell.inside) {
-        float ss = u - ui, tt = v - vi;
-        *adj = (uint32_t(ix[3]) << 24) | (uint32_t(ix[2]) << 16) | (uint32_t(ix[1]) << 8) | uint32_t(ix[0]);
-        *s = static_cast<uint16_t>(std::fmax(ss * (UINT16_MAX - 1), 0.0f));
-        *t = static_cast<uint16_t>(std::fmax(tt * (UINT16_MAX - 1), 0.0f));
-        return;
-    }
-    // cells with no data points lie outside the spectral locus; keep adj = 0
-    if (cell.numPoints < 3) return;
-    const float ex = u - P[ix[0] * 99 + 2], ey = v - P[ix[0] * 99 + 3];
-    float e0x = P[ix[1] * 99 + 2] - P[ix[0] * 99 + 2];
-    float e0y = P[ix[1] * 99 + 3] - P[ix[0] * 99 + 3];
-    float uu = e0x * ey - ex * e0y;
-    for (int i = 1; i < cell.numPoints; ++i) {
-        uint32_t idx = ix[i % (cell.numPoints - 1) + 1];
-        float e1x = P[idx * 99 + 2] - P[ix[0] * 99 + 2];
-        float e1y = P[idx * 99 + 3] - P[ix[0] * 99 + 3];
-        float vv = ex * e1y - e1x * ey;
-        const float area = e0x * e1y - e1x * e0y;
-        const float b0 = uu / area, b1 = vv / area;
-        float b2 = 1.0f - b0 - b1;
-        if (b0 < -1e-6f || b1 < -1e-6f || b2 < -1e-6f) {
-            uu = -vv;
-            e0x = e1x;
-            e0y = e1y;
-            continue;
-        }
-        *adj = (uint32_t(UINT8_MAX) << 24) | (uint32_t(ix[0]) << 16) | (uint32_t(ix[i]) << 8) | idx;
-        *s = static_cast<uint16_t>(std::fmax(b0 * (UINT16_MAX - 1), 0.0f));
-        *t = static_cast<uint16_t>(std::fmax(b1 * (UINT16_MAX - 1), 0.0f));
-        break;
-    }
-}
 } // namespace
 
 bool loadSpectralTables() {
@@ -178,85 +120,40 @@ void cmfBins(float x[16], float y[16], float z[16], float *integral) {
     *integral = g_tables.integral;
 }
 
-void mengXYtoUV(const float xy[2], float uv[2]) {
-    uv[0] = 16.730260708356887f * xy[0] + 7.7801960340706f * xy[1] - 2.170152247475828f;
-    uv[1] = -7.530081094743006f * xy[0] + 16.192422314095225f * xy[1] + 1.1125529268825947f;
+MengTables hostMengTables() { return MengTables{g_tables.grid.data(), g_tables.points.data()}; }
+
+// A sampled spectrum as a rendering-RGB triplet for the RGB build: XYZ by a 1 nm sum of
+// value x CMF over 360-830 nm (values interpolated linearly, held constant beyond the ends),
+// normalised by the sum of ybar, then the reflectance (E) or light (D65) matrix.
+void sampledSpectrumToRGB(SpectrumType type, const float *lambdas, const float *values, uint32_t n, float rgb[3]) {
+    const Tables &T = g_tables;
+    float XYZ[3] = {0, 0, 0}, ysum = 0;
+    for (int i = 0; i < 471; ++i) {
+        const float wl = 360.0f + i;
+        float v;
+        if (wl <= lambdas[0]) {
+            v = values[0];
+        } else if (wl >= lambdas[n - 1]) {
+            v = values[n - 1];
+        } else {
+            uint32_t k = 0;
+            while (k + 1 < n && lambdas[k + 1] < wl) ++k;
+            const float t = (wl - lambdas[k]) / (lambdas[k + 1] - lambdas[k]);
+            v = (1 - t) * values[k] + t * values[k + 1];
+        }
+        XYZ[0] += v * T.xbar[i];
+        XYZ[1] += v * T.ybar[i];
+        XYZ[2] += v * T.zbar[i];
+        ysum += T.ybar[i];
+    }
+    for (float &c : XYZ) c /= ysum;
+    colorTristim(type == SpectrumType::LightSource ? 1 : 3, XYZ, rgb);
+    for (int k = 0; k < 3; ++k) rgb[k] = std::fmax(0.0f, rgb[k]);
 }
 
 DevSpectrum makeDevSpectrum(SpectrumType type, const ImmSpectrum &is, bool spectral) {
-    float e0 = is.e[0], e1 = is.e[1], e2 = is.e[2];
-    DevSpectrum out = {0, 0, 0, 0};
-    const bool light = type == SpectrumType::LightSource;
-    if (!spectral) {
-        float XYZ[3] = {0, 0, 0};
-        switch (is.cs) {
-        case ColorSpace::Rec709_D65_sRGBGamma:
-            e0 = degamma(e0);
-            e1 = degamma(e1);
-            e2 = degamma(e2);
-            [[fallthrough]];
-        case ColorSpace::Rec709_D65: {
-            float RGB[3] = {e0, e1, e2};
-            tristim(light ? mat_Rec709_D65_to_XYZ : mat_Rec709_E_to_XYZ, RGB, XYZ);
-            break;
-        }
-        case ColorSpace::XYZ:
-            XYZ[0] = e0; XYZ[1] = e1; XYZ[2] = e2;
-            break;
-        case ColorSpace::xyY:
-            if (e1 != 0) {
-                float z = 1 - (e0 + e1), b = e2 / e1;
-                XYZ[0] = e0 * b; XYZ[1] = e2; XYZ[2] = z * b;
-            }
-            break;
-        }
-        float RGB[3];
-        tristim(light ? mat_XYZ_to_Rec709_D65 : mat_XYZ_to_Rec709_E, XYZ, RGB);
-        out.x = RGB[0]; out.y = RGB[1]; out.z = RGB[2];
-        return out;
-    }
-    float xy[2] = {0, 0}, brightness = 0;
-    switch (is.cs) {
-    case ColorSpace::Rec709_D65_sRGBGamma:
-        e0 = degamma(e0);
-        e1 = degamma(e1);
-        e2 = degamma(e2);
-        [[fallthrough]];
-    case ColorSpace::Rec709_D65: {
-        float RGB[3] = {e0, e1, e2}, XYZ[3];
-        tristim(light ? mat_Rec709_D65_to_XYZ : mat_Rec709_E_to_XYZ, RGB, XYZ);
-        e0 = XYZ[0]; e1 = XYZ[1]; e2 = XYZ[2];
-        [[fallthrough]];
-    }
-    case ColorSpace::XYZ: {
-        float Y = e1, b = e0 + e1 + e2;
-        e0 = e0 / b;
-        e1 = e1 / b;
-        e2 = Y;
-        [[fallthrough]];
-    }
-    case ColorSpace::xyY:
-        xy[0] = e0;
-        xy[1] = e1;
-        brightness = e2 / e1;
-        if (e2 == 0) {
-            xy[0] = 0.3333f;
-            xy[1] = 0.3333f;
-            brightness = 0;
-        }
-        break;
-    }
-    float scale = brightness / 0.009355121400914532f;
-    float uv[2];
-    mengXYtoUV(xy, uv);
-    uint32_t adj;
-    uint16_t s, t;
-    mengAdj(uv[0], uv[1], &adj, &s, &t);
-    uint32_t st = uint32_t(s) | (uint32_t(t) << 16);
-    std::memcpy(&out.x, &adj, 4);
-    std::memcpy(&out.y, &st, 4);
-    out.z = scale;
-    return out;
+    return colorToDevSpectrum(static_cast<uint32_t>(type), static_cast<uint32_t>(is.cs), is.e[0], is.e[1], is.e[2],
+                              spectral, hostMengTables());
 }
 
 static const char *kColorSpaces[] = {"Rec709(D65) sRGB Gamma", "Rec709(D65)", "XYZ", "xyY"};

@@ -43,6 +43,27 @@ COLOR_SPACES = [REC709_SRGB, REC709, XYZ, XYY]
 
 
 @dataclass
+class ImageDesc:
+    """A LinearImage2D: raw texel bytes in one of the C API's formats."""
+    data: np.ndarray                 # uint8, height x width x bytes per texel
+    width: int
+    height: int
+    format: str                      # "RGBA8x4", "RGB8x3", "RGBA16Fx4", "RGBA32Fx4", "Gray8", "Gray32F", ...
+    spectrum_type: str = "Reflectance"
+    color_space: str = REC709
+
+
+@dataclass
+class NodeDesc:
+    """A shader node: its type name (vlrShaderNodeCreate), immediate parameters and inputs."""
+    type: str
+    floats: dict = field(default_factory=dict)      # name -> list of floats
+    enums: dict = field(default_factory=dict)       # name -> enum string
+    plugs: dict = field(default_factory=dict)       # name -> (NodeDesc, plug type, option)
+    image: Optional[ImageDesc] = None
+
+
+@dataclass
 class MaterialDesc:
     type: str
     # immediate spectra: parameter name -> (colour space, e0, e1, e2)
@@ -51,6 +72,8 @@ class MaterialDesc:
     floats: dict = field(default_factory=dict)
     # environment emitter: emittance from the scene's environment texture
     use_env_texture: bool = False
+    # parameters connected to shader nodes: name -> (NodeDesc, plug type, option)
+    plugs: dict = field(default_factory=dict)
 
 
 @dataclass
@@ -59,6 +82,9 @@ class MeshDesc:
     vertices: np.ndarray            # float32 [N, 11]: position(3) normal(3) tc0Direction(3) texCoord(2)
     groups: List[Tuple[np.ndarray, int]]  # (uint32 indices, material index)
     points: bool = False
+    # shader nodes of a triangle group: group index -> (normal plug, tangent plug), each a
+    # (NodeDesc, plug type, option) or None
+    group_nodes: dict = field(default_factory=dict)
 
 
 @dataclass
