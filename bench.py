@@ -641,7 +641,8 @@ def main():
         "image_mean_rgb": [float(x) for x in img[..., :3].reshape(-1, 3).mean(axis=0)],
         "setup_s": setup_s,
         "accel_build": {"builder": "device", "ms_blas": accel["msBlas"], "ms_tlas": accel["msTlas"],
-                        "blas_count": accel["numBlas"], "triangles": accel["numTriangles"]},
+                        "blas_count": accel["numBlas"], "triangles": accel["numTriangles"],
+                        "node_program_steps": accel["numNodeProgramSteps"]},
         "rmse": None,
     }
     if not args.no_cpu:

@@ -202,6 +202,8 @@ typedef struct VLRAccelStats {
     uint32_t numBlasNodes, numTlasNodes;
     float msBlas, msTlas;        /* wall time of the last scene setup's builds */
     uint32_t blasReused;         /* 1: no geometry changed since the previous setup, BLASes kept */
+    uint32_t numNodeProgramSteps; /* shader-node program steps of the set-up scene; 0: the
+                                     kernels without node evaluation run */
 } VLRAccelStats;
 VLR_API VLRResult vlrContextGetAccelStats(VLRContextConst context, VLRAccelStats* stats);
 /* Copies one part of the acceleration structure of `scene`, as set up for rendering, to host

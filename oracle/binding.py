@@ -142,6 +142,8 @@ def lib():
         L.oracle_render_bpt.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
                                         C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                         C.POINTER(OracleRenderStats)]
+        L.oracle_render_lt.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                                       C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(OracleRenderStats)]
         L.oracle_sample_lights.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]
         L.oracle_env_lookup.argtypes = [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p]
         L.oracle_pcg32_floats.argtypes = [C.c_uint64, C.c_uint32, C.c_void_p]
@@ -283,6 +285,23 @@ class Scene:
         stats = {"closest_rays": st.closestRays, "shadow_rays": st.shadowRays, "paths": st.paths,
                  "dropped_non_finite": st.droppedNonFinite}
         return out, stats, rs, ls, acc
+
+    def render_lt(self, width, height, frames, max_path_length=25, num_light_paths=500000, threads=0,
+                  light_rng_state=None, want_accum=False):
+        """Light tracing render; returns (image, stats, light-path rng states, accumulation)."""
+        out = np.zeros((height, width, 4), np.float32)
+        st = OracleRenderStats()
+        ls = None if light_rng_state is None else np.ascontiguousarray(light_rng_state, np.uint64)
+        ch = 16 if self.spectral else 3
+        acc = np.zeros((height, width, ch, 2), np.float32) if want_accum else None
+        rc = lib().oracle_render_lt(self.handle, width, height, frames, max_path_length, num_light_paths, threads,
+                                    out.ctypes.data, None if ls is None else ls.ctypes.data,
+                                    None if acc is None else acc.ctypes.data, C.byref(st))
+        if rc != 0:
+            raise RuntimeError("oracle_render_lt failed")
+        stats = {"closest_rays": st.closestRays, "shadow_rays": st.shadowRays, "paths": st.paths,
+                 "dropped_non_finite": st.droppedNonFinite}
+        return out, stats, ls, acc
 
     def sample_lights(self, u: np.ndarray) -> np.ndarray:
         """u: (n, 3) uniform numbers -> LIGHT_SAMPLE_DTYPE records."""

@@ -2107,6 +2107,245 @@ static void renderImpl(const Scene &s, uint32_t W, uint32_t H, uint32_t numFrame
 
 using namespace ov;
 
+// ---- light tracing (light_tracing.cu:49-180 ray generation, 182-263 closest hit) ----
+// One light path per launch index with its own wavelengths; every non-delta vertex (the light
+// vertex included) is connected to a sampled lens point and splatted to the pixel it projects
+// to.  The path ends at a miss, at a zero BSDF sample, by Russian roulette on the throughput's
+// importance relative to its initial value, or at maxPathLength segments.
+template <int N>
+static void splatToPixel(const WLS<N> &wls, const SS<N> &contribution, float u0, float u1, uint32_t W, uint32_t H,
+                         std::vector<float> *splat) {
+    constexpr int C = N == 4 ? 16 : 3;
+    // atomicAddToBuffer (light_tracing.cu:32-46): off-image and non-finite values are dropped
+    uint32_t ipx = static_cast<uint32_t>(u0 * W), ipy = static_cast<uint32_t>(u1 * H);
+    if (ipx >= W || ipy >= H || !contribution.allFinite()) return;
+    float *dst = splat->data() + (size_t(ipy) * W + ipx) * C;
+    if constexpr (N == 4) {
+        SS<16> b = binSpectral(wls, contribution);
+        for (int c = 0; c < 16; ++c) {
+#pragma omp atomic
+            dst[c] += b[c];
+        }
+    } else {
+        for (int c = 0; c < 3; ++c) {
+#pragma omp atomic
+            dst[c] += contribution[c];
+        }
+    }
+}
+
+// a sampled lens point of the perspective camera (cameras.cu:43-76)
+static void sampleLensPoint(const PerspectiveCamera &cam, float ul0, float ul1, SurfPt *spE0, LensIDF *idf,
+                            float *lensAreaPDF) {
+    M3 rot = cam.orientation.toMatrix3x3();
+    float lx, ly;
+    concentricSampleDisk(ul0, ul1, &lx, &ly);
+    V3 orgLocal(cam.lensRadius * lx, cam.lensRadius * ly, 0.0f);
+    spE0->frame.z = normalize(rot * V3(0, 0, 1));
+    spE0->frame.x = normalize(rot * V3(1, 0, 0));
+    spE0->frame.y = cross(spE0->frame.z, spE0->frame.x);
+    spE0->position = rot * orgLocal + cam.position;
+    spE0->geometricNormal = spE0->frame.z;
+    spE0->isPoint = false;
+    spE0->atInfinity = false;
+    spE0->u = lx;
+    spE0->v = ly;
+    *lensAreaPDF = cam.lensRadius > 0.0f ? 1.0f / (kPi * cam.lensRadius * cam.lensRadius) : 1.0f;
+    *idf = LensIDF{cam.sensitivity, cam.lensRadius * lx, cam.lensRadius * ly, cam.opWidth, cam.opHeight,
+                   cam.imgPlaneDistance, cam.objPlaneDistance, cam.imgPlaneArea};
+}
+
+template <int N>
+static void lightTracingPath(const Scene &s, const BPTSettings &bs, PCG32 &rng, uint32_t W, uint32_t H,
+                             std::vector<float> *splat, OracleRenderStats &st) {
+    float wlOff = rng.uniform(), wlSel = rng.uniform();
+    float selectWLPDF;
+    WLS<N> wls = WLS<N>::createWithEqualOffsets(wlOff, wlSel, &selectWLPDF);
+    if (s.lightInstDist.integral <= 0.0f) return;
+    float uLight = rng.uniform();
+    float uP0 = rng.uniform(), uP1 = rng.uniform();
+    LightSample ls;
+    float lightProb;
+    bool valid;
+    sampleLight(s, uLight, uP0, uP1, &ls, &lightProb, &valid);
+    const Material &lmat = s.materials[ls.material];
+    EDF<N> edf = setupEDF<N>(s, lmat, ls.sp, wls);
+    SS<N> Le0 = edf.evaluateEmittance();
+    SS<N> alpha = Le0 / (bs.numLightPaths * lightProb * ls.areaPDF * selectWLPDF);
+
+    // lens connection of the light vertex
+    {
+        float ul0 = rng.uniform(), ul1 = rng.uniform();
+        SurfPt spE0;
+        LensIDF idf;
+        float lensPDF;
+        sampleLensPoint(s.camera, ul0, ul1, &spE0, &idf, &lensPDF);
+        SS<N> We0(idf.sensitivity);
+        V3 dir;
+        float dist2;
+        if (We0.hasNonZero() && testVisibility(s, spE0, ls.sp, &dir, &dist2, &st.shadowRays)) {
+            V3 dirLens = spE0.frame.toLocal(dir);
+            V3 dirSn = ls.sp.frame.toLocal(-dir);
+            SS<N> We = We0 * idf.directionalImportance(dirLens);
+            float u0 = 0, u1 = 0;
+            bool onScreen = idf.project(dirLens, &u0, &u1);
+            SS<N> Le1 = edf.evaluate(dirSn);
+            float cosLens = spE0.calcCosTerm(dir);
+            float cosLight = ls.sp.calcCosTerm(-dir);
+            float G = cosLight * cosLens / dist2;
+            float scalarCoeff = G / lensPDF;
+            SS<N> contribution = alpha * We * Le1 * scalarCoeff;
+            if (onScreen) splatToPixel<N>(wls, contribution, u0, u1, W, H, splat);
+        }
+    }
+
+    float uC = rng.uniform(), uD0 = rng.uniform(), uD1 = rng.uniform();
+    (void)uC;
+    V3 dirLocal;
+    float dirPDF;
+    uint32_t sampledType;
+    SS<N> Le1 = edf.sample(uD0, uD1, &dirLocal, &dirPDF, &sampledType);
+    V3 rayOrg = offsetRayOrigin(ls.sp.position, ls.sp.geometricNormal);
+    if (ls.sp.atInfinity) {
+        rayOrg = s.center + 1.1f * s.worldRadius * ls.sp.position + ls.sp.frame.x * dirLocal.x +
+                 ls.sp.frame.y * dirLocal.y;
+        dirLocal.x = 0;
+        dirLocal.y = 0;
+    }
+    V3 rayDir = ls.sp.frame.fromLocal(dirLocal);
+    alpha *= Le1 * (ls.sp.calcCosTerm(rayDir) / dirPDF);
+    const float initImportance = alpha.importance(wls.sel);
+
+    uint32_t pathLength = 0;
+    while (true) {
+        ++pathLength;
+        if (pathLength >= bs.maxPathLength) break;
+        HitRec hit;
+        ++st.closestRays;
+        if (!traceRay(s, rayOrg, rayDir, 0.0f, FLT_MAX, &hit, false, true)) break;
+        SurfPt sp;
+        float hypAreaPDF;
+        calcSurfacePoint(s, hit, &sp, &hypAreaPDF);
+        const GeomInst &g = s.geomInsts[hit.geomInst];
+        BSDF<N> bsdf = setupBSDF<N>(s.materials[g.material], wls);
+        V3 dirInLocal = sp.frame.toLocal(-rayDir);
+        V3 gnLocal = sp.frame.toLocal(sp.geometricNormal);
+        BSDFQuery<N> q{dirInLocal, gnLocal, DT::All, TransportMode::Importance, wls.sel};
+        if (bsdf.hasNonDelta()) {
+            float ul0 = rng.uniform(), ul1 = rng.uniform();
+            SurfPt spE0;
+            LensIDF idf;
+            float lensPDF;
+            sampleLensPoint(s.camera, ul0, ul1, &spE0, &idf, &lensPDF);
+            SS<N> We0(idf.sensitivity);
+            V3 dir;
+            float dist2;
+            if (We0.hasNonZero() && testVisibility(s, spE0, sp, &dir, &dist2, &st.shadowRays)) {
+                V3 dirLens = spE0.frame.toLocal(dir);
+                V3 dirSn = sp.frame.toLocal(-dir);
+                SS<N> We = We0 * idf.directionalImportance(dirLens);
+                float u0 = 0, u1 = 0;
+                bool onScreen = idf.project(dirLens, &u0, &u1);
+                SS<N> fs = bsdf.evaluate(q, dirSn);
+                float cosLens = spE0.calcCosTerm(dir);
+                float G = absDot(dirSn, gnLocal) * cosLens / dist2;
+                float scalarCoeff = G / lensPDF;
+                SS<N> contribution = alpha * We * fs * scalarCoeff;
+                if (onScreen) splatToPixel<N>(wls, contribution, u0, u1, W, H, splat);
+            }
+        }
+        float sC = rng.uniform(), s0 = rng.uniform(), s1 = rng.uniform();
+        BSDFResult<N> res;
+        SS<N> fs = bsdf.sample(q, sC, s0, s1, &res);
+        if (fs == SS<N>(0.0f) || res.dirPDF == 0.0f) break;
+        if (DT::isDispersive(res.sampledType) && !wls.single) {
+            res.dirPDF /= N;
+            wls.single = true;
+        }
+        float cosFactor = dot(res.dirLocal, gnLocal);
+        alpha *= fs * (std::fabs(cosFactor) / res.dirPDF);
+        rayOrg = offsetRayOrigin(sp.position, cosFactor > 0.0f ? sp.geometricNormal : -sp.geometricNormal);
+        rayDir = sp.frame.fromLocal(res.dirLocal);
+        // Russian roulette (light_tracing.cu:165-169)
+        float continueProb = std::fmin(alpha.importance(wls.sel) / initImportance, 1.0f);
+        if (rng.uniform() >= continueProb) break;
+        alpha /= continueProb;
+    }
+}
+
+// Frames of light tracing: per frame the splats of every light path, added to the Kahan
+// accumulation (reset on the first frame), as context.cpp:1444-1463 and post_process.cu:18-31.
+template <int N, int C>
+static void renderLTImpl(const Scene &s, uint32_t W, uint32_t H, uint32_t numFrames, uint32_t maxPathLength,
+                         uint32_t numLightPaths, int numThreads, float *outRGBA, uint64_t *lightRngState,
+                         float *accumOut, OracleRenderStats *stats) {
+    std::vector<uint64_t> lrngs(numLightPaths);
+    if (lightRngState) std::memcpy(lrngs.data(), lightRngState, lrngs.size() * 8);
+    else {
+        std::mt19937_64 seedRng(459182033132123413ULL);
+        for (auto &r : lrngs) r = seedRng();
+    }
+    BPTSettings bs{numLightPaths, maxPathLength};
+    std::vector<Kahan<SS<C>>> accum(size_t(W) * H);
+    for (auto &a : accum) a.reset();
+    OracleRenderStats total = {};
+    if (numThreads > 0) omp_set_num_threads(numThreads);
+    for (uint32_t frame = 1; frame <= numFrames; ++frame) {
+        std::vector<float> splat(size_t(W) * H * C, 0.0f);
+#pragma omp parallel
+        {
+            OracleRenderStats st = {};
+#pragma omp for schedule(dynamic, 256)
+            for (int64_t i = 0; i < int64_t(numLightPaths); ++i) {
+                PCG32 rng{lrngs[i]};
+                lightTracingPath<N>(s, bs, rng, W, H, &splat, st);
+                lrngs[i] = rng.state;
+                ++st.paths;
+            }
+#pragma omp critical
+            {
+                total.closestRays += st.closestRays;
+                total.shadowRays += st.shadowRays;
+                total.paths += st.paths;
+            }
+        }
+        for (size_t p = 0; p < accum.size(); ++p) {
+            if (frame == 1) accum[p].reset();
+            SS<C> sv;
+            for (int c = 0; c < C; ++c) sv[c] = splat[p * C + c];
+            accum[p].add(sv);
+        }
+    }
+    if (lightRngState) std::memcpy(lightRngState, lrngs.data(), lrngs.size() * 8);
+    const float recN = 1.0f / numFrames;
+    const float resCorrection = float(W) * float(H);
+    for (size_t p = 0; p < accum.size(); ++p) {
+        float XYZ[3];
+        if constexpr (N == 3) {
+            float RGB[3] = {accum[p].result[0], accum[p].result[1], accum[p].result[2]};
+            transformTristimulus(mat_Rec709_D65_to_XYZ, RGB, XYZ);
+        } else {
+            strataToXYZ(accum[p].result, XYZ);
+        }
+        XYZ[0] *= recN; XYZ[1] *= recN; XYZ[2] *= recN;
+        float RGB[3];
+        transformTristimulus(mat_XYZ_to_Rec709_D65, XYZ, RGB);
+        if (outRGBA) {
+            outRGBA[4 * p + 0] = RGB[0] * resCorrection;
+            outRGBA[4 * p + 1] = RGB[1] * resCorrection;
+            outRGBA[4 * p + 2] = RGB[2] * resCorrection;
+            outRGBA[4 * p + 3] = 1.0f;
+        }
+        if (accumOut) {
+            for (int c = 0; c < C; ++c) {
+                accumOut[(p * C + c) * 2 + 0] = accum[p].result[c];
+                accumOut[(p * C + c) * 2 + 1] = accum[p].comp[c];
+            }
+        }
+    }
+    if (stats) *stats = total;
+}
+
 extern "C" {
 
 int oracle_load_tables(const char *path) { return g_tables.load(path) ? 0 : 1; }
@@ -2152,6 +2391,20 @@ int oracle_trace_any(void *sp, const OracleRay *rays, uint32_t n, uint8_t *occ, 
         occ[i] = traceRay(s, V3(r.org[0], r.org[1], r.org[2]), V3(r.dir[0], r.dir[1], r.dir[2]), r.tmin, r.tmax,
                           nullptr, true, useBVH != 0) ? 1 : 0;
     }
+    return 0;
+}
+
+int oracle_render_lt(void *sp, uint32_t W, uint32_t H, uint32_t numFrames, uint32_t maxPathLength,
+                     uint32_t numLightPaths, int numThreads, float *outRGBA, uint64_t *lightRngState, float *accumOut,
+                     OracleRenderStats *stats) {
+    Scene *s = static_cast<Scene *>(sp);
+    if (!s || numLightPaths == 0) return 1;
+    if (s->spectral)
+        renderLTImpl<4, 16>(*s, W, H, numFrames, maxPathLength, numLightPaths, numThreads, outRGBA, lightRngState,
+                            accumOut, stats);
+    else
+        renderLTImpl<3, 3>(*s, W, H, numFrames, maxPathLength, numLightPaths, numThreads, outRGBA, lightRngState,
+                           accumOut, stats);
     return 0;
 }
 

@@ -115,6 +115,11 @@ int oracle_render(void* scene, uint32_t width, uint32_t height, uint32_t numFram
 int oracle_render_bpt(void* scene, uint32_t width, uint32_t height, uint32_t numFrames, uint32_t maxPathLength,
                       uint32_t numLightPaths, int numThreads, float* outRGBA, uint64_t* rngState,
                       uint64_t* lightRngState, float* accumOut, OracleRenderStats* stats);
+/* Light tracing (light_tracing.cu:49-263): numLightPaths light paths per frame, each splatting
+ * its lens connections; lightRngState as for oracle_render_bpt. */
+int oracle_render_lt(void* scene, uint32_t width, uint32_t height, uint32_t numFrames, uint32_t maxPathLength,
+                     uint32_t numLightPaths, int numThreads, float* outRGBA, uint64_t* lightRngState,
+                     float* accumOut, OracleRenderStats* stats);
 
 /* BSDF parity helper: builds the BSDF of `mat` (immediate values only) for the given
  * wavelength set and evaluates sample / evaluate / evaluatePDF in the shading frame. */

@@ -291,6 +291,25 @@ def test_two_texel_albedo_matches_split_floor(torch_gpu):
     compare_where_paths_match(img_gpu, rng_gpu, ref, rng_ref)
 
 
+def test_node_free_scenes_run_the_kernels_without_node_evaluation(torch_gpu):
+    """The environment texture is read by the environment emitter itself: the icosphere grid
+    sets up no node program, so its frames run the kernel variants without node evaluation."""
+    desc = scenes.icosphere_grid(grid=2, env_size=(64, 32))
+    with api.Context(spectral=True, max_path_length=4) as ctx:
+        bs = api.build_scene(ctx, desc)
+        ctx.bind_output(16, 16)
+        ctx.render(bs.camera, True, 1)
+        assert ctx.accel_stats()["numNodeProgramSteps"] == 0
+    textured = cornell_with_floor_group()
+    tex = NodeDesc("Image2DTexture", image=flat_map([0.2, 0.4, 0.6, 1.0]))
+    textured.materials[0] = scenes.MaterialDesc(scenes.MATTE, plugs={"albedo": (tex, P["Spectrum"], 0)})
+    with api.Context(spectral=False, max_path_length=4) as ctx:
+        bs = api.build_scene(ctx, textured)
+        ctx.bind_output(16, 16)
+        ctx.render(bs.camera, True, 1)
+        assert ctx.accel_stats()["numNodeProgramSteps"] == 1
+
+
 def flat_map(rgba):
     t = np.array([[rgba]], np.float32)
     return ImageDesc(t.view(np.uint8).reshape(1, 1, 16), 1, 1, "RGBA32Fx4", "NA", REC709)

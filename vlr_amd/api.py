@@ -61,7 +61,7 @@ class VLRShaderNodePlug(C.Structure):
 class VLRAccelStats(C.Structure):
     _fields_ = [("numBlas", C.c_uint32), ("numTlasInstances", C.c_uint32), ("numTriangles", C.c_uint64),
                 ("numBlasNodes", C.c_uint32), ("numTlasNodes", C.c_uint32), ("msBlas", C.c_float),
-                ("msTlas", C.c_float), ("blasReused", C.c_uint32)]
+                ("msTlas", C.c_float), ("blasReused", C.c_uint32), ("numNodeProgramSteps", C.c_uint32)]
 
 
 # 64-byte quantised 4-wide BVH node: child k's box on axis a is

@@ -1459,6 +1459,7 @@ VLR_API VLRResult vlrContextGetAccelStats(VLRContextConst context, VLRAccelStats
     stats->msBlas = a.msBlas;
     stats->msTlas = a.msTlas;
     stats->blasReused = a.blasReused;
+    stats->numNodeProgramSteps = a.numNodeProgramSteps;
     return VLRResult_NoError;
     VLRA_CATCH
 }

@@ -132,9 +132,11 @@ void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCo
     if (!camera) throw ApiError{3, "camera is null"};
     if (!camera->perspective) throw ApiError{4, "only the perspective camera is on the render path"};
     if (width == 0 || height == 0) throw ApiError{3, "bind an output buffer first"};
-    if (renderer != VLRRenderer_PathTracing && renderer != VLRRenderer_BPT)
-        throw ApiError{3, "renderer not available on this build: VLRRenderer_PathTracing and VLRRenderer_BPT"};
+    if (renderer != VLRRenderer_PathTracing && renderer != VLRRenderer_BPT && renderer != VLRRenderer_LightTracing)
+        throw ApiError{3, "renderer not available on this build: VLRRenderer_PathTracing, VLRRenderer_LightTracing "
+                          "and VLRRenderer_BPT"};
     const bool bpt = renderer == VLRRenderer_BPT;
+    const bool lightTracing = renderer == VLRRenderer_LightTracing;
     if (shrinkCoeff == 0) shrinkCoeff = 1;
     if (firstFrame) {
         if (spectralActive != spectralRequested) {
@@ -167,8 +169,8 @@ void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCo
         fp.trainOffset = 0;
         fp.pixelBegin = 0;
         fp.pixelCount = fp.imageW * fp.imageH;
-        if (bpt) {
-            renderBptFrame(stream, fp, firstFrame);
+        if (bpt || lightTracing) {
+            renderBptFrame(stream, fp, firstFrame, lightTracing);
             if (numAccum) *numAccum = numAccumFrames;
             return;
         }
@@ -392,17 +394,17 @@ void Context::ensureBpt(uint32_t slotsPerPath) {
     }
 }
 
-void Context::renderBptFrame(hipStream_t stream, const FrameParams &fp, bool firstFrame) {
+void Context::renderBptFrame(hipStream_t stream, const FrameParams &fp, bool firstFrame, bool lightTracing) {
     joinHalves(stream);
     const uint32_t slotsPerPath = fp.maxPathLength + 1;
     ensureBpt(slotsPerPath);
     if (firstFrame)
         VLRA_HIP(hipMemsetAsync(splat.ptr, 0, size_t(width) * height * accumChannels * sizeof(float), stream));
-    // common wavelengths of the frame: two draws, offset first (WavelengthSamples::
-    // createWithEqualOffsets, spectrum_types.h)
+    // common wavelengths of a BPT frame: two draws, offset first (WavelengthSamples::
+    // createWithEqualOffsets, spectrum_types.h); light tracing draws per light path instead
     std::uniform_real_distribution<float> u01;
-    const float wlOffset = u01(bptWlRng);
-    const float wlSelect = u01(bptWlRng);
+    const float wlOffset = lightTracing ? 0.0f : u01(bptWlRng);
+    const float wlSelect = lightTracing ? 0.0f : u01(bptWlRng);
     const uint32_t N = spectralActive ? 4 : 3;
     BptParams bp;
     bp.wlOffset = wlOffset;
@@ -419,16 +421,17 @@ void Context::renderBptFrame(hipStream_t stream, const FrameParams &fp, bool fir
     bp.lightRng = static_cast<uint64_t *>(lightRng.ptr);
     bp.splat = static_cast<float *>(splat.ptr);
     const FrameBuffers fb = frameBuffers();
+    auto launch = lightTracing ? launchLightTraceFrame : launchBptFrame;
     if (kernelTiming) {
         EventTimer t(stream, eventPool);
-        VLRA_HIP(launchBptFrame(gpu.dev, fp, fb, bp, spectralActive, stream, &t));
+        VLRA_HIP(launch(gpu.dev, fp, fb, bp, spectralActive, stream, &t));
         t.collect(stageMs, launches);
     } else {
-        VLRA_HIP(launchBptFrame(gpu.dev, fp, fb, bp, spectralActive, stream, nullptr));
+        VLRA_HIP(launch(gpu.dev, fp, fb, bp, spectralActive, stream, nullptr));
     }
     lastFrame = fp;
     ++framesRendered;
-    pathsStarted += uint64_t(fp.imageW) * fp.imageH;
+    pathsStarted += lightTracing ? uint64_t(numLightPaths) : uint64_t(fp.imageW) * fp.imageH;
 }
 
 void Context::readLightRngStates(uint64_t *dst) {

@@ -2293,9 +2293,208 @@ __global__ void __launch_bounds__(kTraceBlock) k_bptEye(BptArgs a) {
     }
 }
 
+// ---- light tracing (light_tracing.cu:49-180 ray generation, 182-263 closest hit) ----
+// One thread per light path with its own wavelengths.  The light vertex and every later
+// vertex with a non-delta BSDF are connected to a sampled lens point; the connection is
+// splatted to the pixel it projects to.  The path ends at a miss, a zero BSDF sample, Russian
+// roulette on the throughput's importance relative to its initial value, or maxPathLength.
+template <int N>
+__device__ __forceinline__ void splatContribution(const BptArgs &a, const Wls<N> &wls, const Spec<N> &c, float u0,
+                                                  float u1) {
+    // atomicAddToBuffer (light_tracing.cu:32-46): values off the image or not finite are dropped
+    const uint32_t W = a.fp.imageW, H = a.fp.imageH;
+    const uint32_t ipx = static_cast<uint32_t>(u0 * W), ipy = static_cast<uint32_t>(u1 * H);
+    if (ipx >= W || ipy >= H || !c.allFinite()) return;
+    const size_t plane = static_cast<size_t>(a.fp.width) * a.fp.height;
+    float *dst = a.bp.splat + static_cast<size_t>(ipy) * a.fp.width + ipx;
+    if constexpr (N == 4) {
+        const float recBinWidth = 16 / (830.0f - 360.0f);
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            const float lam = wls.lambda(w);
+            const uint32_t bin = min(static_cast<uint32_t>((lam - 360.0f) / (830.0f - 360.0f) * 16), 15u);
+            atomicAdd(dst + bin * plane, c[w] * recBinWidth);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) atomicAdd(dst + k * plane, c[k]);
+    }
+}
+
+// a sampled point on the perspective camera's lens (cameras.cu:43-76) and its importance function
+__device__ __forceinline__ void sampleLens(const DevCamera &cam, float ul0, float ul1, SurfPt *spE, LensIdf *idf,
+                                           float *lensAreaPDF) {
+    float lx, ly;
+    concentricSampleDisk(ul0, ul1, &lx, &ly);
+    V3 c0(cam.rot[0], cam.rot[1], cam.rot[2]), c1(cam.rot[3], cam.rot[4], cam.rot[5]), c2(cam.rot[6], cam.rot[7], cam.rot[8]);
+    spE->frame.z = normalize(c2);
+    spE->frame.x = normalize(c0);
+    spE->frame.y = cross(spE->frame.z, spE->frame.x);
+    V3 orgLocal(cam.lensRadius * lx, cam.lensRadius * ly, 0.0f);
+    spE->position = (c0 * orgLocal.x + c1 * orgLocal.y + c2 * orgLocal.z) + ld3(cam.position);
+    spE->gn = spE->frame.z;
+    spE->isPoint = false;
+    spE->atInfinity = false;
+    *lensAreaPDF = cam.lensRadius > 0.0f ? 1.0f / (kPiF * cam.lensRadius * cam.lensRadius) : 1.0f;
+    *idf = LensIdf{cam.lensRadius * lx, cam.lensRadius * ly, cam.opWidth, cam.opHeight, cam.imgPlaneDistance,
+                   cam.objPlaneDistance, cam.imgPlaneArea};
+}
+
+template <int N, bool NODES = true>
+__global__ void __launch_bounds__(kTraceBlock) k_lightTrace(BptArgs a) {
+    __shared__ int32_t lds[kTraceLds];
+    const DevScene &sc = a.sc;
+    const BptParams &bp = a.bp;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t closestRays = 0, shadowRays = 0;
+    if (i < bp.numLightPaths) {
+        Pcg32 rng{bp.lightRng[i]};
+        // the path's own wavelengths, offset drawn first (createWithEqualOffsets)
+        Wls<N> wls;
+        wls.offset = rng.uniform();
+        const float wlSelect = rng.uniform();
+        wls.sel = min(static_cast<uint32_t>(static_cast<uint16_t>(N * wlSelect)), static_cast<uint32_t>(N - 1));
+        wls.single = false;
+        const float selectWLPDF = N == 4 ? N / (830.0f - 360.0f) : 1.0f;
+        const DevCamera &cam = sc.camera;
+        if (sc.lightInstIntegral > 0.0f) {
+            const float uLight = rng.uniform();
+            const float uP0 = rng.uniform(), uP1 = rng.uniform();
+            LightSample ls;
+            float lightProb;
+            sampleLight(sc, uLight, uP0, uP1, &ls, &lightProb);
+            const DevMaterial &lm = sc.materials[ls.material];
+            Spec<N> Le0;
+            float fe;
+            evalEDF<N, NODES>(sc, lm, ls.sp, wls, V3(0, 0, 1), &Le0, &fe);
+            Spec<N> alpha = Le0 / (bp.numLightPaths * lightProb * ls.areaPDF * selectWLPDF);
+            {
+                // lens connection of the light vertex
+                const float ul0 = rng.uniform(), ul1 = rng.uniform();
+                SurfPt spE;
+                LensIdf idf;
+                float lensPDF;
+                sampleLens(cam, ul0, ul1, &spE, &idf, &lensPDF);
+                V3 dir;
+                float dist2;
+                if (cam.sensitivity != 0.0f) {
+                    ++shadowRays;
+                    if (connectVisible(sc, spE, ls.sp, &dir, &dist2, lds, threadIdx.x)) {
+                        const V3 dirLens = spE.frame.toLocal(dir);
+                        const V3 dirSn = ls.sp.frame.toLocal(-dir);
+                        float u0 = 0.0f, u1 = 0.0f;
+                        const bool onScreen = idf.project(dirLens, &u0, &u1);
+                        Spec<N> M;
+                        float fe1;
+                        evalEDF<N, NODES>(sc, lm, ls.sp, wls, dirSn, &M, &fe1);
+                        const float G = ls.sp.cosTerm(-dir) * spE.cosTerm(dir) / dist2;
+                        const float scalarCoeff = G / lensPDF;
+                        const Spec<N> contribution = alpha * Spec<N>(cam.sensitivity) * Spec<N>(fe1) * scalarCoeff;
+                        if (onScreen) splatContribution<N>(a, wls, contribution, u0, u1);
+                    }
+                }
+            }
+            // EDF::sample (materials.cu:2052-2063, 2206-2213, 2849-2860)
+            const float uC = rng.uniform(), uD0 = rng.uniform(), uD1 = rng.uniform();
+            (void)uC;
+            V3 dirLocal;
+            float dirPDF, fe1;
+            if (lm.type == MAT_POINT_EMITTER) {
+                dirLocal = uniformSampleSphere(uD0, uD1);
+                dirPDF = 1.0f / (4 * kPiF);
+                fe1 = 1.0f / (4 * kPiF);
+            } else if (lm.type == MAT_ENVIRONMENT_EMITTER) {
+                float dx, dy;
+                concentricSampleDisk(uD0, uD1, &dx, &dy);
+                dirLocal = V3(sc.worldRadius * dx, sc.worldRadius * dy, 1);
+                dirPDF = 1.0f / sc.worldDiscArea;
+                fe1 = 1 / kPiF;
+            } else {
+                dirLocal = cosineSampleHemisphere(uD0, uD1);
+                dirPDF = dirLocal.z / kPiF;
+                fe1 = 1.0f / kPiF;
+            }
+            V3 rayOrg = offsetRayOrigin(ls.sp.position, ls.sp.gn);
+            if (ls.sp.atInfinity) {
+                rayOrg = ld3(sc.center) + 1.1f * sc.worldRadius * ls.sp.position + ls.sp.frame.x * dirLocal.x +
+                         ls.sp.frame.y * dirLocal.y;
+                dirLocal.x = 0;
+                dirLocal.y = 0;
+            }
+            V3 rayDir = ls.sp.frame.fromLocal(dirLocal);
+            alpha *= Spec<N>(fe1) * (ls.sp.cosTerm(rayDir) / dirPDF);
+            const float initImportance = alpha.importance(wls.sel);
+            uint32_t pathLength = 0;
+            while (true) {
+                ++pathLength;
+                if (pathLength >= a.fp.maxPathLength) break;
+                ++closestRays;
+                TraceHit h;
+                h.inst = -1;
+                h.b1 = h.b2 = 0.0f;
+                if (!traceRay<false>(sc, rayOrg, rayDir, 0.0f, 3.402823466e+38f, &h, lds, threadIdx.x)) break;
+                SurfPt sp;
+                float hypAreaPDF;
+                decodeHit<NODES>(sc, h.inst, h.geomInst, h.prim, h.b1, h.b2, &sp, &hypAreaPDF);
+                const DevGeomInst &G = sc.geomInsts[h.geomInst];
+                const Bsdf<N> bsdf = setupBsdf<N, NODES>(sc, sc.materials[G.material], wls, &sp);
+                const V3 dirInLocal = sp.frame.toLocal(-rayDir);
+                const V3 gnLocal = sp.frame.toLocal(sp.gn);
+                if (bsdf.hasNonDelta()) {
+                    const float ul0 = rng.uniform(), ul1 = rng.uniform();
+                    SurfPt spE;
+                    LensIdf idf;
+                    float lensPDF;
+                    sampleLens(cam, ul0, ul1, &spE, &idf, &lensPDF);
+                    V3 dir;
+                    float dist2;
+                    if (cam.sensitivity != 0.0f) {
+                        ++shadowRays;
+                        if (connectVisible(sc, spE, sp, &dir, &dist2, lds, threadIdx.x)) {
+                            const V3 dirLens = spE.frame.toLocal(dir);
+                            const V3 dirSn = sp.frame.toLocal(-dir);
+                            float u0 = 0.0f, u1 = 0.0f;
+                            const bool onScreen = idf.project(dirLens, &u0, &u1);
+                            const Spec<N> fs = bsdf.evaluate(dirInLocal, gnLocal, dirSn, wls.sel, true);
+                            const float Gc = absDot(dirSn, gnLocal) * spE.cosTerm(dir) / dist2;
+                            const float scalarCoeff = Gc / lensPDF;
+                            const Spec<N> contribution = alpha * Spec<N>(cam.sensitivity) * fs * scalarCoeff;
+                            if (onScreen) splatContribution<N>(a, wls, contribution, u0, u1);
+                        }
+                    }
+                }
+                const float sC = rng.uniform(), s0 = rng.uniform(), s1 = rng.uniform();
+                V3 dl;
+                float pdf;
+                uint32_t stype;
+                const Spec<N> fs = bsdf.sample(dirInLocal, gnLocal, wls.sel, sC, s0, s1, &dl, &pdf, &stype, true);
+                if (fs.isZero() || pdf == 0.0f) break;
+                if ((stype & dt::Dispersive) && !wls.single) {
+                    pdf /= N;
+                    wls.single = true;
+                }
+                const float cosFactor = dot(dl, gnLocal);
+                alpha *= fs * (fabsf(cosFactor) / pdf);
+                rayOrg = offsetRayOrigin(sp.position, cosFactor > 0.0f ? sp.gn : -sp.gn);
+                rayDir = sp.frame.fromLocal(dl);
+                // Russian roulette (light_tracing.cu:165-169)
+                const float continueProb = fminf(alpha.importance(wls.sel) / initImportance, 1.0f);
+                if (rng.uniform() >= continueProb) break;
+                alpha = alpha / continueProb;
+            }
+        }
+        bp.lightRng[i] = rng.s;
+    }
+    const uint32_t c = waveSum(closestRays), sh = waveSum(shadowRays);
+    if (__lane_id() == 0) {
+        if (c) atomicAdd(reinterpret_cast<unsigned long long *>(a.fb.counters + kStatClosest), c);
+        if (sh) atomicAdd(reinterpret_cast<unsigned long long *>(a.fb.counters + kStatShadow), sh);
+    }
+}
+
 // Eye contribution, then the frame's splats, into the Kahan accumulation (lvc_bpt.cu:560-565,
 // post_process.cu:18-31); the splat planes are cleared for the next frame.
-template <int N>
+template <int N, bool EYE = true>
 __global__ void __launch_bounds__(256) k_bptAccumulate(BptArgs a) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t numPaths = a.fp.imageW * a.fp.imageH;
@@ -2306,7 +2505,8 @@ __global__ void __launch_bounds__(256) k_bptAccumulate(BptArgs a) {
     const size_t plane = static_cast<size_t>(a.fp.width) * a.fp.height;
     float *acc = a.fb.accum + p;
     const bool reset = a.fp.frameIndex == 1;
-    Spec<N> c = loadSpec<N>(a.fb.ps.contrib[p]);
+    // light tracing (EYE = false) has no eye contribution: only the splats are added
+    Spec<N> c = EYE ? loadSpec<N>(a.fb.ps.contrib[p]) : Spec<N>(0.0f);
     bool finite = c.allFinite();
     if (!finite) atomicAdd(&a.fb.counters[kStatDropped], 1u);
     float addend[C];
@@ -2329,7 +2529,7 @@ __global__ void __launch_bounds__(256) k_bptAccumulate(BptArgs a) {
     for (int k = 0; k < C; ++k) {
         float sum = reset ? 0.0f : acc[(2 * k) * plane];
         float comp = reset ? 0.0f : acc[(2 * k + 1) * plane];
-        if (finite) {
+        if (EYE && finite) {
             float cIn = addend[k] - comp;
             float tmp = sum + cIn;
             comp = (tmp - sum) - cIn;
@@ -2669,6 +2869,49 @@ static hipError_t launchBptFrameT(const DevScene &sc, const FrameParams &fp, con
     hipLaunchKernelGGL(k_convert<N>, dim3(blocksFor(n, 256)), dim3(256), 0, s, k);
     t1(STAGE_CONVERT);
     return hipGetLastError();
+}
+
+// light tracing (context.cpp:1444-1463): the light paths splat, then the splats are added to
+// the accumulation (reset on the first frame) and converted
+template <int N>
+static hipError_t launchLightTraceFrameT(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb,
+                                         const BptParams &bp, hipStream_t s, LaunchTimer *timer) {
+    BptArgs a;
+    a.sc = sc;
+    a.fp = fp;
+    a.fb = fb;
+    a.bp = bp;
+    KArgs k;
+    k.sc = sc;
+    k.fp = fp;
+    k.fb = fb;
+    k.depth = 0;
+    k.jobs = nullptr;
+    k.jobOffsets = nullptr;
+    k.numJobs = 0;
+    const uint32_t n = fp.imageW * fp.imageH;
+    if (bp.numLightPaths == 0) return hipErrorInvalidValue;
+    auto t0 = [&](int st) { if (timer) timer->begin(st); };
+    auto t1 = [&](int st) { if (timer) timer->end(st); };
+    t0(STAGE_LIGHT_PATHS);
+    if (sc.hasNodePrograms)
+        hipLaunchKernelGGL((k_lightTrace<N, true>), dim3(blocksFor(bp.numLightPaths, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_lightTrace<N, false>), dim3(blocksFor(bp.numLightPaths, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
+    t1(STAGE_LIGHT_PATHS);
+    t0(STAGE_ACCUMULATE);
+    hipLaunchKernelGGL((k_bptAccumulate<N, false>), dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
+    t1(STAGE_ACCUMULATE);
+    t0(STAGE_CONVERT);
+    hipLaunchKernelGGL(k_convert<N>, dim3(blocksFor(n, 256)), dim3(256), 0, s, k);
+    t1(STAGE_CONVERT);
+    return hipGetLastError();
+}
+
+hipError_t launchLightTraceFrame(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb,
+                                 const BptParams &bp, bool spectral, hipStream_t s, LaunchTimer *timer) {
+    return spectral ? launchLightTraceFrameT<4>(sc, fp, fb, bp, s, timer)
+                    : launchLightTraceFrameT<3>(sc, fp, fb, bp, s, timer);
 }
 
 hipError_t launchBptFrame(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb, const BptParams &bp,

@@ -209,6 +209,8 @@ hipError_t launchBatchConvert(const BatchLaunch& b, bool spectral, hipStream_t s
 // One LVC-BPT frame: light paths, cache compaction, eye paths, accumulation, conversion.
 hipError_t launchBptFrame(const DevScene& scene, const FrameParams& fp, const FrameBuffers& fb, const BptParams& bp,
                           bool spectral, hipStream_t stream, LaunchTimer* timer);
+hipError_t launchLightTraceFrame(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb,
+                                 const BptParams &bp, bool spectral, hipStream_t s, LaunchTimer *timer);
 
 hipError_t launchTraceRays(const DevScene& scene, const void* rays, uint32_t numRays, void* hits, bool anyHit,
                            hipStream_t stream);

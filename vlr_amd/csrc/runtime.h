@@ -406,6 +406,7 @@ struct AccelStats {
     uint32_t numBlasNodes = 0, numTlasNodes = 0;
     float msBlas = 0.0f, msTlas = 0.0f;
     uint32_t blasReused = 0;
+    uint32_t numNodeProgramSteps = 0;
 };
 
 void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *out);
@@ -528,7 +529,7 @@ public:
     // LVC-BPT buffers for the current light-path count, path length and output size
     void ensureBpt(uint32_t slotsPerPath);
     void setNumLightPaths(uint32_t n);
-    void renderBptFrame(hipStream_t stream, const FrameParams &fp, bool firstFrame);
+    void renderBptFrame(hipStream_t stream, const FrameParams &fp, bool firstFrame, bool lightTracing);
     void readLightRngStates(uint64_t *dst);
     void writeLightRngStates(const uint64_t *src);
     // frames rendered as two pixel halves on two streams (one half's deep paths overlap the

@@ -599,6 +599,7 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
     if (shaders.texels.empty()) shaders.texels.push_back(make_float4(0, 0, 0, 0));
     if (shaders.pool.empty()) shaders.pool.push_back(0.0f);
     const bool hasPrograms = !shaders.steps.empty();
+    as.numNodeProgramSteps = static_cast<uint32_t>(shaders.steps.size());
     if (shaders.steps.empty()) shaders.steps.emplace_back();
     g->shaderNodes.upload(shaders.nodes.data(), shaders.nodes.size() * sizeof(DevShaderNode));
     g->nodeSteps.upload(shaders.steps.data(), shaders.steps.size() * sizeof(DevNodeStep));
