@@ -30,7 +30,8 @@ class OracleMaterial(C.Structure):
 
 class OracleGeomInst(C.Structure):
     _fields_ = [("vertices", C.c_void_p), ("numVertices", C.c_uint32), ("indices", C.c_void_p),
-                ("numIndices", C.c_uint32), ("material", C.c_int32), ("isPoints", C.c_int32)]
+                ("numIndices", C.c_uint32), ("material", C.c_int32), ("isPoints", C.c_int32),
+                ("alpha", C.c_float)]
 
 
 class OracleInstance(C.Structure):
@@ -195,10 +196,11 @@ class Scene:
             v = np.ascontiguousarray(mesh.vertices, np.float32)
             self._keep.append(v)
             ids = []
-            for idx, mat in mesh.groups:
+            for gi, (idx, mat) in enumerate(mesh.groups):
                 ix = np.ascontiguousarray(idx, np.uint32)
                 self._keep.append(ix)
-                g = OracleGeomInst(v.ctypes.data, len(v), ix.ctypes.data, len(ix), mat, 1 if mesh.points else 0)
+                alpha = float(getattr(mesh, "group_alpha", {}).get(gi, -1.0))
+                g = OracleGeomInst(v.ctypes.data, len(v), ix.ctypes.data, len(ix), mat, 1 if mesh.points else 0, alpha)
                 ids.append(len(geoms))
                 geoms.append(g)
             mesh_to_geoms.append(ids)

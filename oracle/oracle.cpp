@@ -409,6 +409,7 @@ struct GeomInst {
     bool isPoints = false;
     int material = 0;
     float importance = 0;
+    float alpha = -1.0f;  // constant alpha node of the group; < 0: none
     Discrete1D primDist;  // only for emitters (scene.cpp:209-219, 378-393)
     AABB aabb;
     uint32_t numPrims() const { return isPoints ? uint32_t(idx.size()) : uint32_t(idx.size() / 3); }
@@ -495,6 +496,7 @@ static Scene *buildScene(const OracleScene *d, bool spectral) {
         o.idx.assign(g.indices, g.indices + g.numIndices);
         o.isPoints = g.isPoints != 0;
         o.material = g.material;
+        o.alpha = g.isPoints ? -1.0f : g.alpha;
         bool emit = s->materials[g.material].emitting();
         o.importance = emit ? 1.0f : 0.0f;
         if (o.isPoints) {
@@ -650,6 +652,23 @@ struct HitRec {
     float t = 0, b1 = 0, b2 = 0;
 };
 
+// Stochastic alpha test of closest-hit rays (path_tracing.cu:14-25) with the path's generator,
+// fractional visibility of any-hit rays (light_transport_common.h:205-218); set per trace by
+// the caller.  Candidates are met in this BVH's traversal order, not the device's, so only the
+// statistics of alpha scenes compare.
+static thread_local PCG32 *tlsAlphaRng = nullptr;
+static thread_local float tlsVisibility = 1.0f;
+
+// false: the candidate is ignored
+static inline bool alphaAccept(const GeomInst &g, bool anyHit) {
+    if (g.alpha < 0.0f) return true;
+    if (anyHit) {
+        tlsVisibility *= 1 - g.alpha;
+        return tlsVisibility == 0.0f;
+    }
+    return !(tlsAlphaRng && tlsAlphaRng->uniform() >= g.alpha);
+}
+
 static void traceInstanceBrute(const Scene &s, uint32_t ii, const RayPre &rw, float tmin, float &tmax, HitRec &h,
                                bool anyHit, bool *found) {
     const Instance &in = s.instances[ii];
@@ -662,6 +681,7 @@ static void traceInstanceBrute(const Scene &s, uint32_t ii, const RayPre &rw, fl
             float tt, b1, b2;
             if (intersectTriangle(r, g.pos(g.idx[3 * t]), g.pos(g.idx[3 * t + 1]), g.pos(g.idx[3 * t + 2]), tmin, tmax,
                                   &tt, &b1, &b2)) {
+                if (!alphaAccept(g, anyHit)) continue;
                 tmax = tt;
                 h.inst = static_cast<int32_t>(ii);
                 h.geomInst = static_cast<int32_t>(gi);
@@ -698,6 +718,7 @@ static void traceInstanceBVH(const Scene &s, uint32_t ii, const RayPre &rw, floa
                 float tt, b1, b2;
                 if (intersectTriangle(r, g.pos(g.idx[3 * t]), g.pos(g.idx[3 * t + 1]), g.pos(g.idx[3 * t + 2]), tmin,
                                       tmax, &tt, &b1, &b2)) {
+                    if (!alphaAccept(g, anyHit)) continue;
                     tmax = tt;
                     h.inst = static_cast<int32_t>(ii);
                     h.geomInst = static_cast<int32_t>(gi);
@@ -1042,7 +1063,7 @@ static void sampleLight(const Scene &s, float uLight, float uP0, float uP1, Ligh
 
 // light_transport_common.h:147-177
 static bool testVisibility(const Scene &s, const SurfPt &shading, const SurfPt &light, V3 *shadowDir, float *dist2,
-                           uint64_t *shadowRays) {
+                           uint64_t *shadowRays, float *fractionalVisibility = nullptr) {
     *shadowDir = light.calcDirectionFrom(shading.position, dist2);
     bool front = dot(shading.geometricNormal, *shadowDir) > 0;
     V3 sp = offsetRayOrigin(shading.position, front ? shading.geometricNormal : -shading.geometricNormal);
@@ -1052,7 +1073,10 @@ static bool testVisibility(const Scene &s, const SurfPt &shading, const SurfPt &
     if (!light.atInfinity)
         tmax = distance(lp, sp) * 0.9999f;
     ++*shadowRays;
-    return !traceRay(s, sp, *shadowDir, 0.0f, tmax, nullptr, true, true);
+    tlsVisibility = 1.0f;
+    const bool visible = !traceRay(s, sp, *shadowDir, 0.0f, tmax, nullptr, true, true);
+    if (fractionalVisibility) *fractionalVisibility = visible ? tlsVisibility : 0.0f;
+    return visible;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -1107,7 +1131,9 @@ static SS<N> tracePath(const Scene &s, PCG32 &rng, uint32_t px, uint32_t py, uin
             maxLengthTerminate = true;
         HitRec hit;
         ++st.closestRays;
+        tlsAlphaRng = &rng;
         bool found = traceRay(s, rayOrg, rayDir, 0.0f, FLT_MAX, &hit, false, true);
+        tlsAlphaRng = nullptr;
         if (!found) {
             // miss program (path_tracing.cu:266-337)
             if (!s.hasEnv || s.envImportance == 0)
@@ -1205,7 +1231,8 @@ static SS<N> tracePath(const Scene &s, PCG32 &rng, uint32_t px, uint32_t py, uin
             SS<N> M = ledf.evaluateEmittance();
             V3 shadowDir;
             float dist2;
-            if (M.hasNonZero() && testVisibility(s, sp, ls.sp, &shadowDir, &dist2, &st.shadowRays)) {
+            float fracVis = 1.0f;
+            if (M.hasNonZero() && testVisibility(s, sp, ls.sp, &shadowDir, &dist2, &st.shadowRays, &fracVis)) {
                 float rec = 1.0f / dist2;
                 V3 sdl = ls.sp.frame.toLocal(-shadowDir);
                 V3 sdsn = sp.frame.toLocal(shadowDir);
@@ -1217,7 +1244,7 @@ static SS<N> tracePath(const Scene &s, PCG32 &rng, uint32_t px, uint32_t py, uin
                 float MISWeight = 1.0f;
                 if (!DT::isDelta(ls.posType) && !std::isinf(lightPDF))
                     MISWeight = (lightPDF * lightPDF) / (lightPDF * lightPDF + bsdfPDF * bsdfPDF);
-                float G = 1.0f * absDot(sdsn, gnLocal) * cosLight * rec;
+                float G = fracVis * absDot(sdsn, gnLocal) * cosLight * rec;
                 float scalarCoeff = G * MISWeight / lightPDF;
                 contribution += alpha * Le * fs * scalarCoeff;
             }

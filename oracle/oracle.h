@@ -49,6 +49,7 @@ typedef struct OracleGeomInst {
     uint32_t numIndices;
     int32_t material;
     int32_t isPoints;
+    float alpha;               /* constant alpha of the group's alpha node; < 0: no alpha node */
 } OracleGeomInst;
 
 typedef struct OracleInstance {

@@ -703,12 +703,12 @@ def build_scene(ctx: Context, desc) -> BuiltScene:
             call("vlrTriangleMeshSurfaceNodeSetVertices", n, v.ctypes.data, len(v))
             for gi, (idx, mi) in enumerate(mesh.groups):
                 ix = np.ascontiguousarray(idx, np.uint32)
-                plugs = [NULL_PLUG, NULL_PLUG]
-                for k, p in enumerate(getattr(mesh, "group_nodes", {}).get(gi, (None, None))):
+                plugs = [NULL_PLUG, NULL_PLUG, NULL_PLUG]
+                for k, p in enumerate(getattr(mesh, "group_nodes", {}).get(gi, ())):
                     if p is not None:
                         plugs[k] = node_plug(ctx, p[0], p[1], p[2], node_cache, bs.keep)
                 call("vlrTriangleMeshSurfaceNodeAddMaterialGroup", n, ix.ctypes.data, len(ix), bs.materials[mi],
-                     plugs[0], plugs[1], NULL_PLUG)
+                     plugs[0], plugs[1], plugs[2])
         bs.meshes.append(n)
     ident = ctx.transform(np.eye(4, dtype=np.float32).reshape(16, order="F"))
     scene = H()

@@ -788,10 +788,7 @@ VLR_API VLRResult vlrTriangleMeshSurfaceNodeAddMaterialGroup(VLRTriangleMeshSurf
     g.nodeTangent = toPlug(nodeTangent);
     if (g.nodeTangent.valid() && !plugConvertible(SocketType::Vector3D, g.nodeTangent.plugType)) g.nodeTangent = PlugRef();
     g.nodeAlpha = toPlug(nodeAlpha);
-    if (g.nodeAlpha.valid()) {
-        g_lastError = "alpha shader nodes are not on the render path";
-        return VLRResult_IncompatibleNodeType;
-    }
+    if (g.nodeAlpha.valid() && !plugConvertible(SocketType::Float, g.nodeAlpha.plugType)) g.nodeAlpha = PlugRef();
     n->groups.push_back(std::move(g));
     n->version = nextGeometryVersion();
     n->ctx->touch();

@@ -726,7 +726,11 @@ void Context::renderBatch(hipStream_t userStream, const std::vector<BatchJobDesc
     L.jobOffsets = static_cast<const uint32_t *>(b.jobOffsets.ptr);
     L.numJobs = static_cast<uint32_t>(jobs.size());
     L.nodePrograms = false;
-    for (const auto &kv : b.scenes) L.nodePrograms = L.nodePrograms || kv.second->dev.hasNodePrograms;
+    for (const auto &kv : b.scenes) {
+        L.nodePrograms = L.nodePrograms || kv.second->dev.hasNodePrograms;
+        // the batch kernels draw no random numbers during traversal
+        if (kv.second->dev.hasAlpha) throw ApiError{4, "alpha nodes are not supported in render batches"};
+    }
 
     // schedule candidates (wavefront depth; the tail grid fixed), each timed on one step when
     // autotuning is on and there are enough steps; otherwise the context's schedule

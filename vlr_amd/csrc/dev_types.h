@@ -197,7 +197,9 @@ struct alignas(16) DevGeomInst {
     float importance;
     uint32_t isPoints;
     // node program of the material group's normal (output vec 0) and tangent (vec 1) nodes
-    uint32_t progBase, progCount, pad2[2];
+    uint32_t progBase, progCount;
+    // node program of its alpha node (output vec 0, a float); alphaCount 0: opaque
+    uint32_t alphaBase, alphaCount;
 };
 static_assert(sizeof(DevGeomInst) == 48, "geomInst layout");
 
@@ -235,6 +237,7 @@ struct DevScene {
     uint32_t numInstances;
     uint32_t hasTlas;
     uint32_t hasNodePrograms;    // some material or material group has a node program
+    uint32_t hasAlpha;           // some triangle group has an alpha node (stochastic alpha test)
     // light selection over [environment, instances...]
     uint32_t lightInstBase;
     uint32_t numLightInsts;

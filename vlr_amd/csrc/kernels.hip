@@ -134,7 +134,13 @@ struct TravState {
     uint32_t blasNode, blasTri;   // bases of the instance being traversed
     bool inBlas, found, anyHit;
     TraceHit hit;
+    Pcg32 *rng;                   // the path's generator for the stochastic alpha test
+    float vis;                    // fractional visibility of an any-hit ray through alpha surfaces
 };
+
+// alpha of a triangle group's alpha node at a candidate hit (renderer_common.h:532-548)
+__device__ __forceinline__ float alphaAt(const DevScene &sc, int32_t inst, uint32_t geomInst, uint32_t prim, float b1,
+                                         float b2);
 
 __device__ __forceinline__ void travBegin(TravState &st, const DevScene &sc, const V3 &org, const V3 &dir, float tmin,
                                           float tmax, bool anyHit) {
@@ -152,11 +158,16 @@ __device__ __forceinline__ void travBegin(TravState &st, const DevScene &sc, con
     st.hit.inst = -1;
     st.hit.geomInst = st.hit.prim = -1;
     st.hit.t = st.hit.b1 = st.hit.b2 = 0.0f;
+    st.rng = nullptr;
+    st.vis = 1.0f;
 }
 
 // Advances the traversal by at most maxSteps steps; true once it has finished (then st.found
 // and st.hit hold the closest hit, or any hit for an any-hit ray).
-template <bool STATS>
+// With ALPHA, triangles of groups with an alpha node pass the stochastic alpha test
+// (path_tracing.cu:14-25) for closest-hit rays and scale the fractional visibility
+// (light_transport_common.h:205-218) for any-hit rays.
+template <bool STATS, bool ALPHA = false>
 __device__ __forceinline__ bool travStep(TravState &st, const DevScene &sc, int32_t *lds, int lane, int32_t *ovf,
                                          float *ovfT, int maxSteps, TravCount *tc) {
     float *ldsT = reinterpret_cast<float *>(lds + kLdsStack * kTraceBlock);
@@ -250,6 +261,18 @@ __device__ __forceinline__ bool travStep(TravState &st, const DevScene &sc, int3
                 const float4 q0 = tp[0], q1 = tp[1], q2 = tp[2];
                 float t, b1, b2;
                 if (intersectTri(st.ray, xyz(q0), xyz(q1), xyz(q2), st.tmin, st.tmax, &t, &b1, &b2)) {
+                    if constexpr (ALPHA) {
+                        const uint32_t gi = __float_as_uint(q0.w);
+                        if (sc.hasAlpha && sc.geomInsts[gi].alphaCount) {
+                            const float alpha = alphaAt(sc, st.curInst, gi, __float_as_uint(q1.w), b1, b2);
+                            if (st.anyHit) {
+                                st.vis *= 1 - alpha;
+                                if (st.vis != 0.0f) continue;
+                            } else if (st.rng && st.rng->uniform() >= alpha) {
+                                continue;
+                            }
+                        }
+                    }
                     st.tmax = t;
                     st.found = true;
                     st.hit.inst = st.curInst;
@@ -294,15 +317,20 @@ __device__ __forceinline__ bool travStep(TravState &st, const DevScene &sc, int3
     return st.cur == kTravDone;
 }
 
-template <bool ANY, bool STATS = false>
+// rng: the path's generator (closest-hit rays through alpha surfaces); vis: the fractional
+// visibility of an any-hit ray (0 when occluded)
+template <bool ANY, bool STATS = false, bool ALPHA = false>
 __device__ __forceinline__ bool traceRay(const DevScene &sc, const V3 &org, const V3 &dir, float tmin, float tmax,
-                                         TraceHit *hit, int32_t *lds, int lane, TravCount *tc = nullptr) {
+                                         TraceHit *hit, int32_t *lds, int lane, TravCount *tc = nullptr,
+                                         Pcg32 *rng = nullptr, float *vis = nullptr) {
     TravState st;
     travBegin(st, sc, org, dir, tmin, tmax, ANY);
+    st.rng = rng;
     int32_t ovf[kScratchStack];
     float ovfT[kScratchStack];
-    travStep<STATS>(st, sc, lds, lane, ovf, ovfT, INT32_MAX, tc);
+    travStep<STATS, ALPHA>(st, sc, lds, lane, ovf, ovfT, INT32_MAX, tc);
     if (st.found) *hit = st.hit;
+    if (vis) *vis = st.found ? 0.0f : st.vis;
     return st.found;
 }
 
@@ -444,6 +472,18 @@ __device__ __forceinline__ void decodeHit(const DevScene &sc, int32_t instIdx, i
     sp->u = b0;
     sp->v = b1;
     if constexpr (NODES) applyGroupNodes(sc, G, sp);   // normal and tangent nodes (renderer_common.h:620-625)
+}
+
+__device__ __forceinline__ float alphaAt(const DevScene &sc, int32_t inst, uint32_t geomInst, uint32_t prim, float b1,
+                                         float b2) {
+    SurfPt sp;
+    float hypAreaPDF;
+    decodeHit<false>(sc, inst, static_cast<int32_t>(geomInst), static_cast<int32_t>(prim), b1, b2, &sp, &hypAreaPDF);
+    const DevGeomInst &G = sc.geomInsts[geomInst];
+    NodeOutputs<3> o;
+    Wls<3> none;
+    runNodeProgram<3, false>(sc, G.alphaBase, G.alphaCount, sp, none, VC_F1, VC_F1, &o);
+    return (o.has & 8u) ? o.vec[0].x : 1.0f;   // calcNode's default 1
 }
 
 // environment surface point from a direction (path_tracing.cu:282-310)
@@ -762,7 +802,7 @@ __device__ __forceinline__ void addTravStats(uint32_t *counters, uint32_t slot, 
 }
 
 // ---- intersect: closest hit ----
-template <bool STATS, bool MULTI = false>
+template <bool STATS, bool MULTI = false, bool ALPHA = false>
 __global__ void __launch_bounds__(kTraceBlock) k_intersect(KArgs a) {
     __shared__ int32_t lds[kTraceLds];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -778,7 +818,13 @@ __global__ void __launch_bounds__(kTraceBlock) k_intersect(KArgs a) {
         if constexpr (MULTI)
             found = traceRay<false, STATS>(a.jobs[jobOf(a, __float_as_uint(o.w))].scene, xyz(o), xyz(d), 0.0f, d.w,
                                            &h, lds, threadIdx.x, &tc);
-        else
+        else if constexpr (ALPHA) {
+            // the alpha test draws from the path's generator before its shading does
+            const uint32_t path = __float_as_uint(o.w);
+            Pcg32 rng{a.fb.ps.rng[path]};
+            found = traceRay<false, STATS, true>(a.sc, xyz(o), xyz(d), 0.0f, d.w, &h, lds, threadIdx.x, &tc, &rng);
+            a.fb.ps.rng[path] = rng.s;
+        } else
             found = traceRay<false, STATS>(a.sc, xyz(o), xyz(d), 0.0f, d.w, &h, lds, threadIdx.x, &tc);
         if (found) {
             a.fb.hits.a[i] = make_int4(h.inst, h.geomInst, h.prim, __float_as_int(h.t));
@@ -1148,7 +1194,7 @@ __global__ void __launch_bounds__(256) k_shade(KArgs a) {
 }
 
 // ---- shadow: any-hit, adds the NEE contribution when unoccluded ----
-template <bool STATS, bool MULTI = false>
+template <bool STATS, bool MULTI = false, bool ALPHA = false>
 __global__ void __launch_bounds__(kTraceBlock) k_shadow(KArgs a) {
     __shared__ int32_t lds[kTraceLds];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1158,14 +1204,17 @@ __global__ void __launch_bounds__(kTraceBlock) k_shadow(KArgs a) {
         float4 o = a.fb.shadow.org[i], d = a.fb.shadow.dir[i];
         TraceHit h;
         bool occluded;
+        float vis = 1.0f;
         if constexpr (MULTI)
             occluded = traceRay<true, STATS>(a.jobs[jobOf(a, __float_as_uint(o.w))].scene, xyz(o), xyz(d), 0.0f,
                                              d.w, &h, lds, threadIdx.x, &tc);
         else
-            occluded = traceRay<true, STATS>(a.sc, xyz(o), xyz(d), 0.0f, d.w, &h, lds, threadIdx.x, &tc);
+            occluded = traceRay<true, STATS, ALPHA>(a.sc, xyz(o), xyz(d), 0.0f, d.w, &h, lds, threadIdx.x, &tc,
+                                                    nullptr, &vis);
         if (!occluded) {
             uint32_t path = __float_as_uint(o.w);
             float4 v = a.fb.shadow.value[i];
+            if constexpr (ALPHA) v = make_float4(v.x * vis, v.y * vis, v.z * vis, v.w * vis);
             float4 c = a.fb.ps.contrib[path];
             a.fb.ps.contrib[path] = make_float4(c.x + v.x, c.y + v.y, c.z + v.z, c.w + v.w);
         }
@@ -1232,7 +1281,9 @@ k_tail(KArgs a) {
         TraceHit h;
         h.inst = -1;
         h.b1 = h.b2 = 0.0f;
-        bool found = traceRay<false, STATS>(psc, org, dir, 0.0f, tmax, &h, lds, threadIdx.x, &tc);
+        constexpr bool kAlpha = NODES && !MULTI;
+        if constexpr (kAlpha) P.rngUsed = P.rngUsed || sc.hasAlpha;
+        bool found = traceRay<false, STATS, kAlpha>(psc, org, dir, 0.0f, tmax, &h, lds, threadIdx.x, &tc, &P.rng);
         int4 ha = found ? make_int4(h.inst, h.geomInst, h.prim, 0) : make_int4(-1, -1, -1, 0);
         bool emitShadow, emitNext;
         float4 shOrg, shDir, shVal, nextOrg, nextDir;
@@ -1241,9 +1292,10 @@ k_tail(KArgs a) {
         if (emitShadow) {
             atomicAdd(&depthCounts[2 * depth + 1], 1u);
             TraceHit sh;
-            bool occluded = traceRay<true, STATS>(psc, xyz(shOrg), xyz(shDir), 0.0f, shDir.w, &sh, lds,
-                                                  threadIdx.x, &tcShadow);
-            if (!occluded) P.contrib += loadSpec<N>(shVal);
+            float vis = 1.0f;
+            bool occluded = traceRay<true, STATS, kAlpha>(psc, xyz(shOrg), xyz(shDir), 0.0f, shDir.w, &sh, lds,
+                                                          threadIdx.x, &tcShadow, nullptr, &vis);
+            if (!occluded) P.contrib += loadSpec<N>(shVal) * vis;
         }
         if (!emitNext) {
             storePath<N>(a.fb, path, depth + 1, P, false);
@@ -1700,9 +1752,11 @@ struct LensIdf {
     }
 };
 
-// testVisibility (light_transport_common.h:147-177) returning the connection direction
+// testVisibility (light_transport_common.h:147-177) returning the connection direction and,
+// with ALPHA, the fractional visibility through alpha surfaces
+template <bool ALPHA = false>
 __device__ __forceinline__ bool connectVisible(const DevScene &sc, const SurfPt &shading, const SurfPt &light, V3 *dir,
-                                               float *dist2, int32_t *lds, int lane) {
+                                               float *dist2, int32_t *lds, int lane, float *vis = nullptr) {
     if (light.atInfinity) {
         *dist2 = 1.0f;
         *dir = normalize(light.position);
@@ -1718,7 +1772,7 @@ __device__ __forceinline__ bool connectVisible(const DevScene &sc, const SurfPt 
     float tmax = 3.402823466e+38f;
     if (!light.atInfinity) tmax = (lo - so).length() * 0.9999f;
     TraceHit h;
-    return !traceRay<true>(sc, so, *dir, 0.0f, tmax, &h, lds, lane);
+    return !traceRay<true, false, ALPHA>(sc, so, *dir, 0.0f, tmax, &h, lds, lane, nullptr, nullptr, vis);
 }
 
 template <int N>
@@ -1811,7 +1865,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_bptLight(BptArgs a) {
                 TraceHit h;
                 h.inst = -1;
                 h.b1 = h.b2 = 0.0f;
-                bool found = traceRay<false>(sc, rayOrg, rayDir, 0.0f, 3.402823466e+38f, &h, lds, threadIdx.x);
+                bool found = traceRay<false, false, NODES>(sc, rayOrg, rayDir, 0.0f, 3.402823466e+38f, &h, lds,
+                                                           threadIdx.x, nullptr, &rng);
                 V3 nextOrg, nextDir;
                 float woDirPDF = 0, woCosTerm = 0, woRevAreaPDF = 0;
                 uint32_t woSampledType = 0;
@@ -2004,7 +2059,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_bptEye(BptArgs a) {
             V3 conRayDir;
             float sqDist;
             ++shadowRays;
-            if (connectVisible(sc, spE0, spL, &conRayDir, &sqDist, lds, threadIdx.x)) {
+            float vis = 1.0f;
+            if (connectVisible<NODES>(sc, spE0, spL, &conRayDir, &sqDist, lds, threadIdx.x, &vis)) {
                 float recSq = 1.0f / sqDist;
                 const DevMaterial &mL = sc.materials[matL];
                 Bsdf<N> bL = vtx.pathLength == 0 ? edfAsBsdf<N>(sc, mL.type) : setupBsdf<N, NODES>(sc, mL, wlsC, &spL);
@@ -2015,7 +2071,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_bptEye(BptArgs a) {
                     float cosL = spL.cosTerm(conRayDir);
                     float cosE = spE0.cosTerm(conRayDir);
                     float G = cosL * cosE * recSq;
-                    float scalarConTerm = G * 1.0f / vertexProb;
+                    float scalarConTerm = G * vis / vertexProb;
                     if (vtx.wlSelected) scalarConTerm *= N;
                     Spec<N> backwardFsL;
                     Spec<N> forwardFsL = bL.evaluate(vtx.dirIn, gnLocalL, conLocalL, wlsC.sel, true, &backwardFsL);
@@ -2091,7 +2147,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_bptEye(BptArgs a) {
             TraceHit h;
             h.inst = -1;
             h.b1 = h.b2 = 0.0f;
-            bool found = traceRay<false>(sc, rayOrg, rayDir, 0.0f, 3.402823466e+38f, &h, lds, threadIdx.x);
+            bool found = traceRay<false, false, NODES>(sc, rayOrg, rayDir, 0.0f, 3.402823466e+38f, &h, lds,
+                                                       threadIdx.x, nullptr, &rng);
             const Wls<N> wls = commonWls<N>(bp, single);
             V3 nextOrg, nextDir;
             float woDirPDF = 0, woCosTerm = 0, woRevAreaPDF = 0;
@@ -2185,7 +2242,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_bptEye(BptArgs a) {
                 V3 conRayDir;
                 float sqDist;
                 ++shadowRays;
-                if (connectVisible(sc, spE, spL, &conRayDir, &sqDist, lds, threadIdx.x)) {
+                float vis = 1.0f;
+                if (connectVisible<NODES>(sc, spE, spL, &conRayDir, &sqDist, lds, threadIdx.x, &vis)) {
                     float recSq = 1.0f / sqDist;
                     const DevMaterial &mL = sc.materials[matL];
                     Bsdf<N> bL = vtx.pathLength == 0 ? edfAsBsdf<N>(sc, mL.type) : setupBsdf<N, NODES>(sc, mL, wls, &spL);
@@ -2196,7 +2254,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_bptEye(BptArgs a) {
                         float cosL = spL.cosTerm(conRayDir);
                         float cosE = spE.cosTerm(conRayDir);
                         float Gc = cosL * cosE * recSq;
-                        float scalarConTerm = Gc * 1.0f / vertexProb;
+                        float scalarConTerm = Gc * vis / vertexProb;
                         if (vtx.wlSelected || single) scalarConTerm *= N;
                         Spec<N> backwardFsL;
                         Spec<N> forwardFsL = bL.evaluate(vtx.dirIn, gnLocalL, conLocalL, wls.sel, true, &backwardFsL);
@@ -2379,7 +2437,8 @@ __global__ void __launch_bounds__(kTraceBlock) k_lightTrace(BptArgs a) {
                 float dist2;
                 if (cam.sensitivity != 0.0f) {
                     ++shadowRays;
-                    if (connectVisible(sc, spE, ls.sp, &dir, &dist2, lds, threadIdx.x)) {
+                    float vis = 1.0f;
+                    if (connectVisible<NODES>(sc, spE, ls.sp, &dir, &dist2, lds, threadIdx.x, &vis)) {
                         const V3 dirLens = spE.frame.toLocal(dir);
                         const V3 dirSn = ls.sp.frame.toLocal(-dir);
                         float u0 = 0.0f, u1 = 0.0f;
@@ -2387,7 +2446,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_lightTrace(BptArgs a) {
                         Spec<N> M;
                         float fe1;
                         evalEDF<N, NODES>(sc, lm, ls.sp, wls, dirSn, &M, &fe1);
-                        const float G = ls.sp.cosTerm(-dir) * spE.cosTerm(dir) / dist2;
+                        const float G = vis * ls.sp.cosTerm(-dir) * spE.cosTerm(dir) / dist2;
                         const float scalarCoeff = G / lensPDF;
                         const Spec<N> contribution = alpha * Spec<N>(cam.sensitivity) * Spec<N>(fe1) * scalarCoeff;
                         if (onScreen) splatContribution<N>(a, wls, contribution, u0, u1);
@@ -2432,7 +2491,9 @@ __global__ void __launch_bounds__(kTraceBlock) k_lightTrace(BptArgs a) {
                 TraceHit h;
                 h.inst = -1;
                 h.b1 = h.b2 = 0.0f;
-                if (!traceRay<false>(sc, rayOrg, rayDir, 0.0f, 3.402823466e+38f, &h, lds, threadIdx.x)) break;
+                if (!traceRay<false, false, NODES>(sc, rayOrg, rayDir, 0.0f, 3.402823466e+38f, &h, lds, threadIdx.x,
+                                                   nullptr, &rng))
+                    break;
                 SurfPt sp;
                 float hypAreaPDF;
                 decodeHit<NODES>(sc, h.inst, h.geomInst, h.prim, h.b1, h.b2, &sp, &hypAreaPDF);
@@ -2450,13 +2511,14 @@ __global__ void __launch_bounds__(kTraceBlock) k_lightTrace(BptArgs a) {
                     float dist2;
                     if (cam.sensitivity != 0.0f) {
                         ++shadowRays;
-                        if (connectVisible(sc, spE, sp, &dir, &dist2, lds, threadIdx.x)) {
+                        float vis = 1.0f;
+                        if (connectVisible<NODES>(sc, spE, sp, &dir, &dist2, lds, threadIdx.x, &vis)) {
                             const V3 dirLens = spE.frame.toLocal(dir);
                             const V3 dirSn = sp.frame.toLocal(-dir);
                             float u0 = 0.0f, u1 = 0.0f;
                             const bool onScreen = idf.project(dirLens, &u0, &u1);
                             const Spec<N> fs = bsdf.evaluate(dirInLocal, gnLocal, dirSn, wls.sel, true);
-                            const float Gc = absDot(dirSn, gnLocal) * spE.cosTerm(dir) / dist2;
+                            const float Gc = vis * absDot(dirSn, gnLocal) * spE.cosTerm(dir) / dist2;
                             const float scalarCoeff = Gc / lensPDF;
                             const Spec<N> contribution = alpha * Spec<N>(cam.sensitivity) * fs * scalarCoeff;
                             if (onScreen) splatContribution<N>(a, wls, contribution, u0, u1);
@@ -2659,7 +2721,9 @@ static hipError_t launchFrameT(const DevScene &sc, const FrameParams &fp, const 
     for (uint32_t d = 0; d < wavefrontDepths; ++d) {
         a.depth = d;
         t0(STAGE_INTERSECT);
-        if (fp.traversalStats)
+        if (sc.hasAlpha)
+            hipLaunchKernelGGL((k_intersect<false, false, true>), dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
+        else if (fp.traversalStats)
             hipLaunchKernelGGL(k_intersect<true>, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
         else
             hipLaunchKernelGGL(k_intersect<false>, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
@@ -2672,7 +2736,9 @@ static hipError_t launchFrameT(const DevScene &sc, const FrameParams &fp, const 
             hipLaunchKernelGGL((k_shade<N, false, false>), dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
         t1(STAGE_SHADE);
         t0(STAGE_SHADOW);
-        if (fp.traversalStats)
+        if (sc.hasAlpha)
+            hipLaunchKernelGGL((k_shadow<false, false, true>), dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
+        else if (fp.traversalStats)
             hipLaunchKernelGGL(k_shadow<true>, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
         else
             hipLaunchKernelGGL(k_shadow<false>, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, a);

@@ -161,6 +161,7 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
 
     // shader nodes reachable from materials and material groups
     ShaderTable shaders;
+    bool hasAlpha = false;
     shaders.spectral = spectral;
 
     // materials
@@ -228,6 +229,12 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
             if (mg.nodeNormal.valid()) roots.push_back({mg.nodeNormal, 8u});
             if (mg.nodeTangent.valid()) roots.push_back({mg.nodeTangent, 16u});
             std::tie(G.progBase, G.progCount) = shaders.program(roots);
+        }
+        if (!G.isPoints && mg.nodeAlpha.valid()) {
+            // the alpha node (renderer_common.h:532-548), its own program: traversal runs it at
+            // candidate hits
+            std::tie(G.alphaBase, G.alphaCount) = shaders.program({{mg.nodeAlpha, 8u}});
+            hasAlpha = true;
         }
         G.numPrims = G.isPoints ? static_cast<uint32_t>(mg.indices.size()) : static_cast<uint32_t>(mg.indices.size() / 3);
         for (uint32_t k = 0; k < (G.isPoints ? G.numPrims : 3 * G.numPrims); ++k) {
@@ -624,6 +631,7 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
     D.nodes = static_cast<const DevShaderNode *>(g->shaderNodes.ptr);
     D.nodeSteps = static_cast<const DevNodeStep *>(g->nodeSteps.ptr);
     D.hasNodePrograms = hasPrograms ? 1 : 0;
+    D.hasAlpha = hasAlpha ? 1 : 0;
     D.textures = static_cast<const DevTexture *>(g->textures.ptr);
     D.texels = static_cast<const float4 *>(g->texels.ptr);
     D.nodePool = static_cast<const float *>(g->nodePool.ptr);

@@ -82,9 +82,12 @@ class MeshDesc:
     vertices: np.ndarray            # float32 [N, 11]: position(3) normal(3) tc0Direction(3) texCoord(2)
     groups: List[Tuple[np.ndarray, int]]  # (uint32 indices, material index)
     points: bool = False
-    # shader nodes of a triangle group: group index -> (normal plug, tangent plug), each a
-    # (NodeDesc, plug type, option) or None
+    # shader nodes of a triangle group: group index -> (normal plug, tangent plug[, alpha plug]),
+    # each a (NodeDesc, plug type, option) or None
     group_nodes: dict = field(default_factory=dict)
+    # constant alpha of a triangle group for the oracle (group index -> alpha); the C API takes
+    # the same value as an alpha node in group_nodes
+    group_alpha: dict = field(default_factory=dict)
 
 
 @dataclass
