@@ -64,11 +64,98 @@ def xy_to_uv(x, y):
             f32(f32(-7.530081094743006) * x + f32(16.192422314095225) * y + f32(1.1125529268825947)))
 
 
+# ---- block compression (Direct3D block-compression specification) ----
+def _rgb565(c):
+    return [f32((c >> 11) & 31) / f32(31), f32((c >> 5) & 63) / f32(63), f32(c & 31) / f32(31)]
+
+
+def _color_block(b, three_color_allowed):
+    c0, c1 = int(b[0]) | int(b[1]) << 8, int(b[2]) | int(b[3]) << 8
+    idx = int.from_bytes(bytes(b[4:8]), "little")
+    p0, p1 = _rgb565(c0), _rgb565(c1)
+    if c0 > c1 or not three_color_allowed:
+        pal = [p0 + [f32(1)], p1 + [f32(1)],
+               [f32((f32(2) * p0[k] + p1[k]) / f32(3)) for k in range(3)] + [f32(1)],
+               [f32((p0[k] + f32(2) * p1[k]) / f32(3)) for k in range(3)] + [f32(1)]]
+    else:
+        pal = [p0 + [f32(1)], p1 + [f32(1)], [f32((p0[k] + p1[k]) / f32(2)) for k in range(3)] + [f32(1)],
+               [f32(0)] * 4]
+    return [pal[(idx >> (2 * t)) & 3] for t in range(16)]
+
+
+def _channel_block(b, signed):
+    if signed:
+        s0, s1 = int(np.int8(b[0])), int(np.int8(b[1]))
+        eight = s0 > s1
+        e0, e1 = f32(max(s0, -127)) / f32(127), f32(max(s1, -127)) / f32(127)
+    else:
+        eight = int(b[0]) > int(b[1])
+        e0, e1 = f32(int(b[0])) / f32(255), f32(int(b[1])) / f32(255)
+    p = [e0, e1]
+    if eight:
+        p += [f32((f32(8 - i) * e0 + f32(i - 1) * e1) / f32(7)) for i in range(2, 8)]
+    else:
+        p += [f32((f32(6 - i) * e0 + f32(i - 1) * e1) / f32(5)) for i in range(2, 6)]
+        p += [f32(-1) if signed else f32(0), f32(1)]
+    idx = int.from_bytes(bytes(b[2:8]), "little")
+    return [p[(idx >> (3 * t)) & 7] for t in range(16)]
+
+
+def decode_bc(fmt, data, width, height):
+    """Mip level 0 of a BC1-BC5 image as (height, width, 4) float32 texels."""
+    data = np.ascontiguousarray(data, np.uint8).reshape(-1)
+    bb = 8 if fmt in ("BC1", "BC4", "BC4_Signed") else 16
+    bw, bh = (width + 3) // 4, (height + 3) // 4
+    out = np.zeros((height, width, 4), f32)
+    out[..., 3] = 1
+    for by in range(bh):
+        for bx in range(bw):
+            b = data[(by * bw + bx) * bb:(by * bw + bx + 1) * bb]
+            if fmt == "BC1":
+                t = _color_block(b, True)
+            elif fmt == "BC2":
+                t = _color_block(b[8:], False)
+                a = int.from_bytes(bytes(b[:8]), "little")
+                t = [c[:3] + [f32((a >> (4 * k)) & 15) / f32(15)] for k, c in enumerate(t)]
+            elif fmt == "BC3":
+                t = _color_block(b[8:], False)
+                a = _channel_block(b[:8], False)
+                t = [c[:3] + [a[k]] for k, c in enumerate(t)]
+            elif fmt in ("BC4", "BC4_Signed"):
+                r = _channel_block(b, fmt == "BC4_Signed")
+                t = [[r[k], f32(0), f32(0), f32(1)] for k in range(16)]
+            else:
+                r = _channel_block(b[:8], fmt == "BC5_Signed")
+                g = _channel_block(b[8:], fmt == "BC5_Signed")
+                t = [[r[k], g[k], f32(0), f32(1)] for k in range(16)]
+            for k in range(16):
+                x, y = bx * 4 + (k & 3), by * 4 + (k >> 2)
+                if x < width and y < height:
+                    out[y, x] = t[k]
+    return out
+
+
 # ---- images ----
 class Texture:
     def __init__(self, img, spectral):
         self.width, self.height = img.width, img.height
         fmt = img.format
+        if fmt.startswith("BC"):
+            # decoded on the host; the texture unit's sRGB read of BC1-BC4 degammas the colour
+            # channels per texel (image.cpp:336-346)
+            cs = COLOR_SPACES[img.color_space]
+            self.uvs, self.kind, self.gray = False, "rgba", False
+            self.spectrum_type = SPEC_TYPES[img.spectrum_type]
+            t = decode_bc(fmt, img.data, img.width, img.height)
+            degam = cs == 0 and fmt in ("BC1", "BC2", "BC3", "BC4")
+            if degam:
+                for y in range(img.height):
+                    for x in range(img.width):
+                        for k in ((0,) if fmt == "BC4" else (0, 1, 2)):
+                            t[y, x, k] = degamma(t[y, x, k])
+            self.color_space = 1 if degam else cs
+            self.texels = t
+            return
         st = SPEC_TYPES[img.spectrum_type]
         cs = COLOR_SPACES[img.color_space]
         srgb = cs == 0

@@ -44,6 +44,10 @@ def random_points(n, seed):
 
 def image(fmt, w, h, spectrum_type, color_space, seed):
     rng = np.random.default_rng(seed)
+    if fmt.startswith("BC"):
+        bb = 8 if fmt in ("BC1", "BC4", "BC4_Signed") else 16
+        data = rng.integers(0, 256, ((w + 3) // 4) * ((h + 3) // 4) * bb, dtype=np.uint8)
+        return ImageDesc(data, w, h, fmt, spectrum_type, color_space)
     if fmt in ("RGBA8x4", "RGB8x3", "Gray8", "GrayA8x2"):
         ch = {"RGBA8x4": 4, "RGB8x3": 3, "Gray8": 1, "GrayA8x2": 2}[fmt]
         data = rng.integers(0, 256, (h, w, ch), dtype=np.uint8)
@@ -65,6 +69,12 @@ IMAGES = {
     "gray32f_srgb_refl": image("Gray32F", 3, 3, "Reflectance", REC709_SRGB, 7),
     "graya8_srgb_refl": image("GrayA8x2", 5, 2, "Reflectance", REC709_SRGB, 8),
     "rg32f_srgb": image("RG32Fx2", 2, 5, "NA", REC709_SRGB, 9),
+    # block-compressed: partial blocks, hardware-style sRGB decode of BC1-BC4, signed BC5
+    "bc1_srgb_refl": image("BC1", 6, 5, "Reflectance", REC709_SRGB, 10),
+    "bc2_linear_light": image("BC2", 4, 4, "Light Source", REC709, 11),
+    "bc3_srgb_refl": image("BC3", 8, 4, "Reflectance", REC709_SRGB, 12),
+    "bc4_srgb_na": image("BC4", 8, 8, "NA", REC709_SRGB, 13),
+    "bc5s_na": image("BC5_Signed", 4, 8, "NA", REC709, 14),
 }
 WRAPS = ["Repeat", "Clamp to Edge", "Mirror", "Clamp to Border"]
 

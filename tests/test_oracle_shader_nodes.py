@@ -228,3 +228,78 @@ def test_float3_to_spectrum_rgb_maps_minus_one_to_one_onto_zero_to_one():
     f3 = NodeDesc("Float3", floats={"0": [-1.0], "1": [0.0], "2": [3.0]})
     node = NodeDesc("Float3ToSpectrum", plugs={"value": (f3, P["float3"], 0)})
     np.testing.assert_array_equal(ev.spectrum((node, P["Spectrum"], 0), None, point()), [0.0, 0.5, 1.0])
+
+
+# ---- block-compressed images: invariants of the Direct3D block formats ----
+def bc1_block(c0, c1, indices):
+    idx = 0
+    for t, i in enumerate(indices):
+        idx |= i << (2 * t)
+    return np.frombuffer(np.array([c0, c1], "<u2").tobytes() + int(idx).to_bytes(4, "little"), np.uint8)
+
+
+def bc4_block(e0, e1, indices, signed=False):
+    idx = 0
+    for t, i in enumerate(indices):
+        idx |= i << (3 * t)
+    head = np.array([e0, e1], np.int8 if signed else np.uint8).tobytes()
+    return np.frombuffer(head + int(idx).to_bytes(6, "little"), np.uint8)
+
+
+def test_bc1_endpoints_and_thirds():
+    white, black = 0xFFFF, 0x0000
+    t = osn.decode_bc("BC1", bc1_block(white, black, [0, 1, 2, 3] * 4), 4, 4)
+    np.testing.assert_array_equal(t[0, 0], [1, 1, 1, 1])      # index 0: first endpoint exactly
+    np.testing.assert_array_equal(t[0, 1], [0, 0, 0, 1])      # index 1: second endpoint
+    np.testing.assert_allclose(t[0, 2, :3], 2 / 3, rtol=1e-6)  # thirds between them
+    np.testing.assert_allclose(t[0, 3, :3], 1 / 3, rtol=1e-6)
+    # c0 <= c1: three colours and transparent black
+    t = osn.decode_bc("BC1", bc1_block(black, white, [2, 3] * 8), 4, 4)
+    np.testing.assert_allclose(t[0, 0], [0.5, 0.5, 0.5, 1], rtol=1e-6)
+    np.testing.assert_array_equal(t[0, 1], [0, 0, 0, 0])
+    # a solid block is constant whatever its indices; 565 channels expand to 1 at full scale
+    red = 0xF800
+    rng = np.random.default_rng(0)
+    t = osn.decode_bc("BC1", bc1_block(red, red, rng.integers(0, 3, 16)), 4, 4)
+    assert np.all(t == np.array([1, 0, 0, 1], np.float32))
+
+
+def test_bc_block_layout_and_partial_blocks():
+    # 6 x 5 texels: 2 x 2 blocks, texels past the edge dropped; block k is solid grey level k
+    greys = [0x0000, 0x4208, 0x8410, 0xFFFF]
+    data = np.concatenate([bc1_block(g, g, [0] * 16) for g in greys])
+    t = osn.decode_bc("BC1", data, 6, 5)
+    assert t.shape == (5, 6, 4)
+    for k, (bx, by) in enumerate([(0, 0), (1, 0), (0, 1), (1, 1)]):
+        blk = t[by * 4:by * 4 + 4, bx * 4:bx * 4 + 4]
+        assert np.all(blk == blk[0, 0])
+    assert t[0, 0, 0] == 0 and t[4, 5, 0] == 1
+
+
+def test_bc4_and_bc5_palettes():
+    t = osn.decode_bc("BC4", bc4_block(255, 0, list(range(8)) * 2), 4, 4)
+    r = t[..., 0].reshape(-1)[:8]
+    np.testing.assert_allclose(r, [1, 0, 6 / 7, 5 / 7, 4 / 7, 3 / 7, 2 / 7, 1 / 7], rtol=1e-6, atol=1e-7)
+    # e0 <= e1: fifths, then the constants 0 and 1
+    t = osn.decode_bc("BC4", bc4_block(0, 255, list(range(8)) * 2), 4, 4)
+    np.testing.assert_allclose(t[..., 0].reshape(-1)[:8], [0, 1, 1 / 5, 2 / 5, 3 / 5, 4 / 5, 0, 1], atol=1e-7)
+    # signed: -128 reads as -127 (-1), constants -1 and 1
+    t = osn.decode_bc("BC4_Signed", bc4_block(-128, 127, [0, 1, 6, 7] * 4, signed=True), 4, 4)
+    np.testing.assert_allclose(t[..., 0].reshape(-1)[:4], [-1, 1, -1, 1], atol=1e-7)
+    # BC5 = two BC4 blocks in red and green
+    d = np.concatenate([bc4_block(255, 0, [0] * 16), bc4_block(255, 0, [1] * 16)])
+    t = osn.decode_bc("BC5", d, 4, 4)
+    assert np.all(t[..., 0] == 1) and np.all(t[..., 1] == 0) and np.all(t[..., 3] == 1)
+
+
+def test_bc2_and_bc3_alpha():
+    a = 0
+    for k in range(16):
+        a |= k << (4 * k)
+    d = np.concatenate([np.frombuffer(int(a).to_bytes(8, "little"), np.uint8), bc1_block(0xFFFF, 0xFFFF, [0] * 16)])
+    t = osn.decode_bc("BC2", d, 4, 4)
+    np.testing.assert_allclose(t[..., 3].reshape(-1), np.arange(16) / 15, rtol=1e-6)
+    # BC3: the colour block is always four-colour, alpha from a BC4 block
+    d = np.concatenate([bc4_block(0, 255, [7] * 16), bc1_block(0x0000, 0xFFFF, [3] * 16)])
+    t = osn.decode_bc("BC3", d, 4, 4)
+    np.testing.assert_allclose(t[0, 0], [2 / 3, 2 / 3, 2 / 3, 1], rtol=1e-6)

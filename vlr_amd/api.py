@@ -615,6 +615,14 @@ def build_image(ctx: Context, img, keep: list):
     data = np.ascontiguousarray(img.data, np.uint8)
     keep.append(data)
     h = H()
+    if img.format.startswith("BC"):
+        # one mip level of blocks (vlrBlockCompressedImage2DCreate, vlr.h)
+        ptrs = (C.c_void_p * 1)(data.ctypes.data)
+        sizes = (C.c_size_t * 1)(data.nbytes)
+        keep += [ptrs, sizes]
+        call("vlrBlockCompressedImage2DCreate", ctx.h, C.cast(ptrs, C.c_void_p), C.cast(sizes, C.c_void_p), 1,
+             img.width, img.height, _b(img.format), _b(img.spectrum_type), _b(img.color_space), C.byref(h))
+        return h
     call("vlrLinearImage2DCreate", ctx.h, data.ctypes.data, img.width, img.height, _b(img.format),
          _b(img.spectrum_type), _b(img.color_space), C.byref(h))
     return h

@@ -549,6 +549,10 @@ VLR_API VLRResult vlrBlockCompressedImage2DCreate(VLRContext context, uint8_t **
     CHECK(context, IS(context, Context));
     if (!data || !sizes || !dataFormat || !spectrumType || !colorSpace || !image || mipCount == 0)
         return VLRResult_InvalidArgument;
+    if (!bcFormatDecodable(dataFormat)) {
+        g_lastError = "block-compressed format not decoded by this build (BC1-BC5 are)";
+        return VLRResult_InvalidArgument;
+    }
     auto *img = new BlockCompressedImage2D(CTX(context));
     img->width = width;
     img->height = height;

@@ -179,6 +179,11 @@ public:
     bool toRGBA32F(std::vector<float> *out) const override;
 };
 
+// BC1-BC5 (signed variants included) decoded into RGBA texels (bc_decode.cpp)
+bool bcFormatDecodable(const std::string &format);
+bool decodeBlockCompressed(const std::string &format, const uint8_t *data, size_t size, uint32_t width, uint32_t height,
+                           std::vector<float4> *texels);
+
 class BlockCompressedImage2D : public Image2D {
 public:
     std::vector<std::vector<uint8_t>> mips;

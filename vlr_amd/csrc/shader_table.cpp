@@ -216,6 +216,36 @@ uint32_t ShaderTable::addTexture(const Image2D *img) {
     DevTexture T;
     std::memset(&T, 0, sizeof(T));
     T.offset = static_cast<uint32_t>(texels.size());
+    if (img && img->type == ObjType::BlockCompressedImage2D) {
+        // mip level 0 decoded; the texture unit's sRGB read of BC1-BC4 (image.cpp:336-346) is a
+        // per-texel degamma of the colour channels, done here
+        const auto *bi = static_cast<const BlockCompressedImage2D *>(img);
+        std::vector<float4> dec;
+        if (!bi->mips.empty() &&
+            decodeBlockCompressed(bi->formatStr, bi->mips[0].data(), bi->mips[0].size(), bi->width, bi->height, &dec)) {
+            const std::string &f = bi->formatStr;
+            const bool degamma = bi->colorSpace == ColorSpace::Rec709_D65_sRGBGamma &&
+                                 (f == "BC1" || f == "BC2" || f == "BC3" || f == "BC4");
+            T.width = bi->width;
+            T.height = bi->height;
+            T.format = kTexRGBA;
+            T.gray = 0;
+            T.spectrumType = static_cast<uint32_t>(bi->spectrumType);
+            T.colorSpace = static_cast<uint32_t>(degamma ? ColorSpace::Rec709_D65 : bi->colorSpace);
+            for (float4 &t : dec) {
+                if (degamma) {
+                    t.x = colorDegamma(t.x);
+                    if (f != "BC4") {
+                        t.y = colorDegamma(t.y);
+                        t.z = colorDegamma(t.z);
+                    }
+                }
+                texels.push_back(t);
+            }
+            textures.push_back(T);
+            return idx;
+        }
+    }
     const LinearImage2D *li =
         img && img->type == ObjType::LinearImage2D ? static_cast<const LinearImage2D *>(img) : nullptr;
     const uint32_t st = li ? li->stride() : 0;
