@@ -174,3 +174,27 @@ def test_autotuned_schedule_matches_fixed_schedule(torch_gpu):
     imgs, _, rng = render_gpu(desc, W, H, frames, 8, False, wavefront_depth=64)
     assert np.array_equal(rng, tuned_rng)
     assert np.array_equal(imgs[-1].view(np.uint32), tuned_img.view(np.uint32))
+
+
+@pytest.mark.parametrize("scene,spectral", [("cornell", False), ("grid", True)])
+def test_checkpoint_and_resume_in_a_new_context(torch_gpu, scene, spectral):
+    """RNG states, the Kahan accumulation and the frame count written to a fresh context resume
+    the render: frames 3-4 there give the image of four frames rendered in one go, bit for bit."""
+    desc = scenes.cornell_box() if scene == "cornell" else scenes.icosphere_grid(grid=4, env_size=(64, 32))
+    W, H, depth = 48, 32, 8
+    full, _, _ = render_gpu(desc, W, H, 4, depth, spectral)
+    with api.Context(spectral=spectral, max_path_length=depth) as ctx:
+        bs = api.build_scene(ctx, desc)
+        ctx.bind_output(W, H)
+        for f in range(2):
+            ctx.render(bs.camera, f == 0, 4)
+        rng, acc = ctx.read_rng(), ctx.read_accumulation()
+    with api.Context(spectral=spectral, max_path_length=depth) as ctx:
+        bs = api.build_scene(ctx, desc)
+        ctx.bind_output(W, H)
+        ctx.write_rng(rng)
+        ctx.write_accumulation(acc, 2)
+        for _ in range(2):
+            ctx.render(bs.camera, False, 4)
+        resumed = ctx.read_output()
+    assert np.array_equal(resumed, full[-1])

@@ -1413,8 +1413,14 @@ VLR_API VLRResult vlrContextWriteAccumulation(VLRContext context, const float *d
     CHECK(context, IS(context, Context));
     Context *c = CTX(context);
     if (!data || !c->width) return VLRResult_InvalidArgument;
+    // a resumed render continues in the requested transport (RGB or spectral channels)
+    if (c->spectralActive != c->spectralRequested) {
+        c->spectralActive = c->spectralRequested;
+        c->resizeAccum();
+    }
     c->writeAccumulation(data);
     c->numAccumFrames = numAccumFrames;
+    c->resumePending = true;
     return VLRResult_NoError;
     VLRA_CATCH
 }

@@ -146,6 +146,11 @@ void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCo
         numAccumFrames = 0;
         // the camera is read on the first frame only (context.cpp:1416-1421)
         frameCamera = camera->toDevice();
+        resumePending = false;
+    } else if (resumePending) {
+        // the first frame after a written accumulation (a resumed render) reads the camera too
+        frameCamera = camera->toDevice();
+        resumePending = false;
     }
     ensureScene(*scene);
     gpu.dev.camera = frameCamera;

@@ -469,6 +469,7 @@ public:
     uint32_t numAccumFrames = 0;
     FrameParams lastFrame = {};
     DevCamera frameCamera = {};
+    bool resumePending = false;   // accumulation written: the next frame reads the camera
     // device buffers
     DeviceBuffer rng, alpha, contrib, meta, rayOrg[2], rayDir[2], hitA, hitB, shOrg, shDir, shVal, counters, accum,
         output, scratch;
