@@ -2721,6 +2721,7 @@ static hipError_t launchFrameT(const DevScene &sc, const FrameParams &fp, const 
     for (uint32_t d = 0; d < wavefrontDepths; ++d) {
         a.depth = d;
         t0(STAGE_INTERSECT);
+        // scenes with alpha nodes run the alpha variants, which keep no traversal statistics
         if (sc.hasAlpha)
             hipLaunchKernelGGL((k_intersect<false, false, true>), dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
         else if (fp.traversalStats)
