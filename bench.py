@@ -287,7 +287,7 @@ def bpt_experiment(width, height, spp, ref_spp, light_paths, seed=20261202):
         built = api.build_scene(ctx, desc)
         ctx.bind_output(width, height)
         ctx.set_renderer(renderer)
-        if renderer == api.RENDERER_BPT:
+        if renderer in (api.RENDERER_BPT, api.RENDERER_LIGHT_TRACING):
             ctx.set_num_light_paths(light_paths)
         if rng_seed is not None:
             ctx.write_rng(np.random.default_rng(rng_seed).integers(1, 2 ** 63, size=n, dtype=np.uint64))
@@ -341,17 +341,20 @@ def bpt_experiment(width, height, spp, ref_spp, light_paths, seed=20261202):
 
     bpt, bst = curve(api.RENDERER_BPT)
     pt, pst = curve(api.RENDERER_PATH_TRACING)
+    lt, lst = curve(api.RENDERER_LIGHT_TRACING)
     equal_time = []
     for p in bpt:
         r = interp_log(pt, "seconds", "rmse", p["seconds"])
+        rl = interp_log(lt, "seconds", "rmse", p["seconds"])
         if r is not None:
-            equal_time.append({"seconds": p["seconds"], "bpt": p["rmse"], "pt": r})
+            equal_time.append({"seconds": p["seconds"], "bpt": p["rmse"], "pt": r, "lt": rl})
     return {
         "workload": f"Cornell box + 64 point lights + 8 quad lights, RGB, max depth 8, {width}x{height}, "
                     f"{light_paths} light paths per frame",
         "reference": {"renderer": "path tracing", "spp": ref_spp, "noise_rmse_removed": float(np.sqrt(ref_var)),
                       "masked_pixels": int((~mask).sum())},
-        "bpt": bpt, "bpt_stats": bst, "pt": pt, "pt_stats": pst, "equal_time": equal_time,
+        "bpt": bpt, "bpt_stats": bst, "pt": pt, "pt_stats": pst, "lt": lt, "lt_stats": lst,
+        "equal_time": equal_time,
     }
 
 
