@@ -209,3 +209,37 @@ def test_knobs(ctx):
     assert code_of("vlrContextSetNumLightPaths", ctx.h, 0) == 3
     assert code_of("vlrContextSetNumLightPaths", ctx.h, (1 << 20) + 1) == 3
     assert ctx.num_light_paths() == 921600
+
+
+def test_block_compressed_formats(ctx):
+    """BC1-BC5 (and signed BC4/BC5) are accepted; BC6H and BC7 are refused with InvalidArgument."""
+    data = np.zeros(16, np.uint8)
+    ptrs = (C.c_void_p * 1)(data.ctypes.data)
+    sizes = (C.c_size_t * 1)(data.nbytes)
+    for fmt, ok in (("BC1", True), ("BC3", True), ("BC4_Signed", True), ("BC5", True), ("BC6H", False), ("BC7", False)):
+        h = api.H()
+        rc = code_of("vlrBlockCompressedImage2DCreate", ctx.h, C.cast(ptrs, C.c_void_p), C.cast(sizes, C.c_void_p), 1,
+                     4, 4, fmt.encode(), b"Reflectance", b"Rec709(D65)", C.byref(h))
+        assert (rc == 0) == ok, (fmt, rc)
+
+
+def test_material_group_plugs(ctx):
+    """Normal, tangent and alpha plugs are taken; an alpha plug that does not convert to a float
+    is ignored, as scene.cpp:225-246 ignores it (the call still succeeds)."""
+    mesh = api.H()
+    api.call("vlrTriangleMeshSurfaceNodeCreate", ctx.h, b"quad", C.byref(mesh))
+    verts = np.zeros((4, 11), np.float32)
+    api.call("vlrTriangleMeshSurfaceNodeSetVertices", mesh, verts.ctypes.data, 4)
+    idx = np.array([0, 1, 2, 0, 2, 3], np.uint32)
+    mat = ctx.material("Matte")
+    f2 = ctx.shader_node("Float2")
+    tex = ctx.shader_node("Image2DTexture")
+
+    def plug(node, ptype, option=0):
+        p = api.VLRShaderNodePlug()
+        api.call("vlrShaderNodeGetPlug", node, ptype, option, C.byref(p))
+        return p
+    assert code_of("vlrTriangleMeshSurfaceNodeAddMaterialGroup", mesh, idx.ctypes.data, 6, mat,
+                   plug(tex, api.PLUG_NORMAL3D), api.NULL_PLUG, plug(f2, api.PLUG_FLOAT1)) == 0
+    assert code_of("vlrTriangleMeshSurfaceNodeAddMaterialGroup", mesh, idx.ctypes.data, 6, mat,
+                   api.NULL_PLUG, api.NULL_PLUG, plug(tex, api.PLUG_SPECTRUM)) == 0
