@@ -109,9 +109,25 @@ def read_traffic(kernel):
         return float(json.load(f)["hbm_bytes_per_launch"])
 
 
+def cpu_threads():
+    """Host threads for the CPU baseline: every core in this process's affinity mask, capped by
+    OMP_NUM_THREADS when the machine sets it (a GPU box's share of the host's cores)."""
+    n = len(os.sched_getaffinity(0))
+    cap = os.environ.get("OMP_NUM_THREADS")
+    if cap and cap.isdigit() and int(cap) > 0:
+        n = min(n, int(cap))
+    return n
+
+
 def cpu_baseline(desc, width, height, depth, budget_s):
+    """The oracle path tracer (a C++/OpenMP restatement of the reference's path tracer), built
+    for this machine's CPU (-march=native) and run on cpu_threads() threads over whole frames
+    of the same scene and per-pixel seeds for about budget_s seconds."""
+    import tempfile
     from oracle import binding as ob
-    threads = min(16, len(os.sched_getaffinity(0)))
+    native_dir = tempfile.mkdtemp(prefix="oracle_native_")
+    ob.lib(ob.build_native(native_dir))
+    threads = cpu_threads()
     scene = ob.Scene(desc)
     rng = ob.seed_states(width, height)
     rays = 0
@@ -125,8 +141,12 @@ def cpu_baseline(desc, width, height, depth, budget_s):
         frames += 1
     return {"value": rays / elapsed, "unit": "rays/s", "cores": threads, "kind": "port",
             "sample": f"{frames} full {width}x{height} frames (1 spp each, depth {depth}) of the same scene "
-                      f"and per-pixel seeds, {elapsed:.1f} s on {threads} threads of {cpu_model()}",
-            "samples_per_s": frames * width * height / elapsed}
+                      f"and per-pixel seeds, {elapsed:.1f} s on {threads} threads of {cpu_model()} "
+                      f"(oracle built with -O3 -march=native)",
+            "samples_per_s": frames * width * height / elapsed,
+            "host": {"nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0)),
+                     "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "model": cpu_model(),
+                     "threads_used": threads}}
 
 
 def rmse_curve(ctx, camera, frames, checkpoints, ref, ref_var, first_frame=True):
@@ -485,15 +505,13 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=64)
-    # the first 20 frames after a scene change also tune the frame schedule
+    # warm-up frames; the warm-up also runs on until the frame-schedule autotuner is done
     ap.add_argument("--warmup", type=int, default=24)
     ap.add_argument("--width", type=int, default=1920)
     ap.add_argument("--height", type=int, default=1080)
     ap.add_argument("--depth", type=int, default=16)
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--stat-frames", type=int, default=2)
-    ap.add_argument("--timing-frames", type=int, default=8,
-                    help="frames rendered with per-launch HIP events, after the timed region")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--wavefront-depth", type=int, default=None,
@@ -545,14 +563,24 @@ def main():
     if args.wavefront_depth is not None:
         ctx.set_wavefront_depth(args.wavefront_depth)
 
+    # warm-up: at least --warmup frames, and on until the frame-schedule autotuner has timed
+    # every candidate (its frames synchronise), so the timed region runs the tuned schedule only
     t_setup = time.perf_counter()
-    for w in range(args.warmup):
-        ctx.render(built.camera, first_frame=(w == 0), limit=args.warmup)
+    warm = 0
+    while warm < args.warmup or (args.wavefront_depth is None and not ctx.schedule()[2]):
+        ctx.render(built.camera, first_frame=(warm == 0), limit=1 << 30)
+        warm += 1
+        if warm > 500:
+            raise RuntimeError("the schedule autotuner did not finish within 500 frames")
     hip_sync()
     setup_s = time.perf_counter() - t_setup
     accel = ctx.accel_stats()
+    sched_tuned_before = ctx.schedule()[2]
 
+    # timed region: HIP events around every launch, on the stream each kernel runs on, read
+    # back after the region (they do not synchronise the frames)
     ctx.reset_stats()
+    ctx.set_kernel_timing(True)
     hip_sync()
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -560,20 +588,12 @@ def main():
     hip_sync()
     wall = time.perf_counter() - t0
     st = ctx.stats()
+    kt = st
+    ctx.set_kernel_timing(False)
     qc, qs = ctx.queue_counts(args.depth)
     sched_wd, sched_tb, sched_tuned = ctx.schedule()
     sched_sort = ctx.shade_sorting()
     img = ctx.read_output()
-
-    # per-kernel device time: more frames with HIP events around every launch (kernel timing
-    # synchronises each frame, so it stays out of the timed region above)
-    ctx.reset_stats()
-    ctx.set_kernel_timing(True)
-    for k in range(args.timing_frames):
-        ctx.render(built.camera, first_frame=(k == 0), limit=args.timing_frames)
-    hip_sync()
-    kt = ctx.stats()
-    ctx.set_kernel_timing(False)
 
     rays = st["closestRays"] + st["shadowRays"]
     value = rays / wall
@@ -629,17 +649,20 @@ def main():
         "samples_per_s": samples_per_s,
         "rays": {"closest": st["closestRays"], "shadow": st["shadowRays"], "paths": st["paths"],
                  "dropped_non_finite": st["droppedNonFinite"]},
-        "stage_ms_per_frame": {k: kt["ms" + k.capitalize()] / args.timing_frames
+        "stage_ms_per_frame": {k: kt["ms" + k.capitalize()] / args.steps
                                for k in ("generate", "intersect", "shade", "shadow", "tail", "accumulate", "convert")},
-        "timing_frames": args.timing_frames,
+        "kernel_timing": "HIP events around every launch inside the timed region",
         "schedule": {"wavefront_depth": sched_wd, "tail_blocks": sched_tb, "material_sort": sched_sort,
-                     "autotuned": sched_tuned},
+                     "autotuned": bool(sched_tuned and sched_tuned_before), "warmup_frames_run": warm},
         "rays_by_stage": {"per_depth_closest": wf_c, "per_depth_shadow": wf_s, "tail_closest": tail_c,
                           "tail_shadow": tail_s},
         "traversal_per_ray": {"closest": per_closest, "shadow": per_shadow},
         "last_frame_rays_per_depth": {"closest": qc.tolist(), "shadow": qs.tolist()},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": None if traffic is None else
+                     f"profiles/{dominant}_icosphere_grid_1080p_traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE "
+                     f"of this workload, scaled by a copy of known size; not measured in this run)",
                      "algorithmic_bytes_per_launch": per_launch_bytes, "mean_launch_ms": mean_launch_s * 1e3},
         "image_mean_rgb": [float(x) for x in img[..., :3].reshape(-1, 3).mean(axis=0)],
         "setup_s": setup_s,

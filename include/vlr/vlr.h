@@ -65,8 +65,10 @@ VLR_API VLRResult vlrContextGetCUcontext(VLRContext context, hipCtx_t* hipContex
 VLR_API VLRResult vlrContextBindOutputBuffer(VLRContext context, uint32_t width, uint32_t height, uint32_t glTexID);
 /* vlr.h:82-84 */
 VLR_API VLRResult vlrContextGetOutputBufferSize(VLRContext context, uint32_t* width, uint32_t* height);
-/* vlr.h:85-87: the output is a linear float4 device buffer; *array receives its device
- * pointer cast to hipArray_t (no texture array is involved) */
+/* vlr.h:85-87: a width x height hipArray of float4 (RGBA32F) that holds the converted output;
+ * made on the first call, then written on the render stream after every rendered frame; owned
+ * by the context (freed when the output is rebound at another size or the context is
+ * destroyed) */
 VLR_API VLRResult vlrContextGetOutputBuffer(VLRContext context, hipArray_t* array);
 /* vlr.h:88-90: synchronous copy of width*height RGBA floats */
 VLR_API VLRResult vlrContextReadOutputBuffer(VLRContext context, float* data);

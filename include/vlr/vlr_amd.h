@@ -56,9 +56,22 @@ typedef struct VLRRenderStats {
 /* Counting the rays reads back one small counter buffer per render call. */
 VLR_API VLRResult vlrContextGetRenderStats(VLRContextConst context, VLRRenderStats* stats);
 VLR_API VLRResult vlrContextResetRenderStats(VLRContext context);
-/* Brackets every kernel launch of vlrContextRender with HIP events on the render stream
- * (synchronising at the end of each render call).  Off by default. */
+/* Brackets every kernel launch of vlrContextRender with HIP events on the stream it runs on.
+ * The events are read back when the statistics are read (or once 16384 are pending), so timed
+ * frames are not synchronised.  Off by default. */
 VLR_API VLRResult vlrContextSetKernelTiming(VLRContext context, bool enable);
+
+/* ---- denoiser guides (path_tracing.cu:95-108, post_process.cu:70-90) ----
+ * With the guides on, VLRRenderer_PathTracing sums each sample's first-hit albedo (the BSDF's
+ * base colour times the D65 white point over the wavelength pdf; zero for a miss) and shading
+ * normal per pixel, restarting with the accumulation.  Reading converts them as the
+ * reference's copyBuffers does: linear Rec.709 albedo averaged over the samples, and the
+ * normal sum rotated into the camera's frame with x negated and normalised (zero stays zero);
+ * width*height RGBA float4 each, alpha 1 (either pointer may be NULL).  The guides are what
+ * the reference hands its denoiser; the denoiser itself (OptiX) is not part of this library,
+ * and vlrContextRender's `denoise` flag is accepted and ignored.  Off by default. */
+VLR_API VLRResult vlrContextSetDenoiserAuxBuffers(VLRContext context, bool enable);
+VLR_API VLRResult vlrContextReadDenoiserAuxBuffers(VLRContextConst context, float* albedoRGBA, float* normalRGBA);
 /* Renders with the traversal kernels' counting variant (slower; off by default). */
 VLR_API VLRResult vlrContextSetTraversalStatistics(VLRContext context, bool enable);
 /* Frame schedule: depths below `depth` run as separate intersect / shade / shadow kernels over
@@ -127,10 +140,21 @@ VLR_API VLRResult vlrContextWriteRNGStates(VLRContext context, const uint64_t* s
 VLR_API VLRResult vlrContextReadLightRNGStates(VLRContextConst context, uint64_t* states);
 VLR_API VLRResult vlrContextWriteLightRNGStates(VLRContext context, const uint64_t* states);
 /* Kahan accumulation pairs: (sum, compensation) per channel, channels = 3 (RGB) or 16
- * (spectral strata); floats = width*height*channels*2. */
+ * (spectral strata) as the context's requested mode (vlrContextSetSpectral) gives them;
+ * numFloats must equal width*height*channels*2, or the call returns InvalidArgument.  Reading
+ * fails with InvalidArgument while the context holds the accumulation of the other mode (the
+ * mode was changed and no first frame has been rendered since), so a checkpoint of one mode
+ * is never read as the other. */
 VLR_API VLRResult vlrContextGetAccumulationChannels(VLRContextConst context, uint32_t* channels);
-VLR_API VLRResult vlrContextReadAccumulation(VLRContextConst context, float* data);
-VLR_API VLRResult vlrContextWriteAccumulation(VLRContext context, const float* data, uint32_t numAccumFrames);
+VLR_API VLRResult vlrContextReadAccumulation(VLRContextConst context, float* data, size_t numFloats);
+VLR_API VLRResult vlrContextWriteAccumulation(VLRContext context, const float* data, size_t numFloats,
+                                              uint32_t numAccumFrames);
+/* The per-frame wavelength generator of VLRRenderer_BPT (a default-seeded std::mt19937 with
+ * two draws per frame, context.cpp:1464-1467 of the reference): the number of frames that
+ * drew from it since the context was created.  Setting it restarts the generator and replays
+ * that many frames' draws, so a resumed BPT render draws the wavelengths of the original. */
+VLR_API VLRResult vlrContextGetBptWavelengthFrames(VLRContextConst context, uint64_t* frames);
+VLR_API VLRResult vlrContextSetBptWavelengthFrames(VLRContext context, uint64_t frames);
 
 /* ---- radiance cache ----
  * A 5 x 64 ReLU MLP (BF16 weights and activations, FP32 accumulation) trained online on the
@@ -204,7 +228,17 @@ typedef struct VLRAccelStats {
     uint32_t blasReused;         /* 1: no geometry changed since the previous setup, BLASes kept */
     uint32_t numNodeProgramSteps; /* shader-node program steps of the set-up scene; 0: the
                                      kernels without node evaluation run */
+    uint32_t tlasDepth, blasDepth; /* levels of 4-wide nodes of the TLAS and of the deepest BLAS;
+                                      a scene whose 3 (tlasDepth + blasDepth) + 1 exceeds the
+                                      traversal stack (80 entries) is refused at set-up */
+    uint32_t tlasRefit;          /* 1: the last set-up refitted the TLAS instead of rebuilding it */
 } VLRAccelStats;
+/* With refit on (the default), a set-up after which only instance transforms changed (the same
+ * instances over the same geometry) keeps the TLAS's tree and recomputes its boxes; it rebuilds
+ * instead once the refitted tree's cost (the summed areas of its nodes over the root's) exceeds
+ * twice the cost it had when built.  Off: the TLAS is rebuilt on every set-up, as the reference
+ * rebuilds its IAS (scene.cpp:1325-1327). */
+VLR_API VLRResult vlrContextSetTlasRefit(VLRContext context, bool enable);
 VLR_API VLRResult vlrContextGetAccelStats(VLRContextConst context, VLRAccelStats* stats);
 /* Copies one part of the acceleration structure of `scene`, as set up for rendering, to host
  * memory: which = 0 TLAS nodes, 1 BLAS nodes (64 B each), 2 BLAS triangle records (48 B: v0,

@@ -121,12 +121,20 @@ def build(force: bool = False) -> Path:
     return LIB_PATH
 
 
-def lib():
+def build_native(out_dir) -> Path:
+    """Compile a copy tuned for this machine's CPU (-march=native) into out_dir."""
+    subprocess.run(["make", "-s", "-C", str(HERE), "-B", "ARCH=-march=native", f"OUT={out_dir}"], check=True)
+    return Path(out_dir) / "liboracle.so"
+
+
+def lib(path=None):
+    """The oracle library: the default build, or `path` (e.g. a build_native copy) when given
+    on the first call."""
     global _lib
     if _lib is None:
-        if not LIB_PATH.exists():
+        if path is None and not LIB_PATH.exists():
             build()
-        L = C.CDLL(str(LIB_PATH))
+        L = C.CDLL(str(path or LIB_PATH))
         L.oracle_load_tables.argtypes = [C.c_char_p]
         L.oracle_scene_create.restype = C.c_void_p
         L.oracle_scene_create.argtypes = [C.POINTER(OracleScene), C.c_int]
@@ -136,6 +144,9 @@ def lib():
         L.oracle_trace_any.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_int, C.c_int]
         L.oracle_render.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int,
                                     C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(OracleRenderStats)]
+        L.oracle_render_guides.argtypes = [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int,
+                                           C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                           C.POINTER(OracleRenderStats)]
         L.oracle_bsdf_queries.argtypes = [C.POINTER(OracleMaterial), C.c_int, C.c_void_p, C.c_uint32, C.c_void_p]
         L.oracle_pcg32_stream.restype = C.c_uint32
         L.oracle_pcg32_stream.argtypes = [C.c_uint64, C.c_uint32, C.c_void_p, C.c_void_p]
@@ -304,6 +315,23 @@ class Scene:
         stats = {"closest_rays": st.closestRays, "shadow_rays": st.shadowRays, "paths": st.paths,
                  "dropped_non_finite": st.droppedNonFinite}
         return out, stats, ls, acc
+
+    def render_guides(self, width, height, frames, max_path_length=25, threads=0, rng_state=None):
+        """Path-traced render with the denoiser guides; returns (image, albedo, normal, rng states,
+        stats), the guides as W x H RGBA after copyBuffers' conversion."""
+        out = np.zeros((height, width, 4), np.float32)
+        alb = np.zeros((height, width, 4), np.float32)
+        nrm = np.zeros((height, width, 4), np.float32)
+        st = OracleRenderStats()
+        rs = None if rng_state is None else np.ascontiguousarray(rng_state, np.uint64)
+        rc = lib().oracle_render_guides(self.handle, width, height, frames, max_path_length, threads, out.ctypes.data,
+                                        None if rs is None else rs.ctypes.data, alb.ctypes.data, nrm.ctypes.data,
+                                        C.byref(st))
+        if rc != 0:
+            raise RuntimeError("oracle_render_guides failed")
+        stats = {"closest_rays": st.closestRays, "shadow_rays": st.shadowRays, "paths": st.paths,
+                 "dropped_non_finite": st.droppedNonFinite}
+        return out, alb, nrm, rs, stats
 
     def sample_lights(self, u: np.ndarray) -> np.ndarray:
         """u: (n, 3) uniform numbers -> LIGHT_SAMPLE_DTYPE records."""

@@ -1081,9 +1081,19 @@ static bool testVisibility(const Scene &s, const SurfPt &shading, const SurfPt &
 
 // ------------------------------------------------------------------------------------------
 // The path tracer (path_tracing.cu:30-337), one pixel sample.
+// What the path's first hit gives the denoiser's guide buffers (path_tracing.cu:95-108,
+// 154-157, 272-275): the BSDF's base colour times the D65 white point over the wavelength
+// pdf, and the shading normal; zero for a miss.
+template <int N>
+struct FirstHit {
+    SS<N> albedo;
+    V3 normal;
+};
+
 template <int N>
 static SS<N> tracePath(const Scene &s, PCG32 &rng, uint32_t px, uint32_t py, uint32_t W, uint32_t H,
-                       uint32_t maxPathLength, WLS<N> *wlsOut, OracleRenderStats &st) {
+                       uint32_t maxPathLength, WLS<N> *wlsOut, OracleRenderStats &st,
+                       FirstHit<N> *firstHit = nullptr) {
     float pX = px + rng.uniform();
     float pY = py + rng.uniform();
     float selectWLPDF;
@@ -1134,6 +1144,10 @@ static SS<N> tracePath(const Scene &s, PCG32 &rng, uint32_t px, uint32_t py, uin
         tlsAlphaRng = &rng;
         bool found = traceRay(s, rayOrg, rayDir, 0.0f, FLT_MAX, &hit, false, true);
         tlsAlphaRng = nullptr;
+        if (firstHit && pathLength == 1 && !found) {
+            firstHit->albedo = SS<N>(0.0f);
+            firstHit->normal = V3(0.0f, 0.0f, 0.0f);
+        }
         if (!found) {
             // miss program (path_tracing.cu:266-337)
             if (!s.hasEnv || s.envImportance == 0)
@@ -1188,6 +1202,11 @@ static SS<N> tracePath(const Scene &s, PCG32 &rng, uint32_t px, uint32_t py, uin
         BSDF<N> bsdf = setupBSDF<N>(mat, wls);
         EDF<N> edf = setupEDF<N>(s, mat, sp, wls);
         V3 dirOutLocal = sp.frame.toLocal(-rayDir);
+        if (firstHit && pathLength == 1) {
+            const Triplet white = Triplet::create(SpectrumType::LightSource, ColorSpace::Rec709_D65, 1, 1, 1);
+            firstHit->albedo = bsdf.getBaseColor() * white.eval<N>(wls) / selectWLPDF;
+            firstHit->normal = sp.frame.z;
+        }
 
         SS<N> spEmittance = edf.evaluateEmittance();
         if (spEmittance.hasNonZero()) {
@@ -2046,7 +2065,11 @@ static void renderBPTImpl(const Scene &s, uint32_t W, uint32_t H, uint32_t numFr
 template <int N, int C>
 static void renderImpl(const Scene &s, uint32_t W, uint32_t H, uint32_t numFrames, uint32_t maxPathLength,
                        int numThreads, float *outRGBA, uint64_t *rngState, float *accumOut,
-                       const int32_t *range, OracleRenderStats *stats) {
+                       const int32_t *range, OracleRenderStats *stats, float *albedoRGBA = nullptr,
+                       float *normalRGBA = nullptr) {
+    const bool guides = albedoRGBA || normalRGBA;
+    std::vector<SS<C>> accumAlbedo(guides ? size_t(W) * H : 0, SS<C>(0.0f));
+    std::vector<V3> accumNormal(guides ? size_t(W) * H : 0, V3(0.0f, 0.0f, 0.0f));
     std::vector<uint64_t> rngs(size_t(W) * H);
     if (rngState) {
         std::memcpy(rngs.data(), rngState, rngs.size() * 8);
@@ -2075,9 +2098,16 @@ static void renderImpl(const Scene &s, uint32_t W, uint32_t H, uint32_t numFrame
                         size_t p = size_t(y) * W + x;
                         PCG32 rng{rngs[p]};
                         WLS<N> wls;
-                        SS<N> c = tracePath<N>(s, rng, x, y, W, H, maxPathLength, &wls, st);
+                        FirstHit<N> fh;
+                        SS<N> c = tracePath<N>(s, rng, x, y, W, H, maxPathLength, &wls, st, guides ? &fh : nullptr);
                         rngs[p] = rng.state;
                         ++st.paths;
+                        if (guides) {
+                            // accumulated every sample, finite or not (path_tracing.cu:95-108)
+                            if constexpr (N == 3) accumAlbedo[p] += fh.albedo;
+                            else accumAlbedo[p] += binSpectral(wls, fh.albedo);
+                            accumNormal[p] = accumNormal[p] + fh.normal;
+                        }
                         if (!c.allFinite()) {
                             ++st.droppedNonFinite;
                             continue;
@@ -2123,6 +2153,37 @@ static void renderImpl(const Scene &s, uint32_t W, uint32_t H, uint32_t numFrame
             for (int c = 0; c < C; ++c) {
                 accumOut[(p * C + c) * 2 + 0] = accum[p].result[c];
                 accumOut[(p * C + c) * 2 + 1] = accum[p].comp[c];
+            }
+        }
+        if (guides) {
+            // copyBuffers (post_process.cu:70-90)
+            float aXYZ[3];
+            if constexpr (N == 3) {
+                float a[3] = {accumAlbedo[p][0], accumAlbedo[p][1], accumAlbedo[p][2]};
+                transformTristimulus(mat_Rec709_D65_to_XYZ, a, aXYZ);
+            } else {
+                strataToXYZ(accumAlbedo[p], aXYZ);
+            }
+            aXYZ[0] *= recN; aXYZ[1] *= recN; aXYZ[2] *= recN;
+            float aRGB[3];
+            transformTristimulus(mat_XYZ_to_Rec709_D65, aXYZ, aRGB);
+            const M3 rot = s.camera.orientation.toMatrix3x3();
+            const V3 n = accumNormal[p];
+            // the inverse orientation: the transposed rotation
+            V3 nc(dot(rot.c0, n), dot(rot.c1, n), dot(rot.c2, n));
+            nc.x *= -1;
+            if (nc.x != 0 || nc.y != 0 || nc.z != 0) nc = normalize(nc);
+            if (albedoRGBA) {
+                albedoRGBA[4 * p + 0] = aRGB[0];
+                albedoRGBA[4 * p + 1] = aRGB[1];
+                albedoRGBA[4 * p + 2] = aRGB[2];
+                albedoRGBA[4 * p + 3] = 1.0f;
+            }
+            if (normalRGBA) {
+                normalRGBA[4 * p + 0] = nc.x;
+                normalRGBA[4 * p + 1] = nc.y;
+                normalRGBA[4 * p + 2] = nc.z;
+                normalRGBA[4 * p + 3] = 1.0f;
             }
         }
     }
@@ -2458,6 +2519,23 @@ int oracle_render(void *sp, uint32_t W, uint32_t H, uint32_t numFrames, uint32_t
         renderImpl<4, 16>(s, W, H, numFrames, maxPathLength, numThreads, outRGBA, rngState, accumOut, range, stats);
     else
         renderImpl<3, 3>(s, W, H, numFrames, maxPathLength, numThreads, outRGBA, rngState, accumOut, range, stats);
+    return 0;
+}
+
+// The path tracer's denoiser guides: first-hit albedo and normal accumulated over numFrames
+// frames and converted as copyBuffers does (linear Rec.709 albedo; camera-space normal with x
+// negated, normalised), W*H RGBA each; outRGBA, rngState and stats as oracle_render.
+int oracle_render_guides(void *sp, uint32_t W, uint32_t H, uint32_t numFrames, uint32_t maxPathLength,
+                         int numThreads, float *outRGBA, uint64_t *rngState, float *albedoRGBA, float *normalRGBA,
+                         OracleRenderStats *stats) {
+    const Scene &s = *static_cast<Scene *>(sp);
+    if (numFrames == 0 || W == 0 || H == 0) return 1;
+    if (s.spectral)
+        renderImpl<4, 16>(s, W, H, numFrames, maxPathLength, numThreads, outRGBA, rngState, nullptr, nullptr, stats,
+                          albedoRGBA, normalRGBA);
+    else
+        renderImpl<3, 3>(s, W, H, numFrames, maxPathLength, numThreads, outRGBA, rngState, nullptr, nullptr, stats,
+                         albedoRGBA, normalRGBA);
     return 0;
 }
 

@@ -161,6 +161,7 @@ def test_transform_change_keeps_blases(torch_gpu, builder):
         img = _render(ctx, bs, W, H, 3)
         st = ctx.accel_stats()
         assert st["blasReused"] == 1, st
+        assert st["tlasRefit"] == 1, st
         # new vertices of one mesh: BLASes are built again
         v = np.ascontiguousarray(desc.meshes[0].vertices, np.float32)
         api.call("vlrTriangleMeshSurfaceNodeSetVertices", bs.meshes[0], v.ctypes.data, len(v))
@@ -171,4 +172,63 @@ def test_transform_change_keeps_blases(torch_gpu, builder):
         bs = api.build_scene(ctx, moved)
         ctx.bind_output(W, H)
         ref = _render(ctx, bs, W, H, 3)
+    assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))
+
+
+def test_refitted_tlas_traces_exactly(torch_gpu):
+    """After a transform-only update the TLAS is refitted (same tree, new boxes): closest hits
+    on the moved scene equal the oracle's brute force bit for bit; with refit off it is rebuilt."""
+    import copy
+    from oracle import binding as ob
+    from tests.helpers import gpu_trace, random_rays_in_box
+    from tests.test_gpu_trace import _compare
+    desc = scenes.icosphere_grid(grid=5, env_size=(64, 32))
+    moved = copy.deepcopy(desc)
+    rng = np.random.default_rng(3)
+    for k in (2, 9, 17):
+        xf = np.array(moved.instances[k].transform, np.float32).reshape(4, 4, order="F").copy()
+        xf[:3, 3] += rng.uniform(-0.6, 0.6, 3).astype(np.float32)
+        moved.instances[k].transform = xf.reshape(16, order="F")
+    half = 4 * 2.2 / 2
+    rays = random_rays_in_box(20000, (-half, 0.05, -half), (half, 2.5, half), seed=8)
+    ref = ob.Scene(moved).trace_closest(rays, use_bvh=False)
+    for refit in (True, False):
+        with api.Context() as ctx:
+            ctx.set_tlas_refit(refit)
+            bs = api.build_scene(ctx, desc)
+            ctx.read_accel(bs.scene, api.ACCEL_TRIS)        # first setup
+            for k in (2, 9, 17):
+                api.call("vlrParentNodeSetTransform", bs.instances[k], ctx.transform(moved.instances[k].transform))
+            gpu = gpu_trace(torch_gpu, ctx, bs.scene, rays)
+            st = ctx.accel_stats()
+        assert st["tlasRefit"] == (1 if refit else 0), st
+        bad, ties, t_ok, b_ok = _compare(gpu, ref)
+        print(f"refit {refit}: mismatches {bad}, ties {ties}")
+        assert bad == 0 and t_ok and b_ok
+
+
+def test_shuffled_instances_rebuild_the_tlas(torch_gpu):
+    """Shuffling every instance's transform leaves a refitted tree whose node boxes overlap
+    across the whole scene (cost far above twice the built cost), so the TLAS is rebuilt; the
+    render equals a fresh setup of the shuffled scene."""
+    import copy
+    desc = scenes.icosphere_grid(grid=5, env_size=(64, 32))
+    shuffled = copy.deepcopy(desc)
+    perm = np.random.default_rng(5).permutation(len(desc.instances) - 1) + 1   # instance 0: the ground
+    for dst, src in zip(range(1, len(desc.instances)), perm):
+        shuffled.instances[dst].transform = np.array(desc.instances[src].transform, np.float32)
+    W, H = 64, 36
+    with api.Context(max_path_length=6) as ctx:
+        bs = api.build_scene(ctx, desc)
+        ctx.bind_output(W, H)
+        ctx.read_accel(bs.scene, api.ACCEL_TRIS)
+        for k in range(1, len(desc.instances)):
+            api.call("vlrParentNodeSetTransform", bs.instances[k], ctx.transform(shuffled.instances[k].transform))
+        img = _render(ctx, bs, W, H, 2)
+        st = ctx.accel_stats()
+    assert st["blasReused"] == 1 and st["tlasRefit"] == 0, st
+    with api.Context(max_path_length=6) as ctx:
+        bs = api.build_scene(ctx, shuffled)
+        ctx.bind_output(W, H)
+        ref = _render(ctx, bs, W, H, 2)
     assert np.array_equal(img.view(np.uint32), ref.view(np.uint32))

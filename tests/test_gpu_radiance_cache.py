@@ -177,3 +177,60 @@ def test_cache_lowers_error_at_equal_samples(torch_gpu):
     assert cst["closestRays"] < pst["closestRays"]          # paths end early in the cache
     assert abs(cached.mean() - ref.mean()) < 0.02 * ref.mean()
     assert rcache < 0.85 * rp
+
+
+# ---------------------------------------------------------------------------------------------
+# spectral mode: the network outputs radiance at the path's 4 hero wavelengths
+def render_spectral(desc, W, H, frames, cache=False, seed=None, warmup_frames=0, **settings):
+    with api.Context(spectral=True, max_path_length=8) as ctx:
+        bs = api.build_scene(ctx, desc)
+        ctx.bind_output(W, H)
+        if seed is not None:
+            ctx.write_rng(np.random.default_rng(seed).integers(1, 2 ** 63, size=W * H, dtype=np.uint64))
+        if cache:
+            ctx.set_radiance_cache(True)
+            ctx.set_cache_settings(**settings)
+            for f in range(warmup_frames):
+                ctx.render(bs.camera, f == 0, 1 << 30)
+            ctx.reset_stats()
+        for f in range(frames):
+            ctx.render(bs.camera, f == 0, frames)
+        img = ctx.read_output()[..., :3].astype(np.float64)
+        st = ctx.stats()
+        loss = ctx.cache_loss() if cache else None
+    return img, st, loss
+
+
+def test_spectral_cache_mean_matches_the_oracle(torch_gpu):
+    """Spectral Cornell box (C5's scene and mode) with the cache on: the image mean per channel
+    is within 3% of the oracle's plain spectral path tracer (the cache's bias at this size,
+    measured well below that), and the network trained on 4-wavelength targets (finite loss)."""
+    from oracle import binding as ob
+    desc = scenes.cornell_box()
+    W = H = 48
+    cached, cst, loss = render_spectral(desc, W, H, 512, cache=True, warmup_frames=64, trainingPaths=W * H // 4)
+    ref, _, _, _ = ob.Scene(desc, spectral=True).render(W, H, 1024, 8)
+    m_c, m_r = cached.reshape(-1, 3).mean(0), ref[..., :3].reshape(-1, 3).astype(np.float64).mean(0)
+    print("cached mean", m_c, "oracle mean", m_r, "loss", loss)
+    assert np.isfinite(loss[0]) and loss[1] > 0
+    assert np.all(np.abs(m_c - m_r) < 0.03 * m_r)
+
+
+@pytest.mark.parametrize("size", [(96, 96), (1920, 1080)])
+def test_spectral_cache_lowers_error_at_equal_samples(torch_gpu, size):
+    """At equal samples per pixel the cached spectral render is closer to a converged plain
+    spectral render than the plain one, at 96x96 and at C5's 1920x1080."""
+    desc = scenes.cornell_box()
+    W, H = size
+    big = W * H > 100000
+    ref, _, _ = render_spectral(desc, W, H, 1024 if big else 4096, seed=11)
+    spp = 64 if big else 256
+    plain, pst, _ = render_spectral(desc, W, H, spp)
+    cached, cst, loss = render_spectral(desc, W, H, spp, cache=True, warmup_frames=64 if big else 128,
+                                        trainingPaths=21845 if big else W * H // 8)
+    rmse = lambda im: math.sqrt(((im - ref) ** 2).mean())
+    rp, rcache = rmse(plain), rmse(cached)
+    print(f"{W}x{H}: RMSE at {spp} spp: plain {rp:.6f}, cached {rcache:.6f}; closest rays plain "
+          f"{pst['closestRays']}, cached {cst['closestRays']}; loss {loss}")
+    assert cst["closestRays"] < pst["closestRays"]
+    assert rcache < rp

@@ -176,25 +176,74 @@ def test_autotuned_schedule_matches_fixed_schedule(torch_gpu):
     assert np.array_equal(imgs[-1].view(np.uint32), tuned_img.view(np.uint32))
 
 
-@pytest.mark.parametrize("scene,spectral", [("cornell", False), ("grid", True)])
-def test_checkpoint_and_resume_in_a_new_context(torch_gpu, scene, spectral):
-    """RNG states, the Kahan accumulation and the frame count written to a fresh context resume
-    the render: frames 3-4 there give the image of four frames rendered in one go, bit for bit."""
+def render_frames(desc, W, H, frames, depth, spectral, renderer):
+    with api.Context(spectral=spectral, max_path_length=depth) as ctx:
+        ctx.set_renderer(renderer)
+        bs = api.build_scene(ctx, desc)
+        ctx.bind_output(W, H)
+        for f in range(frames):
+            ctx.render(bs.camera, f == 0, frames)
+        return ctx.read_output()
+
+
+@pytest.mark.parametrize("scene,spectral,renderer", [("cornell", False, api.RENDERER_PATH_TRACING),
+                                                     ("grid", True, api.RENDERER_PATH_TRACING),
+                                                     ("cornell", True, api.RENDERER_BPT),
+                                                     ("cornell", False, api.RENDERER_LIGHT_TRACING)])
+def test_checkpoint_and_resume_in_a_new_context(torch_gpu, scene, spectral, renderer):
+    """RNG states (eye paths, and light paths for BPT and light tracing), BPT's per-frame
+    wavelength generator, the Kahan accumulation and the frame count written to a fresh context
+    resume the render: frames 3-4 there give the image of four frames rendered in one go, bit
+    for bit."""
     desc = scenes.cornell_box() if scene == "cornell" else scenes.icosphere_grid(grid=4, env_size=(64, 32))
     W, H, depth = 48, 32, 8
-    full, _, _ = render_gpu(desc, W, H, 4, depth, spectral)
+    light = renderer != api.RENDERER_PATH_TRACING
+    full = render_frames(desc, W, H, 4, depth, spectral, renderer)
     with api.Context(spectral=spectral, max_path_length=depth) as ctx:
+        ctx.set_renderer(renderer)
         bs = api.build_scene(ctx, desc)
         ctx.bind_output(W, H)
         for f in range(2):
             ctx.render(bs.camera, f == 0, 4)
         rng, acc = ctx.read_rng(), ctx.read_accumulation()
+        lrng = ctx.read_light_rng() if light else None
+        wl_frames = ctx.bpt_wavelength_frames()
+    assert acc.shape == (H, W, 16 if spectral else 3, 2)
+    assert wl_frames == (2 if renderer == api.RENDERER_BPT else 0)
     with api.Context(spectral=spectral, max_path_length=depth) as ctx:
+        ctx.set_renderer(renderer)
         bs = api.build_scene(ctx, desc)
         ctx.bind_output(W, H)
         ctx.write_rng(rng)
+        if light:
+            ctx.write_light_rng(lrng)
+        ctx.set_bpt_wavelength_frames(wl_frames)
         ctx.write_accumulation(acc, 2)
         for _ in range(2):
             ctx.render(bs.camera, False, 4)
         resumed = ctx.read_output()
-    assert np.array_equal(resumed, full[-1])
+    assert np.array_equal(resumed.view(np.uint32), full.view(np.uint32))
+
+
+def test_checkpoint_of_the_other_mode_is_refused(torch_gpu):
+    """An RGB checkpoint (3 channels) does not fit a spectral context (16 channels): the C call
+    returns InvalidArgument for a size that is not width * height * channels * 2, and a context
+    that holds the other mode's accumulation refuses to read it."""
+    desc = scenes.cornell_box()
+    W, H = 16, 8
+    with api.Context(spectral=False, max_path_length=4) as ctx:
+        bs = api.build_scene(ctx, desc)
+        ctx.bind_output(W, H)
+        ctx.render(bs.camera, True, 1)
+        acc = ctx.read_accumulation()
+    assert acc.shape == (H, W, 3, 2)
+    with api.Context(spectral=True, max_path_length=4) as ctx:
+        api.build_scene(ctx, desc)
+        ctx.bind_output(W, H)
+        assert ctx.accumulation_channels() == 16
+        with pytest.raises(ValueError):
+            ctx.write_accumulation(acc, 1)
+        assert api.try_call("vlrContextWriteAccumulation", ctx.h, acc.ctypes.data, acc.size, 1) == 3
+        # no first frame has switched the fresh context to spectral yet: nothing to read
+        out = np.zeros((H, W, 16, 2), np.float32)
+        assert api.try_call("vlrContextReadAccumulation", ctx.h, out.ctypes.data, out.size) == 3

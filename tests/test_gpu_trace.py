@@ -74,3 +74,59 @@ def test_any_hit_matches_brute_force(torch_gpu):
     occ = (gpu["inst"] >= 0).astype(np.uint8)
     print(f"any-hit: occluded {occ.mean():.3f}")
     assert np.array_equal(occ, ref)
+
+
+def degenerate_scene(kind, n):
+    """One mesh whose BVH is degenerate: `identical` n copies of one triangle (no SAH split
+    separates their centroids), `geometric` triangles in the planes x = 2^-(k mod 120) (the SAH
+    peels one plane per level, past the builder's depth limit)."""
+    k = np.arange(n)
+    if kind == "identical":
+        x = np.full(n, 0.5, np.float32)
+    else:
+        x = np.ldexp(np.float32(1.0), -(k % 120).astype(np.int32)).astype(np.float32)
+    v = np.zeros((3 * n, 11), np.float32)
+    v[0::3, 0:3] = np.stack([x, np.full(n, -0.5), np.full(n, -0.5)], 1)
+    v[1::3, 0:3] = np.stack([x, np.full(n, 0.5), np.full(n, -0.5)], 1)
+    v[2::3, 0:3] = np.stack([x, np.zeros(n), np.full(n, 0.5)], 1)
+    v[:, 3] = 1.0
+    v[:, 7] = 1.0
+    m = scenes.MaterialDesc(scenes.MATTE, {"albedo": (scenes.REC709, 0.5, 0.5, 0.5)})
+    cam = scenes.CameraDesc((3.0, 0.0, 0.0), scenes.q_rotate_y(-np.pi / 2), 1.0, 0.7)
+    return scenes.SceneDesc([m], [scenes.MeshDesc("Degenerate", v, [(np.arange(3 * n, dtype=np.uint32), 0)])],
+                            [scenes.InstanceDesc("Inst", scenes.identity4(), [0])], cam, None, False, kind)
+
+
+@pytest.mark.parametrize("kind,n", [("identical", 2500), ("geometric", 5000)])
+def test_closest_hit_on_degenerate_trees(torch_gpu, kind, n):
+    """The deepest trees the builder makes within the traversal stack's bound: every hit equals
+    the brute force (exact ties aside), so no stack entry is lost."""
+    desc = degenerate_scene(kind, n)
+    oracle = ob.Scene(desc)
+    rays = rays_towards(20000, (1.5, -0.4, -0.4), (3.0, 0.4, 0.4), (-0.1, -0.45, -0.45), (1.1, 0.45, 0.45), 12)
+    rays = np.concatenate([rays, random_rays_in_box(5000, (-0.2, -0.6, -0.6), (1.2, 0.6, 0.6), seed=13)])
+    ref = oracle.trace_closest(rays, use_bvh=False)
+    with api.Context() as ctx:
+        bs = api.build_scene(ctx, desc)
+        gpu = gpu_trace(torch_gpu, ctx, bs.scene, rays)
+        st = ctx.accel_stats()
+    bad, ties, t_ok, b_ok = _compare(gpu, ref)
+    print(f"{kind}: depths TLAS {st['tlasDepth']} BLAS {st['blasDepth']}, mismatches {bad}, ties {ties}, "
+          f"hit rate {(ref['inst'] >= 0).mean():.3f}")
+    assert 3 * (st["tlasDepth"] + st["blasDepth"]) + 1 <= 80
+    if kind == "geometric":
+        assert st["blasDepth"] >= 20
+    assert bad == 0 and t_ok and b_ok
+    assert (ref["inst"] >= 0).mean() > 0.3
+
+
+def test_tree_deeper_than_the_stack_is_refused(torch_gpu):
+    """A mesh whose tree would need more stack entries than a traversal holds fails at scene
+    set-up with InvalidArgument instead of rendering with lost entries."""
+    desc = degenerate_scene("geometric", 100000)
+    with api.Context() as ctx:
+        bs = api.build_scene(ctx, desc)
+        ctx.bind_output(8, 8)
+        with pytest.raises(api.VLRError) as e:
+            ctx.render(bs.camera, True, 1)
+        assert e.value.code == 3

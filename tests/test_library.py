@@ -56,6 +56,56 @@ def test_headers_compile(tmp_path, lang, compiler):
     assert r.returncode == 0, r.stderr
 
 
+def test_cpp_wrapper_compiles(tmp_path):
+    """vlrcpp.h (the vlr::Context wrapper) with a caller in the style of the reference's
+    HostProgram/cpp_compile_test.cpp: scene, mesh with a material group, nodes, camera, render."""
+    if shutil.which("g++") is None:
+        pytest.skip("g++ not available")
+    src = tmp_path / "t.cpp"
+    src.write_text(
+        '#include "vlr/vlrcpp.h"\n'
+        "using namespace vlr;\n"
+        "int main() {\n"
+        "  ContextRef context = Context::create(nullptr, true);\n"
+        "  SceneRef scene = context->createScene();\n"
+        '  TriangleMeshSurfaceNodeRef mesh = context->createTriangleMeshSurfaceNode("mesh");\n'
+        "  Vertex v[] = {Vertex{Point3D(-1, 0, -1), Normal3D(0, 1, 0), Vector3D(1, 0, 0), TexCoord2D(0, 1)}};\n"
+        "  mesh->setVertices(v, 1);\n"
+        '  ShaderNodeRef tex = context->createShaderNode("Image2DTexture");\n'
+        '  tex->set("min filter", "Nearest");\n'
+        '  SurfaceMaterialRef mat = context->createSurfaceMaterial("Matte");\n'
+        '  mat->set("albedo", tex->getPlug(VLRShaderNodePlugType_Spectrum, 0));\n'
+        "  uint32_t group[] = {0, 0, 0};\n"
+        "  mesh->addMaterialGroup(group, 3, mat, ShaderNodePlug(), ShaderNodePlug(), ShaderNodePlug());\n"
+        '  InternalNodeRef node = context->createInternalNode("trf", context->createStaticTransform(scale(2.0f)));\n'
+        "  node->addChild(mesh);\n"
+        "  scene->addChild(node);\n"
+        '  CameraRef camera = context->createCamera("Perspective");\n'
+        '  camera->set("position", Point3D(0, 1.5f, 6.0f));\n'
+        '  camera->set("orientation", qRotateY(3.14159265f));\n'
+        "  context->bindOutputBuffer(64, 64, 0);\n"
+        "  context->setScene(scene);\n"
+        "  uint32_t n;\n"
+        "  context->render(nullptr, camera, false, 1, true, 1, &n);\n"
+        "  hipArray_t array;\n"
+        "  context->getOutputBuffer(&array);\n"
+        "  return 0;\n"
+        "}\n")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-D__HIP_PLATFORM_AMD__",
+                        f"-I{ROOT / 'include'}", "-I/opt/rocm/include", str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
+def test_example_callers_are_built():
+    """build() compiles the C and C++ callers of vlr_amd/examples against the library."""
+    bin_dir = ROOT / "vlr_amd" / "examples" / "bin"
+    for name in ("cornell", "cornell_cpp"):
+        exe = bin_dir / name
+        assert exe.exists(), f"{exe} missing: run vlr_amd/build.py"
+        r = subprocess.run(["ldd", str(exe)], capture_output=True, text=True)
+        assert "libvlr_amd.so" in r.stdout and "not found" not in r.stdout.split("libvlr_amd.so")[1].split("\n")[0]
+
+
 # ---------------------------------------------------------------------------------------------
 @pytest.fixture()
 def ctx():

@@ -61,7 +61,8 @@ class VLRShaderNodePlug(C.Structure):
 class VLRAccelStats(C.Structure):
     _fields_ = [("numBlas", C.c_uint32), ("numTlasInstances", C.c_uint32), ("numTriangles", C.c_uint64),
                 ("numBlasNodes", C.c_uint32), ("numTlasNodes", C.c_uint32), ("msBlas", C.c_float),
-                ("msTlas", C.c_float), ("blasReused", C.c_uint32), ("numNodeProgramSteps", C.c_uint32)]
+                ("msTlas", C.c_float), ("blasReused", C.c_uint32), ("numNodeProgramSteps", C.c_uint32),
+                ("tlasDepth", C.c_uint32), ("blasDepth", C.c_uint32), ("tlasRefit", C.c_uint32)]
 
 
 # 64-byte quantised 4-wide BVH node: child k's box on axis a is
@@ -229,6 +230,8 @@ PROTOTYPES: Dict[str, list] = {
     "vlrContextGetRenderStats": [H, C.POINTER(VLRRenderStats)],
     "vlrContextResetRenderStats": [H],
     "vlrContextSetKernelTiming": [H, B],
+    "vlrContextSetDenoiserAuxBuffers": [H, B],
+    "vlrContextReadDenoiserAuxBuffers": [H, P, P],
     "vlrContextSetTraversalStatistics": [H, B],
     "vlrContextGetQueueCounts": [H, P, P, U32],
     "vlrContextSetWavefrontDepth": [H, U32],
@@ -261,14 +264,17 @@ PROTOTYPES: Dict[str, list] = {
     "vlrContextRadianceCacheTrain": [H, H, P, P, U32, U32],
     "vlrContextGetNumLightPaths": [H, C.POINTER(U32)],
     "vlrContextGetAccumulationChannels": [H, C.POINTER(U32)],
-    "vlrContextReadAccumulation": [H, P],
-    "vlrContextWriteAccumulation": [H, P, U32],
+    "vlrContextReadAccumulation": [H, P, C.c_size_t],
+    "vlrContextWriteAccumulation": [H, P, C.c_size_t, U32],
+    "vlrContextGetBptWavelengthFrames": [H, C.POINTER(C.c_uint64)],
+    "vlrContextSetBptWavelengthFrames": [H, C.c_uint64],
     "vlrContextTraceRays": [H, H, H, P, U32, P, B],
     "vlrContextEvaluateBSDFs": [H, H, H, B, P, U32, P],
     "vlrContextEvaluateShaderNodePlug": [H, H, VLRShaderNodePlug, B, P, U32, P],
     "vlrContextSetAccelBuilder": [H, U32],
     "vlrContextGetAccelBuilder": [H, C.POINTER(U32)],
     "vlrContextGetAccelStats": [H, C.POINTER(VLRAccelStats)],
+    "vlrContextSetTlasRefit": [H, B],
     "vlrContextReadAccel": [H, H, U32, P, C.c_size_t, C.POINTER(C.c_size_t)],
     "vlrBvh4NodeCapacity": [U32, C.POINTER(U32)],
     "vlrBuildBvh4Host": [P, U32, U32, P, P, U32, P, C.POINTER(U32)],
@@ -475,16 +481,30 @@ class Context:
         call("vlrContextRadianceCacheTrain", self.h, stream, C.c_void_p(records_ptr), C.c_void_p(targets_ptr), n,
              num_out)
 
-    def read_accumulation(self) -> np.ndarray:
+    def accumulation_channels(self) -> int:
         ch = U32()
         call("vlrContextGetAccumulationChannels", self.h, C.byref(ch))
-        out = np.zeros((self.height, self.width, ch.value, 2), np.float32)
-        call("vlrContextReadAccumulation", self.h, out.ctypes.data)
+        return ch.value
+
+    def read_accumulation(self) -> np.ndarray:
+        out = np.zeros((self.height, self.width, self.accumulation_channels(), 2), np.float32)
+        call("vlrContextReadAccumulation", self.h, out.ctypes.data, out.size)
         return out
 
     def write_accumulation(self, acc: np.ndarray, frames: int):
         a = np.ascontiguousarray(acc, np.float32)
-        call("vlrContextWriteAccumulation", self.h, a.ctypes.data, frames)
+        expect = (self.height, self.width, self.accumulation_channels(), 2)
+        if a.shape != expect:
+            raise ValueError(f"accumulation of shape {a.shape}, the context expects {expect}")
+        call("vlrContextWriteAccumulation", self.h, a.ctypes.data, a.size, frames)
+
+    def bpt_wavelength_frames(self) -> int:
+        v = C.c_uint64()
+        call("vlrContextGetBptWavelengthFrames", self.h, C.byref(v))
+        return v.value
+
+    def set_bpt_wavelength_frames(self, frames: int):
+        call("vlrContextSetBptWavelengthFrames", self.h, frames)
 
     def stats(self) -> dict:
         s = VLRRenderStats()
@@ -493,6 +513,16 @@ class Context:
 
     def reset_stats(self):
         call("vlrContextResetRenderStats", self.h)
+
+    def set_denoiser_aux_buffers(self, on: bool):
+        call("vlrContextSetDenoiserAuxBuffers", self.h, on)
+
+    def read_denoiser_aux_buffers(self):
+        """(albedo, normal): H x W x 4 float32 each, converted as the reference's copyBuffers."""
+        alb = np.zeros((self.height, self.width, 4), np.float32)
+        nrm = np.zeros((self.height, self.width, 4), np.float32)
+        call("vlrContextReadDenoiserAuxBuffers", self.h, alb.ctypes.data, nrm.ctypes.data)
+        return alb, nrm
 
     def set_kernel_timing(self, on: bool):
         call("vlrContextSetKernelTiming", self.h, on)
@@ -547,6 +577,9 @@ class Context:
         b = U32()
         call("vlrContextGetAccelBuilder", self.h, C.byref(b))
         return b.value
+
+    def set_tlas_refit(self, on: bool):
+        call("vlrContextSetTlasRefit", self.h, on)
 
     def accel_stats(self) -> dict:
         s = VLRAccelStats()

@@ -38,8 +38,13 @@ def _flags(debug: bool = False):
     return f
 
 
-def _compile(src: Path, obj: Path, debug: bool):
-    cmd = [hipcc(), "-x", "hip", "-c", str(src), "-o", str(obj)] + _flags(debug)
+# sources compiled once per unit (-DVLRA_UNIT=1..n); each compilation emits one part of the
+# file's kernels, so the parts compile in parallel
+UNITS = {"kernels.hip": 9}
+
+
+def _compile(src: Path, obj: Path, debug: bool, defines=()):
+    cmd = [hipcc(), "-x", "hip", "-c", str(src), "-o", str(obj)] + _flags(debug) + list(defines)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {src.name}\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
@@ -47,16 +52,46 @@ def _compile(src: Path, obj: Path, debug: bool):
     return src.name, warn
 
 
-def build(debug: bool = False, jobs: int = 8, verbose: bool = False) -> Path:
+def _deps(src: Path, seen=None) -> set:
+    """The file and every quoted include it reaches (csrc/ and include/)."""
+    seen = set() if seen is None else seen
+    if src in seen or not src.exists():
+        return seen
+    seen.add(src)
+    for line in src.read_text(errors="replace").splitlines():
+        line = line.strip()
+        if line.startswith("#include \""):
+            name = line.split('"')[1]
+            for base in (src.parent, PKG.parent / "include"):
+                if (base / name).exists():
+                    _deps(base / name, seen)
+                    break
+    return seen
+
+
+def _stale(src: Path, obj: Path) -> bool:
+    if not obj.exists():
+        return True
+    t = obj.stat().st_mtime
+    return any(t < d.stat().st_mtime for d in _deps(src))
+
+
+def build(debug: bool = False, jobs: int = 8, verbose: bool = False, incremental: bool = False) -> Path:
+    """Compiles every source (incremental=True: only those older than their object or one of
+    the headers they include) and links the library."""
     BUILD_DIR.mkdir(exist_ok=True)
     sources = sorted(list(SRC.glob("*.cpp")) + list(SRC.glob("*.hip")))
     objs = []
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         futs = []
         for s in sources:
-            o = BUILD_DIR / (s.name + ".o")
-            objs.append(o)
-            futs.append(ex.submit(_compile, s, o, debug))
+            units = UNITS.get(s.name)
+            for u in (range(1, units + 1) if units else [None]):
+                o = BUILD_DIR / (s.name + (f".u{u}" if u else "") + ".o")
+                objs.append(o)
+                if incremental and not debug and not _stale(s, o):
+                    continue
+                futs.append(ex.submit(_compile, s, o, debug, [f"-DVLRA_UNIT={u}"] if u else []))
         for f in futs:
             name, warn = f.result()
             if verbose and warn:
@@ -67,8 +102,31 @@ def build(debug: bool = False, jobs: int = 8, verbose: bool = False) -> Path:
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, OUT)
+    build_examples()
     return OUT
 
 
+EXAMPLES = PKG / "examples"
+
+
+def build_examples() -> list:
+    """Compiles the C and C++ callers in vlr_amd/examples (plain gcc / g++ against the public
+    headers, linked to libvlr_amd.so) into vlr_amd/examples/bin."""
+    out_dir = EXAMPLES / "bin"
+    out_dir.mkdir(exist_ok=True)
+    inc = ["-D__HIP_PLATFORM_AMD__", f"-I{PKG.parent / 'include'}", "-I/opt/rocm/include"]
+    link = [f"-L{PKG}", "-lvlr_amd", "-L/opt/rocm/lib", "-lamdhip64", "-lm",
+            "-Wl,-rpath,$ORIGIN/../..", "-Wl,-rpath,/opt/rocm/lib"]
+    built = []
+    for src, compiler, std in (("cornell.c", "gcc", "-std=c11"), ("cornell.cpp", "g++", "-std=c++17")):
+        exe = out_dir / ("cornell_cpp" if src.endswith(".cpp") else "cornell")
+        cmd = [compiler, std, "-O2", "-Wall", *inc, str(EXAMPLES / src), "-o", str(exe), *link]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"example build failed: {src}\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+        built.append(exe)
+    return built
+
+
 if __name__ == "__main__":
-    print(build(debug="--debug" in sys.argv, verbose=True))
+    print(build(debug="--debug" in sys.argv, verbose=True, incremental="--incremental" in sys.argv))

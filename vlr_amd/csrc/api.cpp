@@ -145,8 +145,8 @@ VLR_API VLRResult vlrContextGetOutputBufferSize(VLRContext context, uint32_t *wi
 VLR_API VLRResult vlrContextGetOutputBuffer(VLRContext context, hipArray_t *array) {
     VLRA_TRY
     CHECK(context, IS(context, Context));
-    if (!array) return VLRResult_InvalidArgument;
-    *array = reinterpret_cast<hipArray_t>(CTX(context)->output.ptr);
+    if (!array || !CTX(context)->width) return VLRResult_InvalidArgument;
+    *array = CTX(context)->outputArrayHandle();
     return VLRResult_NoError;
     VLRA_CATCH
 }
@@ -1057,6 +1057,7 @@ VLR_API VLRResult vlrContextGetRenderStats(VLRContextConst context, VLRRenderSta
     stats->instancesShadow = trav[5];
     stats->numFrames = c->framesRendered;
     stats->paths = c->pathsStarted;
+    c->collectTimers();
     stats->msGenerate = c->stageMs[STAGE_GENERATE];
     stats->msIntersect = c->stageMs[STAGE_INTERSECT];
     stats->msShade = c->stageMs[STAGE_SHADE];
@@ -1085,7 +1086,26 @@ VLR_API VLRResult vlrContextResetRenderStats(VLRContext context) {
 VLR_API VLRResult vlrContextSetKernelTiming(VLRContext context, bool enable) {
     VLRA_TRY
     CHECK(context, IS(context, Context));
+    CTX(context)->collectTimers();
     CTX(context)->kernelTiming = enable;
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+VLR_API VLRResult vlrContextSetDenoiserAuxBuffers(VLRContext context, bool enable) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    Context *c = CTX(context);
+    if (enable && !c->guidesEnabled) c->guideFrames = 0;
+    c->guidesEnabled = enable;
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+VLR_API VLRResult vlrContextReadDenoiserAuxBuffers(VLRContextConst context, float *albedoRGBA, float *normalRGBA) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    Context *c = CTX(context);
+    if (!c->width) return VLRResult_InvalidArgument;
+    c->readGuides(albedoRGBA, normalRGBA);
     return VLRResult_NoError;
     VLRA_CATCH
 }
@@ -1395,24 +1415,30 @@ VLR_API VLRResult vlrContextGetAccumulationChannels(VLRContextConst context, uin
     VLRA_TRY
     CHECK(context, IS(context, Context));
     if (!channels) return VLRResult_InvalidArgument;
-    *channels = CTX(context)->accumChannels;
+    *channels = CTX(context)->spectralRequested ? 16u : 3u;
     return VLRResult_NoError;
     VLRA_CATCH
 }
-VLR_API VLRResult vlrContextReadAccumulation(VLRContextConst context, float *data) {
+VLR_API VLRResult vlrContextReadAccumulation(VLRContextConst context, float *data, size_t numFloats) {
     VLRA_TRY
     CHECK(context, IS(context, Context));
     Context *c = CTX(context);
     if (!data || !c->width) return VLRResult_InvalidArgument;
+    // the accumulation in place belongs to the other mode until a first frame switches it
+    if (c->spectralActive != c->spectralRequested) return VLRResult_InvalidArgument;
+    if (numFloats != size_t(c->width) * c->height * c->accumChannels * 2) return VLRResult_InvalidArgument;
     c->readAccumulation(data);
     return VLRResult_NoError;
     VLRA_CATCH
 }
-VLR_API VLRResult vlrContextWriteAccumulation(VLRContext context, const float *data, uint32_t numAccumFrames) {
+VLR_API VLRResult vlrContextWriteAccumulation(VLRContext context, const float *data, size_t numFloats,
+                                              uint32_t numAccumFrames) {
     VLRA_TRY
     CHECK(context, IS(context, Context));
     Context *c = CTX(context);
     if (!data || !c->width) return VLRResult_InvalidArgument;
+    const uint32_t channels = c->spectralRequested ? 16u : 3u;
+    if (numFloats != size_t(c->width) * c->height * channels * 2) return VLRResult_InvalidArgument;
     // a resumed render continues in the requested transport (RGB or spectral channels)
     if (c->spectralActive != c->spectralRequested) {
         c->spectralActive = c->spectralRequested;
@@ -1421,6 +1447,21 @@ VLR_API VLRResult vlrContextWriteAccumulation(VLRContext context, const float *d
     c->writeAccumulation(data);
     c->numAccumFrames = numAccumFrames;
     c->resumePending = true;
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+VLR_API VLRResult vlrContextGetBptWavelengthFrames(VLRContextConst context, uint64_t *frames) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    if (!frames) return VLRResult_InvalidArgument;
+    *frames = CTX(context)->bptWlFrames;
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+VLR_API VLRResult vlrContextSetBptWavelengthFrames(VLRContext context, uint64_t frames) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    CTX(context)->setBptWavelengthFrames(frames);
     return VLRResult_NoError;
     VLRA_CATCH
 }
@@ -1453,6 +1494,13 @@ VLR_API VLRResult vlrContextGetAccelBuilder(VLRContextConst context, uint32_t *b
     VLRA_CATCH
 }
 
+VLR_API VLRResult vlrContextSetTlasRefit(VLRContext context, bool enable) {
+    VLRA_TRY
+    CHECK(context, IS(context, Context));
+    CTX(context)->tlasRefit = enable;
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
 VLR_API VLRResult vlrContextGetAccelStats(VLRContextConst context, VLRAccelStats *stats) {
     VLRA_TRY
     CHECK(context, IS(context, Context));
@@ -1467,6 +1515,9 @@ VLR_API VLRResult vlrContextGetAccelStats(VLRContextConst context, VLRAccelStats
     stats->msTlas = a.msTlas;
     stats->blasReused = a.blasReused;
     stats->numNodeProgramSteps = a.numNodeProgramSteps;
+    stats->tlasDepth = a.tlasDepth;
+    stats->blasDepth = a.blasDepth;
+    stats->tlasRefit = a.tlasRefit;
     return VLRResult_NoError;
     VLRA_CATCH
 }

@@ -41,6 +41,18 @@ constexpr uint32_t kBvhSahMaxDepth = 40;     // root split is depth 1
 constexpr uint32_t kBvhSmallPrims = 1024;    // subtrees of at most this many primitives: one workgroup
 constexpr uint32_t kBvhMaxPrims = 1u << 28;  // leaf code limit (first << kLeafShift)
 
+// Traversal node stack per ray: kLdsStack entries in LDS, kScratchStack more in scratch.  A
+// visit of an inner 4-wide node pushes at most 3 entries and descends one level; entering an
+// instance pushes one marker.  A two-level walk therefore never holds more than
+// travStackBound(TLAS depth, deepest BLAS depth) entries, and scene set-up refuses trees whose
+// bound exceeds the capacity, so no entry is ever lost.
+constexpr int kLdsStack = 16;
+constexpr int kScratchStack = 64;
+constexpr uint32_t kTravStackCapacity = kLdsStack + kScratchStack;
+__host__ __device__ inline uint32_t travStackBound(uint32_t tlasDepth, uint32_t blasDepth) {
+    return 3 * (tlasDepth + blasDepth) + 1;
+}
+
 __host__ __device__ inline float bvhNoNegZero(float v) { return v + 0.0f; }
 
 // Box area as the SAH uses it (0 for an empty box).
@@ -116,6 +128,7 @@ __host__ __device__ inline BVH4QNode quantizeBvh4Node(const BVH4Node &f) {
 struct HostBvh4 {
     std::vector<BVH4QNode> nodes;
     std::vector<uint32_t> order;   // position -> primitive
+    uint32_t depth = 0;            // levels of 4-wide nodes (the root alone: 1)
 };
 void hostBuildBvh4(const float *boxes, uint32_t numPrims, uint32_t maxLeaf, const uint32_t *leafRemap,
                    HostBvh4 *out);

@@ -170,9 +170,11 @@ public:
     Bvh4Collapser(const std::vector<BVHNode> &bin, const uint32_t *leafRemap, const std::vector<uint32_t> &order)
         : bin_(bin), remap_(leafRemap), order_(order) {}
     std::vector<BVH4Node> nodes;
+    uint32_t depth = 0;
     void run() {
         nodes.clear();
-        if (!bin_.empty()) emit(0);
+        depth = 0;
+        if (!bin_.empty()) emit(0, 1);
     }
 
 private:
@@ -201,7 +203,8 @@ private:
         const uint32_t first = static_cast<uint32_t>(~code) >> kLeafShift;
         return static_cast<int32_t>(~(remap_[order_[first]] << kLeafShift));
     }
-    uint32_t emit(uint32_t binNode) {
+    uint32_t emit(uint32_t binNode, uint32_t level) {
+        depth = std::max(depth, level);
         const uint32_t idx = static_cast<uint32_t>(nodes.size());
         nodes.push_back(BVH4Node());
         Slot slots[4];
@@ -237,7 +240,8 @@ private:
         }
         int32_t codes[4] = {0, 0, 0, 0};
         for (int k = 0; k < n; ++k)
-            codes[k] = slots[k].code > 0 ? static_cast<int32_t>(emit(static_cast<uint32_t>(slots[k].code))) : leaf(slots[k].code);
+            codes[k] = slots[k].code > 0 ? static_cast<int32_t>(emit(static_cast<uint32_t>(slots[k].code), level + 1))
+                                         : leaf(slots[k].code);
         BVH4Node &N = nodes[idx];
         for (int k = 0; k < 4; ++k) {
             const bool used = k < n;
@@ -271,6 +275,7 @@ void hostBuildBvh4(const float *boxes, uint32_t numPrims, uint32_t maxLeaf, cons
     out->nodes.resize(c.nodes.size());
     for (size_t i = 0; i < c.nodes.size(); ++i) out->nodes[i] = quantizeBvh4Node(c.nodes[i]);
     out->order = std::move(b2.order);
+    out->depth = c.depth;
 }
 
 } // namespace vlra

@@ -30,10 +30,9 @@ void Context::ensureDevice() {
 
 Context::~Context() {
     if (device >= 0) (void)hipDeviceSynchronize();
-    for (hipEvent_t e : eventPool) (void)hipEventDestroy(e);
+    if (outputArray) (void)hipFreeArray(outputArray);
     for (hipEvent_t e : tuneEvents)
         if (e) (void)hipEventDestroy(e);
-    for (hipEvent_t e : eventPoolB) (void)hipEventDestroy(e);
     for (hipEvent_t e : partEvents)
         if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : wavefrontEvents)
@@ -55,6 +54,10 @@ void Context::bindOutputBuffer(uint32_t w, uint32_t h) {
     if (w == 0 || h == 0) throw ApiError{3, "output size must be positive"};
     ensureDevice();
     VLRA_HIP(hipDeviceSynchronize());
+    if (outputArray && (w != width || h != height)) {
+        VLRA_HIP(hipFreeArray(outputArray));
+        outputArray = nullptr;
+    }
     width = w;
     height = h;
     const size_t n = size_t(w) * h;
@@ -93,38 +96,50 @@ void Context::ensureScene(const Scene &s) {
 namespace {
 // Records a pair of events around each launch, from a pool the context keeps, and folds the
 // elapsed times into the per-stage totals once the frame has been enqueued.
-struct EventTimer : public LaunchTimer {
-    hipStream_t stream;
-    std::vector<hipEvent_t> &pool;
-    std::vector<int> stages;
-    size_t used = 0;
-    EventTimer(hipStream_t s, std::vector<hipEvent_t> &p) : stream(s), pool(p) {}
-    hipEvent_t next() {
-        if (used == pool.size()) {
-            hipEvent_t e;
-            VLRA_HIP(hipEventCreate(&e));
-            pool.push_back(e);
-        }
-        return pool[used++];
-    }
-    void begin(int) override { VLRA_HIP(hipEventRecord(next(), stream)); }
-    void end(int stage) override {
-        VLRA_HIP(hipEventRecord(next(), stream));
-        stages.push_back(stage);
-    }
-    void collect(double *ms, uint64_t *launches) {
-        VLRA_HIP(hipStreamSynchronize(stream));
-        for (size_t k = 0; k < stages.size(); ++k) {
-            float t = 0;
-            VLRA_HIP(hipEventElapsedTime(&t, pool[2 * k], pool[2 * k + 1]));
-            ms[stages[k]] += t;
-            launches[stages[k]] += 1;
-        }
-        stages.clear();
-        used = 0;
-    }
-};
 } // namespace
+
+Context::EventTimer::~EventTimer() {
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+}
+hipEvent_t Context::EventTimer::next() {
+    if (used == pool.size()) {
+        hipEvent_t e;
+        VLRA_HIP(hipEventCreate(&e));
+        pool.push_back(e);
+    }
+    return pool[used++];
+}
+void Context::EventTimer::begin(int) { VLRA_HIP(hipEventRecord(next(), stream)); }
+void Context::EventTimer::end(int stage) {
+    VLRA_HIP(hipEventRecord(next(), stream));
+    stages.push_back(stage);
+}
+void Context::EventTimer::collect(double *ms, uint64_t *launches) {
+    if (used == 0) return;
+    VLRA_HIP(hipEventSynchronize(pool[used - 1]));
+    for (size_t k = 0; k < stages.size(); ++k) {
+        float t = 0;
+        VLRA_HIP(hipEventElapsedTime(&t, pool[2 * k], pool[2 * k + 1]));
+        ms[stages[k]] += t;
+        launches[stages[k]] += 1;
+    }
+    stages.clear();
+    used = 0;
+}
+
+Context::EventTimer *Context::timer(int k, hipStream_t s) {
+    if (!kernelTiming) return nullptr;
+    EventTimer &t = timers[k];
+    // events of one launch are recorded on one stream; launches of different streams do not mix
+    if (t.used && t.stream != s) t.collect(stageMs, launches);
+    if (t.used >= EventTimer::kMaxPending) t.collect(stageMs, launches);
+    t.stream = s;
+    return &t;
+}
+
+void Context::collectTimers() {
+    for (EventTimer &t : timers) t.collect(stageMs, launches);
+}
 
 void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCoeff, bool firstFrame, uint32_t limit,
                      uint32_t *numAccum) {
@@ -167,6 +182,7 @@ void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCo
         fp.wavefrontDepth = wavefrontDepth;
         fp.tailBlocks = tailBlocks;
         fp.sortShade = shadeSort ? 1 : 0;
+        fp.guideReset = 0;
         fp.cacheMode = 0;
         fp.frameStamp = 0;
         fp.cacheThreshold = 0.0f;
@@ -176,6 +192,7 @@ void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCo
         fp.pixelCount = fp.imageW * fp.imageH;
         if (bpt || lightTracing) {
             renderBptFrame(stream, fp, firstFrame, lightTracing);
+            updateOutputArray(stream);
             if (numAccum) *numAccum = numAccumFrames;
             return;
         }
@@ -213,6 +230,20 @@ void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCo
             sortCounts.reserve(2 * kSortClasses * kMaxDepthCounters * sizeof(uint32_t));
         }
         FrameBuffers fb = frameBuffers();
+        if (guidesEnabled) {
+            const size_t n = size_t(width) * height, bytes = n * (accumChannels + 3) * sizeof(float);
+            if (guideBytes != bytes) {
+                guideAlbedo.reserve(n * accumChannels * sizeof(float));
+                guideNormal.reserve(n * 3 * sizeof(float));
+                guideBytes = bytes;
+                guideFrames = 0;
+            }
+            // the guides restart with the accumulation, or when they were just switched on
+            fp.guideReset = (numAccumFrames == 1 || guideFrames == 0) ? 1 : 0;
+            guideFrames = fp.guideReset ? 1 : guideFrames + 1;
+            fb.guideAlbedo = static_cast<float *>(guideAlbedo.ptr);
+            fb.guideNormal = static_cast<float *>(guideNormal.ptr);
+        }
         CacheLaunch cl;
         const CacheLaunch *clp = nullptr;
         if (cacheEnabled) {
@@ -256,12 +287,8 @@ void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCo
         if (!split) joinHalves(stream);
         if (split) {
             renderSplitFrame(stream, fp, fb, tuning);
-        } else if (kernelTiming) {
-            EventTimer t(stream, eventPool);
-            VLRA_HIP(launchFrame(gpu.dev, fp, fb, spectralActive, stream, &t, clp));
-            t.collect(stageMs, launches);
         } else {
-            VLRA_HIP(launchFrame(gpu.dev, fp, fb, spectralActive, stream, nullptr, clp));
+            VLRA_HIP(launchFrame(gpu.dev, fp, fb, spectralActive, stream, timer(0, stream), clp));
         }
         if (tuning) {
             VLRA_HIP(hipEventRecord(tuneEvents[1], stream));
@@ -273,8 +300,31 @@ void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCo
         lastFrame = fp;
         ++framesRendered;
         pathsStarted += uint64_t(fp.imageW) * fp.imageH;
+        updateOutputArray(stream);
     }
     if (numAccum) *numAccum = numAccumFrames;
+}
+
+// The output as a 2D array of float4 (the reference's cudau::Array, context.cpp:1243), made on
+// the first request; from then on every rendered frame is copied into it on the render stream.
+hipArray_t Context::outputArrayHandle() {
+    ensureDevice();
+    if (!width) throw ApiError{3, "bind an output buffer first"};
+    if (!outputArray) {
+        hipChannelFormatDesc desc = hipCreateChannelDesc<float4>();
+        VLRA_HIP(hipMallocArray(&outputArray, &desc, width, height, hipArraySurfaceLoadStore));
+        VLRA_HIP(hipDeviceSynchronize());
+        VLRA_HIP(hipMemcpy2DToArray(outputArray, 0, 0, output.ptr, size_t(width) * sizeof(float4),
+                                    size_t(width) * sizeof(float4), height, hipMemcpyDeviceToDevice));
+    }
+    return outputArray;
+}
+
+void Context::updateOutputArray(hipStream_t stream) {
+    if (!outputArray) return;
+    joinHalves(stream);
+    VLRA_HIP(hipMemcpy2DToArrayAsync(outputArray, 0, 0, output.ptr, size_t(width) * sizeof(float4),
+                                     size_t(width) * sizeof(float4), height, hipMemcpyDeviceToDevice, stream));
 }
 
 // The frame as two pixel halves, each generated, traced, shaded, accumulated and converted on
@@ -295,14 +345,6 @@ void Context::renderSplitFrame(hipStream_t stream, const FrameParams &fp, const 
     }
     VLRA_HIP(hipEventRecord(partEvents[2], stream));
     const uint32_t half = fp.pixelCount / 2;
-    EventTimer *timers[2] = {nullptr, nullptr};
-    std::unique_ptr<EventTimer> t0, t1;
-    if (kernelTiming) {
-        t0 = std::make_unique<EventTimer>(partStreams[0], eventPool);
-        t1 = std::make_unique<EventTimer>(partStreams[1], eventPoolB);
-        timers[0] = t0.get();
-        timers[1] = t1.get();
-    }
     for (int k = 0; k < 2; ++k) {
         VLRA_HIP(hipStreamWaitEvent(partStreams[k], partEvents[2], 0));
         // the halves take turns with their per-depth kernels: each half's tail kernel (a few
@@ -323,16 +365,12 @@ void Context::renderSplitFrame(hipStream_t stream, const FrameParams &fp, const 
         b.shadow.dir += o;
         b.shadow.value += o;
         if (k == 1) b.counters = static_cast<uint32_t *>(counters2.ptr);
-        VLRA_HIP(launchFrame(gpu.dev, p, b, spectralActive, partStreams[k], timers[k], nullptr, true,
-                             wavefrontEvents[k]));
+        VLRA_HIP(launchFrame(gpu.dev, p, b, spectralActive, partStreams[k], timer(k, partStreams[k]), nullptr,
+                             true, wavefrontEvents[k]));
         wavefrontRecorded[k] = true;
         VLRA_HIP(hipEventRecord(partEvents[k], partStreams[k]));
     }
     halvesPending = true;
-    if (kernelTiming) {
-        t0->collect(stageMs, launches);
-        t1->collect(stageMs, launches);
-    }
     if (join) joinHalves(stream);
 }
 
@@ -364,6 +402,7 @@ FrameBuffers Context::frameBuffers() {
     fb.accum = static_cast<float *>(accum.ptr);
     fb.output = static_cast<float4 *>(output.ptr);
     fb.cache = CacheBuffers{};
+    fb.guideAlbedo = fb.guideNormal = nullptr;
     return fb;
 }
 
@@ -407,9 +446,8 @@ void Context::renderBptFrame(hipStream_t stream, const FrameParams &fp, bool fir
         VLRA_HIP(hipMemsetAsync(splat.ptr, 0, size_t(width) * height * accumChannels * sizeof(float), stream));
     // common wavelengths of a BPT frame: two draws, offset first (WavelengthSamples::
     // createWithEqualOffsets, spectrum_types.h); light tracing draws per light path instead
-    std::uniform_real_distribution<float> u01;
-    const float wlOffset = lightTracing ? 0.0f : u01(bptWlRng);
-    const float wlSelect = lightTracing ? 0.0f : u01(bptWlRng);
+    float wlOffset = 0.0f, wlSelect = 0.0f;
+    if (!lightTracing) bptWavelengths(&wlOffset, &wlSelect);
     const uint32_t N = spectralActive ? 4 : 3;
     BptParams bp;
     bp.wlOffset = wlOffset;
@@ -427,13 +465,7 @@ void Context::renderBptFrame(hipStream_t stream, const FrameParams &fp, bool fir
     bp.splat = static_cast<float *>(splat.ptr);
     const FrameBuffers fb = frameBuffers();
     auto launch = lightTracing ? launchLightTraceFrame : launchBptFrame;
-    if (kernelTiming) {
-        EventTimer t(stream, eventPool);
-        VLRA_HIP(launch(gpu.dev, fp, fb, bp, spectralActive, stream, &t));
-        t.collect(stageMs, launches);
-    } else {
-        VLRA_HIP(launch(gpu.dev, fp, fb, bp, spectralActive, stream, nullptr));
-    }
+    VLRA_HIP(launch(gpu.dev, fp, fb, bp, spectralActive, stream, timer(0, stream)));
     lastFrame = fp;
     ++framesRendered;
     pathsStarted += lightTracing ? uint64_t(numLightPaths) : uint64_t(fp.imageW) * fp.imageH;
@@ -487,6 +519,7 @@ RcDevice Context::rcDevice() {
     d.delta = static_cast<uint16_t *>(rcDelta.ptr);
     d.lossPartial = static_cast<float *>(rcLossPartial.ptr);
     d.loss = static_cast<float *>(rcLoss.ptr);
+    d.batchStride = rc::kMaxBatch;
     return d;
 }
 
@@ -971,6 +1004,77 @@ void Context::queueCounts(uint32_t *closest, uint32_t *shadow, uint32_t maxDepth
     }
 }
 
+void Context::bptWavelengths(float *offset, float *select) {
+    std::uniform_real_distribution<float> u01;
+    *offset = u01(bptWlRng);
+    *select = u01(bptWlRng);
+    ++bptWlFrames;
+}
+
+void Context::setBptWavelengthFrames(uint64_t frames) {
+    bptWlRng = std::mt19937();
+    bptWlFrames = 0;
+    float o, s;
+    for (uint64_t f = 0; f < frames; ++f) bptWavelengths(&o, &s);
+}
+
+void Context::readGuides(float *albedoRGBA, float *normalRGBA) {
+    const size_t n = size_t(width) * height;
+    const uint32_t C = accumChannels;
+    std::vector<float> alb(n * C, 0.0f), nrm(n * 3, 0.0f);
+    VLRA_HIP(hipDeviceSynchronize());
+    if (guideFrames > 0) {
+        VLRA_HIP(hipMemcpy(alb.data(), guideAlbedo.ptr, alb.size() * sizeof(float), hipMemcpyDeviceToHost));
+        VLRA_HIP(hipMemcpy(nrm.data(), guideNormal.ptr, nrm.size() * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    float cx[16], cy[16], cz[16], integral = 1.0f;
+    if (C == 16) cmfBins(cx, cy, cz, &integral);
+    const float rec = guideFrames > 0 ? 1.0f / guideFrames : 0.0f;
+    const float *R = frameCamera.rot;   // column-major; its transpose is the inverse orientation
+    for (size_t p = 0; p < n; ++p) {
+        float X = 0.0f, Y = 0.0f, Z = 0.0f;
+        if (C == 16) {
+            for (uint32_t k = 0; k < 16; ++k) {
+                const float v = alb[k * n + p];
+                X += cx[k] * v;
+                Y += cy[k] * v;
+                Z += cz[k] * v;
+            }
+            X /= integral;
+            Y /= integral;
+            Z /= integral;
+        } else {
+            const float r = alb[p], g = alb[n + p], b = alb[2 * n + p];
+            X = 0.4124564f * r + 0.3575761f * g + 0.1804375f * b;
+            Y = 0.2126729f * r + 0.7151522f * g + 0.0721750f * b;
+            Z = 0.0193339f * r + 0.1191920f * g + 0.9503041f * b;
+        }
+        X *= rec;
+        Y *= rec;
+        Z *= rec;
+        if (albedoRGBA) {
+            albedoRGBA[4 * p + 0] = 3.2404542f * X + -1.5371385f * Y + -0.4985314f * Z;
+            albedoRGBA[4 * p + 1] = -0.9692660f * X + 1.8760108f * Y + 0.0415560f * Z;
+            albedoRGBA[4 * p + 2] = 0.0556434f * X + -0.2040259f * Y + 1.0572252f * Z;
+            albedoRGBA[4 * p + 3] = 1.0f;
+        }
+        if (normalRGBA) {
+            const float nx = nrm[p], ny = nrm[n + p], nz = nrm[2 * n + p];
+            float c[3];
+            for (int i = 0; i < 3; ++i) c[i] = R[3 * i + 0] * nx + R[3 * i + 1] * ny + R[3 * i + 2] * nz;
+            c[0] *= -1;
+            if (c[0] != 0 || c[1] != 0 || c[2] != 0) {
+                const float l = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+                for (float &v : c) v /= l;
+            }
+            normalRGBA[4 * p + 0] = c[0];
+            normalRGBA[4 * p + 1] = c[1];
+            normalRGBA[4 * p + 2] = c[2];
+            normalRGBA[4 * p + 3] = 1.0f;
+        }
+    }
+}
+
 void Context::readAccumulation(float *perPixel) {
     const size_t n = size_t(width) * height, planes = size_t(accumChannels) * 2;
     std::vector<float> dev(n * planes);
@@ -990,6 +1094,7 @@ void Context::writeAccumulation(const float *perPixel) {
 }
 
 void Context::resetStats() {
+    collectTimers();
     if (device >= 0) {
         VLRA_HIP(hipDeviceSynchronize());
         VLRA_HIP(hipMemset(counters.ptr, 0, kNumCounters * sizeof(uint32_t)));

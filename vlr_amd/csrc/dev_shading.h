@@ -412,6 +412,16 @@ struct Bsdf {
     }
     __device__ __forceinline__ bool matches(uint32_t flags) const { return dt::matches(typeMask(), flags); }
     __device__ __forceinline__ bool hasNonDelta() const { return matches(dt::WholeSphere | dt::NonDelta); }
+    // getBaseColor of each BSDF (materials.cu:537-539, 592-594, 677-679, 805-807, 1077-1081,
+    // 1551-1553, 1702-1704; an emitter's EDF as a BSDF, 416-419): the denoiser's albedo guide
+    __device__ __forceinline__ Spec<N> baseColor() const {
+        switch (kind) {
+        case BSDF_MATTE: case BSDF_SPEC_BRDF: case BSDF_SPEC_BSDF: case BSDF_MICRO_BSDF: return c0;
+        case BSDF_MICRO_BRDF: return fresnelConductor<N>(c0, c1, 1.0f);
+        case BSDF_DIFFSPEC: return c0 + c1;
+        default: return Spec<N>(0.0f);
+        }
+    }
 
     // UE4 selection weights (materials.cu:1724-1734) for a query direction of cosine qz
     __device__ __forceinline__ void diffSpecWeights(float qz, float vz, uint32_t wl, float *wd, float *ws) const {

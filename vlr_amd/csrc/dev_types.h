@@ -257,6 +257,7 @@ struct DevScene {
     DevCamera camera;
     float cmfX[16], cmfY[16], cmfZ[16];
     float integralCMF;
+    DevSpectrum whitePoint;      // light-source (1, 1, 1) in Rec.709 (D65): the guides' white
 };
 
 // Wavefront queues and per-pixel path state.

@@ -21,6 +21,7 @@
 // in LDS with an overflow area in scratch, watertight ray/triangle test (Woop, Benthin and
 // Wald, JCGT 2(1) 2013), compiled without floating-point contraction so that closest hits
 // reproduce the oracle's brute force bit for bit.
+#include "bvh.h"
 #include "dev_shading.h"
 #include "shader_nodes.h"
 #include "launch.h"
@@ -28,11 +29,42 @@
 
 #pragma clang fp contract(off)
 
+// The build compiles this file once per unit (VLRA_UNIT): 1 RGB path tracing, 2 the same
+// spectral, 3 RGB LVC-BPT and light tracing, 4 the same spectral, 5 dispatchers, conversion
+// and the direct query kernels, 6 RGB render batches, 7 the same spectral, 8 RGB path
+// tracing of scenes with node programs, 9 the same spectral; 0 (or no definition) emits
+// everything.  The units
+// compile in parallel.  Each unit's kernels live in a namespace of their own, so no kernel is
+// defined by two units.
+#ifndef VLRA_UNIT
+#define VLRA_UNIT 0
+#endif
+#if VLRA_UNIT == 1
+#define VLRA_UNIT_NS unit_pt_rgb
+#elif VLRA_UNIT == 2
+#define VLRA_UNIT_NS unit_pt_spectral
+#elif VLRA_UNIT == 3
+#define VLRA_UNIT_NS unit_bpt_rgb
+#elif VLRA_UNIT == 4
+#define VLRA_UNIT_NS unit_bpt_spectral
+#elif VLRA_UNIT == 5
+#define VLRA_UNIT_NS unit_misc
+#elif VLRA_UNIT == 6
+#define VLRA_UNIT_NS unit_batch_rgb
+#elif VLRA_UNIT == 7
+#define VLRA_UNIT_NS unit_batch_spectral
+#elif VLRA_UNIT == 8
+#define VLRA_UNIT_NS unit_pt_nodes_rgb
+#elif VLRA_UNIT == 9
+#define VLRA_UNIT_NS unit_pt_nodes_spectral
+#else
+#define VLRA_UNIT_NS unit_all
+#endif
+
 namespace vlra {
+namespace VLRA_UNIT_NS {
 
 constexpr int kTraceBlock = 128;
-constexpr int kLdsStack = 16;
-constexpr int kScratchStack = 64;
 // per block: kLdsStack entries per lane of node codes, then as many entry distances
 constexpr int kTraceLds = 2 * kLdsStack * kTraceBlock;
 #ifndef VLRA_TAIL_WAVES_PER_EU
@@ -172,7 +204,9 @@ __device__ __forceinline__ bool travStep(TravState &st, const DevScene &sc, int3
                                          float *ovfT, int maxSteps, TravCount *tc) {
     float *ldsT = reinterpret_cast<float *>(lds + kLdsStack * kTraceBlock);
     // each stack entry keeps the distance at which the ray enters its box; once a closer hit
-    // has shrunk tmax past it (with the slab test's widening), the entry is dropped unvisited
+    // has shrunk tmax past it (with the slab test's widening), the entry is dropped unvisited.
+    // Scene set-up bounds the depth by kTravStackCapacity (travStackBound), so the index never
+    // reaches past the scratch area; the clamp only keeps the access inside the array.
     auto push = [&](int32_t v, float tEnter) {
         if (st.sp < kLdsStack) {
             lds[st.sp * kTraceBlock + lane] = v;
@@ -1152,7 +1186,54 @@ __device__ __forceinline__ void storePath(const FrameBuffers &fb, uint32_t path,
     }
 }
 
-template <int N, bool MULTI = false, bool NODES = true>
+// Denoiser guides of a path's first hit (path_tracing.cu:95-108, 154-157, 272-275): the
+// BSDF's base colour times the D65 white point over the wavelength pdf, added to the
+// pixel's strata like a colour sample (without compensation), and the shading normal; a miss
+// adds zero.  Path index = pixel index (row stride fp.width).
+template <int N, bool NODES>
+__device__ __forceinline__ void firstHitGuides(const KArgs &a, uint32_t path, const int4 &ha, const float2 &hb,
+                                               const Wls<N> &wls) {
+    Spec<N> albedo(0.0f);
+    V3 n(0.0f);
+    if (ha.x >= 0) {
+        SurfPt sp;
+        float hypAreaPDF;
+        decodeHit<NODES>(a.sc, ha.x, ha.y, ha.z, hb.x, hb.y, &sp, &hypAreaPDF);
+        const DevMaterial &mat = a.sc.materials[a.sc.geomInsts[ha.y].material];
+        constexpr float selectWLPDF = N == 4 ? 4 / (830.0f - 360.0f) : 1.0f;
+        albedo = setupBsdf<N, NODES>(a.sc, mat, wls, &sp).baseColor() * evalSpectrum<N>(a.sc, a.sc.whitePoint, wls) /
+                 selectWLPDF;
+        n = sp.frame.z;
+    }
+    const size_t plane = static_cast<size_t>(a.fp.width) * a.fp.height;
+    const bool reset = a.fp.guideReset != 0;
+    float *alb = a.fb.guideAlbedo + path;
+    if constexpr (N == 4) {
+        const float recBinWidth = 16 / (830.0f - 360.0f);
+        float addend[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) addend[k] = 0.0f;
+#pragma unroll
+        for (int w = 0; w < 4; ++w) {
+            float lam = 360.0f + (830.0f - 360.0f) * (w + wls.offset) / 4;
+            uint32_t bin = min(static_cast<uint32_t>((lam - 360.0f) / (830.0f - 360.0f) * 16), 15u);
+#pragma unroll
+            for (int k = 0; k < 16; ++k)
+                if (k == static_cast<int>(bin)) addend[k] += albedo[w] * recBinWidth;
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) alb[k * plane] = (reset ? 0.0f : alb[k * plane]) + addend[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) alb[k * plane] = (reset ? 0.0f : alb[k * plane]) + albedo[k];
+    }
+    float *nrm = a.fb.guideNormal + path;
+    nrm[0] = (reset ? 0.0f : nrm[0]) + n.x;
+    nrm[plane] = (reset ? 0.0f : nrm[plane]) + n.y;
+    nrm[2 * plane] = (reset ? 0.0f : nrm[2 * plane]) + n.z;
+}
+
+template <int N, bool MULTI = false, bool NODES = true, bool GUIDES = false>
 __global__ void __launch_bounds__(256) k_shade(KArgs a) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t depth = a.depth;
@@ -1168,6 +1249,9 @@ __global__ void __launch_bounds__(256) k_shade(KArgs a) {
         float2 hb = ha.x >= 0 ? a.fb.hits.b[j] : make_float2(0.0f, 0.0f);
         PathRegs<N> P;
         loadPath<N>(a.fb, path, &P);
+        if constexpr (GUIDES) {
+            if (depth == 0) firstHitGuides<N, NODES>(a, path, ha, hb, P.wls);
+        }
         if constexpr (MULTI) {
             const DevJob &J = a.jobs[jobOf(a, path)];
             shadeStep<N, NODES>(J.scene, depth + 1, J.maxPathLength, path, xyz(o4), xyz(d4), ha, hb, P, cacheCtx(a.fp, a.fb),
@@ -2697,7 +2781,7 @@ __global__ void k_bsdfQueries(DevScene sc, DevMaterial m, const BsdfQueryIn *q, 
 // launchers
 static inline uint32_t blocksFor(uint32_t n, uint32_t bs) { return (n + bs - 1) / bs; }
 
-template <int N>
+template <int N, bool NODES>
 static hipError_t launchFrameT(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb, hipStream_t s,
                                LaunchTimer *timer, const CacheLaunch *cache, bool convert,
                                hipEvent_t afterWavefront) {
@@ -2717,7 +2801,9 @@ static hipError_t launchFrameT(const DevScene &sc, const FrameParams &fp, const 
     hipLaunchKernelGGL(k_generate<N>, dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
     t1(STAGE_GENERATE);
     const uint32_t numDepths = std::min<uint32_t>(fp.maxPathLength, kMaxDepthCounters - 1);
-    const uint32_t wavefrontDepths = std::min(fp.wavefrontDepth, numDepths);
+    // the guides are written by the depth-0 shade kernel
+    const bool guides = fb.guideAlbedo != nullptr;
+    const uint32_t wavefrontDepths = std::min(std::max(fp.wavefrontDepth, guides ? 1u : 0u), numDepths);
     for (uint32_t d = 0; d < wavefrontDepths; ++d) {
         a.depth = d;
         t0(STAGE_INTERSECT);
@@ -2731,10 +2817,10 @@ static hipError_t launchFrameT(const DevScene &sc, const FrameParams &fp, const 
         t1(STAGE_INTERSECT);
         t0(STAGE_SHADE);
         if (fp.sortShade) hipLaunchKernelGGL(k_sortScatter, dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
-        if (sc.hasNodePrograms)
-            hipLaunchKernelGGL((k_shade<N, false, true>), dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
+        if (guides && d == 0)
+            hipLaunchKernelGGL((k_shade<N, false, NODES, true>), dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
         else
-            hipLaunchKernelGGL((k_shade<N, false, false>), dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
+            hipLaunchKernelGGL((k_shade<N, false, NODES>), dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
         t1(STAGE_SHADE);
         t0(STAGE_SHADOW);
         if (sc.hasAlpha)
@@ -2753,14 +2839,10 @@ static hipError_t launchFrameT(const DevScene &sc, const FrameParams &fp, const 
         a.depth = wavefrontDepths;
         const uint32_t blocks = std::max(1u, std::min(blocksFor(n, kTraceBlock), fp.tailBlocks));
         t0(STAGE_TAIL);
-        if (fp.traversalStats && sc.hasNodePrograms)
-            hipLaunchKernelGGL((k_tail<N, true, false, true>), dim3(blocks), dim3(kTraceBlock), 0, s, a);
-        else if (fp.traversalStats)
-            hipLaunchKernelGGL((k_tail<N, true, false, false>), dim3(blocks), dim3(kTraceBlock), 0, s, a);
-        else if (sc.hasNodePrograms)
-            hipLaunchKernelGGL((k_tail<N, false, false, true>), dim3(blocks), dim3(kTraceBlock), 0, s, a);
+        if (fp.traversalStats)
+            hipLaunchKernelGGL((k_tail<N, true, false, NODES>), dim3(blocks), dim3(kTraceBlock), 0, s, a);
         else
-            hipLaunchKernelGGL((k_tail<N, false, false, false>), dim3(blocks), dim3(kTraceBlock), 0, s, a);
+            hipLaunchKernelGGL((k_tail<N, false, false, NODES>), dim3(blocks), dim3(kTraceBlock), 0, s, a);
         t1(STAGE_TAIL);
     }
     if (cache && fp.cacheMode == 2) {
@@ -2802,33 +2884,6 @@ static hipError_t launchFrameT(const DevScene &sc, const FrameParams &fp, const 
     return hipGetLastError();
 }
 
-hipError_t launchFrame(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb, bool spectral,
-                       hipStream_t s, LaunchTimer *timer, const CacheLaunch *cache, bool convert,
-                       hipEvent_t afterWavefront) {
-    return spectral ? launchFrameT<4>(sc, fp, fb, s, timer, cache, convert, afterWavefront)
-                    : launchFrameT<3>(sc, fp, fb, s, timer, cache, convert, afterWavefront);
-}
-
-hipError_t launchConvert(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb, bool spectral,
-                         hipStream_t s, LaunchTimer *timer) {
-    KArgs a;
-    a.sc = sc;
-    a.fp = fp;
-    a.fb = fb;
-    a.depth = 0;
-    a.jobs = nullptr;
-    a.jobOffsets = nullptr;
-    a.numJobs = 0;
-    const uint32_t n = fp.pixelCount;
-    if (timer) timer->begin(STAGE_CONVERT);
-    if (spectral)
-        hipLaunchKernelGGL(k_convert<4>, dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
-    else
-        hipLaunchKernelGGL(k_convert<3>, dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
-    if (timer) timer->end(STAGE_CONVERT);
-    return hipGetLastError();
-}
-
 // One step of a batch: every job with samples left renders one more sample per pixel.  The
 // launch sequence reads nothing from the host that changes between steps (the jobs' frame
 // counters live on the device), so it can be captured once into a graph and replayed.
@@ -2867,29 +2922,6 @@ static hipError_t launchBatchStepT(const BatchLaunch &b, hipStream_t s) {
     }
     hipLaunchKernelGGL((k_accumulate<N, true>), dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_frameEnd, dim3(1), dim3(64), 0, s, b.fb.counters, b.fp.maxPathLength, wavefrontDepths);
-    return hipGetLastError();
-}
-
-hipError_t launchBatchStep(const BatchLaunch &b, bool spectral, hipStream_t s) {
-    if (b.numJobs == 0) return hipSuccess;
-    return spectral ? launchBatchStepT<4>(b, s) : launchBatchStepT<3>(b, s);
-}
-
-hipError_t launchBatchConvert(const BatchLaunch &b, bool spectral, hipStream_t s) {
-    if (b.numJobs == 0) return hipSuccess;
-    KArgs a;
-    a.sc = b.anyScene;
-    a.fp = b.fp;
-    a.fb = b.fb;
-    a.depth = 0;
-    a.jobs = b.jobs;
-    a.jobOffsets = b.jobOffsets;
-    a.numJobs = b.numJobs;
-    const uint32_t n = b.fp.imageW * b.fp.imageH;
-    if (spectral)
-        hipLaunchKernelGGL((k_convert<4, true>), dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
-    else
-        hipLaunchKernelGGL((k_convert<3, true>), dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
     return hipGetLastError();
 }
 
@@ -2975,28 +3007,6 @@ static hipError_t launchLightTraceFrameT(const DevScene &sc, const FrameParams &
     return hipGetLastError();
 }
 
-hipError_t launchLightTraceFrame(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb,
-                                 const BptParams &bp, bool spectral, hipStream_t s, LaunchTimer *timer) {
-    return spectral ? launchLightTraceFrameT<4>(sc, fp, fb, bp, s, timer)
-                    : launchLightTraceFrameT<3>(sc, fp, fb, bp, s, timer);
-}
-
-hipError_t launchBptFrame(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb, const BptParams &bp,
-                          bool spectral, hipStream_t s, LaunchTimer *timer) {
-    return spectral ? launchBptFrameT<4>(sc, fp, fb, bp, s, timer) : launchBptFrameT<3>(sc, fp, fb, bp, s, timer);
-}
-
-hipError_t launchTraceRays(const DevScene &sc, const void *rays, uint32_t n, void *hits, bool anyHit, hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    if (anyHit)
-        hipLaunchKernelGGL(k_traceRays<true>, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, sc,
-                           static_cast<const RayIn *>(rays), n, static_cast<HitOut *>(hits));
-    else
-        hipLaunchKernelGGL(k_traceRays<false>, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, sc,
-                           static_cast<const RayIn *>(rays), n, static_cast<HitOut *>(hits));
-    return hipGetLastError();
-}
-
 // ---- direct evaluation of a node program at given surface points ----
 // point: position 3, geometric normal 3, shading frame x 3, y 3, z 3, texture coordinates 2,
 // wavelength offset 1, pad 2 (20 floats); instance 0 of the scene record is the identity
@@ -3037,6 +3047,166 @@ __global__ void __launch_bounds__(128) k_evalNodePlug(DevScene sc, uint32_t base
     out[i] = r;
 }
 
+}  // namespace VLRA_UNIT_NS
+
+// ---- entry points ----
+// launchers of one channel count each (units 1-4), called by the dispatchers (unit 5)
+hipError_t launchFrameRgb(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb, hipStream_t s,
+                          LaunchTimer *timer, const CacheLaunch *cache, bool convert, hipEvent_t afterWavefront);
+hipError_t launchFrameSpectral(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb, hipStream_t s,
+                               LaunchTimer *timer, const CacheLaunch *cache, bool convert,
+                               hipEvent_t afterWavefront);
+hipError_t launchFrameNodesRgb(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb, hipStream_t s,
+                               LaunchTimer *timer, const CacheLaunch *cache, bool convert,
+                               hipEvent_t afterWavefront);
+hipError_t launchFrameNodesSpectral(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb,
+                                    hipStream_t s, LaunchTimer *timer, const CacheLaunch *cache, bool convert,
+                                    hipEvent_t afterWavefront);
+hipError_t launchBatchStepRgb(const BatchLaunch &b, hipStream_t s);
+hipError_t launchBatchStepSpectral(const BatchLaunch &b, hipStream_t s);
+hipError_t launchBptFrameRgb(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb, const BptParams &bp,
+                             hipStream_t s, LaunchTimer *timer);
+hipError_t launchBptFrameSpectral(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb,
+                                  const BptParams &bp, hipStream_t s, LaunchTimer *timer);
+hipError_t launchLightTraceFrameRgb(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb,
+                                    const BptParams &bp, hipStream_t s, LaunchTimer *timer);
+hipError_t launchLightTraceFrameSpectral(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb,
+                                         const BptParams &bp, hipStream_t s, LaunchTimer *timer);
+
+#if VLRA_UNIT == 0 || VLRA_UNIT == 1
+hipError_t launchFrameRgb(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb, hipStream_t s,
+                          LaunchTimer *timer, const CacheLaunch *cache, bool convert, hipEvent_t afterWavefront) {
+    return VLRA_UNIT_NS::launchFrameT<3, false>(sc, fp, fb, s, timer, cache, convert, afterWavefront);
+}
+#endif
+#if VLRA_UNIT == 0 || VLRA_UNIT == 8
+hipError_t launchFrameNodesRgb(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb, hipStream_t s,
+                               LaunchTimer *timer, const CacheLaunch *cache, bool convert,
+                               hipEvent_t afterWavefront) {
+    return VLRA_UNIT_NS::launchFrameT<3, true>(sc, fp, fb, s, timer, cache, convert, afterWavefront);
+}
+#endif
+#if VLRA_UNIT == 0 || VLRA_UNIT == 6
+hipError_t launchBatchStepRgb(const BatchLaunch &b, hipStream_t s) { return VLRA_UNIT_NS::launchBatchStepT<3>(b, s); }
+#endif
+#if VLRA_UNIT == 0 || VLRA_UNIT == 2
+hipError_t launchFrameSpectral(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb, hipStream_t s,
+                               LaunchTimer *timer, const CacheLaunch *cache, bool convert,
+                               hipEvent_t afterWavefront) {
+    return VLRA_UNIT_NS::launchFrameT<4, false>(sc, fp, fb, s, timer, cache, convert, afterWavefront);
+}
+#endif
+#if VLRA_UNIT == 0 || VLRA_UNIT == 9
+hipError_t launchFrameNodesSpectral(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb,
+                                    hipStream_t s, LaunchTimer *timer, const CacheLaunch *cache, bool convert,
+                                    hipEvent_t afterWavefront) {
+    return VLRA_UNIT_NS::launchFrameT<4, true>(sc, fp, fb, s, timer, cache, convert, afterWavefront);
+}
+#endif
+#if VLRA_UNIT == 0 || VLRA_UNIT == 7
+hipError_t launchBatchStepSpectral(const BatchLaunch &b, hipStream_t s) {
+    return VLRA_UNIT_NS::launchBatchStepT<4>(b, s);
+}
+#endif
+#if VLRA_UNIT == 0 || VLRA_UNIT == 3
+hipError_t launchBptFrameRgb(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb, const BptParams &bp,
+                             hipStream_t s, LaunchTimer *timer) {
+    return VLRA_UNIT_NS::launchBptFrameT<3>(sc, fp, fb, bp, s, timer);
+}
+hipError_t launchLightTraceFrameRgb(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb,
+                                    const BptParams &bp, hipStream_t s, LaunchTimer *timer) {
+    return VLRA_UNIT_NS::launchLightTraceFrameT<3>(sc, fp, fb, bp, s, timer);
+}
+#endif
+#if VLRA_UNIT == 0 || VLRA_UNIT == 4
+hipError_t launchBptFrameSpectral(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb,
+                                  const BptParams &bp, hipStream_t s, LaunchTimer *timer) {
+    return VLRA_UNIT_NS::launchBptFrameT<4>(sc, fp, fb, bp, s, timer);
+}
+hipError_t launchLightTraceFrameSpectral(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb,
+                                         const BptParams &bp, hipStream_t s, LaunchTimer *timer) {
+    return VLRA_UNIT_NS::launchLightTraceFrameT<4>(sc, fp, fb, bp, s, timer);
+}
+#endif
+
+#if VLRA_UNIT == 0 || VLRA_UNIT == 5
+using namespace VLRA_UNIT_NS;
+
+hipError_t launchFrame(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb, bool spectral,
+                       hipStream_t s, LaunchTimer *timer, const CacheLaunch *cache, bool convert,
+                       hipEvent_t afterWavefront) {
+    if (sc.hasNodePrograms)
+        return spectral ? launchFrameNodesSpectral(sc, fp, fb, s, timer, cache, convert, afterWavefront)
+                        : launchFrameNodesRgb(sc, fp, fb, s, timer, cache, convert, afterWavefront);
+    return spectral ? launchFrameSpectral(sc, fp, fb, s, timer, cache, convert, afterWavefront)
+                    : launchFrameRgb(sc, fp, fb, s, timer, cache, convert, afterWavefront);
+}
+
+hipError_t launchConvert(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb, bool spectral,
+                         hipStream_t s, LaunchTimer *timer) {
+    KArgs a;
+    a.sc = sc;
+    a.fp = fp;
+    a.fb = fb;
+    a.depth = 0;
+    a.jobs = nullptr;
+    a.jobOffsets = nullptr;
+    a.numJobs = 0;
+    const uint32_t n = fp.pixelCount;
+    if (timer) timer->begin(STAGE_CONVERT);
+    if (spectral)
+        hipLaunchKernelGGL(k_convert<4>, dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_convert<3>, dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
+    if (timer) timer->end(STAGE_CONVERT);
+    return hipGetLastError();
+}
+
+hipError_t launchBatchStep(const BatchLaunch &b, bool spectral, hipStream_t s) {
+    if (b.numJobs == 0) return hipSuccess;
+    return spectral ? launchBatchStepSpectral(b, s) : launchBatchStepRgb(b, s);
+}
+
+hipError_t launchBatchConvert(const BatchLaunch &b, bool spectral, hipStream_t s) {
+    if (b.numJobs == 0) return hipSuccess;
+    KArgs a;
+    a.sc = b.anyScene;
+    a.fp = b.fp;
+    a.fb = b.fb;
+    a.depth = 0;
+    a.jobs = b.jobs;
+    a.jobOffsets = b.jobOffsets;
+    a.numJobs = b.numJobs;
+    const uint32_t n = b.fp.imageW * b.fp.imageH;
+    if (spectral)
+        hipLaunchKernelGGL((k_convert<4, true>), dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_convert<3, true>), dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launchLightTraceFrame(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb,
+                                 const BptParams &bp, bool spectral, hipStream_t s, LaunchTimer *timer) {
+    return spectral ? launchLightTraceFrameSpectral(sc, fp, fb, bp, s, timer)
+                    : launchLightTraceFrameRgb(sc, fp, fb, bp, s, timer);
+}
+
+hipError_t launchBptFrame(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb, const BptParams &bp,
+                          bool spectral, hipStream_t s, LaunchTimer *timer) {
+    return spectral ? launchBptFrameSpectral(sc, fp, fb, bp, s, timer) : launchBptFrameRgb(sc, fp, fb, bp, s, timer);
+}
+
+hipError_t launchTraceRays(const DevScene &sc, const void *rays, uint32_t n, void *hits, bool anyHit, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    if (anyHit)
+        hipLaunchKernelGGL(k_traceRays<true>, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, sc,
+                           static_cast<const RayIn *>(rays), n, static_cast<HitOut *>(hits));
+    else
+        hipLaunchKernelGGL(k_traceRays<false>, dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, sc,
+                           static_cast<const RayIn *>(rays), n, static_cast<HitOut *>(hits));
+    return hipGetLastError();
+}
+
 hipError_t launchEvalNodePlug(const DevScene &sc, uint32_t base, uint32_t count, uint32_t plugType, bool spectral,
                               const float *points, uint32_t n, float *out, hipStream_t s) {
     if (n == 0) return hipSuccess;
@@ -3061,5 +3231,7 @@ hipError_t launchBsdfQueries(const DevScene &sc, const DevMaterial &m, bool spec
                            static_cast<const BsdfQueryIn *>(q), n, static_cast<BsdfResultOut *>(r));
     return hipGetLastError();
 }
+
+#endif
 
 } // namespace vlra

@@ -31,6 +31,7 @@ struct FrameParams {
     // (row-major over imageW); a frame may be rendered as several such parts
     uint32_t pixelBegin, pixelCount;
     uint32_t sortShade;            // shade each per-depth queue in order of shading class
+    uint32_t guideReset;           // the denoiser guides restart with this frame
 };
 
 // Shading classes of the material sort: 0 = miss, 1 + material type for a hit.
@@ -123,6 +124,10 @@ struct FrameBuffers {
     float* accum;                  // W*H*channels*2 (sum, comp)
     float4* output;                // W*H
     CacheBuffers cache;
+    // denoiser guides (null when off): the first hit's albedo binned like the colour (3 or 16
+    // planes of W*H floats) and its shading normal (3 planes), summed over frames
+    float* guideAlbedo;
+    float* guideNormal;
 };
 
 // timing hook: called before/after each launch (may be null)

@@ -58,10 +58,11 @@ struct RcDevice {
     float* adamV;         // second moments (kNumParams)
     float* grad;          // reduced gradient of the last step (kNumParams)
     float* partial;       // gradient partials [slice][kNumParams] (weights, then biases)
-    uint16_t* act;        // [kActRows][kMaxBatch] bf16
-    uint16_t* delta;      // [kDeltaRows][kMaxBatch] bf16
+    uint16_t* act;        // [kActRows][batchStride] bf16
+    uint16_t* delta;      // [kDeltaRows][batchStride] bf16
     float* lossPartial;   // per training block
     float* loss;          // mean loss of the last step
+    uint32_t batchStride; // samples per row of act / delta: the largest batch they hold
 };
 
 struct RcTrainParams {

@@ -130,7 +130,7 @@ __device__ __forceinline__ void forwardWave(const RcDevice &d, uint16_t *act, ui
                     const uint32_t sample = sampleBase + 16 * q + c;
 #pragma unroll
                     for (uint32_t r = 0; r < 4; ++r)
-                        d.act[(actRow(l + 1) + 16 * t + 4 * g + r) * kMaxBatch + sample] = h[r];
+                        d.act[(actRow(l + 1) + 16 * t + 4 * g + r) * d.batchStride + sample] = h[r];
                 }
             }
             __builtin_amdgcn_wave_barrier();
@@ -232,7 +232,7 @@ __global__ void __launch_bounds__(kTrainBlock) k_rcTrain(RcDevice d, RcTrainPara
     encodeRow(raw + static_cast<size_t>(valid ? s : 0) * kRawFloats, valid, act + lane * kRow);
     // x_0 to HBM
 #pragma unroll 8
-    for (uint32_t k = 0; k < kIn; ++k) d.act[(actRow(0) + k) * kMaxBatch + s] = act[lane * kRow + k];
+    for (uint32_t k = 0; k < kIn; ++k) d.act[(actRow(0) + k) * d.batchStride + s] = act[lane * kRow + k];
     __builtin_amdgcn_wave_barrier();
     f32x4 y[4];
     forwardWave<true>(d, act, lane, base, y);
@@ -258,7 +258,7 @@ __global__ void __launch_bounds__(kTrainBlock) k_rcTrain(RcDevice d, RcTrainPara
         }
         // delta_5 (16 padded rows) to HBM; lanes g = 0 hold rows 0..3
 #pragma unroll
-        for (uint32_t r = 0; r < 4; ++r) d.delta[(deltaRow(5) + 4 * g + r) * kMaxBatch + sq] = h[r];
+        for (uint32_t r = 0; r < 4; ++r) d.delta[(deltaRow(5) + 4 * g + r) * d.batchStride + sq] = h[r];
         // LDS row of sample 16q + c: columns 0..31 = delta_5 rows 0..31 (rows >= 4 are 0)
         if (g == 0) {
             uint16_t *row = dl + (16 * q + c) * kDeltaRowLds;
@@ -306,9 +306,9 @@ __global__ void __launch_bounds__(kTrainBlock) k_rcTrain(RcDevice d, RcTrainPara
 #pragma unroll
                 for (uint32_t r = 0; r < 4; ++r) {
                     const uint32_t nrn = 16 * t + 4 * g + r;
-                    const float x = fromBf16(d.act[(actRow(l) + nrn) * kMaxBatch + sample]);
+                    const float x = fromBf16(d.act[(actRow(l) + nrn) * d.batchStride + sample]);
                     h[r] = toBf16(x > 0.0f ? acc[t][r] : 0.0f);
-                    d.delta[(deltaRow(l - 1) + nrn) * kMaxBatch + sample] = h[r];
+                    d.delta[(deltaRow(l - 1) + nrn) * d.batchStride + sample] = h[r];
                 }
                 *reinterpret_cast<u16x4 *>(row + 16 * t + 4 * g) = u16x4{h[0], h[1], h[2], h[3]};
             }
@@ -352,8 +352,8 @@ __global__ void __launch_bounds__(256) k_rcWeightGrad(RcDevice d, uint32_t nPad,
     tileOf(tile, &l, &ti, &tj);
     const uint32_t s0 = slice * kGradSliceSamples;
     const uint32_t s1 = min(s0 + kGradSliceSamples, nPad);
-    const uint16_t *Arow = d.delta + static_cast<size_t>(deltaRow(l) + 16 * ti + c) * kMaxBatch;
-    const uint16_t *Brow = d.act + static_cast<size_t>(actRow(l) + 16 * tj + c) * kMaxBatch;
+    const uint16_t *Arow = d.delta + static_cast<size_t>(deltaRow(l) + 16 * ti + c) * d.batchStride;
+    const uint16_t *Brow = d.act + static_cast<size_t>(actRow(l) + 16 * tj + c) * d.batchStride;
     f32x4 acc = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     float rowSum = 0.0f;
     for (uint32_t s = s0; s < s1; s += 32) {
@@ -433,7 +433,7 @@ hipError_t rcLaunchQuery(const RcDevice &d, const float *raw, const float4 *thro
 
 hipError_t rcLaunchTrain(const RcDevice &d, const RcTrainParams &p, const float *raw, const float *targets,
                          const uint32_t *countDev, uint32_t n, hipStream_t s) {
-    if (n == 0 || n > kMaxBatch) return hipErrorInvalidValue;
+    if (n == 0 || n > d.batchStride || d.batchStride % kGradSliceSamples != 0) return hipErrorInvalidValue;
     const uint32_t blocks = blocksFor(n, kTrainBlock);
     const uint32_t nPad = blocks * kTrainBlock;
     const uint32_t slices = blocksFor(nPad, kGradSliceSamples);

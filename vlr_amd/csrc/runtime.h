@@ -396,6 +396,13 @@ struct SceneGPU {
     uint32_t numInstances = 0, numGeomInsts = 0;
     uint64_t numTriangles = 0;
     uint32_t numBlasNodes = 0, numTlasNodes = 0;
+    uint32_t blasDepth = 0;
+    // the TLAS as last built or refitted (host copy), its inputs' key (instances and their
+    // BLASes) and the relative cost it had when built; a transform-only update refits it
+    std::vector<BVH4QNode> tlasHost;
+    std::vector<uint64_t> tlasKey;
+    double tlasBuildCost = 0.0;
+    uint32_t tlasDepth = 0;
     // BLAS inputs of the last build (per geometry group: node, version, group, geometry
     // instance, triangle count) and their layout; an equal key reuses the BLASes
     std::vector<uint64_t> blasKey;
@@ -412,6 +419,8 @@ struct AccelStats {
     float msBlas = 0.0f, msTlas = 0.0f;
     uint32_t blasReused = 0;
     uint32_t numNodeProgramSteps = 0;
+    uint32_t tlasDepth = 0, blasDepth = 0;   // levels of 4-wide nodes
+    uint32_t tlasRefit = 0;                  // 1: the TLAS was refitted, not rebuilt
 };
 
 void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *out);
@@ -440,6 +449,7 @@ public:
     int device = -1;
     uint64_t version = 1;     // bumped on every scene-affecting change
     uint32_t accelBuilder = kAccelBuilderDevice;
+    bool tlasRefit = true;             // transform-only updates refit the TLAS
     AccelStats accelStats;
     std::unique_ptr<GpuBvhBuilder> gpuBvh;
     GpuBvhBuilder &bvhBuilder() {
@@ -463,8 +473,23 @@ public:
     bool autotune = true;            // cleared when the caller sets the schedule explicitly
     ScheduleTuner tuner;
     hipEvent_t tuneEvents[2] = {nullptr, nullptr};
-    std::vector<hipEvent_t> eventPool;   // reused by kernel timing, 2 per timed launch
-    std::vector<hipEvent_t> eventPoolB;  // the same for the second half of a split frame
+    // kernel timing: an event before and after every launch, read back lazily, so that timed
+    // frames are not synchronised; timers[1] times the second half of a split frame
+    struct EventTimer : public LaunchTimer {
+        static constexpr size_t kMaxPending = size_t(1) << 14;   // events in flight before a read-back
+        hipStream_t stream = nullptr;
+        std::vector<hipEvent_t> pool;
+        std::vector<int> stages;
+        size_t used = 0;
+        ~EventTimer();
+        hipEvent_t next();
+        void begin(int) override;
+        void end(int stage) override;
+        void collect(double *ms, uint64_t *launches);
+    };
+    EventTimer timers[2];
+    EventTimer *timer(int k, hipStream_t s);   // null while kernel timing is off
+    void collectTimers();                      // adds the finished launches to stageMs / launches
     uint32_t width = 0, height = 0;
     uint32_t numAccumFrames = 0;
     FrameParams lastFrame = {};
@@ -480,6 +505,9 @@ public:
     uint32_t numLightPaths = 500000;
     uint32_t lightRngPaths = 0;      // light paths the light RNG buffer was seeded for
     std::mt19937 bptWlRng;
+    uint64_t bptWlFrames = 0;        // BPT frames that drew their wavelengths from bptWlRng
+    void bptWavelengths(float *offset, float *select);
+    void setBptWavelengthFrames(uint64_t frames);
     DeviceBuffer lightRng, lvSlots, lvCounts, lvOffsets, lvBlockSums, lvCache, lvNum, splat;
     size_t splatLayout = 0;          // bytes of the splat planes last cleared
     // radiance cache (no counterpart in the reference)
@@ -509,6 +537,9 @@ public:
     void touch() { ++version; }
     void ensureDevice();
     void bindOutputBuffer(uint32_t w, uint32_t h);
+    hipArray_t outputArray = nullptr;   // made by vlrContextGetOutputBuffer, then updated every frame
+    hipArray_t outputArrayHandle();
+    void updateOutputArray(hipStream_t stream);
     void render(hipStream_t stream, const Camera *camera, uint32_t shrinkCoeff, bool firstFrame, uint32_t limit,
                 uint32_t *numAccum);
     void readOutput(float *dst);
@@ -531,6 +562,13 @@ public:
     // The device keeps the Kahan pairs as planes (sum of channel k at plane 2k, compensation at
     // 2k + 1, width*height floats each); the API exchanges them per pixel (channel, pair).
     void readAccumulation(float *perPixel);
+    // denoiser guides of the path tracer (first-hit albedo and shading normal per sample,
+    // path_tracing.cu:95-108), converted on read as copyBuffers does (post_process.cu:70-90)
+    bool guidesEnabled = false;
+    DeviceBuffer guideAlbedo, guideNormal;
+    uint32_t guideFrames = 0;        // samples per pixel summed in the guides
+    size_t guideBytes = 0;
+    void readGuides(float *albedoRGBA, float *normalRGBA);
     void writeAccumulation(const float *perPixel);
     // LVC-BPT buffers for the current light-path count, path length and output size
     void ensureBpt(uint32_t slotsPerPath);

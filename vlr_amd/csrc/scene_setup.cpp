@@ -153,6 +153,40 @@ float appendContinuous(std::vector<float> &pool, const float *values, size_t n) 
 
 } // namespace
 
+// TLAS refit: the 4-wide tree over the instances keeps its topology; each node's child boxes
+// are recomputed bottom-up from the instances' world boxes and quantised again.  Returns node
+// n's box and adds the areas of the inner nodes' boxes to *sumArea (the tree's SAH-style cost
+// up to a constant).  Leaf codes carry the instance index.
+static Box refitTlasNode(std::vector<BVH4QNode> &nodes, uint32_t n, const std::vector<Box> &boxOfInst, double *sumArea) {
+    BVH4Node f;
+    std::memset(&f, 0, sizeof(f));
+    Box all;
+    for (int k = 0; k < 4; ++k) {
+        const int32_t code = nodes[n].child[k];
+        f.child[k] = code;
+        if (code == 0) continue;
+        Box b = code > 0 ? refitTlasNode(nodes, static_cast<uint32_t>(code), boxOfInst, sumArea)
+                         : boxOfInst[static_cast<uint32_t>(~code) >> kLeafShift];
+        f.loX[k] = bvhNoNegZero(b.lo[0]);
+        f.loY[k] = bvhNoNegZero(b.lo[1]);
+        f.loZ[k] = bvhNoNegZero(b.lo[2]);
+        f.hiX[k] = bvhNoNegZero(b.hi[0]);
+        f.hiY[k] = bvhNoNegZero(b.hi[1]);
+        f.hiZ[k] = bvhNoNegZero(b.hi[2]);
+        all.grow(b);
+    }
+    nodes[n] = quantizeBvh4Node(f);
+    *sumArea += all.area();
+    return all;
+}
+
+// relative cost of a TLAS over the current instance boxes (rewrites `nodes` with refitted ones)
+static double refitTlas(std::vector<BVH4QNode> &nodes, const std::vector<Box> &boxOfInst) {
+    double sum = 0.0;
+    const Box root = refitTlasNode(nodes, 0, boxOfInst, &sum);
+    return root.area() > 0.0f ? sum / root.area() : sum;
+}
+
 void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g) {
     if (!loadSpectralTables()) throw std::runtime_error("spectral tables not found next to the library");
     std::vector<FlatInstance> flat;
@@ -502,9 +536,11 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
                 }
             }
     }
+    uint32_t blasDepth = 0, tlasDepth = 0;
     if (reuseBlas) {
         for (size_t k = 0; k < blasList.size(); ++k) blasList[k].nodeBase = g->blasLayout[3 * k + 2];
         blasNodeCount = g->numBlasNodes;
+        blasDepth = g->blasDepth;
         as.blasReused = 1;
         as.msBlas = msSince(tStart);
     }
@@ -525,7 +561,10 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
         g->blasNodes.reserve(std::max<size_t>(cap, 1) * sizeof(BVH4QNode));
         g->blasTris.reserve(std::max<size_t>(blasRecs.size(), 1) * sizeof(TriRecord));
         const auto *recs = static_cast<const TriRecord *>(recsDev.ptr);
-        blasNodeCount = B.buildTriangles(recs, segFirst, segCount, 4, static_cast<BVH4QNode *>(g->blasNodes.ptr), nullptr);
+        BvhBuildTimes info;
+        blasNodeCount = B.buildTriangles(recs, segFirst, segCount, 4, static_cast<BVH4QNode *>(g->blasNodes.ptr), nullptr,
+                                         &info);
+        blasDepth = info.collapseLevels;
         gatherTriangles(recs, B.order(), static_cast<uint32_t>(blasRecs.size()),
                         static_cast<TriRecord *>(g->blasTris.ptr), nullptr);
         VLRA_HIP(hipStreamSynchronize(nullptr));
@@ -546,6 +585,7 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
                 }
             }
             hostBuildBvh4(boxes.data(), b.numTris, 4, nullptr, &hb);
+            blasDepth = std::max(blasDepth, hb.depth);
             b.nodeBase = static_cast<uint32_t>(blasQ.size());
             blasQ.insert(blasQ.end(), hb.nodes.begin(), hb.nodes.end());
             for (uint32_t t = 0; t < b.numTris; ++t) ordered[b.triBase + t] = blasRecs[b.triBase + hb.order[t]];
@@ -558,23 +598,67 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
     // the TLAS is rebuilt on every setup (scene.cpp:1325-1327)
     const auto tTlas = std::chrono::steady_clock::now();
     const uint32_t numTlasPrims = static_cast<uint32_t>(instBoxes.size());
-    if (onDevice) {
+    // refit instead of a rebuild when only instance transforms changed since the last build
+    // (the same instances over the same BLASes), unless the refitted tree's cost grew past
+    // twice the cost it had when it was built
+    std::vector<uint64_t> tlasKey(tlasInst.begin(), tlasInst.end());
+    tlasKey.insert(tlasKey.end(), instBlas.begin(), instBlas.end());
+    std::vector<Box> boxOfInst(insts.size());
+    for (size_t p = 0; p < tlasInst.size(); ++p) boxOfInst[tlasInst[p]] = instBoxes[p];
+    bool refitted = false;
+    if (ctx.tlasRefit && reuseBlas && numTlasPrims && g->tlasKey == tlasKey && !g->tlasHost.empty()) {
+        std::vector<BVH4QNode> nodes = g->tlasHost;
+        const double cost = refitTlas(nodes, boxOfInst);
+        if (cost <= 2.0 * g->tlasBuildCost) {
+            g->tlasNodes.upload(nodes.data(), nodes.size() * sizeof(BVH4QNode));
+            g->tlasHost = std::move(nodes);
+            tlasNodeCount = g->numTlasNodes;
+            tlasDepth = g->tlasDepth;
+            refitted = true;
+        }
+    }
+    if (refitted) {
+    } else if (onDevice) {
         if (numTlasPrims) {
             DeviceBuffer boxDev, remapDev;
             boxDev.upload(tlasBoxes.data(), tlasBoxes.size() * sizeof(float));
             remapDev.upload(tlasInst.data(), tlasInst.size() * sizeof(uint32_t));
             g->tlasNodes.reserve(GpuBvhBuilder::nodeCapacity(numTlasPrims) * sizeof(BVH4QNode));
+            BvhBuildTimes info;
             tlasNodeCount = ctx.bvhBuilder().build(static_cast<const float *>(boxDev.ptr), numTlasPrims, 1,
                                                    static_cast<const uint32_t *>(remapDev.ptr),
-                                                   static_cast<BVH4QNode *>(g->tlasNodes.ptr), nullptr);
+                                                   static_cast<BVH4QNode *>(g->tlasNodes.ptr), nullptr, &info);
+            tlasDepth = info.collapseLevels;
         }
     } else {
         HostBvh4 hb;
         hostBuildBvh4(tlasBoxes.data(), numTlasPrims, 1, tlasInst.data(), &hb);
+        tlasDepth = hb.depth;
         tlasNodeCount = static_cast<uint32_t>(hb.nodes.size());
         g->tlasNodes.upload(hb.nodes.data(), hb.nodes.size() * sizeof(BVH4QNode));
     }
+    if (!refitted) {
+        // the host copy a later refit starts from, and the cost it is judged against
+        g->tlasHost.assign(tlasNodeCount, BVH4QNode{});
+        if (tlasNodeCount)
+            VLRA_HIP(hipMemcpy(g->tlasHost.data(), g->tlasNodes.ptr, tlasNodeCount * sizeof(BVH4QNode),
+                               hipMemcpyDeviceToHost));
+        std::vector<BVH4QNode> probe = g->tlasHost;
+        g->tlasBuildCost = tlasNodeCount ? refitTlas(probe, boxOfInst) : 0.0;
+        g->tlasKey = std::move(tlasKey);
+        g->tlasDepth = tlasDepth;
+    }
+    as.tlasRefit = refitted ? 1 : 0;
     as.msTlas = msSince(tTlas);
+    // every entry a traversal pushes must fit its stack (travStackBound)
+    as.tlasDepth = tlasDepth;
+    as.blasDepth = blasDepth;
+    g->blasDepth = blasDepth;
+    if (travStackBound(tlasDepth, blasDepth) > kTravStackCapacity)
+        throw ApiError{3, "acceleration structure too deep for the traversal stack: TLAS depth " +
+                              std::to_string(tlasDepth) + " + BLAS depth " + std::to_string(blasDepth) +
+                              " needs " + std::to_string(travStackBound(tlasDepth, blasDepth)) + " of " +
+                              std::to_string(kTravStackCapacity) + " entries"};
     g->blasKey = std::move(blasKey);
     g->blasBuilder = ctx.accelBuilder;
     g->blasLayout.clear();
@@ -642,6 +726,12 @@ void buildSceneGPU(Context &ctx, const Scene &scene, bool spectral, SceneGPU *g)
     D.numInstances = static_cast<uint32_t>(insts.size());
     D.hasTlas = instBoxes.empty() ? 0 : 1;
     cmfBins(D.cmfX, D.cmfY, D.cmfZ, &D.integralCMF);
+    {
+        ImmSpectrum white;
+        white.cs = ColorSpace::Rec709_D65;
+        white.e[0] = white.e[1] = white.e[2] = 1.0f;
+        D.whitePoint = makeDevSpectrum(SpectrumType::LightSource, white, spectral);
+    }
     g->spectral = spectral;
     g->valid = true;
     g->numInstances = static_cast<uint32_t>(insts.size());

@@ -238,7 +238,9 @@ def quad_indices(num_quads: int) -> np.ndarray:
 # --------------------------------------------------------------------------------------------
 # Cornell box (RGB, Lambert)
 
-def cornell_box(seed: int = 20261202, light_size: float = 0.25) -> SceneDesc:
+def cornell_box(seed: int = 20261202, light_size: float = 0.25, angles=None) -> SceneDesc:
+    """The Cornell box; the two boxes turn about +Y by seeded angles in [0, 30 degrees], or by
+    `angles` (radians: short box, tall box) when given."""
     rng = np.random.default_rng(seed)
     mats = [
         MaterialDesc(MATTE, {"albedo": (REC709, 0.73, 0.73, 0.73)}),
@@ -262,6 +264,8 @@ def cornell_box(seed: int = 20261202, light_size: float = 0.25) -> SceneDesc:
     short_box = MeshDesc("ShortBox", box_mesh(0.6, 0.6, 0.6), [(quad_indices(6), 0)])
     tall_box = MeshDesc("TallBox", box_mesh(0.6, 1.2, 0.6), [(quad_indices(6), 0)])
     a0, a1 = rng.uniform(0.0, math.radians(30.0), size=2)
+    if angles is not None:
+        a0, a1 = float(angles[0]), float(angles[1])
     insts = [
         InstanceDesc("Room", identity4(), [0, 1]),
         InstanceDesc("ShortBoxInst", to_colmajor(translate((0.55, 0.0, 0.45)) @ rotate_y(-a0)), [2]),
