@@ -17,6 +17,12 @@ The JSON line carries
                  region with the traversal kernels' counting variant; `traffic` is the HBM
                  bytes per launch measured by rocprofv3 counters, read from the committed
                  summary under profiles/ when it exists
+  rmse           the learned radiance cache against plain path tracing on the Cornell box,
+                 spectral (4 hero wavelengths), 1920x1080, max depth 8: RMSE of both at equal
+                 samples per pixel (--rmse-spp, 1024) and over wall time against a plain
+                 reference of --rmse-ref-spp (65,536) samples accumulated in FP64 (two
+                 independent halves; the reference's own noise variance is removed), and the
+                 fraction of the plain render's time the cached one needs to reach its RMSE
   cpu_baseline   the oracle (C++/OpenMP restatement of the same integrator) on the host's
                  cores, whole 1920x1080 frames of the same scene and seeds, about 10 s of work
 """
@@ -182,26 +188,36 @@ def interp_log(points, x_key, y_key, x):
     return float(ys[-1])
 
 
-def cache_experiment(width, height, ref_spp, spp, warmup_frames, seed=20261202):
-    """Radiance cache vs plain path tracing on the Cornell box (RGB, max depth 8): RMSE against
-    an FP64-accumulated plain reference over time.  The reference is two independent halves
+def cache_experiment(width, height, ref_spp, spp, warmup_frames, seed=20261202, spectral=True, log=None,
+                     full_radiance=True, training_paths=None, threshold=None):
+    """Radiance cache vs plain path tracing on the Cornell box (max depth 8; spectral: 4 hero
+    wavelengths, the network's 4 outputs): RMSE against an FP64-accumulated plain reference,
+    at equal samples per pixel and over wall time.  The reference is two independent halves
     (their own seeds); their difference estimates the reference's noise variance, which is
     removed from every measured MSE.  The cached run's wall time includes its training-only
     warm-up frames."""
     desc = scenes.cornell_box(seed)
     n = width * height
+    say = log or (lambda *a: None)
 
     def context(cache, rng_seed=None):
-        ctx = api.Context(spectral=False, max_path_length=8)
+        ctx = api.Context(spectral=spectral, max_path_length=8)
         built = api.build_scene(ctx, desc)
         ctx.bind_output(width, height)
         if rng_seed is not None:
             ctx.write_rng(np.random.default_rng(rng_seed).integers(1, 2 ** 63, size=n, dtype=np.uint64))
         if cache:
             ctx.set_radiance_cache(True)
+            kw = {"fullRadiance": int(full_radiance)}
+            if training_paths:
+                kw["trainingPaths"] = training_paths
+            if threshold is not None:
+                kw["terminationThreshold"] = threshold
+            ctx.set_cache_settings(**kw)
         return ctx, built
 
     halves = []
+    t_ref = time.perf_counter()
     for h, rs in enumerate((101, 202)):
         ctx, built = context(False, rs)
         acc = np.zeros((height, width, 3))
@@ -213,6 +229,7 @@ def cache_experiment(width, height, ref_spp, spp, warmup_frames, seed=20261202):
                 ctx.render(built.camera, first_frame=(k == 0), limit=m)
             acc += ctx.read_output()[..., :3].astype(np.float64) * m
             done += m
+            say(f"reference half {h}: {done} of {ref_spp // 2} spp, {time.perf_counter() - t_ref:.0f} s")
         halves.append(acc / done)
         ctx.close()
     ref = 0.5 * (halves[0] + halves[1])
@@ -220,10 +237,14 @@ def cache_experiment(width, height, ref_spp, spp, warmup_frames, seed=20261202):
     checkpoints = sorted({c for c in (16, 32, 64, 128, 256, 512, 1024, 2048, 4096) if c <= spp} | {spp})
 
     ctx, built = context(False)
-    for k in range(8):   # schedule tuning and first-use allocations outside the timed region
+    k = 0
+    while k < 8 or not ctx.schedule()[2]:   # schedule tuning and first-use allocations outside the timed region
         ctx.render(built.camera, first_frame=(k == 0), limit=1 << 30)
+        k += 1
     plain = rmse_curve(ctx, built.camera, spp, checkpoints, ref, ref_var)
+    pst = ctx.stats()
     ctx.close()
+    say(f"plain: {spp} spp in {plain[-1]['seconds']:.2f} s, RMSE {plain[-1]['rmse']:.5f}")
 
     ctx, built = context(True)
     hip_sync()
@@ -260,8 +281,17 @@ def cache_experiment(width, height, ref_spp, spp, warmup_frames, seed=20261202):
                 reach = float(np.exp(np.log(q["seconds"]) + a * (np.log(p["seconds"]) - np.log(q["seconds"]))))
             break
     frames = max(1, st["numFrames"])
+    # equal samples per pixel: both curves share their checkpoints
+    equal_spp = [{"spp": p["spp"], "cached": c["rmse"], "plain": p["rmse"]} for p, c in zip(plain, cached)]
+    say(f"cached: {spp} spp in {cached[-1]['seconds']:.2f} s (warm-up included), RMSE {cached[-1]['rmse']:.5f}")
     return {
-        "workload": f"Cornell box (C1 scene), RGB, max depth 8, {width}x{height}",
+        "workload": f"Cornell box (C1 scene), {'spectral, 4 hero wavelengths' if spectral else 'RGB'}, "
+                    f"max depth 8, {width}x{height}",
+        "equal_spp": equal_spp,
+        "cache_settings": {"full_radiance": bool(full_radiance), "training_paths": training_paths,
+                           "threshold": threshold},
+        "rays_per_path": {"plain": (pst["closestRays"] + pst["shadowRays"]) / max(1, pst["paths"]),
+                          "cached": (st["closestRays"] + st["shadowRays"]) / max(1, st["paths"])},
         "reference": {"spp": ref_spp, "accumulation": "fp64 on the host over fp32 Kahan chunks of <= 1024 frames",
                       "noise_rmse_removed": float(np.sqrt(ref_var))},
         "plain": plain,
@@ -518,9 +548,20 @@ def main():
                     help="depths run as per-depth kernels before the persistent tail kernel (library default)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="also run one device-to-device copy of this many bytes (counter calibration)")
+    ap.add_argument("--rmse-ref-spp", type=int, default=65536,
+                    help="samples per pixel of the FP64-accumulated plain reference of the rmse field")
+    ap.add_argument("--rmse-spp", type=int, default=1024,
+                    help="samples per pixel of the plain and cached renders of the rmse field")
+    ap.add_argument("--no-rmse", action="store_true", help="leave the rmse field out (null)")
     ap.add_argument("--cache-experiment", action="store_true",
                     help="only run the radiance-cache RMSE experiment and print its JSON line")
     ap.add_argument("--cache-ref-spp", type=int, default=2048)
+    ap.add_argument("--cache-indirect", action="store_true",
+                    help="cache mode: the last vertex keeps its next-event estimate (default: the cache "
+                         "supplies all of its reflected radiance)")
+    ap.add_argument("--cache-training-paths", type=int, default=None)
+    ap.add_argument("--cache-threshold", type=float, default=None)
+    ap.add_argument("--cache-rgb", action="store_true")
     ap.add_argument("--cache-spp", type=int, default=256)
     ap.add_argument("--cache-warmup", type=int, default=64)
     ap.add_argument("--batch-experiment", action="store_true",
@@ -552,7 +593,10 @@ def main():
         print(json.dumps({"lvc_bpt": res}), flush=True)
         return
     if args.cache_experiment:
-        res = cache_experiment(args.width, args.height, args.cache_ref_spp, args.cache_spp, args.cache_warmup)
+        log = lambda m: print(f"cache: {m}", file=sys.stderr, flush=True)
+        res = cache_experiment(args.width, args.height, args.cache_ref_spp, args.cache_spp, args.cache_warmup,
+                               spectral=not args.cache_rgb, log=log, full_radiance=not args.cache_indirect,
+                               training_paths=args.cache_training_paths, threshold=args.cache_threshold)
         print(json.dumps({"radiance_cache": res}), flush=True)
         return
 
@@ -671,6 +715,24 @@ def main():
                         "node_program_steps": accel["numNodeProgramSteps"]},
         "rmse": None,
     }
+    if not args.no_rmse:
+        log = lambda m: print(f"rmse: {m}", file=sys.stderr, flush=True)
+        r = cache_experiment(args.width, args.height, args.rmse_ref_spp, args.rmse_spp, args.cache_warmup,
+                             seed=20261202, spectral=True, log=log)
+        last = r["equal_spp"][-1]
+        out["rmse"] = {
+            "workload": r["workload"],
+            "reference": r["reference"],
+            "equal_spp": {"spp": last["spp"], "plain": last["plain"], "cached": last["cached"]},
+            "plain_seconds": r["plain_rmse_at_spp"]["seconds"],
+            "cached_seconds_to_plain_rmse": r["cached_seconds_to_reach_it"],
+            "cached_time_fraction": r["cached_time_fraction"],
+            "equal_time": r["equal_time"],
+            "curves": {"plain": r["plain"], "cached": r["cached"], "equal_spp": r["equal_spp"]},
+            "cache_warmup": r["cache_warmup"],
+            "cache_ms_per_frame": r["cache_ms_per_frame"],
+            "rays_per_path": r["rays_per_path"],
+        }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(desc, args.width, args.height, args.depth, args.cpu_seconds)
         out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]

@@ -68,7 +68,9 @@ VLR_API VLRResult vlrContextGetOutputBufferSize(VLRContext context, uint32_t* wi
 /* vlr.h:85-87: a width x height hipArray of float4 (RGBA32F) that holds the converted output;
  * made on the first call, then written on the render stream after every rendered frame; owned
  * by the context (freed when the output is rebound at another size or the context is
- * destroyed) */
+ * destroyed).  A GPU without image support (hipDeviceAttributeImageSupport 0, as gfx950
+ * reports) has no arrays: the call returns VLRResult_InternalError and the output is read with
+ * vlrContextReadOutputBuffer. */
 VLR_API VLRResult vlrContextGetOutputBuffer(VLRContext context, hipArray_t* array);
 /* vlr.h:88-90: synchronous copy of width*height RGBA floats */
 VLR_API VLRResult vlrContextReadOutputBuffer(VLRContext context, float* data);

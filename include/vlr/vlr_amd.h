@@ -61,10 +61,15 @@ VLR_API VLRResult vlrContextResetRenderStats(VLRContext context);
  * frames are not synchronised.  Off by default. */
 VLR_API VLRResult vlrContextSetKernelTiming(VLRContext context, bool enable);
 
-/* ---- denoiser guides (path_tracing.cu:95-108, post_process.cu:70-90) ----
- * With the guides on, VLRRenderer_PathTracing sums each sample's first-hit albedo (the BSDF's
- * base colour times the D65 white point over the wavelength pdf; zero for a miss) and shading
- * normal per pixel, restarting with the accumulation.  Reading converts them as the
+/* ---- denoiser guides (path_tracing.cu:95-108, lvc_bpt.cu:528-540, aux_buffer_generator.cu:22-93,
+ * post_process.cu:70-90) ----
+ * With the guides on, every renderer sums each sample's first-hit albedo (the BSDF's base colour
+ * times the D65 white point over the wavelength pdf; zero for a miss) and shading normal per
+ * pixel, restarting with the accumulation: VLRRenderer_PathTracing and VLRRenderer_BPT at their
+ * eye paths' first hits, VLRRenderer_LightTracing with one primary ray per pixel before each
+ * frame's light paths, drawn from the pixel's generator as the reference's aux-buffer pass
+ * draws it (with the guides off that pass is skipped, and the pixel generators stay as they
+ * are; the reference runs it on every light-tracing frame).  Reading converts them as the
  * reference's copyBuffers does: linear Rec.709 albedo averaged over the samples, and the
  * normal sum rotated into the camera's frame with x negated and normalised (zero stays zero);
  * width*height RGBA float4 each, alpha 1 (either pointer may be NULL).  The guides are what
@@ -173,6 +178,11 @@ typedef struct VLRRadianceCacheSettings {
     float terminationThreshold;  /* spread threshold c, 0.01 */
     uint32_t warmupSteps;        /* training-only frames after a reset, 16 */
     uint32_t seed;               /* weight initialisation (Glorot uniform from PCG32) */
+    uint32_t fullRadiance;       /* 1 (default): the cache supplies all the radiance the path's
+                                  * last vertex reflects, its next-event estimate included, and
+                                  * samples are what a training path gathers after a vertex's
+                                  * emission; 0: the last vertex keeps its next-event estimate and
+                                  * the cache supplies what a continued path would add after it */
 } VLRRadianceCacheSettings;
 VLR_API VLRResult vlrContextSetRadianceCache(VLRContext context, bool enable);
 VLR_API VLRResult vlrContextGetRadianceCache(VLRContextConst context, bool* enabled);

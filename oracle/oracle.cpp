@@ -2304,6 +2304,11 @@ static void lightTracingPath(const Scene &s, const BPTSettings &bs, PCG32 &rng, 
     alpha *= Le1 * (ls.sp.calcCosTerm(rayDir) / dirPDF);
     const float initImportance = alpha.importance(wls.sel);
 
+    // The ray-generation program hands a copy of its generator to the hit program
+    // (rwPayload.rng = rng, light_tracing.cu:138) and keeps drawing its Russian-roulette numbers
+    // from its own copy (light_tracing.cu:165-169); the hit program's copy is the state stored
+    // at the end (light_tracing.cu:176).  `rng` plays the hit program's copy here.
+    PCG32 rrRng = rng;
     uint32_t pathLength = 0;
     while (true) {
         ++pathLength;
@@ -2354,9 +2359,9 @@ static void lightTracingPath(const Scene &s, const BPTSettings &bs, PCG32 &rng, 
         alpha *= fs * (std::fabs(cosFactor) / res.dirPDF);
         rayOrg = offsetRayOrigin(sp.position, cosFactor > 0.0f ? sp.geometricNormal : -sp.geometricNormal);
         rayDir = sp.frame.fromLocal(res.dirLocal);
-        // Russian roulette (light_tracing.cu:165-169)
+        // Russian roulette (light_tracing.cu:165-169), from the ray-generation program's generator
         float continueProb = std::fmin(alpha.importance(wls.sel) / initImportance, 1.0f);
-        if (rng.uniform() >= continueProb) break;
+        if (rrRng.uniform() >= continueProb) break;
         alpha /= continueProb;
     }
 }

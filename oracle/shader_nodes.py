@@ -101,8 +101,227 @@ def _channel_block(b, signed):
     return [p[(idx >> (3 * t)) & 7] for t in range(16)]
 
 
+# BC6H and BC7 (Direct3D 11 specification: "BC6H Format", "BC7 Format"): partitions of a 4x4
+# block into 2 or 3 subsets (texel t's subset: bit t of the 2-subset mask, digit t of the
+# 3-subset row), the anchor texel of each further subset (whose index has one bit fewer), and
+# the interpolation weights out of 64 for 2-, 3- and 4-bit indices
+BC_PART2 = [0xCCCC, 0x8888, 0xEEEE, 0xECC8, 0xC880, 0xFEEC, 0xFEC8, 0xEC80, 0xC800, 0xFFEC, 0xFE80, 0xE800, 0xFFE8,
+            0xFF00, 0xFFF0, 0xF000, 0xF710, 0x008E, 0x7100, 0x08CE, 0x008C, 0x7310, 0x3100, 0x8CCE, 0x088C, 0x3110,
+            0x6666, 0x366C, 0x17E8, 0x0FF0, 0x718E, 0x399C, 0xAAAA, 0xF0F0, 0x5A5A, 0x33CC, 0x3C3C, 0x55AA, 0x9696,
+            0xA55A, 0x73CE, 0x13C8, 0x324C, 0x3BDC, 0x6996, 0xC33C, 0x9966, 0x0660, 0x0272, 0x04E4, 0x4E40, 0x2720,
+            0xC936, 0x936C, 0x39C6, 0x639C, 0x9336, 0x9CC6, 0x817E, 0xE718, 0xCCF0, 0x0FCC, 0x7744, 0xEE22]
+BC_PART3 = [
+    "0011001102212222", "0001001122112221", "0000200122112211", "0222002200110111",
+    "0000000011221122", "0011001100220022", "0022002211111111", "0011001122112211",
+    "0000000011112222", "0000111111112222", "0000111122222222", "0012001200120012",
+    "0112011201120112", "0122012201220122", "0011011211221222", "0011200122002220",
+    "0001001101121122", "0111001120012200", "0000112211221122", "0022002200221111",
+    "0111011102220222", "0001000122212221", "0000001101220122", "0000110022102210",
+    "0122012200110000", "0012001211222222", "0110122112210110", "0000011012211221",
+    "0022110211020022", "0110011020022222", "0011012201220011", "0000200022112221",
+    "0000000211221222", "0222002200120011", "0011001200220222", "0120012001200120",
+    "0000111122220000", "0120120120120120", "0120201212010120", "0011220011220011",
+    "0011112222000011", "0101010122222222", "0000000021212121", "0022112200221122",
+    "0022001100220011", "0220122102201221", "0101222222220101", "0000212121212121",
+    "0101010101012222", "0222011102220111", "0002111200021112", "0000211221122112",
+    "0222011101110222", "0002111211120002", "0110011001102222", "0000000021122112",
+    "0110011022222222", "0022001100110022", "0022112211220022", "0000000000002112",
+    "0002000100020001", "0222122202221222", "0101222222222222", "0111201122012220",
+]
+BC_ANCHOR2 = [15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 15, 2, 8, 2, 2, 8, 8, 15, 2, 8, 2, 2, 8, 8,
+              2, 2, 15, 15, 6, 8, 2, 8, 15, 15, 2, 8, 2, 2, 2, 15, 15, 6, 6, 2, 6, 8, 15, 15, 2, 2, 15, 15, 15, 15, 15,
+              2, 2, 15]
+BC_ANCHOR3A = [3, 3, 15, 15, 8, 3, 15, 15, 8, 8, 6, 6, 6, 5, 3, 3, 3, 3, 8, 15, 3, 3, 6, 10, 5, 8, 8, 6, 8, 5, 15, 15,
+               8, 15, 3, 5, 6, 10, 8, 15, 15, 3, 15, 5, 15, 15, 15, 15, 3, 15, 5, 5, 5, 8, 5, 10, 5, 10, 8, 13, 15, 12,
+               3, 3]
+BC_ANCHOR3B = [15, 8, 8, 3, 15, 15, 3, 8, 15, 15, 15, 15, 15, 15, 15, 8, 15, 8, 15, 3, 15, 8, 15, 8, 3, 15, 6, 10, 15,
+               15, 10, 8, 15, 3, 15, 10, 10, 8, 9, 10, 6, 15, 8, 15, 3, 6, 6, 8, 15, 3, 15, 15, 15, 15, 15, 15, 15, 15,
+               15, 15, 3, 15, 15, 8]
+BC_WEIGHTS = {2: [0, 21, 43, 64], 3: [0, 9, 18, 27, 37, 46, 55, 64],
+              4: [0, 4, 9, 13, 17, 21, 26, 30, 34, 38, 43, 47, 51, 55, 60, 64]}
+
+
+class _BitReader:
+    """The 128 bits of a block, read from bit 0 up."""
+
+    def __init__(self, b):
+        self.v, self.p = int.from_bytes(bytes(b), "little"), 0
+
+    def get(self, n):
+        r = (self.v >> self.p) & ((1 << n) - 1)
+        self.p += n
+        return r
+
+
+# BC7 modes: subsets, partition bits, rotation bits, index-selection bits, colour bits, alpha
+# bits, per-endpoint p-bits, per-subset shared p-bits, index bits, secondary index bits
+BC7_MODES = [(3, 4, 0, 0, 4, 0, 1, 0, 3, 0), (2, 6, 0, 0, 6, 0, 0, 1, 3, 0), (3, 6, 0, 0, 5, 0, 0, 0, 2, 0),
+             (2, 6, 0, 0, 7, 0, 1, 0, 2, 0), (1, 0, 2, 1, 5, 6, 0, 0, 2, 3), (1, 0, 2, 0, 7, 8, 0, 0, 2, 2),
+             (1, 0, 0, 0, 7, 7, 1, 0, 4, 0), (2, 6, 0, 0, 5, 5, 1, 0, 2, 0)]
+
+
+def _bc7_block(b):
+    """16 RGBA8 texels; the reserved mode (no set bit in the first byte) gives zeros."""
+    br = _BitReader(b)
+    mode = 0
+    while mode < 8 and br.get(1) == 0:
+        mode += 1
+    if mode == 8:
+        return [[0, 0, 0, 0]] * 16
+    ns, pb, rb, isb, cb, ab, epb, spb, ib, ib2 = BC7_MODES[mode]
+    part, rot, isel = br.get(pb), br.get(rb), br.get(isb)
+    ep = [[[0] * 4 for _ in range(2)] for _ in range(ns)]
+    for c in range(3):
+        for s in range(ns):
+            for e in range(2):
+                ep[s][e][c] = br.get(cb)
+    for s in range(ns if ab else 0):
+        for e in range(2):
+            ep[s][e][3] = br.get(ab)
+    cprec, aprec = cb, ab
+    if epb or spb:
+        pbits = [[br.get(1), br.get(1)] for _ in range(ns)] if epb else [[p, p] for p in (br.get(1) for _ in range(ns))]
+        for s in range(ns):
+            for e in range(2):
+                for c in range(4):
+                    ep[s][e][c] = (ep[s][e][c] << 1) | pbits[s][e]
+        cprec, aprec = cb + 1, (ab + 1 if ab else 0)
+
+    def expand(v, prec):              # to 8 bits, the top bits repeated below
+        v <<= 8 - prec
+        return v | (v >> prec)
+    for s in range(ns):
+        for e in range(2):
+            for c in range(3):
+                ep[s][e][c] = expand(ep[s][e][c], cprec)
+            ep[s][e][3] = expand(ep[s][e][3], aprec) if ab else 255
+    anchors = [0] if ns == 1 else ([0, BC_ANCHOR2[part]] if ns == 2 else [0, BC_ANCHOR3A[part], BC_ANCHOR3B[part]])
+    idx1 = [br.get(ib - 1 if t in anchors else ib) for t in range(16)]
+    idx2 = [br.get(ib2 - 1 if t == 0 else ib2) for t in range(16)] if ib2 else idx1
+    out = []
+    for t in range(16):
+        s = 0 if ns == 1 else ((BC_PART2[part] >> t) & 1 if ns == 2 else int(BC_PART3[part][t]))
+        if ib2:
+            ci, cbits, ai, abits = (idx2[t], ib2, idx1[t], ib) if isel else (idx1[t], ib, idx2[t], ib2)
+        else:
+            ci, cbits, ai, abits = idx1[t], ib, idx1[t], ib
+        px = []
+        for c in range(4):
+            w = BC_WEIGHTS[cbits][ci] if c < 3 else BC_WEIGHTS[abits][ai]
+            px.append(((64 - w) * ep[s][0][c] + w * ep[s][1][c] + 32) >> 6)
+        if rot:
+            px[rot - 1], px[3] = px[3], px[rot - 1]
+        out.append(px)
+    return out
+
+
+# BC6H modes: mode value and its field width, transformed (delta) endpoints, endpoint bits,
+# delta bits (R, G, B), regions, and where each endpoint bit lies after the mode field (in
+# stream order; x[a:b] puts bit b first, then on towards a; endpoints 0-3 are region 0's two,
+# then region 1's two)
+BC6H_MODES = [
+    (0b00, 2, True, 10, (5, 5, 5), 2, "g2[4] b2[4] b3[4] r0[9:0] g0[9:0] b0[9:0] r1[4:0] g3[4] g2[3:0] g1[4:0] b3[0] g3[3:0] b1[4:0] b3[1] b2[3:0] r2[4:0] b3[2] r3[4:0] b3[3]"),
+    (0b01, 2, True, 7, (6, 6, 6), 2, "g2[5] g3[4] g3[5] r0[6:0] b3[0] b3[1] b2[4] g0[6:0] b2[5] b3[2] g2[4] b0[6:0] b3[3] b3[5] b3[4] r1[5:0] g2[3:0] g1[5:0] g3[3:0] b1[5:0] b2[3:0] r2[5:0] r3[5:0]"),
+    (0b00010, 5, True, 11, (5, 4, 4), 2, "r0[9:0] g0[9:0] b0[9:0] r1[4:0] r0[10] g2[3:0] g1[3:0] g0[10] b3[0] g3[3:0] b1[3:0] b0[10] b3[1] b2[3:0] r2[4:0] b3[2] r3[4:0] b3[3]"),
+    (0b00110, 5, True, 11, (4, 5, 4), 2, "r0[9:0] g0[9:0] b0[9:0] r1[3:0] r0[10] g3[4] g2[3:0] g1[4:0] g0[10] g3[3:0] b1[3:0] b0[10] b3[1] b2[3:0] r2[3:0] b3[0] b3[2] r3[3:0] g2[4] b3[3]"),
+    (0b01010, 5, True, 11, (4, 4, 5), 2, "r0[9:0] g0[9:0] b0[9:0] r1[3:0] r0[10] b2[4] g2[3:0] g1[3:0] g0[10] b3[0] g3[3:0] b1[4:0] b0[10] b2[3:0] r2[3:0] b3[1] b3[2] r3[3:0] b3[4] b3[3]"),
+    (0b01110, 5, True, 9, (5, 5, 5), 2, "r0[8:0] b2[4] g0[8:0] g2[4] b0[8:0] b3[4] r1[4:0] g3[4] g2[3:0] g1[4:0] b3[0] g3[3:0] b1[4:0] b3[1] b2[3:0] r2[4:0] b3[2] r3[4:0] b3[3]"),
+    (0b10010, 5, True, 8, (6, 5, 5), 2, "r0[7:0] g3[4] b2[4] g0[7:0] b3[2] g2[4] b0[7:0] b3[3] b3[4] r1[5:0] g2[3:0] g1[4:0] b3[0] g3[3:0] b1[4:0] b3[1] b2[3:0] r2[5:0] r3[5:0]"),
+    (0b10110, 5, True, 8, (5, 6, 5), 2, "r0[7:0] b3[0] b2[4] g0[7:0] g2[5] g2[4] b0[7:0] g3[5] b3[4] r1[4:0] g3[4] g2[3:0] g1[5:0] g3[3:0] b1[4:0] b3[1] b2[3:0] r2[4:0] b3[2] r3[4:0] b3[3]"),
+    (0b11010, 5, True, 8, (5, 5, 6), 2, "r0[7:0] b3[1] b2[4] g0[7:0] b2[5] g2[4] b0[7:0] b3[5] b3[4] r1[4:0] g3[4] g2[3:0] g1[4:0] b3[0] g3[3:0] b1[5:0] b2[3:0] r2[4:0] b3[2] r3[4:0] b3[3]"),
+    (0b11110, 5, False, 6, (6, 6, 6), 2, "r0[5:0] g3[4] b3[0] b3[1] b2[4] g0[5:0] g2[5] b2[5] b3[2] g2[4] b0[5:0] g3[5] b3[3] b3[5] b3[4] r1[5:0] g2[3:0] g1[5:0] g3[3:0] b1[5:0] b2[3:0] r2[5:0] r3[5:0]"),
+    (0b00011, 5, False, 10, (10, 10, 10), 1, "r0[9:0] g0[9:0] b0[9:0] r1[9:0] g1[9:0] b1[9:0]"),
+    (0b00111, 5, True, 11, (9, 9, 9), 1, "r0[9:0] g0[9:0] b0[9:0] r1[8:0] r0[10] g1[8:0] g0[10] b1[8:0] b0[10]"),
+    (0b01011, 5, True, 12, (8, 8, 8), 1, "r0[9:0] g0[9:0] b0[9:0] r1[7:0] r0[10:11] g1[7:0] g0[10:11] b1[7:0] b0[10:11]"),
+    (0b01111, 5, True, 16, (4, 4, 4), 1, "r0[9:0] g0[9:0] b0[9:0] r1[3:0] r0[10:15] g1[3:0] g0[10:15] b1[3:0] b0[10:15]"),
+]
+
+
+def _bc6h_layout(text):
+    out = []
+    for seg in text.split():
+        rng = seg[3:-1]
+        a, b = (int(x) for x in rng.split(":")) if ":" in rng else (int(rng), int(rng))
+        step = 1 if a >= b else -1
+        for k in range(b, a + step, step):
+            out.append(("rgb".index(seg[0]), int(seg[1]), k))
+    return out
+
+
+BC6H_LAYOUTS = [_bc6h_layout(m[6]) for m in BC6H_MODES]
+
+
+def _sext(v, n):
+    v &= (1 << n) - 1
+    return v - (1 << n) if v & (1 << (n - 1)) else v
+
+
+def _bc6h_block(b, signed):
+    """16 RGB texels as float32 (half-float values); reserved modes give zeros."""
+    br = _BitReader(b)
+    m = br.get(2)
+    if m & 2:
+        m |= br.get(3) << 2
+    width = 5 if m & 2 else 2
+    mode = next((i for i, md in enumerate(BC6H_MODES) if md[0] == m and md[1] == width), None)
+    if mode is None:
+        return [[f32(0)] * 3] * 16
+    _, _, transformed, eb, db, regions, _ = BC6H_MODES[mode]
+    e = [[0, 0, 0] for _ in range(4)]
+    for c, i, k in BC6H_LAYOUTS[mode]:
+        e[i][c] |= br.get(1) << k
+    part = br.get(5) if regions == 2 else 0
+    n_ep = 2 * regions
+    for c in range(3):
+        if signed:
+            e[0][c] = _sext(e[0][c], eb)
+        for i in range(1, n_ep):
+            if transformed:     # endpoint = base + sign-extended delta, wrapped to eb bits
+                v = (e[0][c] + _sext(e[i][c], db[c])) & ((1 << eb) - 1)
+                e[i][c] = _sext(v, eb) if signed else v
+            elif signed:
+                e[i][c] = _sext(e[i][c], eb)
+
+    def unquantize(c):
+        if not signed:
+            if eb >= 15:
+                return c
+            if c == 0:
+                return 0
+            if c == (1 << eb) - 1:
+                return 0xFFFF
+            return ((c << 16) + 0x8000) >> eb
+        if eb >= 16:
+            return c
+        neg = c < 0
+        c = -c if neg else c
+        q = 0 if c == 0 else (0x7FFF if c >= (1 << (eb - 1)) - 1 else ((c << 15) + 0x4000) >> (eb - 1))
+        return -q if neg else q
+
+    ue = [[unquantize(e[i][c]) for c in range(3)] for i in range(n_ep)]
+    anchors = [0, BC_ANCHOR2[part]] if regions == 2 else [0]
+    ib = 3 if regions == 2 else 4
+    idx = [br.get(ib - 1 if t in anchors else ib) for t in range(16)]
+    out = []
+    for t in range(16):
+        s = (BC_PART2[part] >> t) & 1 if regions == 2 else 0
+        w = BC_WEIGHTS[ib][idx[t]]
+        px = []
+        for c in range(3):
+            q = (ue[2 * s][c] * (64 - w) + ue[2 * s + 1][c] * w + 32) >> 6
+            if signed:   # to half-float bits: magnitude scaled by 31/32, sign in bit 15
+                h = (0x8000 | (((-q) * 31) >> 5)) if q < 0 else (q * 31) >> 5
+            else:
+                h = (q * 31) >> 6
+            px.append(f32(np.array([h], np.uint16).view(np.float16)[0]))
+        out.append(px)
+    return out
+
+
 def decode_bc(fmt, data, width, height):
-    """Mip level 0 of a BC1-BC5 image as (height, width, 4) float32 texels."""
+    """Mip level 0 of a block-compressed image as (height, width, 4) float32 texels: BC1-BC5 as
+    floats in [0, 1] ([-1, 1] signed), BC7 as 8-bit unorm values / 255, BC6H as the half-float
+    values with alpha 1."""
     data = np.ascontiguousarray(data, np.uint8).reshape(-1)
     bb = 8 if fmt in ("BC1", "BC4", "BC4_Signed") else 16
     bw, bh = (width + 3) // 4, (height + 3) // 4
@@ -124,6 +343,10 @@ def decode_bc(fmt, data, width, height):
             elif fmt in ("BC4", "BC4_Signed"):
                 r = _channel_block(b, fmt == "BC4_Signed")
                 t = [[r[k], f32(0), f32(0), f32(1)] for k in range(16)]
+            elif fmt == "BC7":
+                t = [[f32(v) / f32(255) for v in px] for px in _bc7_block(b)]
+            elif fmt in ("BC6H", "BC6H_Signed"):
+                t = [px + [f32(1)] for px in _bc6h_block(b, fmt == "BC6H_Signed")]
             else:
                 r = _channel_block(b[:8], fmt == "BC5_Signed")
                 g = _channel_block(b[8:], fmt == "BC5_Signed")
@@ -141,13 +364,13 @@ class Texture:
         self.width, self.height = img.width, img.height
         fmt = img.format
         if fmt.startswith("BC"):
-            # decoded on the host; the texture unit's sRGB read of BC1-BC4 degammas the colour
+            # decoded on the host; the texture unit's sRGB read of BC1-BC4 and BC7 degammas the colour
             # channels per texel (image.cpp:336-346)
             cs = COLOR_SPACES[img.color_space]
             self.uvs, self.kind, self.gray = False, "rgba", False
             self.spectrum_type = SPEC_TYPES[img.spectrum_type]
             t = decode_bc(fmt, img.data, img.width, img.height)
-            degam = cs == 0 and fmt in ("BC1", "BC2", "BC3", "BC4")
+            degam = cs == 0 and fmt in ("BC1", "BC2", "BC3", "BC4", "BC7")
             if degam:
                 for y in range(img.height):
                     for x in range(img.width):

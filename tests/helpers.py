@@ -46,7 +46,3 @@ def gpu_bsdf(torch, ctx, material, spectral, queries):
     torch.cuda.synchronize()
     return r.cpu().numpy().view(api.BSDF_RESULT_DTYPE)
 
-
-def brute_force_ties(oracle_scene, rays, hits):
-    """Flags rays whose closest t is shared by another primitive (exact ties)."""
-    return np.zeros(len(rays), bool)

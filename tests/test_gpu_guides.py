@@ -76,3 +76,37 @@ def test_guides_leave_the_image_unchanged_and_restart(torch_gpu):
             ctx.render(bs.camera, f == 0, 10)
         alb_fresh, nrm_fresh = ctx.read_denoiser_aux_buffers()
     assert np.array_equal(alb_restart, alb_fresh) and np.array_equal(nrm_restart, nrm_fresh)
+
+
+def render_guides(desc, W, H, frames, spectral, renderer):
+    with api.Context(spectral=spectral, max_path_length=8) as ctx:
+        ctx.set_renderer(renderer)
+        bs = api.build_scene(ctx, desc)
+        ctx.bind_output(W, H)
+        ctx.set_denoiser_aux_buffers(True)
+        for f in range(frames):
+            ctx.render(bs.camera, f == 0, frames)
+        alb, nrm = ctx.read_denoiser_aux_buffers()
+        return alb, nrm, ctx.read_rng()
+
+
+@pytest.mark.parametrize("spectral,renderer", [(False, api.RENDERER_LIGHT_TRACING), (True, api.RENDERER_LIGHT_TRACING),
+                                               (False, api.RENDERER_BPT)])
+def test_light_tracing_and_bpt_guides_equal_the_path_tracers(torch_gpu, spectral, renderer):
+    """Light tracing's guide pass (aux_buffer_generator.cu) draws each pixel's primary ray as the
+    path tracer does (jitter, wavelengths, lens point), and BPT's eye paths draw jitter then lens
+    point (the pinhole lens point does not move the ray; RGB albedos do not depend on the
+    wavelengths): after one frame their guides equal the path tracer's, whose are pinned by the
+    oracle.  Over several frames they stay finite, with unit or zero normals."""
+    desc = scenes.cornell_box()
+    W, H = 48, 32
+    pa, pn, _ = render_guides(desc, W, H, 1, spectral, api.RENDERER_PATH_TRACING)
+    la, ln, lrng = render_guides(desc, W, H, 1, spectral, renderer)
+    np.testing.assert_allclose(la, pa, rtol=1e-6, atol=1e-7)
+    np.testing.assert_allclose(ln, pn, rtol=1e-6, atol=1e-7)
+    if renderer == api.RENDERER_LIGHT_TRACING:
+        assert not np.array_equal(lrng, ob.seed_states(W, H))     # the guide pass drew from the pixels
+    a4, n4, _ = render_guides(desc, W, H, 4, spectral, renderer)
+    assert np.all(np.isfinite(a4)) and np.all(np.isfinite(n4))
+    norms = np.linalg.norm(n4[..., :3], axis=-1)
+    assert np.all((np.abs(norms - 1.0) < 1e-4) | (norms == 0.0))

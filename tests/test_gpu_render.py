@@ -176,14 +176,17 @@ def test_autotuned_schedule_matches_fixed_schedule(torch_gpu):
     assert np.array_equal(imgs[-1].view(np.uint32), tuned_img.view(np.uint32))
 
 
-def render_frames(desc, W, H, frames, depth, spectral, renderer):
+def render_frames(desc, W, H, frames, depth, spectral, renderer, states=False):
     with api.Context(spectral=spectral, max_path_length=depth) as ctx:
         ctx.set_renderer(renderer)
         bs = api.build_scene(ctx, desc)
         ctx.bind_output(W, H)
         for f in range(frames):
             ctx.render(bs.camera, f == 0, frames)
-        return ctx.read_output()
+        if not states:
+            return ctx.read_output()
+        light = renderer != api.RENDERER_PATH_TRACING
+        return ctx.read_output(), ctx.read_rng(), ctx.read_light_rng() if light else None
 
 
 @pytest.mark.parametrize("scene,spectral,renderer", [("cornell", False, api.RENDERER_PATH_TRACING),
@@ -193,12 +196,15 @@ def render_frames(desc, W, H, frames, depth, spectral, renderer):
 def test_checkpoint_and_resume_in_a_new_context(torch_gpu, scene, spectral, renderer):
     """RNG states (eye paths, and light paths for BPT and light tracing), BPT's per-frame
     wavelength generator, the Kahan accumulation and the frame count written to a fresh context
-    resume the render: frames 3-4 there give the image of four frames rendered in one go, bit
-    for bit."""
+    resume the render: frames 3-4 there give the image of four frames rendered in one go.  Path
+    tracing adds each pixel's sample once, so the images are equal bit for bit.  Light tracing
+    and BPT splat lens connections with float atomic adds (atomicAddToBuffer,
+    light_tracing.cu:32-46, lvc_bpt.cu), whose order differs from run to run: there the generator
+    states are equal bit for bit and the images to float rounding of the splat sums."""
     desc = scenes.cornell_box() if scene == "cornell" else scenes.icosphere_grid(grid=4, env_size=(64, 32))
     W, H, depth = 48, 32, 8
     light = renderer != api.RENDERER_PATH_TRACING
-    full = render_frames(desc, W, H, 4, depth, spectral, renderer)
+    full, full_rng, full_lrng = render_frames(desc, W, H, 4, depth, spectral, renderer, states=True)
     with api.Context(spectral=spectral, max_path_length=depth) as ctx:
         ctx.set_renderer(renderer)
         bs = api.build_scene(ctx, desc)
@@ -222,7 +228,18 @@ def test_checkpoint_and_resume_in_a_new_context(torch_gpu, scene, spectral, rend
         for _ in range(2):
             ctx.render(bs.camera, False, 4)
         resumed = ctx.read_output()
-    assert np.array_equal(resumed.view(np.uint32), full.view(np.uint32))
+        resumed_rng = ctx.read_rng()
+        resumed_lrng = ctx.read_light_rng() if light else None
+    assert np.array_equal(resumed_rng, full_rng)
+    if light:
+        assert np.array_equal(resumed_lrng, full_lrng)
+    if renderer == api.RENDERER_PATH_TRACING:
+        assert np.array_equal(resumed.view(np.uint32), full.view(np.uint32))
+    else:
+        scale = float(np.abs(full).max())
+        err = np.abs(resumed.astype(np.float64) - full)
+        print("largest difference from the one-go render:", float(err.max()), "of scale", scale)
+        assert np.all(err <= 1e-5 * np.abs(full) + 1e-6 * scale)
 
 
 def test_checkpoint_of_the_other_mode_is_refused(torch_gpu):
@@ -247,3 +264,24 @@ def test_checkpoint_of_the_other_mode_is_refused(torch_gpu):
         # no first frame has switched the fresh context to spectral yet: nothing to read
         out = np.zeros((H, W, 16, 2), np.float32)
         assert api.try_call("vlrContextReadAccumulation", ctx.h, out.ctypes.data, out.size) == 3
+
+
+def test_non_finite_samples_are_dropped_and_counted(torch_gpu):
+    """A light whose emittance times scale overflows float (17 pi x 1e37 > FLT_MAX) makes every
+    sample that reaches it infinite or NaN.  Such a sample is not added to the accumulation
+    (path_tracing.cu:120-123): the image stays finite, and the GPU counts as many dropped
+    samples as the oracle path tracer, up to the pixels whose paths took other branches."""
+    import dataclasses
+    desc = scenes.cornell_box()
+    desc.materials[3] = dataclasses.replace(desc.materials[3], floats={"scale": 1e37})
+    W = H = 32
+    frames = 4
+    imgs, st, rng_gpu = render_gpu(desc, W, H, frames, 8, False)
+    ref, ost, rng_ref, _ = ob.Scene(desc).render(W, H, frames, max_path_length=8, rng_state=ob.seed_states(W, H))
+    mismatched = int(np.sum(rng_gpu != rng_ref))
+    print("dropped: gpu", st["droppedNonFinite"], "oracle", ost["dropped_non_finite"], "pixels with other paths",
+          mismatched)
+    assert ost["dropped_non_finite"] > W * H // 4          # most paths reach the light
+    assert abs(st["droppedNonFinite"] - ost["dropped_non_finite"]) <= mismatched * frames
+    assert np.all(np.isfinite(imgs[-1]))
+    assert np.all(np.isfinite(ref))

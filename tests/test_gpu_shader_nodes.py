@@ -5,9 +5,12 @@ that the oracle path tracer renders (a two-texel albedo texture against a floor 
 constant normal maps against tilted vertex normals, a radial tangent against vertex tangents).
 
 Tolerance: float32 arithmetic contracted differently on the device (fused multiply-adds) moves
-results by a few ulps; 99.5 % of the values must agree within 2e-4 relative, all of them within
-2e-2 of the case's scale (a value next to a nearest-filter texel edge, a Meng grid cell edge or
-an 8-bit quantisation step may land on the other side)."""
+results by a few ulps, so every value must agree within 2e-4 relative plus 2e-5 of the case's
+scale.  A point whose texture coordinate lands within float rounding of a nearest-filter texel
+edge (or whose wavelength offset lands on a wavelength-bin edge) may fall on the other side on
+the device; such a value is accepted only if it matches, at the same tolerance, the oracle
+evaluated at the point with its texture coordinates or wavelength offset moved by 2e-6 (a few
+float32 ulps of the inputs).  No other value is exempt."""
 import math
 
 import numpy as np
@@ -75,6 +78,10 @@ IMAGES = {
     "bc3_srgb_refl": image("BC3", 8, 4, "Reflectance", REC709_SRGB, 12),
     "bc4_srgb_na": image("BC4", 8, 8, "NA", REC709_SRGB, 13),
     "bc5s_na": image("BC5_Signed", 4, 8, "NA", REC709, 14),
+    # BC7 (sRGB read like BC1-BC4) and BC6H half floats, unsigned and signed
+    "bc7_srgb_refl": image("BC7", 6, 7, "Reflectance", REC709_SRGB, 15),
+    "bc6h_light": image("BC6H", 5, 4, "Light Source", REC709, 16),
+    "bc6hs_na": image("BC6H_Signed", 4, 4, "NA", REC709, 17),
 }
 WRAPS = ["Repeat", "Clamp to Edge", "Mirror", "Clamp to Border"]
 
@@ -196,12 +203,41 @@ def gpu_eval(torch, ctx, plug_desc, spectral, pts, cache, keep):
     return out.cpu().numpy()
 
 
-def check(name, got, want):
+EDGE_STEP = 2e-6
+
+
+def nudged_points(row):
+    """The point with its texture coordinates and wavelength offset moved by EDGE_STEP."""
+    out = []
+    for du in (-EDGE_STEP, 0.0, EDGE_STEP):
+        for dv in (-EDGE_STEP, 0.0, EDGE_STEP):
+            if du or dv:
+                r = row.copy()
+                r[15] += du
+                r[16] += dv
+                out.append(r)
+    for dw in (-EDGE_STEP, EDGE_STEP):
+        r = row.copy()
+        r[17] = np.float32(min(max(r[17] + dw, 0.0), np.nextafter(np.float32(1), np.float32(0))))
+        out.append(r)
+    return out
+
+
+def check(name, got, want, pts=None, oracle_eval=None):
     scale = max(1.0, float(np.abs(want).max()))
-    tight = np.abs(got - want) <= 2e-4 * np.maximum(np.abs(got), np.abs(want)) + 2e-5 * scale
-    frac = tight.all(axis=1).mean()
-    worst = float(np.abs(got - want).max()) / scale
-    assert frac >= 0.995 and worst <= 2e-2, (name, frac, worst, np.argwhere(~tight)[:4])
+
+    def close(a, b):
+        return np.all(np.abs(a - b) <= 2e-4 * np.maximum(np.abs(a), np.abs(b)) + 2e-5 * scale, axis=-1)
+
+    tight = close(got, want)
+    edge = []
+    for i in np.flatnonzero(~tight):
+        if oracle_eval is not None and any(close(got[i], oracle_eval(r)) for r in nudged_points(pts[i])):
+            edge.append(int(i))
+            continue
+        raise AssertionError((name, int(i), got[i].tolist(), want[i].tolist()))
+    if edge:
+        print(f"  {name}: {len(edge)} of {len(got)} points next to an edge, matched by the nudged oracle")
 
 
 @pytest.mark.parametrize("spectral", [False, True])
@@ -216,7 +252,7 @@ def test_node_plugs_match_oracle(torch_gpu, contexts, spectral):
         print("case", name, flush=True)
         got = gpu_eval(torch_gpu, ctx, plug, spectral, pts, cache, keep)
         want = np.stack([ev.evaluate(plug, p) for p in points])
-        check(name, got, want)
+        check(name, got, want, pts, lambda r, plug=plug: ev.evaluate(plug, osn.Point(r)))
     print(len(cases), "plugs checked")
 
 

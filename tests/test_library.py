@@ -262,11 +262,13 @@ def test_knobs(ctx):
 
 
 def test_block_compressed_formats(ctx):
-    """BC1-BC5 (and signed BC4/BC5) are accepted; BC6H and BC7 are refused with InvalidArgument."""
+    """Every block-compressed format of the reference (BC1-BC7, signed BC4/BC5/BC6H) is accepted;
+    a format name that is not one is refused with InvalidArgument."""
     data = np.zeros(16, np.uint8)
     ptrs = (C.c_void_p * 1)(data.ctypes.data)
     sizes = (C.c_size_t * 1)(data.nbytes)
-    for fmt, ok in (("BC1", True), ("BC3", True), ("BC4_Signed", True), ("BC5", True), ("BC6H", False), ("BC7", False)):
+    for fmt, ok in (("BC1", True), ("BC3", True), ("BC4_Signed", True), ("BC5", True), ("BC6H", True),
+                    ("BC6H_Signed", True), ("BC7", True), ("RGBA8x4", False), ("BC8", False)):
         h = api.H()
         rc = code_of("vlrBlockCompressedImage2DCreate", ctx.h, C.cast(ptrs, C.c_void_p), C.cast(sizes, C.c_void_p), 1,
                      4, 4, fmt.encode(), b"Reflectance", b"Rec709(D65)", C.byref(h))

@@ -303,3 +303,102 @@ def test_bc2_and_bc3_alpha():
     d = np.concatenate([bc4_block(0, 255, [7] * 16), bc1_block(0x0000, 0xFFFF, [3] * 16)])
     t = osn.decode_bc("BC3", d, 4, 4)
     np.testing.assert_allclose(t[0, 0], [2 / 3, 2 / 3, 2 / 3, 1], rtol=1e-6)
+
+
+# ---- BC6H and BC7 ----
+def _pillow_bcn(data, n, mode, fmt=None):
+    from PIL import Image
+    args = (n,) if fmt is None else (n, fmt)
+    return np.asarray(Image.frombytes(mode, (4, 4), bytes(data), "bcn", args))
+
+
+def _random_blocks(rng, count, mode_bits, width):
+    for _ in range(count):
+        v = int.from_bytes(rng.integers(0, 256, 16, dtype=np.uint8).tobytes(), "little")
+        yield ((v & ~((1 << width) - 1)) | mode_bits).to_bytes(16, "little")
+
+
+def test_bc7_matches_pillows_decoder():
+    """Pillow's BCn decoder (an implementation of the same Direct3D specification) decodes 300
+    random blocks of each of BC7's eight modes (every partition, rotation, index selection and
+    p-bit) to the same RGBA8 texels as the oracle.  The reserved mode (no bit set in the first
+    byte) decodes to zeros, as the specification says (Pillow returns opaque black)."""
+    rng = np.random.default_rng(71)
+    for mode in range(8):
+        for b in _random_blocks(rng, 300, 1 << mode, mode + 1):
+            want = _pillow_bcn(b, 7, "RGBA").reshape(16, 4)
+            assert np.array_equal(np.array(osn._bc7_block(b)), want), (mode, b.hex())
+    assert osn._bc7_block(bytes(16)) == [[0, 0, 0, 0]] * 16
+    # decode_bc: values / 255, partial blocks
+    data = np.frombuffer(b"".join(_random_blocks(rng, 4, 1 << 6, 7)), np.uint8)
+    t = osn.decode_bc("BC7", data, 6, 5)
+    assert t.shape == (5, 6, 4)
+    np.testing.assert_array_equal(t[:4, :4], _pillow_bcn(data[:16], 7, "RGBA").astype(np.float32) / 255)
+
+
+def test_bc6h_unsigned_matches_pillows_decoder():
+    """200 random blocks of each of BC6H's fourteen modes (every endpoint-bit layout, delta
+    width and partition): the oracle's half-float texels, read as Pillow reads them
+    (clamp to [0, 1], times 255, truncated), equal Pillow's 8-bit texels to within one step
+    (Pillow's values differ by one half-float ulp in about 0.1 % of texels); reserved modes
+    decode to zeros."""
+    rng = np.random.default_rng(72)
+    worst, count = 0, 0
+    for mode_bits, width, *_ in osn.BC6H_MODES:
+        for b in _random_blocks(rng, 200, mode_bits, width):
+            want = _pillow_bcn(b, 6, "RGB", "uf").reshape(16, 3).astype(np.float64)
+            got = np.array(osn._bc6h_block(b, False), np.float64)
+            assert np.all(np.isfinite(got)) and got.min() >= 0 and got.max() <= 65504
+            worst = max(worst, float(np.abs(np.floor(np.clip(got, 0, 1) * 255) - want).max()))
+            count += 1
+    assert worst <= 1, worst
+    for reserved in (0b10011, 0b10111, 0b11011, 0b11111):
+        blk = (reserved | (0xABCDEF << 5)).to_bytes(16, "little")
+        assert osn._bc6h_block(blk, False) == [[0.0] * 3] * 16
+
+
+def _bc6h_mode11(e0, e1, index_bits):
+    """A BC6H mode-11 block (one region, raw 10-bit endpoints) with the given RGB endpoints and
+    16 4-bit indices (texel 0's index has 3 bits)."""
+    v, p = 0b00011, 5
+    for e in (e0, e1):
+        for c in range(3):
+            v |= (e[c] & 0x3FF) << p
+            p += 10
+    for t, i in enumerate(index_bits):
+        n = 3 if t == 0 else 4
+        v |= (i & ((1 << n) - 1)) << p
+        p += n
+    assert p == 128
+    return v.to_bytes(16, "little")
+
+
+def test_bc6h_signed_closed_forms():
+    """The signed variant (Pillow decodes it as unsigned, so it pins nothing there) at values
+    the specification fixes: the largest 10-bit magnitude 511 unquantizes to 0x7FFF and ends as
+    the largest half, +-65504; zero stays zero; weight 64 picks the second endpoint; -1
+    unquantizes to -((2^15 + 2^14) >> 9) = -96, which becomes the half -(96 * 31 >> 5) ulps =
+    -93 * 2^-24 (at 11 bits: -48, then -46 * 2^-24); a transformed mode adds a sign-extended
+    delta to the base (mode 12: base -1, delta +1 gives endpoint 0)."""
+    top = osn._bc6h_block(_bc6h_mode11((511, 511, 511), (-511, 0, 511), [0] * 16), True)
+    assert top[0] == [65504.0, 65504.0, 65504.0]
+    blk = osn._bc6h_block(_bc6h_mode11((0, 511, -511), (-511, 0, 511), [15] * 16), True)
+    assert blk[5] == [-65504.0, 0.0, 65504.0]
+    blk = osn._bc6h_block(_bc6h_mode11((-1, 0, 1), (0, 0, 0), [0] * 16), True)
+    assert blk[3][0] == np.float32(-93 * 2.0 ** -24) and blk[3][1] == 0.0 and blk[3][2] == np.float32(93 * 2.0 ** -24)
+    # unsigned: the largest 10-bit value 1023 becomes 0xFFFF and the half 65504 too
+    assert osn._bc6h_block(_bc6h_mode11((1023, 0, 1), (0, 0, 0), [0] * 16), False)[0][0] == 65504.0
+    # mode 12 (11-bit base, 9-bit deltas): r0 = -1 (all ones), r1 delta = +1 -> endpoint 0
+    v, p = 0b00111, 5
+    fields = [(0x3FF, 10), (0, 10), (0, 10), (1, 9), (1, 1), (0, 9), (0, 1), (0, 9), (0, 1)]
+    for val, n in fields:
+        v |= val << p
+        p += n
+    # indices: texel 0 (3 bits) 0, texel 1 picks weight 64 (index 15)
+    v |= 0 << p
+    v |= 15 << (p + 3)
+    b = v.to_bytes(16, "little")
+    t = osn._bc6h_block(b, True)
+    # endpoint 0: -1 at 11 bits unquantizes to -((2^15 + 2^14) >> 10) = -48, the half -(48 * 31 >> 5)
+    assert t[0][0] == np.float32(-46 * 2.0 ** -24)
+    assert t[1][0] == 0.0                                 # endpoint 1: -1 + 1 = 0

@@ -110,7 +110,7 @@ class VLRBatchJob(C.Structure):
 class VLRRadianceCacheSettings(C.Structure):
     _fields_ = [("learningRate", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float), ("epsilon", C.c_float),
                 ("trainingPaths", C.c_uint32), ("terminationThreshold", C.c_float), ("warmupSteps", C.c_uint32),
-                ("seed", C.c_uint32)]
+                ("seed", C.c_uint32), ("fullRadiance", C.c_uint32)]
 
 
 RAY_DTYPE = np.dtype([("org", np.float32, 3), ("tmin", np.float32), ("dir", np.float32, 3), ("tmax", np.float32)])

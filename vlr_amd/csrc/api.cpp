@@ -550,7 +550,7 @@ VLR_API VLRResult vlrBlockCompressedImage2DCreate(VLRContext context, uint8_t **
     if (!data || !sizes || !dataFormat || !spectrumType || !colorSpace || !image || mipCount == 0)
         return VLRResult_InvalidArgument;
     if (!bcFormatDecodable(dataFormat)) {
-        g_lastError = "block-compressed format not decoded by this build (BC1-BC5 are)";
+        g_lastError = "not a block-compressed format (BC1-BC7 and the signed variants are)";
         return VLRResult_InvalidArgument;
     }
     auto *img = new BlockCompressedImage2D(CTX(context));
@@ -1316,6 +1316,7 @@ VLR_API VLRResult vlrContextSetRadianceCacheSettings(VLRContext context, const V
     c.threshold = st->terminationThreshold;
     c.warmupSteps = st->warmupSteps;
     c.seed = st->seed;
+    c.fullRadiance = st->fullRadiance != 0;
     return VLRResult_NoError;
     VLRA_CATCH
 }
@@ -1332,6 +1333,7 @@ VLR_API VLRResult vlrContextGetRadianceCacheSettings(VLRContextConst context, VL
     st->terminationThreshold = c.threshold;
     st->warmupSteps = c.warmupSteps;
     st->seed = c.seed;
+    st->fullRadiance = c.fullRadiance ? 1u : 0u;
     return VLRResult_NoError;
     VLRA_CATCH
 }

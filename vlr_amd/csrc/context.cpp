@@ -186,6 +186,7 @@ void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCo
         fp.cacheMode = 0;
         fp.frameStamp = 0;
         fp.cacheThreshold = 0.0f;
+        fp.cacheFullRadiance = 0;
         fp.trainStride = 1;
         fp.trainOffset = 0;
         fp.pixelBegin = 0;
@@ -230,20 +231,7 @@ void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCo
             sortCounts.reserve(2 * kSortClasses * kMaxDepthCounters * sizeof(uint32_t));
         }
         FrameBuffers fb = frameBuffers();
-        if (guidesEnabled) {
-            const size_t n = size_t(width) * height, bytes = n * (accumChannels + 3) * sizeof(float);
-            if (guideBytes != bytes) {
-                guideAlbedo.reserve(n * accumChannels * sizeof(float));
-                guideNormal.reserve(n * 3 * sizeof(float));
-                guideBytes = bytes;
-                guideFrames = 0;
-            }
-            // the guides restart with the accumulation, or when they were just switched on
-            fp.guideReset = (numAccumFrames == 1 || guideFrames == 0) ? 1 : 0;
-            guideFrames = fp.guideReset ? 1 : guideFrames + 1;
-            fb.guideAlbedo = static_cast<float *>(guideAlbedo.ptr);
-            fb.guideNormal = static_cast<float *>(guideNormal.ptr);
-        }
+        prepareGuides(fp, fb);
         CacheLaunch cl;
         const CacheLaunch *clp = nullptr;
         if (cacheEnabled) {
@@ -254,6 +242,7 @@ void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCo
             fp.frameStamp = ++frameStamp;
             fp.cacheMode = cacheStep >= cacheSettings.warmupSteps ? 2 : 1;
             fp.cacheThreshold = cacheSettings.threshold;
+            fp.cacheFullRadiance = cacheSettings.fullRadiance ? 1 : 0;
             fp.trainStride = stride;
             fp.trainOffset = fp.frameStamp % stride;
             fb.cache.queryRaw = static_cast<float4 *>(cqRaw.ptr);
@@ -307,9 +296,16 @@ void Context::render(hipStream_t stream, const Camera *camera, uint32_t shrinkCo
 
 // The output as a 2D array of float4 (the reference's cudau::Array, context.cpp:1243), made on
 // the first request; from then on every rendered frame is copied into it on the render stream.
+// A GPU without image support (CDNA parts such as gfx950 report none) has no arrays: the
+// request fails with InternalError and the output stays readable as the linear buffer.
 hipArray_t Context::outputArrayHandle() {
     ensureDevice();
     if (!width) throw ApiError{3, "bind an output buffer first"};
+    int images = 0;
+    VLRA_HIP(hipDeviceGetAttribute(&images, hipDeviceAttributeImageSupport, device));
+    if (!images)
+        throw ApiError{5, "this GPU has no image support (hipDeviceAttributeImageSupport is 0), so the output has "
+                          "no hipArray form; read it with vlrContextReadOutputBuffer"};
     if (!outputArray) {
         hipChannelFormatDesc desc = hipCreateChannelDesc<float4>();
         VLRA_HIP(hipMallocArray(&outputArray, &desc, width, height, hipArraySurfaceLoadStore));
@@ -406,6 +402,24 @@ FrameBuffers Context::frameBuffers() {
     return fb;
 }
 
+// Denoiser guides of the frame, when switched on: their buffers, the restart rule and the
+// frame buffers' pointers.
+void Context::prepareGuides(FrameParams &fp, FrameBuffers &fb) {
+    if (!guidesEnabled) return;
+    const size_t n = size_t(width) * height, bytes = n * (accumChannels + 3) * sizeof(float);
+    if (guideBytes != bytes) {
+        guideAlbedo.reserve(n * accumChannels * sizeof(float));
+        guideNormal.reserve(n * 3 * sizeof(float));
+        guideBytes = bytes;
+        guideFrames = 0;
+    }
+    // the guides restart with the accumulation, or when they were just switched on
+    fp.guideReset = (numAccumFrames == 1 || guideFrames == 0) ? 1 : 0;
+    guideFrames = fp.guideReset ? 1 : guideFrames + 1;
+    fb.guideAlbedo = static_cast<float *>(guideAlbedo.ptr);
+    fb.guideNormal = static_cast<float *>(guideNormal.ptr);
+}
+
 // ---- LVC-BPT (context.cpp:1444-1481 of the reference) ----
 void Context::setNumLightPaths(uint32_t n) {
     if (n == 0 || n > kMaxLightPaths) throw ApiError{3, "number of light paths must be in [1, 1048576]"};
@@ -438,8 +452,9 @@ void Context::ensureBpt(uint32_t slotsPerPath) {
     }
 }
 
-void Context::renderBptFrame(hipStream_t stream, const FrameParams &fp, bool firstFrame, bool lightTracing) {
+void Context::renderBptFrame(hipStream_t stream, const FrameParams &frame, bool firstFrame, bool lightTracing) {
     joinHalves(stream);
+    FrameParams fp = frame;
     const uint32_t slotsPerPath = fp.maxPathLength + 1;
     ensureBpt(slotsPerPath);
     if (firstFrame)
@@ -463,7 +478,8 @@ void Context::renderBptFrame(hipStream_t stream, const FrameParams &fp, bool fir
     bp.numVertices = static_cast<uint32_t *>(lvNum.ptr);
     bp.lightRng = static_cast<uint64_t *>(lightRng.ptr);
     bp.splat = static_cast<float *>(splat.ptr);
-    const FrameBuffers fb = frameBuffers();
+    FrameBuffers fb = frameBuffers();
+    prepareGuides(fp, fb);
     auto launch = lightTracing ? launchLightTraceFrame : launchBptFrame;
     VLRA_HIP(launch(gpu.dev, fp, fb, bp, spectralActive, stream, timer(0, stream)));
     lastFrame = fp;

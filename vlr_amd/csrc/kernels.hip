@@ -953,11 +953,16 @@ __device__ __forceinline__ void loadPath(const FrameBuffers &fb, uint32_t path, 
 struct CacheCtx {
     uint32_t mode, stamp, stride, offset, width, imageW;
     float threshold;
+    // 1: the cache supplies all the radiance a path's last vertex reflects (its next-event
+    // estimate included); 0: the vertex keeps its next-event estimate and the cache supplies
+    // what a continued path would add after it
+    uint32_t full;
     CacheBuffers b;
 };
 __device__ __forceinline__ CacheCtx cacheCtx(const FrameParams &fp, const FrameBuffers &fb) {
     CacheCtx c;
     c.mode = fp.cacheMode;
+    c.full = fp.cacheFullRadiance;
     c.stamp = fp.frameStamp;
     c.stride = fp.trainStride;
     c.offset = fp.trainOffset;
@@ -1000,7 +1005,7 @@ __device__ __forceinline__ void shadeStep(const DevScene &sc, uint32_t pathLengt
     *emitShadow = false;
     *emitNext = false;
     const bool maxLengthTerminate = pathLength >= maxPathLength;
-    if (cc.mode != 0 && pathLength >= 2 && pathLength - 1 <= kCacheTrainVerts) {
+    if (cc.mode != 0 && !cc.full && pathLength >= 2 && pathLength - 1 <= kCacheTrainVerts) {
         // training path: the contribution before this vertex's emission closes the previous
         // vertex's sample (everything it gathered up to and including its next-event estimate)
         const uint32_t pix = (path / cc.width) * cc.imageW + path % cc.width;
@@ -1078,6 +1083,11 @@ __device__ __forceinline__ void shadeStep(const DevScene &sc, uint32_t pathLengt
                 writeCacheRecord(sc, sp, G.material, rayDir, N == 4 ? P.wls.offset : 0.0f, cc.b.trainRaw + 4 * slot);
                 cc.b.trainThroughput[slot] = storeSpec<N>(P.alpha);
                 cc.b.trainStamp[slot] = cc.stamp;
+                if (cc.full) {
+                    // the sample is everything the path gathers after this vertex's emission
+                    cc.b.trainContrib[slot] = storeSpec<N>(P.contrib);
+                    cc.b.trainAfterStamp[slot] = cc.stamp;
+                }
             }
         } else {
             cacheEnd = cc.mode == 2 && pathLength >= 2 && !maxLengthTerminate && !P.wls.single &&
@@ -1092,6 +1102,20 @@ __device__ __forceinline__ void shadeStep(const DevScene &sc, uint32_t pathLengt
         P.alpha = P.alpha / continueProb;
     }
     Bsdf<N> bsdf = setupBsdf<N, NODES>(sc, mat, P.wls, &sp);
+    if (cacheEnd && cc.full) {
+        if (bsdf.hasNonDelta()) {
+            // the cache supplies all the radiance this vertex reflects; no next-event estimate
+            writeCacheRecord(sc, sp, G.material, rayDir, N == 4 ? P.wls.offset : 0.0f, cc.b.queryRaw + 4 * path);
+            cc.b.queryThroughput[path] = storeSpec<N>(P.alpha);
+            cc.b.queryStamp[path] = cc.stamp;
+            return;
+        }
+        // a purely specular vertex does not end in the cache: the path goes on
+        cacheEnd = false;
+        float continueProb = fminf(P.alpha.importance(P.wls.sel) / P.initImportance, 1.0f);
+        if (P.rng.uniform() >= continueProb) return;
+        P.alpha = P.alpha / continueProb;
+    }
     V3 gnL = sp.frame.toLocal(sp.gn);
     if (bsdf.hasNonDelta()) {
         float uLight = P.rng.uniform();
@@ -1186,28 +1210,29 @@ __device__ __forceinline__ void storePath(const FrameBuffers &fb, uint32_t path,
     }
 }
 
-// Denoiser guides of a path's first hit (path_tracing.cu:95-108, 154-157, 272-275): the
-// BSDF's base colour times the D65 white point over the wavelength pdf, added to the
-// pixel's strata like a colour sample (without compensation), and the shading normal; a miss
-// adds zero.  Path index = pixel index (row stride fp.width).
+// Denoiser guides of a path's first hit (path_tracing.cu:95-108, 154-157, 272-275;
+// lvc_bpt.cu:528-540; aux_buffer_generator.cu:57-67): the BSDF's base colour times the D65
+// white point over the wavelength pdf, added to the pixel's strata like a colour sample
+// (without compensation), and the shading normal; a miss adds zero.  Path index = pixel index
+// (row stride fp.width).
 template <int N, bool NODES>
-__device__ __forceinline__ void firstHitGuides(const KArgs &a, uint32_t path, const int4 &ha, const float2 &hb,
-                                               const Wls<N> &wls) {
+__device__ __forceinline__ void addGuides(const DevScene &sc, const FrameParams &fp, const FrameBuffers &fb,
+                                          uint32_t path, const int4 &ha, const float2 &hb, const Wls<N> &wls,
+                                          float selectWLPDF) {
     Spec<N> albedo(0.0f);
     V3 n(0.0f);
     if (ha.x >= 0) {
         SurfPt sp;
         float hypAreaPDF;
-        decodeHit<NODES>(a.sc, ha.x, ha.y, ha.z, hb.x, hb.y, &sp, &hypAreaPDF);
-        const DevMaterial &mat = a.sc.materials[a.sc.geomInsts[ha.y].material];
-        constexpr float selectWLPDF = N == 4 ? 4 / (830.0f - 360.0f) : 1.0f;
-        albedo = setupBsdf<N, NODES>(a.sc, mat, wls, &sp).baseColor() * evalSpectrum<N>(a.sc, a.sc.whitePoint, wls) /
+        decodeHit<NODES>(sc, ha.x, ha.y, ha.z, hb.x, hb.y, &sp, &hypAreaPDF);
+        const DevMaterial &mat = sc.materials[sc.geomInsts[ha.y].material];
+        albedo = setupBsdf<N, NODES>(sc, mat, wls, &sp).baseColor() * evalSpectrum<N>(sc, sc.whitePoint, wls) /
                  selectWLPDF;
         n = sp.frame.z;
     }
-    const size_t plane = static_cast<size_t>(a.fp.width) * a.fp.height;
-    const bool reset = a.fp.guideReset != 0;
-    float *alb = a.fb.guideAlbedo + path;
+    const size_t plane = static_cast<size_t>(fp.width) * fp.height;
+    const bool reset = fp.guideReset != 0;
+    float *alb = fb.guideAlbedo + path;
     if constexpr (N == 4) {
         const float recBinWidth = 16 / (830.0f - 360.0f);
         float addend[16];
@@ -1227,10 +1252,45 @@ __device__ __forceinline__ void firstHitGuides(const KArgs &a, uint32_t path, co
 #pragma unroll
         for (int k = 0; k < 3; ++k) alb[k * plane] = (reset ? 0.0f : alb[k * plane]) + albedo[k];
     }
-    float *nrm = a.fb.guideNormal + path;
+    float *nrm = fb.guideNormal + path;
     nrm[0] = (reset ? 0.0f : nrm[0]) + n.x;
     nrm[plane] = (reset ? 0.0f : nrm[plane]) + n.y;
     nrm[2 * plane] = (reset ? 0.0f : nrm[2 * plane]) + n.z;
+}
+
+template <int N, bool NODES>
+__device__ __forceinline__ void firstHitGuides(const KArgs &a, uint32_t path, const int4 &ha, const float2 &hb,
+                                               const Wls<N> &wls) {
+    addGuides<N, NODES>(a.sc, a.fp, a.fb, path, ha, hb, wls, N == 4 ? 4 / (830.0f - 360.0f) : 1.0f);
+}
+
+// The light tracer's guides (aux_buffer_generator.cu:22-93, launched before every light-tracing
+// frame, context.cpp:1449-1451): per pixel a primary ray drawn from the pixel's generator as the
+// path tracer draws one (pixel jitter, wavelengths, lens point), traced to its first hit (alpha
+// tests draw from the same generator), whose guides are added.
+template <int N, bool NODES>
+__global__ void __launch_bounds__(kTraceBlock) k_ltGuides(KArgs a) {
+    __shared__ int32_t lds[kTraceLds];
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.fp.imageW * a.fp.imageH) return;
+    const uint32_t px = i % a.fp.imageW, py = i / a.fp.imageW;
+    const uint32_t path = py * a.fp.width + px;
+    float4 o, d;
+    generatePath<N>(a.fb, a.sc.camera, a.fp.imageW, a.fp.imageH, px, py, path, &o, &d);
+    Pcg32 rng{a.fb.ps.rng[path]};
+    TraceHit h;
+    h.inst = -1;
+    h.b1 = h.b2 = 0.0f;
+    const bool found = traceRay<false, false, NODES>(a.sc, xyz(o), xyz(d), 0.0f, d.w, &h, lds, threadIdx.x, nullptr,
+                                                     &rng);
+    a.fb.ps.rng[path] = rng.s;
+    const float4 meta = a.fb.ps.meta[path];
+    Wls<N> wls;
+    wls.offset = meta.x;
+    wls.sel = __float_as_uint(meta.w) & kFlagSelMask;
+    wls.single = false;
+    const int4 ha = found ? make_int4(h.inst, h.geomInst, h.prim, 0) : make_int4(-1, -1, -1, 0);
+    firstHitGuides<N, NODES>(a, path, ha, make_float2(h.b1, h.b2), wls);
 }
 
 template <int N, bool MULTI = false, bool NODES = true, bool GUIDES = false>
@@ -2093,7 +2153,7 @@ __global__ void __launch_bounds__(256) k_bptCompact(BptParams bp) {
 }
 
 // ---- eye subpaths (lvc_bpt.cu:338-971) ----
-template <int N, bool NODES = true>
+template <int N, bool NODES = true, bool GUIDES = false>
 __global__ void __launch_bounds__(kTraceBlock) k_bptEye(BptArgs a) {
     __shared__ int32_t lds[kTraceLds];
     const DevScene &sc = a.sc;
@@ -2234,6 +2294,10 @@ __global__ void __launch_bounds__(kTraceBlock) k_bptEye(BptArgs a) {
             bool found = traceRay<false, false, NODES>(sc, rayOrg, rayDir, 0.0f, 3.402823466e+38f, &h, lds,
                                                        threadIdx.x, nullptr, &rng);
             const Wls<N> wls = commonWls<N>(bp, single);
+            if (GUIDES && pathLength == 1)
+                addGuides<N, NODES>(sc, a.fp, a.fb, path,
+                                    found ? make_int4(h.inst, h.geomInst, h.prim, 0) : make_int4(-1, -1, -1, 0),
+                                    make_float2(h.b1, h.b2), wls, bp.wlProb);
             V3 nextOrg, nextDir;
             float woDirPDF = 0, woCosTerm = 0, woRevAreaPDF = 0;
             uint32_t woSampledType = 0;
@@ -2567,6 +2631,10 @@ __global__ void __launch_bounds__(kTraceBlock) k_lightTrace(BptArgs a) {
             V3 rayDir = ls.sp.frame.fromLocal(dirLocal);
             alpha *= Spec<N>(fe1) * (ls.sp.cosTerm(rayDir) / dirPDF);
             const float initImportance = alpha.importance(wls.sel);
+            // two generators from here on, as light_tracing.cu:138-176: the hit program draws
+            // from a copy (rng, the state stored at the end), Russian roulette from the
+            // ray-generation program's own (rrRng)
+            Pcg32 rrRng = rng;
             uint32_t pathLength = 0;
             while (true) {
                 ++pathLength;
@@ -2623,9 +2691,9 @@ __global__ void __launch_bounds__(kTraceBlock) k_lightTrace(BptArgs a) {
                 alpha *= fs * (fabsf(cosFactor) / pdf);
                 rayOrg = offsetRayOrigin(sp.position, cosFactor > 0.0f ? sp.gn : -sp.gn);
                 rayDir = sp.frame.fromLocal(dl);
-                // Russian roulette (light_tracing.cu:165-169)
+                // Russian roulette (light_tracing.cu:165-169), from the ray-generation generator
                 const float continueProb = fminf(alpha.importance(wls.sel) / initImportance, 1.0f);
-                if (rng.uniform() >= continueProb) break;
+                if (rrRng.uniform() >= continueProb) break;
                 alpha = alpha / continueProb;
             }
         }
@@ -2956,8 +3024,13 @@ static hipError_t launchBptFrameT(const DevScene &sc, const FrameParams &fp, con
     hipLaunchKernelGGL(k_bptCompact, dim3(blocksFor(bp.numLightPaths, 256)), dim3(256), 0, s, bp);
     t1(STAGE_LIGHT_PATHS);
     t0(STAGE_EYE_PATHS);
-    if (sc.hasNodePrograms)
+    const bool guides = fb.guideAlbedo != nullptr;
+    if (sc.hasNodePrograms && guides)
+        hipLaunchKernelGGL((k_bptEye<N, true, true>), dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
+    else if (sc.hasNodePrograms)
         hipLaunchKernelGGL((k_bptEye<N, true>), dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
+    else if (guides)
+        hipLaunchKernelGGL((k_bptEye<N, false, true>), dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
     else
         hipLaunchKernelGGL((k_bptEye<N, false>), dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
     t1(STAGE_EYE_PATHS);
@@ -2992,6 +3065,15 @@ static hipError_t launchLightTraceFrameT(const DevScene &sc, const FrameParams &
     if (bp.numLightPaths == 0) return hipErrorInvalidValue;
     auto t0 = [&](int st) { if (timer) timer->begin(st); };
     auto t1 = [&](int st) { if (timer) timer->end(st); };
+    if (fb.guideAlbedo) {
+        // the guides' primary rays come first, as the reference launches its aux-buffer pass
+        t0(STAGE_EYE_PATHS);
+        if (sc.hasNodePrograms)
+            hipLaunchKernelGGL((k_ltGuides<N, true>), dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, k);
+        else
+            hipLaunchKernelGGL((k_ltGuides<N, false>), dim3(blocksFor(n, kTraceBlock)), dim3(kTraceBlock), 0, s, k);
+        t1(STAGE_EYE_PATHS);
+    }
     t0(STAGE_LIGHT_PATHS);
     if (sc.hasNodePrograms)
         hipLaunchKernelGGL((k_lightTrace<N, true>), dim3(blocksFor(bp.numLightPaths, kTraceBlock)), dim3(kTraceBlock), 0, s, a);

@@ -26,6 +26,7 @@ struct FrameParams {
     uint32_t cacheMode;            // 0 off; 1 record training vertices only; 2 also end paths in the cache
     uint32_t frameStamp;           // grows every frame: marks the cache records written this frame
     float cacheThreshold;          // c of the spread test a(x_1..x_n) > c a_0
+    uint32_t cacheFullRadiance;    // the cache query replaces the last vertex's next-event estimate too
     uint32_t trainStride, trainOffset;   // pixel i traces a training path when i % stride == offset
     // the frame's paths are pixels pixelBegin .. pixelBegin + pixelCount - 1 of the image
     // (row-major over imageW); a frame may be rendered as several such parts

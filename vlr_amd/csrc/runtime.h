@@ -179,7 +179,7 @@ public:
     bool toRGBA32F(std::vector<float> *out) const override;
 };
 
-// BC1-BC5 (signed variants included) decoded into RGBA texels (bc_decode.cpp)
+// BC1-BC7 (signed variants included) decoded into RGBA texels (bc_decode.cpp)
 bool bcFormatDecodable(const std::string &format);
 bool decodeBlockCompressed(const std::string &format, const uint8_t *data, size_t size, uint32_t width, uint32_t height,
                            std::vector<float4> *texels);
@@ -518,6 +518,7 @@ public:
         float threshold = 0.01f;
         uint32_t warmupSteps = 16;
         uint32_t seed = 1;
+        bool fullRadiance = true;
     } cacheSettings;
     bool cacheEnabled = false;
     bool cacheInitialized = false;
@@ -574,6 +575,7 @@ public:
     void ensureBpt(uint32_t slotsPerPath);
     void setNumLightPaths(uint32_t n);
     void renderBptFrame(hipStream_t stream, const FrameParams &fp, bool firstFrame, bool lightTracing);
+    void prepareGuides(FrameParams &fp, FrameBuffers &fb);
     void readLightRngStates(uint64_t *dst);
     void writeLightRngStates(const uint64_t *src);
     // frames rendered as two pixel halves on two streams (one half's deep paths overlap the

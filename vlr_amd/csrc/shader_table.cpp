@@ -217,15 +217,15 @@ uint32_t ShaderTable::addTexture(const Image2D *img) {
     std::memset(&T, 0, sizeof(T));
     T.offset = static_cast<uint32_t>(texels.size());
     if (img && img->type == ObjType::BlockCompressedImage2D) {
-        // mip level 0 decoded; the texture unit's sRGB read of BC1-BC4 (image.cpp:336-346) is a
-        // per-texel degamma of the colour channels, done here
+        // mip level 0 decoded; the texture unit's sRGB read of BC1-BC4 and BC7 (image.cpp:336-346)
+        // is a per-texel degamma of the colour channels, done here
         const auto *bi = static_cast<const BlockCompressedImage2D *>(img);
         std::vector<float4> dec;
         if (!bi->mips.empty() &&
             decodeBlockCompressed(bi->formatStr, bi->mips[0].data(), bi->mips[0].size(), bi->width, bi->height, &dec)) {
             const std::string &f = bi->formatStr;
             const bool degamma = bi->colorSpace == ColorSpace::Rec709_D65_sRGBGamma &&
-                                 (f == "BC1" || f == "BC2" || f == "BC3" || f == "BC4");
+                                 (f == "BC1" || f == "BC2" || f == "BC3" || f == "BC4" || f == "BC7");
             T.width = bi->width;
             T.height = bi->height;
             T.format = kTexRGBA;
