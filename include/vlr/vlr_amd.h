@@ -270,6 +270,18 @@ VLR_API VLRResult vlrContextBuildBvh4(VLRContext context, hipStream_t stream, co
                                       uint32_t maxLeafPrims, const uint32_t* leafRemap, void* nodes,
                                       uint32_t nodeCapacity, uint32_t* order, uint32_t* numNodes);
 
+/* vlrBuildBvh4NoAlloc: no counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * the build of vlrContextBuildBvh4 (same arguments and results, no context), on the current
+ * GPU, calling no hipMalloc / hipFree.  Its scratch is one static region per GPU (about 35 MB,
+ * held until the process ends) that every call on that GPU shares, so calls that overlap on
+ * one GPU race.  numPrims above VLR_BVH4_NO_ALLOC_MAX_PRIMS is refused with InvalidArgument.
+ * Returns when the build has finished on `stream`. */
+#define VLR_BVH4_NO_ALLOC_MAX_PRIMS 131072u
+VLR_API VLRResult vlrBuildBvh4NoAlloc(hipStream_t stream, const float* boxes, uint32_t numPrims,
+                                      uint32_t maxLeafPrims, const uint32_t* leafRemap, void* nodes,
+                                      uint32_t nodeCapacity, uint32_t* order, uint32_t* numNodes);
+
 /* ---- shader-node entry point ----
  * Evaluates one shader-node plug (the node programs of shader_nodes.cu:20-483 with calcNode's
  * conversions, kernel_common.h:658-737) at numPoints device-resident surface points, each 20

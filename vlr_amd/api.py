@@ -279,6 +279,7 @@ PROTOTYPES: Dict[str, list] = {
     "vlrBvh4NodeCapacity": [U32, C.POINTER(U32)],
     "vlrBuildBvh4Host": [P, U32, U32, P, P, U32, P, C.POINTER(U32)],
     "vlrContextBuildBvh4": [H, H, P, U32, U32, P, P, U32, P, C.POINTER(U32)],
+    "vlrBuildBvh4NoAlloc": [H, P, U32, U32, P, P, U32, P, C.POINTER(U32)],
 }
 
 _lib = None

@@ -1579,6 +1579,26 @@ VLR_API VLRResult vlrContextBuildBvh4(VLRContext context, hipStream_t stream, co
     VLRA_CATCH
 }
 
+VLR_API VLRResult vlrBuildBvh4NoAlloc(hipStream_t stream, const float *boxes, uint32_t numPrims,
+                                      uint32_t maxLeafPrims, const uint32_t *leafRemap, void *nodes,
+                                      uint32_t nodeCapacity, uint32_t *order, uint32_t *numNodes) {
+    static_assert(VLR_BVH4_NO_ALLOC_MAX_PRIMS == kNoAllocMaxPrims, "header and library agree on the limit");
+    VLRA_TRY
+    if ((!boxes || !nodes || !order) && numPrims) return VLRResult_InvalidArgument;
+    if (!numNodes || nodeCapacity < GpuBvhBuilder::nodeCapacity(numPrims)) return VLRResult_InvalidArgument;
+    if (maxLeafPrims < 1 || maxLeafPrims > kMaxLeafPrims) return VLRResult_InvalidArgument;
+    if (leafRemap && maxLeafPrims != 1) return VLRResult_InvalidArgument;
+    if (numPrims > kNoAllocMaxPrims) {
+        g_lastError = "vlrBuildBvh4NoAlloc: numPrims " + std::to_string(numPrims) + " exceeds the maximum " +
+                      std::to_string(kNoAllocMaxPrims) + " (VLR_BVH4_NO_ALLOC_MAX_PRIMS)";
+        return VLRResult_InvalidArgument;
+    }
+    *numNodes = buildBvh4NoAlloc(boxes, numPrims, maxLeafPrims, leafRemap, static_cast<BVH4QNode *>(nodes), order,
+                                 stream);
+    return VLRResult_NoError;
+    VLRA_CATCH
+}
+
 VLR_API VLRResult vlrContextEvaluateShaderNodePlug(VLRContext context, hipStream_t stream, VLRShaderNodePlug plug,
                                                    bool spectral, const float *points, uint32_t numPoints,
                                                    float *values) {

@@ -164,10 +164,20 @@ public:
                             hipStream_t stream, BvhBuildTimes *info = nullptr);
     const uint32_t *order() const;
 
-private:
     struct Impl;
+
+private:
     Impl *impl_;
 };
+
+// Largest build of buildBvh4NoAlloc.
+constexpr uint32_t kNoAllocMaxPrims = 1u << 17;
+// A single-tree build over n <= kNoAllocMaxPrims device boxes (6 floats each) that allocates no
+// memory: its scratch is one static region per GPU, shared by every call on that GPU (calls that
+// overlap on one GPU race), and the primitive order goes straight to `order` (device, n
+// entries).  Returns the node count after the build has finished on `stream`.
+uint32_t buildBvh4NoAlloc(const float *boxes, uint32_t numPrims, uint32_t maxLeaf, const uint32_t *leafRemap,
+                          BVH4QNode *nodes, uint32_t *order, hipStream_t stream);
 
 // dst[i] = src[order[i]] on the device.
 void gatherTriangles(const TriRecord *src, const uint32_t *order, uint32_t n, TriRecord *dst, hipStream_t stream);

@@ -23,7 +23,9 @@
 #include <algorithm>
 #include <cfloat>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
+#include <string>
 
 namespace vlra {
 namespace {
@@ -795,14 +797,41 @@ uint32_t blocksFor(size_t n, uint32_t per) { return static_cast<uint32_t>((n + p
 
 } // namespace
 
-struct GpuBvhBuilder::Impl {
-    DeviceBuffer boxes, ord, tmp, tasks, blocks, bins, outs, blockLeft, blockOff, misc, errBuf, segFirstDev, bvh2, small,
-        frontA, frontB, segA, segB, slots, inner, innerOff, scanSums, scanTotal;
+// A scratch buffer of the builder: device memory it owns, or (arena mode) a range of one
+// region of device memory it was given, carved at 256-byte alignment.
+struct ScratchBuf {
+    DeviceBuffer own;
+    void *ptr = nullptr;
+};
 
+struct GpuBvhBuilder::Impl {
+    ScratchBuf boxes, ord, tmp, tasks, blocks, bins, outs, blockLeft, blockOff, misc, errBuf, segFirstDev, bvh2, small,
+        frontA, frontB, segA, segB, slots, inner, innerOff, scanSums, scanTotal;
+    // arena mode: every buffer comes from [arena, arena + arenaBytes); nothing is allocated
+    uint8_t *arena = nullptr;
+    size_t arenaBytes = 0, arenaTop = 0;
+    // arena mode: the primitive order is written straight into this device array
+    uint32_t *orderOut = nullptr;
+
+    void take(ScratchBuf &b, size_t bytes) {
+        if (!arena) {
+            b.own.reserve(bytes);
+            b.ptr = b.own.ptr;
+            return;
+        }
+        const size_t at = (arenaTop + 255) & ~size_t(255);
+        if (at + bytes > arenaBytes) throw std::runtime_error("BVH build: scratch region exhausted");
+        b.ptr = arena + at;
+        arenaTop = at + bytes;
+    }
+    void put(ScratchBuf &b, const void *src, size_t bytes, hipStream_t s) {
+        take(b, bytes);
+        if (bytes) VLRA_HIP(hipMemcpyAsync(b.ptr, src, bytes, hipMemcpyHostToDevice, s));
+    }
+
+    // scanSums / scanTotal must hold blocksFor(n, 1024) + 1 words
     void scan(const uint32_t *in, uint32_t *out, uint32_t n, hipStream_t s, uint32_t *hostTotal) {
         const uint32_t nb = std::max(1u, blocksFor(n, 1024));
-        scanSums.reserve(size_t(nb) * 4);
-        scanTotal.reserve(4);
         auto *sums = static_cast<uint32_t *>(scanSums.ptr);
         auto *tot = static_cast<uint32_t *>(scanTotal.ptr);
         k_scanLocal<<<nb, 256, 0, s>>>(in, out, n, sums);
@@ -823,11 +852,16 @@ struct GpuBvhBuilder::Impl {
             n += segCount[k];
         }
         const float4 *bx = static_cast<const float4 *>(boxes.ptr);
-        ord.reserve(size_t(n) * 4);
-        tmp.reserve(size_t(n) * 4);
-        misc.reserve(size_t(numSegs) * 12 * 4);
-        errBuf.reserve(4);
-        segFirstDev.upload(segFirst.data(), size_t(numSegs) * 4);
+        if (orderOut)
+            ord.ptr = orderOut;
+        else
+            take(ord, size_t(n) * 4);
+        take(tmp, size_t(n) * 4);
+        take(misc, size_t(numSegs) * 12 * 4);
+        take(errBuf, 4);
+        put(segFirstDev, segFirst.data(), size_t(numSegs) * 4, s);
+        // buffers of one wide level, and after the wide phase those of the collapse, start here
+        const size_t levelMark = arenaTop;
         auto *ordp = static_cast<uint32_t *>(ord.ptr);
         auto *keys = static_cast<uint32_t *>(misc.ptr);
         auto *err = static_cast<uint32_t *>(errBuf.ptr);
@@ -837,7 +871,7 @@ struct GpuBvhBuilder::Impl {
         for (uint32_t k = 0; k < numSegs; ++k)
             for (uint32_t j = 0; j < blocksFor(segCount[k], kBinChunk); ++j)
                 blk.push_back({k, segFirst[k] + j * kBinChunk, std::min(segFirst[k] + segCount[k], segFirst[k] + (j + 1) * kBinChunk), 0});
-        blocks.upload(blk.data(), blk.size() * sizeof(WBlock));
+        put(blocks, blk.data(), blk.size() * sizeof(WBlock), s);
         k_segBounds<<<static_cast<uint32_t>(blk.size()), 256, 0, s>>>(bx, static_cast<const WBlock *>(blocks.ptr), keys);
         std::vector<uint32_t> rootKeys(size_t(numSegs) * 12);
         VLRA_HIP(hipMemcpyAsync(rootKeys.data(), keys, rootKeys.size() * 4, hipMemcpyDeviceToHost, s));
@@ -888,12 +922,13 @@ struct GpuBvhBuilder::Impl {
                     blk.push_back({t, T.begin + j * kBinChunk, std::min(T.begin + T.count, T.begin + (j + 1) * kBinChunk), 0});
             }
             const uint32_t nt = static_cast<uint32_t>(level.size()), nb = static_cast<uint32_t>(blk.size());
-            tasks.reserve(nt * sizeof(WTask));
-            blocks.reserve(nb * sizeof(WBlock));
-            bins.reserve(size_t(nt) * kBinWords * 4);
-            outs.reserve(nt * sizeof(WOut));
-            blockLeft.reserve(nb * 4);
-            blockOff.reserve(nb * 4);
+            arenaTop = levelMark;
+            take(tasks, nt * sizeof(WTask));
+            take(blocks, nb * sizeof(WBlock));
+            take(bins, size_t(nt) * kBinWords * 4);
+            take(outs, nt * sizeof(WOut));
+            take(blockLeft, nb * 4);
+            take(blockOff, nb * 4);
             VLRA_HIP(hipMemcpyAsync(tasks.ptr, level.data(), nt * sizeof(WTask), hipMemcpyHostToDevice, s));
             VLRA_HIP(hipMemcpyAsync(blocks.ptr, blk.data(), nb * sizeof(WBlock), hipMemcpyHostToDevice, s));
             auto *tp = static_cast<const WTask *>(tasks.ptr);
@@ -961,24 +996,25 @@ struct GpuBvhBuilder::Impl {
             t.resBase = total;
             total += t.count - 1;
         }
-        bvh2.reserve(size_t(total) * sizeof(BVHNode));
+        arenaTop = levelMark;
+        take(bvh2, size_t(total) * sizeof(BVHNode));
         auto *b2 = static_cast<BVHNode *>(bvh2.ptr);
         VLRA_HIP(hipMemcpyAsync(b2, wideNodes.data(), wideNodes.size() * sizeof(BVHNode), hipMemcpyHostToDevice, s));
         if (!smallTasks.empty()) {
-            small.reserve(smallTasks.size() * sizeof(STask));
-            VLRA_HIP(hipMemcpyAsync(small.ptr, smallTasks.data(), smallTasks.size() * sizeof(STask),
-                                    hipMemcpyHostToDevice, s));
+            put(small, smallTasks.data(), smallTasks.size() * sizeof(STask), s);
             k_subtrees<<<static_cast<uint32_t>(smallTasks.size()), 256, 0, s>>>(bx, ordp, static_cast<const STask *>(small.ptr),
                                                                                  maxLeaf, b2, err);
         }
         // collapse: level 0 holds the segments' roots
-        frontA.reserve(size_t(total) * 4);
-        frontB.reserve(size_t(total) * 4);
-        segA.reserve(size_t(total) * 4);
-        segB.reserve(size_t(total) * 4);
-        slots.reserve(size_t(total) * 4 * sizeof(CSlot));
-        inner.reserve(size_t(total) * 4);
-        innerOff.reserve(size_t(total) * 4);
+        take(frontA, size_t(total) * 4);
+        take(frontB, size_t(total) * 4);
+        take(segA, size_t(total) * 4);
+        take(segB, size_t(total) * 4);
+        take(slots, size_t(total) * 4 * sizeof(CSlot));
+        take(inner, size_t(total) * 4);
+        take(innerOff, size_t(total) * 4);
+        take(scanSums, (size_t(std::max(1u, blocksFor(total, 1024))) + 1) * 4);
+        take(scanTotal, 4);
         std::vector<uint32_t> roots(numSegs);
         for (uint32_t k = 0; k < numSegs; ++k) roots[k] = k;
         VLRA_HIP(hipMemcpyAsync(frontA.ptr, roots.data(), size_t(numSegs) * 4, hipMemcpyHostToDevice, s));
@@ -1031,7 +1067,7 @@ uint32_t GpuBvhBuilder::build(const float *boxes, uint32_t n, uint32_t maxLeaf, 
                               BVH4QNode *nodes, hipStream_t s, BvhBuildTimes *info) {
     checkArgs(n, maxLeaf, leafRemap);
     if (n == 0) return 0;
-    impl_->boxes.reserve(size_t(n) * 32);
+    impl_->take(impl_->boxes, size_t(n) * 32);
     k_boxesFromFloats<<<blocksFor(n, 256), 256, 0, s>>>(boxes, n, static_cast<float4 *>(impl_->boxes.ptr));
     return impl_->run({0u}, {n}, maxLeaf, leafRemap, nodes, nodeCapacity(n), s, info);
 }
@@ -1047,7 +1083,7 @@ uint32_t GpuBvhBuilder::buildTriangles(const TriRecord *tris, const std::vector<
     if (n >= kBvhMaxPrims) throw std::runtime_error("BVH build: too many primitives");
     checkArgs(static_cast<uint32_t>(n), maxLeaf, nullptr);
     if (segFirst.empty()) return 0;
-    impl_->boxes.reserve(size_t(n) * 32);
+    impl_->take(impl_->boxes, size_t(n) * 32);
     k_boxesFromTris<<<blocksFor(n, 256), 256, 0, s>>>(tris, static_cast<uint32_t>(n),
                                                       static_cast<float4 *>(impl_->boxes.ptr));
     return impl_->run(segFirst, segCount, maxLeaf, nullptr, nodes, static_cast<uint32_t>(cap), s, info);
@@ -1055,6 +1091,70 @@ uint32_t GpuBvhBuilder::buildTriangles(const TriRecord *tris, const std::vector<
 
 void gatherTriangles(const TriRecord *src, const uint32_t *order, uint32_t n, TriRecord *dst, hipStream_t s) {
     if (n) k_gatherTris<<<blocksFor(n, 256), 256, 0, s>>>(src, order, n, dst);
+}
+
+// ---- builds that allocate nothing ----
+// Scratch of a single-tree build over n <= kNoAllocMaxPrims primitives, taken in the order
+// run() takes it: the boxes, the partition buffer, the segment keys, error word and segment
+// table; then the larger of one wide level's buffers and the collapse's.  Every buffer may
+// cost 255 bytes of alignment.
+constexpr size_t alignUp(size_t b) { return (b + 255) & ~size_t(255); }
+constexpr size_t noAllocScratchBytes(size_t n) {
+    // wide levels: tasks hold more than kBvhSmallPrims primitives each, blocks cover kBinChunk
+    const size_t tasks = n / (kBvhSmallPrims + 1), blks = n / kBinChunk + 1 + tasks;
+    const size_t level = alignUp(tasks * sizeof(WTask)) + alignUp(blks * sizeof(WBlock)) +
+                         alignUp(tasks * kBinWords * 4) + alignUp(tasks * sizeof(WOut)) + 2 * alignUp(blks * 4);
+    // collapse: at most n - 1 binary nodes, subtree tasks of two or more primitives each
+    const size_t total = n > 1 ? n - 1 : 1;
+    const size_t collapse = alignUp(total * sizeof(BVHNode)) + alignUp((n / 2) * sizeof(STask)) +
+                            6 * alignUp(total * 4) + alignUp(total * 4 * sizeof(CSlot)) +
+                            alignUp(((total + 1023) / 1024 + 1) * 4) + alignUp(4);
+    const size_t fixed = alignUp(n * 32) + alignUp(n * 4) + 3 * alignUp(12 * 4);
+    return fixed + (level > collapse ? level : collapse) + 256;
+}
+constexpr size_t kNoAllocScratchBytes = noAllocScratchBytes(kNoAllocMaxPrims);
+static_assert(kNoAllocScratchBytes < (size_t(40) << 20), "the no-alloc scratch stays modest");
+
+// One region per GPU for the builds that allocate nothing, held until the process ends.  The
+// name is a host-side stand-in: host code uses the device address that hipGetSymbolAddress
+// returns for the current GPU, never the name itself.
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex gScratchMutex;
+uint8_t *gScratchPtr[kNoAllocMaxDevices] = {};
+
+// The current GPU's copy of `scratch`; looked up once per GPU, with that GPU current.
+uint8_t *scratchOfCurrentDevice() {
+    int dev = -1;
+    VLRA_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocMaxDevices) throw std::runtime_error("BVH build: device index out of range");
+    std::lock_guard<std::mutex> lock(gScratchMutex);
+    if (!gScratchPtr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw std::runtime_error(std::string("BVH build: no address for the scratch region: ") + hipGetErrorString(e));
+        gScratchPtr[dev] = static_cast<uint8_t *>(p);
+    }
+    return gScratchPtr[dev];
+}
+}  // namespace
+
+uint32_t buildBvh4NoAlloc(const float *boxes, uint32_t n, uint32_t maxLeaf, const uint32_t *leafRemap,
+                          BVH4QNode *nodes, uint32_t *order, hipStream_t s) {
+    checkArgs(n, maxLeaf, leafRemap);
+    if (n > kNoAllocMaxPrims) throw std::runtime_error("BVH build: more primitives than the no-alloc maximum");
+    if (n == 0) return 0;
+    uint8_t *region = scratchOfCurrentDevice();
+    GpuBvhBuilder::Impl impl;
+    impl.arena = region;
+    impl.arenaBytes = kNoAllocScratchBytes;
+    impl.orderOut = order;
+    impl.take(impl.boxes, size_t(n) * 32);
+    k_boxesFromFloats<<<blocksFor(n, 256), 256, 0, s>>>(boxes, n, static_cast<float4 *>(impl.boxes.ptr));
+    return impl.run({0u}, {n}, maxLeaf, leafRemap, nodes, GpuBvhBuilder::nodeCapacity(n), s, nullptr);
 }
 
 } // namespace vlra
