@@ -154,8 +154,7 @@ struct TravCount {
     uint32_t nodes = 0, tris = 0, insts = 0;
 };
 
-// Resumable traversal: the state of one ray's walk through the two-level BVH, advanced a
-// bounded number of steps (node fetches, instance entries or leaves) at a time.  The node stack
+// Traversal state: one ray's walk through the two-level BVH.  The node stack
 // lives in LDS (kLdsStack entries per lane) with an overflow area the caller keeps in scratch.
 constexpr int32_t kTravDone = INT32_MIN + 1;
 struct TravState {
@@ -194,14 +193,18 @@ __device__ __forceinline__ void travBegin(TravState &st, const DevScene &sc, con
     st.vis = 1.0f;
 }
 
-// Advances the traversal by at most maxSteps steps; true once it has finished (then st.found
-// and st.hit hold the closest hit, or any hit for an any-hit ray).
+// Runs the traversal to its end (then st.found and st.hit hold the closest hit, or any hit for
+// an any-hit ray).  The loop is a while-while loop: a lane walks inner nodes until it reaches a
+// leaf, an instance's end marker or the end of the walk, and only then handles that one item
+// (enter the instance, test the leaf's triangles, or leave the instance) before walking inner
+// nodes again.  Each lane visits the same nodes in the same order as a one-item-per-iteration
+// loop; the wave just runs its lanes' inner-node visits together and their leaves together.
 // With ALPHA, triangles of groups with an alpha node pass the stochastic alpha test
 // (path_tracing.cu:14-25) for closest-hit rays and scale the fractional visibility
 // (light_transport_common.h:205-218) for any-hit rays.
 template <bool STATS, bool ALPHA = false>
-__device__ __forceinline__ bool travStep(TravState &st, const DevScene &sc, int32_t *lds, int lane, int32_t *ovf,
-                                         float *ovfT, int maxSteps, TravCount *tc) {
+__device__ __forceinline__ void traverse(TravState &st, const DevScene &sc, int32_t *lds, int lane, int32_t *ovf,
+                                         float *ovfT, TravCount *tc) {
     float *ldsT = reinterpret_cast<float *>(lds + kLdsStack * kTraceBlock);
     // each stack entry keeps the distance at which the ray enters its box; once a closer hit
     // has shrunk tmax past it (with the slab test's widening), the entry is dropped unvisited.
@@ -217,9 +220,28 @@ __device__ __forceinline__ bool travStep(TravState &st, const DevScene &sc, int3
         }
         ++st.sp;
     };
-    for (int step = 0; step < maxSteps && st.cur != kTravDone; ++step) {
-        const int32_t cur = st.cur;
-        if (cur >= 0) {
+    // pop, dropping entries that lie beyond the closest hit so far
+    auto pop = [&]() -> int32_t {
+        const float tCull = st.tmax * (1.0f + 8.0f * __FLT_EPSILON__);
+        while (st.sp > 0) {
+            --st.sp;
+            float te;
+            int32_t v;
+            if (st.sp < kLdsStack) {
+                v = lds[st.sp * kTraceBlock + lane];
+                te = ldsT[st.sp * kTraceBlock + lane];
+            } else {
+                v = ovf[min(st.sp - kLdsStack, kScratchStack - 1)];
+                te = ovfT[min(st.sp - kLdsStack, kScratchStack - 1)];
+            }
+            if (st.anyHit || te <= tCull) return v;
+        }
+        return kTravDone;
+    };
+    while (st.cur != kTravDone) {
+        // inner nodes
+        while (st.cur >= 0) {
+            const int32_t cur = st.cur;
             if constexpr (STATS) ++tc->nodes;
             const BVH4QNode *nodes = st.inBlas ? sc.blasNodes + st.blasNode : sc.tlasNodes;
             const uint4 *np = reinterpret_cast<const uint4 *>(nodes + cur);
@@ -267,9 +289,13 @@ __device__ __forceinline__ bool travStep(TravState &st, const DevScene &sc, int3
                 if (hits > 2) push(c2, t2);
                 if (hits > 1) push(c1, t1);
                 st.cur = c0;
-                continue;
+            } else {
+                st.cur = pop();
             }
-        } else if (cur != kSentinel) {
+        }
+        if (st.cur == kTravDone) break;
+        const int32_t cur = st.cur;
+        if (cur != kSentinel) {
             const uint32_t code = ~static_cast<uint32_t>(cur);
             const uint32_t first = code >> kLeafShift;
             const uint32_t count = (code & (kMaxLeafPrims - 1)) + 1;
@@ -327,28 +353,8 @@ __device__ __forceinline__ bool travStep(TravState &st, const DevScene &sc, int3
             st.inBlas = false;
             st.ray = st.world;
         }
-        // pop, dropping entries that lie beyond the closest hit so far
-        const float tCull = st.tmax * (1.0f + 8.0f * __FLT_EPSILON__);
-        int32_t next = kTravDone;
-        while (st.sp > 0) {
-            --st.sp;
-            float te;
-            int32_t v;
-            if (st.sp < kLdsStack) {
-                v = lds[st.sp * kTraceBlock + lane];
-                te = ldsT[st.sp * kTraceBlock + lane];
-            } else {
-                v = ovf[min(st.sp - kLdsStack, kScratchStack - 1)];
-                te = ovfT[min(st.sp - kLdsStack, kScratchStack - 1)];
-            }
-            if (st.anyHit || te <= tCull) {
-                next = v;
-                break;
-            }
-        }
-        st.cur = next;
+        st.cur = pop();
     }
-    return st.cur == kTravDone;
 }
 
 // rng: the path's generator (closest-hit rays through alpha surfaces); vis: the fractional
@@ -362,7 +368,7 @@ __device__ __forceinline__ bool traceRay(const DevScene &sc, const V3 &org, cons
     st.rng = rng;
     int32_t ovf[kScratchStack];
     float ovfT[kScratchStack];
-    travStep<STATS, ALPHA>(st, sc, lds, lane, ovf, ovfT, INT32_MAX, tc);
+    traverse<STATS, ALPHA>(st, sc, lds, lane, ovf, ovfT, tc);
     if (st.found) *hit = st.hit;
     if (vis) *vis = st.found ? 0.0f : st.vis;
     return st.found;
