@@ -202,7 +202,9 @@ __device__ __forceinline__ void travBegin(TravState &st, const DevScene &sc, con
 // With ALPHA, triangles of groups with an alpha node pass the stochastic alpha test
 // (path_tracing.cu:14-25) for closest-hit rays and scale the fractional visibility
 // (light_transport_common.h:205-218) for any-hit rays.
-template <bool STATS, bool ALPHA = false>
+// ANY: an any-hit ray, which needs no visiting order: its children are stacked as they come,
+// without entry distances.
+template <bool ANY, bool STATS, bool ALPHA = false>
 __device__ __forceinline__ void traverse(TravState &st, const DevScene &sc, int32_t *lds, int lane, int32_t *ovf,
                                          float *ovfT, TravCount *tc) {
     float *ldsT = reinterpret_cast<float *>(lds + kLdsStack * kTraceBlock);
@@ -213,30 +215,37 @@ __device__ __forceinline__ void traverse(TravState &st, const DevScene &sc, int3
     auto push = [&](int32_t v, float tEnter) {
         if (st.sp < kLdsStack) {
             lds[st.sp * kTraceBlock + lane] = v;
-            ldsT[st.sp * kTraceBlock + lane] = tEnter;
+            if constexpr (!ANY) ldsT[st.sp * kTraceBlock + lane] = tEnter;
         } else {
             ovf[min(st.sp - kLdsStack, kScratchStack - 1)] = v;
-            ovfT[min(st.sp - kLdsStack, kScratchStack - 1)] = tEnter;
+            if constexpr (!ANY) ovfT[min(st.sp - kLdsStack, kScratchStack - 1)] = tEnter;
         }
         ++st.sp;
     };
-    // pop, dropping entries that lie beyond the closest hit so far
+    // pop, dropping entries that lie beyond the closest hit so far (an any-hit ray stops at its
+    // first hit, so every entry it pops is still open)
     auto pop = [&]() -> int32_t {
-        const float tCull = st.tmax * (1.0f + 8.0f * __FLT_EPSILON__);
-        while (st.sp > 0) {
+        if constexpr (ANY) {
+            if (st.sp == 0) return kTravDone;
             --st.sp;
-            float te;
-            int32_t v;
-            if (st.sp < kLdsStack) {
-                v = lds[st.sp * kTraceBlock + lane];
-                te = ldsT[st.sp * kTraceBlock + lane];
-            } else {
-                v = ovf[min(st.sp - kLdsStack, kScratchStack - 1)];
-                te = ovfT[min(st.sp - kLdsStack, kScratchStack - 1)];
+            return st.sp < kLdsStack ? lds[st.sp * kTraceBlock + lane] : ovf[min(st.sp - kLdsStack, kScratchStack - 1)];
+        } else {
+            const float tCull = st.tmax * (1.0f + 8.0f * __FLT_EPSILON__);
+            while (st.sp > 0) {
+                --st.sp;
+                float te;
+                int32_t v;
+                if (st.sp < kLdsStack) {
+                    v = lds[st.sp * kTraceBlock + lane];
+                    te = ldsT[st.sp * kTraceBlock + lane];
+                } else {
+                    v = ovf[min(st.sp - kLdsStack, kScratchStack - 1)];
+                    te = ovfT[min(st.sp - kLdsStack, kScratchStack - 1)];
+                }
+                if (te <= tCull) return v;
             }
-            if (st.anyHit || te <= tCull) return v;
+            return kTravDone;
         }
-        return kTravDone;
     };
     while (st.cur != kTravDone) {
         // inner nodes
@@ -272,7 +281,20 @@ __device__ __forceinline__ void traverse(TravState &st, const DevScene &sc, int3
             if (c2 != 0 && slab(lo(2), hi(2), st.ray, st.tmin, st.tmax, &tt)) t2 = tt;
             if (c3 != 0 && slab(lo(3), hi(3), st.ray, st.tmin, st.tmax, &tt)) t3 = tt;
             const int hits = (t0 != kMiss) + (t1 != kMiss) + (t2 != kMiss) + (t3 != kMiss);
-            if (hits > 0) {
+            if (ANY && hits > 0) {
+                // any order will do: stack the hit children but the first, visit that one next
+                int32_t next = 0;   // 0 is never a child code
+                auto take = [&](int32_t c, float t) {
+                    if (t == kMiss) return;
+                    if (next == 0) next = c;
+                    else push(c, t);
+                };
+                take(c0, t0);
+                take(c1, t1);
+                take(c2, t2);
+                take(c3, t3);
+                st.cur = next;
+            } else if (hits > 0) {
                 // sort the four (distance, child) pairs, nearest first; misses sort last
                 auto cswap = [](float &ta, int32_t &ca, float &tb, int32_t &cb) {
                     if (tb < ta) {
@@ -325,7 +347,7 @@ __device__ __forceinline__ void traverse(TravState &st, const DevScene &sc, int3
                         const uint32_t gi = __float_as_uint(q0.w);
                         if (sc.hasAlpha && sc.geomInsts[gi].alphaCount) {
                             const float alpha = alphaAt(sc, st.curInst, gi, __float_as_uint(q1.w), b1, b2);
-                            if (st.anyHit) {
+                            if (ANY) {
                                 st.vis *= 1 - alpha;
                                 if (st.vis != 0.0f) continue;
                             } else if (st.rng && st.rng->uniform() >= alpha) {
@@ -341,10 +363,10 @@ __device__ __forceinline__ void traverse(TravState &st, const DevScene &sc, int3
                     st.hit.t = t;
                     st.hit.b1 = b1;
                     st.hit.b2 = b2;
-                    if (st.anyHit) break;
+                    if (ANY) break;
                 }
             }
-            if (st.anyHit && st.found) {
+            if (ANY && st.found) {
                 st.cur = kTravDone;
                 break;
             }
@@ -368,7 +390,7 @@ __device__ __forceinline__ bool traceRay(const DevScene &sc, const V3 &org, cons
     st.rng = rng;
     int32_t ovf[kScratchStack];
     float ovfT[kScratchStack];
-    traverse<STATS, ALPHA>(st, sc, lds, lane, ovf, ovfT, tc);
+    traverse<ANY, STATS, ALPHA>(st, sc, lds, lane, ovf, ovfT, tc);
     if (st.found) *hit = st.hit;
     if (vis) *vis = st.found ? 0.0f : st.vis;
     return st.found;
