@@ -43,6 +43,115 @@ MAT_D65_TO_XYZ = np.array([0.4124564, 0.2126729, 0.0193339, 0.3575761, 0.7151522
 MAT_E_TO_XYZ = np.array([0.4969, 0.2562, 0.0233, 0.3391, 0.6782, 0.1130, 0.1640, 0.0656, 0.8637], f32)
 
 
+# spectrum_base.h:109-128, XYZ to Rec.709 (D65 white for light sources, E for the rest)
+MAT_XYZ_TO_D65 = np.array([3.2404542, -0.9692660, 0.0556434, -1.5371385, 1.8760108, -0.2040259, -0.4985314,
+                           0.0415560, 1.0572252], f32)
+MAT_XYZ_TO_E = np.array([2.6897, -1.0221, 0.0612, -1.2759, 1.9783, -0.2245, -0.4138, 0.0438, 1.1633], f32)
+
+_CMF = None
+
+
+def cmf_table():
+    """The CIE 1931 2-degree CMF at 1 nm, 360-830 nm (xbar, ybar, zbar), from the data file
+    the product reads (extracted from shared/spectrum_base.cpp:6-190)."""
+    global _CMF
+    if _CMF is None:
+        raw = ob.TABLES.read_bytes()
+        assert raw[:8] == b"VLRSPEC1"
+        n = int(np.frombuffer(raw, np.uint32, 1, 8)[0])
+        t = np.frombuffer(raw, np.float32, 3 * n, 12).reshape(3, n)
+        _CMF = (t[0].copy(), t[1].copy(), t[2].copy())
+    return _CMF
+
+
+class _Kahan:
+    """CompensatedSum (shared/common_internal.h:351-368) in float32."""
+
+    def __init__(self):
+        self.s, self.c = f32(0), f32(0)
+
+    def add(self, v):
+        inp = f32(f32(v) - self.c)
+        t = f32(self.s + inp)
+        self.c = f32(f32(t - self.s) - inp)
+        self.s = t
+
+
+def sampled_rgb(spectrum_type, regular, lo, hi, lambdas, values):
+    """The RGB build's triplet of a sampled spectrum: Regular/IrregularSampledSpectrum::toXYZ
+    (shared/spectrum_types.cpp:414-488, 521-591) over the 1 nm CMF table and the spectrum's
+    samples, divided by integralCMF (shared/spectrum_base.cpp:196-205), then
+    transformToRenderingRGB (spectrum_base.h:138-152) with negative channels clamped to zero
+    (shader_nodes.cpp:1058-1063).  A one-sample spectrum is constant (the reference's bin
+    width would be 0 / 0 there)."""
+    xb, yb, zb = cmf_table()
+    cm = (xb, yb, zb)
+    ncmf = len(yb)
+    low, binw = f32(360), f32(1)
+    values = [f32(v) for v in values]
+    lambdas = [f32(v) for v in lambdas]
+    n = len(values)
+    yint = _Kahan()
+    for i in range(1, ncmf):
+        yint.add(f32(f32(f32(yb[i - 1] + yb[i]) * f32(1)) * f32(0.5)))
+    sums = [_Kahan(), _Kahan(), _Kahan()]
+    if n == 1:
+        for i in range(1, ncmf):
+            for k in range(3):
+                sums[k].add(f32(f32(values[0] * f32(cm[k][i - 1] + cm[k][i])) * f32(0.5)))
+    else:
+        lo, hi = f32(lo), f32(hi)
+        first = lo if regular else lambdas[0]
+        last = hi if regular else lambdas[n - 1]
+        bw = f32(f32(hi - lo) / f32(n - 1)) if regular else f32(0)
+        at = (lambda k: f32(lo + f32(f32(k) * bw))) if regular else (lambda k: lambdas[k])
+        ci, bi = 0, 0
+        wl = min(low, first)
+        prev = [f32(0)] * 3
+        prev_v, half = f32(0), f32(0)
+        while True:
+            if wl < low:
+                c = [f32(0)] * 3
+            elif wl == f32(low + f32(f32(ci) * binw)):
+                c = [cm[k][ci] for k in range(3)]
+                ci += 1
+            else:
+                idx = min(int(f32(f32(wl - low) / binw)), ncmf - 1)
+                t = f32(f32(wl - f32(low + f32(f32(idx) * binw))) / binw)
+                c = [f32(f32(f32(f32(1) - t) * cm[k][idx]) + f32(t * cm[k][idx + 1])) for k in range(3)]
+            if wl < first:
+                v = values[0]
+            elif wl > last:
+                v = values[n - 1]
+            elif wl == at(bi):
+                v = values[bi]
+                bi += 1
+            elif regular:
+                idx = min(int(f32(f32(wl - lo) / bw)), n - 2)
+                t = f32(f32(wl - f32(lo + f32(f32(idx) * bw))) / bw)
+                v = f32(f32(f32(f32(1) - t) * values[idx]) + f32(t * values[idx + 1]))
+            else:
+                j = max(bi - 1, 0)
+                while j < n and lambdas[j] < wl:   # std::lower_bound from max(bi - 1, 0)
+                    j += 1
+                idx = max(j - 1, 0)
+                t = f32(f32(wl - lambdas[idx]) / f32(lambdas[idx + 1] - lambdas[idx]))
+                v = f32(f32(f32(f32(1) - t) * values[idx]) + f32(t * values[idx + 1]))
+            avg = f32(f32(prev_v + v) * f32(0.5))
+            for k in range(3):
+                sums[k].add(f32(f32(avg * f32(prev[k] + c[k])) * half))
+            prev = c
+            prev_v = v
+            prev_wl = wl
+            wl = min(f32(low + f32(f32(ci) * binw)), at(bi) if bi < n else f32(np.inf))
+            half = f32(f32(wl - prev_wl) * f32(0.5))
+            if ci == ncmf:
+                break
+    xyz = [f32(sums[k].s / yint.s) for k in range(3)]
+    rgb = tristim(MAT_XYZ_TO_D65 if spectrum_type == 1 else MAT_XYZ_TO_E, xyz)
+    return np.array([max(f32(0), x) for x in rgb], f32)
+
+
 def wavelengths(offset):
     """WavelengthSamples::createWithEqualOffsets (spectrum_types.h:55-65), 4 samples."""
     return np.array([f32(360.0) + f32(470.0) * (f32(i) + f32(offset)) / f32(4) for i in range(4)], f32)
@@ -629,9 +738,7 @@ class Evaluator:
             return self.triplet(st(), cs(), tr, lam)
         if node.type in ("RegularSampledSpectrum", "IrregularSampledSpectrum"):
             if not self.spectral:
-                # the RGB build's reduction of a sampled spectrum has no counterpart to check
-                # against (parity unpinned)
-                raise NotImplementedError("sampled spectra in the RGB build")
+                return self.sampled_rgb(node, st())
             return self.sampled(node, lam)
         if node.type == "Float3ToSpectrum":
             # shader_nodes.cu:254-274
@@ -661,6 +768,15 @@ class Evaluator:
         if self.spectral:
             return ob.meng_eval(st, cs, tr, lam)
         return ob.triplet_rgb(st, cs, tr)
+
+    def sampled_rgb(self, node, spectrum_type):
+        values = node.floats.get("values", [0.0]) or [0.0]
+        if node.type == "RegularSampledSpectrum":
+            return sampled_rgb(spectrum_type, True, node.floats.get("min wavelength", [360.0])[0],
+                               node.floats.get("max wavelength", [830.0])[0], [], values)
+        lam = list(node.floats.get("wavelengths", []))[:len(values)]
+        lam += [0.0] * (len(values) - len(lam))
+        return sampled_rgb(spectrum_type, False, 0, 0, lam, values)
 
     def sampled(self, node, lam):
         values = [f32(v) for v in node.floats["values"]]

@@ -167,16 +167,19 @@ def spectrum_cases(spectral):
         if src is not None:
             node.plugs["value"] = src
         cases.append((f"float3-to-spectrum-{name}", (node, P["Spectrum"], 0)))
-    if spectral:
-        rng = np.random.default_rng(11)
-        vals = list(rng.uniform(0, 1.5, 17))
-        cases.append(("regular", (NodeDesc("RegularSampledSpectrum", floats={
-            "min wavelength": [380.0], "max wavelength": [780.0], "values": vals}), P["Spectrum"], 0)))
-        cases.append(("regular-narrow", (NodeDesc("RegularSampledSpectrum", floats={
-            "min wavelength": [500.0], "max wavelength": [600.0], "values": vals[:5]}), P["Spectrum"], 0)))
-        lam = list(np.sort(rng.uniform(340, 860, 23)))
-        cases.append(("irregular", (NodeDesc("IrregularSampledSpectrum", floats={
-            "wavelengths": lam, "values": list(rng.uniform(0, 2, 23))}), P["Spectrum"], 0)))
+    # sampled spectra: per-wavelength values in the spectral build, one triplet in the RGB build
+    rng = np.random.default_rng(11)
+    vals = list(rng.uniform(0, 1.5, 17))
+    cases.append(("regular", (NodeDesc("RegularSampledSpectrum", floats={
+        "min wavelength": [380.0], "max wavelength": [780.0], "values": vals}), P["Spectrum"], 0)))
+    cases.append(("regular-narrow", (NodeDesc("RegularSampledSpectrum", floats={
+        "min wavelength": [500.0], "max wavelength": [600.0], "values": vals[:5]}), P["Spectrum"], 0)))
+    lam = list(np.sort(rng.uniform(340, 860, 23)))
+    cases.append(("irregular", (NodeDesc("IrregularSampledSpectrum", floats={
+        "wavelengths": lam, "values": list(rng.uniform(0, 2, 23))}), P["Spectrum"], 0)))
+    cases.append(("irregular-light", (NodeDesc("IrregularSampledSpectrum", floats={
+        "wavelengths": [450.25, 520.0, 610.5], "values": [0.5, 2.0, 1.0]},
+        enums={"spectrum type": "Light Source"}), P["Spectrum"], 0)))
     return cases
 
 
@@ -257,8 +260,8 @@ def test_node_plugs_match_oracle(torch_gpu, contexts, spectral):
 
 
 def test_sampled_spectrum_rgb_build_is_a_constant_triplet(torch_gpu, contexts):
-    """The RGB build reduces a sampled spectrum to one triplet (parity unpinned: no counterpart
-    output exists); it must not depend on the point."""
+    """The RGB build reduces a sampled spectrum to one triplet (its value against the oracle is
+    checked with the other plugs); it must not depend on the point."""
     ctx = contexts[False]
     pts = random_points(64, 7)
     node = NodeDesc("RegularSampledSpectrum", floats={"min wavelength": [400.0], "max wavelength": [700.0],

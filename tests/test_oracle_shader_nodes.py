@@ -1,7 +1,8 @@
 """Pins of the shader-node oracle (oracle/shader_nodes.py) by closed forms and invariants:
 bilinear filtering reproduces affine texel fields, the wrap modes' periodicity and symmetry,
 the gather order against the filter's weights, the plug conversions, sampled spectra against
-numpy's piecewise-linear interpolation, sRGB decoding at known values, the Meng texel
+numpy's piecewise-linear interpolation, the RGB build's triplet of a sampled spectrum against
+the equal-energy white and the CIE table, sRGB decoding at known values, the Meng texel
 encodings against the direct upsampling of the same colour, normal and height maps of known
 fields, and the radial tangents."""
 import math
@@ -228,6 +229,63 @@ def test_float3_to_spectrum_rgb_maps_minus_one_to_one_onto_zero_to_one():
     f3 = NodeDesc("Float3", floats={"0": [-1.0], "1": [0.0], "2": [3.0]})
     node = NodeDesc("Float3ToSpectrum", plugs={"value": (f3, P["float3"], 0)})
     np.testing.assert_array_equal(ev.spectrum((node, P["Spectrum"], 0), None, point()), [0.0, 0.5, 1.0])
+
+
+# ---- the RGB build's triplet of a sampled spectrum ----
+# XYZ -> linear sRGB (D65), Lindbloom's published matrix (rows)
+XYZ_TO_SRGB_D65 = np.array([[3.2404542, -1.5371385, -0.4985314], [-0.9692660, 1.8760108, 0.0415560],
+                            [0.0556434, -0.2040259, 1.0572252]])
+
+
+def cmf64():
+    return [np.asarray(t, np.float64) for t in osn.cmf_table()]
+
+
+def test_sampled_rgb_of_a_flat_spectrum():
+    """A flat reflectance of 1 is the equal-energy white: XYZ = (int xbar, int ybar, int zbar) /
+    int ybar, which the E-white matrix takes to (1, 1, 1) within the CIE table's imbalance; a
+    flat light source goes through the D65 matrix. Both also as a regular spectrum over
+    400-700 nm, held constant beyond its ends, and at half the value (linearity)."""
+    x, y, z = cmf64()
+    xyz = np.array([np.trapezoid(x), np.trapezoid(y), np.trapezoid(z)]) / np.trapezoid(y)
+    refl = osn.sampled_rgb(0, False, 0, 0, [360.0], [1.0])
+    np.testing.assert_allclose(refl, [1, 1, 1], atol=2e-3)
+    light = osn.sampled_rgb(1, False, 0, 0, [360.0], [1.0])
+    np.testing.assert_allclose(light, XYZ_TO_SRGB_D65 @ xyz, rtol=2e-5)
+    for st, want in ((0, refl), (1, light)):
+        np.testing.assert_allclose(osn.sampled_rgb(st, True, 400.0, 700.0, [], [1.0] * 7), want, rtol=2e-6)
+        np.testing.assert_allclose(osn.sampled_rgb(st, False, 0, 0, [412.5, 700.25], [0.5, 0.5]), 0.5 * want,
+                                   rtol=2e-6)
+
+
+@pytest.mark.parametrize("lam", [455, 500, 555, 610])
+def test_sampled_rgb_of_a_one_nanometre_triangle(lam):
+    """Values 0, 1, 0 at lam - 1, lam, lam + 1 nm: each interval adds the mean value times the
+    mean CMF times its width (toXYZ's rule), so XYZ = (c(lam-1) + 2 c(lam) + c(lam+1)) / 4 over
+    int ybar. A trapezoid of the products, or a 1 nm sum, would give c(lam) instead."""
+    cm = cmf64()
+    i = lam - 360
+    xyz = np.array([(c[i - 1] + 2 * c[i] + c[i + 1]) / 4 for c in cm]) / np.trapezoid(cm[1])
+    got = osn.sampled_rgb(1, False, 0, 0, [lam - 1.0, float(lam), lam + 1.0], [0.0, 1.0, 0.0])
+    want = np.maximum(XYZ_TO_SRGB_D65 @ xyz, 0)
+    np.testing.assert_allclose(got, want, rtol=1e-5, atol=1e-9)
+    plain = np.maximum(XYZ_TO_SRGB_D65 @ (np.array([c[i] for c in cm]) / np.trapezoid(cm[1])), 0)
+    assert np.max(np.abs(got - plain)) > 1e-4 * np.max(np.abs(plain))
+
+
+def test_sampled_rgb_between_grid_points():
+    """Samples off the 1 nm grid: a linear ramp sampled at 3 irregular points equals the same
+    ramp given as a regular spectrum of 2 samples (interpolation between samples is linear),
+    and the result is linear in the values."""
+    lo, hi = 380.5, 779.25
+    mid = 512.75
+    f = lambda l: 0.2 + 1.3 * (l - lo) / (hi - lo)
+    for st in (0, 1):
+        a = osn.sampled_rgb(st, False, 0, 0, [lo, mid, hi], [f(lo), f(mid), f(hi)])
+        b = osn.sampled_rgb(st, True, lo, hi, [], [f(lo), f(hi)])
+        np.testing.assert_allclose(a, b, rtol=2e-5)
+        c = osn.sampled_rgb(st, True, lo, hi, [], [2 * f(lo), 2 * f(hi)])
+        np.testing.assert_allclose(c, 2 * b, rtol=2e-6)
 
 
 # ---- block-compressed images: invariants of the Direct3D block formats ----

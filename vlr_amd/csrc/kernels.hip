@@ -4,7 +4,8 @@
 // device queues (the reference runs the same integrator as one OptiX megakernel,
 // libVLR/GPU_kernels/path_tracing.cu:30-337):
 //   k_frameBegin  clear the per-depth queue counters
-//   k_generate    camera ray per pixel (path_tracing.cu:30-70, cameras.cu:43-76, 145-157)
+//   k_generate    camera ray per pixel (path_tracing.cu:30-70, cameras.cu:43-76, 145-157), in
+//                 8 x 8 pixel tiles per wave
 //   per depth d:  k_intersect  closest hit over the two-level BVH
 //                 k_shade      hit/miss programs (path_tracing.cu:133-337): implicit emission
 //                              with MIS, Russian roulette, next-event estimation (builds a
@@ -17,10 +18,10 @@
 // Per-path random numbers are drawn in the reference's order, so a path consumes the same
 // PCG32 stream as the reference's megakernel.
 //
-// Traversal: BVH2 nodes with both child boxes in the parent (64 B), per-thread short stack
-// in LDS with an overflow area in scratch, watertight ray/triangle test (Woop, Benthin and
-// Wald, JCGT 2(1) 2013), compiled without floating-point contraction so that closest hits
-// reproduce the oracle's brute force bit for bit.
+// Traversal: 4-wide nodes with 8-bit quantised child boxes (64 B), inner nodes and leaves in
+// separate loops, per-thread short stack in LDS with an overflow area in scratch, watertight
+// ray/triangle test (Woop, Benthin and Wald, JCGT 2(1) 2013), compiled without floating-point
+// contraction so that closest hits reproduce the oracle's brute force bit for bit.
 #include "bvh.h"
 #include "dev_shading.h"
 #include "shader_nodes.h"
@@ -814,6 +815,20 @@ __device__ __forceinline__ void generatePath(const FrameBuffers &fb, const DevCa
     *rayDirOut = make_float4(rayDir.x, rayDir.y, rayDir.z, 3.402823466e+38f);
 }
 
+// Queue entry i -> pixel of a whole W x H frame: bands of 8 rows, each cut into 8 x 8 tiles
+// (the band's last tile as wide as the image leaves), so the 64 primary rays of a wave cover
+// one tile and leave the camera close together.  A bijection of [0, W*H).
+__device__ __forceinline__ void tilePixel(uint32_t i, uint32_t W, uint32_t H, uint32_t *px, uint32_t *py) {
+    const uint32_t band = i / (8 * W);
+    const uint32_t bandRows = min(8u, H - 8 * band);
+    const uint32_t j = i - band * 8 * W;
+    const uint32_t tile = j / (8 * bandRows);
+    const uint32_t tileW = min(8u, W - 8 * tile);
+    const uint32_t l = j - tile * 8 * bandRows;
+    *px = 8 * tile + l % tileW;
+    *py = 8 * band + l / tileW;
+}
+
 template <int N, bool MULTI = false>
 __global__ void __launch_bounds__(256) k_generate(KArgs a) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -837,8 +852,14 @@ __global__ void __launch_bounds__(256) k_generate(KArgs a) {
         }
     } else {
         if (i >= a.fp.pixelCount) return;
-        const uint32_t pix = a.fp.pixelBegin + i;
-        const uint32_t px = pix % a.fp.imageW, py = pix / a.fp.imageW;
+        uint32_t px, py;
+        if (a.fp.pixelCount == a.fp.imageW * a.fp.imageH) {
+            tilePixel(i, a.fp.imageW, a.fp.imageH, &px, &py);
+        } else {   // a part of a frame: its pixels in row-major order
+            const uint32_t pix = a.fp.pixelBegin + i;
+            px = pix % a.fp.imageW;
+            py = pix / a.fp.imageW;
+        }
         float4 o, d;
         generatePath<N>(a.fb, a.sc.camera, a.fp.imageW, a.fp.imageH, px, py, py * a.fp.width + px, &o, &d);
         a.fb.rays[0].org[i] = o;

@@ -53,7 +53,8 @@ void cmfBins(float x[16], float y[16], float z[16], float *integral);
 // device spectrum parameter for an immediate spectrum (RGB build: r,g,b; spectral: Meng)
 DevSpectrum makeDevSpectrum(SpectrumType type, const ImmSpectrum &s, bool spectral);
 MengTables hostMengTables();
-void sampledSpectrumToRGB(SpectrumType type, const float *lambdas, const float *values, uint32_t n, float rgb[3]);
+void sampledSpectrumToRGB(SpectrumType type, bool regular, float minLambda, float maxLambda, const float *lambdas,
+                          const float *values, uint32_t n, float rgb[3]);
 bool colorSpaceFromString(const char *s, ColorSpace *cs);
 const char *colorSpaceToString(ColorSpace cs);
 bool spectrumTypeFromString(const char *s, SpectrumType *t);

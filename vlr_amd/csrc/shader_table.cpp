@@ -147,11 +147,10 @@ uint32_t ShaderTable::addNode(const ShaderNode *n) {
         std::vector<float> values = vit == n->floats.end() ? std::vector<float>{0.0f} : vit->second;
         if (values.empty()) values.push_back(0.0f);
         std::vector<float> lambdas(values.size());
+        float lo = 0.0f, hi = 0.0f;
         if (regular) {
-            const float lo = floatAt(n->floats, "min wavelength", 0, 360.0f);
-            const float hi = floatAt(n->floats, "max wavelength", 0, 830.0f);
-            for (size_t i = 0; i < values.size(); ++i)
-                lambdas[i] = values.size() > 1 ? lo + (hi - lo) * float(i) / float(values.size() - 1) : lo;
+            lo = floatAt(n->floats, "min wavelength", 0, 360.0f);
+            hi = floatAt(n->floats, "max wavelength", 0, 830.0f);
             d.f[0] = lo;
             d.f[1] = hi;
         } else {
@@ -166,7 +165,8 @@ uint32_t ShaderTable::addNode(const ShaderNode *n) {
             pool.insert(pool.end(), values.begin(), values.end());
         } else {
             float rgb[3];
-            sampledSpectrumToRGB(st, lambdas.data(), values.data(), static_cast<uint32_t>(values.size()), rgb);
+            sampledSpectrumToRGB(st, regular, lo, hi, lambdas.data(), values.data(), static_cast<uint32_t>(values.size()),
+                                 rgb);
             d.f[0] = rgb[0];
             d.f[1] = rgb[1];
             d.f[2] = rgb[2];

@@ -9,6 +9,7 @@
 #include "color.h"
 
 #include <dlfcn.h>
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -122,31 +123,105 @@ void cmfBins(float x[16], float y[16], float z[16], float *integral) {
 
 MengTables hostMengTables() { return MengTables{g_tables.grid.data(), g_tables.points.data()}; }
 
-// A sampled spectrum as a rendering-RGB triplet for the RGB build: XYZ by a 1 nm sum of
-// value x CMF over 360-830 nm (values interpolated linearly, held constant beyond the ends),
-// normalised by the sum of ybar, then the reflectance (E) or light (D65) matrix.
-void sampledSpectrumToRGB(SpectrumType type, const float *lambdas, const float *values, uint32_t n, float rgb[3]) {
-    const Tables &T = g_tables;
-    float XYZ[3] = {0, 0, 0}, ysum = 0;
-    for (int i = 0; i < 471; ++i) {
-        const float wl = 360.0f + i;
-        float v;
-        if (wl <= lambdas[0]) {
-            v = values[0];
-        } else if (wl >= lambdas[n - 1]) {
-            v = values[n - 1];
-        } else {
-            uint32_t k = 0;
-            while (k + 1 < n && lambdas[k + 1] < wl) ++k;
-            const float t = (wl - lambdas[k]) / (lambdas[k + 1] - lambdas[k]);
-            v = (1 - t) * values[k] + t * values[k + 1];
-        }
-        XYZ[0] += v * T.xbar[i];
-        XYZ[1] += v * T.ybar[i];
-        XYZ[2] += v * T.zbar[i];
-        ysum += T.ybar[i];
+// A sampled spectrum as a rendering-RGB triplet for the RGB build, as
+// Regular/IrregularSampledSpectrum::toXYZ and transformToRenderingRGB do
+// (shared/spectrum_types.cpp:414-488, 521-591; shared/spectrum_base.h:138-152): XYZ integrated
+// over the merged break points of the 1 nm CMF table (360-830 nm) and of the spectrum's samples,
+// from min(360 nm, first sample) to 830 nm, each interval adding
+// mean(value) x (cmf_a + cmf_b) x width / 2 to compensated (Kahan) sums; values are
+// interpolated linearly and held constant beyond the end samples, the CMF is zero below 360 nm
+// and interpolated between its samples; the sums are divided by the integral of ybar (the same
+// trapezoid rule over its 471 samples, shared/spectrum_base.cpp:196-205). Then the reflectance
+// (E) or light-source (D65) matrix, and negative channels clamped to zero. A regular spectrum
+// places sample k at minLambda + k * (maxLambda - minLambda) / (n - 1); a spectrum of one sample
+// is that value everywhere. An irregular spectrum's wavelengths must ascend (the walk stops
+// after 471 + 2n + 2 steps, which an ascending list never reaches).
+namespace {
+struct Kahan {
+    float sum = 0.0f, comp = 0.0f;
+    void add(float v) {
+        const float in = v - comp;
+        const float t = sum + in;
+        comp = (t - sum) - in;
+        sum = t;
     }
-    for (float &c : XYZ) c /= ysum;
+};
+}  // namespace
+
+void sampledSpectrumToRGB(SpectrumType type, bool regular, float minLambda, float maxLambda, const float *lambdas,
+                          const float *values, uint32_t n, float rgb[3]) {
+    const Tables &T = g_tables;
+    constexpr uint32_t kCmf = 471;
+    constexpr float kLow = 360.0f, kBin = 1.0f;
+    Kahan yInt;
+    for (uint32_t i = 1; i < kCmf; ++i) yInt.add((T.ybar[i - 1] + T.ybar[i]) * 1 * 0.5f);
+    float XYZ[3];
+    if (n == 1) {
+        Kahan c[3];
+        for (uint32_t i = 1; i < kCmf; ++i) {
+            c[0].add(values[0] * (T.xbar[i - 1] + T.xbar[i]) * 0.5f);
+            c[1].add(values[0] * (T.ybar[i - 1] + T.ybar[i]) * 0.5f);
+            c[2].add(values[0] * (T.zbar[i - 1] + T.zbar[i]) * 0.5f);
+        }
+        for (int k = 0; k < 3; ++k) XYZ[k] = c[k].sum / yInt.sum;
+    } else {
+        const float first = regular ? minLambda : lambdas[0];
+        const float last = regular ? maxLambda : lambdas[n - 1];
+        const float binWidth = regular ? (maxLambda - minLambda) / (n - 1) : 0.0f;
+        auto sampleAt = [&](uint32_t k) { return regular ? minLambda + k * binWidth : lambdas[k]; };
+        uint32_t cmfIdx = 0, baseIdx = 0;
+        float wl = std::fmin(kLow, first);
+        float prevCmf[3] = {0.0f, 0.0f, 0.0f}, prevValue = 0.0f, halfWidth = 0.0f;
+        Kahan sum[3];
+        // each step passes a CMF or a sample point; the bound only stops unsorted wavelengths
+        for (uint32_t step = 0; step < kCmf + 2 * n + 2; ++step) {
+            float cmf[3];
+            if (wl < kLow) {
+                cmf[0] = cmf[1] = cmf[2] = 0.0f;
+            } else if (wl == kLow + cmfIdx * kBin) {
+                cmf[0] = T.xbar[cmfIdx];
+                cmf[1] = T.ybar[cmfIdx];
+                cmf[2] = T.zbar[cmfIdx];
+                ++cmfIdx;
+            } else {
+                const uint32_t idx = std::min(static_cast<uint32_t>((wl - kLow) / kBin), kCmf - 1);
+                const float t = (wl - (kLow + idx * kBin)) / kBin;
+                cmf[0] = (1 - t) * T.xbar[idx] + t * T.xbar[idx + 1];
+                cmf[1] = (1 - t) * T.ybar[idx] + t * T.ybar[idx + 1];
+                cmf[2] = (1 - t) * T.zbar[idx] + t * T.zbar[idx + 1];
+            }
+            float value;
+            if (wl < first) {
+                value = values[0];
+            } else if (wl > last) {
+                value = values[n - 1];
+            } else if (wl == sampleAt(baseIdx)) {
+                value = values[baseIdx];
+                ++baseIdx;
+            } else if (regular) {
+                const uint32_t idx = std::min(static_cast<uint32_t>((wl - minLambda) / binWidth), n - 2);
+                const float t = (wl - (minLambda + idx * binWidth)) / binWidth;
+                value = (1 - t) * values[idx] + t * values[idx + 1];
+            } else {
+                // the last sample at or before wl, searched from the one before baseIdx
+                const float *lb = std::lower_bound(lambdas + (baseIdx > 0 ? baseIdx - 1 : 0), lambdas + n, wl);
+                const uint32_t idx = static_cast<uint32_t>(std::max<long>(long(lb - lambdas) - 1, 0));
+                const float t = (wl - lambdas[idx]) / (lambdas[idx + 1] - lambdas[idx]);
+                value = (1 - t) * values[idx] + t * values[idx + 1];
+            }
+            const float avg = (prevValue + value) * 0.5f;
+            for (int k = 0; k < 3; ++k) {
+                sum[k].add(avg * (prevCmf[k] + cmf[k]) * halfWidth);
+                prevCmf[k] = cmf[k];
+            }
+            prevValue = value;
+            const float prevWl = wl;
+            wl = std::fmin(kLow + cmfIdx * kBin, baseIdx < n ? sampleAt(baseIdx) : INFINITY);
+            halfWidth = (wl - prevWl) * 0.5f;
+            if (cmfIdx == kCmf) break;
+        }
+        for (int k = 0; k < 3; ++k) XYZ[k] = sum[k].sum / yInt.sum;
+    }
     colorTristim(type == SpectrumType::LightSource ? 1 : 3, XYZ, rgb);
     for (int k = 0; k < 3; ++k) rgb[k] = std::fmax(0.0f, rgb[k]);
 }
