@@ -68,6 +68,13 @@ namespace VLRA_UNIT_NS {
 constexpr int kTraceBlock = 128;
 // per block: kLdsStack entries per lane of node codes, then as many entry distances
 constexpr int kTraceLds = 2 * kLdsStack * kTraceBlock;
+// The per-depth closest-hit kernel keeps 12 entries per lane in LDS (12 KB per block), which its
+// 84 VGPRs allow at 6 waves per SIMD; the per-depth any-hit kernel needs no entry distances
+// (8 KB per block for 16 entries).  The tail kernel, limited to 4 waves per SIMD by its
+// registers, keeps the full kTraceLds.
+constexpr int kIntersectStack = 12;
+constexpr int kIntersectLds = 2 * kIntersectStack * kTraceBlock;
+constexpr int kShadowLds = kLdsStack * kTraceBlock;
 #ifndef VLRA_TAIL_WAVES_PER_EU
 #define VLRA_TAIL_WAVES_PER_EU 4
 #endif
@@ -205,21 +212,24 @@ __device__ __forceinline__ void travBegin(TravState &st, const DevScene &sc, con
 // (light_transport_common.h:205-218) for any-hit rays.
 // ANY: an any-hit ray, which needs no visiting order: its children are stacked as they come,
 // without entry distances.
-template <bool ANY, bool STATS, bool ALPHA = false>
+// SD: stack entries per lane in LDS (lds holds SD * kTraceBlock node codes, then as many entry
+// distances unless ANY); the other kTravStackCapacity - SD entries are in ovf / ovfT.
+template <bool ANY, bool STATS, bool ALPHA = false, int SD = kLdsStack>
 __device__ __forceinline__ void traverse(TravState &st, const DevScene &sc, int32_t *lds, int lane, int32_t *ovf,
                                          float *ovfT, TravCount *tc) {
-    float *ldsT = reinterpret_cast<float *>(lds + kLdsStack * kTraceBlock);
+    constexpr int kOvf = static_cast<int>(kTravStackCapacity) - SD;
+    float *ldsT = ANY ? nullptr : reinterpret_cast<float *>(lds + SD * kTraceBlock);
     // each stack entry keeps the distance at which the ray enters its box; once a closer hit
     // has shrunk tmax past it (with the slab test's widening), the entry is dropped unvisited.
     // Scene set-up bounds the depth by kTravStackCapacity (travStackBound), so the index never
     // reaches past the scratch area; the clamp only keeps the access inside the array.
     auto push = [&](int32_t v, float tEnter) {
-        if (st.sp < kLdsStack) {
+        if (st.sp < SD) {
             lds[st.sp * kTraceBlock + lane] = v;
             if constexpr (!ANY) ldsT[st.sp * kTraceBlock + lane] = tEnter;
         } else {
-            ovf[min(st.sp - kLdsStack, kScratchStack - 1)] = v;
-            if constexpr (!ANY) ovfT[min(st.sp - kLdsStack, kScratchStack - 1)] = tEnter;
+            ovf[min(st.sp - SD, kOvf - 1)] = v;
+            if constexpr (!ANY) ovfT[min(st.sp - SD, kOvf - 1)] = tEnter;
         }
         ++st.sp;
     };
@@ -229,19 +239,19 @@ __device__ __forceinline__ void traverse(TravState &st, const DevScene &sc, int3
         if constexpr (ANY) {
             if (st.sp == 0) return kTravDone;
             --st.sp;
-            return st.sp < kLdsStack ? lds[st.sp * kTraceBlock + lane] : ovf[min(st.sp - kLdsStack, kScratchStack - 1)];
+            return st.sp < SD ? lds[st.sp * kTraceBlock + lane] : ovf[min(st.sp - SD, kOvf - 1)];
         } else {
             const float tCull = st.tmax * (1.0f + 8.0f * __FLT_EPSILON__);
             while (st.sp > 0) {
                 --st.sp;
                 float te;
                 int32_t v;
-                if (st.sp < kLdsStack) {
+                if (st.sp < SD) {
                     v = lds[st.sp * kTraceBlock + lane];
                     te = ldsT[st.sp * kTraceBlock + lane];
                 } else {
-                    v = ovf[min(st.sp - kLdsStack, kScratchStack - 1)];
-                    te = ovfT[min(st.sp - kLdsStack, kScratchStack - 1)];
+                    v = ovf[min(st.sp - SD, kOvf - 1)];
+                    te = ovfT[min(st.sp - SD, kOvf - 1)];
                 }
                 if (te <= tCull) return v;
             }
@@ -382,16 +392,16 @@ __device__ __forceinline__ void traverse(TravState &st, const DevScene &sc, int3
 
 // rng: the path's generator (closest-hit rays through alpha surfaces); vis: the fractional
 // visibility of an any-hit ray (0 when occluded)
-template <bool ANY, bool STATS = false, bool ALPHA = false>
+template <bool ANY, bool STATS = false, bool ALPHA = false, int SD = kLdsStack>
 __device__ __forceinline__ bool traceRay(const DevScene &sc, const V3 &org, const V3 &dir, float tmin, float tmax,
                                          TraceHit *hit, int32_t *lds, int lane, TravCount *tc = nullptr,
                                          Pcg32 *rng = nullptr, float *vis = nullptr) {
     TravState st;
     travBegin(st, sc, org, dir, tmin, tmax, ANY);
     st.rng = rng;
-    int32_t ovf[kScratchStack];
-    float ovfT[kScratchStack];
-    traverse<ANY, STATS, ALPHA>(st, sc, lds, lane, ovf, ovfT, tc);
+    int32_t ovf[kTravStackCapacity - SD];
+    float ovfT[ANY ? 1 : kTravStackCapacity - SD];
+    traverse<ANY, STATS, ALPHA, SD>(st, sc, lds, lane, ovf, ovfT, tc);
     if (st.found) *hit = st.hit;
     if (vis) *vis = st.found ? 0.0f : st.vis;
     return st.found;
@@ -887,7 +897,7 @@ __device__ __forceinline__ void addTravStats(uint32_t *counters, uint32_t slot, 
 // ---- intersect: closest hit ----
 template <bool STATS, bool MULTI = false, bool ALPHA = false>
 __global__ void __launch_bounds__(kTraceBlock) k_intersect(KArgs a) {
-    __shared__ int32_t lds[kTraceLds];
+    __shared__ int32_t lds[kIntersectLds];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const RayQueue q = a.fb.rays[a.depth & 1];
     const uint32_t count = a.fb.counters[2 * a.depth];
@@ -899,16 +909,18 @@ __global__ void __launch_bounds__(kTraceBlock) k_intersect(KArgs a) {
         h.inst = -1;
         bool found;
         if constexpr (MULTI)
-            found = traceRay<false, STATS>(a.jobs[jobOf(a, __float_as_uint(o.w))].scene, xyz(o), xyz(d), 0.0f, d.w,
-                                           &h, lds, threadIdx.x, &tc);
+            found = traceRay<false, STATS, false, kIntersectStack>(a.jobs[jobOf(a, __float_as_uint(o.w))].scene,
+                                                                   xyz(o), xyz(d), 0.0f, d.w, &h, lds, threadIdx.x, &tc);
         else if constexpr (ALPHA) {
             // the alpha test draws from the path's generator before its shading does
             const uint32_t path = __float_as_uint(o.w);
             Pcg32 rng{a.fb.ps.rng[path]};
-            found = traceRay<false, STATS, true>(a.sc, xyz(o), xyz(d), 0.0f, d.w, &h, lds, threadIdx.x, &tc, &rng);
+            found = traceRay<false, STATS, true, kIntersectStack>(a.sc, xyz(o), xyz(d), 0.0f, d.w, &h, lds, threadIdx.x,
+                                                                  &tc, &rng);
             a.fb.ps.rng[path] = rng.s;
         } else
-            found = traceRay<false, STATS>(a.sc, xyz(o), xyz(d), 0.0f, d.w, &h, lds, threadIdx.x, &tc);
+            found = traceRay<false, STATS, false, kIntersectStack>(a.sc, xyz(o), xyz(d), 0.0f, d.w, &h, lds,
+                                                                   threadIdx.x, &tc);
         if (found) {
             a.fb.hits.a[i] = make_int4(h.inst, h.geomInst, h.prim, __float_as_int(h.t));
             a.fb.hits.b[i] = make_float2(h.b1, h.b2);
@@ -1389,7 +1401,7 @@ __global__ void __launch_bounds__(256) k_shade(KArgs a) {
 // ---- shadow: any-hit, adds the NEE contribution when unoccluded ----
 template <bool STATS, bool MULTI = false, bool ALPHA = false>
 __global__ void __launch_bounds__(kTraceBlock) k_shadow(KArgs a) {
-    __shared__ int32_t lds[kTraceLds];
+    __shared__ int32_t lds[kShadowLds];
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t count = a.fb.counters[2 * a.depth + 1];
     TravCount tc;
