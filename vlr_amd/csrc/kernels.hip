@@ -68,9 +68,10 @@ namespace VLRA_UNIT_NS {
 constexpr int kTraceBlock = 128;
 // per block: kLdsStack entries per lane of node codes, then as many entry distances
 constexpr int kTraceLds = 2 * kLdsStack * kTraceBlock;
-// The per-depth closest-hit kernel keeps 12 entries per lane in LDS (12 KB per block), which its
-// 84 VGPRs allow at 6 waves per SIMD; the per-depth any-hit kernel needs no entry distances
-// (8 KB per block for 16 entries).  The tail kernel, limited to 4 waves per SIMD by its
+// A CU's 160 KB of LDS holds 9 blocks of kTraceLds (4.5 waves per SIMD).  The per-depth
+// closest-hit kernel keeps 12 entries per lane (12 KB per block), so that its registers (5 waves
+// per SIMD) set the limit; the per-depth any-hit kernel needs no entry distances (8 KB per block
+// for 16 entries; 7 waves per SIMD).  The tail kernel, limited to 4 waves per SIMD by its
 // registers, keeps the full kTraceLds.
 constexpr int kIntersectStack = 12;
 constexpr int kIntersectLds = 2 * kIntersectStack * kTraceBlock;
