@@ -80,6 +80,13 @@ constexpr int kShadowLds = kLdsStack * kTraceBlock;
 #define VLRA_TAIL_WAVES_PER_EU 4
 #endif
 constexpr int kTailWavesPerEU = VLRA_TAIL_WAVES_PER_EU;   // occupancy asked of the tail kernel
+// Occupancy asked of the per-path LVC-BPT and light-tracing kernels.  Left to themselves they
+// take 130-230 VGPRs and run 2 waves per SIMD; with some registers spilled, more waves hide
+// more of their traversals' fetch latency (C3 box at 1280x720: light paths 1.55 -> 1.39 ms,
+// eye paths 2.40 -> 1.89 ms, light tracing 2.60 -> 1.79 ms per frame).
+constexpr int kBptLightWavesPerEU = 4;
+constexpr int kBptEyeWavesPerEU = 3;
+constexpr int kLightTraceWavesPerEU = 4;
 constexpr int32_t kSentinel = INT32_MIN;
 
 // ---------------------------------------------------------------------------------------------
@@ -1992,7 +1999,8 @@ __device__ __forceinline__ Wls<N> commonWls(const BptParams &bp, bool single) {
 
 // ---- light subpaths (lvc_bpt.cu:144-334) ----
 template <int N, bool NODES = true>
-__global__ void __launch_bounds__(kTraceBlock) k_bptLight(BptArgs a) {
+__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kBptLightWavesPerEU)))
+k_bptLight(BptArgs a) {
     __shared__ int32_t lds[kTraceLds];
     const DevScene &sc = a.sc;
     const BptParams &bp = a.bp;
@@ -2216,7 +2224,8 @@ __global__ void __launch_bounds__(256) k_bptCompact(BptParams bp) {
 
 // ---- eye subpaths (lvc_bpt.cu:338-971) ----
 template <int N, bool NODES = true, bool GUIDES = false>
-__global__ void __launch_bounds__(kTraceBlock) k_bptEye(BptArgs a) {
+__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kBptEyeWavesPerEU)))
+k_bptEye(BptArgs a) {
     __shared__ int32_t lds[kTraceLds];
     const DevScene &sc = a.sc;
     const BptParams &bp = a.bp;
@@ -2609,7 +2618,8 @@ __device__ __forceinline__ void sampleLens(const DevCamera &cam, float ul0, floa
 }
 
 template <int N, bool NODES = true>
-__global__ void __launch_bounds__(kTraceBlock) k_lightTrace(BptArgs a) {
+__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(kLightTraceWavesPerEU)))
+k_lightTrace(BptArgs a) {
     __shared__ int32_t lds[kTraceLds];
     const DevScene &sc = a.sc;
     const BptParams &bp = a.bp;
