@@ -5,9 +5,11 @@ directly from the node graph, recursively, as the reference's calcNode does
 (shared/kernel_common.h:658-737), with:
   conversions       NodeTypeInfo<T>::convertFrom (shared/shared.h:390-470); a plug whose value
                     cannot convert to the wanted type gives the default
-  node programs     GPU_kernels/shader_nodes.cu:20-483
+  node programs     GPU_kernels/shader_nodes.cu:20-483 (EnvironmentTexture outside the
+                    environment emitter: :457-483)
   sampled spectra   RegularSampledSpectrum / IrregularSampledSpectrum::evaluate
-                    (shared/spectrum_types.cpp:394-412, 496-518)
+                    (shared/spectrum_types.cpp:394-412, 496-518); in the RGB build toXYZ
+                    (spectrum_types.cpp:414-591) and transformToRenderingRGB
   images            LinearImage2D's conversions at creation (image.cpp:410-790): in the
                     spectral build, spectra from RGB sources stored as quantised Meng grid
                     coordinates (uvsA8x4 from 8-bit sources, uvsA16Fx4 from half / float ones);
@@ -469,7 +471,16 @@ def decode_bc(fmt, data, width, height):
 
 # ---- images ----
 class Texture:
-    def __init__(self, img, spectral):
+    def __init__(self, img, spectral, env=False):
+        """env: the texels as EnvironmentTextureShaderNode's sampler reads them (its read mode is
+        plain normalised float, shader_nodes.cpp:1915-1919: no hardware sRGB read), and for a
+        missing image its null image, 1 x 1 RGBA16F (1, 0, 1, 1) light source in Rec.709 D65
+        (shader_nodes.cpp:1880-1881)."""
+        if img is None:
+            from types import SimpleNamespace
+            px = np.array([1.0, 0.0, 1.0, 1.0], np.float16).view(np.uint8).reshape(1, 1, 8)
+            img = SimpleNamespace(data=px, width=1, height=1, format="RGBA16Fx4", spectrum_type="Light Source",
+                                  color_space="Rec709(D65)")
         self.width, self.height = img.width, img.height
         fmt = img.format
         if fmt.startswith("BC"):
@@ -479,7 +490,7 @@ class Texture:
             self.uvs, self.kind, self.gray = False, "rgba", False
             self.spectrum_type = SPEC_TYPES[img.spectrum_type]
             t = decode_bc(fmt, img.data, img.width, img.height)
-            degam = cs == 0 and fmt in ("BC1", "BC2", "BC3", "BC4", "BC7")
+            degam = not env and cs == 0 and fmt in ("BC1", "BC2", "BC3", "BC4", "BC7")
             if degam:
                 for y in range(img.height):
                     for x in range(img.width):
@@ -497,7 +508,7 @@ class Texture:
         self.kind = ("uvs8" if rgb8 else "uvs16") if self.uvs else "rgba"
         self.gray = fmt in ("Gray32F", "Gray8", "GrayA8x2")
         self.spectrum_type = st
-        hw = srgb and not self.uvs and fmt in ("RGB8x3", "RGB_8x4", "RGBA8x4", "Gray8")
+        hw = not env and srgb and not self.uvs and fmt in ("RGB8x3", "RGB_8x4", "RGBA8x4", "Gray8")
         create = srgb and not self.uvs and ((not spectral and (half or full)) or fmt in ("RG32Fx2", "Gray32F"))
         self.color_space = 1 if (hw or create) else cs
         raw = np.ascontiguousarray(img.data, np.uint8).reshape(img.height, img.width, -1)
@@ -624,10 +635,11 @@ class Evaluator:
         self.spectral = spectral
         self.textures = {}
 
-    def texture(self, img):
-        if id(img) not in self.textures:
-            self.textures[id(img)] = Texture(img, self.spectral)
-        return self.textures[id(img)]
+    def texture(self, img, env=False):
+        key = (id(img), env)
+        if key not in self.textures:
+            self.textures[key] = Texture(img, self.spectral, env)
+        return self.textures[key]
 
     def calc(self, plug, want, default, pt):
         """calcNode for a numeric type."""
@@ -762,6 +774,18 @@ class Evaluator:
                     u, v, s = f32(u * f32(12)), f32(v * f32(14)), f32(s * f32(3))
                 return ob.meng_uvs_eval(u, v, f32(s / EER), lam)
             return ob.meng_eval(tex.spectrum_type, tex.color_space, t[:3], lam)
+        if node.type == "EnvironmentTexture":
+            # shader_nodes.cu:457-483: sampled at the surface's texture coordinates with repeat
+            # wrapping; only uvsA16Fx4 texels are Meng coordinates, other texels a light-source
+            # triplet in the image's colour space; the RGB build takes the texel as it is
+            tex = self.texture(node.image, env=True)
+            linear = node.enums.get("mag filter", "Linear") == "Linear"
+            t = tex.sample(pt.uv[0], pt.uv[1], linear, 0, 0).copy()
+            if not self.spectral:
+                return t[:3].copy()
+            if tex.kind == "uvs16":
+                return ob.meng_uvs_eval(t[0], t[1], f32(t[2] / EER), lam)
+            return ob.meng_eval(1, tex.color_space, t[:3], lam)
         return default()
 
     def triplet(self, st, cs, tr, lam):

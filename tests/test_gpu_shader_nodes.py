@@ -105,6 +105,13 @@ def texture_cases():
             cases.append((f"tex-float2-{tag}", (node, P["float2"], 2)))
             cases.append((f"tex-float1-{tag}", (node, P["float1"], k % 4)))
             cases.append((f"tex-alpha-{tag}", (node, P["Alpha"], 3)))
+    # EnvironmentTexture on a surface: read at the surface's texture coordinates, repeat wrap,
+    # no hardware sRGB read; no image reads the node's magenta null image
+    for k, name in enumerate(("rgba16f_srgb_light", "rgba8_srgb_refl", "rgb8_linear_na", "gray8_srgb",
+                              "rgba32f_xyz", "bc1_srgb_refl", "bc6h_light", None)):
+        filt = ("Nearest", "Linear")[k % 2]
+        node = NodeDesc("EnvironmentTexture", image=IMAGES[name] if name else None, enums={"mag filter": filt})
+        cases.append((f"envtex-{name}-{filt}", (node, P["Spectrum"], 0)))
     for bump in ("Normal Map (DirectX)", "Normal Map (OpenGL)", "Height Map"):
         for coeff in (0.7, 1.0, 1.5):
             img = IMAGES["rgba32f_refl"] if coeff != 1.0 else IMAGES["rgba8_srgb_refl"]

@@ -151,6 +151,35 @@ def test_srgb_texels_decoded():
     assert osn.Texture(ImageDesc(px, 1, 1, "RGBA8x4", "NA", REC709_SRGB), True).kind == "rgba"
 
 
+def test_environment_texture_node_reads():
+    """EnvironmentTexture on a surface (shader_nodes.cu:457-483, shader_nodes.cpp:1862-1921): the
+    sampler reads without the hardware sRGB conversion, the RGB build returns the texel as it is,
+    a missing image reads the null image (1, 0, 1), and in the spectral build an 8-bit image (stored
+    as uvsA8x4) is taken as a light-source triplet while a half-float one is Meng coordinates."""
+    px = np.array([[[0, 128, 255, 255]]], np.uint8)
+    img = ImageDesc(px, 1, 1, "RGBA8x4", "Reflectance", REC709_SRGB)
+    node = NodeDesc("EnvironmentTexture", image=img, enums={"mag filter": "Nearest"})
+    rgb = osn.Evaluator(False).spectrum((node, P["Spectrum"], 0), None, point())
+    np.testing.assert_array_equal(rgb, np.float32([0, 128, 255]) / np.float32(255))
+    null = NodeDesc("EnvironmentTexture")
+    np.testing.assert_array_equal(osn.Evaluator(False).spectrum((null, P["Spectrum"], 0), None, point()), [1, 0, 1])
+    ev = osn.Evaluator(True)
+    pt = point(offset=0.3)
+    lam = osn.wavelengths(0.3)
+    t8 = ev.texture(img, env=True)
+    assert t8.kind == "uvs8"
+    want8 = ob.meng_eval(1, t8.color_space, t8.texels[0, 0, :3], lam)
+    np.testing.assert_array_equal(ev.spectrum((node, P["Spectrum"], 0), None, pt), want8)
+    t16 = ev.texture(None, env=True)
+    assert t16.kind == "uvs16"
+    u, v, b = t16.texels[0, 0, :3]
+    np.testing.assert_array_equal(ev.spectrum((null, P["Spectrum"], 0), None, pt),
+                                  ob.meng_uvs_eval(u, v, np.float32(b / osn.EER), lam))
+    # the null image is magenta: its upsampled spectrum is high at both ends and low in the middle
+    s = ob.meng_uvs_eval(u, v, np.float32(b / osn.EER), np.float32([420, 550, 680, 700]))
+    assert s[0] > 3 * s[1] and s[2] > 3 * s[1]
+
+
 @pytest.mark.parametrize("fmt,tol", [("RGBA8x4", 0.03), ("RGBA16Fx4", 2e-3), ("RGBA32Fx4", 2e-3)])
 def test_meng_texel_encodings_match_direct_upsampling(fmt, tol):
     rng = np.random.default_rng(4)

@@ -508,10 +508,10 @@ DevMaterial SurfaceMaterial::toDevice(bool spectral, bool *usesEnvTexture, const
         if (it != plugs.end() && it->second.valid()) {
             SpectrumType t;
             if (it->second.node->constantSpectrum(&s, &t)) return makeDevSpectrum(t, s, spectral);
-            // the environment texture is read by the environment emitter's own lookup, not by a
-            // node program
+            // the environment emitter reads its environment texture with its own lookup, not
+            // through a node program
             if (roots && it->second.plugType == VLRShaderNodePlugType_Spectrum &&
-                it->second.node->kind != NodeKind::EnvironmentTexture)
+                !(kind == MatKind::EnvironmentEmitter && it->second.node->kind == NodeKind::EnvironmentTexture))
                 roots->push_back({it->second, 1u << slot});
         }
         return makeDevSpectrum(type, s, spectral);

@@ -367,10 +367,12 @@ struct ShaderTable {
     std::vector<float4> texels;
     std::vector<float> pool;
     std::map<const ShaderNode *, uint32_t> nodeIndex;
-    std::map<const Image2D *, uint32_t> texIndex;
+    std::map<std::pair<const Image2D *, bool>, uint32_t> texIndex;
     std::vector<DevNodeStep> steps;
     uint32_t addNode(const ShaderNode *n);
-    uint32_t addTexture(const Image2D *img);
+    // envRead: the texels as EnvironmentTextureShaderNode's sampler reads them (no sRGB read,
+    // and a magenta 1 x 1 light-source image where none is set)
+    uint32_t addTexture(const Image2D *img, bool envRead = false);
     // One node program over the given roots (plug, output bits); returns (first step, count).
     std::pair<uint32_t, uint32_t> program(const std::vector<std::pair<PlugRef, uint32_t>> &roots);
 };

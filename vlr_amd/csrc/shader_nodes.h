@@ -264,6 +264,25 @@ __device__ __forceinline__ void runNodeProgram(const DevScene &sc, uint32_t base
                     }
                     break;
                 }
+                case NODE_ENV_TEXTURE: {
+                    // shader_nodes.cu:457-483: read at the surface's texture coordinates; only a
+                    // uvsA16Fx4 image is taken as Meng coordinates, anything else as a light-source
+                    // triplet in the image's colour space, and the RGB build takes the texel as is
+                    const DevTexture &T = sc.textures[nd.a[0]];
+                    const float4 t = sampleTexture(sc, T, nd.a[1], sp.texU, sp.texV);
+                    if constexpr (N == 3) {
+                        r[0] = t.x;
+                        r[1] = t.y;
+                        r[2] = t.z;
+                    } else if (T.format == kTexUVS16) {
+                        r = evalSpectrum<N>(
+                            sc, mengFromUV(MengTables{sc.mengGrid, sc.mengPoints}, t.x, t.y, t.z / 0.009355121400914532f),
+                            wls);
+                    } else {
+                        r = specFromTriplet<N>(sc, kSpecLightSource, T.colorSpace, t.x, t.y, t.z, wls);
+                    }
+                    break;
+                }
                 default:
                     ok = false;
                     break;

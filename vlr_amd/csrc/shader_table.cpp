@@ -201,6 +201,11 @@ uint32_t ShaderTable::addNode(const ShaderNode *n) {
         d.f[0] = static_cast<float>(q + 1) / 16.0f;
         break;
     }
+    case NodeKind::EnvironmentTexture:
+        // repeat wrap on both axes, filter as set (shader_nodes.cpp:1910-1921)
+        d.a[0] = addTexture(n->image, true);
+        d.a[1] = enumIndex(n->enums, "mag filter", {"Nearest", "Linear", "None"}, 1) == 1 ? 1u : 0u;
+        break;
     default:
         break;
     }
@@ -208,14 +213,39 @@ uint32_t ShaderTable::addNode(const ShaderNode *n) {
     return idx;
 }
 
-uint32_t ShaderTable::addTexture(const Image2D *img) {
-    auto found = texIndex.find(img);
+uint32_t ShaderTable::addTexture(const Image2D *img, bool envRead) {
+    auto found = texIndex.find({img, envRead});
     if (found != texIndex.end()) return found->second;
     const uint32_t idx = static_cast<uint32_t>(textures.size());
-    texIndex[img] = idx;
+    texIndex[{img, envRead}] = idx;
     DevTexture T;
     std::memset(&T, 0, sizeof(T));
     T.offset = static_cast<uint32_t>(texels.size());
+    if (envRead && !img) {
+        // the node's null image: 1 x 1 RGBA16F (1, 0, 1, 1), light source, Rec.709 D65
+        // (shader_nodes.cpp:1880-1881), stored as the spectral build stores such an image
+        T.width = T.height = 1;
+        T.spectrumType = kSpecLightSource;
+        T.colorSpace = kCsRec709;
+        float4 t = make_float4(1.0f, 0.0f, 1.0f, 1.0f);
+        if (spectral) {
+            const float rgb[3] = {1.0f, 0.0f, 1.0f};
+            float XYZ[3];
+            colorTristim(0, rgb, XYZ);
+            const float b = XYZ[0] + XYZ[1] + XYZ[2];
+            const float xy[2] = {XYZ[0] / b, XYZ[1] / b};
+            float uv[2];
+            mengXYtoUV(xy, uv);
+            auto h = [](float v) { return halfToFloat(floatToHalf(v)); };
+            t = make_float4(h(uv[0]), h(uv[1]), h(b), 1.0f);
+            T.format = kTexUVS16;
+        } else {
+            T.format = kTexRGBA;
+        }
+        texels.push_back(t);
+        textures.push_back(T);
+        return idx;
+    }
     if (img && img->type == ObjType::BlockCompressedImage2D) {
         // mip level 0 decoded; the texture unit's sRGB read of BC1-BC4 and BC7 (image.cpp:336-346)
         // is a per-texel degamma of the colour channels, done here
@@ -224,7 +254,7 @@ uint32_t ShaderTable::addTexture(const Image2D *img) {
         if (!bi->mips.empty() &&
             decodeBlockCompressed(bi->formatStr, bi->mips[0].data(), bi->mips[0].size(), bi->width, bi->height, &dec)) {
             const std::string &f = bi->formatStr;
-            const bool degamma = bi->colorSpace == ColorSpace::Rec709_D65_sRGBGamma &&
+            const bool degamma = !envRead && bi->colorSpace == ColorSpace::Rec709_D65_sRGBGamma &&
                                  (f == "BC1" || f == "BC2" || f == "BC3" || f == "BC4" || f == "BC7");
             T.width = bi->width;
             T.height = bi->height;
@@ -269,8 +299,9 @@ uint32_t ShaderTable::addTexture(const Image2D *img) {
     T.format = uvs ? (rgb8 ? kTexUVS8 : kTexUVS16) : kTexRGBA;
     T.gray = (f == "Gray32F" || f == "Gray8" || f == "GrayA8x2") ? 1 : 0;
     T.spectrumType = stype;
-    // texels read through the hardware sRGB path count as degammed (shader_nodes.cpp:1678-1683)
-    const bool hwDegamma = srgb && !uvs && (rgb8 || f == "Gray8");
+    // texels read through the hardware sRGB path count as degammed (shader_nodes.cpp:1678-1683);
+    // the environment texture's sampler reads without it (shader_nodes.cpp:1915-1919)
+    const bool hwDegamma = !envRead && srgb && !uvs && (rgb8 || f == "Gray8");
     const bool createDegamma = srgb && !uvs && ((!spectral && (half || full)) || f == "RG32Fx2" || f == "Gray32F");
     const bool grayA8Degamma = srgb && f == "GrayA8x2";
     T.colorSpace = static_cast<uint32_t>((hwDegamma || createDegamma) ? ColorSpace::Rec709_D65 : li->colorSpace);
