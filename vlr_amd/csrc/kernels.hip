@@ -2212,14 +2212,17 @@ __global__ void __launch_bounds__(kScanBlock) k_bptScanTop(BptParams bp, uint32_
     if (threadIdx.x < numBlocks) bp.blockSums[threadIdx.x] = waveTotals[wave] + x - v;
 }
 
+// One thread per float4 of a light path's slots: thread (path i, element k) copies element k if
+// the path wrote that many vertices, so consecutive threads read and write consecutive float4s.
 __global__ void __launch_bounds__(256) k_bptCompact(BptParams bp) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t perPath = bp.slotsPerPath * kLVF4;
+    const uint64_t g = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+    const uint64_t i = g / perPath;
+    const uint32_t k = static_cast<uint32_t>(g - i * perPath);
     if (i >= bp.numLightPaths) return;
-    const uint32_t n = bp.slotCounts[i];
+    if (k >= bp.slotCounts[i] * kLVF4) return;
     const uint32_t base = bp.blockSums[i / kScanBlock] + bp.slotOffsets[i];
-    const float4 *src = bp.slots + static_cast<size_t>(i) * bp.slotsPerPath * kLVF4;
-    float4 *dst = bp.cache + static_cast<size_t>(base) * kLVF4;
-    for (uint32_t k = 0; k < n * kLVF4; ++k) dst[k] = src[k];
+    bp.cache[static_cast<size_t>(base) * kLVF4 + k] = bp.slots[i * perPath + k];
 }
 
 // ---- eye subpaths (lvc_bpt.cu:338-971) ----
@@ -3093,7 +3096,8 @@ static hipError_t launchBptFrameT(const DevScene &sc, const FrameParams &fp, con
         hipLaunchKernelGGL((k_bptLight<N, false>), dim3(blocksFor(bp.numLightPaths, kTraceBlock)), dim3(kTraceBlock), 0, s, a);
     hipLaunchKernelGGL(k_bptScanBlocks, dim3(scanBlocks), dim3(kScanBlock), 0, s, bp);
     hipLaunchKernelGGL(k_bptScanTop, dim3(1), dim3(kScanBlock), 0, s, bp, scanBlocks);
-    hipLaunchKernelGGL(k_bptCompact, dim3(blocksFor(bp.numLightPaths, 256)), dim3(256), 0, s, bp);
+    const uint64_t compactThreads = static_cast<uint64_t>(bp.numLightPaths) * bp.slotsPerPath * kLVF4;
+    hipLaunchKernelGGL(k_bptCompact, dim3(static_cast<uint32_t>((compactThreads + 255) / 256)), dim3(256), 0, s, bp);
     t1(STAGE_LIGHT_PATHS);
     t0(STAGE_EYE_PATHS);
     const bool guides = fb.guideAlbedo != nullptr;
