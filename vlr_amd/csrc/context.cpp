@@ -751,6 +751,7 @@ void Context::renderBatch(hipStream_t userStream, const std::vector<BatchJobDesc
     VLRA_HIP(hipMemcpy(b.rng.ptr, seeds.data(), n * sizeof(uint64_t), hipMemcpyHostToDevice));
     b.jobs.upload(table.data(), table.size() * sizeof(DevJob));
     b.jobOffsets.upload(offsets.data(), offsets.size() * sizeof(uint32_t));
+    b.activeOffsets.reserve((jobs.size() + 1) * sizeof(uint32_t));
     if (!b.stream) VLRA_HIP(hipStreamCreateWithFlags(&b.stream, hipStreamNonBlocking));
 
     BatchLaunch L;
@@ -778,6 +779,7 @@ void Context::renderBatch(hipStream_t userStream, const std::vector<BatchJobDesc
     L.fb.sortKey = L.fb.sortPerm = L.fb.sortCounts = nullptr;   // no material sort in batches
     L.jobs = static_cast<DevJob *>(b.jobs.ptr);
     L.jobOffsets = static_cast<const uint32_t *>(b.jobOffsets.ptr);
+    L.activeOffsets = static_cast<uint32_t *>(b.activeOffsets.ptr);
     L.numJobs = static_cast<uint32_t>(jobs.size());
     L.nodePrograms = false;
     for (const auto &kv : b.scenes) {

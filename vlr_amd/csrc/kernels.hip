@@ -753,6 +753,9 @@ struct KArgs {
     // batches: path i belongs to the job j with jobOffsets[j] <= i < jobOffsets[j + 1]
     DevJob *jobs;
     const uint32_t *jobOffsets;
+    // per step: where each job's pixels start among the pixels of the jobs taking part (exclusive
+    // prefix over the jobs in order, inactive jobs counting zero), their total at [numJobs]
+    uint32_t *activeOffsets;
     uint32_t numJobs;
 };
 
@@ -766,17 +769,45 @@ __device__ __forceinline__ uint32_t jobOf(const KArgs &a, uint32_t path) {
     return lo;
 }
 
-// one thread per job: a job takes part in this step while it has samples left
-__global__ void k_batchStepBegin(DevJob *jobs, uint32_t numJobs) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= numJobs) return;
-    DevJob &J = jobs[j];
-    if (J.frameIndex < J.spp) {
-        J.frameIndex += 1;
-        J.active = 1;
-    } else {
-        J.active = 0;
+// One block: a job takes part in this step while it has samples left; the exclusive prefix of
+// the taking-part jobs' pixel counts places their pixels one after another in the step's queue.
+constexpr uint32_t kStepBeginThreads = 256;
+__global__ void __launch_bounds__(kStepBeginThreads) k_batchStepBegin(DevJob *jobs, uint32_t numJobs,
+                                                                       uint32_t *activeOffsets) {
+    __shared__ uint32_t waveTotals[kStepBeginThreads / 64];
+    const uint32_t t = threadIdx.x, lane = __lane_id(), wave = t / 64;
+    uint32_t carry = 0;
+    for (uint32_t c = 0; c < numJobs; c += kStepBeginThreads) {
+        const uint32_t j = c + t;
+        uint32_t cnt = 0;
+        if (j < numJobs) {
+            DevJob &J = jobs[j];
+            if (J.frameIndex < J.spp) {
+                J.frameIndex += 1;
+                J.active = 1;
+                cnt = J.width * J.height;
+            } else {
+                J.active = 0;
+            }
+        }
+        uint32_t incl = cnt;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t up = __shfl_up(incl, off);
+            if (static_cast<int>(lane) >= off) incl += up;
+        }
+        if (lane == 63) waveTotals[wave] = incl;
+        __syncthreads();
+        uint32_t before = carry, total = carry;
+        for (uint32_t w = 0; w < kStepBeginThreads / 64; ++w) {
+            if (w < wave) before += waveTotals[w];
+            total += waveTotals[w];
+        }
+        if (j < numJobs) activeOffsets[j] = before + incl - cnt;
+        carry = total;
+        __syncthreads();
     }
+    if (t == 0) activeOffsets[numJobs] = carry;
 }
 
 __global__ void k_frameBegin(uint32_t *counters, uint32_t numPaths, uint32_t *sortCounts) {
@@ -852,22 +883,23 @@ __global__ void __launch_bounds__(256) k_generate(KArgs a) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t numPaths = a.fp.imageW * a.fp.imageH;
     if constexpr (MULTI) {
-        // paths of the jobs taking part in this step; the queue is appended wave by wave
-        bool emit = false;
+        // queue entry i: pixel i of the jobs taking part in this step, taken one after another
+        const uint32_t total = a.activeOffsets[a.numJobs];
+        if (i == 0) a.fb.counters[0] = total;
+        if (i >= total) return;
+        uint32_t lo = 0, hi = a.numJobs;   // the last job whose pixels start at or before i
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (a.activeOffsets[mid] <= i) lo = mid;
+            else hi = mid;
+        }
+        const DevJob &J = a.jobs[lo];
+        const uint32_t local = i - a.activeOffsets[lo];
         float4 o, d;
-        if (i < numPaths) {
-            const DevJob &J = a.jobs[jobOf(a, i)];
-            if (J.active) {
-                const uint32_t local = i - J.pixelOffset;
-                generatePath<N>(a.fb, J.scene.camera, J.width, J.height, local % J.width, local / J.width, i, &o, &d);
-                emit = true;
-            }
-        }
-        const uint32_t slot = appendSlot(&a.fb.counters[0], emit);
-        if (emit) {
-            a.fb.rays[0].org[slot] = o;
-            a.fb.rays[0].dir[slot] = d;
-        }
+        generatePath<N>(a.fb, J.scene.camera, J.width, J.height, local % J.width, local / J.width,
+                        J.pixelOffset + local, &o, &d);
+        a.fb.rays[0].org[i] = o;
+        a.fb.rays[0].dir[i] = d;
     } else {
         if (i >= a.fp.pixelCount) return;
         uint32_t px, py;
@@ -2935,6 +2967,7 @@ static hipError_t launchFrameT(const DevScene &sc, const FrameParams &fp, const 
     a.depth = 0;
     a.jobs = nullptr;
     a.jobOffsets = nullptr;
+    a.activeOffsets = nullptr;
     a.numJobs = 0;
     const uint32_t n = fp.pixelCount;
     auto t0 = [&](int st) { if (timer) timer->begin(st); };
@@ -3039,9 +3072,10 @@ static hipError_t launchBatchStepT(const BatchLaunch &b, hipStream_t s) {
     a.depth = 0;
     a.jobs = b.jobs;
     a.jobOffsets = b.jobOffsets;
+    a.activeOffsets = b.activeOffsets;
     a.numJobs = b.numJobs;
     const uint32_t n = b.fp.imageW * b.fp.imageH;
-    hipLaunchKernelGGL(k_batchStepBegin, dim3(blocksFor(b.numJobs, 64)), dim3(64), 0, s, b.jobs, b.numJobs);
+    hipLaunchKernelGGL(k_batchStepBegin, dim3(1), dim3(kStepBeginThreads), 0, s, b.jobs, b.numJobs, b.activeOffsets);
     hipLaunchKernelGGL(k_frameBegin, dim3(1), dim3(256), 0, s, b.fb.counters, 0u, static_cast<uint32_t *>(nullptr));
     hipLaunchKernelGGL((k_generate<N, true>), dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
     const uint32_t numDepths = std::min<uint32_t>(b.fp.maxPathLength, kMaxDepthCounters - 1);
@@ -3083,6 +3117,7 @@ static hipError_t launchBptFrameT(const DevScene &sc, const FrameParams &fp, con
     k.depth = 0;
     k.jobs = nullptr;
     k.jobOffsets = nullptr;
+    k.activeOffsets = nullptr;
     k.numJobs = 0;
     const uint32_t n = fp.imageW * fp.imageH;
     const uint32_t scanBlocks = blocksFor(bp.numLightPaths, kScanBlock);
@@ -3136,6 +3171,7 @@ static hipError_t launchLightTraceFrameT(const DevScene &sc, const FrameParams &
     k.depth = 0;
     k.jobs = nullptr;
     k.jobOffsets = nullptr;
+    k.activeOffsets = nullptr;
     k.numJobs = 0;
     const uint32_t n = fp.imageW * fp.imageH;
     if (bp.numLightPaths == 0) return hipErrorInvalidValue;
@@ -3309,6 +3345,7 @@ hipError_t launchConvert(const DevScene &sc, const FrameParams &fp, const FrameB
     a.depth = 0;
     a.jobs = nullptr;
     a.jobOffsets = nullptr;
+    a.activeOffsets = nullptr;
     a.numJobs = 0;
     const uint32_t n = fp.pixelCount;
     if (timer) timer->begin(STAGE_CONVERT);
@@ -3334,6 +3371,7 @@ hipError_t launchBatchConvert(const BatchLaunch &b, bool spectral, hipStream_t s
     a.depth = 0;
     a.jobs = b.jobs;
     a.jobOffsets = b.jobOffsets;
+    a.activeOffsets = b.activeOffsets;
     a.numJobs = b.numJobs;
     const uint32_t n = b.fp.imageW * b.fp.imageH;
     if (spectral)

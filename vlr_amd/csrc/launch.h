@@ -206,6 +206,7 @@ struct BatchLaunch {
     FrameBuffers fb;
     DevJob* jobs;
     const uint32_t* jobOffsets;
+    uint32_t* activeOffsets;       // numJobs + 1, written on the device at each step
     uint32_t numJobs;
     bool nodePrograms;             // some job's scene has node programs: the kernels that run them
 };

@@ -438,7 +438,8 @@ struct BatchState {
     std::map<const Scene *, std::unique_ptr<SceneGPU>> scenes;
     uint64_t sceneVersion = ~0ull;
     bool spectral = false;
-    DeviceBuffer rng, alpha, contrib, meta, rayOrg[2], rayDir[2], hitA, hitB, shOrg, shDir, shVal, jobs, jobOffsets;
+    DeviceBuffer rng, alpha, contrib, meta, rayOrg[2], rayDir[2], hitA, hitB, shOrg, shDir, shVal, jobs, jobOffsets,
+        activeOffsets;
     std::vector<std::unique_ptr<DeviceBuffer>> accum, output;
     std::vector<uint32_t> widths, heights;
     hipStream_t stream = nullptr;
