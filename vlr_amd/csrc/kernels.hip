@@ -883,7 +883,7 @@ __global__ void __launch_bounds__(256) k_generate(KArgs a) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t numPaths = a.fp.imageW * a.fp.imageH;
     if constexpr (MULTI) {
-        // queue entry i: pixel i of the jobs taking part in this step, taken one after another
+        // queue entry i: the jobs taking part in this step one after another, each in 8 x 8 tiles
         const uint32_t total = a.activeOffsets[a.numJobs];
         if (i == 0) a.fb.counters[0] = total;
         if (i >= total) return;
@@ -894,10 +894,10 @@ __global__ void __launch_bounds__(256) k_generate(KArgs a) {
             else hi = mid;
         }
         const DevJob &J = a.jobs[lo];
-        const uint32_t local = i - a.activeOffsets[lo];
+        uint32_t px, py;
+        tilePixel(i - a.activeOffsets[lo], J.width, J.height, &px, &py);
         float4 o, d;
-        generatePath<N>(a.fb, J.scene.camera, J.width, J.height, local % J.width, local / J.width,
-                        J.pixelOffset + local, &o, &d);
+        generatePath<N>(a.fb, J.scene.camera, J.width, J.height, px, py, J.pixelOffset + py * J.width + px, &o, &d);
         a.fb.rays[0].org[i] = o;
         a.fb.rays[0].dir[i] = d;
     } else {
