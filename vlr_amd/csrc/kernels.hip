@@ -881,7 +881,6 @@ __device__ __forceinline__ void tilePixel(uint32_t i, uint32_t W, uint32_t H, ui
 template <int N, bool MULTI = false>
 __global__ void __launch_bounds__(256) k_generate(KArgs a) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t numPaths = a.fp.imageW * a.fp.imageH;
     if constexpr (MULTI) {
         // queue entry i: the jobs taking part in this step one after another, each in 8 x 8 tiles
         const uint32_t total = a.activeOffsets[a.numJobs];
