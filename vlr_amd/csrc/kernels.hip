@@ -1656,7 +1656,41 @@ __global__ void __launch_bounds__(256) k_rcGatherSlots(const float4 *trainRaw, c
 }
 
 // ---- accumulate: Kahan sum per channel (common_internal.h:350-368, spectrum_types.h:674-684) ----
-template <int N, bool MULTI = false>
+// XYZ (through the 16 CMF bins, spectral) or linear Rec.709 (RGB) of a pixel's sums, averaged
+// over the frames, as Rec.709 scaled by the image area (post_process.cu:93-163)
+template <int N>
+__device__ __forceinline__ float4 convertSums(const DevScene &sc, const float *sums, uint32_t frames, uint32_t W,
+                                              uint32_t H) {
+    float X, Y, Z;
+    if constexpr (N == 4) {
+        X = Y = Z = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            X += sc.cmfX[k] * sums[k];
+            Y += sc.cmfY[k] * sums[k];
+            Z += sc.cmfZ[k] * sums[k];
+        }
+        X /= sc.integralCMF;
+        Y /= sc.integralCMF;
+        Z /= sc.integralCMF;
+    } else {
+        const float r = sums[0], g = sums[1], b = sums[2];
+        X = 0.4124564f * r + 0.3575761f * g + 0.1804375f * b;
+        Y = 0.2126729f * r + 0.7151522f * g + 0.0721750f * b;
+        Z = 0.0193339f * r + 0.1191920f * g + 0.9503041f * b;
+    }
+    float rec = 1.0f / frames;
+    X *= rec; Y *= rec; Z *= rec;
+    float R = 3.2404542f * X + -1.5371385f * Y + -0.4985314f * Z;
+    float G = -0.9692660f * X + 1.8760108f * Y + 0.0415560f * Z;
+    float B = 0.0556434f * X + -0.2040259f * Y + 1.0572252f * Z;
+    float res = static_cast<float>(W) * static_cast<float>(H);
+    return make_float4(R * res, G * res, B * res, 1.0f);
+}
+
+// CONVERT (single renders): also writes the pixel's output from the sums just stored, which
+// k_convert would read back.
+template <int N, bool MULTI = false, bool CONVERT = false>
 __global__ void __launch_bounds__(256) k_accumulate(KArgs a) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t numPaths = a.fp.imageW * a.fp.imageH;
@@ -1690,6 +1724,12 @@ __global__ void __launch_bounds__(256) k_accumulate(KArgs a) {
         if (reset) {
             for (int k = 0; k < 2 * C; ++k) acc[k * plane] = 0.0f;
         }
+        if constexpr (CONVERT && !MULTI) {
+            float sums[C];
+#pragma unroll
+            for (int k = 0; k < C; ++k) sums[k] = reset ? 0.0f : acc[(2 * k) * plane];
+            a.fb.output[p] = convertSums<N>(a.sc, sums, a.fp.frameIndex, a.fp.imageW, a.fp.imageH);
+        }
         return;
     }
     float addend[C];
@@ -1709,6 +1749,7 @@ __global__ void __launch_bounds__(256) k_accumulate(KArgs a) {
     } else {
         addend[0] = c[0]; addend[1] = c[1]; addend[2] = c[2];
     }
+    float sums[C];
 #pragma unroll
     for (int k = 0; k < C; ++k) {
         float sum = reset ? 0.0f : acc[(2 * k) * plane];
@@ -1718,7 +1759,10 @@ __global__ void __launch_bounds__(256) k_accumulate(KArgs a) {
         comp = (tmp - sum) - cIn;
         acc[(2 * k) * plane] = tmp;
         acc[(2 * k + 1) * plane] = comp;
+        sums[k] = tmp;
     }
+    if constexpr (CONVERT && !MULTI)
+        a.fb.output[p] = convertSums<N>(a.sc, sums, a.fp.frameIndex, a.fp.imageW, a.fp.imageH);
 }
 
 // ---- convert: post_process.cu:93-163 ----
@@ -1756,32 +1800,11 @@ __global__ void __launch_bounds__(256) k_convert(KArgs a) {
     }
     const DevScene &sc = MULTI ? *scp : a.sc;
     if (frames == 0) return;
-    float X, Y, Z;
-    if constexpr (N == 4) {
-        X = Y = Z = 0.0f;
+    constexpr int C = N == 4 ? 16 : 3;
+    float sums[C];
 #pragma unroll
-        for (int k = 0; k < 16; ++k) {
-            float v = acc[(2 * k) * plane];
-            X += sc.cmfX[k] * v;
-            Y += sc.cmfY[k] * v;
-            Z += sc.cmfZ[k] * v;
-        }
-        X /= sc.integralCMF;
-        Y /= sc.integralCMF;
-        Z /= sc.integralCMF;
-    } else {
-        float r = acc[0], g = acc[2 * plane], b = acc[4 * plane];
-        X = 0.4124564f * r + 0.3575761f * g + 0.1804375f * b;
-        Y = 0.2126729f * r + 0.7151522f * g + 0.0721750f * b;
-        Z = 0.0193339f * r + 0.1191920f * g + 0.9503041f * b;
-    }
-    float rec = 1.0f / frames;
-    X *= rec; Y *= rec; Z *= rec;
-    float R = 3.2404542f * X + -1.5371385f * Y + -0.4985314f * Z;
-    float G = -0.9692660f * X + 1.8760108f * Y + 0.0415560f * Z;
-    float B = 0.0556434f * X + -0.2040259f * Y + 1.0572252f * Z;
-    float res = static_cast<float>(W) * static_cast<float>(H);
-    *out = make_float4(R * res, G * res, B * res, 1.0f);
+    for (int k = 0; k < C; ++k) sums[k] = acc[(2 * k) * plane];
+    *out = convertSums<N>(sc, sums, frames, W, H);
 }
 
 __global__ void k_frameEnd(uint32_t *counters, uint32_t maxPathLength, uint32_t wavefrontDepth) {
@@ -3028,14 +3051,13 @@ static hipError_t launchFrameT(const DevScene &sc, const FrameParams &fp, const 
         if (e != hipSuccess) return e;
         t1(STAGE_CACHE_QUERY);
     }
+    // the accumulation also writes the converted output when the frame asks for it
     t0(STAGE_ACCUMULATE);
-    hipLaunchKernelGGL(k_accumulate<N>, dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
+    if (convert)
+        hipLaunchKernelGGL((k_accumulate<N, false, true>), dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
+    else
+        hipLaunchKernelGGL(k_accumulate<N>, dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
     t1(STAGE_ACCUMULATE);
-    if (convert) {
-        t0(STAGE_CONVERT);
-        hipLaunchKernelGGL(k_convert<N>, dim3(blocksFor(n, 256)), dim3(256), 0, s, a);
-        t1(STAGE_CONVERT);
-    }
     hipLaunchKernelGGL(k_frameEnd, dim3(1), dim3(64), 0, s, fb.counters, fp.maxPathLength, wavefrontDepths);
     if (cache && fp.cacheMode != 0) {
         t0(STAGE_CACHE_TRAIN);
