@@ -380,11 +380,22 @@ __global__ void __launch_bounds__(256) k_rcWeightGrad(RcDevice d, uint32_t nPad,
 __global__ void __launch_bounds__(256) k_rcAdam(RcDevice d, RcTrainParams p, uint32_t numSlices,
                                                  uint32_t numLossBlocks, const uint32_t *countDev, uint32_t n) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-    if (i == 0) {
-        const uint32_t count = countDev ? min(*countDev, n) : n;
+    if (blockIdx.x == 0) {
+        // the loss: the blocks' partial sums added in a fixed order (per thread a strided run,
+        // then a tree over the 256 threads), so the value repeats from run to run
+        __shared__ float part[256];
         float sum = 0.0f;
-        for (uint32_t b = 0; b < numLossBlocks; ++b) sum += d.lossPartial[b];
-        *d.loss = sum / (static_cast<float>(max(count, 1u)) * static_cast<float>(p.numOut));
+        for (uint32_t b = threadIdx.x; b < numLossBlocks; b += 256) sum += d.lossPartial[b];
+        part[threadIdx.x] = sum;
+        __syncthreads();
+        for (uint32_t w = 128; w > 0; w >>= 1) {
+            if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            const uint32_t count = countDev ? min(*countDev, n) : n;
+            *d.loss = part[0] / (static_cast<float>(max(count, 1u)) * static_cast<float>(p.numOut));
+        }
     }
     if (i >= kNumParams) return;
     float gsum = 0.0f;
