@@ -184,7 +184,7 @@ def test_autotuned_schedule_matches_fixed_schedule(torch_gpu):
         tuned_img, tuned_rng = ctx.read_output(), ctx.read_rng()
         wd, tb, tuned = ctx.schedule()
     print("tuned schedule: wavefront depth", wd, "tail blocks", tb, "finished", tuned)
-    assert tuned and 1 <= wd <= 6 and tb in (512, 1024, 2048, 4096)
+    assert tuned and 0 <= wd <= 6 and tb in (512, 1024, 2048, 4096)
     imgs, _, rng = render_gpu(desc, W, H, frames, 8, False, wavefront_depth=64)
     assert np.array_equal(rng, tuned_rng)
     assert np.array_equal(imgs[-1].view(np.uint32), tuned_img.view(np.uint32))

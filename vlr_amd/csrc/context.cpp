@@ -791,7 +791,7 @@ void Context::renderBatch(hipStream_t userStream, const std::vector<BatchJobDesc
     // schedule candidates (wavefront depth; the tail grid fixed), each timed on one step when
     // autotuning is on and there are enough steps; otherwise the context's schedule
     std::vector<uint32_t> cands;
-    if (autotune && maxSpp >= 8) cands = {1, 2, 3, 4, 6};
+    if (autotune && maxSpp >= 8) cands = {0, 1, 2, 3, 4, 6};
     else cands = {wavefrontDepth};
     std::map<uint32_t, std::unique_ptr<GraphExec>> graphs;
     auto stepLaunch = [&](uint32_t wd, hipStream_t s) {

@@ -8,11 +8,13 @@
 
 namespace vlra {
 
-// Depths rendered as separate per-depth kernels before the paths move to the tail kernel.
-// On the 1080p icosphere grid (2.07 M, 1.14 M, 0.24 M, 0.11 M, 48 k ... rays at depths 0, 1, 2,
-// 3, 4 ...) a frame takes 5.64 ms with only depth 0 as its own kernels, 6.1 ms with two, 6.7 ms
-// with three: the tail kernel, which refills its lanes per path, carries the rest best.
-constexpr uint32_t kDefaultWavefrontDepth = 1;
+// Depths rendered as separate per-depth kernels before the paths move to the tail kernel
+// (0: the tail kernel runs every depth, primary rays included).  On the 1080p spectral
+// icosphere grid (2.07 M, 1.14 M, 0.24 M, 0.11 M, 48 k ... rays at depths 0, 1, 2, 3, 4 ...)
+// a frame takes 3.88 ms with every depth in the tail kernel, 4.35 ms with depth 0 as its own
+// kernels and 4.88 ms with depths 0 and 1: the tail kernel, which refills its lanes per path,
+// carries the whole frame best.  The denoiser guides need depth 0 as its own kernels.
+constexpr uint32_t kDefaultWavefrontDepth = 0;
 
 struct FrameParams {
     uint32_t width, height;        // buffer size (stride = width)
@@ -79,7 +81,7 @@ struct ScheduleTuner {
         phase = 0;
         index = frames = 0;
         cands.clear();
-        for (uint32_t d : {1u, 2u, 3u, 4u}) cands.push_back({d, kDefaultTailBlocks, 0u, 1e30f});
+        for (uint32_t d : {0u, 1u, 2u, 3u, 4u}) cands.push_back({d, kDefaultTailBlocks, 0u, 1e30f});
     }
     const Cand &current() const { return cands[index]; }
     Cand best() const {
