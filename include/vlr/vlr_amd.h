@@ -81,7 +81,9 @@ VLR_API VLRResult vlrContextReadDenoiserAuxBuffers(VLRContextConst context, floa
 VLR_API VLRResult vlrContextSetTraversalStatistics(VLRContext context, bool enable);
 /* Frame schedule: depths below `depth` run as separate intersect / shade / shadow kernels over
  * device queues; from `depth` on every remaining path runs to its end inside one persistent
- * kernel.  The image does not depend on it (paths draw and add in the same order). */
+ * kernel; 0 (the default) runs every depth, primary rays included, in that kernel.  With the
+ * denoiser guides on, depth 0 always runs as its own kernels.  The image does not depend on it
+ * (paths draw and add in the same order). */
 VLR_API VLRResult vlrContextSetWavefrontDepth(VLRContext context, uint32_t depth);
 VLR_API VLRResult vlrContextGetWavefrontDepth(VLRContextConst context, uint32_t* depth);
 /* Schedule autotuning (on by default; setting the wavefront depth turns it off): the first
