@@ -63,7 +63,8 @@ constexpr uint32_t kDefaultTailBlocks = 2048;   // 8 blocks of 128 per CU
 
 // Frame-schedule autotuner.  The image does not depend on the schedule, so candidates are
 // timed on real frames: first the wavefront depth (tail grid fixed), then the tail grid at
-// the best depth, then the material sort off and on; each candidate gets
+// the best depth, then the material sort off and on (not at depth 0, which has no per-depth
+// queues to sort); each candidate gets
 // kFramesPerCandidate frames and keeps its fastest.
 struct ScheduleTuner {
     static constexpr uint32_t kFramesPerCandidate = 3;
@@ -102,6 +103,10 @@ struct ScheduleTuner {
             phase = 1;
             cands.clear();
             for (uint32_t t : {512u, 1024u, 2048u, 4096u}) cands.push_back({b.wavefrontDepth, t, 0u, 1e30f});
+        } else if (phase == 1 && b.wavefrontDepth == 0) {
+            // no per-depth queues to sort: the material sort has nothing to change
+            phase = 3;
+            cands.assign(1, b);
         } else if (phase == 1) {
             phase = 2;
             cands.clear();
